@@ -1,0 +1,48 @@
+#!/usr/bin/env bash
+# Build the reference SIMP hot path from its own Fortran sources, where they
+# lie, into oracle/_ref/ (objects, modules and the harness binary only).
+#
+#   SIMP_REF_DIR   reference checkout (default /root/reference)
+#
+# Recipe notes:
+#  * modules are compiled in dependency order (globals, data_variables,
+#    domain, boundary_domain, mpi_variables, tree_module, interfaces); the
+#    reference Makefile's order (Makefile:114-121) does not build from clean.
+#  * no -fopenmp: flang rejects the REDUCTION(-:...) clause at
+#    single_step.f90:55, and the sources are not edited.  Without OpenMP the
+#    directives are comments and the build is the serial reference, the same
+#    arithmetic order as OMP_NUM_THREADS=1.  The omp_get_* calls that remain
+#    are ordinary calls resolved by libomp at link time.
+#  * the shipped compiled module files (fortran/src/modules/*.mod) are never
+#    put on the include path; modules are regenerated from source.
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+REF="${SIMP_REF_DIR:-/root/reference}"
+SRC="$REF/fortran/src"
+OUT="$HERE/_ref"
+FC="${FC:-/opt/rocm/bin/amdflang}"
+FFLAGS="-O2 -ffp-contract=off -w"
+
+if [ ! -d "$SRC" ]; then
+    echo "build_ref: no reference sources at $SRC" >&2
+    exit 3
+fi
+mkdir -p "$OUT/obj" "$OUT/mod"
+
+mods="globals data_variables domain boundary_domain mpi_variables tree_module interfaces"
+objs=()
+for m in $mods; do
+    "$FC" $FFLAGS -J"$OUT/mod" -c "$SRC/modules/$m.f90" -o "$OUT/obj/$m.o"
+    objs+=("$OUT/obj/$m.o")
+done
+for d in ext_force in_out int_force integration neighbor phy_quantities sph; do
+    for f in "$SRC/$d"/*.f90; do
+        b="$(basename "$f" .f90)"
+        "$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$f" -o "$OUT/obj/$b.o"
+        objs+=("$OUT/obj/$b.o")
+    done
+done
+"$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$HERE/ref_harness.f90" -o "$OUT/obj/ref_harness.o"
+"$FC" $FFLAGS -o "$OUT/simp_ref_harness" "$OUT/obj/ref_harness.o" "${objs[@]}" \
+    -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+echo "build_ref: $OUT/simp_ref_harness"

@@ -1,0 +1,187 @@
+"""ctypes binding of the FP64 oracle library (test infrastructure only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_simp.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("tse", C.c_int), ("wind_mode", C.c_int),
+                ("wind", C.c_double * 2), ("current", C.c_double * 2),
+                ("max_hsml", C.c_double),
+                ("grid_nx", C.c_int), ("grid_ny", C.c_int),
+                ("grid_x0", C.c_double), ("grid_y0", C.c_double),
+                ("grid_dx", C.c_double), ("grid_dy", C.c_double),
+                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle_simp.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_nnps.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_void_p]
+        L.orc_direct_find.argtypes = L.orc_nnps.argtypes
+        L.orc_nip.argtypes = [C.c_void_p]
+        L.orc_bnip.argtypes = [C.c_void_p]
+        L.orc_get_pairs.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
+        L.orc_get_bpairs.argtypes = [C.c_void_p, _ip, _ip, _dp]
+        L.orc_get_bins.argtypes = [C.c_void_p, _dp]
+        L.orc_single_step.argtypes = [C.c_void_p, C.POINTER(OrcParams), C.c_int,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_run_pc.argtypes = [C.c_void_p, C.POINTER(OrcParams), C.c_int, C.c_int,
+                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 C.c_int, C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]
+        L.orc_vp_dt.argtypes = [C.c_int, _dp]
+        L.orc_vp_dt.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def make_params(sc) -> tuple:
+    """OrcParams for a :class:`simp_amd.scenarios.Scenario` (plus the arrays
+    it points at, which must stay alive)."""
+    p = OrcParams()
+    p.tse = sc.tse
+    p.wind[0], p.wind[1] = sc.wind
+    p.current[0], p.current[1] = sc.current
+    p.max_hsml = sc.max_hsml
+    keep = ()
+    if sc.wind_grid is not None:
+        g = sc.wind_grid
+        u = np.ascontiguousarray(g.u, dtype=np.float64)
+        v = np.ascontiguousarray(g.v, dtype=np.float64)
+        p.wind_mode = 1
+        p.grid_ny, p.grid_nx = u.shape
+        p.grid_x0, p.grid_y0, p.grid_dx, p.grid_dy = g.x0, g.y0, g.dx, g.dy
+        p.grid_u = u.ctypes.data
+        p.grid_v = v.ctypes.data
+        keep = (u, v)
+    return p, keep
+
+
+@dataclass
+class Pairs:
+    p_i: np.ndarray
+    p_j: np.ndarray
+    r_ij: np.ndarray
+    hsml_ij: np.ndarray
+    bp_i: np.ndarray
+    bp_j: np.ndarray
+    br_ij: np.ndarray
+    bins: np.ndarray
+
+
+class Oracle:
+    """One oracle context (owns the persistent pair arrays)."""
+
+    def __init__(self):
+        self._L = lib()
+        self._c = self._L.orc_create()
+
+    def __del__(self):
+        try:
+            self._L.orc_destroy(self._c)
+        except Exception:
+            pass
+
+    def _pairs(self) -> Pairs:
+        L = self._L
+        nip, bnip = L.orc_nip(self._c), L.orc_bnip(self._c)
+        pi = np.zeros(nip, np.int32); pj = np.zeros(nip, np.int32)
+        r = np.zeros(nip); hij = np.zeros(nip)
+        L.orc_get_pairs(self._c, pi, pj, r, hij)
+        bi = np.zeros(bnip, np.int32); bj = np.zeros(bnip, np.int32)
+        br = np.zeros(bnip)
+        L.orc_get_bpairs(self._c, bi, bj, br)
+        bins = np.zeros(5)
+        L.orc_get_bins(self._c, bins)
+        return Pairs(pi, pj, r, hij, bi, bj, br, bins)
+
+    def nnps(self, x, hsml, xb) -> Pairs:
+        x = np.ascontiguousarray(x, np.float64)
+        xb = np.ascontiguousarray(xb, np.float64)
+        rc = self._L.orc_nnps(self._c, x.shape[0], x, np.ascontiguousarray(hsml, np.float64),
+                              xb.shape[0], _ptr(xb))
+        if rc:
+            raise RuntimeError(f"orc_nnps failed ({rc})")
+        return self._pairs()
+
+    def direct_find(self, x, hsml, xb) -> Pairs:
+        x = np.ascontiguousarray(x, np.float64)
+        xb = np.ascontiguousarray(xb, np.float64)
+        rc = self._L.orc_direct_find(self._c, x.shape[0], x,
+                                     np.ascontiguousarray(hsml, np.float64),
+                                     xb.shape[0], _ptr(xb))
+        if rc:
+            raise RuntimeError(f"orc_direct_find failed ({rc})")
+        return self._pairs()
+
+    def single_step(self, sc, current_time=None) -> dict:
+        n = sc.n
+        t = sc.time if current_time is None else current_time
+        prm, keep = make_params(sc)
+        out = {k: np.zeros((n, 2)) for k in ("dxdt", "dvxdt")}
+        out.update({k: np.zeros(n) for k in ("dhdt", "dAdt", "zeta", "eta", "p")})
+        out["eps"] = np.zeros((n, 3))
+        out["sigma"] = np.zeros((n, 3))
+        xb = np.ascontiguousarray(sc.xb)
+        rc = self._L.orc_single_step(
+            self._c, C.byref(prm), n, sc.x, sc.vx, sc.mass, sc.rho, sc.hsml,
+            sc.h, sc.A, sc.nb, _ptr(xb), _ptr(sc.massb), t,
+            out["dxdt"], out["dvxdt"], out["dhdt"], out["dAdt"], out["eps"],
+            out["zeta"], out["eta"], out["p"], out["sigma"])
+        if rc:
+            raise RuntimeError(f"orc_single_step failed ({rc})")
+        out["nip"] = self._L.orc_nip(self._c)
+        out["bnip"] = self._L.orc_bnip(self._c)
+        return out
+
+    def run_pc(self, sc, nsteps: int, itimestep: int = 0) -> dict:
+        """Advance ``sc`` in place by ``nsteps`` predictor-corrector steps."""
+        n = sc.n
+        prm, keep = make_params(sc)
+        dt = C.c_double(sc.dt)
+        t = C.c_double(sc.time)
+        it = C.c_int(itimestep)
+        out = {"eps": np.zeros((n, 3)), "sigma": np.zeros((n, 3)),
+               "zeta": np.zeros(n), "eta": np.zeros(n), "p": np.zeros(n)}
+        xb = np.ascontiguousarray(sc.xb)
+        rc = self._L.orc_run_pc(
+            self._c, C.byref(prm), nsteps, n, sc.x, sc.vx, sc.mass, sc.rho,
+            sc.hsml, sc.h, sc.A, sc.nb, _ptr(xb), _ptr(sc.massb),
+            C.byref(dt), C.byref(t), C.byref(it), out["eps"], out["zeta"],
+            out["eta"], out["p"], out["sigma"])
+        if rc:
+            raise RuntimeError(f"orc_run_pc failed ({rc})")
+        sc.dt, sc.time = dt.value, t.value
+        out["itimestep"] = it.value
+        return out

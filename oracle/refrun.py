@@ -1,0 +1,86 @@
+"""Driver of the compiled reference (``_ref/simp_ref_harness``).
+
+TEST INFRASTRUCTURE ONLY.  The harness is built from the reference's own
+Fortran sources by ``build_ref.sh``; this module writes its binary input,
+runs it in a scratch directory and parses its FP64 output.  Used to write
+the golden vectors under ``tests/golden`` and as the CPU baseline.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(HERE, "_ref", "simp_ref_harness")
+MAXN = 80000    # globals.f90:22
+MAXNB = 5000    # globals.f90:23
+
+MODE_NNPS, MODE_SINGLE_STEP, MODE_TIME_INTEGRATION = 1, 2, 3
+
+
+def available() -> bool:
+    return os.path.exists(HARNESS)
+
+
+def _write_input(path, mode, sc, reps, integration_time):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<4i", mode, sc.n, sc.nb, reps))
+        f.write(struct.pack("<4d", sc.dt, sc.time, integration_time, sc.max_hsml))
+        for a in (sc.x, sc.vx, sc.mass, sc.rho, sc.hsml, sc.A, sc.h):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        if sc.nb:
+            f.write(np.ascontiguousarray(sc.xb, dtype="<f8").tobytes())
+            f.write(np.ascontiguousarray(sc.massb, dtype="<f8").tobytes())
+
+
+def _read_output(path, n):
+    buf = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(buf, dtype=dtype, count=count, offset=off).copy()
+        off += a.nbytes
+        return a
+
+    nip, bnip, itimestep, _ = take("<i4", 4)
+    dt, t, wall = take("<f8", 3)
+    out = {"nip": int(nip), "bnip": int(bnip), "itimestep": int(itimestep),
+           "dt": float(dt), "time": float(t), "wall": float(wall)}
+    out["p_i"] = take("<i4", nip); out["p_j"] = take("<i4", nip)
+    out["r_ij"] = take("<f8", nip); out["hsml_ij"] = take("<f8", nip)
+    out["bp_i"] = take("<i4", bnip); out["bp_j"] = take("<i4", bnip)
+    out["br_ij"] = take("<f8", bnip)
+    out["x"] = take("<f8", 2 * n).reshape(n, 2)
+    out["vx"] = take("<f8", 2 * n).reshape(n, 2)
+    for k in ("rho", "hsml", "A", "h"):
+        out[k] = take("<f8", n)
+    out["dxdt"] = take("<f8", 2 * n).reshape(n, 2)
+    out["dvxdt"] = take("<f8", 2 * n).reshape(n, 2)
+    out["dhdt"] = take("<f8", n); out["dAdt"] = take("<f8", n)
+    e11, e12, e22 = take("<f8", n), take("<f8", n), take("<f8", n)
+    out["eps"] = np.stack([e11, e12, e22], axis=1)
+    out["zeta"] = take("<f8", n); out["eta"] = take("<f8", n); out["p"] = take("<f8", n)
+    s11, s12, s22 = take("<f8", n), take("<f8", n), take("<f8", n)
+    out["sigma"] = np.stack([s11, s12, s22], axis=1)
+    return out
+
+
+def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600):
+    """Run the reference harness on scenario ``sc``; returns a dict of its
+    outputs.  The reference's wind is its own hard-coded field."""
+    if not available():
+        raise FileNotFoundError(HARNESS)
+    if sc.n > MAXN or sc.nb > MAXNB:
+        raise ValueError("scenario exceeds the reference's static capacity")
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        _write_input(fin, mode, sc, reps, integration_time)
+        env = dict(os.environ, PWD=d, OMP_NUM_THREADS=str(threads))
+        subprocess.run([HARNESS, fin, fout], cwd=d, env=env, check=True,
+                       stdout=subprocess.DEVNULL, timeout=timeout)
+        return _read_output(fout, sc.n)

@@ -1,0 +1,570 @@
+/*
+ * simp_oracle.c -- CPU FP64 restatement of SIMP's single_step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Nothing in simp_amd/ links or
+ * calls this file; tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg are its only users.
+ *
+ * Every function restates one reference routine, cited file:line relative to
+ * the SIMP source tree (fortran/src/...).  Arithmetic is written in the
+ * reference's evaluation order, built with -ffp-contract=off, so that a
+ * single-thread reference build and this file agree to rounding of libm
+ * calls (exp, sqrt) and of the compiler's integer-power expansion.
+ *
+ * Parity pins (all in tests/):
+ *   - nnps / direct_find: golden vectors written by the compiled reference
+ *     (tests/golden) and the closed-form set predicate on brute force.
+ *   - single_step, run_pc: golden vectors of the compiled reference, plus
+ *     closed-form invariants (uniform velocity => zero strain rate, rest =>
+ *     zero stress, stress-pair antisymmetry => zero net internal force).
+ */
+#include "simp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants: globals.f90:199, data_variables.f90:153-180 ---------- */
+/* Several reference constants are default-REAL literals widened to REAL(8);
+ * (double)(float) reproduces that widening exactly. */
+static const double PI_ = 3.1415926535897932;
+static const double P_STAR = 2.75e4;
+static const double C_EOS = 20.0;
+static const double RHO_ICE = 900.0;
+static const double RHO_WATER = 1026.0;
+static const double C_WIND = 1.2e-3;
+static const double C_WATER = 5.5e-3;
+static const double DELTA_MIN = 2.0e-9;
+static const double B_LENGTH = 8000.0;
+static const double KAPPA_N = 5.4e12;
+static const double K_T = 0.0;            /* T_star / P_star */
+static const double E_ELL = 2.0;          /* 0.5 (P*+T*) / S*, exact */
+static const double HSML_MULT = 3.0;      /* hsml_multiplicity */
+#define RHO_AIR ((double)1.3f)            /* rho_air = 1.3 (REAL(4) literal) */
+
+enum { ORC_OK = 0, ORC_E_ARG = 1, ORC_E_NOMEM = 2 };
+
+struct orc_ctx {
+    /* particle-pair list (data_variables.f90:81-90) */
+    int nip, cap;
+    int *p_i, *p_j;
+    double *r_ij, *hsml_ij;   /* hsml_ij keeps stale entries beyond nip */
+    double *dwdx;             /* (2, nip) */
+    /* boundary-pair list */
+    int bnip, bcap;
+    int *bp_i, *bp_j;
+    double *br_ij;
+    double bins_info[5];
+    /* scratch */
+    int nscr;
+    double *div, *grad;
+};
+
+orc_ctx *orc_create(void) { return (orc_ctx *)calloc(1, sizeof(orc_ctx)); }
+
+void orc_destroy(orc_ctx *c) {
+    if (!c) return;
+    free(c->p_i); free(c->p_j); free(c->r_ij); free(c->hsml_ij); free(c->dwdx);
+    free(c->bp_i); free(c->bp_j); free(c->br_ij);
+    free(c->div); free(c->grad);
+    free(c);
+}
+
+int orc_nip(const orc_ctx *c) { return c->nip; }
+int orc_bnip(const orc_ctx *c) { return c->bnip; }
+
+void orc_get_pairs(const orc_ctx *c, int *p_i, int *p_j, double *r_ij,
+                   double *hsml_ij) {
+    size_t k = (size_t)c->nip;
+    if (p_i) memcpy(p_i, c->p_i, k * sizeof(int));
+    if (p_j) memcpy(p_j, c->p_j, k * sizeof(int));
+    if (r_ij) memcpy(r_ij, c->r_ij, k * sizeof(double));
+    if (hsml_ij) memcpy(hsml_ij, c->hsml_ij, k * sizeof(double));
+}
+
+void orc_get_bpairs(const orc_ctx *c, int *bp_i, int *bp_j, double *br_ij) {
+    size_t k = (size_t)c->bnip;
+    if (bp_i) memcpy(bp_i, c->bp_i, k * sizeof(int));
+    if (bp_j) memcpy(bp_j, c->bp_j, k * sizeof(int));
+    if (br_ij) memcpy(br_ij, c->br_ij, k * sizeof(double));
+}
+
+void orc_get_bins(const orc_ctx *c, double *out) {
+    memcpy(out, c->bins_info, sizeof(c->bins_info));
+}
+
+/* Grow the pair arrays; new hsml_ij entries start at 0 like the zeroed
+ * module array (initialize.f90:58), old entries are kept (stale reads). */
+static int grow_pairs(orc_ctx *c, int need) {
+    if (need <= c->cap) return 0;
+    int nc = c->cap ? c->cap : 1024;
+    while (nc < need) nc *= 2;
+    int *a = (int *)realloc(c->p_i, (size_t)nc * sizeof(int));
+    if (!a) return -1;
+    c->p_i = a;
+    a = (int *)realloc(c->p_j, (size_t)nc * sizeof(int));
+    if (!a) return -1;
+    c->p_j = a;
+    double *d = (double *)realloc(c->r_ij, (size_t)nc * sizeof(double));
+    if (!d) return -1;
+    c->r_ij = d;
+    d = (double *)realloc(c->hsml_ij, (size_t)nc * sizeof(double));
+    if (!d) return -1;
+    c->hsml_ij = d;
+    memset(c->hsml_ij + c->cap, 0, (size_t)(nc - c->cap) * sizeof(double));
+    d = (double *)realloc(c->dwdx, (size_t)nc * 2 * sizeof(double));
+    if (!d) return -1;
+    c->dwdx = d;
+    c->cap = nc;
+    return 0;
+}
+
+static int grow_bpairs(orc_ctx *c, int need) {
+    if (need <= c->bcap) return 0;
+    int nc = c->bcap ? c->bcap : 1024;
+    while (nc < need) nc *= 2;
+    int *a = (int *)realloc(c->bp_i, (size_t)nc * sizeof(int));
+    if (!a) return -1;
+    c->bp_i = a;
+    a = (int *)realloc(c->bp_j, (size_t)nc * sizeof(int));
+    if (!a) return -1;
+    c->bp_j = a;
+    double *d = (double *)realloc(c->br_ij, (size_t)nc * sizeof(double));
+    if (!d) return -1;
+    c->br_ij = d;
+    c->bcap = nc;
+    return 0;
+}
+
+static int push_pair(orc_ctx *c, int i, int j, double d, double mh) {
+    if (c->nip + 1 > c->cap && grow_pairs(c, c->nip + 1)) return -1;
+    c->p_i[c->nip] = i; c->p_j[c->nip] = j;
+    c->r_ij[c->nip] = d; c->hsml_ij[c->nip] = mh;
+    c->nip++;
+    return 0;
+}
+
+static int push_bpair(orc_ctx *c, int b, int j, double d) {
+    if (c->bnip + 1 > c->bcap && grow_bpairs(c, c->bnip + 1)) return -1;
+    c->bp_i[c->bnip] = b; c->bp_j[c->bnip] = j; c->br_ij[c->bnip] = d;
+    c->bnip++;
+    return 0;
+}
+
+/* x**k for a small integer constant k, expanded as the reference build does
+ * it: a left-to-right product x*x*...*x (k factors). */
+static double ipow(double x, int k) {
+    double r = x;
+    for (int i = 1; i < k; ++i) r = r * x;
+    return r;
+}
+
+/* Fortran NINT of a REAL(8): round half away from zero. */
+static int nint_(double v) { return (int)lround(v); }
+
+/* SQRT(SUM((a - b)**2)) for dim = 2 (nnps.f90:277 and siblings). */
+static double dist2d(const double *a, const double *b) {
+    double dx = a[0] - b[0];
+    double dy = a[1] - b[1];
+    return sqrt(dx * dx + dy * dy);
+}
+
+/* ---- bucket_search, nnps.f90:155-517 (single thread order) ----------- */
+int orc_nnps(orc_ctx *c, int n, const double *x, const double *hsml, int nb,
+             const double *xb) {
+    if (n < 1 || nb < 0) return ORC_E_ARG;
+    c->nip = 0;
+    c->bnip = 0;
+    /* bin_size = 2 * MINVAL(hsml) (:211) */
+    double hmin = hsml[0];
+    for (int i = 1; i < n; ++i) if (hsml[i] < hmin) hmin = hsml[i];
+    double bin = 2 * hmin;
+    /* bucket_coor over [xb, x] (:212-216) */
+    double mnx = x[0], mxx = x[0], mny = x[1], mxy = x[1];
+    for (int i = 0; i < n; ++i) {
+        if (x[2 * i] < mnx) mnx = x[2 * i];
+        if (x[2 * i] > mxx) mxx = x[2 * i];
+        if (x[2 * i + 1] < mny) mny = x[2 * i + 1];
+        if (x[2 * i + 1] > mxy) mxy = x[2 * i + 1];
+    }
+    for (int b = 0; b < nb; ++b) {
+        if (xb[2 * b] < mnx) mnx = xb[2 * b];
+        if (xb[2 * b] > mxx) mxx = xb[2 * b];
+        if (xb[2 * b + 1] < mny) mny = xb[2 * b + 1];
+        if (xb[2 * b + 1] > mxy) mxy = xb[2 * b + 1];
+    }
+    int c1 = nint_(mnx / bin), c2 = nint_(mxx / bin);
+    int c3 = nint_(mny / bin), c4 = nint_(mxy / bin);
+    c->bins_info[0] = bin;
+    c->bins_info[1] = c1; c->bins_info[2] = c2;
+    c->bins_info[3] = c3; c->bins_info[4] = c4;
+    /* number_bin(c1-1:c2+1, c3-1:c4+1) and bins(:, ...) (:217-224);
+     * the reference's per-cell capacity max_ind_int is not enforced here. */
+    long W = (long)(c2 - c1) + 3, H = (long)(c4 - c3) + 3;
+    long ncell = W * H;
+    int *cnt = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *start = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *list = (int *)malloc((size_t)n * sizeof(int));
+    int *fill = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    if (!cnt || !start || !list || !fill) {
+        free(cnt); free(start); free(list); free(fill);
+        return ORC_E_NOMEM;
+    }
+#define CELL(k, l) (((long)(l) - (c3 - 1)) * W + ((long)(k) - (c1 - 1)))
+    for (int i = 0; i < n; ++i) {
+        int xb_ = nint_(x[2 * i] / bin), yb_ = nint_(x[2 * i + 1] / bin);
+        cnt[CELL(xb_, yb_)]++;
+    }
+    for (long q = 0; q < ncell; ++q) start[q + 1] = start[q] + cnt[q];
+    /* serial fill in ascending particle index (:246-254) */
+    for (int i = 0; i < n; ++i) {
+        long q = CELL(nint_(x[2 * i] / bin), nint_(x[2 * i + 1] / bin));
+        list[start[q] + fill[q]++] = i;
+    }
+    int rc = ORC_OK;
+    /* cell loops (:259-411): l = y outer, k = x inner */
+    for (int l = c3; l <= c4 && rc == ORC_OK; ++l) {
+        for (int k = c1; k <= c2 && rc == ORC_OK; ++k) {
+            long q0 = CELL(k, l);
+            int s0 = start[q0], n0 = cnt[q0];
+            /* within the cell, i < j in bin order (:265-297) */
+            for (int a = 0; a < n0 - 1; ++a) {
+                int i = list[s0 + a];
+                for (int b = a + 1; b < n0; ++b) {
+                    int j = list[s0 + b];
+                    double d = dist2d(x + 2 * i, x + 2 * j);
+                    double mh = (hsml[i] + hsml[j]) / 2.;
+                    if (d <= mh && push_pair(c, i + 1, j + 1, d, mh)) rc = ORC_E_NOMEM;
+                }
+            }
+            /* right, up, up-right, up-left neighbours (:300-407) */
+            const long nq[4] = {CELL(k + 1, l), CELL(k, l + 1),
+                                CELL(k + 1, l + 1), CELL(k - 1, l + 1)};
+            for (int a = 0; a < n0; ++a) {
+                int i = list[s0 + a];
+                for (int t = 0; t < 4; ++t) {
+                    int s1 = start[nq[t]], n1 = cnt[nq[t]];
+                    for (int b = 0; b < n1; ++b) {
+                        int j = list[s1 + b];
+                        double d = dist2d(x + 2 * i, x + 2 * j);
+                        double mh = (hsml[i] + hsml[j]) / 2.;
+                        if (d <= mh && push_pair(c, i + 1, j + 1, d, mh)) rc = ORC_E_NOMEM;
+                    }
+                }
+            }
+        }
+    }
+    /* boundary pairs (:451-493): 3x3 cells around each boundary particle,
+     * y outer, x inner, bin order within a cell */
+    for (int b = 0; b < nb && rc == ORC_OK; ++b) {
+        int xbin = nint_(xb[2 * b] / bin), ybin = nint_(xb[2 * b + 1] / bin);
+        for (int kk = ybin - 1; kk <= ybin + 1; ++kk) {
+            for (int jj = xbin - 1; jj <= xbin + 1; ++jj) {
+                long q = CELL(jj, kk);
+                for (int t = 0; t < cnt[q]; ++t) {
+                    int j = list[start[q] + t];
+                    double d = dist2d(xb + 2 * b, x + 2 * j);
+                    if (d <= B_LENGTH && push_bpair(c, b + 1, j + 1, d)) rc = ORC_E_NOMEM;
+                }
+            }
+        }
+    }
+#undef CELL
+    free(cnt); free(start); free(list); free(fill);
+    return rc;
+}
+
+/* ---- direct_find, nnps.f90:41-148 ------------------------------------ */
+int orc_direct_find(orc_ctx *c, int n, const double *x, const double *hsml,
+                    int nb, const double *xb) {
+    if (n < 1 || nb < 0) return ORC_E_ARG;
+    c->nip = 0;
+    c->bnip = 0;
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double d = dist2d(x + 2 * i, x + 2 * j);
+            double mh = (hsml[i] + hsml[j]) / 2.;
+            if (d <= mh && push_pair(c, i + 1, j + 1, d, mh)) return ORC_E_NOMEM;
+        }
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < n; ++j) {
+            double d = dist2d(xb + 2 * b, x + 2 * j);
+            if (d <= B_LENGTH && push_bpair(c, b + 1, j + 1, d)) return ORC_E_NOMEM;
+        }
+    return ORC_OK;
+}
+
+/* Wind at a particle: uniform, or bilinear on a regular grid (clamped at
+ * the grid edge).  The reference hard-codes a uniform field
+ * (wind_forcing.f90:36-37); the grid form carries the synthetic forcing
+ * fields of the larger workloads. */
+static void wind_at(const orc_params *prm, double px, double py, double *u,
+                    double *v) {
+    if (prm->wind_mode != 1) {
+        *u = prm->wind[0];
+        *v = prm->wind[1];
+        return;
+    }
+    int nx = prm->grid_nx, ny = prm->grid_ny;
+    double fx = (px - prm->grid_x0) / prm->grid_dx;
+    double fy = (py - prm->grid_y0) / prm->grid_dy;
+    double flx = floor(fx), fly = floor(fy);
+    int i0 = (int)fmin(fmax(flx, 0.0), (double)(nx - 2));
+    int j0 = (int)fmin(fmax(fly, 0.0), (double)(ny - 2));
+    double tx = fmin(fmax(fx - (double)i0, 0.0), 1.0);
+    double ty = fmin(fmax(fy - (double)j0, 0.0), 1.0);
+    const double *U = prm->grid_u, *V = prm->grid_v;
+    long r0 = (long)j0 * nx, r1 = r0 + nx;
+    double u0 = (1.0 - tx) * U[r0 + i0] + tx * U[r0 + i0 + 1];
+    double u1 = (1.0 - tx) * U[r1 + i0] + tx * U[r1 + i0 + 1];
+    double v0 = (1.0 - tx) * V[r0 + i0] + tx * V[r0 + i0 + 1];
+    double v1 = (1.0 - tx) * V[r1 + i0] + tx * V[r1 + i0 + 1];
+    *u = (1.0 - ty) * u0 + ty * u1;
+    *v = (1.0 - ty) * v0 + ty * v1;
+}
+
+static int ensure_scratch(orc_ctx *c, int n) {
+    if (n <= c->nscr) return 0;
+    free(c->div); free(c->grad);
+    c->div = (double *)malloc((size_t)n * sizeof(double));
+    c->grad = (double *)malloc((size_t)n * 4 * sizeof(double));
+    if (!c->div || !c->grad) { c->nscr = 0; return -1; }
+    c->nscr = n;
+    return 0;
+}
+
+/* ---- single_step, single_step.f90:5-108 ------------------------------ */
+int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
+                    const double *vx, const double *mass, const double *rho,
+                    const double *hsml, const double *h, const double *A,
+                    int nb, const double *xb, const double *massb,
+                    double current_time, double *dxdt, double *dvxdt,
+                    double *dhdt, double *dAdt, double *eps3, double *zeta,
+                    double *eta, double *p, double *sig3) {
+    if (n < 1) return ORC_E_ARG;
+    if (ensure_scratch(c, n)) return ORC_E_NOMEM;
+    double *div = c->div, *grad = c->grad;   /* grad(4, n) */
+    /* reset (:36-44) */
+    for (int i = 0; i < n; ++i) {
+        dxdt[2 * i] = dxdt[2 * i + 1] = 0.;
+        dvxdt[2 * i] = dvxdt[2 * i + 1] = 0.;
+        dAdt[i] = dhdt[i] = 0.;
+        div[i] = 0.;
+        grad[4 * i] = grad[4 * i + 1] = grad[4 * i + 2] = grad[4 * i + 3] = 0.;
+    }
+    /* NNPS (:47) */
+    int rc = orc_nnps(c, n, x, hsml, nb, xb);
+    if (rc) return rc;
+    /* pair loop (:56-75): kernel (Wendland_C6, kernel.f90:471-520),
+     * vector_divergence (vector_divergence.f90:27-28),
+     * vector_gradient (vector_gradient.f90:27-40) */
+    const double c6 = 78. / (7. * PI_);
+    for (int k = 0; k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        double dx0 = x[2 * i] - x[2 * j], dx1 = x[2 * i + 1] - x[2 * j + 1];
+        double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+        double r = c->r_ij[k], hij = c->hsml_ij[k];
+        double q = r / hij * 1.0;                       /* scale_h = 1 */
+        double ih = 1.0 / hij;
+        double factor = c6 * (ih * ih);
+        double omq = 1 - q;
+        double o7 = ipow(omq, 7);
+        double g = factor * (-(22. * q * o7 * (16 * ipow(q, 2) + 7. * q + 1))) / hij;
+        double dw0 = g * (dx0 / r) * 1.0;
+        double dw1 = g * (dx1 / r) * 1.0;
+        c->dwdx[2 * k] = dw0;
+        c->dwdx[2 * k + 1] = dw1;
+        double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
+        div[i] = div[i] - mass[j] * (w1 + w4);
+        div[j] = div[j] - mass[i] * (w1 + w4);
+        grad[4 * i + 0] = grad[4 * i + 0] - mass[j] * w1 / rho[j];
+        grad[4 * j + 0] = grad[4 * j + 0] - mass[i] * w1 / rho[i];
+        grad[4 * i + 1] = grad[4 * i + 1] - mass[j] * w2 / rho[j];
+        grad[4 * j + 1] = grad[4 * j + 1] - mass[i] * w2 / rho[i];
+        grad[4 * i + 2] = grad[4 * i + 2] - mass[j] * w3 / rho[j];
+        grad[4 * j + 2] = grad[4 * j + 2] - mass[i] * w3 / rho[i];
+        grad[4 * i + 3] = grad[4 * i + 3] - mass[j] * w4 / rho[j];
+        grad[4 * j + 3] = grad[4 * j + 3] - mass[i] * w4 / rho[i];
+    }
+    /* strain rate (:80-84) */
+    for (int i = 0; i < n; ++i) {
+        eps3[3 * i + 0] = grad[4 * i + 0];
+        eps3[3 * i + 2] = grad[4 * i + 3];
+        eps3[3 * i + 1] = 0.5 * (grad[4 * i + 1] + grad[4 * i + 2]);
+    }
+    /* internal_stress (internal_stress.f90:21-53): Hibler P (:24),
+     * elliptical_bulk_viscosity (bulk_viscosity.f90:65-76),
+     * elliptical_shear_viscosity (shear_viscosity.f90:57),
+     * kreyscher_eos (eos.f90:141-147), stress_tensor (stress_tensor.f90:31-39) */
+    const double em2 = 1.0 / (E_ELL * E_ELL);       /* e**(-2) */
+    for (int i = 0; i < n; ++i) {
+        double e11 = eps3[3 * i], e12 = eps3[3 * i + 1], e22 = eps3[3 * i + 2];
+        double P = P_STAR * h[i] * exp(-(C_EOS * (1 - A[i])));
+        double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) +
+                            4 * em2 * (e12 * e12) +
+                            2 * e11 * e22 * (1 - em2));
+        double dcl = fabs(delta) < DELTA_MIN ? DELTA_MIN : delta;
+        double z = 0.5 * P * (1 + K_T) / dcl;
+        double et = z / (E_ELL * E_ELL);
+        double pr = z * 2 * delta / (1 + K_T);
+        zeta[i] = z;
+        eta[i] = et;
+        p[i] = pr;
+        sig3[3 * i + 0] = (z + et) * e11 + (z - et) * e22 - pr * (1 - K_T) / 2;
+        sig3[3 * i + 1] = 2 * et * e12;
+        sig3[3 * i + 2] = (z + et) * e22 + (z - et) * e11 - pr * (1 - K_T) / 2;
+    }
+    /* stress_1 (internal_stress.f90:103-136) */
+    for (int k = 0; k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        double dw0 = c->dwdx[2 * k], dw1 = c->dwdx[2 * k + 1];
+        double s11i = sig3[3 * i], s12i = sig3[3 * i + 1], s22i = sig3[3 * i + 2];
+        double s11j = sig3[3 * j], s12j = sig3[3 * j + 1], s22j = sig3[3 * j + 2];
+        double cj = (s11j * dw0 + s12j * dw1) / (rho[j] * rho[j]);
+        double ci = (s11i * dw0 + s12i * dw1) / (rho[i] * rho[i]);
+        dvxdt[2 * i] = dvxdt[2 * i] + rho[i] * mass[j] / RHO_ICE / h[i] * (ci + cj);
+        dvxdt[2 * j] = dvxdt[2 * j] + rho[j] * mass[i] / RHO_ICE / h[j] * (-ci - cj);
+        cj = (s12j * dw0 + s22j * dw1) / (rho[j] * rho[j]);
+        ci = (s12i * dw0 + s22i * dw1) / (rho[i] * rho[i]);
+        dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + rho[i] * mass[j] / RHO_ICE / h[i] * (ci + cj);
+        dvxdt[2 * j + 1] = dvxdt[2 * j + 1] + rho[j] * mass[i] / RHO_ICE / h[j] * (-ci - cj);
+    }
+    /* boundary_kernel -> b_Wendland_C6 (boundary_kernel.f90:477-502) ->
+     * bf_Monaghan (boundary_force.f90:78-82).  The kernel factor reads
+     * hsml_ij at the boundary-pair index, i.e. an entry of the particle-pair
+     * list (:491); stale entries beyond nip are read as they are. */
+    for (int k = 0; k < c->bnip; ++k) {
+        int i = c->bp_j[k] - 1, b = c->bp_i[k] - 1;
+        double bdx0 = xb[2 * b] - x[2 * i], bdx1 = xb[2 * b + 1] - x[2 * i + 1];
+        double br = c->br_ij[k];
+        double q = br / B_LENGTH * 1.0;
+        double hk = k < c->cap ? c->hsml_ij[k] : 0.0;
+        double ih = 1.0 / hk;
+        double factor = c6 * (ih * ih);
+        double omq = 1 - q;
+        double bw = factor * ipow(omq, 8) *
+                    (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
+        double den = (mass[i] + massb[b]) * (br * br);
+        double F0 = KAPPA_N * bdx0 * bw * 2 * massb[b] / den;
+        double F1 = KAPPA_N * bdx1 * bw * 2 * massb[b] / den;
+        dvxdt[2 * i] = dvxdt[2 * i] + (-F0) / rho[i];
+        dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (-F1) / rho[i];
+    }
+    /* wind_forcing (wind_forcing.f90:35-50) and water_drag
+     * (water_drag.f90:35-51): both gated on current_time > 0 */
+    if (current_time > 0.) {
+        for (int i = 0; i < n; ++i) {
+            double wu, wv;
+            wind_at(prm, x[2 * i], x[2 * i + 1], &wu, &wv);
+            double du = wu - vx[2 * i], dv = wv - vx[2 * i + 1];
+            double hr = h[i] * RHO_ICE;
+            dvxdt[2 * i] = dvxdt[2 * i] + (C_WIND * RHO_AIR * du * fabs(du)) / hr;
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (C_WIND * RHO_AIR * dv * fabs(dv)) / hr;
+        }
+        for (int i = 0; i < n; ++i) {
+            double du = prm->current[0] - vx[2 * i], dv = prm->current[1] - vx[2 * i + 1];
+            double hr = h[i] * RHO_ICE;
+            dvxdt[2 * i] = dvxdt[2 * i] + (C_WATER * RHO_WATER * du * fabs(du)) / hr;
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (C_WATER * RHO_WATER * dv * fabs(dv)) / hr;
+        }
+    }
+    /* tendencies (:104-106) */
+    for (int i = 0; i < n; ++i) {
+        dhdt[i] = -(h[i] * div[i] / rho[i]);
+        dAdt[i] = -(A[i] * div[i] / rho[i]);
+        dxdt[2 * i] = dxdt[2 * i] + vx[2 * i];
+        dxdt[2 * i + 1] = dxdt[2 * i + 1] + vx[2 * i + 1];
+    }
+    return ORC_OK;
+}
+
+/* VP_time_step (time_step.f90:155) */
+double orc_vp_dt(int n, const double *hsml) {
+    double dt = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double v = RHO_ICE * (hsml[i] * hsml[i]) * DELTA_MIN / (P_STAR * (1 + K_T));
+        if (i == 0 || v < dt) dt = v;
+    }
+    return dt;
+}
+
+/* ---- time_integration + predictor_corrector -------------------------- */
+int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
+               double *vx, const double *mass, double *rho, double *hsml,
+               double *h, double *A, int nb, const double *xb,
+               const double *massb, double *dt_io, double *time_io,
+               int *itimestep, double *eps3_o, double *zeta_o, double *eta_o,
+               double *p_o, double *sig3_o) {
+    if (n < 1 || nsteps < 0) return ORC_E_ARG;
+    size_t n2 = (size_t)n * 2;
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 29);
+    if (!buf) return ORC_E_NOMEM;
+    double *x1 = buf, *v1 = x1 + n2, *h1 = v1 + n2, *A1 = h1 + n;
+    double *dxdt = A1 + n, *dvxdt = dxdt + n2, *dhdt = dvxdt + n2, *dAdt = dhdt + n;
+    double *eps3 = dAdt + n, *sig3 = eps3 + 3 * (size_t)n;
+    double *zeta = sig3 + 3 * (size_t)n, *eta = zeta + n, *pr = eta + n;
+    double dt = *dt_io, t = *time_io;
+    int it = *itimestep;
+    int rc = ORC_OK;
+    for (int s = 0; s < nsteps && rc == ORC_OK; ++s) {
+        /* monitor current quantities (:197-201) */
+        memcpy(x1, x, n2 * sizeof(double));
+        memcpy(v1, vx, n2 * sizeof(double));
+        memcpy(h1, h, (size_t)n * sizeof(double));
+        memcpy(A1, A, (size_t)n * sizeof(double));
+        rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
+                             massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
+                             eta, pr, sig3);
+        if (rc) break;
+        /* predict half step (:207-215), density (:218) */
+        for (size_t i = 0; i < n2; ++i) {
+            x[i] = x[i] + dxdt[i] * dt / 2;
+            vx[i] = vx[i] + dvxdt[i] * dt / 2;
+        }
+        for (int i = 0; i < n; ++i) {
+            h[i] = h[i] + dhdt[i] * dt / 2;
+            A[i] = A[i] + dAdt[i] * dt / 2;
+            if (A[i] > 1) A[i] = 1;
+            rho[i] = h[i] * RHO_ICE;           /* diagnostic_density */
+        }
+        rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
+                             massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
+                             eta, pr, sig3);
+        if (rc) break;
+        /* corrector (:236-262) */
+        for (size_t i = 0; i < n2; ++i) {
+            double x12 = x1[i] + dxdt[i] * dt / 2;
+            double v12 = v1[i] + dvxdt[i] * dt / 2;
+            x[i] = 2 * x12 - x1[i];
+            vx[i] = 2 * v12 - v1[i];
+        }
+        for (int i = 0; i < n; ++i) {
+            double h12 = h1[i] + dhdt[i] * dt / 2;
+            double A12 = A1[i] + dAdt[i] * dt / 2;
+            h[i] = 2 * h12 - h1[i];
+            A[i] = 2 * A12 - A1[i];
+            if (A[i] > 1) A[i] = 1;
+            rho[i] = h[i] * RHO_ICE;
+            /* Marquis_hsml (smoothing_length.f90:136-142) */
+            hsml[i] = sqrt(mass[i] / rho[i]) * HSML_MULT;
+            if (hsml[i] > prm->max_hsml) hsml[i] = prm->max_hsml;
+        }
+        /* time_step (:280) */
+        if (prm->tse == 2) dt = orc_vp_dt(n, hsml);
+        /* loop bookkeeping (time_integration.f90:68-71) */
+        it += 1;
+        t = t + dt;
+    }
+    if (rc == ORC_OK) {
+        if (eps3_o) memcpy(eps3_o, eps3, 3 * (size_t)n * sizeof(double));
+        if (sig3_o) memcpy(sig3_o, sig3, 3 * (size_t)n * sizeof(double));
+        if (zeta_o) memcpy(zeta_o, zeta, (size_t)n * sizeof(double));
+        if (eta_o) memcpy(eta_o, eta, (size_t)n * sizeof(double));
+        if (p_o) memcpy(p_o, pr, (size_t)n * sizeof(double));
+    }
+    *dt_io = dt;
+    *time_io = t;
+    *itimestep = it;
+    free(buf);
+    return rc;
+}

@@ -1,0 +1,84 @@
+/*
+ * simp_oracle.h -- CPU FP64 restatement of SIMP's SPH viscous-plastic step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path
+ * in simp_amd/ is compared against.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it; the product never does.
+ *
+ * Layout follows the reference's Fortran module state: x(dim, n) is
+ * interleaved (x1, y1, x2, y2, ...); pair and boundary-pair indices are
+ * 1-based, exactly as NNPS writes them.
+ */
+#ifndef SIMP_ORACLE_H
+#define SIMP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* Forcing and switches of one simulation (the runtime form of the
+ * compile-time PARAMETERs the reference uses on this path). */
+typedef struct orc_params {
+    int    tse;          /* 0: keep dt (time_step.f90:21-22); 2: VP dt (:132-157) */
+    int    wind_mode;    /* 0: uniform wind; 1: gridded wind, bilinear sampling */
+    double wind[2];      /* uniform wind (m/s); the reference hard-codes (0, -5.3f) */
+    double current[2];   /* uniform ocean current (m/s); reference: (0, 0) */
+    double max_hsml;     /* smoothing-length cap (initialize.f90:191) */
+    int    grid_nx, grid_ny;
+    double grid_x0, grid_y0, grid_dx, grid_dy;
+    const double *grid_u;  /* grid_ny * grid_nx, row-major (y outer) */
+    const double *grid_v;
+} orc_params;
+
+orc_ctx *orc_create(void);
+void     orc_destroy(orc_ctx *c);
+
+/* bucket_search (nnps.f90:155-517): pair list in the reference's order.
+ * Arrays are owned by the context and valid until the next call. */
+int orc_nnps(orc_ctx *c, int n, const double *x, const double *hsml,
+             int nb, const double *xb);
+/* direct_find (nnps.f90:41-148): O(N^2) brute force, same arrays. */
+int orc_direct_find(orc_ctx *c, int n, const double *x, const double *hsml,
+                    int nb, const double *xb);
+int orc_nip(const orc_ctx *c);
+int orc_bnip(const orc_ctx *c);
+/* Copy the last pair lists out (1-based indices). */
+void orc_get_pairs(const orc_ctx *c, int *p_i, int *p_j, double *r_ij,
+                   double *hsml_ij);
+void orc_get_bpairs(const orc_ctx *c, int *bp_i, int *bp_j, double *br_ij);
+/* Reference NNPS bin statistics of the last orc_nnps call:
+ * out[0] = bin_size, out[1..4] = bucket_coor (xlo, xhi, ylo, yhi). */
+void orc_get_bins(const orc_ctx *c, double *out);
+
+/* single_step (single_step.f90:5-108) on interleaved host arrays.
+ * eps3 and sig3 are (3, n): (e11, e12, e22) and (s11, s12, s22). */
+int orc_single_step(orc_ctx *c, const orc_params *prm, int n,
+                    const double *x, const double *vx, const double *mass,
+                    const double *rho, const double *hsml, const double *h,
+                    const double *A, int nb, const double *xb,
+                    const double *massb, double current_time,
+                    double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                    double *eps3, double *zeta, double *eta, double *p,
+                    double *sig3);
+
+/* nsteps of time_integration's loop (time_integration.f90:38-73) with the
+ * predictor_corrector scheme (:155-282).  State arrays are updated in place;
+ * dt, current_time and itimestep advance as in the reference.  The
+ * diagnostic outputs (eps3, zeta, eta, p, sig3) hold the last single_step's
+ * values and may be NULL. */
+int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n,
+               double *x, double *vx, const double *mass, double *rho,
+               double *hsml, double *h, double *A, int nb, const double *xb,
+               const double *massb, double *dt, double *current_time,
+               int *itimestep, double *eps3, double *zeta, double *eta,
+               double *p, double *sig3);
+
+/* VP_time_step (time_step.f90:155). */
+double orc_vp_dt(int n, const double *hsml);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""Synthetic initial states for the SPH viscous-plastic step.
+
+Each scenario follows the reference's initialisation conventions
+(``in_out/initialize.f90:190-197``): ``rho = rho_ice * h``,
+``mass = rho * (hsml / 3)**2``, ``max_hsml = 10 * max(hsml)``, velocities at
+rest, and boundary particles with ``massb = mean(mass)``
+(``in_out/boundaries.f90:52``) placed on the walls at spacing
+``b_length / 3`` (``in_out/boundaries.f90:204``).  Lattice particles sit at
+``((i + 1/2) s, (j + 1/2) s)`` with i.i.d. uniform jitter of +-5 % of ``s``.
+
+Named workloads:
+
+* ``box``         closed square box (4 walls), uniform thickness spread.
+* ``vortex``      closed box, lognormal thickness, Rankine-vortex wind.
+* ``ice_edge``    open ice edge: concentration ramp, variable ``hsml``,
+                  one coast wall, Gaussian-random-field wind.
+* ``ensemble``    many independent boxes of uneven size.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+RHO_ICE = 900.0
+B_LENGTH = 8000.0
+HSML_MULT = 3.0
+#: the reference's hard-coded wind, a default-REAL literal widened to REAL(8)
+#: (``ext_force/wind_forcing.f90:36-37``)
+REFERENCE_WIND = (0.0, float(np.float32(-5.3)))
+
+
+@dataclasses.dataclass
+class WindGrid:
+    """Static wind field on a regular grid, sampled bilinearly."""
+    x0: float
+    y0: float
+    dx: float
+    dy: float
+    u: np.ndarray  # (ny, nx)
+    v: np.ndarray  # (ny, nx)
+
+
+@dataclasses.dataclass
+class Scenario:
+    """Initial state of one simulation (host arrays, FP64)."""
+    x: np.ndarray        # (n, 2)
+    vx: np.ndarray       # (n, 2)
+    mass: np.ndarray     # (n,)
+    rho: np.ndarray
+    hsml: np.ndarray
+    h: np.ndarray
+    A: np.ndarray
+    xb: np.ndarray       # (nb, 2)
+    massb: np.ndarray    # (nb,)
+    max_hsml: float
+    wind: tuple = REFERENCE_WIND
+    wind_grid: Optional[WindGrid] = None
+    current: tuple = (0.0, 0.0)
+    tse: int = 2
+    dt: float = 0.0
+    time: float = 0.0
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def nb(self) -> int:
+        return int(self.xb.shape[0])
+
+    def copy(self) -> "Scenario":
+        return dataclasses.replace(
+            self, x=self.x.copy(), vx=self.vx.copy(), mass=self.mass.copy(),
+            rho=self.rho.copy(), hsml=self.hsml.copy(), h=self.h.copy(),
+            A=self.A.copy(), xb=self.xb.copy(), massb=self.massb.copy())
+
+
+def vp_dt(hsml: np.ndarray) -> float:
+    """VP time step, ``integration/time_step.f90:155``."""
+    return float(np.min(RHO_ICE * (hsml * hsml) * 2.0e-9 / (2.75e4 * 1.0)))
+
+
+def lattice(nx: int, ny: int, s: float, rng: np.random.Generator,
+            jitter: float = 0.05, x0: float = 0.0, y0: float = 0.0) -> np.ndarray:
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    pts = np.stack([(i.ravel() + 0.5) * s + x0, (j.ravel() + 0.5) * s + y0], axis=1)
+    pts += rng.uniform(-jitter * s, jitter * s, size=pts.shape)
+    return pts
+
+
+def walls(x0: float, y0: float, x1: float, y1: float,
+          sides=("left", "right", "bottom", "top")) -> np.ndarray:
+    """Boundary particles on the named sides of a rectangle, spacing at most
+    ``b_length / 3``; corners are shared, not duplicated."""
+    sp = B_LENGTH / 3.0
+    pts = []
+
+    def seg(a, b):
+        L = math.hypot(b[0] - a[0], b[1] - a[1])
+        m = max(1, int(math.ceil(L / sp)))
+        t = np.arange(m + 1) / m
+        return np.stack([a[0] + t * (b[0] - a[0]), a[1] + t * (b[1] - a[1])], axis=1)
+
+    corners = {"bottom": ((x0, y0), (x1, y0)), "top": ((x0, y1), (x1, y1)),
+               "left": ((x0, y0), (x0, y1)), "right": ((x1, y0), (x1, y1))}
+    for s_ in sides:
+        pts.append(seg(*corners[s_]))
+    if not pts:
+        return np.zeros((0, 2))
+    P = np.concatenate(pts)
+    # drop duplicated corners (exact coordinates)
+    _, idx = np.unique(np.round(P, 6), axis=0, return_index=True)
+    return P[np.sort(idx)]
+
+
+def lognormal_thickness(rng, n, mean=1.5, sd=0.5):
+    s2 = math.log(1.0 + sd * sd / (mean * mean))
+    mu = math.log(mean * mean / math.sqrt(mean * mean + sd * sd))
+    return rng.lognormal(mu, math.sqrt(s2), size=n)
+
+
+def _finish(x, hsml, h, A, xb, rng, **kw) -> Scenario:
+    n = x.shape[0]
+    rho = RHO_ICE * h
+    mass = rho * (hsml / HSML_MULT) ** 2
+    massb = np.full(xb.shape[0], mass.sum() / n)
+    sc = Scenario(x=np.ascontiguousarray(x, dtype=np.float64),
+                  vx=np.zeros((n, 2)), mass=mass, rho=rho,
+                  hsml=np.ascontiguousarray(hsml, dtype=np.float64),
+                  h=np.ascontiguousarray(h, dtype=np.float64),
+                  A=np.ascontiguousarray(A, dtype=np.float64),
+                  xb=np.ascontiguousarray(xb, dtype=np.float64), massb=massb,
+                  max_hsml=float(hsml.max() * 10.0), **kw)
+    if sc.tse == 2 and sc.dt == 0.0:
+        sc.dt = vp_dt(sc.hsml)
+    return sc
+
+
+def box(nside: int = 50, spacing: float = 2000.0, seed: int = 1,
+        hsml_factor: float = 1.3, wind=REFERENCE_WIND, h_range=(0.8, 1.2),
+        tse: int = 2, dt: float = 0.0) -> Scenario:
+    """Closed square box with 4 walls (the 2,500-particle case at defaults)."""
+    rng = np.random.default_rng(seed)
+    L = nside * spacing
+    x = lattice(nside, nside, spacing, rng)
+    n = x.shape[0]
+    hsml = np.full(n, hsml_factor * spacing)
+    h = rng.uniform(h_range[0], h_range[1], size=n)
+    A = np.ones(n)
+    xb = walls(0.0, 0.0, L, L)
+    return _finish(x, hsml, h, A, xb, rng, wind=tuple(wind), tse=tse, dt=dt,
+                   name=f"box{n}")
+
+
+def rankine_grid(L: float, vmax: float, radius: float, gdx: float) -> WindGrid:
+    m = int(math.ceil(L / gdx)) + 1
+    g = np.arange(m) * gdx
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    cx = cy = L / 2.0
+    rx, ry = X - cx, Y - cy
+    r = np.hypot(rx, ry)
+    speed = np.where(r <= radius, vmax * r / radius,
+                     vmax * radius / np.maximum(r, 1e-9))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = np.where(r > 0, -speed * ry / np.maximum(r, 1e-9), 0.0)
+        v = np.where(r > 0, speed * rx / np.maximum(r, 1e-9), 0.0)
+    return WindGrid(0.0, 0.0, gdx, gdx, np.ascontiguousarray(u), np.ascontiguousarray(v))
+
+
+def vortex(nside: int = 200, spacing: float = 1000.0, seed: int = 2,
+           tse: int = 2, dt: float = 0.0) -> Scenario:
+    """Closed box, lognormal thickness, counter-clockwise Rankine vortex wind
+    (15 m/s at 60 km) -- the 40,000-particle case at defaults."""
+    rng = np.random.default_rng(seed)
+    L = nside * spacing
+    x = lattice(nside, nside, spacing, rng)
+    n = x.shape[0]
+    hsml = np.full(n, 1.3 * spacing)
+    h = lognormal_thickness(rng, n)
+    A = np.ones(n)
+    xb = walls(0.0, 0.0, L, L)
+    grid = rankine_grid(L, 15.0, 60000.0 * (L / 200000.0), 2000.0 * (L / 200000.0))
+    return _finish(x, hsml, h, A, xb, rng, wind_grid=grid, tse=tse, dt=dt,
+                   name=f"vortex{n}")
+
+
+def grf_wind(L: float, std: float, corr: float, gdx: float,
+             rng: np.random.Generator) -> WindGrid:
+    """Gaussian random field per wind component with Gaussian covariance of
+    length ``corr``, synthesised by FFT on a ``gdx`` grid."""
+    m = int(math.ceil(L / gdx)) + 1
+    M = 1
+    while M < 2 * m:
+        M *= 2
+    k = np.fft.fftfreq(M, d=gdx) * 2 * math.pi
+    KX, KY = np.meshgrid(k, k, indexing="xy")
+    # spectrum of exp(-r^2 / (2 l^2)) in 2-D
+    spec = np.exp(-0.5 * (KX ** 2 + KY ** 2) * corr ** 2)
+    amp = np.sqrt(spec)
+    out = []
+    for _ in range(2):
+        noise = rng.standard_normal((M, M))
+        f = np.real(np.fft.ifft2(np.fft.fft2(noise) * amp))[:m, :m]
+        f = (f - f.mean()) / max(f.std(), 1e-30) * std
+        out.append(np.ascontiguousarray(f))
+    return WindGrid(0.0, 0.0, gdx, gdx, out[0], out[1])
+
+
+def ice_edge(nside: int = 500, spacing: float = 1000.0, seed: int = 3,
+             tse: int = 2, dt: float = 0.0) -> Scenario:
+    """Open ice edge: concentration 1 up to 80 % of the domain then a linear
+    ramp to 0.2, ``hsml = 1.3 s / sqrt(A)``, one coast wall at x = 0,
+    Gaussian-random-field wind (std 8 m/s, 50 km correlation) -- the
+    250,000-particle case at defaults."""
+    rng = np.random.default_rng(seed)
+    L = nside * spacing
+    x = lattice(nside, nside, spacing, rng)
+    n = x.shape[0]
+    xr = x[:, 0] / L
+    A = np.where(xr <= 0.8, 1.0, 1.0 - 0.8 * (xr - 0.8) / 0.2)
+    A = np.clip(A, 0.2, 1.0)
+    hsml = 1.3 * spacing / np.sqrt(A)
+    h = lognormal_thickness(rng, n)
+    xb = walls(0.0, 0.0, 0.0, L, sides=("left",))
+    grid = grf_wind(L, 8.0, 50000.0 * (L / 500000.0), 5000.0 * (L / 500000.0), rng)
+    return _finish(x, hsml, h, A, xb, rng, wind_grid=grid, tse=tse, dt=dt,
+                   name=f"ice_edge{n}")
+
+
+def ensemble(count: int = 64, nmin: int = 2000, nmax: int = 60000,
+             seed: int = 4, tse: int = 2) -> List[Scenario]:
+    """Independent closed boxes of lognormal particle count in [nmin, nmax],
+    box size U[50, 300] km, uniform wind of random direction, U[5, 20] m/s."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        N = float(np.clip(rng.lognormal(math.log(12000.0), 0.8), nmin, nmax))
+        nside = max(2, int(round(math.sqrt(N))))
+        L = rng.uniform(50e3, 300e3)
+        s = L / nside
+        ang = rng.uniform(0, 2 * math.pi)
+        spd = rng.uniform(5.0, 20.0)
+        sub = np.random.default_rng(seed * 100003 + k)
+        x = lattice(nside, nside, s, sub)
+        n = x.shape[0]
+        hsml = np.full(n, 1.3 * s)
+        h = lognormal_thickness(sub, n)
+        A = np.ones(n)
+        xb = walls(0.0, 0.0, L, L)
+        out.append(_finish(x, hsml, h, A, xb, sub,
+                           wind=(spd * math.cos(ang), spd * math.sin(ang)),
+                           tse=tse, name=f"ens{k}_{n}"))
+    return out
+
+
+def subdomain(sc: Scenario, x0: float, x1: float, y0: float, y1: float) -> Scenario:
+    """The particles and boundary particles of ``sc`` inside a rectangle (a
+    bounded sample of a larger workload)."""
+    m = (sc.x[:, 0] >= x0) & (sc.x[:, 0] < x1) & (sc.x[:, 1] >= y0) & (sc.x[:, 1] < y1)
+    mb = (sc.xb[:, 0] >= x0 - B_LENGTH) & (sc.xb[:, 0] < x1 + B_LENGTH) & \
+         (sc.xb[:, 1] >= y0 - B_LENGTH) & (sc.xb[:, 1] < y1 + B_LENGTH)
+    out = dataclasses.replace(
+        sc, x=np.ascontiguousarray(sc.x[m]), vx=np.ascontiguousarray(sc.vx[m]),
+        mass=sc.mass[m].copy(), rho=sc.rho[m].copy(), hsml=sc.hsml[m].copy(),
+        h=sc.h[m].copy(), A=sc.A[m].copy(), xb=np.ascontiguousarray(sc.xb[mb]),
+        massb=sc.massb[mb].copy(), name=sc.name + "_sub")
+    if out.tse == 2:
+        out.dt = vp_dt(out.hsml)
+    return out
