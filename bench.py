@@ -1,0 +1,179 @@
+"""Throughput and latency of the SPH viscous-plastic step on one MI355X.
+
+Workload: the open-ice-edge case, 250,000 particles (500 x 500 lattice,
+1 km spacing, 500 x 500 km, coast wall at x = 0, concentration ramp 1 -> 0.2
+over the last 100 km, hsml = 1.3 s / sqrt(A), lognormal thickness,
+Gaussian-random-field wind), synthetic, state resident in HBM.  A step is one
+full predictor-corrector step (two right-hand-side evaluations + update).
+
+Prints one JSON line: mixed-precision particle-steps/s (``value``), the FP64
+rate, p50/p99 step latency, the dominant kernel's roofline (HIP events on the
+library's stream), and the host-CPU baseline (the compiled reference on a
+bounded sub-domain sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per particle of one launch of each heavy kernel
+# (compulsory unique traffic, neighbour reads counted once).
+#   passA : state x y hsml vx vy mass rho h A (72) + cellid clpos (8)
+#           -> sigma (3 x 8 | 3 x 4) + div (8) + own_cnt (4)
+#   passB : RHS state x y hsml rho mass h A vx vy (72) + sigma/div (32 | 20)
+#           + cellid clpos own_cnt (12) + step-start x y vx vy h A (48, corrector)
+#           -> x y vx vy h A rho hsml (64)
+ALG_BYTES = {
+    ("passA", 0): 72 + 8 + 24 + 8 + 4,
+    ("passA", 1): 72 + 8 + 12 + 8 + 4,
+    ("passB", 0): 72 + 32 + 12 + 48 + 64,
+    ("passB", 1): 72 + 20 + 12 + 48 + 64,
+}
+# whole step: 576 B (FP64) / 512 B (mixed) per particle-step
+STEP_BYTES = {0: 576, 1: 512}
+
+
+def _timed_steps(dev, steps):
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.step(steps, sync=True)
+    return time.perf_counter() - t0
+
+
+def _latencies(dev, count):
+    lat = []
+    for _ in range(count):
+        t0 = time.perf_counter()
+        dev.step(1, sync=True)
+        lat.append(time.perf_counter() - t0)
+    return np.array(lat)
+
+
+def cpu_baseline(sc_full, target_s=15.0):
+    """The compiled reference (serial) on a bounded sample: the 250 x 250 km
+    corner of the workload (62,500 particles with their part of the coast
+    wall), its own hard-coded wind, VP time step."""
+    from oracle import capi, refrun
+    from simp_amd import scenarios as S
+    L = 250e3
+    sub = S.subdomain(sc_full, 0.0, L, 0.0, L)
+    sub.wind_grid = None
+    sub.wind = S.REFERENCE_WIND
+    sub.time = 0.0
+    if refrun.available():
+        probe = refrun.run(refrun.MODE_TIME_INTEGRATION, sub, integration_time=6.5 * sub.dt)
+        per_step = probe["wall"] / max(probe["itimestep"], 1)
+        steps = int(max(5, min(2000, target_s / max(per_step, 1e-6))))
+        r = refrun.run(refrun.MODE_TIME_INTEGRATION, sub,
+                       integration_time=(steps - 0.5) * sub.dt)
+        k = r["itimestep"]
+        return {"value": sub.n * k / r["wall"], "unit": "particle-steps/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"reference Fortran (serial build, amdflang -O2) on the "
+                          f"250x250 km corner of the workload: {sub.n} particles, "
+                          f"{sub.nb} boundary particles, {k} predictor-corrector steps, "
+                          f"{r['wall']:.2f} s"}
+    o = capi.Oracle()
+    t0 = time.perf_counter()
+    o.run_pc(sub, 5)
+    per_step = (time.perf_counter() - t0) / 5
+    steps = int(max(5, min(400, target_s / max(per_step, 1e-6))))
+    t0 = time.perf_counter()
+    o.run_pc(sub, steps)
+    wall = time.perf_counter() - t0
+    return {"value": sub.n * steps / wall, "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"C restatement of the reference (serial) on the 250x250 km corner: "
+                      f"{sub.n} particles, {steps} steps, {wall:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--latency-steps", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the SPH-VP step of one simulation (or one "
+              "packed ensemble) runs on a single MI355X here; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from simp_amd import _lib
+    from simp_amd import scenarios as S
+    from simp_amd.api import Simp
+
+    sc = S.ice_edge()
+    n = sc.n
+    res = {}
+    prof = {}
+    for prec in (_lib.SIMP_FP64, _lib.SIMP_MIXED):
+        dev = Simp.for_scenario(sc, precision=prec)
+        dev.upload(sc)
+        dev.step(args.warmup, sync=True)
+        wall = _timed_steps(dev, args.steps)
+        lat = _latencies(dev, args.latency_steps)
+        res[prec] = {"rate": n * args.steps / wall, "ms": 1e3 * wall / args.steps,
+                     "p50": 1e3 * float(np.percentile(lat, 50)),
+                     "p99": 1e3 * float(np.percentile(lat, 99))}
+        prof[prec] = dev.profile_steps(20)
+        dev.close()
+
+    mixed = res[_lib.SIMP_MIXED]
+    fp64 = res[_lib.SIMP_FP64]
+    # dominant kernel of the mixed path
+    pm = prof[_lib.SIMP_MIXED]
+    name = max(("passA", "passB"), key=lambda k: pm[k][0])
+    kms = pm[name][0]
+    alg = ALG_BYTES[(name, 1)] * n
+    achieved = alg / (kms * 1e-3) / 1e9
+    line = {
+        "metric": "particle-steps/sec at 250k particles (FP32-mixed vs FP64) + p50/p99 step latency",
+        "value": mixed["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-pair-sums+f64",
+        "data": "synthetic",
+        "config": {"workload": "ice_edge: 250,000 particles, 500x500 km, coast wall, "
+                               "concentration ramp, variable hsml, GRF wind; mixed precision",
+                   "particles": n, "boundary_particles": sc.nb, "time_step": "VP (tse=2)"},
+        "fp64": {"value": fp64["rate"], "ms_per_step": fp64["ms"],
+                 "p50_step_ms": fp64["p50"], "p99_step_ms": fp64["p99"]},
+        "p50_step_ms": mixed["p50"],
+        "p99_step_ms": mixed["p99"],
+        "mixed_over_fp64": mixed["rate"] / fp64["rate"],
+        "step_hbm_roofline_frac": {
+            "mixed": STEP_BYTES[1] * mixed["rate"] / (HBM_PEAK_GBS * 1e9),
+            "fp64": STEP_BYTES[0] * fp64["rate"] / (HBM_PEAK_GBS * 1e9)},
+        "kernels_ms": {k: round(v[0], 5) for k, v in pm.items()},
+        "roofline": {"bound": "hbm", "kernel": f"k_passB<mixed>" if name == "passB"
+                     else "k_passA<mixed>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

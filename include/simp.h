@@ -1,0 +1,154 @@
+/*
+ * simp.h -- C ABI of the MI355X (gfx950) SPH viscous-plastic step.
+ *
+ * The boundary is SIMP's `single_step` (integration/single_step.f90:5,
+ * explicit interface modules/interfaces.f90:243-251): a subroutine without
+ * dummy arguments that reads and writes module state.  Here that state is
+ * passed explicitly as plain host pointers in the reference's Fortran layout
+ * (x(dim, n) interleaved, 1-based pair indices), so a Fortran caller binds
+ * each entry point with iso_c_binding and passes its module arrays as
+ * assumed-size actual arguments.
+ *
+ * Every function returns a status: SIMP_OK or one of SIMP_E_*.  The reference
+ * reports the same conditions with WRITE + STOP (neighbor/nnps.f90:415-423,
+ * 660-668); the Fortran shim turns a nonzero status back into STOP.
+ *
+ * All functions are re-entrant across handles.  One handle must not be used
+ * from two host threads at once.
+ */
+#ifndef SIMP_H
+#define SIMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMP_OK             0
+#define SIMP_E_ARG          1  /* invalid argument or null pointer             */
+#define SIMP_E_HIP          2  /* HIP runtime failure                           */
+#define SIMP_E_CAPACITY     3  /* a capacity was exceeded (cells, pairs, >200   */
+                               /* interactions per particle as nnps.f90:660)    */
+#define SIMP_E_NAN          4  /* non-finite position or smoothing length      */
+#define SIMP_E_UNSUPPORTED  5  /* a switch value this build does not implement */
+#define SIMP_E_NOMEM        6  /* device allocation failed                     */
+#define SIMP_E_STATE        7  /* call out of order (e.g. step before upload)  */
+
+#define SIMP_FP64   0          /* all arithmetic in FP64                        */
+#define SIMP_MIXED  1          /* pair sums in FP32 with compensated sums       */
+
+#define SIMP_WIND_UNIFORM 0
+#define SIMP_WIND_GRID    1
+
+#define SIMP_CONFIG_VERSION 1
+
+/* Runtime form of the compile-time switches of modules/globals.f90:58-187
+ * that reach the hot path.  Only the reference's active values are
+ * implemented; any other value is rejected with SIMP_E_UNSUPPORTED. */
+typedef struct simp_config {
+    int32_t version;        /* SIMP_CONFIG_VERSION                              */
+    int32_t precision;      /* SIMP_FP64 or SIMP_MIXED                          */
+    int32_t int_algorithm;  /* 2: predictor-corrector (globals.f90:64)          */
+    int32_t nnps_algorithm; /* 2: bucket search (globals.f90:73)                */
+    int32_t sle;            /* 2: hsml = 3 sqrt(m/rho) (globals.f90:81)         */
+    int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89)            */
+    int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96)       */
+    int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119)     */
+    int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127)         */
+    int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134)      */
+    int32_t skf;            /* 6: Wendland C6 (globals.f90:147)                 */
+    int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155)     */
+    int32_t wind_mode;      /* SIMP_WIND_UNIFORM or SIMP_WIND_GRID              */
+    int32_t max_particles;  /* capacity of particles (0: size at upload)        */
+    int32_t max_boundary;   /* capacity of boundary particles (0: at upload)    */
+    int32_t reserved;
+    double  wind[2];        /* uniform wind (m/s); reference (0, -5.3f),        */
+                            /* ext_force/wind_forcing.f90:36-37                 */
+    double  current[2];     /* uniform ocean current (m/s); reference (0, 0),   */
+                            /* ext_force/water_drag.f90:37-38                   */
+    double  max_hsml;       /* smoothing-length cap (in_out/initialize.f90:191) */
+} simp_config;
+
+typedef struct simp_handle simp_handle;
+
+/* Fill *cfg with the reference's active switch values (globals.f90). */
+void simp_config_default(simp_config *cfg);
+
+/* Create one simulation context on the current HIP device. */
+int simp_create(const simp_config *cfg, simp_handle **out);
+int simp_destroy(simp_handle *h);
+
+/* Static wind field on a regular grid (row-major, ny rows of nx values),
+ * sampled bilinearly at particle positions when wind_mode is
+ * SIMP_WIND_GRID.  The reference has a uniform field only
+ * (wind_forcing.f90:36-37); the grid carries the synthetic forcing of the
+ * larger workloads. */
+int simp_set_wind_grid(simp_handle *h, int nx, int ny, double x0, double y0,
+                       double dx, double dy, const double *u, const double *v);
+
+/* ---- the drop-in for single_step (integration/single_step.f90:5-108) ----
+ * One right-hand-side evaluation on host arrays: copies the state in, runs
+ * NNPS, the pair sums, the stress, boundary and forcing terms on the GPU and
+ * copies the results out.  x, vx, dxdt, dvxdt are (2, n) interleaved; eps3 =
+ * (epsilon_11, epsilon_12, epsilon_22) and sig3 = (sigma_11, sigma_12,
+ * sigma_22) are (3, n) interleaved.  current_time gates wind and water drag
+ * (wind_forcing.f90:35, water_drag.f90:36); itimestep selects the
+ * interaction-count check of nnps.f90:512-515.  Any output pointer may be
+ * NULL.  nip and bnip receive the pair counts NNPS would hold. */
+int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
+                     const double *mass, const double *rho, const double *hsml,
+                     const double *hth, const double *A, int nb,
+                     const double *xb, const double *massb,
+                     double current_time, int itimestep,
+                     double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                     double *eps3, double *zeta, double *eta, double *p,
+                     double *sig3, int *nip, int *bnip);
+
+/* ---- NNPS (neighbor/nnps.f90:21-30 -> bucket_search :155-517) ----
+ * The pair lists in the reference's exact order (single-thread order):
+ * p_i, p_j 1-based, r_ij, hsml_ij; boundary pairs bp_i (boundary particle),
+ * bp_j (particle), br_ij.  cap / bcap are the array capacities; the counts
+ * are always written, and SIMP_E_CAPACITY is returned if they exceed them. */
+int simp_nnps(simp_handle *h, int n, const double *x, const double *hsml,
+              int nb, const double *xb, int cap, int *nip, int *p_i, int *p_j,
+              double *r_ij, double *hsml_ij, int bcap, int *bnip, int *bp_i,
+              int *bp_j, double *br_ij);
+
+/* ---- device-resident time integration ----
+ * Upload a state once, then advance it by whole predictor-corrector steps
+ * (time_integration.f90:38-73 with predictor_corrector :155-282) without
+ * host round trips, and download it when needed. */
+int simp_upload(simp_handle *h, int n, const double *x, const double *vx,
+                const double *mass, const double *rho, const double *hsml,
+                const double *hth, const double *A, int nb, const double *xb,
+                const double *massb, double dt, double current_time,
+                int itimestep);
+/* Advance nsteps steps (asynchronous on the handle's stream unless sync). */
+int simp_step_pc(simp_handle *h, int nsteps, int sync);
+int simp_download(simp_handle *h, double *x, double *vx, double *rho,
+                  double *hsml, double *hth, double *A, double *dt,
+                  double *current_time, int *itimestep);
+/* Diagnostic fields of the last corrector evaluation (written by
+ * simp_step_pc only when enabled with simp_set_diagnostics). */
+int simp_set_diagnostics(simp_handle *h, int enable);
+int simp_download_diagnostics(simp_handle *h, double *eps3, double *zeta,
+                              double *eta, double *p, double *sig3);
+/* Wait for the handle's stream; returns the first device-side error. */
+int simp_sync(simp_handle *h);
+/* The HIP stream the handle launches on (hipStream_t). */
+void *simp_stream(simp_handle *h);
+
+/* Per-kernel timing of nsteps steps: HIP events bracket every launch on the
+ * handle's stream.  names: '\n'-separated kernel names written to buf;
+ * ms: average duration per launch of each name; counts: launches per name.
+ * Returns the number of names written (at most maxk). */
+int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf,
+                       int buflen, double *ms, int *counts);
+
+const char *simp_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -9,7 +9,9 @@
 ! reference's external NNPS, single_step or time_integration unchanged.
 !
 ! Input (stream, little endian):
-!   int32  mode            1 = NNPS, 2 = single_step, 3 = time_integration
+!   int32  mode            1 = NNPS, 2 = single_step, 3 = time_integration,
+!                          4 = single_step on a first state, then on a second
+!                          state that follows the first in the file
 !   int32  n, nb, reps     reps: number of consecutive single_step calls
 !   real8  dt, current_time, integration_time, max_hsml
 !   real8  x(2,n), vx(2,n), mass(n), rho(n), hsml(n), A(n), h(n)
@@ -74,6 +76,20 @@ program simp_ref_harness
         enddo
     else if (mode == 3) then
         call time_integration
+    else if (mode == 4) then
+        call single_step
+        open(newunit=u, file=trim(fin), access='stream', form='unformatted', &
+             status='old', action='read')
+        read(u) k, k, k, k
+        read(u) dt, current_time, integration_time, max_hsml
+        read(u) x(:, 1:n), vx(:, 1:n), mass(1:n), rho(1:n), hsml(1:n), &
+                A(1:n), h(1:n)
+        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb)
+        read(u) x(:, 1:n), vx(:, 1:n), mass(1:n), rho(1:n), hsml(1:n), &
+                A(1:n), h(1:n)
+        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb)
+        close(u)
+        call single_step
     else
         write(*,*) 'harness: unknown mode'
         stop 2

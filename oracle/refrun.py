@@ -8,6 +8,7 @@ the golden vectors under ``tests/golden`` and as the CPU baseline.
 from __future__ import annotations
 
 import os
+import resource
 import struct
 import subprocess
 import tempfile
@@ -19,22 +20,29 @@ HARNESS = os.path.join(HERE, "_ref", "simp_ref_harness")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
 
-MODE_NNPS, MODE_SINGLE_STEP, MODE_TIME_INTEGRATION = 1, 2, 3
+MODE_NNPS, MODE_SINGLE_STEP, MODE_TIME_INTEGRATION, MODE_TWO_STATES = 1, 2, 3, 4
 
 
 def available() -> bool:
     return os.path.exists(HARNESS)
 
 
-def _write_input(path, mode, sc, reps, integration_time):
+def _write_state(f, sc):
+    for a in (sc.x, sc.vx, sc.mass, sc.rho, sc.hsml, sc.A, sc.h):
+        f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+    if sc.nb:
+        f.write(np.ascontiguousarray(sc.xb, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(sc.massb, dtype="<f8").tobytes())
+
+
+def _write_input(path, mode, sc, reps, integration_time, second=None):
     with open(path, "wb") as f:
         f.write(struct.pack("<4i", mode, sc.n, sc.nb, reps))
         f.write(struct.pack("<4d", sc.dt, sc.time, integration_time, sc.max_hsml))
-        for a in (sc.x, sc.vx, sc.mass, sc.rho, sc.hsml, sc.A, sc.h):
-            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
-        if sc.nb:
-            f.write(np.ascontiguousarray(sc.xb, dtype="<f8").tobytes())
-            f.write(np.ascontiguousarray(sc.massb, dtype="<f8").tobytes())
+        _write_state(f, sc)
+        if second is not None:
+            assert second.n == sc.n and second.nb == sc.nb
+            _write_state(f, second)
 
 
 def _read_output(path, n):
@@ -70,17 +78,30 @@ def _read_output(path, n):
     return out
 
 
-def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600):
-    """Run the reference harness on scenario ``sc``; returns a dict of its
-    outputs.  The reference's wind is its own hard-coded field."""
+def _big_stack():
+    # the reference keeps its per-call work arrays (single_step.f90:31,
+    # time_integration.f90:190-194, nnps.f90:196) on the stack
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK,
+                           (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    except (ValueError, OSError):
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600, second=None):
+    """Run the reference harness on scenario ``sc`` (and ``second`` for
+    MODE_TWO_STATES); returns a dict of its outputs.  The reference's wind is
+    its own hard-coded field."""
     if not available():
         raise FileNotFoundError(HARNESS)
     if sc.n > MAXN or sc.nb > MAXNB:
         raise ValueError("scenario exceeds the reference's static capacity")
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
-        _write_input(fin, mode, sc, reps, integration_time)
+        _write_input(fin, mode, sc, reps, integration_time, second)
         env = dict(os.environ, PWD=d, OMP_NUM_THREADS=str(threads))
         subprocess.run([HARNESS, fin, fout], cwd=d, env=env, check=True,
-                       stdout=subprocess.DEVNULL, timeout=timeout)
+                       stdout=subprocess.DEVNULL, timeout=timeout,
+                       preexec_fn=_big_stack)
         return _read_output(fout, sc.n)

@@ -1,0 +1,106 @@
+"""ctypes binding of ``simp_amd/lib/libsimp_amd.so`` (the C ABI of
+``include/simp.h``).  Loading fails loudly when the library is missing:
+there is no CPU fallback."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsimp_amd.so")
+
+SIMP_OK = 0
+SIMP_E_ARG = 1
+SIMP_E_HIP = 2
+SIMP_E_CAPACITY = 3
+SIMP_E_NAN = 4
+SIMP_E_UNSUPPORTED = 5
+SIMP_E_NOMEM = 6
+SIMP_E_STATE = 7
+SIMP_FP64 = 0
+SIMP_MIXED = 1
+SIMP_WIND_UNIFORM = 0
+SIMP_WIND_GRID = 1
+SIMP_CONFIG_VERSION = 1
+
+#: every entry point declared in include/simp.h
+EXPORTS = (
+    "simp_config_default", "simp_create", "simp_destroy", "simp_set_wind_grid",
+    "simp_single_step", "simp_nnps", "simp_upload", "simp_step_pc",
+    "simp_download", "simp_set_diagnostics", "simp_download_diagnostics",
+    "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
+)
+
+
+class SimpConfig(C.Structure):
+    _fields_ = [("version", C.c_int32), ("precision", C.c_int32),
+                ("int_algorithm", C.c_int32), ("nnps_algorithm", C.c_int32),
+                ("sle", C.c_int32), ("tse", C.c_int32), ("rheology", C.c_int32),
+                ("bfa", C.c_int32), ("eos_alg", C.c_int32), ("pa_sph", C.c_int32),
+                ("skf", C.c_int32), ("den_alg", C.c_int32), ("wind_mode", C.c_int32),
+                ("max_particles", C.c_int32), ("max_boundary", C.c_int32),
+                ("reserved", C.c_int32), ("wind", C.c_double * 2),
+                ("current", C.c_double * 2), ("max_hsml", C.c_double)]
+
+
+class SimpError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        msg = lib().simp_strerror(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{where}: {msg} (status {status})")
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with "
+                          "`make -C simp_amd/csrc` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    L.simp_config_default.argtypes = [C.POINTER(SimpConfig)]
+    L.simp_config_default.restype = None
+    L.simp_strerror.argtypes = [C.c_int]
+    L.simp_strerror.restype = C.c_char_p
+    L.simp_create.argtypes = [C.POINTER(SimpConfig), C.POINTER(_vp)]
+    L.simp_destroy.argtypes = [_vp]
+    L.simp_set_wind_grid.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, _vp, _vp]
+    L.simp_single_step.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp,
+                                   C.c_double, C.c_int] + [_vp] * 9 + [_vp, _vp])
+    L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
+                            _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]
+    L.simp_upload.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp, C.c_double,
+                              C.c_double, C.c_int])
+    L.simp_step_pc.argtypes = [_vp, C.c_int, C.c_int]
+    L.simp_download.argtypes = [_vp] + [_vp] * 9
+    L.simp_set_diagnostics.argtypes = [_vp, C.c_int]
+    L.simp_download_diagnostics.argtypes = [_vp] + [_vp] * 5
+    L.simp_sync.argtypes = [_vp]
+    L.simp_stream.argtypes = [_vp]
+    L.simp_stream.restype = _vp
+    L.simp_profile_steps.argtypes = [_vp, C.c_int, C.c_int, C.c_char_p, C.c_int, _vp, _vp]
+    _lib = L
+    return L
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the C ABI must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def check(status: int, where: str) -> None:
+    if status != SIMP_OK:
+        raise SimpError(status, where)

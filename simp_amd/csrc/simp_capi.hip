@@ -1,0 +1,300 @@
+// simp_capi.hip -- the extern "C" boundary declared in include/simp.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/simp.h"
+#include "simp_engine.h"
+
+using simp::Engine;
+using simp::InstanceInput;
+
+struct simp_handle {
+  simp_config cfg;
+  Engine *eng = nullptr;
+  bool uploaded = false;
+  int n = 0, nb = 0;
+  int g_nx = 0, g_ny = 0;
+  double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
+  std::vector<double> g_u, g_v;
+};
+
+static int map_rc(int rc) {
+  switch (rc) {
+    case 0: return SIMP_OK;
+    case -1: return SIMP_E_ARG;
+    case -3: return SIMP_E_CAPACITY;
+    case -4: return SIMP_E_NAN;
+    case -6: return SIMP_E_NOMEM;
+    default: return SIMP_E_HIP;
+  }
+}
+
+extern "C" {
+
+void simp_config_default(simp_config *c) {
+  if (!c) return;
+  *c = simp_config{};
+  c->version = SIMP_CONFIG_VERSION;
+  c->precision = SIMP_FP64;
+  c->int_algorithm = 2;
+  c->nnps_algorithm = 2;
+  c->sle = 2;
+  c->tse = 2;
+  c->rheology = 1;
+  c->bfa = 1;
+  c->eos_alg = 3;
+  c->pa_sph = 1;
+  c->skf = 6;
+  c->den_alg = 3;
+  c->wind_mode = SIMP_WIND_UNIFORM;
+  c->wind[0] = 0.0;
+  c->wind[1] = (double)(-5.3f);
+  c->current[0] = c->current[1] = 0.0;
+  c->max_hsml = 0.0;
+}
+
+const char *simp_strerror(int s) {
+  switch (s) {
+    case SIMP_OK: return "ok";
+    case SIMP_E_ARG: return "invalid argument";
+    case SIMP_E_HIP: return "HIP runtime failure";
+    case SIMP_E_CAPACITY: return "capacity exceeded";
+    case SIMP_E_NAN: return "non-finite position or smoothing length";
+    case SIMP_E_UNSUPPORTED: return "switch value not implemented";
+    case SIMP_E_NOMEM: return "device allocation failed";
+    case SIMP_E_STATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+static int validate(const simp_config *c) {
+  if (!c || c->version != SIMP_CONFIG_VERSION) return SIMP_E_ARG;
+  if (c->precision != SIMP_FP64 && c->precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
+  if (c->int_algorithm != 2 || c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
+  if (c->tse != 0 && c->tse != 2) return SIMP_E_UNSUPPORTED;
+  if (c->rheology != 1 || c->bfa != 1 || c->eos_alg != 3 || c->pa_sph != 1) return SIMP_E_UNSUPPORTED;
+  if (c->skf != 6 || c->den_alg != 3) return SIMP_E_UNSUPPORTED;
+  if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
+  return SIMP_OK;
+}
+
+int simp_create(const simp_config *cfg, simp_handle **out) {
+  if (!out) return SIMP_E_ARG;
+  *out = nullptr;
+  int v = validate(cfg);
+  if (v) return v;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SIMP_E_HIP;
+  simp_handle *h = new (std::nothrow) simp_handle;
+  if (!h) return SIMP_E_NOMEM;
+  h->cfg = *cfg;
+  h->eng = new (std::nothrow) Engine(cfg->precision);
+  if (!h->eng || !h->eng->stream()) {
+    delete h->eng;
+    delete h;
+    return SIMP_E_HIP;
+  }
+  *out = h;
+  return SIMP_OK;
+}
+
+int simp_destroy(simp_handle *h) {
+  if (!h) return SIMP_E_ARG;
+  delete h->eng;
+  delete h;
+  return SIMP_OK;
+}
+
+int simp_set_wind_grid(simp_handle *h, int nx, int ny, double x0, double y0, double dx,
+                       double dy, const double *u, const double *v) {
+  if (!h || nx < 2 || ny < 2 || !u || !v || !(dx > 0) || !(dy > 0)) return SIMP_E_ARG;
+  h->g_nx = nx; h->g_ny = ny;
+  h->g_x0 = x0; h->g_y0 = y0; h->g_dx = dx; h->g_dy = dy;
+  h->g_u.assign(u, u + (size_t)nx * ny);
+  h->g_v.assign(v, v + (size_t)nx * ny);
+  return SIMP_OK;
+}
+
+static int fill_input(simp_handle *h, InstanceInput &in) {
+  const simp_config &c = h->cfg;
+  in.tse = c.tse;
+  in.wind_mode = c.wind_mode;
+  in.wind[0] = c.wind[0]; in.wind[1] = c.wind[1];
+  in.current[0] = c.current[0]; in.current[1] = c.current[1];
+  in.max_hsml = c.max_hsml;
+  if (c.wind_mode == SIMP_WIND_GRID) {
+    if (h->g_nx < 2) return SIMP_E_STATE;
+    in.g_nx = h->g_nx; in.g_ny = h->g_ny;
+    in.g_x0 = h->g_x0; in.g_y0 = h->g_y0; in.g_dx = h->g_dx; in.g_dy = h->g_dy;
+    in.g_u = h->g_u.data(); in.g_v = h->g_v.data();
+  }
+  if (!(in.max_hsml > 0)) {
+    // initialize.f90:191 convention when the caller gives no cap
+    double mx = 0;
+    for (int i = 0; i < in.n; ++i) mx = std::max(mx, in.hsml[i]);
+    in.max_hsml = mx * 10.;
+  }
+  return SIMP_OK;
+}
+
+int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
+                     const double *mass, const double *rho, const double *hsml,
+                     const double *hth, const double *A, int nb, const double *xb,
+                     const double *massb, double current_time, int itimestep, double *dxdt,
+                     double *dvxdt, double *dhdt, double *dAdt, double *eps3, double *zeta,
+                     double *eta, double *p, double *sig3, int *nip, int *bnip) {
+  if (!h || n < 1 || nb < 0) return SIMP_E_ARG;
+  InstanceInput in;
+  in.n = n; in.nb = nb;
+  in.x = x; in.vx = vx; in.mass = mass; in.rho = rho; in.hsml = hsml; in.hth = hth; in.A = A;
+  in.xb = xb; in.massb = massb;
+  in.time = current_time;
+  in.itimestep = itimestep;
+  int rc = fill_input(h, in);
+  if (rc) return rc;
+  rc = h->eng->upload({in}, true);
+  if (rc) return map_rc(rc);
+  h->uploaded = false;  // the device state now holds this call's inputs
+  rc = h->eng->eval_single(eps3 || zeta || eta || p);
+  if (rc) return map_rc(rc);
+  rc = h->eng->download_tend(0, dxdt, dvxdt, dhdt, dAdt);
+  if (rc) return map_rc(rc);
+  rc = h->eng->download_diag(0, eps3, zeta, eta, p, sig3);
+  if (rc) return map_rc(rc);
+  if (nip || bnip) {
+    int a = 0, b = 0;
+    rc = h->eng->download_scalars(0, nullptr, nullptr, nullptr, &a, &b, nullptr);
+    if (rc) return map_rc(rc);
+    if (nip) *nip = a;
+    if (bnip) *bnip = nb > 0 ? b : 0;
+  }
+  return SIMP_OK;
+}
+
+int simp_nnps(simp_handle *h, int n, const double *x, const double *hsml, int nb,
+              const double *xb, int cap, int *nip, int *p_i, int *p_j, double *r_ij,
+              double *hsml_ij, int bcap, int *bnip, int *bp_i, int *bp_j, double *br_ij) {
+  if (!h || n < 1 || nb < 0 || !x || !hsml || !nip || !bnip || cap < 0 || bcap < 0)
+    return SIMP_E_ARG;
+  if (nb > 0 && !xb) return SIMP_E_ARG;
+  std::vector<double> zero2(2 * (size_t)n, 0.0), one(n, 1.0), mb(std::max(nb, 1), 1.0);
+  InstanceInput in;
+  in.n = n; in.nb = nb;
+  in.x = x; in.vx = zero2.data(); in.mass = one.data(); in.rho = one.data();
+  in.hsml = hsml; in.hth = one.data(); in.A = one.data();
+  in.xb = xb; in.massb = mb.data();
+  int rc = fill_input(h, in);
+  if (rc) return rc;
+  rc = h->eng->upload({in}, true);
+  if (rc) return map_rc(rc);
+  h->uploaded = false;
+  rc = h->eng->export_pairs(cap, nip, p_i, p_j, r_ij, hsml_ij, bcap, bnip, bp_i, bp_j, br_ij);
+  return map_rc(rc);
+}
+
+int simp_upload(simp_handle *h, int n, const double *x, const double *vx, const double *mass,
+                const double *rho, const double *hsml, const double *hth, const double *A,
+                int nb, const double *xb, const double *massb, double dt,
+                double current_time, int itimestep) {
+  if (!h || n < 1 || nb < 0) return SIMP_E_ARG;
+  InstanceInput in;
+  in.n = n; in.nb = nb;
+  in.x = x; in.vx = vx; in.mass = mass; in.rho = rho; in.hsml = hsml; in.hth = hth; in.A = A;
+  in.xb = xb; in.massb = massb;
+  in.dt = dt; in.time = current_time; in.itimestep = itimestep;
+  int rc = fill_input(h, in);
+  if (rc) return rc;
+  rc = h->eng->upload({in}, false);
+  if (rc) return map_rc(rc);
+  h->uploaded = true;
+  h->n = n;
+  h->nb = nb;
+  return SIMP_OK;
+}
+
+int simp_step_pc(simp_handle *h, int nsteps, int sync) {
+  if (!h || nsteps < 0) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  int rc = h->eng->steps(nsteps, true);
+  if (rc) return map_rc(rc);
+  if (sync) return map_rc(h->eng->sync_and_check());
+  return SIMP_OK;
+}
+
+int simp_sync(simp_handle *h) {
+  if (!h) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_OK;
+  return map_rc(h->eng->sync_and_check());
+}
+
+void *simp_stream(simp_handle *h) { return h ? (void *)h->eng->stream() : nullptr; }
+
+int simp_download(simp_handle *h, double *x, double *vx, double *rho, double *hsml, double *hth,
+                  double *A, double *dt, double *current_time, int *itimestep) {
+  if (!h) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  int rc = h->eng->sync_and_check();
+  if (rc) return map_rc(rc);
+  rc = h->eng->download_state(0, x, vx, rho, hsml, hth, A);
+  if (rc) return map_rc(rc);
+  rc = h->eng->download_scalars(0, dt, current_time, itimestep, nullptr, nullptr, nullptr);
+  return map_rc(rc);
+}
+
+int simp_set_diagnostics(simp_handle *h, int enable) {
+  if (!h) return SIMP_E_ARG;
+  h->eng->set_diag(enable != 0);
+  return SIMP_OK;
+}
+
+int simp_download_diagnostics(simp_handle *h, double *eps3, double *zeta, double *eta,
+                              double *p, double *sig3) {
+  if (!h) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  if (!h->eng->diag() && (eps3 || zeta || eta || p)) return SIMP_E_STATE;
+  return map_rc(h->eng->download_diag(0, eps3, zeta, eta, p, sig3));
+}
+
+int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf, int buflen, double *ms,
+                       int *counts) {
+  if (!h || nsteps < 1 || maxk < 1 || !buf || buflen < 1 || !ms || !counts) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  simp::Profiler prof;
+  h->eng->prof = &prof;
+  int rc = h->eng->steps(nsteps, false);
+  h->eng->prof = nullptr;
+  if (rc) return -map_rc(rc);
+  if (hipStreamSynchronize(h->eng->stream()) != hipSuccess) return -SIMP_E_HIP;
+  std::vector<std::string> names;
+  std::map<std::string, std::pair<double, int>> agg;
+  for (auto &r : prof.recs) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, r.a, r.b);
+    if (!agg.count(r.name)) names.push_back(r.name);
+    agg[r.name].first += t;
+    agg[r.name].second += 1;
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  std::string out;
+  int k = 0;
+  for (auto &nm : names) {
+    if (k >= maxk) break;
+    ms[k] = agg[nm].first / agg[nm].second;
+    counts[k] = agg[nm].second;
+    out += nm;
+    out += "\n";
+    ++k;
+  }
+  int len = std::min((int)out.size(), buflen - 1);
+  std::copy(out.begin(), out.begin() + len, buf);
+  buf[len] = 0;
+  return k;
+}
+
+}  // extern "C"

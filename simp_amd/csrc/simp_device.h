@@ -1,0 +1,97 @@
+// simp_device.h -- device data layout shared by the kernels and the host
+// engine.  Particles of many independent simulations ("instances") are
+// packed into one set of structure-of-arrays buffers; every instance owns a
+// contiguous particle range, boundary range, cell range and boundary-pair
+// range, described by InstMeta (fixed between uploads) and InstDyn (changes
+// every evaluation).
+#pragma once
+#include <stdint.h>
+
+namespace simp {
+
+// Physical constants of data_variables.f90:153-180 and globals.f90:199.
+// rho_air is a default-REAL literal widened to REAL(8) in the reference.
+constexpr double kPi = 3.1415926535897932;
+constexpr double kPStar = 2.75e4;
+constexpr double kCEos = 20.0;
+constexpr double kRhoIce = 900.0;
+constexpr double kRhoWater = 1026.0;
+constexpr double kRhoAir = (double)1.3f;
+constexpr double kCWind = 1.2e-3;
+constexpr double kCWater = 5.5e-3;
+constexpr double kDeltaMin = 2.0e-9;
+constexpr double kBLength = 8000.0;
+constexpr double kKappaN = 5.4e12;
+constexpr double kKT = 0.0;
+constexpr double kE = 2.0;
+constexpr double kHsmlMult = 3.0;
+constexpr int kMaxIndInt = 200;     // globals.f90:24
+constexpr int kPrintStep = 2000;    // globals.f90:55
+
+constexpr int kTPB = 256;           // threads per block of particle kernels
+constexpr int kScanItems = 2048;    // elements per block of the scan
+constexpr int kMaxBoundaryPerInst = 8192;
+
+// Error bits (OR-ed into InstDyn::err); translated to SIMP_E_* on the host.
+constexpr int kErrCapacity = 1;
+constexpr int kErrNan = 2;
+constexpr int kErrBndNbr = 4;
+
+struct InstMeta {
+  int p_off, n;          // particle range
+  int b_off, nb;         // boundary-particle range
+  int c_off, c_cap;      // cell range (capacity)
+  int q_off, q_cap;      // boundary-pair / hsml_ij-store range
+  int tse, wind_mode;
+  int g_off, g_nx, g_ny, pad0;  // wind grid range
+  double g_x0, g_y0, g_dx, g_dy;
+  double wind_u, wind_v, cur_u, cur_v;
+  double max_hsml;
+  double bxmin, bxmax, bymin, bymax;   // boundary bounding box (static)
+};
+
+struct InstDyn {
+  double dt, time;
+  int itimestep, err;
+  // grid of the current evaluation (bucket_search :211-224)
+  double bin;
+  int xlo, ylo, W, H;
+  int nip, bnip;
+  // reductions of the current evaluation, order-preserving u64 encodings
+  unsigned long long red_xmin, red_xmax, red_ymin, red_ymax, red_hmin;
+  unsigned long long dtmin;   // next VP dt
+};
+
+// Per-launch block table entry: one block covers kTPB particles of exactly
+// one instance.
+struct BlockEntry {
+  int inst, first;   // instance, first local particle index
+};
+
+// Device pointers of one engine.
+struct DevState {
+  // state buffers: [0] = S (step start / current), [1] = T (half step)
+  double *x[2], *y[2], *vx[2], *vy[2], *h[2], *A[2], *rho[2], *hsml[2];
+  double *mass;
+  int *orig;            // local (0-based) particle id inside its instance
+  int *inst;            // instance of each packed particle
+  // per evaluation
+  int *cellid, *slot, *cl, *clpos, *count, *cell_start, *scan_tmp;
+  int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
+  void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
+  double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N), optional
+  double *tend;          // dxdt, dydt, dvxdt, dvydt, dhdt, dAdt (6 x N), optional
+  // boundary
+  double *xb, *yb, *massb;
+  int *bpre, *bbase, *bcount;
+  unsigned long long *bsorted;   // (cell << 32 | b) sorted, per instance
+  double *hq;            // persistent hsml_ij store (stale semantics)
+  double *grid;          // wind grids, u then v per instance
+  InstMeta *meta;
+  InstDyn *dyn;
+  BlockEntry *blk;       // particle blocks
+  int nblk;
+  int ntot, nbtot, ctot, qtot, ninst;
+};
+
+}  // namespace simp

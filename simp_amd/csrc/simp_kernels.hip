@@ -1,0 +1,1006 @@
+// simp_kernels.hip -- gfx950 kernels of the SPH viscous-plastic step.
+//
+// One right-hand-side evaluation (the work of SIMP's single_step,
+// integration/single_step.f90:5-108) is a short pipeline of kernels over the
+// packed particles of every instance:
+//
+//   eval_begin   reset the per-instance reductions
+//   reduce       bounding box and MINVAL(hsml)            (nnps.f90:211-216)
+//   count        cell key NINT(x/bin) + cell histogram    (nnps.f90:246-254)
+//   scan x2      exclusive scan of the cell histogram
+//   scatter      cell lists
+//   cellsort     cell members in ascending particle id (the reference's bin
+//                order), inverse positions
+//   bprep        boundary-pair counts, ranks and the boundary cell index
+//                (nnps.f90:451-493)
+//   passA        neighbour gather: Wendland C6 gradient, velocity divergence
+//                and gradient, strain rate and the VP rheology epilogue
+//                (kernel.f90:471-520, vector_divergence.f90:27-28,
+//                vector_gradient.f90:27-40, single_step.f90:80-84,
+//                internal_stress.f90:21-53 and its callees)
+//   pairtot,scan pair ranks in the reference's pair order
+//   materialize  hsml_ij of the first bnip pairs (read by b_Wendland_C6,
+//                boundary_kernel.f90:491)
+//   passB        neighbour gather: stress divergence (internal_stress.f90:
+//                103-136), boundary force (boundary_kernel.f90:477-502,
+//                boundary_force.f90:78-82), wind and water drag, tendencies
+//                (single_step.f90:104-106), and the predictor or corrector
+//                update (time_integration.f90:207-280)
+//
+// Every particle gathers its neighbours itself (no atomics on sums), visiting
+// the 3x3 cell block in the order in which the reference's single-thread
+// pair list reaches that particle: cells (-1,-1) (0,-1) (+1,-1) (-1,0), its
+// own cell, (+1,0) (0,+1) (+1,+1) (-1,+1), members in ascending id.  With the
+// per-pair arithmetic written in the reference's evaluation order and FP
+// contraction off, each FP64 sum is formed in the same order as the
+// reference's, so the FP64 path reproduces the reference's results.
+#include <hip/hip_runtime.h>
+
+#include "simp_device.h"
+#include "simp_kernels.h"
+
+namespace simp {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int nint(double v) {
+  // Fortran NINT: round half away from zero
+  return (int)__builtin_round(v);
+}
+
+__device__ __forceinline__ unsigned long long ord_u64(double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double ord_f64(unsigned long long u) {
+  return __longlong_as_double(
+      (long long)((u >> 63) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+// x**k as the reference build expands it: x*x*...*x, left to right.
+template <int K, class T>
+__device__ __forceinline__ T ipow(T x) {
+  T r = x;
+#pragma unroll
+  for (int i = 1; i < K; ++i) r = r * x;
+  return r;
+}
+
+constexpr double kC6 = 78.0 / (7.0 * kPi);   // 78. / (7. * pi)
+constexpr float kC6f = (float)(78.0 / (7.0 * kPi));
+
+// particle handled by this thread
+__device__ __forceinline__ bool particle_of(const DevState &d, int &p, int &k) {
+  BlockEntry e = d.blk[blockIdx.x];
+  k = e.inst;
+  int local = e.first + (int)threadIdx.x;
+  if (local >= d.meta[k].n) return false;
+  p = d.meta[k].p_off + local;
+  return true;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Exclusive scan across the block (kTPB threads); returns the exclusive
+// prefix of v and the block total in *total.
+__device__ int block_exclusive_scan(int v, int *total) {
+  __shared__ int wsum[kTPB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kTPB / 64; ++w) {
+    int s = wsum[w];
+    if (w < wid) pre += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + incl - v;
+}
+
+__device__ int block_sum(int v) {
+  __shared__ int wsum[kTPB / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int tot = 0;
+#pragma unroll
+  for (int w = 0; w < kTPB / 64; ++w) tot += wsum[w];
+  __syncthreads();
+  return tot;
+}
+
+// -------------------------------------------------------- eval_begin/reduce
+
+__global__ void k_eval_begin(DevState d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.ninst) return;
+  InstDyn &y = d.dyn[k];
+  y.red_xmin = ~0ull;
+  y.red_ymin = ~0ull;
+  y.red_hmin = ~0ull;
+  y.red_xmax = 0ull;
+  y.red_ymax = 0ull;
+}
+
+__global__ void __launch_bounds__(kTPB) k_reduce(DevState d, int buf) {
+  int p = 0, k = 0;
+  bool ok = particle_of(d, p, k);
+  double xv = ok ? d.x[buf][p] : 0.0, yv = ok ? d.y[buf][p] : 0.0;
+  double hv = ok ? d.hsml[buf][p] : 0.0;
+  bool bad = ok && (!(xv == xv) || !(yv == yv) || !(hv > 0.0) ||
+                    __builtin_isinf(xv) || __builtin_isinf(yv) ||
+                    __builtin_isinf(hv));
+  if (__any(bad)) {
+    if ((threadIdx.x & 63) == 0) atomicOr(&d.dyn[k].err, kErrNan);
+  }
+  const double big = __builtin_huge_val();
+  double xmn = wave_min(ok ? xv : big), xmx = wave_max(ok ? xv : -big);
+  double ymn = wave_min(ok ? yv : big), ymx = wave_max(ok ? yv : -big);
+  double hmn = wave_min(ok ? hv : big);
+  if ((threadIdx.x & 63) == 0 && __any(ok)) {
+    InstDyn &y = d.dyn[k];
+    atomicMin(&y.red_xmin, ord_u64(xmn));
+    atomicMax(&y.red_xmax, ord_u64(xmx));
+    atomicMin(&y.red_ymin, ord_u64(ymn));
+    atomicMax(&y.red_ymax, ord_u64(ymx));
+    atomicMin(&y.red_hmin, ord_u64(hmn));
+  }
+}
+
+// ------------------------------------------------------------------ count
+
+__global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
+  int p = 0, k = 0;
+  bool ok = particle_of(d, p, k);
+  InstDyn &y = d.dyn[k];
+  if (y.err) return;
+  const InstMeta &m = d.meta[k];
+  // bucket_search grid (nnps.f90:211-216): bin = 2 MINVAL(hsml), the cell
+  // range spans particles and boundary particles
+  double bin = 2 * ord_f64(y.red_hmin);
+  double xmn = ord_f64(y.red_xmin), xmx = ord_f64(y.red_xmax);
+  double ymn = ord_f64(y.red_ymin), ymx = ord_f64(y.red_ymax);
+  if (m.nb > 0) {
+    xmn = fmin(xmn, m.bxmin); xmx = fmax(xmx, m.bxmax);
+    ymn = fmin(ymn, m.bymin); ymx = fmax(ymx, m.bymax);
+  }
+  int xlo = nint(xmn / bin), xhi = nint(xmx / bin);
+  int ylo = nint(ymn / bin), yhi = nint(ymx / bin);
+  long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
+  if (W * H > (long long)m.c_cap || W * H <= 0) {
+    if (threadIdx.x == 0) atomicOr(&y.err, kErrCapacity);
+    return;
+  }
+  if (threadIdx.x == 0 && d.blk[blockIdx.x].first == 0) {
+    y.bin = bin;
+    y.xlo = xlo;
+    y.ylo = ylo;
+    y.W = (int)W;
+    y.H = (int)H;
+  }
+  if (!ok) return;
+  int cx = nint(d.x[buf][p] / bin) - xlo + 1;
+  int cy = nint(d.y[buf][p] / bin) - ylo + 1;
+  int c = cy * (int)W + cx;
+  d.cellid[p] = c;
+  d.slot[p] = atomicAdd(&d.count[m.c_off + c], 1);
+}
+
+// ------------------------------------------------------------------- scan
+
+__global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum) {
+  constexpr int IPT = kScanItems / kTPB;
+  int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    int idx = base + i;
+    if (idx < L) s += in[idx];
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+// out[i] = sum(in[0..i-1]); out[L] = total.  Optionally clears in[].
+__global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum, int *out,
+                                                int zero_in) {
+  constexpr int IPT = kScanItems / kTPB;
+  int acc = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += kTPB) acc += bsum[j];
+  int pre = block_sum(acc);
+  int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
+  int v[IPT];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    int idx = base + i;
+    v[i] = idx < L ? in[idx] : 0;
+    s += v[i];
+  }
+  int tot = 0;
+  int ex = block_exclusive_scan(s, &tot) + pre;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    int idx = base + i;
+    if (idx < L) {
+      out[idx] = ex;
+      if (zero_in) in[idx] = 0;
+    }
+    ex += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kTPB - 1) out[L] = ex;
+}
+
+// ------------------------------------------------------- scatter/cellsort
+
+__global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k)) return;
+  if (d.dyn[k].err) return;
+  d.cl[d.cell_start[d.meta[k].c_off + d.cellid[p]] + d.slot[p]] = p;
+}
+
+// members of each cell in ascending particle id: the order in which the
+// reference fills its bins (nnps.f90:246-254)
+__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
+  int c = blockIdx.x * kTPB + threadIdx.x;
+  if (c >= d.ctot) return;
+  int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
+  if (s1 <= s0) return;
+  for (int s = s0; s < s1; ++s)
+    if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: skipped instance
+  for (int a = s0 + 1; a < s1; ++a) {
+    int v = d.cl[a];
+    int key = d.orig[v];
+    int b = a - 1;
+    while (b >= s0) {
+      int u = d.cl[b];
+      if (d.orig[u] <= key) break;
+      d.cl[b + 1] = u;
+      --b;
+    }
+    d.cl[b + 1] = v;
+  }
+  for (int s = s0; s < s1; ++s) d.clpos[d.cl[s]] = s;
+}
+
+// -------------------------------------------------------------- boundary
+
+// One block per instance: per boundary particle, the number of boundary
+// pairs found in each of its 3x3 cells in the reference's scan order
+// (y outer, x inner; nnps.f90:458-491), the exclusive prefix over boundary
+// particles (= rank of its first pair), and the boundary particles sorted
+// by cell for the particle-side lookup.
+__global__ void __launch_bounds__(kTPB) k_bprep(DevState d, int buf) {
+  __shared__ unsigned long long keys[kMaxBoundaryPerInst];
+  const int k = blockIdx.x;
+  const InstMeta &m = d.meta[k];
+  InstDyn &y = d.dyn[k];
+  if (m.nb == 0 || y.err) return;
+  const double bin = y.bin;
+  const int W = y.W, xlo = y.xlo, ylo = y.ylo;
+  const double *X = d.x[buf], *Y = d.y[buf];
+  const int *cs = d.cell_start + m.c_off;
+  for (int b = threadIdx.x; b < m.nb; b += kTPB) {
+    double bx = d.xb[m.b_off + b], by = d.yb[m.b_off + b];
+    int cxb = nint(bx / bin) - xlo + 1, cyb = nint(by / bin) - ylo + 1;
+    int cnt = 0;
+    for (int s9 = 0; s9 < 9; ++s9) {
+      int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
+      d.bpre[(m.b_off + b) * 9 + s9] = cnt;
+      for (int s = cs[c]; s < cs[c + 1]; ++s) {
+        int q = d.cl[s];
+        double dx = bx - X[q], dy = by - Y[q];
+        double dd = sqrt(dx * dx + dy * dy);
+        if (dd <= kBLength) ++cnt;
+      }
+    }
+    d.bcount[m.b_off + b] = cnt;
+    keys[b] = ((unsigned long long)(unsigned)(cyb * W + cxb) << 32) | (unsigned)b;
+  }
+  int P = 1;
+  while (P < m.nb) P <<= 1;
+  for (int i = m.nb + threadIdx.x; i < P; i += kTPB) keys[i] = ~0ull;
+  __syncthreads();
+  // exclusive scan of bcount over the instance's boundary particles
+  const int per = (m.nb + kTPB - 1) / kTPB;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int b = b0; b < b0 + per && b < m.nb; ++b) s += d.bcount[m.b_off + b];
+  int tot = 0;
+  int ex = block_exclusive_scan(s, &tot);
+  for (int b = b0; b < b0 + per && b < m.nb; ++b) {
+    int c = d.bcount[m.b_off + b];
+    d.bbase[m.b_off + b] = ex;
+    ex += c;
+  }
+  if (threadIdx.x == 0) {
+    y.bnip = tot;
+    if (tot > m.q_cap) atomicOr(&y.err, kErrCapacity);
+  }
+  // bitonic sort of (cell, b)
+  for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += kTPB) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          unsigned long long a = keys[i], b = keys[ixj];
+          bool up = (i & k2) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < m.nb; i += kTPB) d.bsorted[m.b_off + i] = keys[i];
+}
+
+// --------------------------------------------------------------- neighbours
+
+// The 3x3 cell block of a particle as six contiguous ranges of the cell
+// lists, in the reference's pair order for that particle.  kind: 0 = pair
+// owned by the neighbour's cell (or by an earlier member of the own cell),
+// 1 = own-cell pair owned by this particle (intra, nnps.f90:265-297),
+// 2 = cross-cell pair owned by this particle (right, up, up-right, up-left;
+// nnps.f90:300-407).
+struct Ranges {
+  int a[6], b[6];
+};
+
+__device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx, int cy) {
+  Ranges r;
+  const int rm = (cy - 1) * W, r0 = cy * W, rp = (cy + 1) * W;
+  r.a[0] = cs[rm + cx - 1]; r.b[0] = cs[rm + cx + 2];   // row below
+  r.a[1] = cs[r0 + cx - 1]; r.b[1] = cs[r0 + cx];       // left
+  r.a[2] = r.b[1];          r.b[2] = cs[r0 + cx + 1];   // own cell
+  r.a[3] = r.b[2];          r.b[3] = cs[r0 + cx + 2];   // right
+  r.a[4] = cs[rp + cx];     r.b[4] = cs[rp + cx + 2];   // up, up-right
+  r.a[5] = cs[rp + cx - 1]; r.b[5] = r.a[4];            // up-left
+  return r;
+}
+
+__device__ __forceinline__ int range_kind(int g, int s, int self) {
+  if (g <= 1) return 0;
+  if (g == 2) return s > self ? 1 : 0;
+  return 2;
+}
+
+// Exact FP64 form of `d = SQRT(SUM(dx**2)); IF (d .LE. mhsml)`: the square
+// root is taken only when d^2 is within 1e-13 of mhsml^2, which decides the
+// far candidates without it.
+__device__ __forceinline__ bool in_support(double d2, double mh, double &r) {
+  if (d2 > (mh * mh) * (1.0 + 1e-13)) return false;
+  r = sqrt(d2);
+  return r <= mh;
+}
+
+// Wendland C6 gradient factor (kernel.f90:499-508):
+// dW/dx = g * dx / r with g = factor * (-22 q (1-q)^7 (16 q^2 + 7 q + 1)) / hsml_ij
+__device__ __forceinline__ double c6_grad(double r, double mh) {
+  double q = r / mh * 1.0;
+  double ih = 1.0 / mh;
+  double factor = kC6 * (ih * ih);
+  double omq = 1 - q;
+  double o7 = ipow<7>(omq);
+  return factor * (-(22. * q * o7 * (16 * ipow<2>(q) + 7. * q + 1))) / mh;
+}
+
+__device__ __forceinline__ float c6_grad_f(float r, float mh) {
+  float ih = __builtin_amdgcn_rcpf(mh);
+  float q = r * ih;
+  float factor = kC6f * (ih * ih);
+  float omq = 1.0f - q;
+  float o2 = omq * omq, o4 = o2 * o2;
+  float o7 = o4 * o2 * omq;
+  float poly = __builtin_fmaf(__builtin_fmaf(16.0f, q, 7.0f), q, 1.0f);
+  return -(factor * 22.0f * q * o7 * poly) * ih;
+}
+
+// Compensated (TwoSum) FP32 accumulator.
+struct KSum {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float v) {
+    float t = s + v;
+    float bp = t - s;
+    c += (s - (t - bp)) + (v - bp);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+// ---------------------------------------------------------------- pass A
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k)) return;
+  const InstDyn &y = d.dyn[k];
+  if (y.err) return;
+  const InstMeta &m = d.meta[k];
+  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
+  const double *VX = d.vx[buf], *VY = d.vy[buf], *RHO = d.rho[buf];
+  const int W = y.W;
+  const int c = d.cellid[p];
+  const int cy = c / W, cx = c - cy * W;
+  const int self = d.clpos[p];
+  const double xi = X[p], yi = Y[p], hi = HS[p], vxi = VX[p], vyi = VY[p];
+  const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+
+  double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
+  KSum fdiv, f1, f2, f3, f4;
+  int nI = 0, nX = 0, nn = 0;
+  for (int g = 0; g < 6; ++g) {
+    for (int s = R.a[g]; s < R.b[g]; ++s) {
+      const int q = d.cl[s];
+      if (q == p) continue;
+      const double dx = xi - X[q], dy = yi - Y[q];
+      const double mh = (hi + HS[q]) / 2.;
+      double r;
+      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      const int kind = range_kind(g, s, self);
+      nI += kind == 1;
+      nX += kind == 2;
+      ++nn;
+      const double mq = d.mass[q], rq = RHO[q];
+      const double dv0 = vxi - VX[q], dv1 = vyi - VY[q];
+      if (!MIXED) {
+        const double gg = c6_grad(r, mh);
+        const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+        const double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
+        div = div - mq * (w1 + w4);
+        g1 = g1 - mq * w1 / rq;
+        g2 = g2 - mq * w2 / rq;
+        g3 = g3 - mq * w3 / rq;
+        g4 = g4 - mq * w4 / rq;
+      } else {
+        const float rf = (float)r;
+        const float gg = c6_grad_f(rf, (float)mh);
+        const float ir = __builtin_amdgcn_rcpf(rf);
+        const float dw0 = gg * ((float)dx * ir), dw1 = gg * ((float)dy * ir);
+        const float a0 = (float)dv0, a1 = (float)dv1;
+        const float w1 = a0 * dw0, w2 = a1 * dw0, w3 = a0 * dw1, w4 = a1 * dw1;
+        const float mf = (float)mq, mr = (float)(mq / rq);
+        fdiv.add(-(mf * (w1 + w4)));
+        f1.add(-(mr * w1));
+        f2.add(-(mr * w2));
+        f3.add(-(mr * w3));
+        f4.add(-(mr * w4));
+      }
+    }
+  }
+  if (MIXED) {
+    div = fdiv.value();
+    g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
+  }
+  if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
+  d.own_cnt[p] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
+
+  // strain rate (single_step.f90:80-84)
+  const double e11 = g1, e22 = g4, e12 = 0.5 * (g2 + g3);
+  // Hibler P (internal_stress.f90:24), elliptical zeta / eta
+  // (bulk_viscosity.f90:65-76, shear_viscosity.f90:57), Kreyscher
+  // replacement pressure (eos.f90:141-147), stress tensor
+  // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
+  // delta -> delta_min switch is decided here.
+  const double hth = d.h[buf][p], Ac = d.A[buf][p];
+  const double em2 = 1.0 / (kE * kE);
+  const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
+  const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
+                            2 * e11 * e22 * (1 - em2));
+  const double dcl = fabs(delta) < kDeltaMin ? kDeltaMin : delta;
+  const double z = 0.5 * P * (1 + kKT) / dcl;
+  const double et = z / (kE * kE);
+  const double pr = z * 2 * delta / (1 + kKT);
+  const double s11 = (z + et) * e11 + (z - et) * e22 - pr * (1 - kKT) / 2;
+  const double s12 = 2 * et * e12;
+  const double s22 = (z + et) * e22 + (z - et) * e11 - pr * (1 - kKT) / 2;
+  if (!MIXED) {
+    ((double *)d.sig11)[p] = s11;
+    ((double *)d.sig12)[p] = s12;
+    ((double *)d.sig22)[p] = s22;
+    ((double *)d.div)[p] = div;
+  } else {
+    ((float *)d.sig11)[p] = (float)s11;
+    ((float *)d.sig12)[p] = (float)s12;
+    ((float *)d.sig22)[p] = (float)s22;
+    ((double *)d.div)[p] = div;
+  }
+  if (want_diag) {
+    const int N = d.ntot;
+    d.diag[p] = e11;
+    d.diag[N + p] = e12;
+    d.diag[2 * N + p] = e22;
+    d.diag[3 * N + p] = z;
+    d.diag[4 * N + p] = et;
+    d.diag[5 * N + p] = pr;
+  }
+}
+
+// ------------------------------------------------------- pair-rank store
+
+// Per cell: within-cell offsets of each member's owned intra and cross
+// pairs, and the cell's pair total.
+__global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
+  int c = blockIdx.x * kTPB + threadIdx.x;
+  if (c >= d.ctot) return;
+  int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
+  int I = 0, X = 0;
+  for (int s = s0; s < s1; ++s) {
+    int oc = d.own_cnt[d.cl[s]];
+    d.moff[s] = I | (X << 16);
+    I += oc & 1023;
+    X += (oc >> 10) & 1023;
+  }
+  d.pair_count[c] = I + X;
+  d.cell_intra[c] = I;
+}
+
+// hsml_ij of the particle pairs ranked below min(bnip, nip): the entries
+// b_Wendland_C6 reads at the boundary-pair index (boundary_kernel.f90:491).
+// Entries at or above nip keep what an earlier evaluation wrote, as the
+// reference's module array does.
+__global__ void __launch_bounds__(kTPB) k_materialize(DevState d, int buf) {
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k)) return;
+  const InstDyn &y = d.dyn[k];
+  const InstMeta &m = d.meta[k];
+  if (m.nb == 0 || y.err) return;
+  const int pbase = d.pair_start[m.c_off];
+  const int nipk = d.pair_start[m.c_off + m.c_cap] - pbase;
+  const int lim = min(min(y.bnip, m.q_cap), nipk);
+  const int oc = d.own_cnt[p];
+  const int nI = oc & 1023, nX = (oc >> 10) & 1023;
+  if (nI + nX == 0) return;
+  const int c = d.cellid[p];
+  const int s = d.clpos[p];
+  const int base = d.pair_start[m.c_off + c] - pbase;
+  int rI = base + (d.moff[s] & 0xffff);
+  int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
+  if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
+  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
+  const int W = y.W, cy = c / W, cx = c - cy * W;
+  const double xi = X[p], yi = Y[p], hi = HS[p];
+  const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+  for (int g = 2; g < 6; ++g) {
+    for (int t = R.a[g]; t < R.b[g]; ++t) {
+      if (g == 2 && t <= s) continue;
+      const int q = d.cl[t];
+      const double dx = xi - X[q], dy = yi - Y[q];
+      const double mh = (hi + HS[q]) / 2.;
+      double r;
+      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      int &rank = (g == 2) ? rI : rX;
+      if (rank < lim) d.hq[m.q_off + rank] = mh;
+      ++rank;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+
+__device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, double px,
+                                        double py, double &u, double &v) {
+  if (m.wind_mode != 1) {
+    u = m.wind_u;
+    v = m.wind_v;
+    return;
+  }
+  const int nx = m.g_nx, ny = m.g_ny;
+  const double fx = (px - m.g_x0) / m.g_dx, fy = (py - m.g_y0) / m.g_dy;
+  const int i0 = (int)fmin(fmax(floor(fx), 0.0), (double)(nx - 2));
+  const int j0 = (int)fmin(fmax(floor(fy), 0.0), (double)(ny - 2));
+  const double tx = fmin(fmax(fx - (double)i0, 0.0), 1.0);
+  const double ty = fmin(fmax(fy - (double)j0, 0.0), 1.0);
+  const double *U = d.grid + m.g_off;
+  const double *V = U + (long)nx * ny;
+  const long r0 = (long)j0 * nx, r1 = r0 + nx;
+  const double u0 = (1.0 - tx) * U[r0 + i0] + tx * U[r0 + i0 + 1];
+  const double u1 = (1.0 - tx) * U[r1 + i0] + tx * U[r1 + i0 + 1];
+  const double v0 = (1.0 - tx) * V[r0 + i0] + tx * V[r0 + i0 + 1];
+  const double v1 = (1.0 - tx) * V[r1 + i0] + tx * V[r1 + i0 + 1];
+  u = (1.0 - ty) * u0 + ty * u1;
+  v = (1.0 - ty) * v0 + ty * v1;
+}
+
+// phase: 0 = predictor (state S -> T), 1 = corrector (RHS on T, update S in
+// place), 2 = single evaluation (tendencies to d.tend)
+template <bool MIXED>
+__global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
+  int p = 0, k = 0;
+  const bool ok = particle_of(d, p, k);
+  const InstDyn &y = d.dyn[k];
+  const InstMeta &m = d.meta[k];
+  const bool live = ok && !y.err;
+  const int buf = phase == 1 ? 1 : 0;
+  double hsml_new = __builtin_huge_val();
+  if (live) {
+    const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RHO = d.rho[buf];
+    const int W = y.W;
+    const int c = d.cellid[p];
+    const int cy = c / W, cx = c - cy * W;
+    const int self = d.clpos[p];
+    const double xi = X[p], yi = Y[p], hi = HS[p], rhoi = RHO[p], mi = d.mass[p];
+    const double hthi = d.h[buf][p], Ai = d.A[buf][p];
+    const double vxi = d.vx[buf][p], vyi = d.vy[buf][p];
+    const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+
+    double ax = 0., ay = 0.;
+    // stress_1 (internal_stress.f90:103-136)
+    if (!MIXED) {
+      const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
+                   *S22 = (const double *)d.sig22;
+      const double s11i = S11[p], s12i = S12[p], s22i = S22[p];
+      const double ri2 = rhoi * rhoi;
+      for (int g = 0; g < 6; ++g) {
+        for (int s = R.a[g]; s < R.b[g]; ++s) {
+          const int q = d.cl[s];
+          if (q == p) continue;
+          const double dx = xi - X[q], dy = yi - Y[q];
+          const double mh = (hi + HS[q]) / 2.;
+          double r;
+          if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          const double gg = c6_grad(r, mh);
+          const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+          const double rq = RHO[q], mq = d.mass[q];
+          const double rq2 = rq * rq;
+          const double coef = rhoi * mq / kRhoIce / hthi;
+          double cj = (S11[q] * dw0 + S12[q] * dw1) / rq2;
+          double ci = (s11i * dw0 + s12i * dw1) / ri2;
+          ax = ax + coef * (ci + cj);
+          cj = (S12[q] * dw0 + S22[q] * dw1) / rq2;
+          ci = (s12i * dw0 + s22i * dw1) / ri2;
+          ay = ay + coef * (ci + cj);
+        }
+      }
+    } else {
+      const float *S11 = (const float *)d.sig11, *S12 = (const float *)d.sig12,
+                  *S22 = (const float *)d.sig22;
+      const float ir2 = (float)(1.0 / (rhoi * rhoi));
+      const float s11i = S11[p] * ir2, s12i = S12[p] * ir2, s22i = S22[p] * ir2;
+      const double cf = rhoi / kRhoIce / hthi;
+      KSum sx, sy;
+      for (int g = 0; g < 6; ++g) {
+        for (int s = R.a[g]; s < R.b[g]; ++s) {
+          const int q = d.cl[s];
+          if (q == p) continue;
+          const double dx = xi - X[q], dy = yi - Y[q];
+          const double mh = (hi + HS[q]) / 2.;
+          double r;
+          if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          const float rf = (float)r;
+          const float gg = c6_grad_f(rf, (float)mh);
+          const float ir = __builtin_amdgcn_rcpf(rf);
+          const float dw0 = gg * ((float)dx * ir), dw1 = gg * ((float)dy * ir);
+          const double rq = RHO[q];
+          const float irq2 = (float)(1.0 / (rq * rq));
+          const float mq = (float)d.mass[q];
+          const float t11 = __builtin_fmaf(S11[q], irq2, s11i);
+          const float t12 = __builtin_fmaf(S12[q], irq2, s12i);
+          const float t22 = __builtin_fmaf(S22[q], irq2, s22i);
+          sx.add(mq * (t11 * dw0 + t12 * dw1));
+          sy.add(mq * (t12 * dw0 + t22 * dw1));
+        }
+      }
+      ax = cf * sx.value();
+      ay = cf * sy.value();
+    }
+
+    // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
+    // ascending boundary index (boundary_kernel.f90:477-502)
+    if (m.nb > 0) {
+      // Boundary neighbours are taken in ascending boundary index by a
+      // selection over the 3x3 cell block (a particle sees a handful).
+      int nbn = 0;
+      int last = -1;
+      const unsigned long long *BS = d.bsorted + m.b_off;
+      int lo_t[3], hi_t[3];
+      for (int dyc = 0; dyc < 3; ++dyc) {
+        const unsigned lo = (unsigned)((cy + dyc - 1) * W + cx - 1);
+        int a = 0, b = m.nb;
+        while (a < b) {
+          int mid = (a + b) >> 1;
+          if ((unsigned)(BS[mid] >> 32) < lo) a = mid + 1; else b = mid;
+        }
+        int e = a;
+        while (e < m.nb && (unsigned)(BS[e] >> 32) <= lo + 2) ++e;
+        lo_t[dyc] = a;
+        hi_t[dyc] = e;
+      }
+      while (true) {
+        int best = 0x7fffffff, bcell = 0;
+        for (int dyc = 0; dyc < 3; ++dyc) {
+          for (int t = lo_t[dyc]; t < hi_t[dyc]; ++t) {
+            const unsigned long long key = BS[t];
+            const int bl = (int)(key & 0xffffffffu);
+            if (bl <= last || bl >= best) continue;
+            const int bg = m.b_off + bl;
+            const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
+            if (!(sqrt(bdx * bdx + bdy * bdy) <= kBLength)) continue;
+            best = bl;
+            bcell = (int)(key >> 32);
+          }
+        }
+        if (best == 0x7fffffff) break;
+        last = best;
+        ++nbn;
+        const int bg = m.b_off + best;
+        const int cyb = bcell / W, cxb = bcell - cyb * W;
+        const int slot9 = (cy - cyb + 1) * 3 + (cx - cxb + 1);
+        int before = 0;
+        for (int s = R.a[2]; s < self; ++s) {
+          const int q = d.cl[s];
+          const double ex = d.xb[bg] - X[q], ey = d.yb[bg] - Y[q];
+          if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
+        }
+        const int rank = d.bbase[bg] + d.bpre[bg * 9 + slot9] + before;
+        const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
+        const double br = sqrt(bdx * bdx + bdy * bdy);
+        const double q = br / kBLength * 1.0;
+        const double hk = rank < m.q_cap ? d.hq[m.q_off + rank] : 0.0;
+        const double ih = 1.0 / hk;
+        const double factor = kC6 * (ih * ih);
+        const double omq = 1 - q;
+        const double bw = factor * ipow<8>(omq) *
+                          (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
+        const double mb = d.massb[bg];
+        const double den = (mi + mb) * (br * br);
+        const double F0 = kKappaN * bdx * bw * 2 * mb / den;
+        const double F1 = kKappaN * bdy * bw * 2 * mb / den;
+        ax = ax + (-F0) / rhoi;
+        ay = ay + (-F1) / rhoi;
+      }
+      // interaction_stats capacity check (nnps.f90:660-668)
+      if (y.itimestep % kPrintStep == 0 && (d.own_cnt[p] >> 20) + nbn > kMaxIndInt)
+        atomicOr(&d.dyn[k].err, kErrCapacity);
+    } else if (y.itimestep % kPrintStep == 0 && (d.own_cnt[p] >> 20) > kMaxIndInt) {
+      atomicOr(&d.dyn[k].err, kErrCapacity);
+    }
+
+    // wind_forcing / water_drag (wind_forcing.f90:35-50, water_drag.f90:35-51)
+    if (y.time > 0.) {
+      double wu, wv;
+      wind_at(d, m, xi, yi, wu, wv);
+      const double du = wu - vxi, dv = wv - vyi;
+      const double hr = hthi * kRhoIce;
+      ax = ax + (kCWind * kRhoAir * du * fabs(du)) / hr;
+      ay = ay + (kCWind * kRhoAir * dv * fabs(dv)) / hr;
+      const double cu = m.cur_u - vxi, cv = m.cur_v - vyi;
+      ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
+      ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
+    }
+    // tendencies (single_step.f90:104-106)
+    const double divi = ((const double *)d.div)[p];
+    const double dhdt = -(hthi * divi / rhoi);
+    const double dAdt = -(Ai * divi / rhoi);
+    const double dt = y.dt;
+    if (phase == 0) {
+      // predict half step (time_integration.f90:207-218)
+      d.x[1][p] = xi + vxi * dt / 2;
+      d.y[1][p] = yi + vyi * dt / 2;
+      d.vx[1][p] = vxi + ax * dt / 2;
+      d.vy[1][p] = vyi + ay * dt / 2;
+      const double hh = hthi + dhdt * dt / 2;
+      double Ah = Ai + dAdt * dt / 2;
+      if (Ah > 1) Ah = 1;
+      d.h[1][p] = hh;
+      d.A[1][p] = Ah;
+      d.rho[1][p] = hh * kRhoIce;
+      d.hsml[1][p] = hi;
+    } else if (phase == 1) {
+      // corrector (time_integration.f90:236-280) on the step-start state S
+      const double x1 = d.x[0][p], y1 = d.y[0][p];
+      const double u1 = d.vx[0][p], v1 = d.vy[0][p];
+      const double h1 = d.h[0][p], A1 = d.A[0][p];
+      const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
+      const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
+      const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
+      d.x[0][p] = 2 * x12 - x1;
+      d.y[0][p] = 2 * y12 - y1;
+      d.vx[0][p] = 2 * u12 - u1;
+      d.vy[0][p] = 2 * v12 - v1;
+      const double hn = 2 * h12 - h1;
+      double An = 2 * A12 - A1;
+      if (An > 1) An = 1;
+      d.h[0][p] = hn;
+      d.A[0][p] = An;
+      const double rn = hn * kRhoIce;
+      d.rho[0][p] = rn;
+      // Marquis_hsml (smoothing_length.f90:136-142)
+      double hs = sqrt(mi / rn) * kHsmlMult;
+      if (hs > m.max_hsml) hs = m.max_hsml;
+      d.hsml[0][p] = hs;
+      hsml_new = hs;
+    } else {
+      const int N = d.ntot;
+      d.tend[p] = vxi;
+      d.tend[N + p] = vyi;
+      d.tend[2 * N + p] = ax;
+      d.tend[3 * N + p] = ay;
+      d.tend[4 * N + p] = dhdt;
+      d.tend[5 * N + p] = dAdt;
+    }
+  }
+  if (phase == 1 && m.tse == 2) {
+    // VP_time_step (time_step.f90:155): per-instance MINVAL
+    double v = live ? kRhoIce * (hsml_new * hsml_new) * kDeltaMin / (kPStar * (1 + kKT))
+                    : __builtin_huge_val();
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0 && __any(live)) atomicMin(&d.dyn[k].dtmin, ord_u64(v));
+  }
+}
+
+// End of a predictor-corrector step (time_integration.f90:68-71).
+__global__ void k_step_end(DevState d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.ninst) return;
+  InstDyn &y = d.dyn[k];
+  if (y.err) return;
+  if (d.meta[k].tse == 2) y.dt = ord_f64(y.dtmin);
+  y.dtmin = ~0ull;
+  y.itimestep += 1;
+  y.time = y.time + y.dt;
+}
+
+// ------------------------------------------------------- pair-list export
+
+// Particle pairs owned by p at their ranks in the reference's order
+// (nnps.f90:259-411): p_i, p_j 1-based, r_ij, hsml_ij.
+__global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int cap, int *pi,
+                                                       int *pj, double *rij, double *hij) {
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k)) return;
+  const InstDyn &y = d.dyn[k];
+  const InstMeta &m = d.meta[k];
+  if (y.err) return;
+  const int oc = d.own_cnt[p];
+  if ((oc & 0xfffff) == 0) return;
+  const int c = d.cellid[p];
+  const int s = d.clpos[p];
+  const int pbase = d.pair_start[m.c_off];
+  const int base = d.pair_start[m.c_off + c] - pbase;
+  int rI = base + (d.moff[s] & 0xffff);
+  int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
+  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
+  const int W = y.W, cy = c / W, cx = c - cy * W;
+  const double xi = X[p], yi = Y[p], hi = HS[p];
+  const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+  for (int g = 2; g < 6; ++g) {
+    for (int t = R.a[g]; t < R.b[g]; ++t) {
+      if (g == 2 && t <= s) continue;
+      const int q = d.cl[t];
+      const double dx = xi - X[q], dy = yi - Y[q];
+      const double mh = (hi + HS[q]) / 2.;
+      double r;
+      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      int &rank = (g == 2) ? rI : rX;
+      if (rank < cap) {
+        pi[rank] = d.orig[p] + 1;
+        pj[rank] = d.orig[q] + 1;
+        rij[rank] = r;
+        hij[rank] = mh;
+      }
+      ++rank;
+    }
+  }
+}
+
+// Boundary pairs (nnps.f90:453-492) at their ranks: bp_i (boundary), bp_j
+// (particle), br_ij.  One thread per boundary particle of instance 0.
+__global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int bcap,
+                                                        int *bpi, int *bpj, double *brij) {
+  const int k = 0;
+  const InstMeta &m = d.meta[k];
+  const InstDyn &y = d.dyn[k];
+  int b = blockIdx.x * kTPB + threadIdx.x;
+  if (b >= m.nb || y.err) return;
+  const int bg = m.b_off + b;
+  const double bin = y.bin;
+  const int W = y.W;
+  const double bx = d.xb[bg], by = d.yb[bg];
+  const int cxb = nint(bx / bin) - y.xlo + 1, cyb = nint(by / bin) - y.ylo + 1;
+  const int *cs = d.cell_start + m.c_off;
+  int rank = d.bbase[bg];
+  for (int s9 = 0; s9 < 9; ++s9) {
+    int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
+    for (int s = cs[c]; s < cs[c + 1]; ++s) {
+      int q = d.cl[s];
+      double dx = bx - d.x[buf][q], dy = by - d.y[buf][q];
+      double dd = sqrt(dx * dx + dy * dy);
+      if (dd <= kBLength) {
+        if (rank < bcap) {
+          bpi[rank] = b + 1;
+          bpj[rank] = d.orig[q] + 1;
+          brij[rank] = dd;
+        }
+        ++rank;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------- launches
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+void launch_eval_begin(const DevState &d, hipStream_t st) {
+  k_eval_begin<<<cdiv(d.ninst, 64), 64, 0, st>>>(d);
+}
+void launch_reduce(const DevState &d, int buf, hipStream_t st) {
+  k_reduce<<<d.nblk, kTPB, 0, st>>>(d, buf);
+}
+void launch_count(const DevState &d, int buf, hipStream_t st) {
+  k_count<<<d.nblk, kTPB, 0, st>>>(d, buf);
+}
+void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st) {
+  int nb = cdiv(L, kScanItems);
+  if (nb < 1) nb = 1;
+  k_scan1<<<nb, kTPB, 0, st>>>(in, L, tmp);
+  k_scan2<<<nb, kTPB, 0, st>>>(in, L, tmp, out, zero_in);
+}
+void launch_scatter(const DevState &d, hipStream_t st) {
+  k_scatter<<<d.nblk, kTPB, 0, st>>>(d);
+}
+void launch_cellsort(const DevState &d, hipStream_t st) {
+  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
+}
+void launch_bprep(const DevState &d, int buf, hipStream_t st) {
+  k_bprep<<<d.ninst, kTPB, 0, st>>>(d, buf);
+}
+void launch_passA(const DevState &d, int buf, int mixed, int want_diag, hipStream_t st) {
+  if (mixed)
+    k_passA<true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+  else
+    k_passA<false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+}
+void launch_pairtot(const DevState &d, hipStream_t st) {
+  k_pairtot<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
+}
+void launch_materialize(const DevState &d, int buf, hipStream_t st) {
+  k_materialize<<<d.nblk, kTPB, 0, st>>>(d, buf);
+}
+void launch_passB(const DevState &d, int mixed, int phase, hipStream_t st) {
+  if (mixed)
+    k_passB<true><<<d.nblk, kTPB, 0, st>>>(d, phase);
+  else
+    k_passB<false><<<d.nblk, kTPB, 0, st>>>(d, phase);
+}
+void launch_step_end(const DevState &d, hipStream_t st) {
+  k_step_end<<<cdiv(d.ninst, 64), 64, 0, st>>>(d);
+}
+void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
+                         double *hij, hipStream_t st) {
+  k_export_pairs<<<d.nblk, kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij);
+}
+void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
+                          double *brij, hipStream_t st) {
+  if (nb > 0) k_export_bpairs<<<cdiv(nb, kTPB), kTPB, 0, st>>>(d, buf, bcap, bpi, bpj, brij);
+}
+
+}  // namespace simp

@@ -1,0 +1,136 @@
+"""Write the golden vectors of the reference SIMP hot path.
+
+Runs the compiled reference (``oracle/_ref/simp_ref_harness``, built from the
+reference's own Fortran sources by ``oracle/build_ref.sh``) on small seeded
+inputs and stores inputs and outputs as ``.npz`` fixtures next to this
+script.  The reference has no tests or fixtures of its own; these vectors are
+the pin for the oracle.  Regenerate with::
+
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import capi, refrun  # noqa: E402
+from simp_amd import scenarios as S  # noqa: E402
+
+STATE_KEYS = ("x", "vx", "mass", "rho", "hsml", "h", "A", "xb", "massb")
+
+
+def _inputs(sc, prefix="in_"):
+    d = {prefix + k: np.asarray(getattr(sc, k)) for k in STATE_KEYS}
+    d[prefix + "scalars"] = np.array([sc.dt, sc.time, sc.max_hsml])
+    return d
+
+
+def _save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def _outputs(r, keys):
+    return {"out_" + k: np.asarray(r[k]) for k in keys}
+
+
+STEP_KEYS = ("p_i", "p_j", "r_ij", "hsml_ij", "bp_i", "bp_j", "br_ij", "dxdt", "dvxdt",
+             "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma")
+STATE_OUT = ("x", "vx", "rho", "hsml", "A", "h", "eps", "zeta", "eta", "p", "sigma")
+
+
+def ties():
+    """Particles on exact half-bin coordinates (NINT ties) at pair distance
+    exactly equal to mhsml, plus boundary particles on ties."""
+    n1 = 6
+    s = 1000.0
+    i, j = np.meshgrid(np.arange(n1), np.arange(n1), indexing="xy")
+    x = np.stack([(2 * i.ravel() + 1) * s, (2 * j.ravel() + 1) * s], axis=1).astype(float)
+    n = x.shape[0]
+    hsml = np.full(n, 2 * s)          # bin = 4000; x / bin = k/2 + 1/4 or ties
+    hsml[::5] = 2.5 * s
+    h = np.linspace(0.8, 1.2, n)
+    A = np.ones(n)
+    xb = S.walls(0.0, 0.0, 2 * n1 * s, 2 * n1 * s)
+    xb = np.round(xb / 2000.0) * 2000.0     # boundary particles on half-bin ties
+    xb = np.unique(xb, axis=0)
+    sc = S._finish(x, hsml, h, A, xb, None, tse=2, name="ties")
+    return sc
+
+
+def main():
+    if not refrun.available():
+        raise SystemExit("build the reference first: bash oracle/build_ref.sh")
+    # 1. the 2,500-particle closed box: NNPS and one single_step at t > 0
+    sc = S.box(nside=50, spacing=2000.0, seed=1)
+    sc.vx = np.random.default_rng(101).normal(0.0, 0.1, size=(sc.n, 2))
+    sc.time = 10.0
+    r = refrun.run(refrun.MODE_SINGLE_STEP, sc)
+    _save("box2500_single_step.npz", **_inputs(sc), **_outputs(r, STEP_KEYS),
+          out_nip=r["nip"], out_bnip=r["bnip"])
+
+    # 2. the same box, 20 predictor-corrector steps from rest at t = 0
+    #    (no forcing in the first step, VP time step, stale hsml_ij reads)
+    sc = S.box(nside=50, spacing=2000.0, seed=1)
+    K = 20
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save("box2500_pc20.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end)
+
+    # 3. open ice edge with variable hsml and one coast wall (reference wind)
+    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
+    sc.wind_grid = None
+    sc.wind = S.REFERENCE_WIND
+    K = 10
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save("edge3600_pc10.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end)
+
+    # 4. NINT ties and d == mhsml pairs
+    sc = ties()
+    r = refrun.run(refrun.MODE_NNPS, sc)
+    _save("ties_nnps.npz", **_inputs(sc),
+          **_outputs(r, ("p_i", "p_j", "r_ij", "hsml_ij", "bp_i", "bp_j", "br_ij")),
+          out_nip=r["nip"], out_bnip=r["bnip"])
+
+    # 5. two consecutive single_step calls where the second has fewer
+    #    particle pairs than boundary pairs: b_Wendland_C6 reads hsml_ij
+    #    entries the first call left behind (boundary_kernel.f90:491)
+    a = S.box(nside=12, spacing=1000.0, seed=5, hsml_factor=2.2)
+    b = a.copy()
+    b.hsml = a.hsml * (1.3 / 2.2)
+    b.vx = np.random.default_rng(7).normal(0.0, 0.1, size=(b.n, 2))
+    a.time = b.time = 3.0
+    r = refrun.run(refrun.MODE_TWO_STATES, a, second=b)
+    assert r["bnip"] > r["nip"], (r["nip"], r["bnip"])
+    _save("stale_two_calls.npz", **_inputs(a, "a_"), **_inputs(b, "in_"),
+          **_outputs(r, STEP_KEYS), out_nip=r["nip"], out_bnip=r["bnip"])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,35 @@
+"""Loading of the golden fixtures written by tests/golden/make_golden.py."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from simp_amd.scenarios import REFERENCE_WIND, Scenario
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+
+
+def scenario(g, prefix="in_") -> Scenario:
+    dt, t, max_hsml = g[prefix + "scalars"]
+    sc = Scenario(x=g[prefix + "x"].copy(), vx=g[prefix + "vx"].copy(),
+                  mass=g[prefix + "mass"].copy(), rho=g[prefix + "rho"].copy(),
+                  hsml=g[prefix + "hsml"].copy(), h=g[prefix + "h"].copy(),
+                  A=g[prefix + "A"].copy(), xb=g[prefix + "xb"].reshape(-1, 2).copy(),
+                  massb=g[prefix + "massb"].copy(), max_hsml=float(max_hsml),
+                  wind=REFERENCE_WIND, tse=2, dt=float(dt), time=float(t))
+    return sc
+
+
+def run_pc_like_reference(o, sc, integration_time):
+    """Oracle steps until current_time reaches integration_time, the loop
+    condition of time_integration.f90:38."""
+    it = 0
+    while sc.time < integration_time:
+        out = o.run_pc(sc, 1, itimestep=it)
+        it += 1
+    return it, out
