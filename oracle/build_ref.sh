@@ -46,3 +46,22 @@ done
 "$FC" $FFLAGS -o "$OUT/simp_ref_harness" "$OUT/obj/ref_harness.o" "${objs[@]}" \
     -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
 echo "build_ref: $OUT/simp_ref_harness"
+
+# The drop-in check: the same harness and the reference's own
+# time_integration, with single_step replaced by the iso_c_binding shim of
+# simp_amd/fortran that calls libsimp_amd.so on the GPU.
+LIB="$HERE/../simp_amd/lib/libsimp_amd.so"
+if [ -f "$LIB" ]; then
+    FSRC="$HERE/../simp_amd/fortran"
+    "$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$FSRC/simp_binding.f90" -o "$OUT/obj/simp_binding.o"
+    "$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$FSRC/single_step_gpu.f90" -o "$OUT/obj/single_step_gpu.o"
+    gobjs=()
+    for o in "${objs[@]}"; do
+        [ "$(basename "$o")" = "single_step.o" ] || gobjs+=("$o")
+    done
+    "$FC" $FFLAGS -o "$OUT/simp_ref_harness_gpu" "$OUT/obj/ref_harness.o" "${gobjs[@]}" \
+        "$OUT/obj/simp_binding.o" "$OUT/obj/single_step_gpu.o" \
+        -L"$(dirname "$LIB")" -lsimp_amd -L/opt/rocm/llvm/lib -lomp \
+        -Wl,-rpath,'$ORIGIN/../../simp_amd/lib' -Wl,-rpath,/opt/rocm/llvm/lib -Wl,-rpath,/opt/rocm/lib
+    echo "build_ref: $OUT/simp_ref_harness_gpu"
+fi

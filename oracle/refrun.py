@@ -17,6 +17,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(HERE, "_ref", "simp_ref_harness")
+#: the same harness with single_step replaced by the GPU shim
+#: (simp_amd/fortran/single_step_gpu.f90 -> libsimp_amd.so)
+HARNESS_GPU = os.path.join(HERE, "_ref", "simp_ref_harness_gpu")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
 
@@ -89,19 +92,21 @@ def _big_stack():
         resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
-def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600, second=None):
+def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600, second=None,
+        harness=None):
     """Run the reference harness on scenario ``sc`` (and ``second`` for
     MODE_TWO_STATES); returns a dict of its outputs.  The reference's wind is
     its own hard-coded field."""
-    if not available():
-        raise FileNotFoundError(HARNESS)
+    exe = harness or HARNESS
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
     if sc.n > MAXN or sc.nb > MAXNB:
         raise ValueError("scenario exceeds the reference's static capacity")
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         _write_input(fin, mode, sc, reps, integration_time, second)
         env = dict(os.environ, PWD=d, OMP_NUM_THREADS=str(threads))
-        subprocess.run([HARNESS, fin, fout], cwd=d, env=env, check=True,
+        subprocess.run([exe, fin, fout], cwd=d, env=env, check=True,
                        stdout=subprocess.DEVNULL, timeout=timeout,
                        preexec_fn=_big_stack)
         return _read_output(fout, sc.n)

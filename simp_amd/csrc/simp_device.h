@@ -39,6 +39,7 @@ constexpr int kErrBndNbr = 4;
 
 struct InstMeta {
   int p_off, n;          // particle range
+  int blk_off, nblk;     // logical particle-block range
   int b_off, nb;         // boundary-particle range
   int c_off, c_cap;      // cell range (capacity)
   int q_off, q_cap;      // boundary-pair / hsml_ij-store range
@@ -57,15 +58,16 @@ struct InstDyn {
   double bin;
   int xlo, ylo, W, H;
   int nip, bnip;
-  // reductions of the current evaluation, order-preserving u64 encodings
-  unsigned long long red_xmin, red_xmax, red_ymin, red_ymax, red_hmin;
-  unsigned long long dtmin;   // next VP dt
+  int stamp;            // evaluation counter (boundary cell flags)
+  int pad;
+  double hmin;          // MINVAL(hsml) of the current evaluation
 };
 
 // Per-launch block table entry: one block covers kTPB particles of exactly
 // one instance.
 struct BlockEntry {
   int inst, first;   // instance, first local particle index
+  int lid, pad;      // logical block id (index of the block's partials)
 };
 
 // Device pointers of one engine.
@@ -77,6 +79,10 @@ struct DevState {
   int *inst;            // instance of each packed particle
   // per evaluation
   int *cellid, *slot, *cl, *clpos, *count, *cell_start, *scan_tmp;
+  double *xs, *ys, *hs;  // x, y, hsml in cell-list order (candidate loads)
+  double *part;          // per-block partial reductions (5 x nblk)
+  int *bflag;            // per cell: stamp of the last evaluation in which a
+                         // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
   double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N), optional

@@ -99,6 +99,11 @@ int Engine::ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk,
     rc |= dalloc(allocs_, &d_.clpos, N);
     rc |= dalloc(allocs_, &d_.own_cnt, N);
     rc |= dalloc(allocs_, &d_.moff, N);
+    rc |= dalloc(allocs_, &d_.xs, N);
+    rc |= dalloc(allocs_, &d_.ys, N);
+    rc |= dalloc(allocs_, &d_.hs, N);
+    rc |= dalloc(allocs_, &d_.part, 5 * (size_t)cap_blk_);
+    rc |= dalloc(allocs_, &d_.bflag, C);
     rc |= dalloc(allocs_, &d_.count, C);
     rc |= dalloc(allocs_, &d_.cell_start, C);
     rc |= dalloc(allocs_, &d_.pair_count, C);
@@ -162,6 +167,7 @@ int Engine::upload(const std::vector<InstanceInput> &ins, bool keep_store) {
     InstMeta &m = meta[k];
     std::memset(&m, 0, sizeof(m));
     m.p_off = ntot; m.n = in.n;
+    m.blk_off = nblk; m.nblk = cdiv(in.n, kTPB);
     m.b_off = nbtot; m.nb = in.nb;
     long cells = in.c_cap;
     if (cells <= 0) {
@@ -242,11 +248,10 @@ int Engine::upload(const std::vector<InstanceInput> &ins, bool keep_store) {
       std::memcpy(&hgrid[m.g_off], in.g_u, nn * sizeof(double));
       std::memcpy(&hgrid[m.g_off + nn], in.g_v, nn * sizeof(double));
     }
-    for (int f = 0; f < in.n; f += kTPB) hblk.push_back(BlockEntry{k, f});
+    for (int f = 0; f < in.n; f += kTPB) hblk.push_back(BlockEntry{k, f, (int)hblk.size(), 0});
     InstDyn &y = hdyn[k];
     std::memset(&y, 0, sizeof(y));
     y.dt = in.dt; y.time = in.time; y.itimestep = in.itimestep;
-    y.dtmin = ~0ull;
   }
   auto up = [&](void *dst, const void *src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st_);
@@ -271,13 +276,27 @@ int Engine::upload(const std::vector<InstanceInput> &ins, bool keep_store) {
   if (gtot > 0) HIPCHK(up(d_.grid, hgrid.data(), (size_t)gtot * 8));
   HIPCHK(up(d_.meta, meta.data(), sizeof(InstMeta) * ninst));
   HIPCHK(up(d_.dyn, hdyn.data(), sizeof(InstDyn) * ninst));
-  HIPCHK(up(d_.blk, hblk.data(), sizeof(BlockEntry) * hblk.size()));
+  // Workgroups are dispatched to the 8 XCDs round-robin by block id; the
+  // physical table gives each XCD a contiguous run of logical blocks.
+  {
+    const int nb_ = (int)hblk.size(), q = nb_ / 8, r = nb_ % 8;
+    std::vector<BlockEntry> phys(nb_);
+    for (int b = 0; b < nb_; ++b) {
+      const int x = b % 8;
+      phys[b] = hblk[x * q + std::min(x, r) + b / 8];
+    }
+    HIPCHK(up(d_.blk, phys.data(), sizeof(BlockEntry) * nb_));
+  }
+  HIPCHK(hipMemsetAsync(d_.bflag, 0xff, sizeof(int) * ((size_t)ctot + 1), st_));
   HIPCHK(hipStreamSynchronize(st_));
   d_.ntot = ntot; d_.nbtot = nbtot; d_.ctot = ctot; d_.qtot = qtot; d_.ninst = ninst;
   d_.nblk = (int)hblk.size();
   meta_ = meta;
   if (gexec_) { hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
   if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
+  // grid of the first evaluation
+  launch_partial(d_, 0, st_);
+  launch_finalize(d_, 0, st_);
 
   // size the boundary-pair stores from a probe of the first evaluation
   if (any_boundary_) {
@@ -310,6 +329,8 @@ int Engine::upload(const std::vector<InstanceInput> &ins, bool keep_store) {
     }
     // the probe may have flagged the store capacity: restart clean
     HIPCHK(up(d_.dyn, hdyn.data(), sizeof(InstDyn) * ninst));
+    launch_partial(d_, 0, st_);
+    launch_finalize(d_, 0, st_);
     HIPCHK(hipStreamSynchronize(st_));
   }
   return 0;
@@ -333,13 +354,11 @@ void Engine::L(const char *name, F &&f) {
 
 void Engine::enqueue_prefix(int buf) {
   const DevState &d = d_;
-  L("eval_begin", [&] { launch_eval_begin(d, st_); });
-  L("reduce", [&] { launch_reduce(d, buf, st_); });
   L("count", [&] { launch_count(d, buf, st_); });
   L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
-  L("cellsort", [&] { launch_cellsort(d, st_); });
-  if (any_boundary_) L("bprep", [&] { launch_bprep(d, buf, st_); });
+  L("cellsort", [&] { launch_cellsort(d, buf, st_); });
+  if (any_boundary_) L("bprep", [&] { launch_bprep(d, st_); });
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
@@ -351,14 +370,15 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
     L("pairtot", [&] { launch_pairtot(d, st_); });
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_); });
   }
-  if (any_boundary_) L("materialize", [&] { launch_materialize(d, buf, st_); });
+  if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
   L("passB", [&] { launch_passB(d, precision_, phase, st_); });
 }
 
 void Engine::enqueue_step() {
   enqueue_eval(0, 0, false);
+  L("finalize", [&] { launch_finalize(d_, 0, st_); });
   enqueue_eval(1, 1, diag_);
-  L("step_end", [&] { launch_step_end(d_, st_); });
+  L("finalize", [&] { launch_finalize(d_, 1, st_); });
 }
 
 int Engine::probe_bnip(std::vector<int> &bnip) {

@@ -4,13 +4,15 @@
 // integration/single_step.f90:5-108) is a short pipeline of kernels over the
 // packed particles of every instance:
 //
-//   eval_begin   reset the per-instance reductions
-//   reduce       bounding box and MINVAL(hsml)            (nnps.f90:211-216)
+//   finalize     grid of the evaluation from per-block partial reductions of
+//                the bounding box and MINVAL(hsml) (nnps.f90:211-224); at
+//                the end of a step also the VP dt, time and step counter
+//                (time_step.f90:155, time_integration.f90:68-71)
 //   count        cell key NINT(x/bin) + cell histogram    (nnps.f90:246-254)
 //   scan x2      exclusive scan of the cell histogram
 //   scatter      cell lists
 //   cellsort     cell members in ascending particle id (the reference's bin
-//                order), inverse positions
+//                order), inverse positions, x/y/hsml copied in list order
 //   bprep        boundary-pair counts, ranks and the boundary cell index
 //                (nnps.f90:451-493)
 //   passA        neighbour gather: Wendland C6 gradient, velocity divergence
@@ -25,7 +27,9 @@
 //                103-136), boundary force (boundary_kernel.f90:477-502,
 //                boundary_force.f90:78-82), wind and water drag, tendencies
 //                (single_step.f90:104-106), and the predictor or corrector
-//                update (time_integration.f90:207-280)
+//                update (time_integration.f90:207-280); its blocks leave
+//                partial min/max of the new positions and smoothing lengths
+//                for the next finalize
 //
 // Every particle gathers its neighbours itself (no atomics on sums), visiting
 // the 3x3 cell block in the order in which the reference's single-thread
@@ -70,7 +74,10 @@ __device__ __forceinline__ T ipow(T x) {
 constexpr double kC6 = 78.0 / (7.0 * kPi);   // 78. / (7. * pi)
 constexpr float kC6f = (float)(78.0 / (7.0 * kPi));
 
-// particle handled by this thread
+// particle handled by this thread.  The block table is laid out so that
+// the blocks the hardware sends to one XCD (block ids congruent mod 8) cover
+// a contiguous run of particles, which keeps their neighbours in that XCD's
+// L2.
 __device__ __forceinline__ bool particle_of(const DevState &d, int &p, int &k) {
   BlockEntry e = d.blk[blockIdx.x];
   k = e.inst;
@@ -96,10 +103,11 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// Exclusive scan across the block (kTPB threads); returns the exclusive
+// Exclusive scan across the block (NT threads); returns the exclusive
 // prefix of v and the block total in *total.
+template <int NT = kTPB>
 __device__ int block_exclusive_scan(int v, int *total) {
-  __shared__ int wsum[kTPB / 64];
+  __shared__ int wsum[NT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int incl = v;
 #pragma unroll
@@ -111,7 +119,7 @@ __device__ int block_exclusive_scan(int v, int *total) {
   __syncthreads();
   int pre = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kTPB / 64; ++w) {
+  for (int w = 0; w < NT / 64; ++w) {
     int s = wsum[w];
     if (w < wid) pre += s;
     tot += s;
@@ -133,81 +141,120 @@ __device__ int block_sum(int v) {
   return tot;
 }
 
-// -------------------------------------------------------- eval_begin/reduce
+// ------------------------------------------------------ reductions / grid
 
-__global__ void k_eval_begin(DevState d) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.ninst) return;
-  InstDyn &y = d.dyn[k];
-  y.red_xmin = ~0ull;
-  y.red_ymin = ~0ull;
-  y.red_hmin = ~0ull;
-  y.red_xmax = 0ull;
-  y.red_ymax = 0ull;
+// Per-block min/max of x, y and min of hsml (invalid lanes give neutral
+// values), stored at the block's logical id.  Non-finite input raises the
+// instance's NaN flag.
+__device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok, double xv,
+                                               double yv, double hv) {
+  __shared__ double sh[5][kTPB / 64];
+  bool bad = ok && (!(xv == xv) || !(yv == yv) || !(hv > 0.0) || __builtin_isinf(xv) ||
+                    __builtin_isinf(yv) || __builtin_isinf(hv));
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.dyn[k].err, kErrNan);
+  const double big = __builtin_huge_val();
+  double v0 = wave_min(ok ? xv : big), v1 = wave_max(ok ? xv : -big);
+  double v2 = wave_min(ok ? yv : big), v3 = wave_max(ok ? yv : -big);
+  double v4 = wave_min(ok ? hv : big);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int f = threadIdx.x;
+    double r = sh[f][0];
+    for (int i = 1; i < kTPB / 64; ++i)
+      r = (f == 1 || f == 3) ? fmax(r, sh[f][i]) : fmin(r, sh[f][i]);
+    d.part[(size_t)f * d.nblk + d.blk[blockIdx.x].lid] = r;
+  }
 }
 
-__global__ void __launch_bounds__(kTPB) k_reduce(DevState d, int buf) {
+__global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   int p = 0, k = 0;
   bool ok = particle_of(d, p, k);
   double xv = ok ? d.x[buf][p] : 0.0, yv = ok ? d.y[buf][p] : 0.0;
-  double hv = ok ? d.hsml[buf][p] : 0.0;
-  bool bad = ok && (!(xv == xv) || !(yv == yv) || !(hv > 0.0) ||
-                    __builtin_isinf(xv) || __builtin_isinf(yv) ||
-                    __builtin_isinf(hv));
-  if (__any(bad)) {
-    if ((threadIdx.x & 63) == 0) atomicOr(&d.dyn[k].err, kErrNan);
-  }
+  double hv = ok ? d.hsml[buf][p] : 1.0;
+  block_partials(d, k, ok, xv, yv, hv);
+}
+
+// One block per instance: fold the instance's block partials into the grid
+// of the next evaluation (bucket_search, nnps.f90:211-224).  With
+// end_of_step it first closes the step: VP dt from MINVAL of the new hsml
+// (time_step.f90:155), itimestep + 1 and current_time + dt
+// (time_integration.f90:68-71).
+__global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) {
+  __shared__ double sh[5][kTPB / 64];
+  const int k = blockIdx.x;
+  const InstMeta &m = d.meta[k];
+  InstDyn &y = d.dyn[k];
   const double big = __builtin_huge_val();
-  double xmn = wave_min(ok ? xv : big), xmx = wave_max(ok ? xv : -big);
-  double ymn = wave_min(ok ? yv : big), ymx = wave_max(ok ? yv : -big);
-  double hmn = wave_min(ok ? hv : big);
-  if ((threadIdx.x & 63) == 0 && __any(ok)) {
-    InstDyn &y = d.dyn[k];
-    atomicMin(&y.red_xmin, ord_u64(xmn));
-    atomicMax(&y.red_xmax, ord_u64(xmx));
-    atomicMin(&y.red_ymin, ord_u64(ymn));
-    atomicMax(&y.red_ymax, ord_u64(ymx));
-    atomicMin(&y.red_hmin, ord_u64(hmn));
+  double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big;
+  for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
+    v0 = fmin(v0, d.part[b]);
+    v1 = fmax(v1, d.part[(size_t)d.nblk + b]);
+    v2 = fmin(v2, d.part[2 * (size_t)d.nblk + b]);
+    v3 = fmax(v3, d.part[3 * (size_t)d.nblk + b]);
+    v4 = fmin(v4, d.part[4 * (size_t)d.nblk + b]);
   }
+  v0 = wave_min(v0); v1 = wave_max(v1); v2 = wave_min(v2); v3 = wave_max(v3);
+  v4 = wave_min(v4);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 1; i < kTPB / 64; ++i) {
+    v0 = fmin(sh[0][0], sh[0][i]); sh[0][0] = v0;
+    v1 = fmax(sh[1][0], sh[1][i]); sh[1][0] = v1;
+    v2 = fmin(sh[2][0], sh[2][i]); sh[2][0] = v2;
+    v3 = fmax(sh[3][0], sh[3][i]); sh[3][0] = v3;
+    v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
+  }
+  double xmn = sh[0][0], xmx = sh[1][0], ymn = sh[2][0], ymx = sh[3][0], hmn = sh[4][0];
+  if (y.err) return;
+  if (end_of_step) {
+    // VP_time_step: MINVAL(rho_ice hsml^2 delta_min / (P_star (1 + k_t))) is
+    // the same expression at MINVAL(hsml), each operation being monotone
+    if (m.tse == 2) y.dt = kRhoIce * (hmn * hmn) * kDeltaMin / (kPStar * (1 + kKT));
+    y.itimestep += 1;
+    y.time = y.time + y.dt;
+  }
+  if (m.nb > 0) {
+    xmn = fmin(xmn, m.bxmin); xmx = fmax(xmx, m.bxmax);
+    ymn = fmin(ymn, m.bymin); ymx = fmax(ymx, m.bymax);
+  }
+  const double bin = 2 * hmn;
+  const int xlo = nint(xmn / bin), xhi = nint(xmx / bin);
+  const int ylo = nint(ymn / bin), yhi = nint(ymx / bin);
+  const long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
+  if (!(hmn > 0.0) || W * H > (long long)m.c_cap || W * H <= 0) {
+    y.err |= kErrCapacity;
+    return;
+  }
+  y.hmin = hmn;
+  y.bin = bin;
+  y.xlo = xlo;
+  y.ylo = ylo;
+  y.W = (int)W;
+  y.H = (int)H;
+  y.stamp += 1;
 }
 
 // ------------------------------------------------------------------ count
 
 __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   int p = 0, k = 0;
-  bool ok = particle_of(d, p, k);
-  InstDyn &y = d.dyn[k];
+  if (!particle_of(d, p, k)) return;
+  const InstDyn &y = d.dyn[k];
   if (y.err) return;
-  const InstMeta &m = d.meta[k];
-  // bucket_search grid (nnps.f90:211-216): bin = 2 MINVAL(hsml), the cell
-  // range spans particles and boundary particles
-  double bin = 2 * ord_f64(y.red_hmin);
-  double xmn = ord_f64(y.red_xmin), xmx = ord_f64(y.red_xmax);
-  double ymn = ord_f64(y.red_ymin), ymx = ord_f64(y.red_ymax);
-  if (m.nb > 0) {
-    xmn = fmin(xmn, m.bxmin); xmx = fmax(xmx, m.bxmax);
-    ymn = fmin(ymn, m.bymin); ymx = fmax(ymx, m.bymax);
-  }
-  int xlo = nint(xmn / bin), xhi = nint(xmx / bin);
-  int ylo = nint(ymn / bin), yhi = nint(ymx / bin);
-  long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
-  if (W * H > (long long)m.c_cap || W * H <= 0) {
-    if (threadIdx.x == 0) atomicOr(&y.err, kErrCapacity);
-    return;
-  }
-  if (threadIdx.x == 0 && d.blk[blockIdx.x].first == 0) {
-    y.bin = bin;
-    y.xlo = xlo;
-    y.ylo = ylo;
-    y.W = (int)W;
-    y.H = (int)H;
-  }
-  if (!ok) return;
-  int cx = nint(d.x[buf][p] / bin) - xlo + 1;
-  int cy = nint(d.y[buf][p] / bin) - ylo + 1;
-  int c = cy * (int)W + cx;
+  const double bin = y.bin;
+  const int cx = nint(d.x[buf][p] / bin) - y.xlo + 1;
+  const int cy = nint(d.y[buf][p] / bin) - y.ylo + 1;
+  const int c = cy * y.W + cx;
   d.cellid[p] = c;
-  d.slot[p] = atomicAdd(&d.count[m.c_off + c], 1);
+  d.slot[p] = atomicAdd(&d.count[d.meta[k].c_off + c], 1);
 }
 
 // ------------------------------------------------------------------- scan
@@ -265,8 +312,9 @@ __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
 }
 
 // members of each cell in ascending particle id: the order in which the
-// reference fills its bins (nnps.f90:246-254)
-__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
+// reference fills its bins (nnps.f90:246-254); x, y, hsml copied in that
+// order so that the neighbour loops read candidates contiguously
+__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   int c = blockIdx.x * kTPB + threadIdx.x;
   if (c >= d.ctot) return;
   int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
@@ -285,7 +333,14 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
     }
     d.cl[b + 1] = v;
   }
-  for (int s = s0; s < s1; ++s) d.clpos[d.cl[s]] = s;
+  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
+  for (int s = s0; s < s1; ++s) {
+    const int q = d.cl[s];
+    d.clpos[q] = s;
+    d.xs[s] = X[q];
+    d.ys[s] = Y[q];
+    d.hs[s] = HS[q];
+  }
 }
 
 // -------------------------------------------------------------- boundary
@@ -293,63 +348,73 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
 // One block per instance: per boundary particle, the number of boundary
 // pairs found in each of its 3x3 cells in the reference's scan order
 // (y outer, x inner; nnps.f90:458-491), the exclusive prefix over boundary
-// particles (= rank of its first pair), and the boundary particles sorted
-// by cell for the particle-side lookup.
-__global__ void __launch_bounds__(kTPB) k_bprep(DevState d, int buf) {
+// particles (= rank of its first pair), the boundary particles sorted by
+// cell for the particle-side lookup, and the stamp of every cell a boundary
+// particle's 3x3 block covers.
+constexpr int kBprepThreads = 1024;
+
+__global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
   __shared__ unsigned long long keys[kMaxBoundaryPerInst];
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
-  InstDyn &y = d.dyn[k];
+  const InstDyn &y = d.dyn[k];
   if (m.nb == 0 || y.err) return;
   const double bin = y.bin;
-  const int W = y.W, xlo = y.xlo, ylo = y.ylo;
-  const double *X = d.x[buf], *Y = d.y[buf];
+  const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
   const int *cs = d.cell_start + m.c_off;
-  for (int b = threadIdx.x; b < m.nb; b += kTPB) {
-    double bx = d.xb[m.b_off + b], by = d.yb[m.b_off + b];
-    int cxb = nint(bx / bin) - xlo + 1, cyb = nint(by / bin) - ylo + 1;
+  // counts per (boundary particle, cell of its 3x3 block)
+  for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
+    const int b = idx / 9, s9 = idx - 9 * (idx / 9);
+    const int bg = m.b_off + b;
+    const double bx = d.xb[bg], by = d.yb[bg];
+    const int cxb = nint(bx / bin) - xlo + 1, cyb = nint(by / bin) - ylo + 1;
+    const int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
     int cnt = 0;
-    for (int s9 = 0; s9 < 9; ++s9) {
-      int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
-      d.bpre[(m.b_off + b) * 9 + s9] = cnt;
-      for (int s = cs[c]; s < cs[c + 1]; ++s) {
-        int q = d.cl[s];
-        double dx = bx - X[q], dy = by - Y[q];
-        double dd = sqrt(dx * dx + dy * dy);
-        if (dd <= kBLength) ++cnt;
-      }
+    for (int s = cs[c]; s < cs[c + 1]; ++s) {
+      const double dx = bx - d.xs[s], dy = by - d.ys[s];
+      if (sqrt(dx * dx + dy * dy) <= kBLength) ++cnt;
     }
-    d.bcount[m.b_off + b] = cnt;
-    keys[b] = ((unsigned long long)(unsigned)(cyb * W + cxb) << 32) | (unsigned)b;
+    d.bpre[(size_t)bg * 9 + s9] = cnt;
+    d.bflag[m.c_off + c] = stamp;
+    if (s9 == 4) keys[b] = ((unsigned long long)(unsigned)(cyb * W + cxb) << 32) | (unsigned)b;
   }
   int P = 1;
   while (P < m.nb) P <<= 1;
-  for (int i = m.nb + threadIdx.x; i < P; i += kTPB) keys[i] = ~0ull;
+  for (int i = m.nb + threadIdx.x; i < P; i += kBprepThreads) keys[i] = ~0ull;
   __syncthreads();
-  // exclusive scan of bcount over the instance's boundary particles
-  const int per = (m.nb + kTPB - 1) / kTPB;
+  // prefix over the 9 cells of each boundary particle, then over particles
+  const int per = (m.nb + kBprepThreads - 1) / kBprepThreads;
   const int b0 = threadIdx.x * per;
-  int s = 0;
-  for (int b = b0; b < b0 + per && b < m.nb; ++b) s += d.bcount[m.b_off + b];
-  int tot = 0;
-  int ex = block_exclusive_scan(s, &tot);
+  int tsum = 0;
   for (int b = b0; b < b0 + per && b < m.nb; ++b) {
-    int c = d.bcount[m.b_off + b];
+    const size_t bg = (size_t)(m.b_off + b);
+    int run = 0;
+    for (int s9 = 0; s9 < 9; ++s9) {
+      const int c9 = d.bpre[bg * 9 + s9];
+      d.bpre[bg * 9 + s9] = run;
+      run += c9;
+    }
+    d.bcount[bg] = run;
+    tsum += run;
+  }
+  int tot = 0;
+  int ex = block_exclusive_scan<kBprepThreads>(tsum, &tot);
+  for (int b = b0; b < b0 + per && b < m.nb; ++b) {
     d.bbase[m.b_off + b] = ex;
-    ex += c;
+    ex += d.bcount[m.b_off + b];
   }
   if (threadIdx.x == 0) {
-    y.bnip = tot;
-    if (tot > m.q_cap) atomicOr(&y.err, kErrCapacity);
+    d.dyn[k].bnip = tot;
+    if (tot > m.q_cap) atomicOr(&d.dyn[k].err, kErrCapacity);
   }
   // bitonic sort of (cell, b)
   for (int k2 = 2; k2 <= P; k2 <<= 1) {
     for (int j = k2 >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += kTPB) {
-        int ixj = i ^ j;
+      for (int i = threadIdx.x; i < P; i += kBprepThreads) {
+        const int ixj = i ^ j;
         if (ixj > i) {
-          unsigned long long a = keys[i], b = keys[ixj];
-          bool up = (i & k2) == 0;
+          const unsigned long long a = keys[i], b = keys[ixj];
+          const bool up = (i & k2) == 0;
           if ((a > b) == up) {
             keys[i] = b;
             keys[ixj] = a;
@@ -359,7 +424,7 @@ __global__ void __launch_bounds__(kTPB) k_bprep(DevState d, int buf) {
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < m.nb; i += kTPB) d.bsorted[m.b_off + i] = keys[i];
+  for (int i = threadIdx.x; i < m.nb; i += kBprepThreads) d.bsorted[m.b_off + i] = keys[i];
 }
 
 // --------------------------------------------------------------- neighbours
@@ -458,12 +523,12 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   int nI = 0, nX = 0, nn = 0;
   for (int g = 0; g < 6; ++g) {
     for (int s = R.a[g]; s < R.b[g]; ++s) {
-      const int q = d.cl[s];
-      if (q == p) continue;
-      const double dx = xi - X[q], dy = yi - Y[q];
-      const double mh = (hi + HS[q]) / 2.;
+      if (s == self) continue;
+      const double dx = xi - d.xs[s], dy = yi - d.ys[s];
+      const double mh = (hi + d.hs[s]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      const int q = d.cl[s];
       const int kind = range_kind(g, s, self);
       nI += kind == 1;
       nX += kind == 2;
@@ -565,40 +630,49 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
 // hsml_ij of the particle pairs ranked below min(bnip, nip): the entries
 // b_Wendland_C6 reads at the boundary-pair index (boundary_kernel.f90:491).
 // Entries at or above nip keep what an earlier evaluation wrote, as the
-// reference's module array does.
-__global__ void __launch_bounds__(kTPB) k_materialize(DevState d, int buf) {
-  int p = 0, k = 0;
-  if (!particle_of(d, p, k)) return;
+// reference's module array does.  Ranks grow along the cell lists, so the
+// threads of an instance walk the list from its start and stop at the first
+// member whose cell ranks at or beyond the limit.
+constexpr int kMatBlocks = 4;
+
+__global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
+  const int k = blockIdx.x / kMatBlocks;
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || y.err) return;
   const int pbase = d.pair_start[m.c_off];
   const int nipk = d.pair_start[m.c_off + m.c_cap] - pbase;
   const int lim = min(min(y.bnip, m.q_cap), nipk);
-  const int oc = d.own_cnt[p];
-  const int nI = oc & 1023, nX = (oc >> 10) & 1023;
-  if (nI + nX == 0) return;
-  const int c = d.cellid[p];
-  const int s = d.clpos[p];
-  const int base = d.pair_start[m.c_off + c] - pbase;
-  int rI = base + (d.moff[s] & 0xffff);
-  int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
-  if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
-  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
-  const int W = y.W, cy = c / W, cx = c - cy * W;
-  const double xi = X[p], yi = Y[p], hi = HS[p];
-  const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
-  for (int g = 2; g < 6; ++g) {
-    for (int t = R.a[g]; t < R.b[g]; ++t) {
-      if (g == 2 && t <= s) continue;
-      const int q = d.cl[t];
-      const double dx = xi - X[q], dy = yi - Y[q];
-      const double mh = (hi + HS[q]) / 2.;
-      double r;
-      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-      int &rank = (g == 2) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = mh;
-      ++rank;
+  const int W = y.W;
+  const int *cs = d.cell_start + m.c_off;
+  const int stride = kMatBlocks * kTPB;
+  const int s_end = cs[m.c_cap];
+  for (int s = cs[0] + (blockIdx.x % kMatBlocks) * kTPB + threadIdx.x; s < s_end;
+       s += stride) {
+    const int p = d.cl[s];
+    const int c = d.cellid[p];
+    const int base = d.pair_start[m.c_off + c] - pbase;
+    if (base >= lim) break;
+    const int oc = d.own_cnt[p];
+    const int nI = oc & 1023, nX = (oc >> 10) & 1023;
+    if (nI + nX == 0) continue;
+    int rI = base + (d.moff[s] & 0xffff);
+    int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
+    if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) continue;
+    const int cy = c / W, cx = c - cy * W;
+    const double xi = d.xs[s], yi = d.ys[s], hi = d.hs[s];
+    const Ranges R = neighbour_ranges(cs, W, cx, cy);
+    for (int g = 2; g < 6; ++g) {
+      for (int t = R.a[g]; t < R.b[g]; ++t) {
+        if (g == 2 && t <= s) continue;
+        const double dx = xi - d.xs[t], dy = yi - d.ys[t];
+        const double mh = (hi + d.hs[t]) / 2.;
+        double r;
+        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        int &rank = (g == 2) ? rI : rX;
+        if (rank < lim) d.hq[m.q_off + rank] = mh;
+        ++rank;
+      }
     }
   }
 }
@@ -639,7 +713,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
   const InstMeta &m = d.meta[k];
   const bool live = ok && !y.err;
   const int buf = phase == 1 ? 1 : 0;
-  double hsml_new = __builtin_huge_val();
+  double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   if (live) {
     const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RHO = d.rho[buf];
     const int W = y.W;
@@ -660,12 +734,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       const double ri2 = rhoi * rhoi;
       for (int g = 0; g < 6; ++g) {
         for (int s = R.a[g]; s < R.b[g]; ++s) {
-          const int q = d.cl[s];
-          if (q == p) continue;
-          const double dx = xi - X[q], dy = yi - Y[q];
-          const double mh = (hi + HS[q]) / 2.;
+          if (s == self) continue;
+          const double dx = xi - d.xs[s], dy = yi - d.ys[s];
+          const double mh = (hi + d.hs[s]) / 2.;
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          const int q = d.cl[s];
           const double gg = c6_grad(r, mh);
           const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
           const double rq = RHO[q], mq = d.mass[q];
@@ -688,12 +762,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       KSum sx, sy;
       for (int g = 0; g < 6; ++g) {
         for (int s = R.a[g]; s < R.b[g]; ++s) {
-          const int q = d.cl[s];
-          if (q == p) continue;
-          const double dx = xi - X[q], dy = yi - Y[q];
-          const double mh = (hi + HS[q]) / 2.;
+          if (s == self) continue;
+          const double dx = xi - d.xs[s], dy = yi - d.ys[s];
+          const double mh = (hi + d.hs[s]) / 2.;
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          const int q = d.cl[s];
           const float rf = (float)r;
           const float gg = c6_grad_f(rf, (float)mh);
           const float ir = __builtin_amdgcn_rcpf(rf);
@@ -714,7 +788,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
 
     // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
     // ascending boundary index (boundary_kernel.f90:477-502)
-    if (m.nb > 0) {
+    if (m.nb > 0 && d.bflag[m.c_off + c] == y.stamp) {
       // Boundary neighbours are taken in ascending boundary index by a
       // selection over the 3x3 cell block (a particle sees a handful).
       int nbn = 0;
@@ -755,8 +829,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
         const int slot9 = (cy - cyb + 1) * 3 + (cx - cxb + 1);
         int before = 0;
         for (int s = R.a[2]; s < self; ++s) {
-          const int q = d.cl[s];
-          const double ex = d.xb[bg] - X[q], ey = d.yb[bg] - Y[q];
+          const double ex = d.xb[bg] - d.xs[s], ey = d.yb[bg] - d.ys[s];
           if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
         }
         const int rank = d.bbase[bg] + d.bpre[bg * 9 + slot9] + before;
@@ -802,8 +875,11 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
     const double dt = y.dt;
     if (phase == 0) {
       // predict half step (time_integration.f90:207-218)
-      d.x[1][p] = xi + vxi * dt / 2;
-      d.y[1][p] = yi + vyi * dt / 2;
+      px_new = xi + vxi * dt / 2;
+      py_new = yi + vyi * dt / 2;
+      hsml_new = hi;
+      d.x[1][p] = px_new;
+      d.y[1][p] = py_new;
       d.vx[1][p] = vxi + ax * dt / 2;
       d.vy[1][p] = vyi + ay * dt / 2;
       const double hh = hthi + dhdt * dt / 2;
@@ -821,8 +897,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
       const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
       const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
-      d.x[0][p] = 2 * x12 - x1;
-      d.y[0][p] = 2 * y12 - y1;
+      px_new = 2 * x12 - x1;
+      py_new = 2 * y12 - y1;
+      d.x[0][p] = px_new;
+      d.y[0][p] = py_new;
       d.vx[0][p] = 2 * u12 - u1;
       d.vy[0][p] = 2 * v12 - v1;
       const double hn = 2 * h12 - h1;
@@ -847,25 +925,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       d.tend[5 * N + p] = dAdt;
     }
   }
-  if (phase == 1 && m.tse == 2) {
-    // VP_time_step (time_step.f90:155): per-instance MINVAL
-    double v = live ? kRhoIce * (hsml_new * hsml_new) * kDeltaMin / (kPStar * (1 + kKT))
-                    : __builtin_huge_val();
-    v = wave_min(v);
-    if ((threadIdx.x & 63) == 0 && __any(live)) atomicMin(&d.dyn[k].dtmin, ord_u64(v));
-  }
-}
-
-// End of a predictor-corrector step (time_integration.f90:68-71).
-__global__ void k_step_end(DevState d) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.ninst) return;
-  InstDyn &y = d.dyn[k];
-  if (y.err) return;
-  if (d.meta[k].tse == 2) y.dt = ord_f64(y.dtmin);
-  y.dtmin = ~0ull;
-  y.itimestep += 1;
-  y.time = y.time + y.dt;
+  if (phase != 2) block_partials(d, k, live, px_new, py_new, hsml_new);
 }
 
 // ------------------------------------------------------- pair-list export
@@ -887,18 +947,17 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   const int base = d.pair_start[m.c_off + c] - pbase;
   int rI = base + (d.moff[s] & 0xffff);
   int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
-  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
   const int W = y.W, cy = c / W, cx = c - cy * W;
-  const double xi = X[p], yi = Y[p], hi = HS[p];
+  const double xi = d.xs[s], yi = d.ys[s], hi = d.hs[s];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int t = R.a[g]; t < R.b[g]; ++t) {
       if (g == 2 && t <= s) continue;
-      const int q = d.cl[t];
-      const double dx = xi - X[q], dy = yi - Y[q];
-      const double mh = (hi + HS[q]) / 2.;
+      const double dx = xi - d.xs[t], dy = yi - d.ys[t];
+      const double mh = (hi + d.hs[t]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      const int q = d.cl[t];
       int &rank = (g == 2) ? rI : rX;
       if (rank < cap) {
         pi[rank] = d.orig[p] + 1;
@@ -931,7 +990,7 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
     int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
     for (int s = cs[c]; s < cs[c + 1]; ++s) {
       int q = d.cl[s];
-      double dx = bx - d.x[buf][q], dy = by - d.y[buf][q];
+      double dx = bx - d.xs[s], dy = by - d.ys[s];
       double dd = sqrt(dx * dx + dy * dy);
       if (dd <= kBLength) {
         if (rank < bcap) {
@@ -949,11 +1008,11 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-void launch_eval_begin(const DevState &d, hipStream_t st) {
-  k_eval_begin<<<cdiv(d.ninst, 64), 64, 0, st>>>(d);
+void launch_partial(const DevState &d, int buf, hipStream_t st) {
+  k_partial<<<d.nblk, kTPB, 0, st>>>(d, buf);
 }
-void launch_reduce(const DevState &d, int buf, hipStream_t st) {
-  k_reduce<<<d.nblk, kTPB, 0, st>>>(d, buf);
+void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
+  k_finalize<<<d.ninst, kTPB, 0, st>>>(d, end_of_step);
 }
 void launch_count(const DevState &d, int buf, hipStream_t st) {
   k_count<<<d.nblk, kTPB, 0, st>>>(d, buf);
@@ -967,11 +1026,11 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st
 void launch_scatter(const DevState &d, hipStream_t st) {
   k_scatter<<<d.nblk, kTPB, 0, st>>>(d);
 }
-void launch_cellsort(const DevState &d, hipStream_t st) {
-  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
+void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
+  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d, buf);
 }
-void launch_bprep(const DevState &d, int buf, hipStream_t st) {
-  k_bprep<<<d.ninst, kTPB, 0, st>>>(d, buf);
+void launch_bprep(const DevState &d, hipStream_t st) {
+  k_bprep<<<d.ninst, kBprepThreads, 0, st>>>(d);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int want_diag, hipStream_t st) {
   if (mixed)
@@ -982,17 +1041,14 @@ void launch_passA(const DevState &d, int buf, int mixed, int want_diag, hipStrea
 void launch_pairtot(const DevState &d, hipStream_t st) {
   k_pairtot<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
 }
-void launch_materialize(const DevState &d, int buf, hipStream_t st) {
-  k_materialize<<<d.nblk, kTPB, 0, st>>>(d, buf);
+void launch_materialize(const DevState &d, hipStream_t st) {
+  k_materialize<<<d.ninst * kMatBlocks, kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int phase, hipStream_t st) {
   if (mixed)
     k_passB<true><<<d.nblk, kTPB, 0, st>>>(d, phase);
   else
     k_passB<false><<<d.nblk, kTPB, 0, st>>>(d, phase);
-}
-void launch_step_end(const DevState &d, hipStream_t st) {
-  k_step_end<<<cdiv(d.ninst, 64), 64, 0, st>>>(d);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, hipStream_t st) {

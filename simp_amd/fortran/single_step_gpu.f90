@@ -1,0 +1,52 @@
+!---------------------------------------------------------------------------
+! single_step_gpu.f90 -- replacement of SIMP's single_step
+! (integration/single_step.f90:5-108) that runs the right-hand side on the
+! MI355X through the C ABI.  Same name, no dummy arguments, same module
+! state read and written, so time_integration (and every other caller)
+! links against it unchanged.
+!
+! Reads  ntotal, x, vx, mass, rho, hsml, h, A, ntotalb, xb, massb,
+!        current_time, itimestep, max_hsml
+! Writes dxdt, dvxdt, dhdt, dAdt, epsilon_11/12/22, zeta, eta, p,
+!        sigma_11/12/22, nip, bnip
+! The pair lists themselves (p_i, p_j, r_ij, hsml_ij, w, dwdx) stay on the
+! device; only the inactive options that read them on the host (sum_density,
+! Benz_hsml, Monaghan_time_step) would need them (simp_nnps exports them).
+!---------------------------------------------------------------------------
+subroutine single_step
+    use globals
+    use data_variables
+    use simp_binding
+    implicit none
+
+    type(c_ptr), save :: handle = c_null_ptr
+    double precision, allocatable, save :: eps3(:, :), sig3(:, :)
+    type(simp_config) :: cfg
+    integer(c_int) :: st, nipc, bnipc
+
+    if (.not. c_associated(handle)) then
+        call simp_config_default(cfg)
+        cfg%max_hsml = max_hsml
+        st = simp_create(cfg, handle)
+        call simp_check(st, 'simp_create')
+    endif
+    if (allocated(eps3)) then
+        if (size(eps3, 2) /= ntotal) deallocate(eps3, sig3)
+    endif
+    if (.not. allocated(eps3)) allocate(eps3(3, ntotal), sig3(3, ntotal))
+
+    st = simp_single_step(handle, ntotal, x, vx, mass, rho, hsml, h, A,    &
+                          ntotalb, xb, massb, current_time, itimestep,     &
+                          dxdt, dvxdt, dhdt, dAdt, eps3, zeta, eta, p,     &
+                          sig3, nipc, bnipc)
+    call simp_check(st, 'single_step')
+
+    epsilon_11(:ntotal) = eps3(1, :)
+    epsilon_12(:ntotal) = eps3(2, :)
+    epsilon_22(:ntotal) = eps3(3, :)
+    sigma_11(:ntotal) = sig3(1, :)
+    sigma_12(:ntotal) = sig3(2, :)
+    sigma_22(:ntotal) = sig3(3, :)
+    nip = nipc
+    bnip = bnipc
+end subroutine single_step
