@@ -160,7 +160,7 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
   rc = h->eng->upload({in}, true);
   if (rc) return map_rc(rc);
   h->uploaded = false;  // the device state now holds this call's inputs
-  rc = h->eng->eval_single(eps3 || zeta || eta || p);
+  rc = h->eng->eval_single(eps3 || zeta || eta || p || sig3);
   if (rc) return map_rc(rc);
   rc = h->eng->download_tend(0, dxdt, dvxdt, dhdt, dAdt);
   if (rc) return map_rc(rc);
@@ -256,7 +256,7 @@ int simp_download_diagnostics(simp_handle *h, double *eps3, double *zeta, double
                               double *p, double *sig3) {
   if (!h) return SIMP_E_ARG;
   if (!h->uploaded) return SIMP_E_STATE;
-  if (!h->eng->diag() && (eps3 || zeta || eta || p)) return SIMP_E_STATE;
+  if (!h->eng->diag() && (eps3 || zeta || eta || p || sig3)) return SIMP_E_STATE;
   return map_rc(h->eng->download_diag(0, eps3, zeta, eta, p, sig3));
 }
 

@@ -5,6 +5,7 @@
 // range, described by InstMeta (fixed between uploads) and InstDyn (changes
 // every evaluation).
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace simp {
@@ -31,6 +32,7 @@ constexpr int kPrintStep = 2000;    // globals.f90:55
 constexpr int kTPB = 256;           // threads per block of particle kernels
 constexpr int kScanItems = 2048;    // elements per block of the scan
 constexpr int kMaxBoundaryPerInst = 8192;
+constexpr int kNbrMax = 32;         // neighbours kept per particle in the list
 
 // Error bits (OR-ed into InstDyn::err); translated to SIMP_E_* on the host.
 constexpr int kErrCapacity = 1;
@@ -59,7 +61,7 @@ struct InstDyn {
   int xlo, ylo, W, H;
   int nip, bnip;
   int stamp;            // evaluation counter (boundary cell flags)
-  int pad;
+  float pf_slack;       // bound of the prefilter's FP32 distance error (m)
   double hmin;          // MINVAL(hsml) of the current evaluation
 };
 
@@ -79,13 +81,24 @@ struct DevState {
   int *inst;            // instance of each packed particle
   // per evaluation
   int *cellid, *slot, *cl, *clpos, *count, *cell_start, *scan_tmp;
-  double *xs, *ys, *hs;  // x, y, hsml in cell-list order (candidate loads)
+  // the right-hand-side state in cell-list order (copied by cellsort): the
+  // gathers run one thread per list position and read neighbours here
+  double *xs, *ys, *hs, *vxs, *vys, *ms, *rs;
+  int *cells;            // cell (local id) of each list position
+  float4 *pf;            // FP32 (x, y, hsml) in list order, x and y relative to
+                         // the grid origin: the candidate prefilter
+  // neighbour list of the evaluation, built by pass A in the reference's
+  // pair order for each particle: nbr[j * ntot + s] = list position of the
+  // j-th neighbour of position s; nnb[s] = count (> kNbrMax: not stored,
+  // the gathers rescan the 3x3 block)
+  int *nbr, *nnb;
   double *part;          // per-block partial reductions (5 x nblk)
   int *bflag;            // per cell: stamp of the last evaluation in which a
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
   double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N), optional
+  double *sig_out;       // sigma_11, sigma_12, sigma_22 (3 x N), with diag
   double *tend;          // dxdt, dydt, dvxdt, dvydt, dhdt, dAdt (6 x N), optional
   // boundary
   double *xb, *yb, *massb;

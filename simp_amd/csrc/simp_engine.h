@@ -75,7 +75,7 @@ class Engine {
 
  private:
   void enqueue_eval(int buf, int phase, bool want_diag);
-  void enqueue_prefix(int buf);
+  void enqueue_prefix(int buf, bool fork_boundary);
   void enqueue_step();
   int ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk, long gtot);
   void free_all();
@@ -85,6 +85,8 @@ class Engine {
 
   int precision_;
   hipStream_t st_ = nullptr;
+  hipStream_t side_ = nullptr;        // boundary prep, concurrent with pass A
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   DevState d_{};
   std::vector<InstMeta> meta_;
   bool diag_ = false;

@@ -38,12 +38,21 @@ static long grid_cells(int n, const double *x, const double *hsml, int nb, const
   return W * H;
 }
 
-Engine::Engine(int precision) : precision_(precision) { hipStreamCreate(&st_); }
+Engine::Engine(int precision) : precision_(precision) {
+  hipStreamCreate(&st_);
+  hipStreamCreate(&side_);
+  hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming);
+  hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming);
+}
 
 Engine::~Engine() {
   if (gexec_) hipGraphExecDestroy(gexec_);
   if (graph_) hipGraphDestroy(graph_);
   free_all();
+  if (d_.hq) hipFree(d_.hq);
+  if (ev_fork_) hipEventDestroy(ev_fork_);
+  if (ev_join_) hipEventDestroy(ev_join_);
+  if (side_) hipStreamDestroy(side_);
   if (st_) hipStreamDestroy(st_);
 }
 
@@ -102,6 +111,14 @@ int Engine::ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk,
     rc |= dalloc(allocs_, &d_.xs, N);
     rc |= dalloc(allocs_, &d_.ys, N);
     rc |= dalloc(allocs_, &d_.hs, N);
+    rc |= dalloc(allocs_, &d_.vxs, N);
+    rc |= dalloc(allocs_, &d_.vys, N);
+    rc |= dalloc(allocs_, &d_.ms, N);
+    rc |= dalloc(allocs_, &d_.rs, N);
+    rc |= dalloc(allocs_, &d_.cells, N);
+    rc |= dalloc(allocs_, &d_.pf, N);
+    rc |= dalloc(allocs_, &d_.nbr, (size_t)kNbrMax * N);
+    rc |= dalloc(allocs_, &d_.nnb, N);
     rc |= dalloc(allocs_, &d_.part, 5 * (size_t)cap_blk_);
     rc |= dalloc(allocs_, &d_.bflag, C);
     rc |= dalloc(allocs_, &d_.count, C);
@@ -115,6 +132,7 @@ int Engine::ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk,
     rc |= dalloc(allocs_, &sg, N); d_.sig22 = sg;
     rc |= dalloc(allocs_, &sg, N); d_.div = sg;
     rc |= dalloc(allocs_, &d_.diag, 6 * N);
+    rc |= dalloc(allocs_, &d_.sig_out, 3 * N);
     rc |= dalloc(allocs_, &d_.tend, 6 * N);
     rc |= dalloc(allocs_, &d_.xb, NB);
     rc |= dalloc(allocs_, &d_.yb, NB);
@@ -352,24 +370,36 @@ void Engine::L(const char *name, F &&f) {
   prof->recs.push_back(r);
 }
 
-void Engine::enqueue_prefix(int buf) {
+void Engine::enqueue_prefix(int buf, bool fork_boundary) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
   L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
-  if (any_boundary_) L("bprep", [&] { launch_bprep(d, st_); });
+  if (!any_boundary_) return;
+  if (fork_boundary && !prof) {
+    // the boundary prep only feeds materialize and pass B: it runs on a
+    // second stream (a parallel branch of the captured graph) beside pass A
+    hipEventRecord(ev_fork_, st_);
+    hipStreamWaitEvent(side_, ev_fork_, 0);
+    launch_bprep(d, side_);
+    hipEventRecord(ev_join_, side_);
+  } else {
+    L("bprep", [&] { launch_bprep(d, st_); });
+  }
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const DevState &d = d_;
-  enqueue_prefix(buf);
+  const bool fork = any_boundary_ && !prof;
+  enqueue_prefix(buf, fork);
   L("passA", [&] { launch_passA(d, buf, precision_, want_diag ? 1 : 0, st_); });
   if (any_boundary_ || phase == 2) {
     // pair ranks: needed by the boundary kernel's hsml_ij read, and for nip
     L("pairtot", [&] { launch_pairtot(d, st_); });
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_); });
   }
+  if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
   if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
   L("passB", [&] { launch_passB(d, precision_, phase, st_); });
 }
@@ -382,7 +412,7 @@ void Engine::enqueue_step() {
 }
 
 int Engine::probe_bnip(std::vector<int> &bnip) {
-  enqueue_prefix(0);
+  enqueue_prefix(0, false);
   HIPCHK(hipStreamSynchronize(st_));
   std::vector<InstDyn> hd(d_.ninst);
   HIPCHK(hipMemcpy(hd.data(), d_.dyn, sizeof(InstDyn) * d_.ninst, hipMemcpyDeviceToHost));
@@ -534,17 +564,10 @@ int Engine::download_diag(int k, double *eps3, double *zeta, double *eta, double
   if (eta) HIPCHK(get(eta, 4));
   if (p) HIPCHK(get(p, 5));
   if (sig3) {
-    if (precision_ == 0) {
-      HIPCHK(hipMemcpy(c0.data(), (double *)d_.sig11 + m.p_off, n * 8, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(c1.data(), (double *)d_.sig12 + m.p_off, n * 8, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(c2.data(), (double *)d_.sig22 + m.p_off, n * 8, hipMemcpyDeviceToHost));
-    } else {
-      std::vector<float> f0(n), f1(n), f2(n);
-      HIPCHK(hipMemcpy(f0.data(), (float *)d_.sig11 + m.p_off, n * 4, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(f1.data(), (float *)d_.sig12 + m.p_off, n * 4, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(f2.data(), (float *)d_.sig22 + m.p_off, n * 4, hipMemcpyDeviceToHost));
-      for (int i = 0; i < n; ++i) { c0[i] = f0[i]; c1[i] = f1[i]; c2[i] = f2[i]; }
-    }
+    HIPCHK(hipMemcpy(c0.data(), d_.sig_out + m.p_off, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c1.data(), d_.sig_out + (size_t)N + m.p_off, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c2.data(), d_.sig_out + 2 * (size_t)N + m.p_off, n * 8,
+                     hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) {
       sig3[3 * i] = c0[i]; sig3[3 * i + 1] = c1[i]; sig3[3 * i + 2] = c2[i];
     }
@@ -556,10 +579,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
                          int bcap, int *bnip, int *bpi, int *bpj, double *brij) {
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
-  enqueue_prefix(0);
-  if (!any_boundary_) {
-    // bprep is skipped without boundaries; nothing to rank there
-  }
+  enqueue_prefix(0, false);
   launch_passA(d, 0, precision_, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_);

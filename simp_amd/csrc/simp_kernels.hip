@@ -40,6 +40,8 @@
 // reference's, so the FP64 path reproduces the reference's results.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "simp_device.h"
 #include "simp_kernels.h"
 
@@ -235,6 +237,10 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   }
   y.hmin = hmn;
   y.bin = bin;
+  // FP32 coordinates relative to the origin are within (max(W, H) + 1) bin;
+  // each rounds by at most 2^-24 of that, a difference by twice as much, and
+  // mhsml by 2^-24 relative: 8 ulp of the extent bounds the distance error
+  y.pf_slack = (float)(8.0 * 5.9604644775390625e-8 * (double)(max(W, H) + 1) * bin + 1e-3);
   y.xlo = xlo;
   y.ylo = ylo;
   y.W = (int)W;
@@ -312,15 +318,37 @@ __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
 }
 
 // members of each cell in ascending particle id: the order in which the
-// reference fills its bins (nnps.f90:246-254); x, y, hsml copied in that
-// order so that the neighbour loops read candidates contiguously
-__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
+// reference fills its bins (nnps.f90:246-254).  Cells of up to kRankMax
+// members (the common case) are ranked in registers; larger ones are
+// insertion-sorted in place.
+constexpr int kRankMax = 16;
+
+__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
   int c = blockIdx.x * kTPB + threadIdx.x;
   if (c >= d.ctot) return;
-  int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
-  if (s1 <= s0) return;
+  const int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
+  const int n = s1 - s0;
+  if (n <= 1) return;
   for (int s = s0; s < s1; ++s)
     if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: skipped instance
+  if (n <= kRankMax) {
+    int v[kRankMax], key[kRankMax];
+#pragma unroll
+    for (int i = 0; i < kRankMax; ++i) {
+      v[i] = i < n ? d.cl[s0 + i] : 0;
+      key[i] = i < n ? d.orig[v[i]] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int i = 0; i < kRankMax; ++i) {
+      if (i < n) {
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < kRankMax; ++j) r += key[j] < key[i];
+        d.cl[s0 + r] = v[i];
+      }
+    }
+    return;
+  }
   for (int a = s0 + 1; a < s1; ++a) {
     int v = d.cl[a];
     int key = d.orig[v];
@@ -333,14 +361,33 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
     }
     d.cl[b + 1] = v;
   }
-  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
-  for (int s = s0; s < s1; ++s) {
-    const int q = d.cl[s];
-    d.clpos[q] = s;
-    d.xs[s] = X[q];
-    d.ys[s] = Y[q];
-    d.hs[s] = HS[q];
-  }
+}
+
+// The right-hand-side state copied into cell-list order for the gathers,
+// one thread per list position.  Also the FP32 positions relative to the
+// instance's grid origin used by the candidate prefilter.
+__global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
+  const BlockEntry e = d.blk[blockIdx.x];
+  const int k = e.inst;
+  const InstMeta &m = d.meta[k];
+  const int local = e.first + (int)threadIdx.x;
+  if (local >= m.n) return;
+  const InstDyn &y = d.dyn[k];
+  if (y.err) return;
+  const int s = d.cell_start[m.c_off] + local;
+  const int q = d.cl[s];
+  d.clpos[q] = s;
+  d.cells[s] = d.cellid[q];
+  const double xq = d.x[buf][q], yq = d.y[buf][q];
+  d.xs[s] = xq;
+  d.ys[s] = yq;
+  d.hs[s] = d.hsml[buf][q];
+  d.vxs[s] = d.vx[buf][q];
+  d.vys[s] = d.vy[buf][q];
+  d.ms[s] = d.mass[q];
+  d.rs[s] = d.rho[buf][q];
+  const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
+  d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)d.hs[s], 0.0f);
 }
 
 // -------------------------------------------------------------- boundary
@@ -457,6 +504,49 @@ __device__ __forceinline__ int range_kind(int g, int s, int self) {
   return 2;
 }
 
+// The six ranges form four contiguous segments of the cell lists (the left,
+// own and right cells of a row are adjacent): [a0,b0) [a1,b3) [a4,b4)
+// [a5,a4).  A cursor walks them in order; batches of kBatch candidates are
+// loaded together so that a wavefront keeps several loads in flight.
+constexpr int kBatch = 4;
+
+struct Cursor {
+  int seg, s, end;                  // current segment and its end
+  int n1a, n1e, n2a, n2e, n3a, n3e; // the segments still ahead, in order
+  int own0, own1;                   // own cell range
+  __device__ __forceinline__ void init(const Ranges &R) {
+    n1a = R.a[1]; n1e = R.b[3];
+    n2a = R.a[4]; n2e = R.b[4];
+    n3a = R.a[5]; n3e = R.a[4];
+    own0 = R.a[2]; own1 = R.b[2];
+    seg = 0;
+    s = R.a[0];
+    end = R.b[0];
+    skip();
+  }
+  // register shifts only: an indexed select here becomes a scratch table
+  __device__ __forceinline__ void skip() {
+    while (seg < 4 && s >= end) {
+      ++seg;
+      s = n1a; end = n1e;
+      n1a = n2a; n1e = n2e;
+      n2a = n3a; n2e = n3e;
+    }
+  }
+  __device__ __forceinline__ bool done() const { return seg >= 4; }
+  __device__ __forceinline__ void next() {
+    ++s;
+    skip();
+  }
+  // 0: not owned, 1: intra pair owned by self, 2: cross pair owned by self
+  __device__ __forceinline__ int kind(int self) const {
+    if (seg >= 2) return 2;
+    if (seg == 0 || s < own0) return 0;
+    if (s < own1) return s > self ? 1 : 0;
+    return 2;
+  }
+};
+
 // Exact FP64 form of `d = SQRT(SUM(dx**2)); IF (d .LE. mhsml)`: the square
 // root is taken only when d^2 is within 1e-13 of mhsml^2, which decides the
 // far candidates without it.
@@ -502,68 +592,159 @@ struct KSum {
 
 // ---------------------------------------------------------------- pass A
 
+// The gathers run one thread per cell-list position: the threads of one
+// cell walk the same candidate ranges, so a wavefront's loads coalesce.
+__device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
+  const BlockEntry e = d.blk[blockIdx.x];
+  k = e.inst;
+  const int local = e.first + (int)threadIdx.x;
+  const InstMeta &m = d.meta[k];
+  if (local >= m.n) return false;
+  s = d.cell_start[m.c_off] + local;
+  return true;
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
-  int p = 0, k = 0;
-  if (!particle_of(d, p, k)) return;
+  int self = 0, k = 0;
+  if (!list_slot(d, self, k)) return;
   const InstDyn &y = d.dyn[k];
   if (y.err) return;
   const InstMeta &m = d.meta[k];
-  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf];
-  const double *VX = d.vx[buf], *VY = d.vy[buf], *RHO = d.rho[buf];
   const int W = y.W;
-  const int c = d.cellid[p];
+  const int c = d.cells[self];
   const int cy = c / W, cx = c - cy * W;
-  const int self = d.clpos[p];
-  const double xi = X[p], yi = Y[p], hi = HS[p], vxi = VX[p], vyi = VY[p];
+  const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
+  const double vxi = d.vxs[self], vyi = d.vys[self];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
   KSum fdiv, f1, f2, f3, f4;
-  int nI = 0, nX = 0, nn = 0;
-  for (int g = 0; g < 6; ++g) {
-    for (int s = R.a[g]; s < R.b[g]; ++s) {
-      if (s == self) continue;
-      const double dx = xi - d.xs[s], dy = yi - d.ys[s];
-      const double mh = (hi + d.hs[s]) / 2.;
-      double r;
-      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-      const int q = d.cl[s];
-      const int kind = range_kind(g, s, self);
-      nI += kind == 1;
-      nX += kind == 2;
-      ++nn;
-      const double mq = d.mass[q], rq = RHO[q];
-      const double dv0 = vxi - VX[q], dv1 = vyi - VY[q];
-      if (!MIXED) {
-        const double gg = c6_grad(r, mh);
-        const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
-        const double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
-        div = div - mq * (w1 + w4);
-        g1 = g1 - mq * w1 / rq;
-        g2 = g2 - mq * w2 / rq;
-        g3 = g3 - mq * w3 / rq;
-        g4 = g4 - mq * w4 / rq;
-      } else {
-        const float rf = (float)r;
-        const float gg = c6_grad_f(rf, (float)mh);
-        const float ir = __builtin_amdgcn_rcpf(rf);
-        const float dw0 = gg * ((float)dx * ir), dw1 = gg * ((float)dy * ir);
-        const float a0 = (float)dv0, a1 = (float)dv1;
-        const float w1 = a0 * dw0, w2 = a1 * dw0, w3 = a0 * dw1, w4 = a1 * dw1;
-        const float mf = (float)mq, mr = (float)(mq / rq);
-        fdiv.add(-(mf * (w1 + w4)));
-        f1.add(-(mr * w1));
-        f2.add(-(mr * w2));
-        f3.add(-(mr * w3));
-        f4.add(-(mr * w4));
+  // kernel gradient, vector_divergence and vector_gradient of one pair
+  auto pair = [&](int s, double dx, double dy, double r, double mh) {
+    const double mq = d.ms[s], rq = d.rs[s];
+    const double dv0 = vxi - d.vxs[s], dv1 = vyi - d.vys[s];
+    if (!MIXED) {
+      const double gg = c6_grad(r, mh);
+      const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+      const double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
+      div = div - mq * (w1 + w4);
+      g1 = g1 - mq * w1 / rq;
+      g2 = g2 - mq * w2 / rq;
+      g3 = g3 - mq * w3 / rq;
+      g4 = g4 - mq * w4 / rq;
+    } else {
+      const float fx = (float)dx, fy = (float)dy;
+      const float rf = __builtin_sqrtf(fx * fx + fy * fy);
+      const float gg = c6_grad_f(rf, (float)mh);
+      const float ir = __builtin_amdgcn_rcpf(rf);
+      const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
+      const float a0 = (float)dv0, a1 = (float)dv1;
+      const float w1 = a0 * dw0, w2 = a1 * dw0, w3 = a0 * dw1, w4 = a1 * dw1;
+      const float mf = (float)mq, mr = mf * __builtin_amdgcn_rcpf((float)rq);
+      fdiv.add(-(mf * (w1 + w4)));
+      f1.add(-(mr * w1));
+      f2.add(-(mr * w2));
+      f3.add(-(mr * w3));
+      f4.add(-(mr * w4));
+    }
+  };
+  // 1) FP32 prefilter of the 3x3 block into a candidate list (batched
+  //    loads); 2) exact FP64 test and pair arithmetic over the candidates,
+  //    which also writes the neighbour list.  Splitting the two keeps the
+  //    expensive arithmetic off the ~90 % of candidates that fail.
+  __shared__ int lst[kNbrMax * kTPB];
+  int nI = 0, nX = 0, nn = 0, nc = 0;
+  {
+    const float4 pi4 = d.pf[self];
+    const float slack = y.pf_slack;
+    Cursor cur;
+    cur.init(R);
+    while (!cur.done()) {
+      int sc[kBatch], kc[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const bool live = !cur.done();
+        sc[u] = live ? cur.s : self;
+        kc[u] = live ? cur.kind(self) : -1;
+        if (live) cur.next();
+      }
+      float4 q4[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) q4[u] = d.pf[sc[u]];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (kc[u] < 0 || sc[u] == self) continue;
+        // rejects only candidates certainly beyond mhsml
+        const float fx = pi4.x - q4[u].x, fy = pi4.y - q4[u].y;
+        const float fm = (pi4.z + q4[u].z) * 0.5f + slack;
+        if (fx * fx + fy * fy > fm * fm * 1.00001f) continue;
+        if (nc < kNbrMax)
+          lst[nc * kTPB + threadIdx.x] = (int)((unsigned)sc[u] | ((unsigned)kc[u] << 30));
+        ++nc;
       }
     }
   }
+  const size_t N = (size_t)d.ntot;
+  if (nc <= kNbrMax) {
+    for (int j0 = 0; j0 < nc; j0 += kBatch) {
+      int sj[kBatch], kj[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const bool v = j0 + u < nc;
+        const unsigned e = v ? (unsigned)lst[(j0 + u) * kTPB + threadIdx.x] : 0u;
+        sj[u] = v ? (int)(e & 0x3fffffffu) : self;
+        kj[u] = v ? (int)(e >> 30) : -1;
+      }
+      double xq[kBatch], yq[kBatch], hq[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        xq[u] = d.xs[sj[u]];
+        yq[u] = d.ys[sj[u]];
+        hq[u] = d.hs[sj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (kj[u] < 0) continue;
+        const double dx = xi - xq[u], dy = yi - yq[u];
+        const double mh = (hi + hq[u]) / 2.;
+        double r;
+        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        nI += kj[u] == 1;
+        nX += kj[u] == 2;
+        d.nbr[nn * N + self] = sj[u];
+        ++nn;
+        pair(sj[u], dx, dy, r, mh);
+      }
+    }
+  } else {
+    nn = 0;
+    int ntrue = 0;
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+      for (int s = R.a[g]; s < R.b[g]; ++s) {
+        if (s == self) continue;
+        const double dx = xi - d.xs[s], dy = yi - d.ys[s];
+        const double mh = (hi + d.hs[s]) / 2.;
+        double r;
+        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        const int kind = range_kind(g, s, self);
+        nI += kind == 1;
+        nX += kind == 2;
+        ++ntrue;
+        pair(s, dx, dy, r, mh);
+      }
+    }
+    nn = ntrue;
+  }
+  // list length, or kNbrMax + 1 when it was not stored (the other gathers
+  // then rescan the 3x3 block)
+  d.nnb[self] = nc <= kNbrMax ? nn : kNbrMax + 1;
   if (MIXED) {
     div = fdiv.value();
     g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
   }
+  const int p = d.cl[self];
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
   d.own_cnt[p] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
 
@@ -587,16 +768,15 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const double s12 = 2 * et * e12;
   const double s22 = (z + et) * e22 + (z - et) * e11 - pr * (1 - kKT) / 2;
   if (!MIXED) {
-    ((double *)d.sig11)[p] = s11;
-    ((double *)d.sig12)[p] = s12;
-    ((double *)d.sig22)[p] = s22;
-    ((double *)d.div)[p] = div;
+    ((double *)d.sig11)[self] = s11;
+    ((double *)d.sig12)[self] = s12;
+    ((double *)d.sig22)[self] = s22;
   } else {
-    ((float *)d.sig11)[p] = (float)s11;
-    ((float *)d.sig12)[p] = (float)s12;
-    ((float *)d.sig22)[p] = (float)s22;
-    ((double *)d.div)[p] = div;
+    ((float *)d.sig11)[self] = (float)s11;
+    ((float *)d.sig12)[self] = (float)s12;
+    ((float *)d.sig22)[self] = (float)s22;
   }
+  ((double *)d.div)[self] = div;
   if (want_diag) {
     const int N = d.ntot;
     d.diag[p] = e11;
@@ -605,6 +785,15 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     d.diag[3 * N + p] = z;
     d.diag[4 * N + p] = et;
     d.diag[5 * N + p] = pr;
+    if (!MIXED) {
+      d.sig_out[p] = s11;
+      d.sig_out[N + p] = s12;
+      d.sig_out[2 * N + p] = s22;
+    } else {
+      d.sig_out[p] = (double)(float)s11;
+      d.sig_out[N + p] = (double)(float)s12;
+      d.sig_out[2 * N + p] = (double)(float)s22;
+    }
   }
 }
 
@@ -633,7 +822,7 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
 // reference's module array does.  Ranks grow along the cell lists, so the
 // threads of an instance walk the list from its start and stop at the first
 // member whose cell ranks at or beyond the limit.
-constexpr int kMatBlocks = 4;
+constexpr int kMatBlocks = 32;
 
 __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
   const int k = blockIdx.x / kMatBlocks;
@@ -659,8 +848,21 @@ __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
     int rI = base + (d.moff[s] & 0xffff);
     int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
     if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) continue;
+    const double hi = d.hs[s];
+    const int nn = d.nnb[s];
+    if (nn <= kNbrMax) {
+      // the owned pairs are the tail of the list: nI intra, then nX cross
+      const size_t N = (size_t)d.ntot;
+      for (int j = nn - nI - nX; j < nn; ++j) {
+        const int t = d.nbr[j * N + s];
+        int &rank = (j < nn - nX) ? rI : rX;
+        if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[t]) / 2.;
+        ++rank;
+      }
+      continue;
+    }
     const int cy = c / W, cx = c - cy * W;
-    const double xi = d.xs[s], yi = d.ys[s], hi = d.hs[s];
+    const double xi = d.xs[s], yi = d.ys[s];
     const Ranges R = neighbour_ranges(cs, W, cx, cy);
     for (int g = 2; g < 6; ++g) {
       for (int t = R.a[g]; t < R.b[g]; ++t) {
@@ -707,59 +909,89 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 // place), 2 = single evaluation (tendencies to d.tend)
 template <bool MIXED>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
-  int p = 0, k = 0;
-  const bool ok = particle_of(d, p, k);
+  int self = 0, k = 0;
+  const bool ok = list_slot(d, self, k);
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   const bool live = ok && !y.err;
   const int buf = phase == 1 ? 1 : 0;
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   if (live) {
-    const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RHO = d.rho[buf];
     const int W = y.W;
-    const int c = d.cellid[p];
+    const int c = d.cells[self];
     const int cy = c / W, cx = c - cy * W;
-    const int self = d.clpos[p];
-    const double xi = X[p], yi = Y[p], hi = HS[p], rhoi = RHO[p], mi = d.mass[p];
+    const int p = d.cl[self];
+    const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
+    const double rhoi = d.rs[self], mi = d.ms[self];
+    const double vxi = d.vxs[self], vyi = d.vys[self];
     const double hthi = d.h[buf][p], Ai = d.A[buf][p];
-    const double vxi = d.vx[buf][p], vyi = d.vy[buf][p];
     const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
     double ax = 0., ay = 0.;
-    // stress_1 (internal_stress.f90:103-136)
-    if (!MIXED) {
-      const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
-                   *S22 = (const double *)d.sig22;
-      const double s11i = S11[p], s12i = S12[p], s22i = S22[p];
-      const double ri2 = rhoi * rhoi;
-      for (int g = 0; g < 6; ++g) {
-        for (int s = R.a[g]; s < R.b[g]; ++s) {
-          if (s == self) continue;
-          const double dx = xi - d.xs[s], dy = yi - d.ys[s];
-          const double mh = (hi + d.hs[s]) / 2.;
-          double r;
-          if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-          const int q = d.cl[s];
-          const double gg = c6_grad(r, mh);
-          const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
-          const double rq = RHO[q], mq = d.mass[q];
-          const double rq2 = rq * rq;
-          const double coef = rhoi * mq / kRhoIce / hthi;
-          double cj = (S11[q] * dw0 + S12[q] * dw1) / rq2;
-          double ci = (s11i * dw0 + s12i * dw1) / ri2;
-          ax = ax + coef * (ci + cj);
-          cj = (S12[q] * dw0 + S22[q] * dw1) / rq2;
-          ci = (s12i * dw0 + s22i * dw1) / ri2;
-          ay = ay + coef * (ci + cj);
+    // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
+    // in batches of kBatch neighbours whose loads are issued together
+    const int nn = d.nnb[self];
+    const size_t N = (size_t)d.ntot;
+    using ST = typename std::conditional<MIXED, float, double>::type;
+    const ST *S11 = (const ST *)d.sig11, *S12 = (const ST *)d.sig12, *S22 = (const ST *)d.sig22;
+    const double ri2 = rhoi * rhoi;
+    const double s11d = (double)S11[self], s12d = (double)S12[self], s22d = (double)S22[self];
+    const float ir2 = (float)(1.0 / ri2);
+    const float s11f = (float)s11d * ir2, s12f = (float)s12d * ir2, s22f = (float)s22d * ir2;
+    KSum sx, sy;
+    auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
+                           ST q11, ST q12, ST q22) {
+      if (!MIXED) {
+        const double gg = c6_grad(r, mh);
+        const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+        const double rq2 = rq * rq;
+        const double coef = rhoi * mq / kRhoIce / hthi;
+        double cj = ((double)q11 * dw0 + (double)q12 * dw1) / rq2;
+        double ci = (s11d * dw0 + s12d * dw1) / ri2;
+        ax = ax + coef * (ci + cj);
+        cj = ((double)q12 * dw0 + (double)q22 * dw1) / rq2;
+        ci = (s12d * dw0 + s22d * dw1) / ri2;
+        ay = ay + coef * (ci + cj);
+      } else {
+        const float fx = (float)dx, fy = (float)dy;
+        const float rf = __builtin_sqrtf(fx * fx + fy * fy);
+        const float gg = c6_grad_f(rf, (float)mh);
+        const float ir = __builtin_amdgcn_rcpf(rf);
+        const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
+        const float rqf = (float)rq;
+        const float irq2 = __builtin_amdgcn_rcpf(rqf * rqf);
+        const float t11 = __builtin_fmaf((float)q11, irq2, s11f);
+        const float t12 = __builtin_fmaf((float)q12, irq2, s12f);
+        const float t22 = __builtin_fmaf((float)q22, irq2, s22f);
+        const float mf = (float)mq;
+        sx.add(mf * (t11 * dw0 + t12 * dw1));
+        sy.add(mf * (t12 * dw0 + t22 * dw1));
+      }
+    };
+    if (nn <= kNbrMax) {
+      for (int j0 = 0; j0 < nn; j0 += kBatch) {
+        int sj[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) sj[u] = j0 + u < nn ? d.nbr[(j0 + u) * N + self] : self;
+        double xq[kBatch], yq[kBatch], hq[kBatch], rq[kBatch], mq[kBatch];
+        ST q11[kBatch], q12[kBatch], q22[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          const int t = sj[u];
+          xq[u] = d.xs[t]; yq[u] = d.ys[t]; hq[u] = d.hs[t];
+          rq[u] = d.rs[t]; mq[u] = d.ms[t];
+          q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          if (j0 + u >= nn) break;
+          const double dx = xi - xq[u], dy = yi - yq[u];
+          const double mh = (hi + hq[u]) / 2.;
+          stress_pair(dx, dy, MIXED ? 0.0 : sqrt(dx * dx + dy * dy), mh, rq[u], mq[u], q11[u],
+                      q12[u], q22[u]);
         }
       }
     } else {
-      const float *S11 = (const float *)d.sig11, *S12 = (const float *)d.sig12,
-                  *S22 = (const float *)d.sig22;
-      const float ir2 = (float)(1.0 / (rhoi * rhoi));
-      const float s11i = S11[p] * ir2, s12i = S12[p] * ir2, s22i = S22[p] * ir2;
-      const double cf = rhoi / kRhoIce / hthi;
-      KSum sx, sy;
       for (int g = 0; g < 6; ++g) {
         for (int s = R.a[g]; s < R.b[g]; ++s) {
           if (s == self) continue;
@@ -767,21 +999,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
           const double mh = (hi + d.hs[s]) / 2.;
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-          const int q = d.cl[s];
-          const float rf = (float)r;
-          const float gg = c6_grad_f(rf, (float)mh);
-          const float ir = __builtin_amdgcn_rcpf(rf);
-          const float dw0 = gg * ((float)dx * ir), dw1 = gg * ((float)dy * ir);
-          const double rq = RHO[q];
-          const float irq2 = (float)(1.0 / (rq * rq));
-          const float mq = (float)d.mass[q];
-          const float t11 = __builtin_fmaf(S11[q], irq2, s11i);
-          const float t12 = __builtin_fmaf(S12[q], irq2, s12i);
-          const float t22 = __builtin_fmaf(S22[q], irq2, s22i);
-          sx.add(mq * (t11 * dw0 + t12 * dw1));
-          sy.add(mq * (t12 * dw0 + t22 * dw1));
+          stress_pair(dx, dy, r, mh, d.rs[s], d.ms[s], S11[s], S12[s], S22[s]);
         }
       }
+    }
+    if (MIXED) {
+      const double cf = rhoi / kRhoIce / hthi;
       ax = cf * sx.value();
       ay = cf * sy.value();
     }
@@ -869,7 +1092,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
     }
     // tendencies (single_step.f90:104-106)
-    const double divi = ((const double *)d.div)[p];
+    const double divi = ((const double *)d.div)[self];
     const double dhdt = -(hthi * divi / rhoi);
     const double dAdt = -(Ai * divi / rhoi);
     const double dt = y.dt;
@@ -1027,7 +1250,8 @@ void launch_scatter(const DevState &d, hipStream_t st) {
   k_scatter<<<d.nblk, kTPB, 0, st>>>(d);
 }
 void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
-  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d, buf);
+  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
+  k_listcopy<<<d.nblk, kTPB, 0, st>>>(d, buf);
 }
 void launch_bprep(const DevState &d, hipStream_t st) {
   k_bprep<<<d.ninst, kBprepThreads, 0, st>>>(d);
