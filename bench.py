@@ -97,6 +97,77 @@ def cpu_baseline(sc_full, target_s=15.0):
                       f"{sub.n} particles, {steps} steps, {wall:.2f} s"}
 
 
+def _copy_calibration(mib):
+    """A plain device-to-device copy of known size through the HIP runtime
+    (its blit kernel is the reference for the FETCH_SIZE / WRITE_SIZE
+    counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def hbm_traffic(kernel_substr, timeout=240):
+    """HBM bytes per launch of the kernel whose name contains kernel_substr,
+    from two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a
+    short mixed-precision run of this script.  Each counter is scaled by
+    what it reads for plain device-to-device copies of known size (3 x 256
+    MiB, run first in the profiled process).  Returns (bytes, factors) or
+    (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    expect_kib = 3 * 256 * 1024
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--",
+                   sys.executable, os.path.abspath(__file__), "--precision", "mixed",
+                   "--steps", "10", "--warmup", "3", "--latency-steps", "1", "--no-cpu",
+                   "--copy-calib", "256"]
+            try:
+                subprocess.run(cmd, check=True, timeout=timeout, cwd=tmp,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            except Exception as e:  # noqa: BLE001
+                return None, f"rocprofv3 {counter} pass failed: {e}"
+            rows = []
+            for root, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        rows += list(csv.DictReader(open(os.path.join(root, f))))
+            rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+            calib, vals = 0.0, []
+            seen_simp = False
+            for r in rows:
+                name = r["Kernel_Name"]
+                if "simp::" in name:
+                    seen_simp = True
+                if not seen_simp and "copyBuffer" in name:
+                    calib += float(r["Counter_Value"])
+                if kernel_substr in name:
+                    vals.append(float(r["Counter_Value"]))
+            if calib <= 0 or not vals:
+                return None, f"no {counter} samples"
+            out[counter] = (sum(vals) / len(vals), expect_kib / calib)
+    fetch, ff = out["FETCH_SIZE"]
+    write, fw = out["WRITE_SIZE"]
+    return 1024.0 * (fetch * ff + write * fw), {"fetch_factor": ff, "write_factor": fw,
+                                                 "fetch_kib": fetch, "write_kib": write}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +175,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--latency-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run a plain device-to-device copy of MIB MiB (the "
+                         "reference kernel for scaling HBM counters)")
+    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes for roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SPH-VP step of one simulation (or one "
@@ -115,11 +192,15 @@ def main():
     from simp_amd import scenarios as S
     from simp_amd.api import Simp
 
+    if args.copy_calib > 0:
+        _copy_calibration(args.copy_calib)
     sc = S.ice_edge()
     n = sc.n
     res = {}
     prof = {}
-    for prec in (_lib.SIMP_FP64, _lib.SIMP_MIXED):
+    precs = {"both": (_lib.SIMP_FP64, _lib.SIMP_MIXED), "mixed": (_lib.SIMP_MIXED,),
+             "fp64": (_lib.SIMP_FP64,)}[args.precision]
+    for prec in precs:
         dev = Simp.for_scenario(sc, precision=prec)
         dev.upload(sc)
         dev.step(args.warmup, sync=True)
@@ -131,6 +212,12 @@ def main():
         prof[prec] = dev.profile_steps(20)
         dev.close()
 
+    if args.precision != "both":
+        r = res[precs[0]]
+        print(json.dumps({"precision": args.precision, "particle_steps_per_s": r["rate"],
+                          "ms_per_step": r["ms"], "kernels_ms": {
+                              k: v[0] for k, v in prof[precs[0]].items()}}), flush=True)
+        return
     mixed = res[_lib.SIMP_MIXED]
     fp64 = res[_lib.SIMP_FP64]
     # dominant kernel of the mixed path
@@ -170,6 +257,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    if not args.no_traffic:
+        kname = "k_passB<true>" if name == "passB" else "k_passA<true>"
+        traffic, info = hbm_traffic(kname)
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_note"] = info
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)
