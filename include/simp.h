@@ -148,6 +148,50 @@ int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf,
 
 const char *simp_strerror(int status);
 
+/* ---- ensembles: many independent simulations in one packed launch ----
+ * No reference counterpart (the reference runs one simulation per process,
+ * modules/globals.f90:21-25).  Every step of a batch advances all its active
+ * instances with one launch per kernel; instances of different sizes, domains
+ * and forcing are packed with per-instance offsets.  An instance freezes when
+ * its step budget is used up or when it fails (its status says why); the
+ * others continue.  Adding or removing instances rebuilds the packed layout
+ * on the device (live state is copied device to device). */
+typedef struct simp_batch simp_batch;
+
+typedef struct simp_instance_desc {
+    int32_t n, nb;                       /* particles, boundary particles      */
+    const double *x, *vx, *mass, *rho, *hsml, *hth, *A;   /* as simp_upload   */
+    const double *xb, *massb;
+    double  dt, current_time;
+    int32_t itimestep;
+    int32_t nsteps;                      /* steps to run, then freeze; 0: none */
+    int32_t tse;                         /* 0 (keep dt) or 2 (VP dt)           */
+    int32_t wind_mode;                   /* SIMP_WIND_UNIFORM / SIMP_WIND_GRID */
+    double  wind[2], current[2], max_hsml;
+    int32_t grid_nx, grid_ny;            /* wind grid (wind_mode = GRID)       */
+    double  grid_x0, grid_y0, grid_dx, grid_dy;
+    const double *grid_u, *grid_v;
+} simp_instance_desc;
+
+int simp_batch_create(int precision, simp_batch **out);
+int simp_batch_destroy(simp_batch *b);
+/* Add count instances; ids[k] receives the id of desc[k]. */
+int simp_batch_add(simp_batch *b, int count, const simp_instance_desc *desc, int *ids);
+int simp_batch_remove(simp_batch *b, int count, const int *ids);
+/* Advance every active instance by nsteps predictor-corrector steps. */
+int simp_batch_step(simp_batch *b, int nsteps, int sync);
+int simp_batch_sync(simp_batch *b);
+/* itimestep, steps still to run (< 0: no limit), active flag, and the
+ * instance's status (SIMP_OK, SIMP_E_NAN, SIMP_E_CAPACITY). */
+int simp_batch_status(simp_batch *b, int id, int *itimestep, int *steps_left, int *active,
+                      int *status);
+int simp_batch_download(simp_batch *b, int id, double *x, double *vx, double *rho,
+                        double *hsml, double *hth, double *A, double *dt,
+                        double *current_time, int *itimestep);
+int simp_batch_count(simp_batch *b, int *instances, long long *particles);
+/* Device bytes an instance would occupy (for admission against a budget). */
+long long simp_instance_footprint(const simp_instance_desc *desc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,9 @@ EXPORTS = (
     "simp_single_step", "simp_nnps", "simp_upload", "simp_step_pc",
     "simp_download", "simp_set_diagnostics", "simp_download_diagnostics",
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
+    "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
+    "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
+    "simp_batch_count", "simp_instance_footprint",
 )
 
 
@@ -43,6 +46,21 @@ class SimpConfig(C.Structure):
                 ("max_particles", C.c_int32), ("max_boundary", C.c_int32),
                 ("reserved", C.c_int32), ("wind", C.c_double * 2),
                 ("current", C.c_double * 2), ("max_hsml", C.c_double)]
+
+
+class SimpInstanceDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nb", C.c_int32),
+                ("x", C.c_void_p), ("vx", C.c_void_p), ("mass", C.c_void_p),
+                ("rho", C.c_void_p), ("hsml", C.c_void_p), ("hth", C.c_void_p),
+                ("A", C.c_void_p), ("xb", C.c_void_p), ("massb", C.c_void_p),
+                ("dt", C.c_double), ("current_time", C.c_double),
+                ("itimestep", C.c_int32), ("nsteps", C.c_int32), ("tse", C.c_int32),
+                ("wind_mode", C.c_int32), ("wind", C.c_double * 2),
+                ("current", C.c_double * 2), ("max_hsml", C.c_double),
+                ("grid_nx", C.c_int32), ("grid_ny", C.c_int32),
+                ("grid_x0", C.c_double), ("grid_y0", C.c_double),
+                ("grid_dx", C.c_double), ("grid_dy", C.c_double),
+                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p)]
 
 
 class SimpError(RuntimeError):
@@ -89,6 +107,17 @@ def lib():
     L.simp_stream.argtypes = [_vp]
     L.simp_stream.restype = _vp
     L.simp_profile_steps.argtypes = [_vp, C.c_int, C.c_int, C.c_char_p, C.c_int, _vp, _vp]
+    L.simp_batch_create.argtypes = [C.c_int, C.POINTER(_vp)]
+    L.simp_batch_destroy.argtypes = [_vp]
+    L.simp_batch_add.argtypes = [_vp, C.c_int, C.POINTER(SimpInstanceDesc), _vp]
+    L.simp_batch_remove.argtypes = [_vp, C.c_int, _vp]
+    L.simp_batch_step.argtypes = [_vp, C.c_int, C.c_int]
+    L.simp_batch_sync.argtypes = [_vp]
+    L.simp_batch_status.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]
+    L.simp_batch_download.argtypes = [_vp, C.c_int] + [_vp] * 9
+    L.simp_batch_count.argtypes = [_vp, _vp, _vp]
+    L.simp_instance_footprint.argtypes = [C.POINTER(SimpInstanceDesc)]
+    L.simp_instance_footprint.restype = C.c_longlong
     _lib = L
     return L
 

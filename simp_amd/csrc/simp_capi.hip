@@ -297,4 +297,145 @@ int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf, int bufl
   return k;
 }
 
+/* ------------------------------------------------------------ batches */
+
+struct simp_batch {
+  Engine *eng = nullptr;
+};
+
+static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
+  if (d.n < 1 || d.nb < 0 || !d.x || !d.vx || !d.mass || !d.rho || !d.hsml || !d.hth || !d.A)
+    return SIMP_E_ARG;
+  if (d.nb > 0 && (!d.xb || !d.massb)) return SIMP_E_ARG;
+  if (d.tse != 0 && d.tse != 2) return SIMP_E_UNSUPPORTED;
+  if (d.wind_mode != SIMP_WIND_UNIFORM && d.wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
+  if (d.wind_mode == SIMP_WIND_GRID &&
+      (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
+       !(d.grid_dy > 0)))
+    return SIMP_E_ARG;
+  in.n = d.n; in.nb = d.nb;
+  in.x = d.x; in.vx = d.vx; in.mass = d.mass; in.rho = d.rho; in.hsml = d.hsml;
+  in.hth = d.hth; in.A = d.A; in.xb = d.xb; in.massb = d.massb;
+  in.dt = d.dt; in.time = d.current_time; in.itimestep = d.itimestep;
+  in.nsteps = d.nsteps > 0 ? d.nsteps : 0;
+  in.tse = d.tse;
+  in.wind_mode = d.wind_mode;
+  in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
+  in.current[0] = d.current[0]; in.current[1] = d.current[1];
+  in.max_hsml = d.max_hsml;
+  if (!(in.max_hsml > 0)) {
+    double mx = 0;
+    for (int i = 0; i < in.n; ++i) mx = std::max(mx, in.hsml[i]);
+    in.max_hsml = mx * 10.;
+  }
+  in.g_nx = d.grid_nx; in.g_ny = d.grid_ny;
+  in.g_x0 = d.grid_x0; in.g_y0 = d.grid_y0; in.g_dx = d.grid_dx; in.g_dy = d.grid_dy;
+  in.g_u = d.grid_u; in.g_v = d.grid_v;
+  return SIMP_OK;
+}
+
+int simp_batch_create(int precision, simp_batch **out) {
+  if (!out) return SIMP_E_ARG;
+  *out = nullptr;
+  if (precision != SIMP_FP64 && precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SIMP_E_HIP;
+  simp_batch *b = new (std::nothrow) simp_batch;
+  if (!b) return SIMP_E_NOMEM;
+  b->eng = new (std::nothrow) Engine(precision);
+  if (!b->eng || !b->eng->stream()) {
+    delete b->eng;
+    delete b;
+    return SIMP_E_HIP;
+  }
+  *out = b;
+  return SIMP_OK;
+}
+
+int simp_batch_destroy(simp_batch *b) {
+  if (!b) return SIMP_E_ARG;
+  delete b->eng;
+  delete b;
+  return SIMP_OK;
+}
+
+int simp_batch_add(simp_batch *b, int count, const simp_instance_desc *desc, int *ids) {
+  if (!b || count < 0 || (count > 0 && (!desc || !ids))) return SIMP_E_ARG;
+  std::vector<InstanceInput> ins(count);
+  for (int k = 0; k < count; ++k) {
+    int rc = fill_from_desc(desc[k], ins[k]);
+    if (rc) return rc;
+  }
+  std::vector<int> got;
+  int rc = b->eng->add(ins, &got);
+  if (rc) return map_rc(rc);
+  for (int k = 0; k < count; ++k) ids[k] = got[k];
+  return SIMP_OK;
+}
+
+int simp_batch_remove(simp_batch *b, int count, const int *ids) {
+  if (!b || count < 0 || (count > 0 && !ids)) return SIMP_E_ARG;
+  std::vector<int> v(ids, ids + count);
+  for (int id : v)
+    if (b->eng->index_of(id) < 0) return SIMP_E_ARG;
+  return map_rc(b->eng->remove(v));
+}
+
+int simp_batch_step(simp_batch *b, int nsteps, int sync) {
+  if (!b || nsteps < 0) return SIMP_E_ARG;
+  int rc = b->eng->steps(nsteps, true);
+  if (rc) return map_rc(rc);
+  if (sync && hipStreamSynchronize(b->eng->stream()) != hipSuccess) return SIMP_E_HIP;
+  return SIMP_OK;
+}
+
+int simp_batch_sync(simp_batch *b) {
+  if (!b) return SIMP_E_ARG;
+  return hipStreamSynchronize(b->eng->stream()) == hipSuccess ? SIMP_OK : SIMP_E_HIP;
+}
+
+int simp_batch_status(simp_batch *b, int id, int *itimestep, int *steps_left, int *active,
+                      int *status) {
+  if (!b) return SIMP_E_ARG;
+  const int k = b->eng->index_of(id);
+  if (k < 0) return SIMP_E_ARG;
+  if (hipStreamSynchronize(b->eng->stream()) != hipSuccess) return SIMP_E_HIP;
+  int err = 0, act = 0;
+  int rc = b->eng->download_status(k, itimestep, steps_left, &act, &err);
+  if (rc) return map_rc(rc);
+  if (active) *active = (act && !err) ? 1 : 0;   // a failed instance no longer steps
+  if (status) {
+    *status = SIMP_OK;
+    if (err & simp::kErrCapacity) *status = SIMP_E_CAPACITY;
+    if (err & simp::kErrNan) *status = SIMP_E_NAN;
+  }
+  return SIMP_OK;
+}
+
+int simp_batch_download(simp_batch *b, int id, double *x, double *vx, double *rho, double *hsml,
+                        double *hth, double *A, double *dt, double *current_time,
+                        int *itimestep) {
+  if (!b) return SIMP_E_ARG;
+  const int k = b->eng->index_of(id);
+  if (k < 0) return SIMP_E_ARG;
+  int rc = b->eng->download_state(k, x, vx, rho, hsml, hth, A);
+  if (rc) return map_rc(rc);
+  return map_rc(b->eng->download_scalars(k, dt, current_time, itimestep, nullptr, nullptr,
+                                         nullptr));
+}
+
+int simp_batch_count(simp_batch *b, int *instances, long long *particles) {
+  if (!b) return SIMP_E_ARG;
+  if (instances) *instances = b->eng->ninst();
+  if (particles) *particles = b->eng->total_particles();
+  return SIMP_OK;
+}
+
+long long simp_instance_footprint(const simp_instance_desc *desc) {
+  if (!desc) return -1;
+  InstanceInput in;
+  if (fill_from_desc(*desc, in) != SIMP_OK) return -1;
+  return (long long)simp::instance_footprint(in);
+}
+
 }  // extern "C"

@@ -62,6 +62,8 @@ struct InstDyn {
   int nip, bnip;
   int stamp;            // evaluation counter (boundary cell flags)
   float pf_slack;       // bound of the prefilter's FP32 distance error (m)
+  int active;           // 0: frozen (finished or not started), kernels skip it
+  int steps_left;       // steps until the instance freezes; < 0: no limit
   double hmin;          // MINVAL(hsml) of the current evaluation
 };
 

@@ -2,7 +2,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <map>
 #include <string>
 #include <vector>
 
@@ -18,6 +17,7 @@ struct InstanceInput {
   const double *xb = nullptr, *massb = nullptr;
   double dt = 0.0, time = 0.0;
   int itimestep = 0;
+  int nsteps = 0;        // steps to run before the instance freezes (0: no limit)
   int tse = 2;
   int wind_mode = 0;
   double wind[2] = {0.0, 0.0}, current[2] = {0.0, 0.0};
@@ -26,8 +26,6 @@ struct InstanceInput {
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
   const double *g_u = nullptr, *g_v = nullptr;
-  // capacity hints (0 = derive)
-  int c_cap = 0, q_cap = 0;
 };
 
 struct Profiler {
@@ -38,48 +36,61 @@ struct Profiler {
   std::vector<Rec> recs;
 };
 
+// Device bytes one instance occupies (state, per-evaluation buffers, cells,
+// boundary-pair store), for admission against an HBM budget.
+size_t instance_footprint(const InstanceInput &in);
+
 class Engine {
  public:
   explicit Engine(int precision);
   ~Engine();
 
-  // Pack and upload the instances (replaces the previous content).  The
-  // hsml_ij store keeps its contents when the layout is unchanged.
+  // Replace the whole content by these instances.  With keep_store and a
+  // single instance, the hsml_ij store of the previous content's first
+  // instance is carried over (the reference's module array persists across
+  // single_step calls).
   int upload(const std::vector<InstanceInput> &ins, bool keep_store);
+  // Add instances (ids returned) / remove instances by id: the packed
+  // layout is rebuilt on the device, live state is copied device to device.
+  int add(const std::vector<InstanceInput> &ins, std::vector<int> *ids);
+  int remove(const std::vector<int> &ids);
+  int index_of(int id) const;
+  const std::vector<int> &ids() const { return ids_; }
+
   // One right-hand-side evaluation of the state S (single_step); writes
   // tendencies and diagnostics on the device.
   int eval_single(bool want_diag);
-  // nsteps predictor-corrector steps.
+  // nsteps predictor-corrector steps of every active instance.
   int steps(int nsteps, bool use_graph);
   int sync_and_check();
-  // host copies
+  // host copies (by packed index)
   int download_state(int inst, double *x, double *vx, double *rho, double *hsml, double *hth,
                      double *A);
   int download_scalars(int inst, double *dt, double *time, int *itimestep, int *nip, int *bnip,
                        int *err);
+  int download_status(int inst, int *itimestep, int *steps_left, int *active, int *err);
   int download_tend(int inst, double *dxdt, double *dvxdt, double *dhdt, double *dAdt);
   int download_diag(int inst, double *eps3, double *zeta, double *eta, double *p,
                     double *sig3);
   int export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, double *hij, int bcap,
                    int *bnip, int *bpi, int *bpj, double *brij);
-  // A probe evaluation prefix (binning + boundary prep) to size the
-  // boundary-pair store; returns bnip per instance.
-  int probe_bnip(std::vector<int> &bnip);
 
   void set_diag(bool on) { diag_ = on; }
   bool diag() const { return diag_; }
   hipStream_t stream() const { return st_; }
   int ninst() const { return (int)meta_.size(); }
+  long total_particles() const { return d_.ntot; }
   const InstMeta &meta(int k) const { return meta_[k]; }
   Profiler *prof = nullptr;
 
  private:
+  int repack(const std::vector<int> &keep, const std::vector<InstanceInput> &add,
+             int carry_store_from, std::vector<int> *new_ids);
   void enqueue_eval(int buf, int phase, bool want_diag);
   void enqueue_prefix(int buf, bool fork_boundary);
   void enqueue_step();
-  int ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk, long gtot);
-  void free_all();
   int graph_build();
+  void graph_drop();
   template <class F>
   void L(const char *name, F &&f);
 
@@ -88,17 +99,15 @@ class Engine {
   hipStream_t side_ = nullptr;        // boundary prep, concurrent with pass A
   hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   DevState d_{};
+  std::vector<void *> allocs_;        // device buffers of d_
+  int *scan_tmp_ = nullptr;
   std::vector<InstMeta> meta_;
+  std::vector<int> ids_;
+  int next_id_ = 0;
   bool diag_ = false;
   bool any_boundary_ = false;
-  // capacities of the current allocation
-  int cap_n_ = 0, cap_nb_ = 0, cap_c_ = 0, cap_q_ = 0, cap_inst_ = 0, cap_blk_ = 0;
-  long cap_g_ = 0;
-  int *scan_tmp_ = nullptr;
-  int cap_scan_ = 0;
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
-  std::vector<void *> allocs_;
 };
 
 }  // namespace simp

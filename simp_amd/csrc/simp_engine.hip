@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "simp_kernels.h"
 
@@ -16,6 +17,8 @@ namespace simp {
   } while (0)
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------ host sizing
 
 // Host copy of the bucket_search grid size (nnps.f90:211-224) for sizing.
 static long grid_cells(int n, const double *x, const double *hsml, int nb, const double *xb,
@@ -38,6 +41,148 @@ static long grid_cells(int n, const double *x, const double *hsml, int nb, const
   return W * H;
 }
 
+static long cell_capacity(const InstanceInput &in) {
+  long g = grid_cells(in.n, in.x, in.hsml, in.nb, in.xb, 1.0);
+  if (g < 0) return -1;
+  // room for smoothing lengths shrinking to 70 % and a drifting domain
+  long g2 = grid_cells(in.n, in.x, in.hsml, in.nb, in.xb, 0.7);
+  return std::max(g2 * 2, g + 1024);
+}
+
+// Number of boundary pairs of the first evaluation (the boundary part of
+// bucket_search, nnps.f90:451-493), counted on the host to size the
+// hsml_ij store.
+static long boundary_pairs(const InstanceInput &in) {
+  if (in.nb == 0) return 0;
+  double hmin = in.hsml[0];
+  for (int i = 1; i < in.n; ++i) hmin = std::min(hmin, in.hsml[i]);
+  const double bin = 2 * hmin;
+  std::unordered_map<long long, std::vector<int>> cells;
+  auto key = [](long cx, long cy) { return (long long)cx * 2147483647LL + cy; };
+  for (int i = 0; i < in.n; ++i)
+    cells[key(std::lround(in.x[2 * i] / bin), std::lround(in.x[2 * i + 1] / bin))].push_back(i);
+  long total = 0;
+  for (int b = 0; b < in.nb; ++b) {
+    const double bx = in.xb[2 * b], by = in.xb[2 * b + 1];
+    const long cx = std::lround(bx / bin), cy = std::lround(by / bin);
+    for (long yy = cy - 1; yy <= cy + 1; ++yy)
+      for (long xx = cx - 1; xx <= cx + 1; ++xx) {
+        auto it = cells.find(key(xx, yy));
+        if (it == cells.end()) continue;
+        for (int i : it->second) {
+          const double dx = bx - in.x[2 * i], dy = by - in.x[2 * i + 1];
+          if (std::sqrt(dx * dx + dy * dy) <= kBLength) ++total;
+        }
+      }
+  }
+  return total;
+}
+
+static int store_capacity(const InstanceInput &in) {
+  if (in.nb == 0) return 0;
+  const long bp = boundary_pairs(in);
+  return (int)std::min<long>(2 * bp + 1024, 1L << 30);
+}
+
+// bytes per particle / boundary particle / cell of the device layout
+constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
+                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8;
+constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 8;
+constexpr size_t kBytesPerCell = 6 * 4;
+
+size_t instance_footprint(const InstanceInput &in) {
+  long c = cell_capacity(in);
+  if (c < 0) c = 0;
+  return kBytesPerParticle * (size_t)in.n + kBytesPerBoundary * (size_t)in.nb +
+         kBytesPerCell * (size_t)c + 8 * (size_t)store_capacity(in) +
+         16 * (size_t)in.g_nx * (size_t)in.g_ny + 4096;
+}
+
+// ------------------------------------------------------------ allocation
+
+template <class T>
+static int dalloc(std::vector<void *> &list, T **p, size_t count) {
+  if (count == 0) count = 1;
+  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return -6;
+  }
+  list.push_back(*p);
+  return 0;
+}
+
+static void free_list(std::vector<void *> &list) {
+  for (void *p : list) hipFree(p);
+  list.clear();
+}
+
+// Every buffer of a layout with the given totals (the hsml_ij store apart).
+static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, size_t N,
+                       size_t NB, size_t C, size_t BLK, size_t INST, size_t G) {
+  int rc = 0;
+  N = std::max<size_t>(N, 1);
+  NB = std::max<size_t>(NB, 1);
+  C = std::max<size_t>(C, 1) + 1;
+  for (int b = 0; b < 2; ++b) {
+    rc |= dalloc(list, &d.x[b], N);
+    rc |= dalloc(list, &d.y[b], N);
+    rc |= dalloc(list, &d.vx[b], N);
+    rc |= dalloc(list, &d.vy[b], N);
+    rc |= dalloc(list, &d.h[b], N);
+    rc |= dalloc(list, &d.A[b], N);
+    rc |= dalloc(list, &d.rho[b], N);
+    rc |= dalloc(list, &d.hsml[b], N);
+  }
+  rc |= dalloc(list, &d.mass, N);
+  rc |= dalloc(list, &d.orig, N);
+  rc |= dalloc(list, &d.inst, N);
+  rc |= dalloc(list, &d.cellid, N);
+  rc |= dalloc(list, &d.slot, N);
+  rc |= dalloc(list, &d.cl, N);
+  rc |= dalloc(list, &d.clpos, N);
+  rc |= dalloc(list, &d.own_cnt, N);
+  rc |= dalloc(list, &d.moff, N);
+  rc |= dalloc(list, &d.xs, N);
+  rc |= dalloc(list, &d.ys, N);
+  rc |= dalloc(list, &d.hs, N);
+  rc |= dalloc(list, &d.vxs, N);
+  rc |= dalloc(list, &d.vys, N);
+  rc |= dalloc(list, &d.ms, N);
+  rc |= dalloc(list, &d.rs, N);
+  rc |= dalloc(list, &d.cells, N);
+  rc |= dalloc(list, &d.pf, N);
+  rc |= dalloc(list, &d.nbr, (size_t)kNbrMax * N);
+  rc |= dalloc(list, &d.nnb, N);
+  rc |= dalloc(list, &d.part, 5 * BLK);
+  rc |= dalloc(list, &d.bflag, C);
+  rc |= dalloc(list, &d.count, C);
+  rc |= dalloc(list, &d.cell_start, C);
+  rc |= dalloc(list, &d.pair_count, C);
+  rc |= dalloc(list, &d.pair_start, C);
+  rc |= dalloc(list, &d.cell_intra, C);
+  double *sg = nullptr;
+  rc |= dalloc(list, &sg, N); d.sig11 = sg;
+  rc |= dalloc(list, &sg, N); d.sig12 = sg;
+  rc |= dalloc(list, &sg, N); d.sig22 = sg;
+  rc |= dalloc(list, &sg, N); d.div = sg;
+  rc |= dalloc(list, &d.diag, 6 * N);
+  rc |= dalloc(list, &d.sig_out, 3 * N);
+  rc |= dalloc(list, &d.tend, 6 * N);
+  rc |= dalloc(list, &d.xb, NB);
+  rc |= dalloc(list, &d.yb, NB);
+  rc |= dalloc(list, &d.massb, NB);
+  rc |= dalloc(list, &d.bpre, 9 * NB);
+  rc |= dalloc(list, &d.bbase, NB);
+  rc |= dalloc(list, &d.bcount, NB);
+  rc |= dalloc(list, &d.bsorted, NB);
+  rc |= dalloc(list, &d.grid, std::max<size_t>(G, 1));
+  rc |= dalloc(list, &d.meta, std::max<size_t>(INST, 1));
+  rc |= dalloc(list, &d.dyn, std::max<size_t>(INST, 1));
+  rc |= dalloc(list, &d.blk, std::max<size_t>(BLK, 1));
+  rc |= dalloc(list, scan_tmp, (size_t)cdiv((long)C + 1, kScanItems) + 1);
+  return rc ? -6 : 0;
+}
+
 Engine::Engine(int precision) : precision_(precision) {
   hipStreamCreate(&st_);
   hipStreamCreate(&side_);
@@ -46,9 +191,9 @@ Engine::Engine(int precision) : precision_(precision) {
 }
 
 Engine::~Engine() {
-  if (gexec_) hipGraphExecDestroy(gexec_);
-  if (graph_) hipGraphDestroy(graph_);
-  free_all();
+  graph_drop();
+  if (st_) hipStreamSynchronize(st_);
+  free_list(allocs_);
   if (d_.hq) hipFree(d_.hq);
   if (ev_fork_) hipEventDestroy(ev_fork_);
   if (ev_join_) hipEventDestroy(ev_join_);
@@ -56,300 +201,257 @@ Engine::~Engine() {
   if (st_) hipStreamDestroy(st_);
 }
 
-void Engine::free_all() {
-  for (void *p : allocs_) hipFree(p);
-  allocs_.clear();
-  if (scan_tmp_) hipFree(scan_tmp_);
-  scan_tmp_ = nullptr;
-  cap_n_ = cap_nb_ = cap_c_ = cap_inst_ = cap_blk_ = 0;
-  cap_g_ = 0;
-  cap_scan_ = 0;
-  // the hsml_ij store is owned separately (kept across uploads)
+void Engine::graph_drop() {
+  if (gexec_) hipGraphExecDestroy(gexec_);
+  if (graph_) hipGraphDestroy(graph_);
+  gexec_ = nullptr;
+  graph_ = nullptr;
 }
 
-template <class T>
-static int dalloc(std::vector<void *> &list, T **p, size_t count) {
-  if (count == 0) count = 1;
-  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) return -6;
-  list.push_back(*p);
-  return 0;
+int Engine::index_of(int id) const {
+  for (size_t k = 0; k < ids_.size(); ++k)
+    if (ids_[k] == id) return (int)k;
+  return -1;
 }
 
-int Engine::ensure(int ntot, int nbtot, int ctot, int qtot, int ninst, int nblk, long gtot) {
-  int rc = 0;
-  if (ntot > cap_n_ || nbtot > cap_nb_ || ctot > cap_c_ || ninst > cap_inst_ ||
-      nblk > cap_blk_ || gtot > cap_g_) {
-    double *hq_keep = d_.hq;
-    free_all();
-    d_.hq = hq_keep;
-    cap_n_ = std::max(ntot, 1);
-    cap_nb_ = std::max(nbtot, 1);
-    cap_c_ = std::max(ctot, 1);
-    cap_inst_ = std::max(ninst, 1);
-    cap_blk_ = std::max(nblk, 1);
-    cap_g_ = std::max(gtot, 1L);
-    const size_t N = cap_n_, NB = cap_nb_, C = cap_c_ + 1;
-    for (int b = 0; b < 2; ++b) {
-      rc |= dalloc(allocs_, &d_.x[b], N);
-      rc |= dalloc(allocs_, &d_.y[b], N);
-      rc |= dalloc(allocs_, &d_.vx[b], N);
-      rc |= dalloc(allocs_, &d_.vy[b], N);
-      rc |= dalloc(allocs_, &d_.h[b], N);
-      rc |= dalloc(allocs_, &d_.A[b], N);
-      rc |= dalloc(allocs_, &d_.rho[b], N);
-      rc |= dalloc(allocs_, &d_.hsml[b], N);
-    }
-    rc |= dalloc(allocs_, &d_.mass, N);
-    rc |= dalloc(allocs_, &d_.orig, N);
-    rc |= dalloc(allocs_, &d_.inst, N);
-    rc |= dalloc(allocs_, &d_.cellid, N);
-    rc |= dalloc(allocs_, &d_.slot, N);
-    rc |= dalloc(allocs_, &d_.cl, N);
-    rc |= dalloc(allocs_, &d_.clpos, N);
-    rc |= dalloc(allocs_, &d_.own_cnt, N);
-    rc |= dalloc(allocs_, &d_.moff, N);
-    rc |= dalloc(allocs_, &d_.xs, N);
-    rc |= dalloc(allocs_, &d_.ys, N);
-    rc |= dalloc(allocs_, &d_.hs, N);
-    rc |= dalloc(allocs_, &d_.vxs, N);
-    rc |= dalloc(allocs_, &d_.vys, N);
-    rc |= dalloc(allocs_, &d_.ms, N);
-    rc |= dalloc(allocs_, &d_.rs, N);
-    rc |= dalloc(allocs_, &d_.cells, N);
-    rc |= dalloc(allocs_, &d_.pf, N);
-    rc |= dalloc(allocs_, &d_.nbr, (size_t)kNbrMax * N);
-    rc |= dalloc(allocs_, &d_.nnb, N);
-    rc |= dalloc(allocs_, &d_.part, 5 * (size_t)cap_blk_);
-    rc |= dalloc(allocs_, &d_.bflag, C);
-    rc |= dalloc(allocs_, &d_.count, C);
-    rc |= dalloc(allocs_, &d_.cell_start, C);
-    rc |= dalloc(allocs_, &d_.pair_count, C);
-    rc |= dalloc(allocs_, &d_.pair_start, C);
-    rc |= dalloc(allocs_, &d_.cell_intra, C);
-    double *sg = nullptr;
-    rc |= dalloc(allocs_, &sg, N); d_.sig11 = sg;
-    rc |= dalloc(allocs_, &sg, N); d_.sig12 = sg;
-    rc |= dalloc(allocs_, &sg, N); d_.sig22 = sg;
-    rc |= dalloc(allocs_, &sg, N); d_.div = sg;
-    rc |= dalloc(allocs_, &d_.diag, 6 * N);
-    rc |= dalloc(allocs_, &d_.sig_out, 3 * N);
-    rc |= dalloc(allocs_, &d_.tend, 6 * N);
-    rc |= dalloc(allocs_, &d_.xb, NB);
-    rc |= dalloc(allocs_, &d_.yb, NB);
-    rc |= dalloc(allocs_, &d_.massb, NB);
-    rc |= dalloc(allocs_, &d_.bpre, 9 * NB);
-    rc |= dalloc(allocs_, &d_.bbase, NB);
-    rc |= dalloc(allocs_, &d_.bcount, NB);
-    rc |= dalloc(allocs_, &d_.bsorted, NB);
-    rc |= dalloc(allocs_, &d_.grid, (size_t)cap_g_);
-    rc |= dalloc(allocs_, &d_.meta, (size_t)cap_inst_);
-    rc |= dalloc(allocs_, &d_.dyn, (size_t)cap_inst_);
-    rc |= dalloc(allocs_, &d_.blk, (size_t)cap_blk_);
-    cap_scan_ = cdiv(C + 1, kScanItems) + 1;
-    if (hipMalloc((void **)&scan_tmp_, cap_scan_ * sizeof(int)) != hipSuccess) rc = -6;
-    if (rc) return -6;
-    if (hipMemsetAsync(d_.count, 0, C * sizeof(int), st_) != hipSuccess) return -2;
-    if (gexec_) { hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
-    if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
-  }
-  if (qtot > cap_q_) {
-    double *nq = nullptr;
-    if (hipMalloc((void **)&nq, (size_t)qtot * sizeof(double)) != hipSuccess) return -6;
-    if (hipMemsetAsync(nq, 0, (size_t)qtot * sizeof(double), st_) != hipSuccess) return -2;
-    if (d_.hq && cap_q_ > 0)
-      if (hipMemcpyAsync(nq, d_.hq, (size_t)cap_q_ * sizeof(double), hipMemcpyDeviceToDevice,
-                         st_) != hipSuccess)
-        return -2;
-    if (hipStreamSynchronize(st_) != hipSuccess) return -2;
-    if (d_.hq) hipFree(d_.hq);
-    d_.hq = nq;
-    cap_q_ = qtot;
-    if (gexec_) { hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
-    if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
-  }
-  return 0;
-}
+// ----------------------------------------------------------------- repack
 
 int Engine::upload(const std::vector<InstanceInput> &ins, bool keep_store) {
-  const int ninst = (int)ins.size();
-  if (ninst < 1) return -1;
-  std::vector<InstMeta> meta(ninst);
-  int ntot = 0, nbtot = 0, ctot = 0, qtot = 0, nblk = 0;
-  long gtot = 0;
-  any_boundary_ = false;
-  for (int k = 0; k < ninst; ++k) {
-    const InstanceInput &in = ins[k];
+  const int carry = (keep_store && ins.size() == 1 && !meta_.empty()) ? 0 : -1;
+  return repack({}, ins, carry, nullptr);
+}
+
+int Engine::add(const std::vector<InstanceInput> &ins, std::vector<int> *ids) {
+  std::vector<int> keep(meta_.size());
+  for (size_t k = 0; k < keep.size(); ++k) keep[k] = (int)k;
+  return repack(keep, ins, -1, ids);
+}
+
+int Engine::remove(const std::vector<int> &ids) {
+  std::vector<int> keep;
+  for (size_t k = 0; k < meta_.size(); ++k)
+    if (std::find(ids.begin(), ids.end(), ids_[k]) == ids.end()) keep.push_back((int)k);
+  return repack(keep, {}, -1, nullptr);
+}
+
+// Rebuild the packed layout from the kept instances (old packed indices,
+// their live state copied device to device) followed by the added ones
+// (copied from the host).  carry_store_from: old index whose hsml_ij store
+// seeds the first added instance's store.
+int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput> &add,
+                   int carry_store_from, std::vector<int> *new_ids) {
+  const int nk = (int)keep.size(), na = (int)add.size(), ninst = nk + na;
+  std::vector<InstMeta> nm(ninst);
+  std::vector<int> nids(ninst);
+  for (int j = 0; j < na; ++j) {
+    const InstanceInput &in = add[j];
     if (in.n < 1 || in.nb < 0 || in.nb > kMaxBoundaryPerInst) return -1;
     if (!in.x || !in.vx || !in.mass || !in.rho || !in.hsml || !in.hth || !in.A) return -1;
     if (in.nb > 0 && (!in.xb || !in.massb)) return -1;
-    InstMeta &m = meta[k];
+    if (in.wind_mode == 1 && (in.g_nx < 2 || in.g_ny < 2 || !in.g_u || !in.g_v)) return -1;
+  }
+  // sizes and forcing
+  for (int j = 0; j < nk; ++j) {
+    if (keep[j] < 0 || keep[j] >= (int)meta_.size()) return -1;
+    nm[j] = meta_[keep[j]];
+    nids[j] = ids_[keep[j]];
+  }
+  for (int j = 0; j < na; ++j) {
+    const InstanceInput &in = add[j];
+    InstMeta &m = nm[nk + j];
     std::memset(&m, 0, sizeof(m));
-    m.p_off = ntot; m.n = in.n;
-    m.blk_off = nblk; m.nblk = cdiv(in.n, kTPB);
-    m.b_off = nbtot; m.nb = in.nb;
-    long cells = in.c_cap;
-    if (cells <= 0) {
-      long g = grid_cells(in.n, in.x, in.hsml, in.nb, in.xb, 1.0);
-      if (g < 0) return -4;
-      // room for smoothing lengths shrinking to 70 % and a drifting domain
-      long g2 = grid_cells(in.n, in.x, in.hsml, in.nb, in.xb, 0.7);
-      cells = std::max(g2 * 2, g + 1024);
-    }
+    m.n = in.n;
+    m.nb = in.nb;
+    const long cells = cell_capacity(in);
+    if (cells < 0) return -4;
     if (cells > (1L << 28)) return -3;
-    m.c_off = ctot; m.c_cap = (int)cells;
-    int q = in.q_cap > 0 ? in.q_cap : (in.nb > 0 ? std::max(64 * in.nb, 1024) : 0);
-    m.q_off = qtot; m.q_cap = q;
+    m.c_cap = (int)cells;
+    m.q_cap = store_capacity(in);
+    if (j == 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size())
+      m.q_cap = std::max(m.q_cap, meta_[carry_store_from].q_cap);
     m.tse = in.tse;
     m.wind_mode = in.wind_mode;
     m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
     m.cur_u = in.current[0]; m.cur_v = in.current[1];
     m.max_hsml = in.max_hsml;
     if (in.wind_mode == 1) {
-      if (in.g_nx < 2 || in.g_ny < 2 || !in.g_u || !in.g_v) return -1;
-      m.g_off = (int)gtot; m.g_nx = in.g_nx; m.g_ny = in.g_ny;
+      m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
-      gtot += 2L * in.g_nx * in.g_ny;
     }
     m.bxmin = m.bymin = 1e300;
     m.bxmax = m.bymax = -1e300;
     for (int b = 0; b < in.nb; ++b) {
       m.bxmin = std::min(m.bxmin, in.xb[2 * b]); m.bxmax = std::max(m.bxmax, in.xb[2 * b]);
-      m.bymin = std::min(m.bymin, in.xb[2 * b + 1]); m.bymax = std::max(m.bymax, in.xb[2 * b + 1]);
+      m.bymin = std::min(m.bymin, in.xb[2 * b + 1]);
+      m.bymax = std::max(m.bymax, in.xb[2 * b + 1]);
     }
-    if (in.nb > 0) any_boundary_ = true;
-    ntot += in.n; nbtot += in.nb; ctot += m.c_cap; qtot += q;
-    nblk += cdiv(in.n, kTPB);
+    nids[nk + j] = next_id_++;
   }
-  bool layout_same = keep_store && meta_.size() == meta.size();
-  if (layout_same)
-    for (int k = 0; k < ninst; ++k)
-      if (meta_[k].q_off != meta[k].q_off || meta_[k].q_cap > meta[k].q_cap) layout_same = false;
-  if (layout_same) {
-    qtot = 0;
-    for (int k = 0; k < ninst; ++k) {
-      meta[k].q_cap = std::max(meta[k].q_cap, meta_[k].q_cap);
-      meta[k].q_off = qtot;
-      qtot += meta[k].q_cap;
-    }
-  }
-  int rc = ensure(ntot, nbtot, ctot, qtot, ninst, nblk, gtot);
-  if (rc) return rc;
-  if (!layout_same && d_.hq)
-    HIPCHK(hipMemsetAsync(d_.hq, 0, (size_t)cap_q_ * sizeof(double), st_));
-
-  // host staging in SoA
-  std::vector<double> hx(ntot), hy(ntot), hvx(ntot), hvy(ntot), hm(ntot), hr(ntot), hs(ntot),
-      hh(ntot), hA(ntot);
-  std::vector<int> horig(ntot), hinst(ntot);
-  std::vector<double> hxb(std::max(nbtot, 1)), hyb(std::max(nbtot, 1)), hmb(std::max(nbtot, 1));
-  std::vector<double> hgrid(std::max(gtot, 1L));
-  std::vector<BlockEntry> hblk;
-  std::vector<InstDyn> hdyn(ninst);
+  // offsets of the new layout
+  long ntot = 0, nbtot = 0, ctot = 0, qtot = 0, gtot = 0, nblk = 0;
   for (int k = 0; k < ninst; ++k) {
-    const InstanceInput &in = ins[k];
-    const InstMeta &m = meta[k];
-    for (int i = 0; i < in.n; ++i) {
-      int p = m.p_off + i;
-      hx[p] = in.x[2 * i]; hy[p] = in.x[2 * i + 1];
-      hvx[p] = in.vx[2 * i]; hvy[p] = in.vx[2 * i + 1];
-      hm[p] = in.mass[i]; hr[p] = in.rho[i]; hs[p] = in.hsml[i];
-      hh[p] = in.hth[i]; hA[p] = in.A[i];
-      horig[p] = i; hinst[p] = k;
+    InstMeta &m = nm[k];
+    m.p_off = (int)ntot; m.b_off = (int)nbtot; m.c_off = (int)ctot; m.q_off = (int)qtot;
+    m.blk_off = (int)nblk; m.nblk = cdiv(m.n, kTPB);
+    m.g_off = (int)gtot;
+    ntot += m.n; nbtot += m.nb; ctot += m.c_cap; qtot += m.q_cap;
+    nblk += m.nblk;
+    if (m.wind_mode == 1) gtot += 2L * m.g_nx * m.g_ny;
+  }
+  if (ntot > INT32_MAX / kNbrMax || ctot > INT32_MAX - 2 || qtot > INT32_MAX) return -3;
+
+  // new buffers
+  DevState nd{};
+  std::vector<void *> nlist;
+  int *nscan = nullptr;
+  int rc = alloc_state(nd, nlist, &nscan, ntot, nbtot, ctot, nblk, ninst, gtot);
+  if (rc == 0 && hipMalloc((void **)&nd.hq, std::max<long>(qtot, 1) * sizeof(double)) != hipSuccess)
+    rc = -6;
+  if (rc) {
+    free_list(nlist);
+    if (nscan) hipFree(nscan);
+    if (nd.hq) hipFree(nd.hq);
+    return rc;
+  }
+  auto fail = [&](int code) {
+    hipStreamSynchronize(st_);
+    free_list(nlist);
+    hipFree(nscan);
+    hipFree(nd.hq);
+    return code;
+  };
+#define RCHK(x) \
+  do {          \
+    if ((x) != hipSuccess) return fail(-2); \
+  } while (0)
+  RCHK(hipMemsetAsync(nd.hq, 0, std::max<long>(qtot, 1) * sizeof(double), st_));
+  RCHK(hipMemsetAsync(nd.count, 0, (ctot + 1) * sizeof(int), st_));
+  RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
+
+  // kept instances: device to device
+  auto d2d = [&](void *dst, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st_) : hipSuccess;
+  };
+  for (int j = 0; j < nk; ++j) {
+    const InstMeta &o = meta_[keep[j]];
+    const InstMeta &m = nm[j];
+    const size_t n8 = (size_t)m.n * 8, n4 = (size_t)m.n * 4;
+    RCHK(d2d(nd.x[0] + m.p_off, d_.x[0] + o.p_off, n8));
+    RCHK(d2d(nd.y[0] + m.p_off, d_.y[0] + o.p_off, n8));
+    RCHK(d2d(nd.vx[0] + m.p_off, d_.vx[0] + o.p_off, n8));
+    RCHK(d2d(nd.vy[0] + m.p_off, d_.vy[0] + o.p_off, n8));
+    RCHK(d2d(nd.h[0] + m.p_off, d_.h[0] + o.p_off, n8));
+    RCHK(d2d(nd.A[0] + m.p_off, d_.A[0] + o.p_off, n8));
+    RCHK(d2d(nd.rho[0] + m.p_off, d_.rho[0] + o.p_off, n8));
+    RCHK(d2d(nd.hsml[0] + m.p_off, d_.hsml[0] + o.p_off, n8));
+    RCHK(d2d(nd.mass + m.p_off, d_.mass + o.p_off, n8));
+    RCHK(d2d(nd.orig + m.p_off, d_.orig + o.p_off, n4));
+    RCHK(d2d(nd.xb + m.b_off, d_.xb + o.b_off, (size_t)m.nb * 8));
+    RCHK(d2d(nd.yb + m.b_off, d_.yb + o.b_off, (size_t)m.nb * 8));
+    RCHK(d2d(nd.massb + m.b_off, d_.massb + o.b_off, (size_t)m.nb * 8));
+    RCHK(d2d(nd.hq + m.q_off, d_.hq + o.q_off, (size_t)m.q_cap * 8));
+    if (m.wind_mode == 1)
+      RCHK(d2d(nd.grid + m.g_off, d_.grid + o.g_off, 16 * (size_t)m.g_nx * m.g_ny));
+    RCHK(d2d(nd.dyn + j, d_.dyn + keep[j], sizeof(InstDyn)));
+  }
+  if (na > 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size() && d_.hq) {
+    const InstMeta &o = meta_[carry_store_from];
+    RCHK(d2d(nd.hq + nm[nk].q_off, d_.hq + o.q_off, (size_t)std::min(o.q_cap, nm[nk].q_cap) * 8));
+  }
+
+  // added instances: host to device (staged in SoA)
+  std::vector<std::vector<double>> keepalive;
+  std::vector<std::vector<int>> keepalive_i;
+  auto h2d = [&](void *dst, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st_) : hipSuccess;
+  };
+  std::vector<InstDyn> ndyn(na);
+  for (int j = 0; j < na; ++j) {
+    const InstanceInput &in = add[j];
+    const InstMeta &m = nm[nk + j];
+    const int n = in.n;
+    keepalive.emplace_back(12 * (size_t)n + 3 * (size_t)in.nb);
+    std::vector<double> &st = keepalive.back();
+    double *hx = st.data(), *hy = hx + n, *hvx = hy + n, *hvy = hvx + n;
+    for (int i = 0; i < n; ++i) {
+      hx[i] = in.x[2 * i]; hy[i] = in.x[2 * i + 1];
+      hvx[i] = in.vx[2 * i]; hvy[i] = in.vx[2 * i + 1];
     }
+    double *hxb = hvy + n, *hyb = hxb + in.nb;
     for (int b = 0; b < in.nb; ++b) {
-      hxb[m.b_off + b] = in.xb[2 * b];
-      hyb[m.b_off + b] = in.xb[2 * b + 1];
-      hmb[m.b_off + b] = in.massb[b];
+      hxb[b] = in.xb[2 * b];
+      hyb[b] = in.xb[2 * b + 1];
+    }
+    keepalive_i.emplace_back(n);
+    for (int i = 0; i < n; ++i) keepalive_i.back()[i] = i;
+    const size_t n8 = (size_t)n * 8;
+    RCHK(h2d(nd.x[0] + m.p_off, hx, n8));
+    RCHK(h2d(nd.y[0] + m.p_off, hy, n8));
+    RCHK(h2d(nd.vx[0] + m.p_off, hvx, n8));
+    RCHK(h2d(nd.vy[0] + m.p_off, hvy, n8));
+    RCHK(h2d(nd.h[0] + m.p_off, in.hth, n8));
+    RCHK(h2d(nd.A[0] + m.p_off, in.A, n8));
+    RCHK(h2d(nd.rho[0] + m.p_off, in.rho, n8));
+    RCHK(h2d(nd.hsml[0] + m.p_off, in.hsml, n8));
+    RCHK(h2d(nd.mass + m.p_off, in.mass, n8));
+    RCHK(h2d(nd.orig + m.p_off, keepalive_i.back().data(), (size_t)n * 4));
+    if (in.nb > 0) {
+      RCHK(h2d(nd.xb + m.b_off, hxb, (size_t)in.nb * 8));
+      RCHK(h2d(nd.yb + m.b_off, hyb, (size_t)in.nb * 8));
+      RCHK(h2d(nd.massb + m.b_off, in.massb, (size_t)in.nb * 8));
     }
     if (in.wind_mode == 1) {
-      long nn = (long)in.g_nx * in.g_ny;
-      std::memcpy(&hgrid[m.g_off], in.g_u, nn * sizeof(double));
-      std::memcpy(&hgrid[m.g_off + nn], in.g_v, nn * sizeof(double));
+      const size_t gg = (size_t)in.g_nx * in.g_ny;
+      RCHK(h2d(nd.grid + m.g_off, in.g_u, gg * 8));
+      RCHK(h2d(nd.grid + m.g_off + gg, in.g_v, gg * 8));
     }
-    for (int f = 0; f < in.n; f += kTPB) hblk.push_back(BlockEntry{k, f, (int)hblk.size(), 0});
-    InstDyn &y = hdyn[k];
+    InstDyn &y = ndyn[j];
     std::memset(&y, 0, sizeof(y));
-    y.dt = in.dt; y.time = in.time; y.itimestep = in.itimestep;
+    y.dt = in.dt;
+    y.time = in.time;
+    y.itimestep = in.itimestep;
+    y.active = 1;
+    y.steps_left = in.nsteps > 0 ? in.nsteps : -1;
   }
-  auto up = [&](void *dst, const void *src, size_t bytes) {
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st_);
-  };
-  const size_t N8 = (size_t)ntot * 8;
-  HIPCHK(up(d_.x[0], hx.data(), N8));
-  HIPCHK(up(d_.y[0], hy.data(), N8));
-  HIPCHK(up(d_.vx[0], hvx.data(), N8));
-  HIPCHK(up(d_.vy[0], hvy.data(), N8));
-  HIPCHK(up(d_.mass, hm.data(), N8));
-  HIPCHK(up(d_.rho[0], hr.data(), N8));
-  HIPCHK(up(d_.hsml[0], hs.data(), N8));
-  HIPCHK(up(d_.h[0], hh.data(), N8));
-  HIPCHK(up(d_.A[0], hA.data(), N8));
-  HIPCHK(up(d_.orig, horig.data(), (size_t)ntot * 4));
-  HIPCHK(up(d_.inst, hinst.data(), (size_t)ntot * 4));
-  if (nbtot > 0) {
-    HIPCHK(up(d_.xb, hxb.data(), (size_t)nbtot * 8));
-    HIPCHK(up(d_.yb, hyb.data(), (size_t)nbtot * 8));
-    HIPCHK(up(d_.massb, hmb.data(), (size_t)nbtot * 8));
-  }
-  if (gtot > 0) HIPCHK(up(d_.grid, hgrid.data(), (size_t)gtot * 8));
-  HIPCHK(up(d_.meta, meta.data(), sizeof(InstMeta) * ninst));
-  HIPCHK(up(d_.dyn, hdyn.data(), sizeof(InstDyn) * ninst));
+  if (na > 0) RCHK(h2d(nd.dyn + nk, ndyn.data(), sizeof(InstDyn) * na));
+  RCHK(h2d(nd.meta, nm.data(), sizeof(InstMeta) * ninst));
   // Workgroups are dispatched to the 8 XCDs round-robin by block id; the
   // physical table gives each XCD a contiguous run of logical blocks.
+  std::vector<BlockEntry> lg;
+  for (int k = 0; k < ninst; ++k)
+    for (int f = 0; f < nm[k].n; f += kTPB) lg.push_back(BlockEntry{k, f, (int)lg.size(), 0});
+  std::vector<BlockEntry> phys(lg.size());
   {
-    const int nb_ = (int)hblk.size(), q = nb_ / 8, r = nb_ % 8;
-    std::vector<BlockEntry> phys(nb_);
+    const int nb_ = (int)lg.size(), q = nb_ / 8, r = nb_ % 8;
     for (int b = 0; b < nb_; ++b) {
       const int x = b % 8;
-      phys[b] = hblk[x * q + std::min(x, r) + b / 8];
+      phys[b] = lg[x * q + std::min(x, r) + b / 8];
     }
-    HIPCHK(up(d_.blk, phys.data(), sizeof(BlockEntry) * nb_));
   }
-  HIPCHK(hipMemsetAsync(d_.bflag, 0xff, sizeof(int) * ((size_t)ctot + 1), st_));
-  HIPCHK(hipStreamSynchronize(st_));
-  d_.ntot = ntot; d_.nbtot = nbtot; d_.ctot = ctot; d_.qtot = qtot; d_.ninst = ninst;
-  d_.nblk = (int)hblk.size();
-  meta_ = meta;
-  if (gexec_) { hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
-  if (graph_) { hipGraphDestroy(graph_); graph_ = nullptr; }
-  // grid of the first evaluation
-  launch_partial(d_, 0, st_);
-  launch_finalize(d_, 0, st_);
+  if (!phys.empty()) RCHK(h2d(nd.blk, phys.data(), sizeof(BlockEntry) * phys.size()));
+  RCHK(hipStreamSynchronize(st_));
+#undef RCHK
 
-  // size the boundary-pair stores from a probe of the first evaluation
-  if (any_boundary_) {
-    std::vector<int> bnip;
-    rc = probe_bnip(bnip);
-    if (rc) return rc;
-    bool grow = false;
-    for (int k = 0; k < ninst; ++k)
-      if (bnip[k] > meta_[k].q_cap) grow = true;
-    if (grow) {
-      int q = 0;
-      std::vector<double> old_hq;
-      for (int k = 0; k < ninst; ++k) {
-        int cap = std::max(meta_[k].q_cap, bnip[k] > meta_[k].q_cap ? 2 * bnip[k] + 1024 : 0);
-        meta_[k].q_off = q;
-        meta_[k].q_cap = cap;
-        q += cap;
-      }
-      if (ninst == 1) {
-        rc = ensure(ntot, nbtot, ctot, q, ninst, nblk, gtot);  // keeps old entries
-      } else {
-        hipFree(d_.hq);
-        d_.hq = nullptr;
-        cap_q_ = 0;
-        rc = ensure(ntot, nbtot, ctot, q, ninst, nblk, gtot);
-      }
-      if (rc) return rc;
-      d_.qtot = q;
-      HIPCHK(up(d_.meta, meta_.data(), sizeof(InstMeta) * ninst));
-    }
-    // the probe may have flagged the store capacity: restart clean
-    HIPCHK(up(d_.dyn, hdyn.data(), sizeof(InstDyn) * ninst));
+  // swap in the new layout
+  graph_drop();
+  free_list(allocs_);
+  if (scan_tmp_) hipFree(scan_tmp_);
+  if (d_.hq) hipFree(d_.hq);
+  d_ = nd;
+  allocs_.swap(nlist);
+  scan_tmp_ = nscan;
+  d_.ntot = (int)ntot; d_.nbtot = (int)nbtot; d_.ctot = (int)ctot; d_.qtot = (int)qtot;
+  d_.ninst = ninst;
+  d_.nblk = (int)lg.size();
+  meta_ = nm;
+  ids_ = nids;
+  any_boundary_ = nbtot > 0;
+  if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
+  if (ninst > 0) {
+    // grid of the next evaluation of every instance
     launch_partial(d_, 0, st_);
     launch_finalize(d_, 0, st_);
-    HIPCHK(hipStreamSynchronize(st_));
+    if (hipStreamSynchronize(st_) != hipSuccess) return -2;
   }
   return 0;
 }
@@ -411,19 +513,6 @@ void Engine::enqueue_step() {
   L("finalize", [&] { launch_finalize(d_, 1, st_); });
 }
 
-int Engine::probe_bnip(std::vector<int> &bnip) {
-  enqueue_prefix(0, false);
-  HIPCHK(hipStreamSynchronize(st_));
-  std::vector<InstDyn> hd(d_.ninst);
-  HIPCHK(hipMemcpy(hd.data(), d_.dyn, sizeof(InstDyn) * d_.ninst, hipMemcpyDeviceToHost));
-  bnip.resize(d_.ninst);
-  for (int k = 0; k < d_.ninst; ++k) {
-    if (hd[k].err & ~kErrCapacity) return (hd[k].err & kErrNan) ? -4 : -3;
-    bnip[k] = hd[k].bnip;
-  }
-  return 0;
-}
-
 int Engine::eval_single(bool want_diag) {
   enqueue_eval(0, 2, want_diag);
   return sync_and_check();
@@ -431,6 +520,7 @@ int Engine::eval_single(bool want_diag) {
 
 int Engine::graph_build() {
   if (gexec_) return 0;
+  if (d_.ninst == 0) return 0;
   HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
   enqueue_step();
   HIPCHK(hipStreamEndCapture(st_, &graph_));
@@ -440,6 +530,7 @@ int Engine::graph_build() {
 
 int Engine::steps(int nsteps, bool use_graph) {
   if (nsteps < 0) return -1;
+  if (d_.ninst == 0) return 0;
   if (use_graph && !prof) {
     int rc = graph_build();
     if (rc) return rc;
@@ -479,6 +570,17 @@ int Engine::download_scalars(int k, double *dt, double *time, int *itimestep, in
     HIPCHK(hipMemcpy(&b, d_.pair_start + m.c_off + m.c_cap, 4, hipMemcpyDeviceToHost));
     *nip = b - a;
   }
+  return 0;
+}
+
+int Engine::download_status(int k, int *itimestep, int *steps_left, int *active, int *err) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  InstDyn y;
+  HIPCHK(hipMemcpy(&y, d_.dyn + k, sizeof(y), hipMemcpyDeviceToHost));
+  if (itimestep) *itimestep = y.itimestep;
+  if (steps_left) *steps_left = y.steps_left;
+  if (active) *active = y.active;
+  if (err) *err = y.err;
   return 0;
 }
 

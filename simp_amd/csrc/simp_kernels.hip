@@ -143,6 +143,9 @@ __device__ int block_sum(int v) {
   return tot;
 }
 
+// An instance is skipped by every kernel once it has an error or is frozen.
+__device__ __forceinline__ bool inst_off(const InstDyn &y) { return y.err != 0 || y.active == 0; }
+
 // ------------------------------------------------------ reductions / grid
 
 // Per-block min/max of x, y and min of hsml (invalid lanes give neutral
@@ -215,13 +218,17 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
     v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
   }
   double xmn = sh[0][0], xmx = sh[1][0], ymn = sh[2][0], ymx = sh[3][0], hmn = sh[4][0];
-  if (y.err) return;
+  if (inst_off(y)) return;
   if (end_of_step) {
     // VP_time_step: MINVAL(rho_ice hsml^2 delta_min / (P_star (1 + k_t))) is
     // the same expression at MINVAL(hsml), each operation being monotone
     if (m.tse == 2) y.dt = kRhoIce * (hmn * hmn) * kDeltaMin / (kPStar * (1 + kKT));
     y.itimestep += 1;
     y.time = y.time + y.dt;
+    if (y.steps_left > 0 && --y.steps_left == 0) {
+      y.active = 0;   // finished: frozen from the next evaluation on
+      return;
+    }
   }
   if (m.nb > 0) {
     xmn = fmin(xmn, m.bxmin); xmx = fmax(xmx, m.bxmax);
@@ -254,7 +261,7 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
   const InstDyn &y = d.dyn[k];
-  if (y.err) return;
+  if (inst_off(y)) return;
   const double bin = y.bin;
   const int cx = nint(d.x[buf][p] / bin) - y.xlo + 1;
   const int cy = nint(d.y[buf][p] / bin) - y.ylo + 1;
@@ -313,7 +320,7 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
-  if (d.dyn[k].err) return;
+  if (inst_off(d.dyn[k])) return;
   d.cl[d.cell_start[d.meta[k].c_off + d.cellid[p]] + d.slot[p]] = p;
 }
 
@@ -373,7 +380,7 @@ __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
   const int local = e.first + (int)threadIdx.x;
   if (local >= m.n) return;
   const InstDyn &y = d.dyn[k];
-  if (y.err) return;
+  if (inst_off(y)) return;
   const int s = d.cell_start[m.c_off] + local;
   const int q = d.cl[s];
   d.clpos[q] = s;
@@ -405,7 +412,7 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
   const InstDyn &y = d.dyn[k];
-  if (m.nb == 0 || y.err) return;
+  if (m.nb == 0 || inst_off(y)) return;
   const double bin = y.bin;
   const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
   const int *cs = d.cell_start + m.c_off;
@@ -609,7 +616,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   int self = 0, k = 0;
   if (!list_slot(d, self, k)) return;
   const InstDyn &y = d.dyn[k];
-  if (y.err) return;
+  if (inst_off(y)) return;
   const InstMeta &m = d.meta[k];
   const int W = y.W;
   const int c = d.cells[self];
@@ -828,7 +835,7 @@ __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
   const int k = blockIdx.x / kMatBlocks;
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
-  if (m.nb == 0 || y.err) return;
+  if (m.nb == 0 || inst_off(y)) return;
   const int pbase = d.pair_start[m.c_off];
   const int nipk = d.pair_start[m.c_off + m.c_cap] - pbase;
   const int lim = min(min(y.bnip, m.q_cap), nipk);
@@ -913,7 +920,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
   const bool ok = list_slot(d, self, k);
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
-  const bool live = ok && !y.err;
+  const bool live = ok && !inst_off(y);
   const int buf = phase == 1 ? 1 : 0;
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   if (live) {
@@ -1161,7 +1168,7 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   if (!particle_of(d, p, k)) return;
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
-  if (y.err) return;
+  if (inst_off(y)) return;
   const int oc = d.own_cnt[p];
   if ((oc & 0xfffff) == 0) return;
   const int c = d.cellid[p];
@@ -1201,7 +1208,7 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
   const InstMeta &m = d.meta[k];
   const InstDyn &y = d.dyn[k];
   int b = blockIdx.x * kTPB + threadIdx.x;
-  if (b >= m.nb || y.err) return;
+  if (b >= m.nb || inst_off(y)) return;
   const int bg = m.b_off + b;
   const double bin = y.bin;
   const int W = y.W;

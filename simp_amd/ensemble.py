@@ -1,0 +1,126 @@
+"""Many independent simulations stepped together in one packed launch.
+
+The reference runs one simulation per process (``modules/globals.f90:21-25``);
+an :class:`Ensemble` packs instances of uneven size, domain and forcing into
+one set of device buffers with per-instance offsets, so every kernel of a
+step covers all of them.  Instances can be added and removed between steps
+(the layout is rebuilt on the device); each instance carries its own step
+budget and freezes when it is used up or when it fails, without affecting
+the others.  Results are identical to running each instance alone (the
+kernels never mix instances; see ``tests/test_gpu_ensemble.py``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import SimpInstanceDesc, check, lib
+
+
+def _desc(sc, nsteps: int, keep: list) -> SimpInstanceDesc:
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (sc.x, sc.vx, sc.mass, sc.rho, sc.hsml, sc.h, sc.A, sc.xb, sc.massb)]
+    keep.extend(arrs)
+    d = SimpInstanceDesc()
+    d.n, d.nb = sc.n, sc.nb
+    (d.x, d.vx, d.mass, d.rho, d.hsml, d.hth, d.A, d.xb, d.massb) = [
+        a.ctypes.data if a.size else None for a in arrs]
+    d.dt, d.current_time = sc.dt, sc.time
+    d.itimestep = 0
+    d.nsteps = int(nsteps)
+    d.tse = sc.tse
+    d.wind[0], d.wind[1] = sc.wind
+    d.current[0], d.current[1] = sc.current
+    d.max_hsml = sc.max_hsml
+    if sc.wind_grid is not None:
+        g = sc.wind_grid
+        u = np.ascontiguousarray(g.u, dtype=np.float64)
+        v = np.ascontiguousarray(g.v, dtype=np.float64)
+        keep.extend([u, v])
+        d.wind_mode = _lib.SIMP_WIND_GRID
+        d.grid_ny, d.grid_nx = u.shape
+        d.grid_x0, d.grid_y0, d.grid_dx, d.grid_dy = g.x0, g.y0, g.dx, g.dy
+        d.grid_u, d.grid_v = u.ctypes.data, v.ctypes.data
+    else:
+        d.wind_mode = _lib.SIMP_WIND_UNIFORM
+    return d
+
+
+def footprint(sc) -> int:
+    """Device bytes the scenario would occupy inside an ensemble."""
+    keep: list = []
+    d = _desc(sc, 0, keep)
+    return int(lib().simp_instance_footprint(C.byref(d)))
+
+
+class Ensemble:
+    def __init__(self, precision: int = _lib.SIMP_MIXED):
+        h = C.c_void_p()
+        check(lib().simp_batch_create(precision, C.byref(h)), "simp_batch_create")
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().simp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, scenarios: Sequence, nsteps: Optional[Iterable[int]] = None) -> List[int]:
+        scs = list(scenarios)
+        ns = list(nsteps) if nsteps is not None else [0] * len(scs)
+        keep: list = []
+        arr = (SimpInstanceDesc * len(scs))(*[_desc(sc, k, keep) for sc, k in zip(scs, ns)])
+        ids = np.zeros(len(scs), np.int32)
+        check(lib().simp_batch_add(self._h, len(scs), arr, ids.ctypes.data), "simp_batch_add")
+        for sc, i in zip(scs, ids):
+            self._n[int(i)] = sc.n
+        return [int(i) for i in ids]
+
+    def remove(self, ids: Iterable[int]) -> None:
+        v = np.asarray(list(ids), np.int32)
+        check(lib().simp_batch_remove(self._h, len(v), v.ctypes.data), "simp_batch_remove")
+        for i in v:
+            self._n.pop(int(i), None)
+
+    def step(self, nsteps: int, sync: bool = True) -> None:
+        check(lib().simp_batch_step(self._h, nsteps, 1 if sync else 0), "simp_batch_step")
+
+    def sync(self) -> None:
+        check(lib().simp_batch_sync(self._h), "simp_batch_sync")
+
+    def status(self, id_: int) -> dict:
+        it, left, act, st = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().simp_batch_status(self._h, id_, C.byref(it), C.byref(left), C.byref(act),
+                                      C.byref(st)), "simp_batch_status")
+        return {"itimestep": it.value, "steps_left": left.value, "active": bool(act.value),
+                "status": st.value}
+
+    def download(self, id_: int, sc=None) -> dict:
+        n = self._n[id_]
+        out = {"x": np.zeros((n, 2)), "vx": np.zeros((n, 2)), "rho": np.zeros(n),
+               "hsml": np.zeros(n), "h": np.zeros(n), "A": np.zeros(n)}
+        dt, t, it = C.c_double(), C.c_double(), C.c_int()
+        check(lib().simp_batch_download(
+            self._h, id_, out["x"].ctypes.data, out["vx"].ctypes.data, out["rho"].ctypes.data,
+            out["hsml"].ctypes.data, out["h"].ctypes.data, out["A"].ctypes.data,
+            C.byref(dt), C.byref(t), C.byref(it)), "simp_batch_download")
+        out.update(dt=dt.value, time=t.value, itimestep=it.value)
+        if sc is not None:
+            sc.x[:], sc.vx[:], sc.rho[:] = out["x"], out["vx"], out["rho"]
+            sc.hsml[:], sc.h[:], sc.A[:] = out["hsml"], out["h"], out["A"]
+            sc.dt, sc.time = out["dt"], out["time"]
+        return out
+
+    def count(self) -> tuple:
+        ni, npart = C.c_int(), C.c_longlong()
+        check(lib().simp_batch_count(self._h, C.byref(ni), C.byref(npart)), "simp_batch_count")
+        return ni.value, npart.value
