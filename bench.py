@@ -168,6 +168,62 @@ def hbm_traffic(kernel_substr, timeout=240):
                                                  "fetch_kib": fetch, "write_kib": write}
 
 
+def bench_ensemble(steps=200, baseline_steps=50):
+    """Config 4: 64 closed boxes, particle counts lognormal in [2k, 60k],
+    box 50-300 km, random uniform wind; all stepped in one packed launch per
+    kernel (mixed precision), against the plain FP64 path stepping each
+    instance with its own launches (one instance after the other)."""
+    from simp_amd import _lib
+    from simp_amd import scenarios as S
+    from simp_amd.api import Simp
+    from simp_amd.ensemble import Ensemble
+    scs = S.ensemble(count=64)
+    total = sum(sc.n for sc in scs)
+    out = {"instances": len(scs), "particles": total,
+           "n_min": min(sc.n for sc in scs), "n_max": max(sc.n for sc in scs)}
+    for prec, key in ((_lib.SIMP_MIXED, "packed_mixed"), (_lib.SIMP_FP64, "packed_fp64")):
+        ens = Ensemble(prec)
+        ens.add(scs)
+        ens.step(5)
+        t0 = time.perf_counter()
+        ens.step(steps)
+        dt = time.perf_counter() - t0
+        out[key] = total * steps / dt
+        out[key + "_ms_per_step"] = 1e3 * dt / steps
+        ens.close()
+    hs = []
+    for sc in scs:
+        h = Simp.for_scenario(sc, precision=_lib.SIMP_FP64)
+        h.upload(sc)
+        h.step(2)
+        hs.append(h)
+    t0 = time.perf_counter()
+    for h in hs:
+        h.step(baseline_steps, sync=True)
+    dt = time.perf_counter() - t0
+    out["per_instance_fp64"] = total * baseline_steps / dt
+    out["packed_mixed_over_per_instance_fp64"] = out["packed_mixed"] / out["per_instance_fp64"]
+    for h in hs:
+        h.close()
+    return out
+
+
+def bench_serve(count=80, budget_gib=16.0, chunk=10):
+    """Config 5: Poisson arrivals (20/s) of closed-box simulations, N in
+    [5k, 250k], 100-2000 steps each, HBM-budget admission, continuous
+    batching, mixed precision."""
+    from simp_amd import _lib
+    from simp_amd.serve import Server, poisson_requests
+    reqs = poisson_requests(count=count)
+    srv = Server(precision=_lib.SIMP_MIXED, budget_bytes=int(budget_gib * (1 << 30)),
+                 chunk=chunk)
+    m = srv.run(reqs)
+    m.pop("results")
+    m["budget_gib"] = budget_gib
+    m["chunk_steps"] = chunk
+    return m
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +237,10 @@ def main():
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes for roofline.traffic")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--workload", choices=("ice_edge", "ensemble", "serve"), default="ice_edge",
+                    help="ensemble / serve: print only that workload's line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the SPH-VP step of one simulation (or one "
@@ -194,6 +254,12 @@ def main():
 
     if args.copy_calib > 0:
         _copy_calibration(args.copy_calib)
+    if args.workload == "ensemble":
+        print(json.dumps({"workload": "ensemble", **bench_ensemble(args.steps)}), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps({"workload": "serve", **bench_serve()}), flush=True)
+        return
     sc = S.ice_edge()
     n = sc.n
     res = {}
@@ -257,6 +323,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    if not args.no_ensemble:
+        line["ensemble"] = bench_ensemble()
+    if not args.no_serve:
+        line["serve"] = bench_serve()
     if not args.no_traffic:
         kname = "k_passB<true>" if name == "passB" else "k_passA<true>"
         traffic, info = hbm_traffic(kname)

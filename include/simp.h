@@ -189,6 +189,10 @@ int simp_batch_download(simp_batch *b, int id, double *x, double *vx, double *rh
                         double *hsml, double *hth, double *A, double *dt,
                         double *current_time, int *itimestep);
 int simp_batch_count(simp_batch *b, int *instances, long long *particles);
+/* Status of every instance in one call (one device-to-host copy): writes up
+ * to cap entries and returns the number of instances (or -status). */
+int simp_batch_list(simp_batch *b, int cap, int *ids, int *itimestep, int *steps_left,
+                    int *active, int *status);
 /* Device bytes an instance would occupy (for admission against a budget). */
 long long simp_instance_footprint(const simp_instance_desc *desc);
 

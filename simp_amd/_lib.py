@@ -33,7 +33,7 @@ EXPORTS = (
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
     "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
-    "simp_batch_count", "simp_instance_footprint",
+    "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
 )
 
 
@@ -116,6 +116,7 @@ def lib():
     L.simp_batch_status.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]
     L.simp_batch_download.argtypes = [_vp, C.c_int] + [_vp] * 9
     L.simp_batch_count.argtypes = [_vp, _vp, _vp]
+    L.simp_batch_list.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_instance_footprint.argtypes = [C.POINTER(SimpInstanceDesc)]
     L.simp_instance_footprint.restype = C.c_longlong
     _lib = L

@@ -431,6 +431,29 @@ int simp_batch_count(simp_batch *b, int *instances, long long *particles) {
   return SIMP_OK;
 }
 
+int simp_batch_list(simp_batch *b, int cap, int *ids, int *itimestep, int *steps_left,
+                    int *active, int *status) {
+  if (!b || cap < 0) return -SIMP_E_ARG;
+  std::vector<simp::InstDyn> dyn;
+  int rc = b->eng->download_all_status(dyn);
+  if (rc) return -map_rc(rc);
+  const std::vector<int> &idv = b->eng->ids();
+  const int n = (int)dyn.size();
+  for (int k = 0; k < n && k < cap; ++k) {
+    const simp::InstDyn &y = dyn[k];
+    if (ids) ids[k] = idv[k];
+    if (itimestep) itimestep[k] = y.itimestep;
+    if (steps_left) steps_left[k] = y.steps_left;
+    if (active) active[k] = (y.active && !y.err) ? 1 : 0;
+    if (status) {
+      status[k] = SIMP_OK;
+      if (y.err & simp::kErrCapacity) status[k] = SIMP_E_CAPACITY;
+      if (y.err & simp::kErrNan) status[k] = SIMP_E_NAN;
+    }
+  }
+  return n;
+}
+
 long long simp_instance_footprint(const simp_instance_desc *desc) {
   if (!desc) return -1;
   InstanceInput in;

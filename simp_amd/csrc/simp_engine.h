@@ -69,6 +69,7 @@ class Engine {
   int download_scalars(int inst, double *dt, double *time, int *itimestep, int *nip, int *bnip,
                        int *err);
   int download_status(int inst, int *itimestep, int *steps_left, int *active, int *err);
+  int download_all_status(std::vector<InstDyn> &out);
   int download_tend(int inst, double *dxdt, double *dvxdt, double *dhdt, double *dAdt);
   int download_diag(int inst, double *eps3, double *zeta, double *eta, double *p,
                     double *sig3);

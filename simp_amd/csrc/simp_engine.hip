@@ -584,6 +584,14 @@ int Engine::download_status(int k, int *itimestep, int *steps_left, int *active,
   return 0;
 }
 
+int Engine::download_all_status(std::vector<InstDyn> &out) {
+  out.resize(d_.ninst);
+  if (d_.ninst == 0) return 0;
+  HIPCHK(hipStreamSynchronize(st_));
+  HIPCHK(hipMemcpy(out.data(), d_.dyn, sizeof(InstDyn) * d_.ninst, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 static void interleave(const std::vector<double> &a, const std::vector<double> &b, double *out,
                        int n) {
   for (int i = 0; i < n; ++i) {

@@ -120,6 +120,18 @@ class Ensemble:
             sc.dt, sc.time = out["dt"], out["time"]
         return out
 
+    def list(self) -> dict:
+        """Status of every instance (one device-to-host copy):
+        {id: {itimestep, steps_left, active, status}}."""
+        cap = max(len(self._n), 1)
+        a = [np.zeros(cap, np.int32) for _ in range(5)]
+        k = lib().simp_batch_list(self._h, cap, *[x.ctypes.data for x in a])
+        if k < 0:
+            check(-k, "simp_batch_list")
+        return {int(a[0][j]): {"itimestep": int(a[1][j]), "steps_left": int(a[2][j]),
+                               "active": bool(a[3][j]), "status": int(a[4][j])}
+                for j in range(min(k, cap))}
+
     def count(self) -> tuple:
         ni, npart = C.c_int(), C.c_longlong()
         check(lib().simp_batch_count(self._h, C.byref(ni), C.byref(npart)), "simp_batch_count")

@@ -231,6 +231,22 @@ def ice_edge(nside: int = 500, spacing: float = 1000.0, seed: int = 3,
                    name=f"ice_edge{n}")
 
 
+def box_member(N: float, L: float, wind, seed: int, tse: int = 2) -> Scenario:
+    """One ensemble member: a closed box of side L holding about N particles
+    (lattice spacing L / sqrt(N), hsml = 1.3 s), lognormal thickness,
+    uniform wind."""
+    nside = max(2, int(round(math.sqrt(N))))
+    s = L / nside
+    sub = np.random.default_rng(seed)
+    x = lattice(nside, nside, s, sub)
+    n = x.shape[0]
+    hsml = np.full(n, 1.3 * s)
+    h = lognormal_thickness(sub, n)
+    A = np.ones(n)
+    xb = walls(0.0, 0.0, L, L)
+    return _finish(x, hsml, h, A, xb, sub, wind=tuple(wind), tse=tse, name=f"box{n}_L{L:.0f}")
+
+
 def ensemble(count: int = 64, nmin: int = 2000, nmax: int = 60000,
              seed: int = 4, tse: int = 2) -> List[Scenario]:
     """Independent closed boxes of lognormal particle count in [nmin, nmax],
@@ -239,21 +255,11 @@ def ensemble(count: int = 64, nmin: int = 2000, nmax: int = 60000,
     out = []
     for k in range(count):
         N = float(np.clip(rng.lognormal(math.log(12000.0), 0.8), nmin, nmax))
-        nside = max(2, int(round(math.sqrt(N))))
         L = rng.uniform(50e3, 300e3)
-        s = L / nside
         ang = rng.uniform(0, 2 * math.pi)
         spd = rng.uniform(5.0, 20.0)
-        sub = np.random.default_rng(seed * 100003 + k)
-        x = lattice(nside, nside, s, sub)
-        n = x.shape[0]
-        hsml = np.full(n, 1.3 * s)
-        h = lognormal_thickness(sub, n)
-        A = np.ones(n)
-        xb = walls(0.0, 0.0, L, L)
-        out.append(_finish(x, hsml, h, A, xb, sub,
-                           wind=(spd * math.cos(ang), spd * math.sin(ang)),
-                           tse=tse, name=f"ens{k}_{n}"))
+        out.append(box_member(N, L, (spd * math.cos(ang), spd * math.sin(ang)),
+                              seed * 100003 + k, tse=tse))
     return out
 
 
