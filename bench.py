@@ -328,7 +328,7 @@ def main():
     if not args.no_serve:
         line["serve"] = bench_serve()
     if not args.no_traffic:
-        kname = "k_passB<true>" if name == "passB" else "k_passA<true>"
+        kname = "k_passB<true, false>" if name == "passB" else "k_passA<true, false>"
         traffic, info = hbm_traffic(kname)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_note"] = info

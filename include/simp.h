@@ -58,7 +58,8 @@ typedef struct simp_config {
     int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127)         */
     int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134)      */
     int32_t skf;            /* 6: Wendland C6 (globals.f90:147)                 */
-    int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155)     */
+    int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155);    */
+                            /* 1: summation density (density.f90:41-100)        */
     int32_t wind_mode;      /* SIMP_WIND_UNIFORM or SIMP_WIND_GRID              */
     int32_t max_particles;  /* capacity of particles (0: size at upload)        */
     int32_t max_boundary;   /* capacity of boundary particles (0: at upload)    */
@@ -104,6 +105,19 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
                      double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
                      double *eps3, double *zeta, double *eta, double *p,
                      double *sig3, int *nip, int *bnip);
+
+/* ---- kernel values of the last single_step, for a host-side density ----
+ * Replaces the module arrays single_step leaves for sum_density
+ * (phy_quantities/density.f90:41-100; written at sph/kernel.f90:503,511):
+ * the particle pairs of the last simp_single_step in the reference's order
+ * (p_i, p_j 1-based), their Wendland C6 values w (kernel.f90:506), and
+ * kernel_zero(1:n), the factor of the last pair each particle is the first
+ * member of; it keeps its value across calls for particles that are first
+ * in no pair, as the module array does, and is tracked only when the
+ * handle's den_alg is 1 (SIMP_E_STATE otherwise).  Any array may be NULL;
+ * nip is always written; SIMP_E_CAPACITY when it exceeds cap. */
+int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
+                       double *w, double *kernel_zero);
 
 /* ---- NNPS (neighbor/nnps.f90:21-30 -> bucket_search :155-517) ----
  * The pair lists in the reference's exact order (single-thread order):
@@ -171,6 +185,8 @@ typedef struct simp_instance_desc {
     int32_t grid_nx, grid_ny;            /* wind grid (wind_mode = GRID)       */
     double  grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u, *grid_v;
+    int32_t den_alg;                     /* 0 or 3: diagnostic; 1: summation   */
+    int32_t reserved;
 } simp_instance_desc;
 
 int simp_batch_create(int precision, simp_batch **out);

@@ -47,21 +47,56 @@ done
     -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
 echo "build_ref: $OUT/simp_ref_harness"
 
+# Summation-density variant (den_alg = 1, density.f90:41-100).  Run
+# parameters are compile-time PARAMETERs that the reference configures by
+# editing globals.f90 (README.md:68-70); the edit is applied on the way into
+# the compiler (stdin), so no copy of the source is written, and everything
+# that uses the module is rebuilt against it.
+D1="$OUT/den1"
+mkdir -p "$D1/obj" "$D1/mod"
+d1objs=()
+grep -q 'parameter(den_alg = 3)' "$SRC/modules/globals.f90"   # the line the edit targets
+sed 's/parameter(den_alg = 3)/parameter(den_alg = 1)/' "$SRC/modules/globals.f90" \
+    | "$FC" $FFLAGS -x f95 -J"$D1/mod" -c - -o "$D1/obj/globals.o"
+d1objs+=("$D1/obj/globals.o")
+for m in data_variables domain boundary_domain mpi_variables tree_module interfaces; do
+    "$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$SRC/modules/$m.f90" -o "$D1/obj/$m.o"
+    d1objs+=("$D1/obj/$m.o")
+done
+for d in ext_force in_out int_force integration neighbor phy_quantities sph; do
+    for f in "$SRC/$d"/*.f90; do
+        b="$(basename "$f" .f90)"
+        "$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$f" -o "$D1/obj/$b.o"
+        d1objs+=("$D1/obj/$b.o")
+    done
+done
+"$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$HERE/ref_harness.f90" -o "$D1/obj/ref_harness.o"
+"$FC" $FFLAGS -o "$OUT/simp_ref_harness_den1" "$D1/obj/ref_harness.o" "${d1objs[@]}" \
+    -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+echo "build_ref: $OUT/simp_ref_harness_den1"
+
 # The drop-in check: the same harness and the reference's own
 # time_integration, with single_step replaced by the iso_c_binding shim of
-# simp_amd/fortran that calls libsimp_amd.so on the GPU.
+# simp_amd/fortran that calls libsimp_amd.so on the GPU; once per density
+# variant (the shim compiled against that variant's modules).
 LIB="$HERE/../simp_amd/lib/libsimp_amd.so"
-if [ -f "$LIB" ]; then
-    FSRC="$HERE/../simp_amd/fortran"
-    "$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$FSRC/simp_binding.f90" -o "$OUT/obj/simp_binding.o"
-    "$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$FSRC/single_step_gpu.f90" -o "$OUT/obj/single_step_gpu.o"
-    gobjs=()
-    for o in "${objs[@]}"; do
+gpu_harness() {   # <obj dir> <mod dir> <output> <objects...>
+    local odir="$1" mdir="$2" out="$3"
+    shift 3
+    local FSRC="$HERE/../simp_amd/fortran"
+    "$FC" $FFLAGS -I"$mdir" -J"$mdir" -c "$FSRC/simp_binding.f90" -o "$odir/simp_binding.o"
+    "$FC" $FFLAGS -I"$mdir" -J"$mdir" -c "$FSRC/single_step_gpu.f90" -o "$odir/single_step_gpu.o"
+    local gobjs=()
+    for o in "$@"; do
         [ "$(basename "$o")" = "single_step.o" ] || gobjs+=("$o")
     done
-    "$FC" $FFLAGS -o "$OUT/simp_ref_harness_gpu" "$OUT/obj/ref_harness.o" "${gobjs[@]}" \
-        "$OUT/obj/simp_binding.o" "$OUT/obj/single_step_gpu.o" \
+    "$FC" $FFLAGS -o "$out" "$odir/ref_harness.o" "${gobjs[@]}" \
+        "$odir/simp_binding.o" "$odir/single_step_gpu.o" \
         -L"$(dirname "$LIB")" -lsimp_amd -L/opt/rocm/llvm/lib -lomp \
         -Wl,-rpath,'$ORIGIN/../../simp_amd/lib' -Wl,-rpath,/opt/rocm/llvm/lib -Wl,-rpath,/opt/rocm/lib
-    echo "build_ref: $OUT/simp_ref_harness_gpu"
+    echo "build_ref: $out"
+}
+if [ -f "$LIB" ]; then
+    gpu_harness "$OUT/obj" "$OUT/mod" "$OUT/simp_ref_harness_gpu" "${objs[@]}"
+    gpu_harness "$D1/obj" "$D1/mod" "$OUT/simp_ref_harness_gpu_den1" "${d1objs[@]}"
 fi

@@ -18,7 +18,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 class OrcParams(C.Structure):
     _fields_ = [("tse", C.c_int), ("wind_mode", C.c_int),
                 ("wind", C.c_double * 2), ("current", C.c_double * 2),
-                ("max_hsml", C.c_double),
+                ("max_hsml", C.c_double), ("den_alg", C.c_int),
                 ("grid_nx", C.c_int), ("grid_ny", C.c_int),
                 ("grid_x0", C.c_double), ("grid_y0", C.c_double),
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
@@ -56,6 +56,8 @@ def lib():
                                  C.c_int, C.c_void_p, C.c_void_p,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]
+        L.orc_sum_density.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_get_kernel.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_vp_dt.argtypes = [C.c_int, _dp]
         L.orc_vp_dt.restype = C.c_double
         _lib = L
@@ -74,6 +76,7 @@ def make_params(sc) -> tuple:
     p.wind[0], p.wind[1] = sc.wind
     p.current[0], p.current[1] = sc.current
     p.max_hsml = sc.max_hsml
+    p.den_alg = getattr(sc, "den_alg", 3)
     keep = ()
     if sc.wind_grid is not None:
         g = sc.wind_grid
@@ -164,6 +167,24 @@ class Oracle:
         out["nip"] = self._L.orc_nip(self._c)
         out["bnip"] = self._L.orc_bnip(self._c)
         return out
+
+    def kernel_values(self, n: int) -> dict:
+        """w (pair order) and kernel_zero(1:n) of the last single_step."""
+        nip = self._L.orc_nip(self._c)
+        w = np.zeros(max(nip, 1)); kz = np.zeros(n)
+        if self._L.orc_get_kernel(self._c, n, w, kz):
+            raise RuntimeError("orc_get_kernel failed")
+        return {"w": w[:nip], "kernel_zero": kz}
+
+    def sum_density(self, mass, rho) -> np.ndarray:
+        """sum_density with the last single_step's kernel values (rho
+        returned updated)."""
+        rho = np.array(rho, dtype=np.float64)
+        rc = self._L.orc_sum_density(self._c, rho.shape[0],
+                                     np.ascontiguousarray(mass, np.float64), rho)
+        if rc:
+            raise RuntimeError(f"orc_sum_density failed ({rc})")
+        return rho
 
     def run_pc(self, sc, nsteps: int, itimestep: int = 0) -> dict:
         """Advance ``sc`` in place by ``nsteps`` predictor-corrector steps."""

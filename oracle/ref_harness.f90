@@ -25,6 +25,7 @@
 !   real8  dxdt(2,n), dvxdt(2,n), dhdt(n), dAdt(n)
 !   real8  epsilon_11(n), epsilon_12(n), epsilon_22(n), zeta(n), eta(n),
 !          p(n), sigma_11(n), sigma_12(n), sigma_22(n)
+!   real8  w(nip), kernel_zero(n)   (kernel values of the last single_step)
 !---------------------------------------------------------------------------
 program simp_ref_harness
     use globals
@@ -108,5 +109,6 @@ program simp_ref_harness
     write(u) dxdt(:, 1:n), dvxdt(:, 1:n), dhdt(1:n), dAdt(1:n)
     write(u) epsilon_11(1:n), epsilon_12(1:n), epsilon_22(1:n), zeta(1:n), &
              eta(1:n), p(1:n), sigma_11(1:n), sigma_12(1:n), sigma_22(1:n)
+    write(u) w(1:nip), kernel_zero(1:n)
     close(u)
 end program simp_ref_harness

@@ -20,6 +20,9 @@ HARNESS = os.path.join(HERE, "_ref", "simp_ref_harness")
 #: the same harness with single_step replaced by the GPU shim
 #: (simp_amd/fortran/single_step_gpu.f90 -> libsimp_amd.so)
 HARNESS_GPU = os.path.join(HERE, "_ref", "simp_ref_harness_gpu")
+#: the harness built with summation density (den_alg = 1)
+HARNESS_DEN1 = os.path.join(HERE, "_ref", "simp_ref_harness_den1")
+HARNESS_GPU_DEN1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den1")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
 
@@ -78,6 +81,8 @@ def _read_output(path, n):
     out["zeta"] = take("<f8", n); out["eta"] = take("<f8", n); out["p"] = take("<f8", n)
     s11, s12, s22 = take("<f8", n), take("<f8", n), take("<f8", n)
     out["sigma"] = np.stack([s11, s12, s22], axis=1)
+    out["w"] = take("<f8", nip)
+    out["kernel_zero"] = take("<f8", n)
     return out
 
 

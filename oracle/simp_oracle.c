@@ -17,6 +17,9 @@
  *   - single_step, run_pc: golden vectors of the compiled reference, plus
  *     closed-form invariants (uniform velocity => zero strain rate, rest =>
  *     zero stress, stress-pair antisymmetry => zero net internal force).
+ *   - sum_density (den_alg = 1): golden vectors of the reference built with
+ *     that PARAMETER, including the stale kernel_zero reads, plus the
+ *     uniform-density fixed point.
  */
 #include "simp_oracle.h"
 
@@ -50,6 +53,11 @@ struct orc_ctx {
     int *p_i, *p_j;
     double *r_ij, *hsml_ij;   /* hsml_ij keeps stale entries beyond nip */
     double *dwdx;             /* (2, nip) */
+    double *w;                /* kernel values w(k) (kernel.f90:511) */
+    /* kernel_zero(maxn) (kernel.f90:503): the factor of the last pair each
+     * particle was the first member of; persists across calls */
+    int nkz;
+    double *kz;
     /* boundary-pair list */
     int bnip, bcap;
     int *bp_i, *bp_j;
@@ -65,6 +73,7 @@ orc_ctx *orc_create(void) { return (orc_ctx *)calloc(1, sizeof(orc_ctx)); }
 void orc_destroy(orc_ctx *c) {
     if (!c) return;
     free(c->p_i); free(c->p_j); free(c->r_ij); free(c->hsml_ij); free(c->dwdx);
+    free(c->w); free(c->kz);
     free(c->bp_i); free(c->bp_j); free(c->br_ij);
     free(c->div); free(c->grad);
     free(c);
@@ -93,6 +102,13 @@ void orc_get_bins(const orc_ctx *c, double *out) {
     memcpy(out, c->bins_info, sizeof(c->bins_info));
 }
 
+int orc_get_kernel(const orc_ctx *c, int n, double *w, double *kernel_zero) {
+    if (n > c->nkz) return ORC_E_ARG;
+    if (w && c->nip) memcpy(w, c->w, (size_t)c->nip * sizeof(double));
+    if (kernel_zero) memcpy(kernel_zero, c->kz, (size_t)n * sizeof(double));
+    return ORC_OK;
+}
+
 /* Grow the pair arrays; new hsml_ij entries start at 0 like the zeroed
  * module array (initialize.f90:58), old entries are kept (stale reads). */
 static int grow_pairs(orc_ctx *c, int need) {
@@ -115,6 +131,9 @@ static int grow_pairs(orc_ctx *c, int need) {
     d = (double *)realloc(c->dwdx, (size_t)nc * 2 * sizeof(double));
     if (!d) return -1;
     c->dwdx = d;
+    d = (double *)realloc(c->w, (size_t)nc * sizeof(double));
+    if (!d) return -1;
+    c->w = d;
     c->cap = nc;
     return 0;
 }
@@ -343,6 +362,13 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
                     double *eta, double *p, double *sig3) {
     if (n < 1) return ORC_E_ARG;
     if (ensure_scratch(c, n)) return ORC_E_NOMEM;
+    if (n > c->nkz) {   /* zero-initialised like the module array */
+        double *kz = (double *)realloc(c->kz, (size_t)n * sizeof(double));
+        if (!kz) return ORC_E_NOMEM;
+        memset(kz + c->nkz, 0, (size_t)(n - c->nkz) * sizeof(double));
+        c->kz = kz;
+        c->nkz = n;
+    }
     double *div = c->div, *grad = c->grad;   /* grad(4, n) */
     /* reset (:36-44) */
     for (int i = 0; i < n; ++i) {
@@ -369,6 +395,9 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double factor = c6 * (ih * ih);
         double omq = 1 - q;
         double o7 = ipow(omq, 7);
+        /* kernel_zero(cp_i) = factor; w = factor (1-q)^8 (32q^3 + 25q^2 + 8q + 1) */
+        c->kz[i] = factor;
+        c->w[k] = factor * ipow(omq, 8) * (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
         double g = factor * (-(22. * q * o7 * (16 * ipow(q, 2) + 7. * q + 1))) / hij;
         double dw0 = g * (dx0 / r) * 1.0;
         double dw1 = g * (dx1 / r) * 1.0;
@@ -478,6 +507,25 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
     return ORC_OK;
 }
 
+/* ---- sum_density, density.f90:41-100 ---------------------------------- */
+int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho) {
+    if (n < 1 || n > c->nkz) return ORC_E_ARG;
+    double *num = c->div, *den = c->grad;   /* scratch, n each */
+    for (int i = 0; i < n; ++i) {
+        num[i] = c->kz[i] * mass[i];
+        den[i] = c->kz[i] * mass[i] / rho[i];
+    }
+    for (int k = 0; k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        den[i] = den[i] + mass[j] * c->w[k] / rho[j];
+        den[j] = den[j] + mass[i] * c->w[k] / rho[i];
+        num[i] = num[i] + mass[j] * c->w[k];
+        num[j] = num[j] + mass[i] * c->w[k];
+    }
+    for (int i = 0; i < n; ++i) rho[i] = num[i] / den[i];
+    return ORC_OK;
+}
+
 /* VP_time_step (time_step.f90:155) */
 double orc_vp_dt(int n, const double *hsml) {
     double dt = 0.0;
@@ -525,13 +573,16 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             h[i] = h[i] + dhdt[i] * dt / 2;
             A[i] = A[i] + dAdt[i] * dt / 2;
             if (A[i] > 1) A[i] = 1;
-            rho[i] = h[i] * RHO_ICE;           /* diagnostic_density */
+            if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;   /* diagnostic_density */
         }
+        if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
         rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
         if (rc) break;
-        /* corrector (:236-262) */
+        /* corrector (:236-262); with summation density, density() after
+         * the half-step values (:244) already changes rho */
+        if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
         for (size_t i = 0; i < n2; ++i) {
             double x12 = x1[i] + dxdt[i] * dt / 2;
             double v12 = v1[i] + dvxdt[i] * dt / 2;
@@ -544,7 +595,10 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             h[i] = 2 * h12 - h1[i];
             A[i] = 2 * A12 - A1[i];
             if (A[i] > 1) A[i] = 1;
-            rho[i] = h[i] * RHO_ICE;
+            if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+        }
+        if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+        for (int i = 0; i < n; ++i) {
             /* Marquis_hsml (smoothing_length.f90:136-142) */
             hsml[i] = sqrt(mass[i] / rho[i]) * HSML_MULT;
             if (hsml[i] > prm->max_hsml) hsml[i] = prm->max_hsml;

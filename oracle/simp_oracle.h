@@ -26,6 +26,8 @@ typedef struct orc_params {
     double wind[2];      /* uniform wind (m/s); the reference hard-codes (0, -5.3f) */
     double current[2];   /* uniform ocean current (m/s); reference: (0, 0) */
     double max_hsml;     /* smoothing-length cap (initialize.f90:191) */
+    int    den_alg;      /* 3: diagnostic rho = h rho_ice (density.f90:172-194);
+                            1: summation (density.f90:41-100) */
     int    grid_nx, grid_ny;
     double grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u;  /* grid_ny * grid_nx, row-major (y outer) */
@@ -51,6 +53,8 @@ void orc_get_bpairs(const orc_ctx *c, int *bp_i, int *bp_j, double *br_ij);
 /* Reference NNPS bin statistics of the last orc_nnps call:
  * out[0] = bin_size, out[1..4] = bucket_coor (xlo, xhi, ylo, yhi). */
 void orc_get_bins(const orc_ctx *c, double *out);
+/* Kernel values w(1:nip) and kernel_zero(1:n) of the last orc_single_step. */
+int orc_get_kernel(const orc_ctx *c, int n, double *w, double *kernel_zero);
 
 /* single_step (single_step.f90:5-108) on interleaved host arrays.
  * eps3 and sig3 are (3, n): (e11, e12, e22) and (s11, s12, s22). */
@@ -74,6 +78,10 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n,
                const double *massb, double *dt, double *current_time,
                int *itimestep, double *eps3, double *zeta, double *eta,
                double *p, double *sig3);
+
+/* sum_density (density.f90:41-100) with the kernel values and kernel_zero
+ * of the last orc_single_step on this context; rho is updated in place. */
+int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho);
 
 /* VP_time_step (time_step.f90:155). */
 double orc_vp_dt(int n, const double *hsml);

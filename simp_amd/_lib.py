@@ -28,7 +28,7 @@ SIMP_CONFIG_VERSION = 1
 #: every entry point declared in include/simp.h
 EXPORTS = (
     "simp_config_default", "simp_create", "simp_destroy", "simp_set_wind_grid",
-    "simp_single_step", "simp_nnps", "simp_upload", "simp_step_pc",
+    "simp_single_step", "simp_kernel_values", "simp_nnps", "simp_upload", "simp_step_pc",
     "simp_download", "simp_set_diagnostics", "simp_download_diagnostics",
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
@@ -60,7 +60,8 @@ class SimpInstanceDesc(C.Structure):
                 ("grid_nx", C.c_int32), ("grid_ny", C.c_int32),
                 ("grid_x0", C.c_double), ("grid_y0", C.c_double),
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
-                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p)]
+                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p),
+                ("den_alg", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SimpError(RuntimeError):
@@ -95,6 +96,7 @@ def lib():
                                      C.c_double, C.c_double, _vp, _vp]
     L.simp_single_step.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp,
                                    C.c_double, C.c_int] + [_vp] * 9 + [_vp, _vp])
+    L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
                             _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]
     L.simp_upload.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp, C.c_double,

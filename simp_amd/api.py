@@ -27,11 +27,13 @@ from ._lib import SimpConfig, check, lib, ptr
 
 
 def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(np.float32(-5.3))),
-                current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False) -> SimpConfig:
+                current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False,
+                den_alg: int = 3) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
     cfg.tse = tse
+    cfg.den_alg = den_alg
     cfg.wind[0], cfg.wind[1] = wind
     cfg.current[0], cfg.current[1] = current
     cfg.max_hsml = max_hsml
@@ -55,8 +57,9 @@ class Simp:
 
     def __init__(self, precision: int = _lib.SIMP_FP64, tse: int = 2,
                  wind=(0.0, float(np.float32(-5.3))), current=(0.0, 0.0),
-                 max_hsml: float = 0.0, wind_grid=None):
-        self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None)
+                 max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3):
+        self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
+                               den_alg)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -71,7 +74,8 @@ class Simp:
     @classmethod
     def for_scenario(cls, sc, precision: int = _lib.SIMP_FP64) -> "Simp":
         return cls(precision=precision, tse=sc.tse, wind=sc.wind, current=sc.current,
-                   max_hsml=sc.max_hsml, wind_grid=sc.wind_grid)
+                   max_hsml=sc.max_hsml, wind_grid=sc.wind_grid,
+                   den_alg=getattr(sc, "den_alg", 3))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -127,6 +131,24 @@ class Simp:
             ptr(out["sigma"]), C.byref(nip), C.byref(bnip))
         check(st, "simp_single_step")
         out["nip"], out["bnip"] = nip.value, bnip.value
+        return out
+
+    def kernel_values(self, n: int, cap: Optional[int] = None) -> dict:
+        """The last single_step's pairs with their kernel values w and the
+        per-particle kernel_zero (what sum_density reads on the host)."""
+        cap = cap if cap is not None else 16 * n + 64
+        nip = C.c_int()
+        pi = np.zeros(cap, np.int32); pj = np.zeros(cap, np.int32); w = np.zeros(cap)
+        kz = np.zeros(n)
+        st = lib().simp_kernel_values(self._h, cap, C.byref(nip), ptr(pi), ptr(pj), ptr(w),
+                                      ptr(kz) if self.cfg.den_alg == 1 else None)
+        if st == _lib.SIMP_E_CAPACITY and nip.value > cap:
+            return self.kernel_values(n, cap=nip.value)
+        check(st, "simp_kernel_values")
+        k = nip.value
+        out = {"nip": k, "p_i": pi[:k], "p_j": pj[:k], "w": w[:k]}
+        if self.cfg.den_alg == 1:
+            out["kernel_zero"] = kz
         return out
 
     # --------------------------------------------- device-resident stepping

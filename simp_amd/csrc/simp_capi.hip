@@ -78,7 +78,7 @@ static int validate(const simp_config *c) {
   if (c->int_algorithm != 2 || c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
   if (c->tse != 0 && c->tse != 2) return SIMP_E_UNSUPPORTED;
   if (c->rheology != 1 || c->bfa != 1 || c->eos_alg != 3 || c->pa_sph != 1) return SIMP_E_UNSUPPORTED;
-  if (c->skf != 6 || c->den_alg != 3) return SIMP_E_UNSUPPORTED;
+  if (c->skf != 6 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
   if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   return SIMP_OK;
 }
@@ -127,6 +127,7 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
   in.wind[0] = c.wind[0]; in.wind[1] = c.wind[1];
   in.current[0] = c.current[0]; in.current[1] = c.current[1];
   in.max_hsml = c.max_hsml;
+  in.den_alg = c.den_alg;
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
     in.g_nx = h->g_nx; in.g_ny = h->g_ny;
@@ -194,6 +195,22 @@ int simp_nnps(simp_handle *h, int n, const double *x, const double *hsml, int nb
   if (rc) return map_rc(rc);
   h->uploaded = false;
   rc = h->eng->export_pairs(cap, nip, p_i, p_j, r_ij, hsml_ij, bcap, bnip, bp_i, bp_j, br_ij);
+  return map_rc(rc);
+}
+
+int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j, double *w,
+                       double *kernel_zero) {
+  if (!h || cap < 0 || !nip) return SIMP_E_ARG;
+  if (h->eng->ninst() != 1) return SIMP_E_STATE;
+  if (kernel_zero && h->cfg.den_alg != 1) return SIMP_E_STATE;
+  int bnip = 0;
+  int rc = h->eng->export_pairs(cap, nip, p_i, p_j, nullptr, nullptr, 0, &bnip, nullptr,
+                                nullptr, nullptr, w);
+  // the boundary pairs are not asked for here: only a particle-pair
+  // overflow is a capacity error
+  if (rc == -3 && *nip <= cap) rc = 0;
+  if (rc) return map_rc(rc);
+  if (kernel_zero) rc = h->eng->download_kernel_zero(0, kernel_zero);
   return map_rc(rc);
 }
 
@@ -309,6 +326,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.nb > 0 && (!d.xb || !d.massb)) return SIMP_E_ARG;
   if (d.tse != 0 && d.tse != 2) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode != SIMP_WIND_UNIFORM && d.wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
+  if (d.den_alg != 0 && d.den_alg != 1 && d.den_alg != 3) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))
@@ -319,6 +337,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.dt = d.dt; in.time = d.current_time; in.itimestep = d.itimestep;
   in.nsteps = d.nsteps > 0 ? d.nsteps : 0;
   in.tse = d.tse;
+  in.den_alg = d.den_alg == 1 ? 1 : 3;
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
   in.current[0] = d.current[0]; in.current[1] = d.current[1];

@@ -46,7 +46,8 @@ struct InstMeta {
   int c_off, c_cap;      // cell range (capacity)
   int q_off, q_cap;      // boundary-pair / hsml_ij-store range
   int tse, wind_mode;
-  int g_off, g_nx, g_ny, pad0;  // wind grid range
+  int g_off, g_nx, g_ny;  // wind grid range
+  int den_alg;           // 3: rho = h rho_ice; 1: summation density
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;
   double max_hsml;
@@ -99,6 +100,11 @@ struct DevState {
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
+  // summation density (den_alg = 1): kernel_zero of each particle (by
+  // packed index; persists across evaluations like the reference's module
+  // array), and per list position the numerator and the density that
+  // pass A's sums give for the state it evaluated
+  double *kz, *dnum, *dsum;
   double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N), optional
   double *sig_out;       // sigma_11, sigma_12, sigma_22 (3 x N), with diag
   double *tend;          // dxdt, dydt, dvxdt, dvydt, dhdt, dAdt (6 x N), optional

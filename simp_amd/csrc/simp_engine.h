@@ -22,6 +22,7 @@ struct InstanceInput {
   int wind_mode = 0;
   double wind[2] = {0.0, 0.0}, current[2] = {0.0, 0.0};
   double max_hsml = 0.0;
+  int den_alg = 3;       // 3: diagnostic density; 1: summation density
   // optional wind grid
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
@@ -74,7 +75,9 @@ class Engine {
   int download_diag(int inst, double *eps3, double *zeta, double *eta, double *p,
                     double *sig3);
   int export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, double *hij, int bcap,
-                   int *bnip, int *bpi, int *bpj, double *brij);
+                   int *bnip, int *bpi, int *bpj, double *brij, double *w = nullptr);
+  // kernel_zero of instance inst (n values)
+  int download_kernel_zero(int inst, double *kz);
 
   void set_diag(bool on) { diag_ = on; }
   bool diag() const { return diag_; }
@@ -107,6 +110,7 @@ class Engine {
   int next_id_ = 0;
   bool diag_ = false;
   bool any_boundary_ = false;
+  bool any_sumd_ = false;             // some instance uses summation density
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
 };

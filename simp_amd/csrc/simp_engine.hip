@@ -86,7 +86,7 @@ static int store_capacity(const InstanceInput &in) {
 
 // bytes per particle / boundary particle / cell of the device layout
 constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
-                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8;
+                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8;
 constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 8;
 constexpr size_t kBytesPerCell = 6 * 4;
 
@@ -165,6 +165,9 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &sg, N); d.sig12 = sg;
   rc |= dalloc(list, &sg, N); d.sig22 = sg;
   rc |= dalloc(list, &sg, N); d.div = sg;
+  rc |= dalloc(list, &d.kz, N);
+  rc |= dalloc(list, &d.dnum, N);
+  rc |= dalloc(list, &d.dsum, N);
   rc |= dalloc(list, &d.diag, 6 * N);
   rc |= dalloc(list, &d.sig_out, 3 * N);
   rc |= dalloc(list, &d.tend, 6 * N);
@@ -237,7 +240,7 @@ int Engine::remove(const std::vector<int> &ids) {
 // Rebuild the packed layout from the kept instances (old packed indices,
 // their live state copied device to device) followed by the added ones
 // (copied from the host).  carry_store_from: old index whose hsml_ij store
-// seeds the first added instance's store.
+// and kernel_zero seed the first added instance's.
 int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput> &add,
                    int carry_store_from, std::vector<int> *new_ids) {
   const int nk = (int)keep.size(), na = (int)add.size(), ninst = nk + na;
@@ -274,6 +277,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
     m.cur_u = in.current[0]; m.cur_v = in.current[1];
     m.max_hsml = in.max_hsml;
+    m.den_alg = in.den_alg;
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -326,6 +330,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   } while (0)
   RCHK(hipMemsetAsync(nd.hq, 0, std::max<long>(qtot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.count, 0, (ctot + 1) * sizeof(int), st_));
+  // kernel_zero starts at zero, as the reference's module array does
+  RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
 
   // kept instances: device to device
@@ -345,6 +351,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     RCHK(d2d(nd.rho[0] + m.p_off, d_.rho[0] + o.p_off, n8));
     RCHK(d2d(nd.hsml[0] + m.p_off, d_.hsml[0] + o.p_off, n8));
     RCHK(d2d(nd.mass + m.p_off, d_.mass + o.p_off, n8));
+    RCHK(d2d(nd.kz + m.p_off, d_.kz + o.p_off, n8));
     RCHK(d2d(nd.orig + m.p_off, d_.orig + o.p_off, n4));
     RCHK(d2d(nd.xb + m.b_off, d_.xb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.yb + m.b_off, d_.yb + o.b_off, (size_t)m.nb * 8));
@@ -357,6 +364,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   if (na > 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size() && d_.hq) {
     const InstMeta &o = meta_[carry_store_from];
     RCHK(d2d(nd.hq + nm[nk].q_off, d_.hq + o.q_off, (size_t)std::min(o.q_cap, nm[nk].q_cap) * 8));
+    // kernel_zero persists by particle index across calls as well
+    RCHK(d2d(nd.kz + nm[nk].p_off, d_.kz + o.p_off, (size_t)std::min(o.n, nm[nk].n) * 8));
   }
 
   // added instances: host to device (staged in SoA)
@@ -446,6 +455,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   meta_ = nm;
   ids_ = nids;
   any_boundary_ = nbtot > 0;
+  any_sumd_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.den_alg == 1; });
   if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
   if (ninst > 0) {
     // grid of the next evaluation of every instance
@@ -495,7 +505,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const DevState &d = d_;
   const bool fork = any_boundary_ && !prof;
   enqueue_prefix(buf, fork);
-  L("passA", [&] { launch_passA(d, buf, precision_, want_diag ? 1 : 0, st_); });
+  L("passA", [&] { launch_passA(d, buf, precision_, any_sumd_, want_diag ? 1 : 0, st_); });
   if (any_boundary_ || phase == 2) {
     // pair ranks: needed by the boundary kernel's hsml_ij read, and for nip
     L("pairtot", [&] { launch_pairtot(d, st_); });
@@ -503,7 +513,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   }
   if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
   if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
-  L("passB", [&] { launch_passB(d, precision_, phase, st_); });
+  L("passB", [&] { launch_passB(d, precision_, any_sumd_, phase, st_); });
 }
 
 void Engine::enqueue_step() {
@@ -685,12 +695,20 @@ int Engine::download_diag(int k, double *eps3, double *zeta, double *eta, double
   return 0;
 }
 
+int Engine::download_kernel_zero(int k, double *kz) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  const InstMeta &m = meta_[k];
+  HIPCHK(hipMemcpy(kz, d_.kz + m.p_off, (size_t)m.n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, double *hij,
-                         int bcap, int *bnip, int *bpi, int *bpj, double *brij) {
+                         int bcap, int *bnip, int *bpi, int *bpj, double *brij, double *w) {
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
   enqueue_prefix(0, false);
-  launch_passA(d, 0, precision_, 0, st_);
+  launch_passA(d, 0, precision_, any_sumd_, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_);
   int rc = sync_and_check();
@@ -702,17 +720,18 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   *nip = n_ip;
   *bnip = n_bp;
   int *dpi = nullptr, *dpj = nullptr, *dbi = nullptr, *dbj = nullptr;
-  double *dr = nullptr, *dh = nullptr, *dbr = nullptr;
+  double *dr = nullptr, *dh = nullptr, *dbr = nullptr, *dw = nullptr;
   const int c1 = std::max(std::min(cap, n_ip), 1), c2 = std::max(std::min(bcap, n_bp), 1);
   bool ok = hipMalloc((void **)&dpi, c1 * 4) == hipSuccess &&
             hipMalloc((void **)&dpj, c1 * 4) == hipSuccess &&
             hipMalloc((void **)&dr, c1 * 8) == hipSuccess &&
             hipMalloc((void **)&dh, c1 * 8) == hipSuccess &&
+            (!w || hipMalloc((void **)&dw, c1 * 8) == hipSuccess) &&
             hipMalloc((void **)&dbi, c2 * 4) == hipSuccess &&
             hipMalloc((void **)&dbj, c2 * 4) == hipSuccess &&
             hipMalloc((void **)&dbr, c2 * 8) == hipSuccess;
   if (ok) {
-    launch_export_pairs(d, 0, std::min(cap, n_ip), dpi, dpj, dr, dh, st_);
+    launch_export_pairs(d, 0, std::min(cap, n_ip), dpi, dpj, dr, dh, dw, st_);
     launch_export_bpairs(d, 0, any_boundary_ ? meta_[0].nb : 0, std::min(bcap, n_bp), dbi, dbj,
                          dbr, st_);
     ok = hipStreamSynchronize(st_) == hipSuccess;
@@ -722,6 +741,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
       if (pj) ok &= hipMemcpy(pj, dpj, k1 * 4, hipMemcpyDeviceToHost) == hipSuccess;
       if (rij) ok &= hipMemcpy(rij, dr, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
       if (hij) ok &= hipMemcpy(hij, dh, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
+      if (w) ok &= hipMemcpy(w, dw, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
     }
     if (ok && k2 > 0) {
       if (bpi) ok &= hipMemcpy(bpi, dbi, k2 * 4, hipMemcpyDeviceToHost) == hipSuccess;
@@ -729,7 +749,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
       if (brij) ok &= hipMemcpy(brij, dbr, k2 * 8, hipMemcpyDeviceToHost) == hipSuccess;
     }
   }
-  hipFree(dpi); hipFree(dpj); hipFree(dr); hipFree(dh);
+  hipFree(dpi); hipFree(dpj); hipFree(dr); hipFree(dh); hipFree(dw);
   hipFree(dbi); hipFree(dbj); hipFree(dbr);
   if (!ok) return -2;
   if (n_ip > cap || n_bp > bcap) return -3;

@@ -13,12 +13,13 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_cellsort(const DevState &d, int buf, hipStream_t st);
 void launch_bprep(const DevState &d, hipStream_t st);
-void launch_passA(const DevState &d, int buf, int mixed, int want_diag, hipStream_t st);
+void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
+                  hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);
 void launch_materialize(const DevState &d, hipStream_t st);
-void launch_passB(const DevState &d, int mixed, int phase, hipStream_t st);
+void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
-                         double *hij, hipStream_t st);
+                         double *hij, double *w, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st);
 

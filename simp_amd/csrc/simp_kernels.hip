@@ -574,6 +574,21 @@ __device__ __forceinline__ double c6_grad(double r, double mh) {
   return factor * (-(22. * q * o7 * (16 * ipow<2>(q) + 7. * q + 1))) / mh;
 }
 
+// Wendland C6 kernel value and the factor kernel_zero stores
+// (kernel.f90:499-506)
+__device__ __forceinline__ double c6_factor(double mh) {
+  double ih = 1.0 / mh;
+  return kC6 * (ih * ih);
+}
+
+__device__ __forceinline__ double c6_w(double r, double mh) {
+  double q = r / mh * 1.0;
+  double ih = 1.0 / mh;
+  double factor = kC6 * (ih * ih);
+  double omq = 1 - q;
+  return factor * ipow<8>(omq) * (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
+}
+
 __device__ __forceinline__ float c6_grad_f(float r, float mh) {
   float ih = __builtin_amdgcn_rcpf(mh);
   float q = r * ih;
@@ -611,7 +626,9 @@ __device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
   return true;
 }
 
-template <bool MIXED>
+// SUMD: instances with summation density may be present (the density
+// sums are compiled in; each instance still follows its own den_alg)
+template <bool MIXED, bool SUMD>
 __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
   int self = 0, k = 0;
   if (!list_slot(d, self, k)) return;
@@ -625,12 +642,25 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const double vxi = d.vxs[self], vyi = d.vys[self];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
+  const int p = d.cl[self];
+  // summation density (density.f90:41-100) on this evaluation's kernel
+  // values, accumulated in the same per-particle order as the reference.
+  // FP64 in both precision modes: the density sets the next smoothing
+  // length, and through it the pair predicate d <= mhsml, which is not
+  // continuous -- an FP32 error there flips borderline pairs.
+  const bool sumd = SUMD && m.den_alg == 1;
   double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
+  double num = 0., den = 0.;
   KSum fdiv, f1, f2, f3, f4;
   // kernel gradient, vector_divergence and vector_gradient of one pair
   auto pair = [&](int s, double dx, double dy, double r, double mh) {
     const double mq = d.ms[s], rq = d.rs[s];
     const double dv0 = vxi - d.vxs[s], dv1 = vyi - d.vys[s];
+    if (sumd) {
+      const double w = c6_w(r, mh);
+      den = den + mq * w / rq;
+      num = num + mq * w;
+    }
     if (!MIXED) {
       const double gg = c6_grad(r, mh);
       const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
@@ -693,6 +723,46 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     }
   }
   const size_t N = (size_t)d.ntot;
+  // kernel_zero(i) (kernel.f90:503): the factor of the last pair in the
+  // reference's order that has this particle first, i.e. its last owned
+  // pair; particles that own none keep the value of an earlier evaluation.
+  // Owned candidates form the tail of the visiting order, so a backward
+  // walk finds it.  It must be known before the density sums start.
+  double kzi = 0.;
+  if (sumd) {
+    kzi = d.kz[p];
+    bool found = false;
+    if (nc <= kNbrMax) {
+      for (int j = nc - 1; j >= 0 && !found; --j) {
+        const unsigned e = (unsigned)lst[j * kTPB + threadIdx.x];
+        if ((e >> 30) == 0) break;
+        const int t = (int)(e & 0x3fffffffu);
+        const double dx = xi - d.xs[t], dy = yi - d.ys[t];
+        const double mh = (hi + d.hs[t]) / 2.;
+        double r;
+        if (in_support(dx * dx + dy * dy, mh, r)) {
+          kzi = c6_factor(mh);
+          found = true;
+        }
+      }
+    } else {
+      for (int g = 5; g >= 2 && !found; --g) {
+        for (int t = R.b[g] - 1; t >= R.a[g] && !found; --t) {
+          if (g == 2 && t <= self) break;
+          const double dx = xi - d.xs[t], dy = yi - d.ys[t];
+          const double mh = (hi + d.hs[t]) / 2.;
+          double r;
+          if (in_support(dx * dx + dy * dy, mh, r)) {
+            kzi = c6_factor(mh);
+            found = true;
+          }
+        }
+      }
+    }
+    const double mi = d.ms[self], rhoi = d.rs[self];
+    num = kzi * mi;
+    den = kzi * mi / rhoi;
+  }
   if (nc <= kNbrMax) {
     for (int j0 = 0; j0 < nc; j0 += kBatch) {
       int sj[kBatch], kj[kBatch];
@@ -751,7 +821,11 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     div = fdiv.value();
     g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
   }
-  const int p = d.cl[self];
+  if (sumd) {
+    d.kz[p] = kzi;
+    d.dnum[self] = num;
+    d.dsum[self] = num / den;
+  }
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
   d.own_cnt[p] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
 
@@ -914,7 +988,7 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 
 // phase: 0 = predictor (state S -> T), 1 = corrector (RHS on T, update S in
 // place), 2 = single evaluation (tendencies to d.tend)
-template <bool MIXED>
+template <bool MIXED, bool SUMD>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
   int self = 0, k = 0;
   const bool ok = list_slot(d, self, k);
@@ -946,6 +1020,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
     const float ir2 = (float)(1.0 / ri2);
     const float s11f = (float)s11d * ir2, s12f = (float)s12d * ir2, s22f = (float)s22d * ir2;
     KSum sx, sy;
+    // third density call of the step (time_integration.f90:262): the
+    // corrector's sum on the half-step kernel values with the densities
+    // the second call produced (pass A's dsum)
+    // (FP64 in both precision modes, as in pass A)
+    const bool sumd3 = SUMD && phase == 1 && m.den_alg == 1;
+    double den3 = 0.;
+    if (sumd3) den3 = d.kz[p] * mi / d.dsum[self];
+    auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
+      const double w = c6_w(sqrt(dx * dx + dy * dy), mh);
+      den3 = den3 + mq * w / rho2;
+    };
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
                            ST q11, ST q12, ST q22) {
       if (!MIXED) {
@@ -980,7 +1065,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
         int sj[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) sj[u] = j0 + u < nn ? d.nbr[(j0 + u) * N + self] : self;
-        double xq[kBatch], yq[kBatch], hq[kBatch], rq[kBatch], mq[kBatch];
+        double xq[kBatch], yq[kBatch], hq[kBatch], rq[kBatch], mq[kBatch], r2[kBatch];
         ST q11[kBatch], q12[kBatch], q22[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -988,6 +1073,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
           xq[u] = d.xs[t]; yq[u] = d.ys[t]; hq[u] = d.hs[t];
           rq[u] = d.rs[t]; mq[u] = d.ms[t];
           q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
+          r2[u] = sumd3 ? d.dsum[t] : 1.0;
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -996,6 +1082,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
           const double mh = (hi + hq[u]) / 2.;
           stress_pair(dx, dy, MIXED ? 0.0 : sqrt(dx * dx + dy * dy), mh, rq[u], mq[u], q11[u],
                       q12[u], q22[u]);
+          if (sumd3) dens_pair(dx, dy, mh, mq[u], r2[u]);
         }
       }
     } else {
@@ -1007,6 +1094,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
           stress_pair(dx, dy, r, mh, d.rs[s], d.ms[s], S11[s], S12[s], S22[s]);
+          if (sumd3) dens_pair(dx, dy, mh, d.ms[s], d.dsum[s]);
         }
       }
     }
@@ -1117,7 +1205,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       if (Ah > 1) Ah = 1;
       d.h[1][p] = hh;
       d.A[1][p] = Ah;
-      d.rho[1][p] = hh * kRhoIce;
+      // density (time_integration.f90:218): diagnostic, or the summation
+      // pass A computed on the step-start kernel values
+      d.rho[1][p] = (SUMD && m.den_alg == 1) ? d.dsum[self] : hh * kRhoIce;
       d.hsml[1][p] = hi;
     } else if (phase == 1) {
       // corrector (time_integration.f90:236-280) on the step-start state S
@@ -1138,7 +1228,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       if (An > 1) An = 1;
       d.h[0][p] = hn;
       d.A[0][p] = An;
-      const double rn = hn * kRhoIce;
+      double rn = hn * kRhoIce;
+      if (sumd3) rn = d.dnum[self] / den3;
       d.rho[0][p] = rn;
       // Marquis_hsml (smoothing_length.f90:136-142)
       double hs = sqrt(mi / rn) * kHsmlMult;
@@ -1163,7 +1254,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
 // Particle pairs owned by p at their ranks in the reference's order
 // (nnps.f90:259-411): p_i, p_j 1-based, r_ij, hsml_ij.
 __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int cap, int *pi,
-                                                       int *pj, double *rij, double *hij) {
+                                                       int *pj, double *rij, double *hij,
+                                                       double *w) {
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
   const InstDyn &y = d.dyn[k];
@@ -1194,6 +1286,7 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
         pj[rank] = d.orig[q] + 1;
         rij[rank] = r;
         hij[rank] = mh;
+        if (w) w[rank] = c6_w(r, mh);
       }
       ++rank;
     }
@@ -1263,11 +1356,16 @@ void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
 void launch_bprep(const DevState &d, hipStream_t st) {
   k_bprep<<<d.ninst, kBprepThreads, 0, st>>>(d);
 }
-void launch_passA(const DevState &d, int buf, int mixed, int want_diag, hipStream_t st) {
-  if (mixed)
-    k_passA<true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
+                  hipStream_t st) {
+  if (mixed && sumd)
+    k_passA<true, true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+  else if (mixed)
+    k_passA<true, false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+  else if (sumd)
+    k_passA<false, true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
   else
-    k_passA<false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<false, false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
 }
 void launch_pairtot(const DevState &d, hipStream_t st) {
   k_pairtot<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
@@ -1275,15 +1373,19 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
 void launch_materialize(const DevState &d, hipStream_t st) {
   k_materialize<<<d.ninst * kMatBlocks, kTPB, 0, st>>>(d);
 }
-void launch_passB(const DevState &d, int mixed, int phase, hipStream_t st) {
-  if (mixed)
-    k_passB<true><<<d.nblk, kTPB, 0, st>>>(d, phase);
+void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
+  if (mixed && sumd)
+    k_passB<true, true><<<d.nblk, kTPB, 0, st>>>(d, phase);
+  else if (mixed)
+    k_passB<true, false><<<d.nblk, kTPB, 0, st>>>(d, phase);
+  else if (sumd)
+    k_passB<false, true><<<d.nblk, kTPB, 0, st>>>(d, phase);
   else
-    k_passB<false><<<d.nblk, kTPB, 0, st>>>(d, phase);
+    k_passB<false, false><<<d.nblk, kTPB, 0, st>>>(d, phase);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
-                         double *hij, hipStream_t st) {
-  k_export_pairs<<<d.nblk, kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij);
+                         double *hij, double *w, hipStream_t st) {
+  k_export_pairs<<<d.nblk, kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w);
 }
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st) {

@@ -55,6 +55,15 @@ module simp_binding
             integer(c_int), intent(out) :: nip, bnip
         end function simp_single_step
 
+        integer(c_int) function simp_kernel_values(h, cap, nip, p_i, p_j,    &
+                w, kernel_zero) bind(C, name='simp_kernel_values')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+            integer(c_int), value :: cap
+            integer(c_int), intent(out) :: nip, p_i(*), p_j(*)
+            real(c_double), intent(out) :: w(*), kernel_zero(*)
+        end function simp_kernel_values
+
         integer(c_int) function simp_upload(h, n, x, vx, mass, rho, hsml,   &
                 hth, A, nb, xb, massb, dt, current_time, itimestep)          &
                 bind(C, name='simp_upload')

@@ -8,10 +8,10 @@
 ! Reads  ntotal, x, vx, mass, rho, hsml, h, A, ntotalb, xb, massb,
 !        current_time, itimestep, max_hsml
 ! Writes dxdt, dvxdt, dhdt, dAdt, epsilon_11/12/22, zeta, eta, p,
-!        sigma_11/12/22, nip, bnip
-! The pair lists themselves (p_i, p_j, r_ij, hsml_ij, w, dwdx) stay on the
-! device; only the inactive options that read them on the host (sum_density,
-! Benz_hsml, Monaghan_time_step) would need them (simp_nnps exports them).
+!        sigma_11/12/22, nip, bnip; with den_alg = 1 also p_i, p_j, w and
+!        kernel_zero, which sum_density (density.f90:41-100) reads next
+! The other pair arrays (r_ij, hsml_ij, dwdx, boundary pairs) stay on the
+! device; no active caller reads them on the host.
 !---------------------------------------------------------------------------
 subroutine single_step
     use globals
@@ -27,6 +27,7 @@ subroutine single_step
     if (.not. c_associated(handle)) then
         call simp_config_default(cfg)
         cfg%max_hsml = max_hsml
+        cfg%den_alg = den_alg
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif
@@ -49,4 +50,9 @@ subroutine single_step
     sigma_22(:ntotal) = sig3(3, :)
     nip = nipc
     bnip = bnipc
+    if (den_alg == 1) then
+        st = simp_kernel_values(handle, size(w), nipc, p_i, p_j, w,       &
+                                kernel_zero)
+        call simp_check(st, 'single_step kernel values')
+    endif
 end subroutine single_step

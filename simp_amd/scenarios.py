@@ -60,6 +60,7 @@ class Scenario:
     wind_grid: Optional[WindGrid] = None
     current: tuple = (0.0, 0.0)
     tse: int = 2
+    den_alg: int = 3         # 3 diagnostic density, 1 summation density
     dt: float = 0.0
     time: float = 0.0
     name: str = ""

@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1]
 """
 from __future__ import annotations
 
@@ -132,5 +132,55 @@ def main():
           **_outputs(r, STEP_KEYS), out_nip=r["nip"], out_bnip=r["bnip"])
 
 
+def summation_density():
+    """6. the ice edge with summation density (den_alg = 1, density.f90:41-100),
+    from the harness built with that PARAMETER: three density calls per step
+    on the last single_step's kernel values and the stale kernel_zero."""
+    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
+    sc.wind_grid = None
+    sc.wind = S.REFERENCE_WIND
+    sc.den_alg = 1
+    K = 10
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
+                   harness=refrun.HARNESS_DEN1)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save("edge3600_pc10_den1.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end, den_alg=1)
+
+
+def stale_kernel_zero():
+    """7. two consecutive single_step calls, the second on a permuted state
+    with a smaller support: particles that are first in no pair of the
+    second call keep the kernel_zero the first call wrote (kernel.f90:503)."""
+    a = S.box(nside=12, spacing=1000.0, seed=5, hsml_factor=2.2)
+    perm = np.random.default_rng(11).permutation(a.n)
+    b = a.copy()
+    for k in ("x", "vx", "mass", "rho", "hsml", "h", "A"):
+        setattr(b, k, np.ascontiguousarray(getattr(a, k)[perm]))
+    b.hsml = b.hsml * (1.3 / 2.2)
+    b.vx = np.random.default_rng(7).normal(0.0, 0.1, size=(b.n, 2))
+    a.time = b.time = 3.0
+    r = refrun.run(refrun.MODE_TWO_STATES, a, second=b, harness=refrun.HARNESS_DEN1)
+    owners = set(r["p_i"].tolist())
+    stale = [i for i in range(b.n) if i + 1 not in owners]
+    assert stale and np.all(r["kernel_zero"][stale] != 0)
+    _save("stale_kernel_zero.npz", **_inputs(a, "a_"), **_inputs(b, "in_"),
+          **_outputs(r, ("p_i", "p_j", "w", "kernel_zero")), out_nip=r["nip"], den_alg=1)
+
+
 if __name__ == "__main__":
-    main()
+    # no argument: every fixture; otherwise the named groups ("main", "den1")
+    todo = sys.argv[1:] or ["main", "den1"]
+    if "main" in todo:
+        main()
+    if "den1" in todo:
+        summation_density()
+        stale_kernel_zero()

@@ -22,6 +22,7 @@ def scenario(g, prefix="in_") -> Scenario:
                   A=g[prefix + "A"].copy(), xb=g[prefix + "xb"].reshape(-1, 2).copy(),
                   massb=g[prefix + "massb"].copy(), max_hsml=float(max_hsml),
                   wind=REFERENCE_WIND, tse=2, dt=float(dt), time=float(t))
+    sc.den_alg = int(g.get("den_alg", 3))
     return sc
 
 

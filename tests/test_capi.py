@@ -47,7 +47,7 @@ def test_config_defaults_are_the_reference_switches():
 
 
 @pytest.mark.parametrize("field,value", [("skf", 4), ("rheology", 2), ("tse", 1),
-                                         ("den_alg", 1), ("int_algorithm", 1),
+                                         ("den_alg", 2), ("int_algorithm", 1),
                                          ("precision", 7)])
 def test_unsupported_switches_are_rejected_before_any_device_work(field, value):
     cfg = make_config()

@@ -15,17 +15,25 @@ from oracle import refrun
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.skipif(not (os.path.exists(refrun.HARNESS) and os.path.exists(refrun.HARNESS_GPU)),
-                    reason="oracle/_ref harnesses not built (oracle/build_ref.sh)")
-@pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz"])
-def test_reference_time_integration_with_gpu_single_step(name):
+CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
+         ("edge3600_pc10.npz", refrun.HARNESS, refrun.HARNESS_GPU),
+         # summation density: the reference's sum_density runs on the host on
+         # the kernel values and kernel_zero the shim exports
+         ("edge3600_pc10_den1.npz", refrun.HARNESS_DEN1, refrun.HARNESS_GPU_DEN1)]
+
+
+@pytest.mark.parametrize("name,cpu_exe,gpu_exe", CASES)
+def test_reference_time_integration_with_gpu_single_step(name, cpu_exe, gpu_exe):
+    if not (os.path.exists(cpu_exe) and os.path.exists(gpu_exe)):
+        pytest.skip("oracle/_ref harnesses not built (oracle/build_ref.sh)")
     print(f"[gpu] drop-in harness {name}", flush=True)
     g = load(name)
     sc = scenario(g)
     t_end = float(g["integration_time"])
-    cpu = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end)
+    cpu = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
+                     harness=cpu_exe)
     gpu = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
-                     harness=refrun.HARNESS_GPU)
+                     harness=gpu_exe)
     assert gpu["itimestep"] == cpu["itimestep"] == int(g["out_itimestep"])
     assert gpu["nip"] == cpu["nip"] and gpu["bnip"] == cpu["bnip"]
     for k in ("x", "vx", "rho", "hsml", "A", "h", "eps", "sigma", "p", "zeta"):

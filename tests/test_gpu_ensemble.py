@@ -110,3 +110,21 @@ def test_a_failing_instance_does_not_disturb_the_others():
         want = _alone(sc, 4, _lib.SIMP_FP64)
         for f in FIELDS:
             assert np.array_equal(got[f], getattr(want, f)), (i, f)
+
+
+def test_summation_density_members_across_a_repack():
+    say("ensemble den_alg=1 members across add")
+    scs = _members(count=3, seed=59)
+    for sc in scs:
+        sc.den_alg = 1
+    prec = _lib.SIMP_FP64
+    ens = Ensemble(prec)
+    ia, ib = ens.add(scs[:2])
+    ens.step(3)
+    (ic,) = ens.add([scs[2]])      # the repack carries kernel_zero of a and b
+    ens.step(3)
+    for sc, i, k in ((scs[0], ia, 6), (scs[1], ib, 6), (scs[2], ic, 3)):
+        got = ens.download(i)
+        want = _alone(sc, k, prec)
+        for f in FIELDS:
+            assert np.array_equal(got[f], getattr(want, f)), (i, f)

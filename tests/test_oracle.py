@@ -33,7 +33,8 @@ def test_single_step_bit_exact_vs_reference():
         assert _same(out[k], g["out_" + k]), k
 
 
-@pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz"])
+@pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz",
+                                  "edge3600_pc10_den1.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
@@ -45,6 +46,22 @@ def test_time_integration_bit_exact_vs_reference(name):
         assert _same(getattr(sc, k), g["out_" + k]), k
     for k in ("eps", "zeta", "eta", "p", "sigma"):
         assert _same(out[k], g["out_" + k]), k
+
+
+def test_stale_kernel_zero_two_calls_vs_reference():
+    """w and kernel_zero after two single_step calls (kernel.f90:503-511):
+    particles first in no pair of the second call keep the first call's
+    kernel_zero, as the reference's module array does."""
+    g = load("stale_kernel_zero.npz")
+    a, b = scenario(g, "a_"), scenario(g, "in_")
+    o = capi.Oracle()
+    o.single_step(a)
+    o.single_step(b)
+    P = o._pairs()
+    kv = o.kernel_values(b.n)
+    assert _same(P.p_i, g["out_p_i"]) and _same(P.p_j, g["out_p_j"])
+    assert _same(kv["w"], g["out_w"])
+    assert _same(kv["kernel_zero"], g["out_kernel_zero"])
 
 
 def test_nnps_ties_vs_reference():
@@ -259,3 +276,20 @@ def test_vp_time_step_and_hsml_update():
     assert dt == min(900.0 * h * h * 2e-9 / 2.75e4 for h in sc.hsml)
     # Marquis hsml: mass = rho (hsml/3)^2 => hsml = 3 sqrt(m / rho) is a fixed point
     assert np.allclose(3 * np.sqrt(sc.mass / sc.rho), sc.hsml, rtol=1e-15)
+
+
+def test_summation_density_keeps_a_uniform_density():
+    """sum_density (density.f90:78-98) is a normalised (Shepard) sum: with one
+    density everywhere, numerator / denominator returns it."""
+    sc = S.box(nside=20, spacing=1000.0, seed=9)
+    sc.rho[:] = 950.0
+    o = capi.Oracle()
+    o.single_step(sc)
+    rho = o.sum_density(sc.mass, sc.rho)
+    assert np.max(np.abs(rho - 950.0)) <= 1e-12 * 950.0
+    # and a non-uniform density moves towards the kernel-weighted mean
+    sc.rho[:] = np.linspace(900.0, 1000.0, sc.n)
+    o.single_step(sc)
+    rho2 = o.sum_density(sc.mass, sc.rho)
+    assert rho2.min() >= sc.rho.min() and rho2.max() <= sc.rho.max()
+    assert np.std(rho2) < np.std(sc.rho)
