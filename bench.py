@@ -168,6 +168,24 @@ def hbm_traffic(kernel_substr, timeout=240):
                                                  "fetch_kib": fetch, "write_kib": write}
 
 
+def bench_dropin(sc, calls=10):
+    """The per-call drop-in (simp_single_step): one right-hand side on host
+    arrays per call, the state copied in and the tendencies and diagnostics
+    copied out over PCIe, the way a Fortran time_integration calls it."""
+    from simp_amd import _lib
+    from simp_amd.api import Simp
+    dev = Simp.for_scenario(sc, precision=_lib.SIMP_FP64)
+    dev.single_step(sc)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        dev.single_step(sc)
+    dt = time.perf_counter() - t0
+    dev.close()
+    return {"calls": calls, "ms_per_call": 1e3 * dt / calls,
+            "particle_evaluations_per_s": sc.n * calls / dt,
+            "note": "PCIe-inclusive, FP64; one call = one right-hand side (half a step)"}
+
+
 def bench_ensemble(steps=200, baseline_steps=50):
     """Config 4: 64 closed boxes, particle counts lognormal in [2k, 60k],
     box 50-300 km, random uniform wind; all stepped in one packed launch per
@@ -323,6 +341,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    line["dropin_single_step"] = bench_dropin(sc)
     if not args.no_ensemble:
         line["ensemble"] = bench_ensemble()
     if not args.no_serve:
