@@ -88,6 +88,7 @@ struct DevState {
   // gathers run one thread per list position and read neighbours here
   double *xs, *ys, *hs, *vxs, *vys, *ms, *rs;
   int *cells;            // cell (local id) of each list position
+  float *cellh;          // per cell: largest member hsml (FP32, rounded up)
   float4 *pf;            // FP32 (x, y, hsml) in list order, x and y relative to
                          // the grid origin: the candidate prefilter
   // neighbour list of the evaluation, built by pass A in the reference's
@@ -99,6 +100,7 @@ struct DevState {
   int *bflag;            // per cell: stamp of the last evaluation in which a
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
+  int2 *gpre;            // per list position: pair prefixes (intra, cross) of k_ranks
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
   // summation density (den_alg = 1): kernel_zero of each particle (by
   // packed index; persists across evaluations like the reference's module
@@ -111,7 +113,12 @@ struct DevState {
   // boundary
   double *xb, *yb, *massb;
   int *bpre, *bbase, *bcount;
-  unsigned long long *bsorted;   // (cell << 32 | b) sorted, per instance
+  // per cell covered by a boundary particle's 3x3 block: the run
+  // [brun.x, brun.y) of blist holding those boundary particles (local index
+  // | slot of the cell in its block << 24) in ascending index; bcellc holds
+  // the stamp of the evaluation whose builder last claimed the cell
+  int2 *brun;
+  int *blist, *bcellc;
   double *hq;            // persistent hsml_ij store (stale semantics)
   double *grid;          // wind grids, u then v per instance
   InstMeta *meta;

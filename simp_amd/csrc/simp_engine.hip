@@ -86,9 +86,9 @@ static int store_capacity(const InstanceInput &in) {
 
 // bytes per particle / boundary particle / cell of the device layout
 constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
-                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8;
-constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 8;
-constexpr size_t kBytesPerCell = 6 * 4;
+                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8 + 8;
+constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4;
+constexpr size_t kBytesPerCell = 10 * 4;
 
 size_t instance_footprint(const InstanceInput &in) {
   long c = cell_capacity(in);
@@ -142,6 +142,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.clpos, N);
   rc |= dalloc(list, &d.own_cnt, N);
   rc |= dalloc(list, &d.moff, N);
+  rc |= dalloc(list, &d.gpre, N);
   rc |= dalloc(list, &d.xs, N);
   rc |= dalloc(list, &d.ys, N);
   rc |= dalloc(list, &d.hs, N);
@@ -160,6 +161,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.pair_count, C);
   rc |= dalloc(list, &d.pair_start, C);
   rc |= dalloc(list, &d.cell_intra, C);
+  rc |= dalloc(list, &d.cellh, C);
   double *sg = nullptr;
   rc |= dalloc(list, &sg, N); d.sig11 = sg;
   rc |= dalloc(list, &sg, N); d.sig12 = sg;
@@ -177,7 +179,9 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.bpre, 9 * NB);
   rc |= dalloc(list, &d.bbase, NB);
   rc |= dalloc(list, &d.bcount, NB);
-  rc |= dalloc(list, &d.bsorted, NB);
+  rc |= dalloc(list, &d.brun, C);
+  rc |= dalloc(list, &d.bcellc, C);
+  rc |= dalloc(list, &d.blist, 9 * NB);
   rc |= dalloc(list, &d.grid, std::max<size_t>(G, 1));
   rc |= dalloc(list, &d.meta, std::max<size_t>(INST, 1));
   rc |= dalloc(list, &d.dyn, std::max<size_t>(INST, 1));
@@ -333,6 +337,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // kernel_zero starts at zero, as the reference's module array does
   RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
+  RCHK(hipMemsetAsync(nd.bcellc, 0xff, (ctot + 1) * sizeof(int), st_));
 
   // kept instances: device to device
   auto d2d = [&](void *dst, const void *src, size_t bytes) {
@@ -506,13 +511,17 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const bool fork = any_boundary_ && !prof;
   enqueue_prefix(buf, fork);
   L("passA", [&] { launch_passA(d, buf, precision_, any_sumd_, want_diag ? 1 : 0, st_); });
-  if (any_boundary_ || phase == 2) {
-    // pair ranks: needed by the boundary kernel's hsml_ij read, and for nip
+  if (phase == 2) {
+    // all pair ranks (nip is reported), then the boundary kernel's hsml_ij
     L("pairtot", [&] { launch_pairtot(d, st_); });
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_); });
+    if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
+    if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
+  } else {
+    // the step loop: only the head of the rank order the boundary kernel reads
+    if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
+    if (any_boundary_) L("ranks", [&] { launch_ranks(d, st_); });
   }
-  if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
-  if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
   L("passB", [&] { launch_passB(d, precision_, any_sumd_, phase, st_); });
 }
 

@@ -40,6 +40,7 @@
 // reference's, so the FP64 path reproduces the reference's results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "simp_device.h"
@@ -79,8 +80,13 @@ constexpr float kC6f = (float)(78.0 / (7.0 * kPi));
 // particle handled by this thread.  The block table is laid out so that
 // the blocks the hardware sends to one XCD (block ids congruent mod 8) cover
 // a contiguous run of particles, which keeps their neighbours in that XCD's
-// L2.
+// L2.  Workgroups go to the XCDs round-robin, continuing from where the
+// previous dispatch stopped, so every grid is a multiple of 8 blocks: then a
+// block id lands on the same XCD in every kernel, and the data one kernel
+// writes for a run of particles is in the L2 the next kernel reads it from.
+// The padding blocks (id >= the real count) exit at once.
 __device__ __forceinline__ bool particle_of(const DevState &d, int &p, int &k) {
+  if (blockIdx.x >= (unsigned)d.nblk) return false;
   BlockEntry e = d.blk[blockIdx.x];
   k = e.inst;
   int local = e.first + (int)threadIdx.x;
@@ -176,6 +182,7 @@ __device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok
 }
 
 __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
   int p = 0, k = 0;
   bool ok = particle_of(d, p, k);
   double xv = ok ? d.x[buf][p] : 0.0, yv = ok ? d.y[buf][p] : 0.0;
@@ -190,6 +197,7 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
 // (time_integration.f90:68-71).
 __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) {
   __shared__ double sh[5][kTPB / 64];
+  if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
   InstDyn &y = d.dyn[k];
@@ -272,7 +280,8 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
 
 // ------------------------------------------------------------------- scan
 
-__global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum) {
+__global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum, int nblocks) {
+  if (blockIdx.x >= (unsigned)nblocks) return;
   constexpr int IPT = kScanItems / kTPB;
   int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
   int s = 0;
@@ -287,7 +296,8 @@ __global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum)
 
 // out[i] = sum(in[0..i-1]); out[L] = total.  Optionally clears in[].
 __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum, int *out,
-                                                int zero_in) {
+                                                int zero_in, int nblocks) {
+  if (blockIdx.x >= (unsigned)nblocks) return;
   constexpr int IPT = kScanItems / kTPB;
   int acc = 0;
   for (int j = threadIdx.x; j < (int)blockIdx.x; j += kTPB) acc += bsum[j];
@@ -312,7 +322,7 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
     }
     ex += v[i];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kTPB - 1) out[L] = ex;
+  if (blockIdx.x == (unsigned)nblocks - 1 && threadIdx.x == kTPB - 1) out[L] = ex;
 }
 
 // ------------------------------------------------------- scatter/cellsort
@@ -327,24 +337,33 @@ __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
 // members of each cell in ascending particle id: the order in which the
 // reference fills its bins (nnps.f90:246-254).  Cells of up to kRankMax
 // members (the common case) are ranked in registers; larger ones are
-// insertion-sorted in place.
+// insertion-sorted in place.  Also the cell's largest smoothing length
+// (rounded up to FP32), which bounds the support of any pair with a member
+// for the gathers' cell culling.
 constexpr int kRankMax = 16;
 
-__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
+__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   int c = blockIdx.x * kTPB + threadIdx.x;
   if (c >= d.ctot) return;
   const int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
   const int n = s1 - s0;
-  if (n <= 1) return;
+  if (n == 0) {
+    d.cellh[c] = 0.0f;
+    return;
+  }
   for (int s = s0; s < s1; ++s)
     if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: skipped instance
+  const double *hs = d.hsml[buf];
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
+    double hm = 0.0;
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
       v[i] = i < n ? d.cl[s0 + i] : 0;
       key[i] = i < n ? d.orig[v[i]] : 0x7fffffff;
+      if (i < n) hm = fmax(hm, hs[v[i]]);
     }
+    d.cellh[c] = (float)hm * 1.000001f;
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
       if (i < n) {
@@ -356,6 +375,9 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
     }
     return;
   }
+  double hm = 0.0;
+  for (int s = s0; s < s1; ++s) hm = fmax(hm, hs[d.cl[s]]);
+  d.cellh[c] = (float)hm * 1.000001f;
   for (int a = s0 + 1; a < s1; ++a) {
     int v = d.cl[a];
     int key = d.orig[v];
@@ -374,6 +396,7 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d) {
 // one thread per list position.  Also the FP32 positions relative to the
 // instance's grid origin used by the candidate prefilter.
 __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
   const BlockEntry e = d.blk[blockIdx.x];
   const int k = e.inst;
   const InstMeta &m = d.meta[k];
@@ -402,13 +425,27 @@ __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
 // One block per instance: per boundary particle, the number of boundary
 // pairs found in each of its 3x3 cells in the reference's scan order
 // (y outer, x inner; nnps.f90:458-491), the exclusive prefix over boundary
-// particles (= rank of its first pair), the boundary particles sorted by
-// cell for the particle-side lookup, and the stamp of every cell a boundary
-// particle's 3x3 block covers.
+// particles (= rank of its first pair), the stamp of every cell a boundary
+// particle's 3x3 block covers, and for each such cell the list of those
+// boundary particles in ascending index -- the order in which the
+// reference's boundary pair list reaches a particle of that cell -- with the
+// slot the cell takes in each one's 3x3 block.
 constexpr int kBprepThreads = 1024;
+
+__device__ __forceinline__ int lower_bound_key(const unsigned long long *keys, int n,
+                                               unsigned long long v) {
+  int a = 0, b = n;
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (keys[mid] < v) a = mid + 1; else b = mid;
+  }
+  return a;
+}
 
 __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
   __shared__ unsigned long long keys[kMaxBoundaryPerInst];
+  __shared__ int alloc;
+  if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
   const InstDyn &y = d.dyn[k];
@@ -435,6 +472,7 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
   int P = 1;
   while (P < m.nb) P <<= 1;
   for (int i = m.nb + threadIdx.x; i < P; i += kBprepThreads) keys[i] = ~0ull;
+  if (threadIdx.x == 0) alloc = 0;
   __syncthreads();
   // prefix over the 9 cells of each boundary particle, then over particles
   const int per = (m.nb + kBprepThreads - 1) / kBprepThreads;
@@ -478,7 +516,54 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < m.nb; i += kBprepThreads) d.bsorted[m.b_off + i] = keys[i];
+  // Per covered cell, one thread (the first to stamp the cell's claim word
+  // in this evaluation) lists the boundary particles whose own cell lies in
+  // the cell's 3x3 block -- three key ranges, one per row -- in ascending
+  // index.
+  int2 *brun = d.brun + m.c_off;
+  int *blist = d.blist + 9 * (size_t)m.b_off;
+  for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
+    const int b = idx / 9, s9 = idx - 9 * (idx / 9);
+    const int bg = m.b_off + b;
+    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
+    const int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
+    if (atomicExch(&d.bcellc[m.c_off + c], stamp) == stamp) continue;
+    int lo[3], hi[3], total = 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      // cells c0 .. c0 + 2 of one row (a row outside the grid is empty;
+      // the cells a row range borrows from a neighbouring row are margin
+      // cells, which hold no boundary particle)
+      const int c0 = c + (r - 1) * W - 1;
+      lo[r] = hi[r] = 0;
+      if (c0 + 3 > 0) {
+        lo[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)max(c0, 0) << 32);
+        hi[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)(c0 + 3) << 32);
+      }
+      total += hi[r] - lo[r];
+    }
+    const int start = atomicAdd(&alloc, total);
+    brun[c] = make_int2(start, start + total);
+    int last = -1;
+    for (int j = 0; j < total; ++j) {
+      int best = 0x7fffffff, bcell = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        for (int t = lo[r]; t < hi[r]; ++t) {
+          const int bl = (int)(keys[t] & 0xffffffffu);
+          if (bl > last && bl < best) {
+            best = bl;
+            bcell = (int)(keys[t] >> 32);
+          }
+        }
+      }
+      last = best;
+      // slot of c in the 3x3 block of the boundary particle's cell
+      const int dyc = (c - bcell + W + 1) / W;     // 0, 1, 2
+      const int dxc = c - bcell - (dyc - 1) * W + 1;
+      blist[start + j] = best | ((dyc * 3 + dxc) << 24);
+    }
+  }
 }
 
 // --------------------------------------------------------------- neighbours
@@ -505,54 +590,22 @@ __device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx,
   return r;
 }
 
-__device__ __forceinline__ int range_kind(int g, int s, int self) {
-  if (g <= 1) return 0;
-  if (g == 2) return s > self ? 1 : 0;
-  return 2;
-}
-
 // The six ranges form four contiguous segments of the cell lists (the left,
 // own and right cells of a row are adjacent): [a0,b0) [a1,b3) [a4,b4)
-// [a5,a4).  A cursor walks them in order; batches of kBatch candidates are
-// loaded together so that a wavefront keeps several loads in flight.
-constexpr int kBatch = 4;
+// [a5,a4), walked in that order.  Rows lie in the lists in ascending order,
+// so a list position alone tells its kind: below the middle row 0, in the
+// top row 2, in the middle row 0 / 1 / 2 by left-or-earlier / later own-cell
+// member / right.
+constexpr int kBatch = 4;    // neighbour loads issued together (exact test, pass B)
+constexpr int kBatchF = 4;   // candidate loads issued together (prefilter)
 
-struct Cursor {
-  int seg, s, end;                  // current segment and its end
-  int n1a, n1e, n2a, n2e, n3a, n3e; // the segments still ahead, in order
-  int own0, own1;                   // own cell range
-  __device__ __forceinline__ void init(const Ranges &R) {
-    n1a = R.a[1]; n1e = R.b[3];
-    n2a = R.a[4]; n2e = R.b[4];
-    n3a = R.a[5]; n3e = R.a[4];
-    own0 = R.a[2]; own1 = R.b[2];
-    seg = 0;
-    s = R.a[0];
-    end = R.b[0];
-    skip();
-  }
-  // register shifts only: an indexed select here becomes a scratch table
-  __device__ __forceinline__ void skip() {
-    while (seg < 4 && s >= end) {
-      ++seg;
-      s = n1a; end = n1e;
-      n1a = n2a; n1e = n2e;
-      n2a = n3a; n2e = n3e;
-    }
-  }
-  __device__ __forceinline__ bool done() const { return seg >= 4; }
-  __device__ __forceinline__ void next() {
-    ++s;
-    skip();
-  }
-  // 0: not owned, 1: intra pair owned by self, 2: cross pair owned by self
-  __device__ __forceinline__ int kind(int self) const {
-    if (seg >= 2) return 2;
-    if (seg == 0 || s < own0) return 0;
-    if (s < own1) return s > self ? 1 : 0;
-    return 2;
-  }
-};
+__device__ __forceinline__ int pair_kind(const Ranges &R, int s, int self) {
+  if (s < R.a[1]) return 0;
+  if (s >= R.a[5]) return 2;
+  if (s < R.a[2]) return 0;
+  if (s < R.b[2]) return s > self ? 1 : 0;
+  return 2;
+}
 
 // Exact FP64 form of `d = SQRT(SUM(dx**2)); IF (d .LE. mhsml)`: the square
 // root is taken only when d^2 is within 1e-13 of mhsml^2, which decides the
@@ -617,6 +670,7 @@ struct KSum {
 // The gathers run one thread per cell-list position: the threads of one
 // cell walk the same candidate ranges, so a wavefront's loads coalesce.
 __device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
+  if (blockIdx.x >= (unsigned)d.nblk) return false;
   const BlockEntry e = d.blk[blockIdx.x];
   k = e.inst;
   const int local = e.first + (int)threadIdx.x;
@@ -626,16 +680,54 @@ __device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
   return true;
 }
 
+// LDS tile of the block's candidate region (FP32 prefilter data)
+constexpr int kTile = 1280;
+
 // SUMD: instances with summation density may be present (the density
 // sums are compiled in; each instance still follows its own den_alg)
 template <bool MIXED, bool SUMD>
 __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
-  int self = 0, k = 0;
-  if (!list_slot(d, self, k)) return;
+  __shared__ float4 tile[kTile];
+  __shared__ unsigned short lst[kNbrMax * kTPB];
+  if (blockIdx.x >= (unsigned)d.nblk) return;
+  const BlockEntry be = d.blk[blockIdx.x];
+  const int k = be.inst;
   const InstDyn &y = d.dyn[k];
-  if (inst_off(y)) return;
+  if (inst_off(y)) return;                // whole block: blocks never straddle instances
   const InstMeta &m = d.meta[k];
   const int W = y.W;
+  const int *cs0 = d.cell_start + m.c_off;
+  const int pos0 = cs0[0] + be.first;     // first list position of the block
+  const int nloc = min(m.n - be.first, kTPB);
+  const bool valid = (int)threadIdx.x < nloc;
+  const int self = pos0 + min((int)threadIdx.x, nloc - 1);
+  // The candidates of every particle of the block lie in three runs of the
+  // cell lists: its cell range shifted by -W, 0 and +W cells, widened by one
+  // cell on each side.  Overlapping runs (a block spanning most of a row)
+  // are merged.  The runs' prefilter data goes to LDS in one round of
+  // coalesced loads by the whole block; a block whose region does not fit
+  // takes the exact rescan path instead.
+  int As, Ae, Bs, Be, Cs, Ce;
+  {
+    const int cf = d.cells[pos0], cl = d.cells[pos0 + nloc - 1];
+    As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
+    Bs = cs0[cf - 1];     Be = cs0[cl + 2];
+    Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
+    if (Ae >= Bs) { Bs = As; Be = max(Ae, Be); As = Ae = 0x7fffffff; }
+    if (Be >= Cs) { Cs = min(Bs, Cs); Ce = max(Be, Ce); Bs = Be = 0x7fffffff; }
+  }
+  const int lA = Ae - As, lB = Be - Bs, lC = Ce - Cs;
+  const int tB = lA, tC = lA + lB, T = lA + lB + lC;
+  const int lbase = min(As, min(Bs, Cs));  // list entries are offsets from here
+  const bool tiled = T <= kTile && Ce - lbase < 65535;
+  if (tiled) {
+    for (int i = threadIdx.x; i < T; i += kTPB) {
+      const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
+      tile[i] = d.pf[s];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
   const int c = d.cells[self];
   const int cy = c / W, cx = c - cy * W;
   const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
@@ -653,9 +745,9 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   double num = 0., den = 0.;
   KSum fdiv, f1, f2, f3, f4;
   // kernel gradient, vector_divergence and vector_gradient of one pair
-  auto pair = [&](int s, double dx, double dy, double r, double mh) {
-    const double mq = d.ms[s], rq = d.rs[s];
-    const double dv0 = vxi - d.vxs[s], dv1 = vyi - d.vys[s];
+  auto pair = [&](double dx, double dy, double r, double mh, double mq, double rq, double vxq,
+                  double vyq) {
+    const double dv0 = vxi - vxq, dv1 = vyi - vyq;
     if (sumd) {
       const double w = c6_w(r, mh);
       den = den + mq * w / rq;
@@ -686,40 +778,64 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       f4.add(-(mr * w4));
     }
   };
-  // 1) FP32 prefilter of the 3x3 block into a candidate list (batched
-  //    loads); 2) exact FP64 test and pair arithmetic over the candidates,
-  //    which also writes the neighbour list.  Splitting the two keeps the
-  //    expensive arithmetic off the ~90 % of candidates that fail.
-  __shared__ int lst[kNbrMax * kTPB];
+  // 1) FP32 prefilter of the 3x3 block into a candidate list (from the
+  //    LDS tile); 2) exact FP64 test and pair arithmetic over the
+  //    candidates, which also writes the neighbour list.  Splitting the two
+  //    keeps the expensive arithmetic off the ~90 % of candidates that fail.
   int nI = 0, nX = 0, nn = 0, nc = 0;
-  {
-    const float4 pi4 = d.pf[self];
+  if (!tiled) {
+    nc = kNbrMax + 1;
+  } else {
+    const float4 pi4 = tile[self >= Cs ? self - Cs + tC : (self >= Bs ? self - Bs + tB : self - As)];
     const float slack = y.pf_slack;
-    Cursor cur;
-    cur.init(R);
-    while (!cur.done()) {
-      int sc[kBatch], kc[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const bool live = !cur.done();
-        sc[u] = live ? cur.s : self;
-        kc[u] = live ? cur.kind(self) : -1;
-        if (live) cur.next();
-      }
-      float4 q4[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) q4[u] = d.pf[sc[u]];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        if (kc[u] < 0 || sc[u] == self) continue;
+    // one segment of the 3x3 block (it lies inside one run of the tile)
+    auto scan = [&](int a, int b) {
+      const int off = a >= Cs ? tC - Cs : (a >= Bs ? tB - Bs : -As);
+      for (int sc = a; sc < b; ++sc) {
+        const float4 q = tile[sc + off];
         // rejects only candidates certainly beyond mhsml
-        const float fx = pi4.x - q4[u].x, fy = pi4.y - q4[u].y;
-        const float fm = (pi4.z + q4[u].z) * 0.5f + slack;
-        if (fx * fx + fy * fy > fm * fm * 1.00001f) continue;
-        if (nc < kNbrMax)
-          lst[nc * kTPB + threadIdx.x] = (int)((unsigned)sc[u] | ((unsigned)kc[u] << 30));
-        ++nc;
+        const float fx = pi4.x - q.x, fy = pi4.y - q.y;
+        const float fm = __builtin_fmaf(pi4.z + q.z, 0.5f, slack);
+        const bool hit = (sc != self) & (__builtin_fmaf(fx, fx, fy * fy) <= fm * fm * 1.00001f);
+        if (hit) {
+          if (nc < kNbrMax) lst[nc * kTPB + threadIdx.x] = (unsigned short)(sc - lbase);
+          ++nc;
+        }
       }
+    };
+    // Cell culling: a neighbour cell is scanned only if its nearest point
+    // is within the largest support a pair with one of its members can
+    // have, (hsml_i + max hsml of the cell) / 2 (cells span
+    // [(c - 1/2) bin, (c + 1/2) bin) in the prefilter's frame).  The
+    // scanned ranges shrink at their ends only, so the visiting order and
+    // the list are those of the full block.
+    const float binf = (float)y.bin;
+    const float dxl = fmaxf(pi4.x - ((float)cx - 0.5f) * binf - slack, 0.0f);
+    const float dxr = fmaxf(((float)cx + 0.5f) * binf - pi4.x - slack, 0.0f);
+    const float dyb = fmaxf(pi4.y - ((float)cy - 0.5f) * binf - slack, 0.0f);
+    const float dyt = fmaxf(((float)cy + 0.5f) * binf - pi4.y - slack, 0.0f);
+    const float *ch = d.cellh + m.c_off;
+    const int *cs = d.cell_start + m.c_off;
+    const int rm = (cy - 1) * W + cx, r0 = cy * W + cx, rp = (cy + 1) * W + cx;
+    auto need = [&](int c, float ex, float ey) {
+      const float bnd = __builtin_fmaf(pi4.z + ch[c], 0.5f, slack);
+      return __builtin_fmaf(ex, ex, ey * ey) <= bnd * bnd * 1.00001f;
+    };
+    // row below: cells rm-1, rm, rm+1 -> the span from the first to the
+    // last cell needed
+    {
+      const bool l = need(rm - 1, dxl, dyb), c0 = need(rm, 0.0f, dyb), r = need(rm + 1, dxr, dyb);
+      const int f = l ? 0 : c0 ? 1 : r ? 2 : 3, e = r ? 3 : c0 ? 2 : l ? 1 : 0;
+      if (f < e) scan(cs[rm - 1 + f], cs[rm - 1 + e]);
+    }
+    // left, own, right
+    scan(need(r0 - 1, dxl, 0.0f) ? R.a[1] : R.a[2], need(r0 + 1, dxr, 0.0f) ? R.b[3] : R.b[2]);
+    // up, up-right, then up-left
+    {
+      const bool u = need(rp, 0.0f, dyt), ur = need(rp + 1, dxr, dyt);
+      const int f = u ? 0 : ur ? 1 : 2, e = ur ? 2 : u ? 1 : 0;
+      if (f < e) scan(cs[rp + f], cs[rp + e]);
+      if (need(rp - 1, dxl, dyt)) scan(R.a[5], R.a[4]);
     }
   }
   const size_t N = (size_t)d.ntot;
@@ -734,9 +850,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     bool found = false;
     if (nc <= kNbrMax) {
       for (int j = nc - 1; j >= 0 && !found; --j) {
-        const unsigned e = (unsigned)lst[j * kTPB + threadIdx.x];
-        if ((e >> 30) == 0) break;
-        const int t = (int)(e & 0x3fffffffu);
+        const int t = (int)lst[j * kTPB + threadIdx.x] + lbase;
+        if (pair_kind(R, t, self) == 0) break;
         const double dx = xi - d.xs[t], dy = yi - d.ys[t];
         const double mh = (hi + d.hs[t]) / 2.;
         double r;
@@ -765,33 +880,35 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   }
   if (nc <= kNbrMax) {
     for (int j0 = 0; j0 < nc; j0 += kBatch) {
-      int sj[kBatch], kj[kBatch];
+      int sj[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const bool v = j0 + u < nc;
-        const unsigned e = v ? (unsigned)lst[(j0 + u) * kTPB + threadIdx.x] : 0u;
-        sj[u] = v ? (int)(e & 0x3fffffffu) : self;
-        kj[u] = v ? (int)(e >> 30) : -1;
-      }
-      double xq[kBatch], yq[kBatch], hq[kBatch];
+      for (int u = 0; u < kBatch; ++u)
+        sj[u] = j0 + u < nc ? (int)lst[(j0 + u) * kTPB + threadIdx.x] + lbase : self;
+      // positions and pair data of the batch in one round of loads
+      double xq[kBatch], yq[kBatch], hq[kBatch], mq[kBatch], rq[kBatch], uq[kBatch], wq[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         xq[u] = d.xs[sj[u]];
         yq[u] = d.ys[sj[u]];
         hq[u] = d.hs[sj[u]];
+        mq[u] = d.ms[sj[u]];
+        rq[u] = d.rs[sj[u]];
+        uq[u] = d.vxs[sj[u]];
+        wq[u] = d.vys[sj[u]];
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        if (kj[u] < 0) continue;
+        if (j0 + u >= nc) continue;
         const double dx = xi - xq[u], dy = yi - yq[u];
         const double mh = (hi + hq[u]) / 2.;
         double r;
         if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-        nI += kj[u] == 1;
-        nX += kj[u] == 2;
+        const int kind = pair_kind(R, sj[u], self);
+        nI += kind == 1;
+        nX += kind == 2;
         d.nbr[nn * N + self] = sj[u];
         ++nn;
-        pair(sj[u], dx, dy, r, mh);
+        pair(dx, dy, r, mh, mq[u], rq[u], uq[u], wq[u]);
       }
     }
   } else {
@@ -805,11 +922,11 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         const double mh = (hi + d.hs[s]) / 2.;
         double r;
         if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-        const int kind = range_kind(g, s, self);
+        const int kind = pair_kind(R, s, self);
         nI += kind == 1;
         nX += kind == 2;
         ++ntrue;
-        pair(s, dx, dy, r, mh);
+        pair(dx, dy, r, mh, d.ms[s], d.rs[s], d.vxs[s], d.vys[s]);
       }
     }
     nn = ntrue;
@@ -827,7 +944,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     d.dsum[self] = num / den;
   }
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
-  d.own_cnt[p] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
+  d.own_cnt[self] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
 
   // strain rate (single_step.f90:80-84)
   const double e11 = g1, e22 = g4, e12 = 0.5 * (g2 + g3);
@@ -888,7 +1005,7 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
   int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
   int I = 0, X = 0;
   for (int s = s0; s < s1; ++s) {
-    int oc = d.own_cnt[d.cl[s]];
+    int oc = d.own_cnt[s];
     d.moff[s] = I | (X << 16);
     I += oc & 1023;
     X += (oc >> 10) & 1023;
@@ -906,6 +1023,7 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
 constexpr int kMatBlocks = 32;
 
 __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
+  if (blockIdx.x >= (unsigned)(d.ninst * kMatBlocks)) return;
   const int k = blockIdx.x / kMatBlocks;
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
@@ -923,7 +1041,7 @@ __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
     const int c = d.cellid[p];
     const int base = d.pair_start[m.c_off + c] - pbase;
     if (base >= lim) break;
-    const int oc = d.own_cnt[p];
+    const int oc = d.own_cnt[s];
     const int nI = oc & 1023, nX = (oc >> 10) & 1023;
     if (nI + nX == 0) continue;
     int rI = base + (d.moff[s] & 0xffff);
@@ -950,6 +1068,112 @@ __global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
         if (g == 2 && t <= s) continue;
         const double dx = xi - d.xs[t], dy = yi - d.ys[t];
         const double mh = (hi + d.hs[t]) / 2.;
+        double r;
+        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        int &rank = (g == 2) ? rI : rX;
+        if (rank < lim) d.hq[m.q_off + rank] = mh;
+        ++rank;
+      }
+    }
+  }
+}
+
+// The same store in one kernel per instance for the step loop, where nip
+// itself is not reported: the ranks are the pair prefix over the cell lists
+// (per cell: all intra pairs, then all cross pairs, each in member order),
+// and only the pairs ranked below min(bnip, store size) are read by the
+// boundary kernel -- a short head of the list.  One block streams the
+// prefix over list positions until it passes that limit, then writes those
+// pairs' hsml_ij.
+constexpr int kRankThreads = 1024;
+
+__global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
+  __shared__ int sh_stop;
+  if (blockIdx.x >= (unsigned)d.ninst) return;
+  const int k = blockIdx.x;
+  const InstDyn &y = d.dyn[k];
+  const InstMeta &m = d.meta[k];
+  if (m.nb == 0 || inst_off(y)) return;
+  const int lim = min(y.bnip, m.q_cap);
+  if (lim <= 0) return;
+  const int *cs = d.cell_start + m.c_off;
+  const int p0 = cs[0], np = m.n;
+  int2 *G = d.gpre + p0;       // per position: exclusive prefixes of intra, cross counts
+  if (threadIdx.x == 0) sh_stop = np;
+  __syncthreads();
+  // 1) prefixes up to the end of the last cell whose first pair ranks
+  //    below lim
+  constexpr int kPer = 4;      // consecutive positions per thread
+  int baseI = 0, baseX = 0;
+  for (int c0 = 0; c0 < np && c0 <= sh_stop; c0 += kRankThreads * kPer) {
+    const int t0 = c0 + (int)threadIdx.x * kPer;
+    int vI[kPer], vX[kPer], sI = 0, sX = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int oc = t0 + u < np ? d.own_cnt[p0 + t0 + u] : 0;
+      vI[u] = oc & 1023;
+      vX[u] = (oc >> 10) & 1023;
+      sI += vI[u];
+      sX += vX[u];
+    }
+    int totI = 0, totX = 0;
+    int gI = baseI + block_exclusive_scan<kRankThreads>(sI, &totI);
+    int gX = baseX + block_exclusive_scan<kRankThreads>(sX, &totX);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int t = t0 + u;
+      if (t < np) {
+        G[t] = make_int2(gI, gX);
+        // the last position whose prefix is below lim: its cell is the
+        // last one needed
+        if (gI + gX < lim && gI + gX + vI[u] + vX[u] >= lim)
+          sh_stop = cs[d.cells[p0 + t] + 1] - p0;
+      }
+      gI += vI[u];
+      gX += vX[u];
+    }
+    baseI += totI;
+    baseX += totX;
+    __syncthreads();
+  }
+  const int stop = sh_stop;
+  const int2 Gend = make_int2(baseI, baseX);   // prefix at np when the loop reached it
+  // 2) the owned pairs of the needed cells that rank below lim
+  const size_t N = (size_t)d.ntot;
+  const int W = y.W;
+  for (int t = threadIdx.x; t < stop; t += kRankThreads) {
+    const int s = p0 + t;
+    const int c = d.cells[s];
+    const int f = cs[c] - p0, e = cs[c + 1] - p0;
+    const int2 Gf = G[f], Gt = G[t];
+    const int2 Ge = e < np ? G[e] : Gend;
+    const int base = Gf.x + Gf.y;
+    if (base >= lim) continue;
+    const int oc = d.own_cnt[s];
+    const int nI = oc & 1023, nX = (oc >> 10) & 1023;
+    int rI = base + (Gt.x - Gf.x);
+    int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
+    if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) continue;
+    const double hi = d.hs[s];
+    const int nn = d.nnb[s];
+    if (nn <= kNbrMax) {
+      // the owned pairs are the tail of the list: nI intra, then nX cross
+      for (int j = nn - nI - nX; j < nn; ++j) {
+        const int q = d.nbr[j * N + s];
+        int &rank = (j < nn - nX) ? rI : rX;
+        if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[q]) / 2.;
+        ++rank;
+      }
+      continue;
+    }
+    const int cy = c / W, cx = c - cy * W;
+    const double xi = d.xs[s], yi = d.ys[s];
+    const Ranges R = neighbour_ranges(cs, W, cx, cy);
+    for (int g = 2; g < 6; ++g) {
+      for (int q = R.a[g]; q < R.b[g]; ++q) {
+        if (g == 2 && q <= s) continue;
+        const double dx = xi - d.xs[q], dy = yi - d.ys[q];
+        const double mh = (hi + d.hs[q]) / 2.;
         double r;
         if (!in_support(dx * dx + dy * dy, mh, r)) continue;
         int &rank = (g == 2) ? rI : rX;
@@ -990,6 +1214,7 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 // place), 2 = single evaluation (tendencies to d.tend)
 template <bool MIXED, bool SUMD>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
   int self = 0, k = 0;
   const bool ok = list_slot(d, self, k);
   const InstDyn &y = d.dyn[k];
@@ -1107,52 +1332,25 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
     // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
     // ascending boundary index (boundary_kernel.f90:477-502)
     if (m.nb > 0 && d.bflag[m.c_off + c] == y.stamp) {
-      // Boundary neighbours are taken in ascending boundary index by a
-      // selection over the 3x3 cell block (a particle sees a handful).
+      // the cell's boundary particles in ascending index, each with the
+      // slot this cell takes in its 3x3 block
       int nbn = 0;
-      int last = -1;
-      const unsigned long long *BS = d.bsorted + m.b_off;
-      int lo_t[3], hi_t[3];
-      for (int dyc = 0; dyc < 3; ++dyc) {
-        const unsigned lo = (unsigned)((cy + dyc - 1) * W + cx - 1);
-        int a = 0, b = m.nb;
-        while (a < b) {
-          int mid = (a + b) >> 1;
-          if ((unsigned)(BS[mid] >> 32) < lo) a = mid + 1; else b = mid;
-        }
-        int e = a;
-        while (e < m.nb && (unsigned)(BS[e] >> 32) <= lo + 2) ++e;
-        lo_t[dyc] = a;
-        hi_t[dyc] = e;
-      }
-      while (true) {
-        int best = 0x7fffffff, bcell = 0;
-        for (int dyc = 0; dyc < 3; ++dyc) {
-          for (int t = lo_t[dyc]; t < hi_t[dyc]; ++t) {
-            const unsigned long long key = BS[t];
-            const int bl = (int)(key & 0xffffffffu);
-            if (bl <= last || bl >= best) continue;
-            const int bg = m.b_off + bl;
-            const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
-            if (!(sqrt(bdx * bdx + bdy * bdy) <= kBLength)) continue;
-            best = bl;
-            bcell = (int)(key >> 32);
-          }
-        }
-        if (best == 0x7fffffff) break;
-        last = best;
+      const int2 run = d.brun[m.c_off + c];
+      const int *bl = d.blist + 9 * (size_t)m.b_off;
+      for (int e = run.x; e < run.y; ++e) {
+        const int ent = bl[e];
+        const int bg = m.b_off + (ent & 0xffffff);
+        const int slot9 = ent >> 24;
+        const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
+        const double br = sqrt(bdx * bdx + bdy * bdy);
+        if (!(br <= kBLength)) continue;
         ++nbn;
-        const int bg = m.b_off + best;
-        const int cyb = bcell / W, cxb = bcell - cyb * W;
-        const int slot9 = (cy - cyb + 1) * 3 + (cx - cxb + 1);
         int before = 0;
         for (int s = R.a[2]; s < self; ++s) {
           const double ex = d.xb[bg] - d.xs[s], ey = d.yb[bg] - d.ys[s];
           if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
         }
         const int rank = d.bbase[bg] + d.bpre[bg * 9 + slot9] + before;
-        const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
-        const double br = sqrt(bdx * bdx + bdy * bdy);
         const double q = br / kBLength * 1.0;
         const double hk = rank < m.q_cap ? d.hq[m.q_off + rank] : 0.0;
         const double ih = 1.0 / hk;
@@ -1168,9 +1366,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
         ay = ay + (-F1) / rhoi;
       }
       // interaction_stats capacity check (nnps.f90:660-668)
-      if (y.itimestep % kPrintStep == 0 && (d.own_cnt[p] >> 20) + nbn > kMaxIndInt)
+      if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) + nbn > kMaxIndInt)
         atomicOr(&d.dyn[k].err, kErrCapacity);
-    } else if (y.itimestep % kPrintStep == 0 && (d.own_cnt[p] >> 20) > kMaxIndInt) {
+    } else if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) > kMaxIndInt) {
       atomicOr(&d.dyn[k].err, kErrCapacity);
     }
 
@@ -1261,10 +1459,10 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (inst_off(y)) return;
-  const int oc = d.own_cnt[p];
+  const int s = d.clpos[p];
+  const int oc = d.own_cnt[s];
   if ((oc & 0xfffff) == 0) return;
   const int c = d.cellid[p];
-  const int s = d.clpos[p];
   const int pbase = d.pair_start[m.c_off];
   const int base = d.pair_start[m.c_off + c] - pbase;
   int rI = base + (d.moff[s] & 0xffff);
@@ -1330,66 +1528,71 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
 // --------------------------------------------------------------- launches
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+// grid rounded up to a multiple of the 8 XCDs (see particle_of)
+static inline int g8(long blocks) { return (int)((std::max<long>(blocks, 1) + 7) / 8 * 8); }
 
 void launch_partial(const DevState &d, int buf, hipStream_t st) {
-  k_partial<<<d.nblk, kTPB, 0, st>>>(d, buf);
+  k_partial<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
-  k_finalize<<<d.ninst, kTPB, 0, st>>>(d, end_of_step);
+  k_finalize<<<g8(d.ninst), kTPB, 0, st>>>(d, end_of_step);
 }
 void launch_count(const DevState &d, int buf, hipStream_t st) {
-  k_count<<<d.nblk, kTPB, 0, st>>>(d, buf);
+  k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st) {
   int nb = cdiv(L, kScanItems);
   if (nb < 1) nb = 1;
-  k_scan1<<<nb, kTPB, 0, st>>>(in, L, tmp);
-  k_scan2<<<nb, kTPB, 0, st>>>(in, L, tmp, out, zero_in);
+  k_scan1<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, nb);
+  k_scan2<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, out, zero_in, nb);
 }
 void launch_scatter(const DevState &d, hipStream_t st) {
-  k_scatter<<<d.nblk, kTPB, 0, st>>>(d);
+  k_scatter<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
-  k_cellsort<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
-  k_listcopy<<<d.nblk, kTPB, 0, st>>>(d, buf);
+  k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
+  k_listcopy<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 void launch_bprep(const DevState &d, hipStream_t st) {
-  k_bprep<<<d.ninst, kBprepThreads, 0, st>>>(d);
+  k_bprep<<<g8(d.ninst), kBprepThreads, 0, st>>>(d);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st) {
   if (mixed && sumd)
-    k_passA<true, true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<true, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else if (mixed)
-    k_passA<true, false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<true, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else if (sumd)
-    k_passA<false, true><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<false, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else
-    k_passA<false, false><<<d.nblk, kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<false, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
 }
 void launch_pairtot(const DevState &d, hipStream_t st) {
-  k_pairtot<<<cdiv(d.ctot, kTPB), kTPB, 0, st>>>(d);
+  k_pairtot<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d);
+}
+void launch_ranks(const DevState &d, hipStream_t st) {
+  k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
 }
 void launch_materialize(const DevState &d, hipStream_t st) {
-  k_materialize<<<d.ninst * kMatBlocks, kTPB, 0, st>>>(d);
+  k_materialize<<<g8(d.ninst * kMatBlocks), kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
   if (mixed && sumd)
-    k_passB<true, true><<<d.nblk, kTPB, 0, st>>>(d, phase);
+    k_passB<true, true><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
   else if (mixed)
-    k_passB<true, false><<<d.nblk, kTPB, 0, st>>>(d, phase);
+    k_passB<true, false><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
   else if (sumd)
-    k_passB<false, true><<<d.nblk, kTPB, 0, st>>>(d, phase);
+    k_passB<false, true><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
   else
-    k_passB<false, false><<<d.nblk, kTPB, 0, st>>>(d, phase);
+    k_passB<false, false><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st) {
-  k_export_pairs<<<d.nblk, kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w);
+  k_export_pairs<<<g8(d.nblk), kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w);
 }
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st) {
-  if (nb > 0) k_export_bpairs<<<cdiv(nb, kTPB), kTPB, 0, st>>>(d, buf, bcap, bpi, bpj, brij);
+  if (nb > 0) k_export_bpairs<<<g8(cdiv(nb, kTPB)), kTPB, 0, st>>>(d, buf, bcap, bpi, bpj, brij);
 }
 
 }  // namespace simp
