@@ -88,7 +88,10 @@ struct DevState {
   // gathers run one thread per list position and read neighbours here
   double *xs, *ys, *hs, *vxs, *vys, *ms, *rs;
   int *cells;            // cell (local id) of each list position
-  float *cellh;          // per cell: largest member hsml (FP32, rounded up)
+  // per cell: (stamp of the evaluation << 32) | largest member hsml as
+  // FP32 bits, rounded up
+  unsigned long long *cellh;
+  int *rebuild;          // 1: the cell lists are rebuilt in this evaluation
   float4 *pf;            // FP32 (x, y, hsml) in list order, x and y relative to
                          // the grid origin: the candidate prefilter
   // neighbour list of the evaluation, built by pass A in the reference's

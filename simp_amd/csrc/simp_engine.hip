@@ -88,7 +88,7 @@ static int store_capacity(const InstanceInput &in) {
 constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
                                      kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8 + 8;
 constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4;
-constexpr size_t kBytesPerCell = 10 * 4;
+constexpr size_t kBytesPerCell = 11 * 4;
 
 size_t instance_footprint(const InstanceInput &in) {
   long c = cell_capacity(in);
@@ -162,6 +162,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.pair_start, C);
   rc |= dalloc(list, &d.cell_intra, C);
   rc |= dalloc(list, &d.cellh, C);
+  rc |= dalloc(list, &d.rebuild, 1);
   double *sg = nullptr;
   rc |= dalloc(list, &sg, N); d.sig11 = sg;
   rc |= dalloc(list, &sg, N); d.sig12 = sg;
@@ -338,6 +339,12 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.bcellc, 0xff, (ctot + 1) * sizeof(int), st_));
+  RCHK(hipMemsetAsync(nd.cellh, 0, (ctot + 1) * sizeof(unsigned long long), st_));
+  {
+    // the first evaluation of a new layout builds the cell lists
+    static const int one = 1;
+    RCHK(hipMemcpyAsync(nd.rebuild, &one, sizeof(int), hipMemcpyHostToDevice, st_));
+  }
 
   // kept instances: device to device
   auto d2d = [&](void *dst, const void *src, size_t bytes) {
@@ -490,7 +497,7 @@ void Engine::L(const char *name, F &&f) {
 void Engine::enqueue_prefix(int buf, bool fork_boundary) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
-  L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, st_); });
+  L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, d.rebuild, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
   if (!any_boundary_) return;
@@ -514,7 +521,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   if (phase == 2) {
     // all pair ranks (nip is reported), then the boundary kernel's hsml_ij
     L("pairtot", [&] { launch_pairtot(d, st_); });
-    L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_); });
+    L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_); });
     if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
     if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
   } else {
@@ -719,7 +726,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   enqueue_prefix(0, false);
   launch_passA(d, 0, precision_, any_sumd_, 0, st_);
   launch_pairtot(d, st_);
-  launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, st_);
+  launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);
   int rc = sync_and_check();
   if (rc) return rc;
   int n_ip = 0, n_bp = 0, err = 0;

@@ -9,7 +9,9 @@ namespace simp {
 void launch_partial(const DevState &d, int buf, hipStream_t st);
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st);
 void launch_count(const DevState &d, int buf, hipStream_t st);
-void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st);
+// gate: device flag; the scan does nothing while it reads 0 (nullptr: always runs)
+void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
+                 hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_cellsort(const DevState &d, int buf, hipStream_t st);
 void launch_bprep(const DevState &d, hipStream_t st);

@@ -265,23 +265,40 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
 
 // ------------------------------------------------------------------ count
 
+// Cell key of every particle (bucket_search, nnps.f90:249-250).  The cell
+// lists are rebuilt only when some key differs from the one the current
+// lists were built with (or on the first evaluation after an upload); the
+// rebuild kernels below return at once otherwise.  Between the predictor
+// and the corrector evaluation of a step the grid and the keys normally
+// stay the same, so the corrector reuses the lists.
 __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   int p = 0, k = 0;
-  if (!particle_of(d, p, k)) return;
-  const InstDyn &y = d.dyn[k];
-  if (inst_off(y)) return;
-  const double bin = y.bin;
-  const int cx = nint(d.x[buf][p] / bin) - y.xlo + 1;
-  const int cy = nint(d.y[buf][p] / bin) - y.ylo + 1;
-  const int c = cy * y.W + cx;
-  d.cellid[p] = c;
-  d.slot[p] = atomicAdd(&d.count[d.meta[k].c_off + c], 1);
+  bool changed = false;
+  if (particle_of(d, p, k) && !inst_off(d.dyn[k])) {
+    const InstDyn &y = d.dyn[k];
+    const double bin = y.bin;
+    const int cx = nint(d.x[buf][p] / bin) - y.xlo + 1;
+    const int cy = nint(d.y[buf][p] / bin) - y.ylo + 1;
+    const int c = cy * y.W + cx;
+    changed = d.cellid[p] != c;
+    d.cellid[p] = c;
+  }
+  if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(d.rebuild, 1);
+}
+
+// Histogram of the cell keys (slot of each particle in its cell).
+__global__ void __launch_bounds__(kTPB) k_hist(DevState d) {
+  if (*d.rebuild == 0) return;
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k) || inst_off(d.dyn[k])) return;
+  d.slot[p] = atomicAdd(&d.count[d.meta[k].c_off + d.cellid[p]], 1);
 }
 
 // ------------------------------------------------------------------- scan
 
-__global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum, int nblocks) {
-  if (blockIdx.x >= (unsigned)nblocks) return;
+__global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum, int nblocks,
+                                                const int *gate) {
+  if (blockIdx.x >= (unsigned)nblocks || (gate && *gate == 0)) return;
   constexpr int IPT = kScanItems / kTPB;
   int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
   int s = 0;
@@ -296,8 +313,8 @@ __global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum,
 
 // out[i] = sum(in[0..i-1]); out[L] = total.  Optionally clears in[].
 __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum, int *out,
-                                                int zero_in, int nblocks) {
-  if (blockIdx.x >= (unsigned)nblocks) return;
+                                                int zero_in, int nblocks, const int *gate) {
+  if (blockIdx.x >= (unsigned)nblocks || (gate && *gate == 0)) return;
   constexpr int IPT = kScanItems / kTPB;
   int acc = 0;
   for (int j = threadIdx.x; j < (int)blockIdx.x; j += kTPB) acc += bsum[j];
@@ -328,6 +345,7 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 // ------------------------------------------------------- scatter/cellsort
 
 __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
+  if (*d.rebuild == 0) return;
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
   if (inst_off(d.dyn[k])) return;
@@ -337,33 +355,25 @@ __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
 // members of each cell in ascending particle id: the order in which the
 // reference fills its bins (nnps.f90:246-254).  Cells of up to kRankMax
 // members (the common case) are ranked in registers; larger ones are
-// insertion-sorted in place.  Also the cell's largest smoothing length
-// (rounded up to FP32), which bounds the support of any pair with a member
-// for the gathers' cell culling.
+// insertion-sorted in place.
 constexpr int kRankMax = 16;
 
 __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
+  if (*d.rebuild == 0) return;
   int c = blockIdx.x * kTPB + threadIdx.x;
   if (c >= d.ctot) return;
   const int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
   const int n = s1 - s0;
-  if (n == 0) {
-    d.cellh[c] = 0.0f;
-    return;
-  }
+  if (n <= 1) return;
   for (int s = s0; s < s1; ++s)
     if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: skipped instance
-  const double *hs = d.hsml[buf];
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
-    double hm = 0.0;
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
       v[i] = i < n ? d.cl[s0 + i] : 0;
       key[i] = i < n ? d.orig[v[i]] : 0x7fffffff;
-      if (i < n) hm = fmax(hm, hs[v[i]]);
     }
-    d.cellh[c] = (float)hm * 1.000001f;
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
       if (i < n) {
@@ -375,9 +385,6 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
     }
     return;
   }
-  double hm = 0.0;
-  for (int s = s0; s < s1; ++s) hm = fmax(hm, hs[d.cl[s]]);
-  d.cellh[c] = (float)hm * 1.000001f;
   for (int a = s0 + 1; a < s1; ++a) {
     int v = d.cl[a];
     int key = d.orig[v];
@@ -395,29 +402,53 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
 // The right-hand-side state copied into cell-list order for the gathers,
 // one thread per list position.  Also the FP32 positions relative to the
 // instance's grid origin used by the candidate prefilter.
+// Also the largest smoothing length of each cell (rounded up to FP32), the
+// bound the gathers' cell culling uses: a max over each run of equal cells
+// in the wavefront (list positions are in cell order), then one 64-bit
+// atomic max per run of (evaluation stamp, value) -- a stale stamp reads as
+// an empty cell, so nothing needs clearing.
 __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
   if (blockIdx.x >= (unsigned)d.nblk) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // its readers have run
   const BlockEntry e = d.blk[blockIdx.x];
   const int k = e.inst;
   const InstMeta &m = d.meta[k];
-  const int local = e.first + (int)threadIdx.x;
-  if (local >= m.n) return;
   const InstDyn &y = d.dyn[k];
-  if (inst_off(y)) return;
-  const int s = d.cell_start[m.c_off] + local;
-  const int q = d.cl[s];
-  d.clpos[q] = s;
-  d.cells[s] = d.cellid[q];
-  const double xq = d.x[buf][q], yq = d.y[buf][q];
-  d.xs[s] = xq;
-  d.ys[s] = yq;
-  d.hs[s] = d.hsml[buf][q];
-  d.vxs[s] = d.vx[buf][q];
-  d.vys[s] = d.vy[buf][q];
-  d.ms[s] = d.mass[q];
-  d.rs[s] = d.rho[buf][q];
-  const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
-  d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)d.hs[s], 0.0f);
+  if (inst_off(y)) return;                  // whole block
+  const int local = e.first + (int)threadIdx.x;
+  const bool valid = local < m.n;
+  const int lane = (int)threadIdx.x & 63;
+  int key = -1 - lane;
+  unsigned long long hv = 0;
+  if (valid) {
+    const int s = d.cell_start[m.c_off] + local;
+    const int q = d.cl[s];
+    const int c = d.cellid[q];
+    d.clpos[q] = s;
+    d.cells[s] = c;
+    const double xq = d.x[buf][q], yq = d.y[buf][q], hq = d.hsml[buf][q];
+    d.xs[s] = xq;
+    d.ys[s] = yq;
+    d.hs[s] = hq;
+    d.vxs[s] = d.vx[buf][q];
+    d.vys[s] = d.vy[buf][q];
+    d.ms[s] = d.mass[q];
+    d.rs[s] = d.rho[buf][q];
+    const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
+    d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
+    key = c;
+    hv = ((unsigned long long)(unsigned)y.stamp << 32) |
+         __float_as_uint((float)hq * 1.000001f);
+  }
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned lo = __shfl_down((unsigned)hv, o, 64);
+    const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
+    const int ok = __shfl_down(key, o, 64);
+    const unsigned long long ov = ((unsigned long long)hi << 32) | lo;
+    if (lane + o < 64 && ok == key && ov > hv) hv = ov;
+  }
+  const int prev = __shfl_up(key, 1, 64);
+  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[m.c_off + key], hv);
 }
 
 // -------------------------------------------------------------- boundary
@@ -814,11 +845,14 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     const float dxr = fmaxf(((float)cx + 0.5f) * binf - pi4.x - slack, 0.0f);
     const float dyb = fmaxf(pi4.y - ((float)cy - 0.5f) * binf - slack, 0.0f);
     const float dyt = fmaxf(((float)cy + 0.5f) * binf - pi4.y - slack, 0.0f);
-    const float *ch = d.cellh + m.c_off;
+    const unsigned long long *ch = d.cellh + m.c_off;
+    const unsigned stampu = (unsigned)y.stamp;
     const int *cs = d.cell_start + m.c_off;
     const int rm = (cy - 1) * W + cx, r0 = cy * W + cx, rp = (cy + 1) * W + cx;
     auto need = [&](int c, float ex, float ey) {
-      const float bnd = __builtin_fmaf(pi4.z + ch[c], 0.5f, slack);
+      const unsigned long long hc = ch[c];
+      const float hmax = (unsigned)(hc >> 32) == stampu ? __uint_as_float((unsigned)hc) : 0.0f;
+      const float bnd = __builtin_fmaf(pi4.z + hmax, 0.5f, slack);
       return __builtin_fmaf(ex, ex, ey * ey) <= bnd * bnd * 1.00001f;
     };
     // row below: cells rm-1, rm, rm+1 -> the span from the first to the
@@ -1539,12 +1573,14 @@ void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
 }
 void launch_count(const DevState &d, int buf, hipStream_t st) {
   k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  k_hist<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
-void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, hipStream_t st) {
+void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
+                 hipStream_t st) {
   int nb = cdiv(L, kScanItems);
   if (nb < 1) nb = 1;
-  k_scan1<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, nb);
-  k_scan2<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, out, zero_in, nb);
+  k_scan1<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, nb, gate);
+  k_scan2<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, out, zero_in, nb, gate);
 }
 void launch_scatter(const DevState &d, hipStream_t st) {
   k_scatter<<<g8(d.nblk), kTPB, 0, st>>>(d);
