@@ -65,6 +65,7 @@ struct InstDyn {
   float pf_slack;       // bound of the prefilter's FP32 distance error (m)
   int active;           // 0: frozen (finished or not started), kernels skip it
   int steps_left;       // steps until the instance freezes; < 0: no limit
+  int lstamp;           // stamp of the evaluation that built the boundary cell lists
   double hmin;          // MINVAL(hsml) of the current evaluation
 };
 
@@ -116,6 +117,8 @@ struct DevState {
   // boundary
   double *xb, *yb, *massb;
   int *bpre, *bbase, *bcount;
+  int *bcnt;             // per (boundary particle, cell of its block): pair counts
+  int *bkey;             // cell key of each boundary particle when the lists were built
   // per cell covered by a boundary particle's 3x3 block: the run
   // [brun.x, brun.y) of blist holding those boundary particles (local index
   // | slot of the cell in its block << 24) in ascending index; bcellc holds

@@ -91,7 +91,7 @@ class Engine {
   int repack(const std::vector<int> &keep, const std::vector<InstanceInput> &add,
              int carry_store_from, std::vector<int> *new_ids);
   void enqueue_eval(int buf, int phase, bool want_diag);
-  void enqueue_prefix(int buf, bool fork_boundary);
+  void enqueue_prefix(int buf);
   void enqueue_step();
   int graph_build();
   void graph_drop();
@@ -100,8 +100,6 @@ class Engine {
 
   int precision_;
   hipStream_t st_ = nullptr;
-  hipStream_t side_ = nullptr;        // boundary prep, concurrent with pass A
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   DevState d_{};
   std::vector<void *> allocs_;        // device buffers of d_
   int *scan_tmp_ = nullptr;

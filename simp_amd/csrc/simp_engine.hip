@@ -87,7 +87,7 @@ static int store_capacity(const InstanceInput &in) {
 // bytes per particle / boundary particle / cell of the device layout
 constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
                                      kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8 + 8;
-constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4;
+constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4 + 9 * 4 + 4;
 constexpr size_t kBytesPerCell = 11 * 4;
 
 size_t instance_footprint(const InstanceInput &in) {
@@ -180,6 +180,8 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.bpre, 9 * NB);
   rc |= dalloc(list, &d.bbase, NB);
   rc |= dalloc(list, &d.bcount, NB);
+  rc |= dalloc(list, &d.bcnt, 9 * NB);
+  rc |= dalloc(list, &d.bkey, NB);
   rc |= dalloc(list, &d.brun, C);
   rc |= dalloc(list, &d.bcellc, C);
   rc |= dalloc(list, &d.blist, 9 * NB);
@@ -193,9 +195,6 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
 
 Engine::Engine(int precision) : precision_(precision) {
   hipStreamCreate(&st_);
-  hipStreamCreate(&side_);
-  hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming);
-  hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming);
 }
 
 Engine::~Engine() {
@@ -203,9 +202,6 @@ Engine::~Engine() {
   if (st_) hipStreamSynchronize(st_);
   free_list(allocs_);
   if (d_.hq) hipFree(d_.hq);
-  if (ev_fork_) hipEventDestroy(ev_fork_);
-  if (ev_join_) hipEventDestroy(ev_join_);
-  if (side_) hipStreamDestroy(side_);
   if (st_) hipStreamDestroy(st_);
 }
 
@@ -339,6 +335,9 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.bcellc, 0xff, (ctot + 1) * sizeof(int), st_));
+  // boundary lists are (re)built at the first evaluation of a layout
+  RCHK(hipMemsetAsync(nd.bkey, 0xff, std::max<long>(nbtot, 1) * sizeof(int), st_));
+  RCHK(hipMemsetAsync(nd.bcnt, 0, 9 * std::max<long>(nbtot, 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.cellh, 0, (ctot + 1) * sizeof(unsigned long long), st_));
   {
     // the first evaluation of a new layout builds the cell lists
@@ -494,41 +493,26 @@ void Engine::L(const char *name, F &&f) {
   prof->recs.push_back(r);
 }
 
-void Engine::enqueue_prefix(int buf, bool fork_boundary) {
+void Engine::enqueue_prefix(int buf) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
   L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, d.rebuild, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
-  if (!any_boundary_) return;
-  if (fork_boundary && !prof) {
-    // the boundary prep only feeds materialize and pass B: it runs on a
-    // second stream (a parallel branch of the captured graph) beside pass A
-    hipEventRecord(ev_fork_, st_);
-    hipStreamWaitEvent(side_, ev_fork_, 0);
-    launch_bprep(d, side_);
-    hipEventRecord(ev_join_, side_);
-  } else {
-    L("bprep", [&] { launch_bprep(d, st_); });
-  }
+  if (any_boundary_) L("bprep", [&] { launch_bprep(d, st_); });
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const DevState &d = d_;
-  const bool fork = any_boundary_ && !prof;
-  enqueue_prefix(buf, fork);
+  enqueue_prefix(buf);
   L("passA", [&] { launch_passA(d, buf, precision_, any_sumd_, want_diag ? 1 : 0, st_); });
   if (phase == 2) {
-    // all pair ranks (nip is reported), then the boundary kernel's hsml_ij
+    // all pair ranks: nip is reported
     L("pairtot", [&] { launch_pairtot(d, st_); });
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_); });
-    if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
-    if (any_boundary_) L("materialize", [&] { launch_materialize(d, st_); });
-  } else {
-    // the step loop: only the head of the rank order the boundary kernel reads
-    if (fork) hipStreamWaitEvent(st_, ev_join_, 0);
-    if (any_boundary_) L("ranks", [&] { launch_ranks(d, st_); });
   }
+  // boundary-pair ranks and the head of the pair ranks the boundary kernel reads
+  if (any_boundary_) L("ranks", [&] { launch_ranks(d, st_); });
   L("passB", [&] { launch_passB(d, precision_, any_sumd_, phase, st_); });
 }
 
@@ -723,8 +707,9 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
                          int bcap, int *bnip, int *bpi, int *bpj, double *brij, double *w) {
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
-  enqueue_prefix(0, false);
+  enqueue_prefix(0);
   launch_passA(d, 0, precision_, any_sumd_, 0, st_);
+  if (any_boundary_) launch_ranks(d, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);
   int rc = sync_and_check();

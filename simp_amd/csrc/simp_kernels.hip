@@ -8,21 +8,28 @@
 //                the bounding box and MINVAL(hsml) (nnps.f90:211-224); at
 //                the end of a step also the VP dt, time and step counter
 //                (time_step.f90:155, time_integration.f90:68-71)
-//   count        cell key NINT(x/bin) + cell histogram    (nnps.f90:246-254)
-//   scan x2      exclusive scan of the cell histogram
-//   scatter      cell lists
-//   cellsort     cell members in ascending particle id (the reference's bin
-//                order), inverse positions, x/y/hsml copied in list order
-//   bprep        boundary-pair counts, ranks and the boundary cell index
-//                (nnps.f90:451-493)
+//   count        cell key NINT(x/bin) (nnps.f90:246-254); flags a rebuild
+//                of the cell lists when any key changed
+//   hist, scan x2, scatter, cellsort
+//                the cell lists, members in ascending particle id (the
+//                reference's bin order); all return at once when no key
+//                changed (the corrector evaluation normally reuses the
+//                predictor's lists)
+//   listcopy     the right-hand-side state copied in list order, FP32
+//                prefilter data, per-cell largest hsml
+//   bprep        per-cell lists of the boundary particles whose 3x3 block
+//                covers the cell (nnps.f90:451-493); rebuilt only when a
+//                boundary particle's cell changes
 //   passA        neighbour gather: Wendland C6 gradient, velocity divergence
 //                and gradient, strain rate and the VP rheology epilogue
 //                (kernel.f90:471-520, vector_divergence.f90:27-28,
 //                vector_gradient.f90:27-40, single_step.f90:80-84,
-//                internal_stress.f90:21-53 and its callees)
-//   pairtot,scan pair ranks in the reference's pair order
-//   materialize  hsml_ij of the first bnip pairs (read by b_Wendland_C6,
+//                internal_stress.f90:21-53 and its callees); also the
+//                boundary-pair counts per (boundary particle, cell)
+//   ranks        boundary-pair ranks and bnip; hsml_ij of the first bnip
+//                particle pairs (read by b_Wendland_C6,
 //                boundary_kernel.f90:491)
+//   (pairtot, scan: all pair ranks, for the reported nip of single_step)
 //   passB        neighbour gather: stress divergence (internal_stress.f90:
 //                103-136), boundary force (boundary_kernel.f90:477-502,
 //                boundary_force.f90:78-82), wind and water drag, tendencies
@@ -453,14 +460,14 @@ __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
 
 // -------------------------------------------------------------- boundary
 
-// One block per instance: per boundary particle, the number of boundary
-// pairs found in each of its 3x3 cells in the reference's scan order
-// (y outer, x inner; nnps.f90:458-491), the exclusive prefix over boundary
-// particles (= rank of its first pair), the stamp of every cell a boundary
-// particle's 3x3 block covers, and for each such cell the list of those
-// boundary particles in ascending index -- the order in which the
-// reference's boundary pair list reaches a particle of that cell -- with the
-// slot the cell takes in each one's 3x3 block.
+// One block per instance: the cells of the boundary particles (whose
+// positions never change) under the current grid.  When none moved to
+// another cell since the lists were built, nothing is done.  Otherwise
+// every cell a boundary particle's 3x3 block covers is stamped with a new
+// list stamp and gets the list of the boundary particles whose own cell lies
+// in its 3x3 block, in ascending index -- the order in which the
+// reference's boundary pair list reaches a particle of that cell
+// (nnps.f90:451-493) -- with the slot the cell takes in each one's block.
 constexpr int kBprepThreads = 1024;
 
 __device__ __forceinline__ int lower_bound_key(const unsigned long long *keys, int n,
@@ -483,53 +490,21 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
   if (m.nb == 0 || inst_off(y)) return;
   const double bin = y.bin;
   const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
-  const int *cs = d.cell_start + m.c_off;
-  // counts per (boundary particle, cell of its 3x3 block)
-  for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
-    const int b = idx / 9, s9 = idx - 9 * (idx / 9);
+  int changed = 0;
+  for (int b = threadIdx.x; b < m.nb; b += kBprepThreads) {
     const int bg = m.b_off + b;
-    const double bx = d.xb[bg], by = d.yb[bg];
-    const int cxb = nint(bx / bin) - xlo + 1, cyb = nint(by / bin) - ylo + 1;
-    const int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
-    int cnt = 0;
-    for (int s = cs[c]; s < cs[c + 1]; ++s) {
-      const double dx = bx - d.xs[s], dy = by - d.ys[s];
-      if (sqrt(dx * dx + dy * dy) <= kBLength) ++cnt;
-    }
-    d.bpre[(size_t)bg * 9 + s9] = cnt;
-    d.bflag[m.c_off + c] = stamp;
-    if (s9 == 4) keys[b] = ((unsigned long long)(unsigned)(cyb * W + cxb) << 32) | (unsigned)b;
+    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
+    const int key = cyb * W + cxb;
+    changed |= d.bkey[bg] != key;
+    d.bkey[bg] = key;
+    keys[b] = ((unsigned long long)(unsigned)key << 32) | (unsigned)b;
   }
+  if (!__syncthreads_or(changed)) return;
   int P = 1;
   while (P < m.nb) P <<= 1;
   for (int i = m.nb + threadIdx.x; i < P; i += kBprepThreads) keys[i] = ~0ull;
   if (threadIdx.x == 0) alloc = 0;
   __syncthreads();
-  // prefix over the 9 cells of each boundary particle, then over particles
-  const int per = (m.nb + kBprepThreads - 1) / kBprepThreads;
-  const int b0 = threadIdx.x * per;
-  int tsum = 0;
-  for (int b = b0; b < b0 + per && b < m.nb; ++b) {
-    const size_t bg = (size_t)(m.b_off + b);
-    int run = 0;
-    for (int s9 = 0; s9 < 9; ++s9) {
-      const int c9 = d.bpre[bg * 9 + s9];
-      d.bpre[bg * 9 + s9] = run;
-      run += c9;
-    }
-    d.bcount[bg] = run;
-    tsum += run;
-  }
-  int tot = 0;
-  int ex = block_exclusive_scan<kBprepThreads>(tsum, &tot);
-  for (int b = b0; b < b0 + per && b < m.nb; ++b) {
-    d.bbase[m.b_off + b] = ex;
-    ex += d.bcount[m.b_off + b];
-  }
-  if (threadIdx.x == 0) {
-    d.dyn[k].bnip = tot;
-    if (tot > m.q_cap) atomicOr(&d.dyn[k].err, kErrCapacity);
-  }
   // bitonic sort of (cell, b)
   for (int k2 = 2; k2 <= P; k2 <<= 1) {
     for (int j = k2 >> 1; j > 0; j >>= 1) {
@@ -548,16 +523,14 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
     }
   }
   // Per covered cell, one thread (the first to stamp the cell's claim word
-  // in this evaluation) lists the boundary particles whose own cell lies in
-  // the cell's 3x3 block -- three key ranges, one per row -- in ascending
-  // index.
+  // with this stamp) lists the boundary particles whose own cell lies in
+  // the cell's 3x3 block -- three key ranges, one per row.
   int2 *brun = d.brun + m.c_off;
   int *blist = d.blist + 9 * (size_t)m.b_off;
   for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
     const int b = idx / 9, s9 = idx - 9 * (idx / 9);
-    const int bg = m.b_off + b;
-    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
-    const int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
+    const int c = d.bkey[m.b_off + b] + (s9 / 3 - 1) * W + (s9 % 3 - 1);
+    d.bflag[m.c_off + c] = stamp;
     if (atomicExch(&d.bcellc[m.c_off + c], stamp) == stamp) continue;
     int lo[3], hi[3], total = 0;
 #pragma unroll
@@ -595,6 +568,7 @@ __global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
       blist[start + j] = best | ((dyc * 3 + dxc) << 24);
     }
   }
+  if (threadIdx.x == 0) d.dyn[k].lstamp = stamp;
 }
 
 // --------------------------------------------------------------- neighbours
@@ -977,6 +951,20 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     d.dnum[self] = num;
     d.dsum[self] = num / den;
   }
+  // boundary pairs per (boundary particle, cell of its 3x3 block): the
+  // counts bucket_search's boundary loop finds there (nnps.f90:453-492),
+  // gathered from the particle side; k_ranks turns them into ranks
+  if (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp) {
+    const int2 run = d.brun[m.c_off + c];
+    const int *bl = d.blist + 9 * (size_t)m.b_off;
+    for (int e = run.x; e < run.y; ++e) {
+      const int ent = bl[e];
+      const int bg = m.b_off + (ent & 0xffffff);
+      const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
+      if (sqrt(bdx * bdx + bdy * bdy) <= kBLength)
+        atomicAdd(&d.bcnt[(size_t)bg * 9 + (ent >> 24)], 1);
+    }
+  }
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
   d.own_cnt[self] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
 
@@ -1048,72 +1036,11 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
   d.cell_intra[c] = I;
 }
 
-// hsml_ij of the particle pairs ranked below min(bnip, nip): the entries
-// b_Wendland_C6 reads at the boundary-pair index (boundary_kernel.f90:491).
-// Entries at or above nip keep what an earlier evaluation wrote, as the
-// reference's module array does.  Ranks grow along the cell lists, so the
-// threads of an instance walk the list from its start and stop at the first
-// member whose cell ranks at or beyond the limit.
-constexpr int kMatBlocks = 32;
-
-__global__ void __launch_bounds__(kTPB) k_materialize(DevState d) {
-  if (blockIdx.x >= (unsigned)(d.ninst * kMatBlocks)) return;
-  const int k = blockIdx.x / kMatBlocks;
-  const InstDyn &y = d.dyn[k];
-  const InstMeta &m = d.meta[k];
-  if (m.nb == 0 || inst_off(y)) return;
-  const int pbase = d.pair_start[m.c_off];
-  const int nipk = d.pair_start[m.c_off + m.c_cap] - pbase;
-  const int lim = min(min(y.bnip, m.q_cap), nipk);
-  const int W = y.W;
-  const int *cs = d.cell_start + m.c_off;
-  const int stride = kMatBlocks * kTPB;
-  const int s_end = cs[m.c_cap];
-  for (int s = cs[0] + (blockIdx.x % kMatBlocks) * kTPB + threadIdx.x; s < s_end;
-       s += stride) {
-    const int p = d.cl[s];
-    const int c = d.cellid[p];
-    const int base = d.pair_start[m.c_off + c] - pbase;
-    if (base >= lim) break;
-    const int oc = d.own_cnt[s];
-    const int nI = oc & 1023, nX = (oc >> 10) & 1023;
-    if (nI + nX == 0) continue;
-    int rI = base + (d.moff[s] & 0xffff);
-    int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
-    if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) continue;
-    const double hi = d.hs[s];
-    const int nn = d.nnb[s];
-    if (nn <= kNbrMax) {
-      // the owned pairs are the tail of the list: nI intra, then nX cross
-      const size_t N = (size_t)d.ntot;
-      for (int j = nn - nI - nX; j < nn; ++j) {
-        const int t = d.nbr[j * N + s];
-        int &rank = (j < nn - nX) ? rI : rX;
-        if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[t]) / 2.;
-        ++rank;
-      }
-      continue;
-    }
-    const int cy = c / W, cx = c - cy * W;
-    const double xi = d.xs[s], yi = d.ys[s];
-    const Ranges R = neighbour_ranges(cs, W, cx, cy);
-    for (int g = 2; g < 6; ++g) {
-      for (int t = R.a[g]; t < R.b[g]; ++t) {
-        if (g == 2 && t <= s) continue;
-        const double dx = xi - d.xs[t], dy = yi - d.ys[t];
-        const double mh = (hi + d.hs[t]) / 2.;
-        double r;
-        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-        int &rank = (g == 2) ? rI : rX;
-        if (rank < lim) d.hq[m.q_off + rank] = mh;
-        ++rank;
-      }
-    }
-  }
-}
-
-// The same store in one kernel per instance for the step loop, where nip
-// itself is not reported: the ranks are the pair prefix over the cell lists
+// hsml_ij of the particle pairs ranked below min(bnip, store size): the
+// entries b_Wendland_C6 reads at the boundary-pair index
+// (boundary_kernel.f90:491); entries at or above nip keep what an earlier
+// evaluation wrote, as the reference's module array does.  The ranks are
+// the pair prefix over the cell lists
 // (per cell: all intra pairs, then all cross pairs, each in member order),
 // and only the pairs ranked below min(bnip, store size) are read by the
 // boundary kernel -- a short head of the list.  One block streams the
@@ -1128,7 +1055,38 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || inst_off(y)) return;
-  const int lim = min(y.bnip, m.q_cap);
+  // 0) boundary-pair ranks: per boundary particle the prefix over its 9
+  //    cells of the counts pass A gathered, then the prefix over boundary
+  //    particles (= rank of its first pair) and bnip; the counters are
+  //    cleared for the next evaluation
+  int bnip = 0;
+  {
+    const int per = (m.nb + kRankThreads - 1) / kRankThreads;
+    const int b0 = threadIdx.x * per;
+    int tsum = 0;
+    for (int b = b0; b < b0 + per && b < m.nb; ++b) {
+      const size_t bg = (size_t)(m.b_off + b);
+      int run = 0;
+      for (int s9 = 0; s9 < 9; ++s9) {
+        const int c9 = d.bcnt[bg * 9 + s9];
+        d.bcnt[bg * 9 + s9] = 0;
+        d.bpre[bg * 9 + s9] = run;
+        run += c9;
+      }
+      d.bcount[bg] = run;
+      tsum += run;
+    }
+    int ex = block_exclusive_scan<kRankThreads>(tsum, &bnip);
+    for (int b = b0; b < b0 + per && b < m.nb; ++b) {
+      d.bbase[m.b_off + b] = ex;
+      ex += d.bcount[m.b_off + b];
+    }
+    if (threadIdx.x == 0) {
+      d.dyn[k].bnip = bnip;
+      if (bnip > m.q_cap) atomicOr(&d.dyn[k].err, kErrCapacity);
+    }
+  }
+  const int lim = min(bnip, m.q_cap);
   if (lim <= 0) return;
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n;
@@ -1365,7 +1323,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
 
     // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
     // ascending boundary index (boundary_kernel.f90:477-502)
-    if (m.nb > 0 && d.bflag[m.c_off + c] == y.stamp) {
+    if (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp) {
       // the cell's boundary particles in ascending index, each with the
       // slot this cell takes in its 3x3 block
       int nbn = 0;
@@ -1608,9 +1566,6 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
 }
 void launch_ranks(const DevState &d, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
-}
-void launch_materialize(const DevState &d, hipStream_t st) {
-  k_materialize<<<g8(d.ninst * kMatBlocks), kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
   if (mixed && sumd)
