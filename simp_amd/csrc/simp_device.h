@@ -66,6 +66,8 @@ struct InstDyn {
   int active;           // 0: frozen (finished or not started), kernels skip it
   int steps_left;       // steps until the instance freezes; < 0: no limit
   int lstamp;           // stamp of the evaluation that built the boundary cell lists
+  int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
+  int gendI, gendX;     //   and the pair prefixes at the end of the lists
   double hmin;          // MINVAL(hsml) of the current evaluation
 };
 

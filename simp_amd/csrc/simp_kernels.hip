@@ -602,7 +602,6 @@ __device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx,
 // top row 2, in the middle row 0 / 1 / 2 by left-or-earlier / later own-cell
 // member / right.
 constexpr int kBatch = 4;    // neighbour loads issued together (exact test, pass B)
-constexpr int kBatchF = 4;   // candidate loads issued together (prefilter)
 
 __device__ __forceinline__ int pair_kind(const Ranges &R, int s, int self) {
   if (s < R.a[1]) return 0;
@@ -733,6 +732,9 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   }
   __syncthreads();
   if (!valid) return;
+  // waves with large supports (more neighbours) get issue priority: they
+  // set the kernel's length
+  if (__any(d.hs[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
   const int c = d.cells[self];
   const int cy = c / W, cx = c - cy * W;
   const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
@@ -1087,7 +1089,10 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
     }
   }
   const int lim = min(bnip, m.q_cap);
-  if (lim <= 0) return;
+  if (lim <= 0) {
+    if (threadIdx.x == 0) d.dyn[k].rlim = 0;
+    return;
+  }
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n;
   int2 *G = d.gpre + p0;       // per position: exclusive prefixes of intra, cross counts
@@ -1128,50 +1133,66 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
     baseX += totX;
     __syncthreads();
   }
-  const int stop = sh_stop;
-  const int2 Gend = make_int2(baseI, baseX);   // prefix at np when the loop reached it
-  // 2) the owned pairs of the needed cells that rank below lim
+  if (threadIdx.x == 0) {
+    InstDyn &yw = d.dyn[k];
+    yw.rlim = lim;
+    yw.rstop = sh_stop;
+    yw.gendI = baseI;   // prefixes at np, when the loop reached it
+    yw.gendX = baseX;
+  }
+}
+
+// 2) one thread per list position of the cells k_ranks found needed: the
+//    owned pairs that rank below lim get their hsml_ij
+__global__ void __launch_bounds__(kTPB) k_rankput(DevState d) {
+  int s = 0, k = 0;
+  if (!list_slot(d, s, k)) return;
+  const InstDyn &y = d.dyn[k];
+  const InstMeta &m = d.meta[k];
+  if (m.nb == 0 || inst_off(y)) return;
+  const int *cs = d.cell_start + m.c_off;
+  const int p0 = cs[0], np = m.n, t = s - p0;
+  const int lim = y.rlim;
+  if (t >= y.rstop || lim <= 0) return;
+  const int2 *G = d.gpre + p0;
   const size_t N = (size_t)d.ntot;
   const int W = y.W;
-  for (int t = threadIdx.x; t < stop; t += kRankThreads) {
-    const int s = p0 + t;
-    const int c = d.cells[s];
-    const int f = cs[c] - p0, e = cs[c + 1] - p0;
-    const int2 Gf = G[f], Gt = G[t];
-    const int2 Ge = e < np ? G[e] : Gend;
-    const int base = Gf.x + Gf.y;
-    if (base >= lim) continue;
-    const int oc = d.own_cnt[s];
-    const int nI = oc & 1023, nX = (oc >> 10) & 1023;
-    int rI = base + (Gt.x - Gf.x);
-    int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
-    if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) continue;
-    const double hi = d.hs[s];
-    const int nn = d.nnb[s];
-    if (nn <= kNbrMax) {
-      // the owned pairs are the tail of the list: nI intra, then nX cross
-      for (int j = nn - nI - nX; j < nn; ++j) {
-        const int q = d.nbr[j * N + s];
-        int &rank = (j < nn - nX) ? rI : rX;
-        if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[q]) / 2.;
-        ++rank;
-      }
-      continue;
+  const int c = d.cells[s];
+  const int f = cs[c] - p0, e = cs[c + 1] - p0;
+  const int2 Gf = G[f], Gt = G[t];
+  const int2 Ge = e < np ? G[e] : make_int2(y.gendI, y.gendX);
+  const int base = Gf.x + Gf.y;
+  if (base >= lim) return;
+  const int oc = d.own_cnt[s];
+  const int nI = oc & 1023, nX = (oc >> 10) & 1023;
+  int rI = base + (Gt.x - Gf.x);
+  int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
+  if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
+  const double hi = d.hs[s];
+  const int nn = d.nnb[s];
+  if (nn <= kNbrMax) {
+    // the owned pairs are the tail of the list: nI intra, then nX cross
+    for (int j = nn - nI - nX; j < nn; ++j) {
+      const int q = d.nbr[j * N + s];
+      int &rank = (j < nn - nX) ? rI : rX;
+      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[q]) / 2.;
+      ++rank;
     }
-    const int cy = c / W, cx = c - cy * W;
-    const double xi = d.xs[s], yi = d.ys[s];
-    const Ranges R = neighbour_ranges(cs, W, cx, cy);
-    for (int g = 2; g < 6; ++g) {
-      for (int q = R.a[g]; q < R.b[g]; ++q) {
-        if (g == 2 && q <= s) continue;
-        const double dx = xi - d.xs[q], dy = yi - d.ys[q];
-        const double mh = (hi + d.hs[q]) / 2.;
-        double r;
-        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-        int &rank = (g == 2) ? rI : rX;
-        if (rank < lim) d.hq[m.q_off + rank] = mh;
-        ++rank;
-      }
+    return;
+  }
+  const int cy = c / W, cx = c - cy * W;
+  const double xi = d.xs[s], yi = d.ys[s];
+  const Ranges R = neighbour_ranges(cs, W, cx, cy);
+  for (int g = 2; g < 6; ++g) {
+    for (int q = R.a[g]; q < R.b[g]; ++q) {
+      if (g == 2 && q <= s) continue;
+      const double dx = xi - d.xs[q], dy = yi - d.ys[q];
+      const double mh = (hi + d.hs[q]) / 2.;
+      double r;
+      if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+      int &rank = (g == 2) ? rI : rX;
+      if (rank < lim) d.hq[m.q_off + rank] = mh;
+      ++rank;
     }
   }
 }
@@ -1229,6 +1250,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
     // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
     // in batches of kBatch neighbours whose loads are issued together
     const int nn = d.nnb[self];
+    // waves with many or boundary neighbours get issue priority
+    if (__any(nn > 10 || (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp)))
+      __builtin_amdgcn_s_setprio(2);
     const size_t N = (size_t)d.ntot;
     using ST = typename std::conditional<MIXED, float, double>::type;
     const ST *S11 = (const ST *)d.sig11, *S12 = (const ST *)d.sig12, *S22 = (const ST *)d.sig22;
@@ -1566,6 +1590,7 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
 }
 void launch_ranks(const DevState &d, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
+  k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
   if (mixed && sumd)
