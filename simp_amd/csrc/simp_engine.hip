@@ -499,7 +499,6 @@ void Engine::enqueue_prefix(int buf) {
   L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, d.rebuild, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
-  if (any_boundary_) L("bprep", [&] { launch_bprep(d, st_); });
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {

@@ -14,7 +14,6 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
                  hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_cellsort(const DevState &d, int buf, hipStream_t st);
-void launch_bprep(const DevState &d, hipStream_t st);
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);

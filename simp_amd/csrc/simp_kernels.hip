@@ -7,7 +7,10 @@
 //   finalize     grid of the evaluation from per-block partial reductions of
 //                the bounding box and MINVAL(hsml) (nnps.f90:211-224); at
 //                the end of a step also the VP dt, time and step counter
-//                (time_step.f90:155, time_integration.f90:68-71)
+//                (time_step.f90:155, time_integration.f90:68-71); and the
+//                per-cell lists of the boundary particles whose 3x3 block
+//                covers the cell (nnps.f90:451-493), rebuilt only when a
+//                boundary particle's cell changes
 //   count        cell key NINT(x/bin) (nnps.f90:246-254); flags a rebuild
 //                of the cell lists when any key changed
 //   hist, scan x2, scatter, cellsort
@@ -17,9 +20,6 @@
 //                predictor's lists)
 //   listcopy     the right-hand-side state copied in list order, FP32
 //                prefilter data, per-cell largest hsml
-//   bprep        per-cell lists of the boundary particles whose 3x3 block
-//                covers the cell (nnps.f90:451-493); rebuilt only when a
-//                boundary particle's cell changes
 //   passA        neighbour gather: Wendland C6 gradient, velocity divergence
 //                and gradient, strain rate and the VP rheology epilogue
 //                (kernel.f90:471-520, vector_divergence.f90:27-28,
@@ -188,6 +188,113 @@ __device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok
   }
 }
 
+// For one instance, by one block: the cells of the boundary particles (whose
+// positions never change) under the grid of the next evaluation.  When none moved to
+// another cell since the lists were built, nothing is done.  Otherwise
+// every cell a boundary particle's 3x3 block covers is stamped with a new
+// list stamp and gets the list of the boundary particles whose own cell lies
+// in its 3x3 block, in ascending index -- the order in which the
+// reference's boundary pair list reaches a particle of that cell
+// (nnps.f90:451-493) -- with the slot the cell takes in each one's block.
+__device__ __forceinline__ int lower_bound_key(const unsigned long long *keys, int n,
+                                               unsigned long long v) {
+  int a = 0, b = n;
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (keys[mid] < v) a = mid + 1; else b = mid;
+  }
+  return a;
+}
+
+__device__ void boundary_lists(const DevState &d, int k) {
+  __shared__ unsigned long long keys[kMaxBoundaryPerInst];
+  __shared__ int alloc;
+  const int kBprepThreads = (int)blockDim.x;
+  const InstMeta &m = d.meta[k];
+  const InstDyn &y = d.dyn[k];
+  const double bin = y.bin;
+  const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
+  int changed = 0;
+  for (int b = threadIdx.x; b < m.nb; b += kBprepThreads) {
+    const int bg = m.b_off + b;
+    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
+    const int key = cyb * W + cxb;
+    changed |= d.bkey[bg] != key;
+    d.bkey[bg] = key;
+    keys[b] = ((unsigned long long)(unsigned)key << 32) | (unsigned)b;
+  }
+  if (!__syncthreads_or(changed)) return;
+  int P = 1;
+  while (P < m.nb) P <<= 1;
+  for (int i = m.nb + threadIdx.x; i < P; i += kBprepThreads) keys[i] = ~0ull;
+  if (threadIdx.x == 0) alloc = 0;
+  __syncthreads();
+  // bitonic sort of (cell, b)
+  for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += kBprepThreads) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long a = keys[i], b = keys[ixj];
+          const bool up = (i & k2) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Per covered cell, one thread (the first to stamp the cell's claim word
+  // with this stamp) lists the boundary particles whose own cell lies in
+  // the cell's 3x3 block -- three key ranges, one per row.
+  int2 *brun = d.brun + m.c_off;
+  int *blist = d.blist + 9 * (size_t)m.b_off;
+  for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
+    const int b = idx / 9, s9 = idx - 9 * (idx / 9);
+    const int c = d.bkey[m.b_off + b] + (s9 / 3 - 1) * W + (s9 % 3 - 1);
+    d.bflag[m.c_off + c] = stamp;
+    if (atomicExch(&d.bcellc[m.c_off + c], stamp) == stamp) continue;
+    int lo[3], hi[3], total = 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      // cells c0 .. c0 + 2 of one row (a row outside the grid is empty;
+      // the cells a row range borrows from a neighbouring row are margin
+      // cells, which hold no boundary particle)
+      const int c0 = c + (r - 1) * W - 1;
+      lo[r] = hi[r] = 0;
+      if (c0 + 3 > 0) {
+        lo[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)max(c0, 0) << 32);
+        hi[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)(c0 + 3) << 32);
+      }
+      total += hi[r] - lo[r];
+    }
+    const int start = atomicAdd(&alloc, total);
+    brun[c] = make_int2(start, start + total);
+    int last = -1;
+    for (int j = 0; j < total; ++j) {
+      int best = 0x7fffffff, bcell = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        for (int t = lo[r]; t < hi[r]; ++t) {
+          const int bl = (int)(keys[t] & 0xffffffffu);
+          if (bl > last && bl < best) {
+            best = bl;
+            bcell = (int)(keys[t] >> 32);
+          }
+        }
+      }
+      last = best;
+      // slot of c in the 3x3 block of the boundary particle's cell
+      const int dyc = (c - bcell + W + 1) / W;     // 0, 1, 2
+      const int dxc = c - bcell - (dyc - 1) * W + 1;
+      blist[start + j] = best | ((dyc * 3 + dxc) << 24);
+    }
+  }
+  if (threadIdx.x == 0) d.dyn[k].lstamp = stamp;
+}
+
 __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   if (blockIdx.x >= (unsigned)d.nblk) return;
   int p = 0, k = 0;
@@ -196,6 +303,9 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   double hv = ok ? d.hsml[buf][p] : 1.0;
   block_partials(d, k, ok, xv, yv, hv);
 }
+
+__device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
+                             double ymn, double ymx, double hmn);
 
 // One block per instance: fold the instance's block partials into the grid
 // of the next evaluation (bucket_search, nnps.f90:211-224).  With
@@ -207,7 +317,6 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
-  InstDyn &y = d.dyn[k];
   const double big = __builtin_huge_val();
   double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big;
   for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
@@ -224,16 +333,29 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
     sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int i = 1; i < kTPB / 64; ++i) {
-    v0 = fmin(sh[0][0], sh[0][i]); sh[0][0] = v0;
-    v1 = fmax(sh[1][0], sh[1][i]); sh[1][0] = v1;
-    v2 = fmin(sh[2][0], sh[2][i]); sh[2][0] = v2;
-    v3 = fmax(sh[3][0], sh[3][i]); sh[3][0] = v3;
-    v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
+  __shared__ int go;
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kTPB / 64; ++i) {
+      v0 = fmin(sh[0][0], sh[0][i]); sh[0][0] = v0;
+      v1 = fmax(sh[1][0], sh[1][i]); sh[1][0] = v1;
+      v2 = fmin(sh[2][0], sh[2][i]); sh[2][0] = v2;
+      v3 = fmax(sh[3][0], sh[3][i]); sh[3][0] = v3;
+      v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
+    }
+    go = finalize_grid(d, k, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0]);
   }
-  double xmn = sh[0][0], xmx = sh[1][0], ymn = sh[2][0], ymx = sh[3][0], hmn = sh[4][0];
-  if (inst_off(y)) return;
+  __syncthreads();
+  // the boundary particles' cell lists for the next evaluation
+  if (go && m.nb > 0) boundary_lists(d, k);
+}
+
+// Thread 0 of k_finalize: close the step, then the grid of the next
+// evaluation.  Returns whether the instance evaluates next.
+__device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
+                             double ymn, double ymx, double hmn) {
+  const InstMeta &m = d.meta[k];
+  InstDyn &y = d.dyn[k];
+  if (inst_off(y)) return 0;
   if (end_of_step) {
     // VP_time_step: MINVAL(rho_ice hsml^2 delta_min / (P_star (1 + k_t))) is
     // the same expression at MINVAL(hsml), each operation being monotone
@@ -242,7 +364,7 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
     y.time = y.time + y.dt;
     if (y.steps_left > 0 && --y.steps_left == 0) {
       y.active = 0;   // finished: frozen from the next evaluation on
-      return;
+      return 0;
     }
   }
   if (m.nb > 0) {
@@ -255,7 +377,7 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   const long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
   if (!(hmn > 0.0) || W * H > (long long)m.c_cap || W * H <= 0) {
     y.err |= kErrCapacity;
-    return;
+    return 0;
   }
   y.hmin = hmn;
   y.bin = bin;
@@ -268,6 +390,7 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   y.W = (int)W;
   y.H = (int)H;
   y.stamp += 1;
+  return 1;
 }
 
 // ------------------------------------------------------------------ count
@@ -459,117 +582,6 @@ __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
 }
 
 // -------------------------------------------------------------- boundary
-
-// One block per instance: the cells of the boundary particles (whose
-// positions never change) under the current grid.  When none moved to
-// another cell since the lists were built, nothing is done.  Otherwise
-// every cell a boundary particle's 3x3 block covers is stamped with a new
-// list stamp and gets the list of the boundary particles whose own cell lies
-// in its 3x3 block, in ascending index -- the order in which the
-// reference's boundary pair list reaches a particle of that cell
-// (nnps.f90:451-493) -- with the slot the cell takes in each one's block.
-constexpr int kBprepThreads = 1024;
-
-__device__ __forceinline__ int lower_bound_key(const unsigned long long *keys, int n,
-                                               unsigned long long v) {
-  int a = 0, b = n;
-  while (a < b) {
-    const int mid = (a + b) >> 1;
-    if (keys[mid] < v) a = mid + 1; else b = mid;
-  }
-  return a;
-}
-
-__global__ void __launch_bounds__(kBprepThreads) k_bprep(DevState d) {
-  __shared__ unsigned long long keys[kMaxBoundaryPerInst];
-  __shared__ int alloc;
-  if (blockIdx.x >= (unsigned)d.ninst) return;
-  const int k = blockIdx.x;
-  const InstMeta &m = d.meta[k];
-  const InstDyn &y = d.dyn[k];
-  if (m.nb == 0 || inst_off(y)) return;
-  const double bin = y.bin;
-  const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
-  int changed = 0;
-  for (int b = threadIdx.x; b < m.nb; b += kBprepThreads) {
-    const int bg = m.b_off + b;
-    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
-    const int key = cyb * W + cxb;
-    changed |= d.bkey[bg] != key;
-    d.bkey[bg] = key;
-    keys[b] = ((unsigned long long)(unsigned)key << 32) | (unsigned)b;
-  }
-  if (!__syncthreads_or(changed)) return;
-  int P = 1;
-  while (P < m.nb) P <<= 1;
-  for (int i = m.nb + threadIdx.x; i < P; i += kBprepThreads) keys[i] = ~0ull;
-  if (threadIdx.x == 0) alloc = 0;
-  __syncthreads();
-  // bitonic sort of (cell, b)
-  for (int k2 = 2; k2 <= P; k2 <<= 1) {
-    for (int j = k2 >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += kBprepThreads) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const unsigned long long a = keys[i], b = keys[ixj];
-          const bool up = (i & k2) == 0;
-          if ((a > b) == up) {
-            keys[i] = b;
-            keys[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  // Per covered cell, one thread (the first to stamp the cell's claim word
-  // with this stamp) lists the boundary particles whose own cell lies in
-  // the cell's 3x3 block -- three key ranges, one per row.
-  int2 *brun = d.brun + m.c_off;
-  int *blist = d.blist + 9 * (size_t)m.b_off;
-  for (int idx = threadIdx.x; idx < m.nb * 9; idx += kBprepThreads) {
-    const int b = idx / 9, s9 = idx - 9 * (idx / 9);
-    const int c = d.bkey[m.b_off + b] + (s9 / 3 - 1) * W + (s9 % 3 - 1);
-    d.bflag[m.c_off + c] = stamp;
-    if (atomicExch(&d.bcellc[m.c_off + c], stamp) == stamp) continue;
-    int lo[3], hi[3], total = 0;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      // cells c0 .. c0 + 2 of one row (a row outside the grid is empty;
-      // the cells a row range borrows from a neighbouring row are margin
-      // cells, which hold no boundary particle)
-      const int c0 = c + (r - 1) * W - 1;
-      lo[r] = hi[r] = 0;
-      if (c0 + 3 > 0) {
-        lo[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)max(c0, 0) << 32);
-        hi[r] = lower_bound_key(keys, m.nb, (unsigned long long)(unsigned)(c0 + 3) << 32);
-      }
-      total += hi[r] - lo[r];
-    }
-    const int start = atomicAdd(&alloc, total);
-    brun[c] = make_int2(start, start + total);
-    int last = -1;
-    for (int j = 0; j < total; ++j) {
-      int best = 0x7fffffff, bcell = 0;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        for (int t = lo[r]; t < hi[r]; ++t) {
-          const int bl = (int)(keys[t] & 0xffffffffu);
-          if (bl > last && bl < best) {
-            best = bl;
-            bcell = (int)(keys[t] >> 32);
-          }
-        }
-      }
-      last = best;
-      // slot of c in the 3x3 block of the boundary particle's cell
-      const int dyc = (c - bcell + W + 1) / W;     // 0, 1, 2
-      const int dxc = c - bcell - (dyc - 1) * W + 1;
-      blist[start + j] = best | ((dyc * 3 + dxc) << 24);
-    }
-  }
-  if (threadIdx.x == 0) d.dyn[k].lstamp = stamp;
-}
 
 // --------------------------------------------------------------- neighbours
 
@@ -1570,9 +1582,6 @@ void launch_scatter(const DevState &d, hipStream_t st) {
 void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
   k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
   k_listcopy<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
-}
-void launch_bprep(const DevState &d, hipStream_t st) {
-  k_bprep<<<g8(d.ninst), kBprepThreads, 0, st>>>(d);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st) {
