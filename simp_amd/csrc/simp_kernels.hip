@@ -734,8 +734,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   }
   const int lA = Ae - As, lB = Be - Bs, lC = Ce - Cs;
   const int tB = lA, tC = lA + lB, T = lA + lB + lC;
-  const int lbase = min(As, min(Bs, Cs));  // list entries are offsets from here
-  const bool tiled = T <= kTile && Ce - lbase < 65535;
+  const bool tiled = T <= kTile;      // list entries are tile indices (< kTile)
   if (tiled) {
     for (int i = threadIdx.x; i < T; i += kTPB) {
       const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
@@ -805,23 +804,12 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   if (!tiled) {
     nc = kNbrMax + 1;
   } else {
-    const float4 pi4 = tile[self >= Cs ? self - Cs + tC : (self >= Bs ? self - Bs + tB : self - As)];
+    // tile index of a list position (every position of the block's 3x3
+    // blocks lies in one of the runs)
+    auto tix = [&](int sp) { return sp >= Cs ? sp - Cs + tC : (sp >= Bs ? sp - Bs + tB : sp - As); };
+    const int tself = tix(self);
+    const float4 pi4 = tile[tself];
     const float slack = y.pf_slack;
-    // one segment of the 3x3 block (it lies inside one run of the tile)
-    auto scan = [&](int a, int b) {
-      const int off = a >= Cs ? tC - Cs : (a >= Bs ? tB - Bs : -As);
-      for (int sc = a; sc < b; ++sc) {
-        const float4 q = tile[sc + off];
-        // rejects only candidates certainly beyond mhsml
-        const float fx = pi4.x - q.x, fy = pi4.y - q.y;
-        const float fm = __builtin_fmaf(pi4.z + q.z, 0.5f, slack);
-        const bool hit = (sc != self) & (__builtin_fmaf(fx, fx, fy * fy) <= fm * fm * 1.00001f);
-        if (hit) {
-          if (nc < kNbrMax) lst[nc * kTPB + threadIdx.x] = (unsigned short)(sc - lbase);
-          ++nc;
-        }
-      }
-    };
     // Cell culling: a neighbour cell is scanned only if its nearest point
     // is within the largest support a pair with one of its members can
     // have, (hsml_i + max hsml of the cell) / 2 (cells span
@@ -843,23 +831,46 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       const float bnd = __builtin_fmaf(pi4.z + hmax, 0.5f, slack);
       return __builtin_fmaf(ex, ex, ey * ey) <= bnd * bnd * 1.00001f;
     };
-    // row below: cells rm-1, rm, rm+1 -> the span from the first to the
-    // last cell needed
+    // The four segments, in visiting order: row below (cells rm-1, rm,
+    // rm+1 -> the span from the first to the last needed), left / own /
+    // right, up / up-right, up-left.
+    int a0 = 0, b0 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
     {
       const bool l = need(rm - 1, dxl, dyb), c0 = need(rm, 0.0f, dyb), r = need(rm + 1, dxr, dyb);
       const int f = l ? 0 : c0 ? 1 : r ? 2 : 3, e = r ? 3 : c0 ? 2 : l ? 1 : 0;
-      if (f < e) scan(cs[rm - 1 + f], cs[rm - 1 + e]);
+      if (f < e) { a0 = cs[rm - 1 + f]; b0 = cs[rm - 1 + e]; }
     }
-    // left, own, right
-    scan(need(r0 - 1, dxl, 0.0f) ? R.a[1] : R.a[2], need(r0 + 1, dxr, 0.0f) ? R.b[3] : R.b[2]);
-    // up, up-right, then up-left
+    const int a1 = need(r0 - 1, dxl, 0.0f) ? R.a[1] : R.a[2];
+    const int b1 = need(r0 + 1, dxr, 0.0f) ? R.b[3] : R.b[2];
     {
       const bool u = need(rp, 0.0f, dyt), ur = need(rp + 1, dxr, dyt);
       const int f = u ? 0 : ur ? 1 : 2, e = ur ? 2 : u ? 1 : 0;
-      if (f < e) scan(cs[rp + f], cs[rp + e]);
-      if (need(rp - 1, dxl, dyt)) scan(R.a[5], R.a[4]);
+      if (f < e) { a2 = cs[rp + f]; b2 = cs[rp + e]; }
+      if (need(rp - 1, dxl, dyt)) { a3 = R.a[5]; b3 = R.a[4]; }
+    }
+    // One loop over the concatenated segments: a wavefront then runs as
+    // long as its longest lane's total, not the sum of per-segment maxima.
+    // j -> tile index j + o_k on segment k.
+    const int e0 = b0 - a0, e1 = e0 + (b1 - a1), e2 = e1 + (b2 - a2), e3 = e2 + (b3 - a3);
+    const int o0 = e0 > 0 ? tix(a0) : 0;
+    const int o1 = tix(a1) - e0;
+    const int o2 = (b2 > a2 ? tix(a2) : 0) - e1;
+    const int o3 = (b3 > a3 ? tix(a3) : 0) - e2;
+    for (int j = 0; j < e3; ++j) {
+      const int ti = j + (j < e0 ? o0 : (j < e1 ? o1 : (j < e2 ? o2 : o3)));
+      const float4 q = tile[ti];
+      // rejects only candidates certainly beyond mhsml
+      const float fx = pi4.x - q.x, fy = pi4.y - q.y;
+      const float fm = __builtin_fmaf(pi4.z + q.z, 0.5f, slack);
+      const bool hit = (ti != tself) & (__builtin_fmaf(fx, fx, fy * fy) <= fm * fm * 1.00001f);
+      if (hit) {
+        if (nc < kNbrMax) lst[nc * kTPB + threadIdx.x] = (unsigned short)ti;
+        ++nc;
+      }
     }
   }
+  // list entry (tile index) -> list position
+  auto lpos = [&](int ti) { return ti < tB ? As + ti : (ti < tC ? Bs + ti - tB : Cs + ti - tC); };
   const size_t N = (size_t)d.ntot;
   // kernel_zero(i) (kernel.f90:503): the factor of the last pair in the
   // reference's order that has this particle first, i.e. its last owned
@@ -872,7 +883,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     bool found = false;
     if (nc <= kNbrMax) {
       for (int j = nc - 1; j >= 0 && !found; --j) {
-        const int t = (int)lst[j * kTPB + threadIdx.x] + lbase;
+        const int t = lpos((int)lst[j * kTPB + threadIdx.x]);
         if (pair_kind(R, t, self) == 0) break;
         const double dx = xi - d.xs[t], dy = yi - d.ys[t];
         const double mh = (hi + d.hs[t]) / 2.;
@@ -905,7 +916,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       int sj[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u)
-        sj[u] = j0 + u < nc ? (int)lst[(j0 + u) * kTPB + threadIdx.x] + lbase : self;
+        sj[u] = j0 + u < nc ? lpos((int)lst[(j0 + u) * kTPB + threadIdx.x]) : self;
       // positions and pair data of the batch in one round of loads
       double xq[kBatch], yq[kBatch], hq[kBatch], mq[kBatch], rq[kBatch], uq[kBatch], wq[kBatch];
 #pragma unroll
