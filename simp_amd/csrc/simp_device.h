@@ -75,7 +75,8 @@ struct InstDyn {
 // one instance.
 struct BlockEntry {
   int inst, first;   // instance, first local particle index
-  int lid, pad;      // logical block id (index of the block's partials)
+  int lid;           // logical block id (index of the block's partials)
+  int nloc;          // particles in the block (min(n - first, kTPB))
 };
 
 // Device pointers of one engine.
@@ -91,6 +92,7 @@ struct DevState {
   // gathers run one thread per list position and read neighbours here
   double *xs, *ys, *hs, *vxs, *vys, *ms, *rs;
   int *cells;            // cell (local id) of each list position
+  int *lbase;            // per block: list position of its first particle
   // per cell: (stamp of the evaluation << 32) | largest member hsml as
   // FP32 bits, rounded up
   unsigned long long *cellh;

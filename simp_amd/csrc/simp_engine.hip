@@ -189,6 +189,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.meta, std::max<size_t>(INST, 1));
   rc |= dalloc(list, &d.dyn, std::max<size_t>(INST, 1));
   rc |= dalloc(list, &d.blk, std::max<size_t>(BLK, 1));
+  rc |= dalloc(list, &d.lbase, std::max<size_t>(BLK, 1));
   rc |= dalloc(list, scan_tmp, (size_t)cdiv((long)C + 1, kScanItems) + 1);
   return rc ? -6 : 0;
 }
@@ -439,7 +440,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // physical table gives each XCD a contiguous run of logical blocks.
   std::vector<BlockEntry> lg;
   for (int k = 0; k < ninst; ++k)
-    for (int f = 0; f < nm[k].n; f += kTPB) lg.push_back(BlockEntry{k, f, (int)lg.size(), 0});
+    for (int f = 0; f < nm[k].n; f += kTPB)
+      lg.push_back(BlockEntry{k, f, (int)lg.size(), std::min(nm[k].n - f, kTPB)});
   std::vector<BlockEntry> phys(lg.size());
   {
     const int nb_ = (int)lg.size(), q = nb_ / 8, r = nb_ % 8;

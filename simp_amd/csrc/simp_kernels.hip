@@ -544,6 +544,7 @@ __global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
   const int k = e.inst;
   const InstMeta &m = d.meta[k];
   const InstDyn &y = d.dyn[k];
+  if (threadIdx.x == 0) d.lbase[blockIdx.x] = d.cell_start[m.c_off] + e.first;
   if (inst_off(y)) return;                  // whole block
   const int local = e.first + (int)threadIdx.x;
   const bool valid = local < m.n;
@@ -685,14 +686,14 @@ struct KSum {
 
 // The gathers run one thread per cell-list position: the threads of one
 // cell walk the same candidate ranges, so a wavefront's loads coalesce.
+// (The block's first list position comes from lbase, read in parallel with
+// the block table entry.)
 __device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
   if (blockIdx.x >= (unsigned)d.nblk) return false;
   const BlockEntry e = d.blk[blockIdx.x];
   k = e.inst;
-  const int local = e.first + (int)threadIdx.x;
-  const InstMeta &m = d.meta[k];
-  if (local >= m.n) return false;
-  s = d.cell_start[m.c_off] + local;
+  if ((int)threadIdx.x >= e.nloc) return false;
+  s = d.lbase[blockIdx.x] + (int)threadIdx.x;
   return true;
 }
 
@@ -713,8 +714,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const InstMeta &m = d.meta[k];
   const int W = y.W;
   const int *cs0 = d.cell_start + m.c_off;
-  const int pos0 = cs0[0] + be.first;     // first list position of the block
-  const int nloc = min(m.n - be.first, kTPB);
+  const int pos0 = d.lbase[blockIdx.x];   // first list position of the block
+  const int nloc = be.nloc;
   const bool valid = (int)threadIdx.x < nloc;
   const int self = pos0 + min((int)threadIdx.x, nloc - 1);
   // The candidates of every particle of the block lie in three runs of the
@@ -750,6 +751,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const int cy = c / W, cx = c - cy * W;
   const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
   const double vxi = d.vxs[self], vyi = d.vys[self];
+  const double rhoi = SUMD ? d.rs[self] : 0.0;
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   const int p = d.cl[self];
@@ -907,7 +909,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         }
       }
     }
-    const double mi = d.ms[self], rhoi = d.rs[self];
+    const double mi = d.ms[self];
     num = kzi * mi;
     den = kzi * mi / rhoi;
   }
@@ -1248,8 +1250,8 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 
 // phase: 0 = predictor (state S -> T), 1 = corrector (RHS on T, update S in
 // place), 2 = single evaluation (tendencies to d.tend)
-template <bool MIXED, bool SUMD>
-__global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
+template <bool MIXED, bool SUMD, int phase>
+__global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
   int self = 0, k = 0;
   const bool ok = list_slot(d, self, k);
@@ -1260,37 +1262,107 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   if (live) {
     const int W = y.W;
-    const int c = d.cells[self];
-    const int cy = c / W, cx = c - cy * W;
-    const int p = d.cl[self];
-    const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
-    const double rhoi = d.rs[self], mi = d.ms[self];
-    const double vxi = d.vxs[self], vyi = d.vys[self];
-    const double hthi = d.h[buf][p], Ai = d.A[buf][p];
-    const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
-
-    double ax = 0., ay = 0.;
-    // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
-    // in batches of kBatch neighbours whose loads are issued together
-    const int nn = d.nnb[self];
-    // waves with many or boundary neighbours get issue priority
-    if (__any(nn > 10 || (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp)))
-      __builtin_amdgcn_s_setprio(2);
     const size_t N = (size_t)d.ntot;
     using ST = typename std::conditional<MIXED, float, double>::type;
     const ST *S11 = (const ST *)d.sig11, *S12 = (const ST *)d.sig12, *S22 = (const ST *)d.sig22;
-    const double ri2 = rhoi * rhoi;
-    const double s11d = (double)S11[self], s12d = (double)S12[self], s22d = (double)S22[self];
-    const float ir2 = (float)(1.0 / ri2);
-    const float s11f = (float)s11d * ir2, s12f = (float)s12d * ir2, s22f = (float)s22d * ir2;
-    KSum sx, sy;
     // third density call of the step (time_integration.f90:262): the
     // corrector's sum on the half-step kernel values with the densities
     // the second call produced (pass A's dsum)
     // (FP64 in both precision modes, as in pass A)
     const bool sumd3 = SUMD && phase == 1 && m.den_alg == 1;
+    // The loads go out in two rounds, each issued whole before any of its
+    // values is used: (1) everything indexed by the list position,
+    // including the first kBatch neighbour-list entries (read whole: the
+    // entries past the count are never dereferenced); (2) everything the
+    // particle id, the cell and those entries index.  A wave so waits for
+    // two memory round trips before its first neighbour arithmetic.
+    const int c = d.cells[self];
+    const int p = d.cl[self];
+    const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
+    const double rhoi = d.rs[self], mi = d.ms[self];
+    const double vxi = d.vxs[self], vyi = d.vys[self];
+    const int nn = d.nnb[self];
+    const double divi = ((const double *)d.div)[self];
+    const double s11d = (double)S11[self], s12d = (double)S12[self], s22d = (double)S22[self];
+    int nx[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) nx[u] = d.nbr[u * N + self];
+    // summation-density instance: pass A's density (predictor) or its
+    // numerator and density (corrector)
+    const bool sd = SUMD && m.den_alg == 1;
+    double dsi = 1.0, dnumi = 0.;
+    if (sd && phase != 2) dsi = d.dsum[self];
+    if (sumd3) dnumi = d.dnum[self];
+    // round 2
+    const double hthi = d.h[buf][p], Ai = d.A[buf][p];
+    const int bfl = d.bflag[m.c_off + c];
+    double x1 = 0., y1 = 0., h1 = 0., A1 = 0., kzi = 0.;
+    if (phase == 1) {
+      x1 = d.x[0][p]; y1 = d.y[0][p];
+      h1 = d.h[0][p]; A1 = d.A[0][p];
+    }
+    if (sumd3) kzi = d.kz[p];
+
+    // Everything of the update but the velocities is known before the
+    // stress sums (single_step.f90:104-106 tendencies; predictor
+    // time_integration.f90:207-218, corrector :236-280 on the step-start
+    // state x1 ... A1) and is stored now: the registers stay free.
+    const double dhdt = -(hthi * divi / rhoi);
+    const double dAdt = -(Ai * divi / rhoi);
+    const double dt = y.dt;
+    if (phase == 0) {
+      px_new = xi + vxi * dt / 2;
+      py_new = yi + vyi * dt / 2;
+      hsml_new = hi;
+      d.x[1][p] = px_new;
+      d.y[1][p] = py_new;
+      const double hh = hthi + dhdt * dt / 2;
+      double Ah = Ai + dAdt * dt / 2;
+      if (Ah > 1) Ah = 1;
+      d.h[1][p] = hh;
+      d.A[1][p] = Ah;
+      // density (time_integration.f90:218): diagnostic, or the summation
+      // pass A computed on the step-start kernel values
+      d.rho[1][p] = sd ? dsi : hh * kRhoIce;
+      d.hsml[1][p] = hi;
+    } else if (phase == 1) {
+      const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
+      const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
+      px_new = 2 * x12 - x1;
+      py_new = 2 * y12 - y1;
+      d.x[0][p] = px_new;
+      d.y[0][p] = py_new;
+      const double hn = 2 * h12 - h1;
+      double An = 2 * A12 - A1;
+      if (An > 1) An = 1;
+      d.h[0][p] = hn;
+      d.A[0][p] = An;
+      if (!sumd3) {
+        const double rn = hn * kRhoIce;
+        d.rho[0][p] = rn;
+        // Marquis_hsml (smoothing_length.f90:136-142)
+        double hs = sqrt(mi / rn) * kHsmlMult;
+        if (hs > m.max_hsml) hs = m.max_hsml;
+        d.hsml[0][p] = hs;
+        hsml_new = hs;
+      }
+    }
+    const bool listed = nn <= kNbrMax;
+    int sj[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) sj[u] = (listed && u < nn) ? nx[u] : self;
+
+    double ax = 0., ay = 0.;
+    // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
+    // in batches of kBatch neighbours whose loads are issued together
+    // waves with many or boundary neighbours get issue priority
+    if (__any(nn > 10 || (m.nb > 0 && bfl == y.lstamp))) __builtin_amdgcn_s_setprio(2);
+    const double ri2 = rhoi * rhoi;
+    const float ir2 = (float)(1.0 / ri2);
+    const float s11f = (float)s11d * ir2, s12f = (float)s12d * ir2, s22f = (float)s22d * ir2;
+    KSum sx, sy;
     double den3 = 0.;
-    if (sumd3) den3 = d.kz[p] * mi / d.dsum[self];
+    if (sumd3) den3 = kzi * mi / dsi;
     auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
       const double w = c6_w(sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
@@ -1324,11 +1396,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
         sy.add(mf * (t12 * dw0 + t22 * dw1));
       }
     };
-    if (nn <= kNbrMax) {
+    if (listed) {
       for (int j0 = 0; j0 < nn; j0 += kBatch) {
-        int sj[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) sj[u] = j0 + u < nn ? d.nbr[(j0 + u) * N + self] : self;
         double xq[kBatch], yq[kBatch], hq[kBatch], rq[kBatch], mq[kBatch], r2[kBatch];
         ST q11[kBatch], q12[kBatch], q22[kBatch];
 #pragma unroll
@@ -1339,6 +1408,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
           q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
           r2[u] = sumd3 ? d.dsum[t] : 1.0;
         }
+        // the next batch's list entries go out with this batch's data
+        // (past the end of the list, the last entry again: no line of the
+        // list is read that the particle does not use)
+        const int j1 = j0 + kBatch;
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) nx[u] = d.nbr[min(j1 + u, nn - 1) * N + self];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
           if (j0 + u >= nn) break;
@@ -1348,8 +1423,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
                       q12[u], q22[u]);
           if (sumd3) dens_pair(dx, dy, mh, mq[u], r2[u]);
         }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) sj[u] = j1 + u < nn ? nx[u] : self;
       }
     } else {
+      const int cy = c / W, cx = c - cy * W;
+      const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
       for (int g = 0; g < 6; ++g) {
         for (int s = R.a[g]; s < R.b[g]; ++s) {
           if (s == self) continue;
@@ -1370,9 +1449,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
 
     // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
     // ascending boundary index (boundary_kernel.f90:477-502)
-    if (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp) {
+    if (m.nb > 0 && bfl == y.lstamp) {
       // the cell's boundary particles in ascending index, each with the
       // slot this cell takes in its 3x3 block
+      const int own0 = d.cell_start[m.c_off + c];   // first member of the own cell
       int nbn = 0;
       const int2 run = d.brun[m.c_off + c];
       const int *bl = d.blist + 9 * (size_t)m.b_off;
@@ -1385,7 +1465,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
         if (!(br <= kBLength)) continue;
         ++nbn;
         int before = 0;
-        for (int s = R.a[2]; s < self; ++s) {
+        for (int s = own0; s < self; ++s) {
           const double ex = d.xb[bg] - d.xs[s], ey = d.yb[bg] - d.ys[s];
           if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
         }
@@ -1423,56 +1503,24 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d, int phase) {
       ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
       ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
     }
-    // tendencies (single_step.f90:104-106)
-    const double divi = ((const double *)d.div)[self];
-    const double dhdt = -(hthi * divi / rhoi);
-    const double dAdt = -(Ai * divi / rhoi);
-    const double dt = y.dt;
+    // velocities (time_integration.f90:210-211, 240-241) or tendencies
     if (phase == 0) {
-      // predict half step (time_integration.f90:207-218)
-      px_new = xi + vxi * dt / 2;
-      py_new = yi + vyi * dt / 2;
-      hsml_new = hi;
-      d.x[1][p] = px_new;
-      d.y[1][p] = py_new;
       d.vx[1][p] = vxi + ax * dt / 2;
       d.vy[1][p] = vyi + ay * dt / 2;
-      const double hh = hthi + dhdt * dt / 2;
-      double Ah = Ai + dAdt * dt / 2;
-      if (Ah > 1) Ah = 1;
-      d.h[1][p] = hh;
-      d.A[1][p] = Ah;
-      // density (time_integration.f90:218): diagnostic, or the summation
-      // pass A computed on the step-start kernel values
-      d.rho[1][p] = (SUMD && m.den_alg == 1) ? d.dsum[self] : hh * kRhoIce;
-      d.hsml[1][p] = hi;
     } else if (phase == 1) {
-      // corrector (time_integration.f90:236-280) on the step-start state S
-      const double x1 = d.x[0][p], y1 = d.y[0][p];
       const double u1 = d.vx[0][p], v1 = d.vy[0][p];
-      const double h1 = d.h[0][p], A1 = d.A[0][p];
-      const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
       const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
-      const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
-      px_new = 2 * x12 - x1;
-      py_new = 2 * y12 - y1;
-      d.x[0][p] = px_new;
-      d.y[0][p] = py_new;
       d.vx[0][p] = 2 * u12 - u1;
       d.vy[0][p] = 2 * v12 - v1;
-      const double hn = 2 * h12 - h1;
-      double An = 2 * A12 - A1;
-      if (An > 1) An = 1;
-      d.h[0][p] = hn;
-      d.A[0][p] = An;
-      double rn = hn * kRhoIce;
-      if (sumd3) rn = d.dnum[self] / den3;
-      d.rho[0][p] = rn;
-      // Marquis_hsml (smoothing_length.f90:136-142)
-      double hs = sqrt(mi / rn) * kHsmlMult;
-      if (hs > m.max_hsml) hs = m.max_hsml;
-      d.hsml[0][p] = hs;
-      hsml_new = hs;
+      if (sumd3) {
+        // summation density of the corrector, then Marquis_hsml
+        const double rn = dnumi / den3;
+        d.rho[0][p] = rn;
+        double hs = sqrt(mi / rn) * kHsmlMult;
+        if (hs > m.max_hsml) hs = m.max_hsml;
+        d.hsml[0][p] = hs;
+        hsml_new = hs;
+      }
     } else {
       const int N = d.ntot;
       d.tend[p] = vxi;
@@ -1612,15 +1660,24 @@ void launch_ranks(const DevState &d, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
   k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
+template <bool MIXED, bool SUMD>
+static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
+  if (phase == 0)
+    k_passB<MIXED, SUMD, 0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else if (phase == 1)
+    k_passB<MIXED, SUMD, 1><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else
+    k_passB<MIXED, SUMD, 2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
 void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
   if (mixed && sumd)
-    k_passB<true, true><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
+    launch_passB_t<true, true>(d, phase, st);
   else if (mixed)
-    k_passB<true, false><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
+    launch_passB_t<true, false>(d, phase, st);
   else if (sumd)
-    k_passB<false, true><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
+    launch_passB_t<false, true>(d, phase, st);
   else
-    k_passB<false, false><<<g8(d.nblk), kTPB, 0, st>>>(d, phase);
+    launch_passB_t<false, false>(d, phase, st);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st) {
