@@ -3,7 +3,10 @@
 // packed into one set of structure-of-arrays buffers; every instance owns a
 // contiguous particle range, boundary range, cell range and boundary-pair
 // range, described by InstMeta (fixed between uploads) and InstDyn (changes
-// every evaluation).
+// every evaluation).  A particle's slot in its instance's range is its
+// position in the cell lists: whenever the lists are rebuilt the particle
+// state moves with them, so the gathers read the state itself, in list
+// order, and orig[] keeps each slot's reference particle index.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +36,10 @@ constexpr int kTPB = 256;           // threads per block of particle kernels
 constexpr int kScanItems = 2048;    // elements per block of the scan
 constexpr int kMaxBoundaryPerInst = 8192;
 constexpr int kNbrMax = 32;         // neighbours kept per particle in the list
+// moved with the slots at a rebuild: x, y, vx, vy, h, A, rho, hsml of both
+// buffers and the mass; the reference index and the cell
+constexpr int kPermDoubles = 17;
+constexpr int kPermInts = 2;
 
 // Error bits (OR-ed into InstDyn::err); translated to SIMP_E_* on the host.
 constexpr int kErrCapacity = 1;
@@ -43,7 +50,8 @@ struct InstMeta {
   int p_off, n;          // particle range
   int blk_off, nblk;     // logical particle-block range
   int b_off, nb;         // boundary-particle range
-  int c_off, c_cap;      // cell range (capacity)
+  int c_off, c_cap;      // cell range (capacity; the last cell is kept for the
+                         // slots of a frozen instance at a rebuild)
   int q_off, q_cap;      // boundary-pair / hsml_ij-store range
   int tse, wind_mode;
   int g_off, g_nx, g_ny;  // wind grid range
@@ -71,34 +79,37 @@ struct InstDyn {
   double hmin;          // MINVAL(hsml) of the current evaluation
 };
 
-// Per-launch block table entry: one block covers kTPB particles of exactly
-// one instance.
+// Per-launch block table entry: one block covers kTPB particle slots of
+// exactly one instance.
 struct BlockEntry {
-  int inst, first;   // instance, first local particle index
+  int inst, first;   // instance, first local slot
   int lid;           // logical block id (index of the block's partials)
-  int nloc;          // particles in the block (min(n - first, kTPB))
+  int nloc;          // slots in the block (min(n - first, kTPB))
+  int pos;           // packed index of the first slot (p_off + first)
+  int pad[3];
 };
 
 // Device pointers of one engine.
 struct DevState {
-  // state buffers: [0] = S (step start / current), [1] = T (half step)
+  // state buffers, by slot (= cell-list position): [0] = S (step start /
+  // current), [1] = T (half step)
   double *x[2], *y[2], *vx[2], *vy[2], *h[2], *A[2], *rho[2], *hsml[2];
   double *mass;
-  int *orig;            // local (0-based) particle id inside its instance
-  int *inst;            // instance of each packed particle
-  // per evaluation
-  int *cellid, *slot, *cl, *clpos, *count, *cell_start, *scan_tmp;
-  // the right-hand-side state in cell-list order (copied by cellsort): the
-  // gathers run one thread per list position and read neighbours here
-  double *xs, *ys, *hs, *vxs, *vys, *ms, *rs;
-  int *cells;            // cell (local id) of each list position
-  int *lbase;            // per block: list position of its first particle
+  int *orig;            // reference (0-based) particle index of each slot
+  int *cellid;          // cell (local id) of each slot
+  // rebuild of the cell lists: slot of each particle in its cell, new list
+  // order (cl[new slot] = old slot), cell counts and starts
+  int *slot, *cl, *count, *cell_start, *scan_tmp;
+  // the state moved to the new order goes through these (kPermDoubles x N
+  // doubles, kPermInts x N ints) on its way back
+  double *perm_d;
+  int *perm_i;
   // per cell: (stamp of the evaluation << 32) | largest member hsml as
   // FP32 bits, rounded up
   unsigned long long *cellh;
   int *rebuild;          // 1: the cell lists are rebuilt in this evaluation
-  float4 *pf;            // FP32 (x, y, hsml) in list order, x and y relative to
-                         // the grid origin: the candidate prefilter
+  float4 *pf;            // FP32 (x, y, hsml) by slot, x and y relative to the
+                         // grid origin: the candidate prefilter
   // neighbour list of the evaluation, built by pass A in the reference's
   // pair order for each particle: nbr[j * ntot + s] = list position of the
   // j-th neighbour of position s; nnb[s] = count (> kNbrMax: not stored,
@@ -111,13 +122,13 @@ struct DevState {
   int2 *gpre;            // per list position: pair prefixes (intra, cross) of k_ranks
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
   // summation density (den_alg = 1): kernel_zero of each particle (by
-  // packed index; persists across evaluations like the reference's module
-  // array), and per list position the numerator and the density that
-  // pass A's sums give for the state it evaluated
+  // p_off + reference index; persists across evaluations like the
+  // reference's module array), and per slot the numerator and the density
+  // that pass A's sums give for the state it evaluated
   double *kz, *dnum, *dsum;
-  double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N), optional
-  double *sig_out;       // sigma_11, sigma_12, sigma_22 (3 x N), with diag
-  double *tend;          // dxdt, dydt, dvxdt, dvydt, dhdt, dAdt (6 x N), optional
+  double *diag;          // eps11, eps12, eps22, zeta, eta, p (6 x N by slot), optional
+  double *sig_out;       // sigma_11, sigma_12, sigma_22 (3 x N by slot), with diag
+  double *tend;          // dxdt, dydt, dvxdt, dvydt, dhdt, dAdt (6 x N by slot), optional
   // boundary
   double *xb, *yb, *massb;
   int *bpre, *bbase, *bcount;

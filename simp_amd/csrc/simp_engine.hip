@@ -135,22 +135,14 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   }
   rc |= dalloc(list, &d.mass, N);
   rc |= dalloc(list, &d.orig, N);
-  rc |= dalloc(list, &d.inst, N);
   rc |= dalloc(list, &d.cellid, N);
   rc |= dalloc(list, &d.slot, N);
   rc |= dalloc(list, &d.cl, N);
-  rc |= dalloc(list, &d.clpos, N);
+  rc |= dalloc(list, &d.perm_d, (size_t)kPermDoubles * N);
+  rc |= dalloc(list, &d.perm_i, (size_t)kPermInts * N);
   rc |= dalloc(list, &d.own_cnt, N);
   rc |= dalloc(list, &d.moff, N);
   rc |= dalloc(list, &d.gpre, N);
-  rc |= dalloc(list, &d.xs, N);
-  rc |= dalloc(list, &d.ys, N);
-  rc |= dalloc(list, &d.hs, N);
-  rc |= dalloc(list, &d.vxs, N);
-  rc |= dalloc(list, &d.vys, N);
-  rc |= dalloc(list, &d.ms, N);
-  rc |= dalloc(list, &d.rs, N);
-  rc |= dalloc(list, &d.cells, N);
   rc |= dalloc(list, &d.pf, N);
   rc |= dalloc(list, &d.nbr, (size_t)kNbrMax * N);
   rc |= dalloc(list, &d.nnb, N);
@@ -189,7 +181,6 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.meta, std::max<size_t>(INST, 1));
   rc |= dalloc(list, &d.dyn, std::max<size_t>(INST, 1));
   rc |= dalloc(list, &d.blk, std::max<size_t>(BLK, 1));
-  rc |= dalloc(list, &d.lbase, std::max<size_t>(BLK, 1));
   rc |= dalloc(list, scan_tmp, (size_t)cdiv((long)C + 1, kScanItems) + 1);
   return rc ? -6 : 0;
 }
@@ -270,7 +261,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     const long cells = cell_capacity(in);
     if (cells < 0) return -4;
     if (cells > (1L << 28)) return -3;
-    m.c_cap = (int)cells;
+    m.c_cap = (int)cells + 1;   // + the frozen-instance cell
     m.q_cap = store_capacity(in);
     if (j == 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size())
       m.q_cap = std::max(m.q_cap, meta_[carry_store_from].q_cap);
@@ -441,7 +432,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   std::vector<BlockEntry> lg;
   for (int k = 0; k < ninst; ++k)
     for (int f = 0; f < nm[k].n; f += kTPB)
-      lg.push_back(BlockEntry{k, f, (int)lg.size(), std::min(nm[k].n - f, kTPB)});
+      lg.push_back(BlockEntry{k, f, (int)lg.size(), std::min(nm[k].n - f, kTPB),
+                              nm[k].p_off + f, {0, 0, 0}});
   std::vector<BlockEntry> phys(lg.size());
   {
     const int nb_ = (int)lg.size(), q = nb_ / 8, r = nb_ % 8;
@@ -513,7 +505,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_); });
   }
   // boundary-pair ranks and the head of the pair ranks the boundary kernel reads
-  if (any_boundary_) L("ranks", [&] { launch_ranks(d, st_); });
+  if (any_boundary_) L("ranks", [&] { launch_ranks(d, buf, st_); });
   L("passB", [&] { launch_passB(d, precision_, any_sumd_, phase, st_); });
 }
 
@@ -603,13 +595,33 @@ int Engine::download_all_status(std::vector<InstDyn> &out) {
   return 0;
 }
 
-static void interleave(const std::vector<double> &a, const std::vector<double> &b, double *out,
-                       int n) {
-  for (int i = 0; i < n; ++i) {
-    out[2 * i] = a[i];
-    out[2 * i + 1] = b[i];
+// Per-slot arrays come back in reference order through the reference index
+// of each slot (orig).
+struct SlotOrder {
+  std::vector<int> ref;
+  int fetch(const int *orig, int off, int n) {
+    ref.resize(n);
+    return hipMemcpy(ref.data(), orig + off, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess
+               ? 0 : -2;
   }
-}
+  void put(const std::vector<double> &v, double *out) const {
+    for (size_t i = 0; i < ref.size(); ++i) out[ref[i]] = v[i];
+  }
+  void put_pairs(const std::vector<double> &a, const std::vector<double> &b, double *out) const {
+    for (size_t i = 0; i < ref.size(); ++i) {
+      out[2 * ref[i]] = a[i];
+      out[2 * ref[i] + 1] = b[i];
+    }
+  }
+  void put_triples(const std::vector<double> &a, const std::vector<double> &b,
+                   const std::vector<double> &c, double *out) const {
+    for (size_t i = 0; i < ref.size(); ++i) {
+      out[3 * ref[i]] = a[i];
+      out[3 * ref[i] + 1] = b[i];
+      out[3 * ref[i] + 2] = c[i];
+    }
+  }
+};
 
 int Engine::download_state(int k, double *x, double *vx, double *rho, double *hsml, double *hth,
                            double *A) {
@@ -617,48 +629,62 @@ int Engine::download_state(int k, double *x, double *vx, double *rho, double *hs
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n;
+  SlotOrder so;
+  if (so.fetch(d_.orig, m.p_off, n)) return -2;
   std::vector<double> a(n), b(n);
-  auto get = [&](double *dst, const double *src) {
-    return hipMemcpy(dst, src + m.p_off, (size_t)n * 8, hipMemcpyDeviceToHost);
+  auto get = [&](std::vector<double> &dst, const double *src) {
+    return hipMemcpy(dst.data(), src + m.p_off, (size_t)n * 8, hipMemcpyDeviceToHost);
   };
   if (x) {
-    HIPCHK(get(a.data(), d_.x[0]));
-    HIPCHK(get(b.data(), d_.y[0]));
-    interleave(a, b, x, n);
+    HIPCHK(get(a, d_.x[0]));
+    HIPCHK(get(b, d_.y[0]));
+    so.put_pairs(a, b, x);
   }
   if (vx) {
-    HIPCHK(get(a.data(), d_.vx[0]));
-    HIPCHK(get(b.data(), d_.vy[0]));
-    interleave(a, b, vx, n);
+    HIPCHK(get(a, d_.vx[0]));
+    HIPCHK(get(b, d_.vy[0]));
+    so.put_pairs(a, b, vx);
   }
-  if (rho) HIPCHK(get(rho, d_.rho[0]));
-  if (hsml) HIPCHK(get(hsml, d_.hsml[0]));
-  if (hth) HIPCHK(get(hth, d_.h[0]));
-  if (A) HIPCHK(get(A, d_.A[0]));
+  const std::pair<double *, const double *> one[] = {
+      {rho, d_.rho[0]}, {hsml, d_.hsml[0]}, {hth, d_.h[0]}, {A, d_.A[0]}};
+  for (const auto &f : one) {
+    if (!f.first) continue;
+    HIPCHK(get(a, f.second));
+    so.put(a, f.first);
+  }
   return 0;
 }
 
 int Engine::download_tend(int k, double *dxdt, double *dvxdt, double *dhdt, double *dAdt) {
   if (k < 0 || k >= d_.ninst) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n, N = d_.ntot;
+  SlotOrder so;
+  if (so.fetch(d_.orig, m.p_off, n)) return -2;
   std::vector<double> a(n), b(n);
-  auto get = [&](double *dst, int comp) {
-    return hipMemcpy(dst, d_.tend + (size_t)comp * N + m.p_off, (size_t)n * 8,
+  auto get = [&](std::vector<double> &dst, int comp) {
+    return hipMemcpy(dst.data(), d_.tend + (size_t)comp * N + m.p_off, (size_t)n * 8,
                      hipMemcpyDeviceToHost);
   };
   if (dxdt) {
-    HIPCHK(get(a.data(), 0));
-    HIPCHK(get(b.data(), 1));
-    interleave(a, b, dxdt, n);
+    HIPCHK(get(a, 0));
+    HIPCHK(get(b, 1));
+    so.put_pairs(a, b, dxdt);
   }
   if (dvxdt) {
-    HIPCHK(get(a.data(), 2));
-    HIPCHK(get(b.data(), 3));
-    interleave(a, b, dvxdt, n);
+    HIPCHK(get(a, 2));
+    HIPCHK(get(b, 3));
+    so.put_pairs(a, b, dvxdt);
   }
-  if (dhdt) HIPCHK(get(dhdt, 4));
-  if (dAdt) HIPCHK(get(dAdt, 5));
+  if (dhdt) {
+    HIPCHK(get(a, 4));
+    so.put(a, dhdt);
+  }
+  if (dAdt) {
+    HIPCHK(get(a, 5));
+    so.put(a, dAdt);
+  }
   return 0;
 }
 
@@ -668,30 +694,30 @@ int Engine::download_diag(int k, double *eps3, double *zeta, double *eta, double
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n, N = d_.ntot;
+  SlotOrder so;
+  if (so.fetch(d_.orig, m.p_off, n)) return -2;
   std::vector<double> c0(n), c1(n), c2(n);
-  auto get = [&](double *dst, int comp) {
-    return hipMemcpy(dst, d_.diag + (size_t)comp * N + m.p_off, (size_t)n * 8,
+  auto get = [&](std::vector<double> &dst, const double *src, int comp) {
+    return hipMemcpy(dst.data(), src + (size_t)comp * N + m.p_off, (size_t)n * 8,
                      hipMemcpyDeviceToHost);
   };
   if (eps3) {
-    HIPCHK(get(c0.data(), 0));
-    HIPCHK(get(c1.data(), 1));
-    HIPCHK(get(c2.data(), 2));
-    for (int i = 0; i < n; ++i) {
-      eps3[3 * i] = c0[i]; eps3[3 * i + 1] = c1[i]; eps3[3 * i + 2] = c2[i];
-    }
+    HIPCHK(get(c0, d_.diag, 0));
+    HIPCHK(get(c1, d_.diag, 1));
+    HIPCHK(get(c2, d_.diag, 2));
+    so.put_triples(c0, c1, c2, eps3);
   }
-  if (zeta) HIPCHK(get(zeta, 3));
-  if (eta) HIPCHK(get(eta, 4));
-  if (p) HIPCHK(get(p, 5));
+  const std::pair<double *, int> one[] = {{zeta, 3}, {eta, 4}, {p, 5}};
+  for (const auto &f : one) {
+    if (!f.first) continue;
+    HIPCHK(get(c0, d_.diag, f.second));
+    so.put(c0, f.first);
+  }
   if (sig3) {
-    HIPCHK(hipMemcpy(c0.data(), d_.sig_out + m.p_off, n * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(c1.data(), d_.sig_out + (size_t)N + m.p_off, n * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(c2.data(), d_.sig_out + 2 * (size_t)N + m.p_off, n * 8,
-                     hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) {
-      sig3[3 * i] = c0[i]; sig3[3 * i + 1] = c1[i]; sig3[3 * i + 2] = c2[i];
-    }
+    HIPCHK(get(c0, d_.sig_out, 0));
+    HIPCHK(get(c1, d_.sig_out, 1));
+    HIPCHK(get(c2, d_.sig_out, 2));
+    so.put_triples(c0, c1, c2, sig3);
   }
   return 0;
 }
@@ -710,7 +736,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   const DevState &d = d_;
   enqueue_prefix(0);
   launch_passA(d, 0, precision_, any_sumd_, 0, st_);
-  if (any_boundary_) launch_ranks(d, st_);
+  if (any_boundary_) launch_ranks(d, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);
   int rc = sync_and_check();

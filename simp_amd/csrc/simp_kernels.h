@@ -17,7 +17,7 @@ void launch_cellsort(const DevState &d, int buf, hipStream_t st);
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);
-void launch_ranks(const DevState &d, hipStream_t st);
+void launch_ranks(const DevState &d, int buf, hipStream_t st);
 void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st);

@@ -94,11 +94,10 @@ constexpr float kC6f = (float)(78.0 / (7.0 * kPi));
 // The padding blocks (id >= the real count) exit at once.
 __device__ __forceinline__ bool particle_of(const DevState &d, int &p, int &k) {
   if (blockIdx.x >= (unsigned)d.nblk) return false;
-  BlockEntry e = d.blk[blockIdx.x];
+  const BlockEntry e = d.blk[blockIdx.x];
   k = e.inst;
-  int local = e.first + (int)threadIdx.x;
-  if (local >= d.meta[k].n) return false;
-  p = d.meta[k].p_off + local;
+  if ((int)threadIdx.x >= e.nloc) return false;
+  p = e.pos + (int)threadIdx.x;
   return true;
 }
 
@@ -375,7 +374,7 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
   const int xlo = nint(xmn / bin), xhi = nint(xmx / bin);
   const int ylo = nint(ymn / bin), yhi = nint(ymx / bin);
   const long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
-  if (!(hmn > 0.0) || W * H > (long long)m.c_cap || W * H <= 0) {
+  if (!(hmn > 0.0) || W * H > (long long)m.c_cap - 1 || W * H <= 0) {
     y.err |= kErrCapacity;
     return 0;
   }
@@ -395,33 +394,65 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
 
 // ------------------------------------------------------------------ count
 
-// Cell key of every particle (bucket_search, nnps.f90:249-250).  The cell
-// lists are rebuilt only when some key differs from the one the current
-// lists were built with (or on the first evaluation after an upload); the
+// Cell key of every slot (bucket_search, nnps.f90:249-250), the FP32
+// prefilter positions of the evaluation, and per cell the largest member
+// smoothing length (rounded up to FP32), the bound the gathers' cell
+// culling uses: a max over each run of equal keys in the wavefront (slots
+// are in cell order, but for particles that just changed cell), then one
+// 64-bit atomic max per run of (evaluation stamp, value) -- a stale stamp
+// reads as an empty cell, so nothing needs clearing.
+// The cell lists are rebuilt only when some key differs from the one its
+// slot was sorted with (or on the first evaluation after an upload); the
 // rebuild kernels below return at once otherwise.  Between the predictor
 // and the corrector evaluation of a step the grid and the keys normally
 // stay the same, so the corrector reuses the lists.
 __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   int p = 0, k = 0;
   bool changed = false;
-  if (particle_of(d, p, k) && !inst_off(d.dyn[k])) {
+  const int lane = (int)threadIdx.x & 63;
+  int key = -1 - lane;
+  unsigned long long hv = 0;
+  const bool valid = particle_of(d, p, k) && !inst_off(d.dyn[k]);
+  if (valid) {
     const InstDyn &y = d.dyn[k];
     const double bin = y.bin;
-    const int cx = nint(d.x[buf][p] / bin) - y.xlo + 1;
-    const int cy = nint(d.y[buf][p] / bin) - y.ylo + 1;
+    const double xq = d.x[buf][p], yq = d.y[buf][p], hq = d.hsml[buf][p];
+    const int cx = nint(xq / bin) - y.xlo + 1;
+    const int cy = nint(yq / bin) - y.ylo + 1;
     const int c = cy * y.W + cx;
     changed = d.cellid[p] != c;
     d.cellid[p] = c;
+    const double ox = (y.xlo - 1) * bin, oy = (y.ylo - 1) * bin;
+    d.pf[p] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
+    key = c;
+    hv = ((unsigned long long)(unsigned)y.stamp << 32) |
+         __float_as_uint((float)hq * 1.000001f);
   }
-  if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(d.rebuild, 1);
+  if (__any(changed) && lane == 0) atomicOr(d.rebuild, 1);
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned lo = __shfl_down((unsigned)hv, o, 64);
+    const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
+    const int ok = __shfl_down(key, o, 64);
+    const unsigned long long ov = ((unsigned long long)hi << 32) | lo;
+    if (lane + o < 64 && ok == key && ov > hv) hv = ov;
+  }
+  const int prev = __shfl_up(key, 1, 64);
+  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[d.meta[k].c_off + key], hv);
 }
 
 // Histogram of the cell keys (slot of each particle in its cell).
+// A frozen instance keeps its slots where they are: all of them counted in
+// its last cell, so every instance's lists start at its p_off.
 __global__ void __launch_bounds__(kTPB) k_hist(DevState d) {
   if (*d.rebuild == 0) return;
   int p = 0, k = 0;
-  if (!particle_of(d, p, k) || inst_off(d.dyn[k])) return;
-  d.slot[p] = atomicAdd(&d.count[d.meta[k].c_off + d.cellid[p]], 1);
+  if (!particle_of(d, p, k)) return;
+  const InstMeta &m = d.meta[k];
+  if (inst_off(d.dyn[k])) {
+    if (p == m.p_off) d.count[m.c_off + m.c_cap - 1] = m.n;
+    return;
+  }
+  d.slot[p] = atomicAdd(&d.count[m.c_off + d.cellid[p]], 1);
 }
 
 // ------------------------------------------------------------------- scan
@@ -474,11 +505,16 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 
 // ------------------------------------------------------- scatter/cellsort
 
+// cl[new slot] = old slot; a frozen instance's last cell is marked unfilled
+// (never sorted, never moved)
 __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
   if (*d.rebuild == 0) return;
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
-  if (inst_off(d.dyn[k])) return;
+  if (inst_off(d.dyn[k])) {
+    d.cl[p] = -1;
+    return;
+  }
   d.cl[d.cell_start[d.meta[k].c_off + d.cellid[p]] + d.slot[p]] = p;
 }
 
@@ -496,7 +532,7 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   const int n = s1 - s0;
   if (n <= 1) return;
   for (int s = s0; s < s1; ++s)
-    if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: skipped instance
+    if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: frozen instance
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
 #pragma unroll
@@ -529,57 +565,51 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   }
 }
 
-// The right-hand-side state copied into cell-list order for the gathers,
-// one thread per list position.  Also the FP32 positions relative to the
-// instance's grid origin used by the candidate prefilter.
-// Also the largest smoothing length of each cell (rounded up to FP32), the
-// bound the gathers' cell culling uses: a max over each run of equal cells
-// in the wavefront (list positions are in cell order), then one 64-bit
-// atomic max per run of (evaluation stamp, value) -- a stale stamp reads as
-// an empty cell, so nothing needs clearing.
-__global__ void __launch_bounds__(kTPB) k_listcopy(DevState d, int buf) {
-  if (blockIdx.x >= (unsigned)d.nblk) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // its readers have run
-  const BlockEntry e = d.blk[blockIdx.x];
-  const int k = e.inst;
-  const InstMeta &m = d.meta[k];
+// The state moves to the new slot order of a rebuild: each slot gathers
+// the particle the sorted lists put there (cl[slot] = its old slot) into
+// perm_d / perm_i, with the prefilter positions of the evaluation (buffer
+// buf); k_unpermute copies it back.
+__device__ __forceinline__ void perm_arrays(const DevState &d, double *(&a)[kPermDoubles]) {
+  for (int b = 0; b < 2; ++b) {
+    a[8 * b + 0] = d.x[b]; a[8 * b + 1] = d.y[b];
+    a[8 * b + 2] = d.vx[b]; a[8 * b + 3] = d.vy[b];
+    a[8 * b + 4] = d.h[b]; a[8 * b + 5] = d.A[b];
+    a[8 * b + 6] = d.rho[b]; a[8 * b + 7] = d.hsml[b];
+  }
+  a[16] = d.mass;
+}
+
+__global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
+  if (*d.rebuild == 0) return;
+  int s = 0, k = 0;
+  if (!particle_of(d, s, k)) return;
   const InstDyn &y = d.dyn[k];
-  if (threadIdx.x == 0) d.lbase[blockIdx.x] = d.cell_start[m.c_off] + e.first;
-  if (inst_off(y)) return;                  // whole block
-  const int local = e.first + (int)threadIdx.x;
-  const bool valid = local < m.n;
-  const int lane = (int)threadIdx.x & 63;
-  int key = -1 - lane;
-  unsigned long long hv = 0;
-  if (valid) {
-    const int s = d.cell_start[m.c_off] + local;
-    const int q = d.cl[s];
-    const int c = d.cellid[q];
-    d.clpos[q] = s;
-    d.cells[s] = c;
-    const double xq = d.x[buf][q], yq = d.y[buf][q], hq = d.hsml[buf][q];
-    d.xs[s] = xq;
-    d.ys[s] = yq;
-    d.hs[s] = hq;
-    d.vxs[s] = d.vx[buf][q];
-    d.vys[s] = d.vy[buf][q];
-    d.ms[s] = d.mass[q];
-    d.rs[s] = d.rho[buf][q];
-    const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
-    d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
-    key = c;
-    hv = ((unsigned long long)(unsigned)y.stamp << 32) |
-         __float_as_uint((float)hq * 1.000001f);
-  }
-  for (int o = 1; o < 64; o <<= 1) {
-    const unsigned lo = __shfl_down((unsigned)hv, o, 64);
-    const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
-    const int ok = __shfl_down(key, o, 64);
-    const unsigned long long ov = ((unsigned long long)hi << 32) | lo;
-    if (lane + o < 64 && ok == key && ov > hv) hv = ov;
-  }
-  const int prev = __shfl_up(key, 1, 64);
-  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[m.c_off + key], hv);
+  if (inst_off(y)) return;
+  const size_t N = (size_t)d.ntot;
+  const int q = d.cl[s];
+  double *a[kPermDoubles];
+  perm_arrays(d, a);
+#pragma unroll
+  for (int j = 0; j < kPermDoubles; ++j) d.perm_d[j * N + s] = a[j][q];
+  d.perm_i[s] = d.orig[q];
+  d.perm_i[N + s] = d.cellid[q];
+  const double xq = d.x[buf][q], yq = d.y[buf][q], hq = d.hsml[buf][q];
+  const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
+  d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
+}
+
+__global__ void __launch_bounds__(kTPB) k_unpermute(DevState d) {
+  if (*d.rebuild == 0) return;
+  int s = 0, k = 0;
+  if (!particle_of(d, s, k)) return;
+  if (inst_off(d.dyn[k])) return;
+  const size_t N = (size_t)d.ntot;
+  double *a[kPermDoubles];
+  perm_arrays(d, a);
+#pragma unroll
+  for (int j = 0; j < kPermDoubles; ++j) a[j][s] = d.perm_d[j * N + s];
+  d.orig[s] = d.perm_i[s];
+  d.cellid[s] = d.perm_i[N + s];
 }
 
 // -------------------------------------------------------------- boundary
@@ -684,19 +714,6 @@ struct KSum {
 
 // ---------------------------------------------------------------- pass A
 
-// The gathers run one thread per cell-list position: the threads of one
-// cell walk the same candidate ranges, so a wavefront's loads coalesce.
-// (The block's first list position comes from lbase, read in parallel with
-// the block table entry.)
-__device__ __forceinline__ bool list_slot(const DevState &d, int &s, int &k) {
-  if (blockIdx.x >= (unsigned)d.nblk) return false;
-  const BlockEntry e = d.blk[blockIdx.x];
-  k = e.inst;
-  if ((int)threadIdx.x >= e.nloc) return false;
-  s = d.lbase[blockIdx.x] + (int)threadIdx.x;
-  return true;
-}
-
 // LDS tile of the block's candidate region (FP32 prefilter data)
 constexpr int kTile = 1280;
 
@@ -707,6 +724,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   __shared__ float4 tile[kTile];
   __shared__ unsigned short lst[kNbrMax * kTPB];
   if (blockIdx.x >= (unsigned)d.nblk) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // the rebuild kernels have run
   const BlockEntry be = d.blk[blockIdx.x];
   const int k = be.inst;
   const InstDyn &y = d.dyn[k];
@@ -714,7 +732,10 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const InstMeta &m = d.meta[k];
   const int W = y.W;
   const int *cs0 = d.cell_start + m.c_off;
-  const int pos0 = d.lbase[blockIdx.x];   // first list position of the block
+  // the evaluated state, by slot (= list position)
+  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RS = d.rho[buf];
+  const double *VX = d.vx[buf], *VY = d.vy[buf], *MS = d.mass;
+  const int pos0 = be.pos;                // first slot of the block
   const int nloc = be.nloc;
   const bool valid = (int)threadIdx.x < nloc;
   const int self = pos0 + min((int)threadIdx.x, nloc - 1);
@@ -726,7 +747,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // takes the exact rescan path instead.
   int As, Ae, Bs, Be, Cs, Ce;
   {
-    const int cf = d.cells[pos0], cl = d.cells[pos0 + nloc - 1];
+    const int cf = d.cellid[pos0], cl = d.cellid[pos0 + nloc - 1];
     As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
     Bs = cs0[cf - 1];     Be = cs0[cl + 2];
     Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
@@ -746,15 +767,15 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   if (!valid) return;
   // waves with large supports (more neighbours) get issue priority: they
   // set the kernel's length
-  if (__any(d.hs[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
-  const int c = d.cells[self];
+  if (__any(HS[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
+  const int c = d.cellid[self];
   const int cy = c / W, cx = c - cy * W;
-  const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
-  const double vxi = d.vxs[self], vyi = d.vys[self];
-  const double rhoi = SUMD ? d.rs[self] : 0.0;
+  const double xi = X[self], yi = Y[self], hi = HS[self];
+  const double vxi = VX[self], vyi = VY[self];
+  const double rhoi = SUMD ? RS[self] : 0.0;
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
-  const int p = d.cl[self];
+  const int p = self;
   // summation density (density.f90:41-100) on this evaluation's kernel
   // values, accumulated in the same per-particle order as the reference.
   // FP64 in both precision modes: the density sets the next smoothing
@@ -881,14 +902,14 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // walk finds it.  It must be known before the density sums start.
   double kzi = 0.;
   if (sumd) {
-    kzi = d.kz[p];
+    kzi = d.kz[m.p_off + d.orig[self]];
     bool found = false;
     if (nc <= kNbrMax) {
       for (int j = nc - 1; j >= 0 && !found; --j) {
         const int t = lpos((int)lst[j * kTPB + threadIdx.x]);
         if (pair_kind(R, t, self) == 0) break;
-        const double dx = xi - d.xs[t], dy = yi - d.ys[t];
-        const double mh = (hi + d.hs[t]) / 2.;
+        const double dx = xi - X[t], dy = yi - Y[t];
+        const double mh = (hi + HS[t]) / 2.;
         double r;
         if (in_support(dx * dx + dy * dy, mh, r)) {
           kzi = c6_factor(mh);
@@ -899,8 +920,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       for (int g = 5; g >= 2 && !found; --g) {
         for (int t = R.b[g] - 1; t >= R.a[g] && !found; --t) {
           if (g == 2 && t <= self) break;
-          const double dx = xi - d.xs[t], dy = yi - d.ys[t];
-          const double mh = (hi + d.hs[t]) / 2.;
+          const double dx = xi - X[t], dy = yi - Y[t];
+          const double mh = (hi + HS[t]) / 2.;
           double r;
           if (in_support(dx * dx + dy * dy, mh, r)) {
             kzi = c6_factor(mh);
@@ -909,7 +930,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         }
       }
     }
-    const double mi = d.ms[self];
+    const double mi = MS[self];
     num = kzi * mi;
     den = kzi * mi / rhoi;
   }
@@ -923,13 +944,13 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       double xq[kBatch], yq[kBatch], hq[kBatch], mq[kBatch], rq[kBatch], uq[kBatch], wq[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        xq[u] = d.xs[sj[u]];
-        yq[u] = d.ys[sj[u]];
-        hq[u] = d.hs[sj[u]];
-        mq[u] = d.ms[sj[u]];
-        rq[u] = d.rs[sj[u]];
-        uq[u] = d.vxs[sj[u]];
-        wq[u] = d.vys[sj[u]];
+        xq[u] = X[sj[u]];
+        yq[u] = Y[sj[u]];
+        hq[u] = HS[sj[u]];
+        mq[u] = MS[sj[u]];
+        rq[u] = RS[sj[u]];
+        uq[u] = VX[sj[u]];
+        wq[u] = VY[sj[u]];
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
@@ -953,15 +974,15 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     for (int g = 0; g < 6; ++g) {
       for (int s = R.a[g]; s < R.b[g]; ++s) {
         if (s == self) continue;
-        const double dx = xi - d.xs[s], dy = yi - d.ys[s];
-        const double mh = (hi + d.hs[s]) / 2.;
+        const double dx = xi - X[s], dy = yi - Y[s];
+        const double mh = (hi + HS[s]) / 2.;
         double r;
         if (!in_support(dx * dx + dy * dy, mh, r)) continue;
         const int kind = pair_kind(R, s, self);
         nI += kind == 1;
         nX += kind == 2;
         ++ntrue;
-        pair(dx, dy, r, mh, d.ms[s], d.rs[s], d.vxs[s], d.vys[s]);
+        pair(dx, dy, r, mh, MS[s], RS[s], VX[s], VY[s]);
       }
     }
     nn = ntrue;
@@ -974,7 +995,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
   }
   if (sumd) {
-    d.kz[p] = kzi;
+    d.kz[m.p_off + d.orig[self]] = kzi;
     d.dnum[self] = num;
     d.dsum[self] = num / den;
   }
@@ -1149,7 +1170,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
         // the last position whose prefix is below lim: its cell is the
         // last one needed
         if (gI + gX < lim && gI + gX + vI[u] + vX[u] >= lim)
-          sh_stop = cs[d.cells[p0 + t] + 1] - p0;
+          sh_stop = cs[d.cellid[p0 + t] + 1] - p0;
       }
       gI += vI[u];
       gX += vX[u];
@@ -1169,9 +1190,9 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
 
 // 2) one thread per list position of the cells k_ranks found needed: the
 //    owned pairs that rank below lim get their hsml_ij
-__global__ void __launch_bounds__(kTPB) k_rankput(DevState d) {
+__global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   int s = 0, k = 0;
-  if (!list_slot(d, s, k)) return;
+  if (!particle_of(d, s, k)) return;
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || inst_off(y)) return;
@@ -1182,7 +1203,7 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d) {
   const int2 *G = d.gpre + p0;
   const size_t N = (size_t)d.ntot;
   const int W = y.W;
-  const int c = d.cells[s];
+  const int c = d.cellid[s];
   const int f = cs[c] - p0, e = cs[c + 1] - p0;
   const int2 Gf = G[f], Gt = G[t];
   const int2 Ge = e < np ? G[e] : make_int2(y.gendI, y.gendX);
@@ -1193,26 +1214,26 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d) {
   int rI = base + (Gt.x - Gf.x);
   int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
   if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
-  const double hi = d.hs[s];
+  const double hi = d.hsml[buf][s];
   const int nn = d.nnb[s];
   if (nn <= kNbrMax) {
     // the owned pairs are the tail of the list: nI intra, then nX cross
     for (int j = nn - nI - nX; j < nn; ++j) {
       const int q = d.nbr[j * N + s];
       int &rank = (j < nn - nX) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hs[q]) / 2.;
+      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hsml[buf][q]) / 2.;
       ++rank;
     }
     return;
   }
   const int cy = c / W, cx = c - cy * W;
-  const double xi = d.xs[s], yi = d.ys[s];
+  const double xi = d.x[buf][s], yi = d.y[buf][s];
   const Ranges R = neighbour_ranges(cs, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int q = R.a[g]; q < R.b[g]; ++q) {
       if (g == 2 && q <= s) continue;
-      const double dx = xi - d.xs[q], dy = yi - d.ys[q];
-      const double mh = (hi + d.hs[q]) / 2.;
+      const double dx = xi - d.x[buf][q], dy = yi - d.y[buf][q];
+      const double mh = (hi + d.hsml[buf][q]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
       int &rank = (g == 2) ? rI : rX;
@@ -1254,7 +1275,7 @@ template <bool MIXED, bool SUMD, int phase>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
   int self = 0, k = 0;
-  const bool ok = list_slot(d, self, k);
+  const bool ok = particle_of(d, self, k);
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   const bool live = ok && !inst_off(y);
@@ -1276,11 +1297,14 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // entries past the count are never dereferenced); (2) everything the
     // particle id, the cell and those entries index.  A wave so waits for
     // two memory round trips before its first neighbour arithmetic.
-    const int c = d.cells[self];
-    const int p = d.cl[self];
-    const double xi = d.xs[self], yi = d.ys[self], hi = d.hs[self];
-    const double rhoi = d.rs[self], mi = d.ms[self];
-    const double vxi = d.vxs[self], vyi = d.vys[self];
+    // the evaluated state, by slot (= list position)
+    const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RS = d.rho[buf];
+    const double *VX = d.vx[buf], *VY = d.vy[buf], *MS = d.mass;
+    const int c = d.cellid[self];
+    const int p = self;
+    const double xi = X[self], yi = Y[self], hi = HS[self];
+    const double rhoi = RS[self], mi = MS[self];
+    const double vxi = VX[self], vyi = VY[self];
     const int nn = d.nnb[self];
     const double divi = ((const double *)d.div)[self];
     const double s11d = (double)S11[self], s12d = (double)S12[self], s22d = (double)S22[self];
@@ -1301,7 +1325,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       x1 = d.x[0][p]; y1 = d.y[0][p];
       h1 = d.h[0][p]; A1 = d.A[0][p];
     }
-    if (sumd3) kzi = d.kz[p];
+    if (sumd3) kzi = d.kz[m.p_off + d.orig[self]];
 
     // Everything of the update but the velocities is known before the
     // stress sums (single_step.f90:104-106 tendencies; predictor
@@ -1403,8 +1427,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
           const int t = sj[u];
-          xq[u] = d.xs[t]; yq[u] = d.ys[t]; hq[u] = d.hs[t];
-          rq[u] = d.rs[t]; mq[u] = d.ms[t];
+          xq[u] = X[t]; yq[u] = Y[t]; hq[u] = HS[t];
+          rq[u] = RS[t]; mq[u] = MS[t];
           q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
           r2[u] = sumd3 ? d.dsum[t] : 1.0;
         }
@@ -1432,12 +1456,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       for (int g = 0; g < 6; ++g) {
         for (int s = R.a[g]; s < R.b[g]; ++s) {
           if (s == self) continue;
-          const double dx = xi - d.xs[s], dy = yi - d.ys[s];
-          const double mh = (hi + d.hs[s]) / 2.;
+          const double dx = xi - X[s], dy = yi - Y[s];
+          const double mh = (hi + HS[s]) / 2.;
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-          stress_pair(dx, dy, r, mh, d.rs[s], d.ms[s], S11[s], S12[s], S22[s]);
-          if (sumd3) dens_pair(dx, dy, mh, d.ms[s], d.dsum[s]);
+          stress_pair(dx, dy, r, mh, RS[s], MS[s], S11[s], S12[s], S22[s]);
+          if (sumd3) dens_pair(dx, dy, mh, MS[s], d.dsum[s]);
         }
       }
     }
@@ -1466,7 +1490,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         ++nbn;
         int before = 0;
         for (int s = own0; s < self; ++s) {
-          const double ex = d.xb[bg] - d.xs[s], ey = d.yb[bg] - d.ys[s];
+          const double ex = d.xb[bg] - X[s], ey = d.yb[bg] - Y[s];
           if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
         }
         const int rank = d.bbase[bg] + d.bpre[bg * 9 + slot9] + before;
@@ -1546,7 +1570,7 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (inst_off(y)) return;
-  const int s = d.clpos[p];
+  const int s = p;
   const int oc = d.own_cnt[s];
   if ((oc & 0xfffff) == 0) return;
   const int c = d.cellid[p];
@@ -1555,16 +1579,16 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   int rI = base + (d.moff[s] & 0xffff);
   int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
   const int W = y.W, cy = c / W, cx = c - cy * W;
-  const double xi = d.xs[s], yi = d.ys[s], hi = d.hs[s];
+  const double xi = d.x[buf][s], yi = d.y[buf][s], hi = d.hsml[buf][s];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int t = R.a[g]; t < R.b[g]; ++t) {
       if (g == 2 && t <= s) continue;
-      const double dx = xi - d.xs[t], dy = yi - d.ys[t];
-      const double mh = (hi + d.hs[t]) / 2.;
+      const double dx = xi - d.x[buf][t], dy = yi - d.y[buf][t];
+      const double mh = (hi + d.hsml[buf][t]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-      const int q = d.cl[t];
+      const int q = t;
       int &rank = (g == 2) ? rI : rX;
       if (rank < cap) {
         pi[rank] = d.orig[p] + 1;
@@ -1597,8 +1621,8 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
   for (int s9 = 0; s9 < 9; ++s9) {
     int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
     for (int s = cs[c]; s < cs[c + 1]; ++s) {
-      int q = d.cl[s];
-      double dx = bx - d.xs[s], dy = by - d.ys[s];
+      const int q = s;
+      double dx = bx - d.x[buf][s], dy = by - d.y[buf][s];
       double dd = sqrt(dx * dx + dy * dy);
       if (dd <= kBLength) {
         if (rank < bcap) {
@@ -1640,7 +1664,8 @@ void launch_scatter(const DevState &d, hipStream_t st) {
 }
 void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
   k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
-  k_listcopy<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  k_unpermute<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st) {
@@ -1656,9 +1681,9 @@ void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag
 void launch_pairtot(const DevState &d, hipStream_t st) {
   k_pairtot<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d);
 }
-void launch_ranks(const DevState &d, hipStream_t st) {
+void launch_ranks(const DevState &d, int buf, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
-  k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d);
+  k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 template <bool MIXED, bool SUMD>
 static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
