@@ -4,9 +4,10 @@
 // contiguous particle range, boundary range, cell range and boundary-pair
 // range, described by InstMeta (fixed between uploads) and InstDyn (changes
 // every evaluation).  A particle's slot in its instance's range is its
-// position in the cell lists: whenever the lists are rebuilt the particle
-// state moves with them, so the gathers read the state itself, in list
-// order, and orig[] keeps each slot's reference particle index.
+// position in the cell lists: whenever an instance's lists are rebuilt its
+// particle state moves with them (from one of two state sets to the
+// other), so the gathers read the state itself, in list order, and orig[]
+// keeps each slot's reference particle index.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,10 +37,6 @@ constexpr int kTPB = 256;           // threads per block of particle kernels
 constexpr int kScanItems = 2048;    // elements per block of the scan
 constexpr int kMaxBoundaryPerInst = 8192;
 constexpr int kNbrMax = 32;         // neighbours kept per particle in the list
-// moved with the slots at a rebuild: x, y, vx, vy, h, A, rho, hsml of both
-// buffers and the mass; the reference index and the cell
-constexpr int kPermDoubles = 17;
-constexpr int kPermInts = 2;
 
 // Error bits (OR-ed into InstDyn::err); translated to SIMP_E_* on the host.
 constexpr int kErrCapacity = 1;
@@ -50,8 +47,7 @@ struct InstMeta {
   int p_off, n;          // particle range
   int blk_off, nblk;     // logical particle-block range
   int b_off, nb;         // boundary-particle range
-  int c_off, c_cap;      // cell range (capacity; the last cell is kept for the
-                         // slots of a frozen instance at a rebuild)
+  int c_off, c_cap;      // cell range (capacity)
   int q_off, q_cap;      // boundary-pair / hsml_ij-store range
   int tse, wind_mode;
   int g_off, g_nx, g_ny;  // wind grid range
@@ -77,6 +73,9 @@ struct InstDyn {
   int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
   double hmin;          // MINVAL(hsml) of the current evaluation
+  int sel;              // state set holding the instance at the start of an evaluation
+  int rb;               // 1: its lists are rebuilt in this evaluation (the state
+                        //    then moves to set sel ^ 1: the current set is sel ^ rb)
 };
 
 // Per-launch block table entry: one block covers kTPB particle slots of
@@ -91,19 +90,18 @@ struct BlockEntry {
 
 // Device pointers of one engine.
 struct DevState {
-  // state buffers, by slot (= cell-list position): [0] = S (step start /
-  // current), [1] = T (half step)
-  double *x[2], *y[2], *vx[2], *vy[2], *h[2], *A[2], *rho[2], *hsml[2];
-  double *mass;
-  int *orig;            // reference (0-based) particle index of each slot
-  int *cellid;          // cell (local id) of each slot
-  // rebuild of the cell lists: slot of each particle in its cell, new list
-  // order (cl[new slot] = old slot), cell counts and starts
-  int *slot, *cl, *count, *cell_start, *scan_tmp;
-  // the state moved to the new order goes through these (kPermDoubles x N
-  // doubles, kPermInts x N ints) on its way back
-  double *perm_d;
-  int *perm_i;
+  // particle state by slot (= cell-list position), in two sets: index
+  // [2 * set + buf], buf 0 = S (step start / current), 1 = T (half step)
+  double *x[4], *y[4], *vx[4], *vy[4], *h[4], *A[4], *rho[4], *hsml[4];
+  double *mass[2];
+  int *orig[2];         // reference (0-based) particle index of each slot
+  int *cellid[2];       // cell (local id) of each slot
+  // cell lists: members per cell (kept current by the key updates), their
+  // prefix, and for a rebuild the slot of each particle in its new cell
+  // (from fill, zero between rebuilds) and the new order (cl[new slot] =
+  // old slot)
+  int *count, *cell_start, *scan_tmp, *fill, *slot, *cl;
+  int *cinst;           // instance of each cell
   // per cell: (stamp of the evaluation << 32) | largest member hsml as
   // FP32 bits, rounded up
   unsigned long long *cellh;

@@ -78,6 +78,7 @@ class Engine {
                    int *bnip, int *bpi, int *bpj, double *brij, double *w = nullptr);
   // kernel_zero of instance inst (n values)
   int download_kernel_zero(int inst, double *kz);
+  int cur_set(int inst);
 
   void set_diag(bool on) { diag_ = on; }
   bool diag() const { return diag_; }

@@ -123,7 +123,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   N = std::max<size_t>(N, 1);
   NB = std::max<size_t>(NB, 1);
   C = std::max<size_t>(C, 1) + 1;
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 4; ++b) {
     rc |= dalloc(list, &d.x[b], N);
     rc |= dalloc(list, &d.y[b], N);
     rc |= dalloc(list, &d.vx[b], N);
@@ -133,13 +133,13 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
     rc |= dalloc(list, &d.rho[b], N);
     rc |= dalloc(list, &d.hsml[b], N);
   }
-  rc |= dalloc(list, &d.mass, N);
-  rc |= dalloc(list, &d.orig, N);
-  rc |= dalloc(list, &d.cellid, N);
+  for (int b = 0; b < 2; ++b) {
+    rc |= dalloc(list, &d.mass[b], N);
+    rc |= dalloc(list, &d.orig[b], N);
+    rc |= dalloc(list, &d.cellid[b], N);
+  }
   rc |= dalloc(list, &d.slot, N);
   rc |= dalloc(list, &d.cl, N);
-  rc |= dalloc(list, &d.perm_d, (size_t)kPermDoubles * N);
-  rc |= dalloc(list, &d.perm_i, (size_t)kPermInts * N);
   rc |= dalloc(list, &d.own_cnt, N);
   rc |= dalloc(list, &d.moff, N);
   rc |= dalloc(list, &d.gpre, N);
@@ -149,6 +149,8 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.part, 5 * BLK);
   rc |= dalloc(list, &d.bflag, C);
   rc |= dalloc(list, &d.count, C);
+  rc |= dalloc(list, &d.fill, C);
+  rc |= dalloc(list, &d.cinst, C);
   rc |= dalloc(list, &d.cell_start, C);
   rc |= dalloc(list, &d.pair_count, C);
   rc |= dalloc(list, &d.pair_start, C);
@@ -261,7 +263,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     const long cells = cell_capacity(in);
     if (cells < 0) return -4;
     if (cells > (1L << 28)) return -3;
-    m.c_cap = (int)cells + 1;   // + the frozen-instance cell
+    m.c_cap = (int)cells;
     m.q_cap = store_capacity(in);
     if (j == 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size())
       m.q_cap = std::max(m.q_cap, meta_[carry_store_from].q_cap);
@@ -323,6 +325,9 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   } while (0)
   RCHK(hipMemsetAsync(nd.hq, 0, std::max<long>(qtot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.count, 0, (ctot + 1) * sizeof(int), st_));
+  RCHK(hipMemsetAsync(nd.fill, 0, (ctot + 1) * sizeof(int), st_));
+  // no key yet: an added instance's first evaluation builds its lists
+  RCHK(hipMemsetAsync(nd.cellid[0], 0xff, std::max<long>(ntot, 1) * sizeof(int), st_));
   // kernel_zero starts at zero, as the reference's module array does
   RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
@@ -337,33 +342,45 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     RCHK(hipMemcpyAsync(nd.rebuild, &one, sizeof(int), hipMemcpyHostToDevice, st_));
   }
 
-  // kept instances: device to device
+  // kept instances: device to device, from the set holding each one (its
+  // cell lists and member counts stay valid: no rebuild unless a key changes)
   auto d2d = [&](void *dst, const void *src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st_) : hipSuccess;
   };
+  std::vector<InstDyn> kdyn(nk);
+  for (int j = 0; j < nk; ++j) {
+    RCHK(hipMemcpyAsync(&kdyn[j], d_.dyn + keep[j], sizeof(InstDyn), hipMemcpyDeviceToHost, st_));
+  }
+  RCHK(hipStreamSynchronize(st_));
   for (int j = 0; j < nk; ++j) {
     const InstMeta &o = meta_[keep[j]];
     const InstMeta &m = nm[j];
     const size_t n8 = (size_t)m.n * 8, n4 = (size_t)m.n * 4;
-    RCHK(d2d(nd.x[0] + m.p_off, d_.x[0] + o.p_off, n8));
-    RCHK(d2d(nd.y[0] + m.p_off, d_.y[0] + o.p_off, n8));
-    RCHK(d2d(nd.vx[0] + m.p_off, d_.vx[0] + o.p_off, n8));
-    RCHK(d2d(nd.vy[0] + m.p_off, d_.vy[0] + o.p_off, n8));
-    RCHK(d2d(nd.h[0] + m.p_off, d_.h[0] + o.p_off, n8));
-    RCHK(d2d(nd.A[0] + m.p_off, d_.A[0] + o.p_off, n8));
-    RCHK(d2d(nd.rho[0] + m.p_off, d_.rho[0] + o.p_off, n8));
-    RCHK(d2d(nd.hsml[0] + m.p_off, d_.hsml[0] + o.p_off, n8));
-    RCHK(d2d(nd.mass + m.p_off, d_.mass + o.p_off, n8));
+    const int cs = kdyn[j].sel ^ kdyn[j].rb;
+    kdyn[j].sel = 0;
+    kdyn[j].rb = 0;
+    RCHK(d2d(nd.x[0] + m.p_off, d_.x[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.y[0] + m.p_off, d_.y[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.vx[0] + m.p_off, d_.vx[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.vy[0] + m.p_off, d_.vy[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.h[0] + m.p_off, d_.h[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.A[0] + m.p_off, d_.A[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.rho[0] + m.p_off, d_.rho[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.hsml[0] + m.p_off, d_.hsml[2 * cs] + o.p_off, n8));
+    RCHK(d2d(nd.mass[0] + m.p_off, d_.mass[cs] + o.p_off, n8));
     RCHK(d2d(nd.kz + m.p_off, d_.kz + o.p_off, n8));
-    RCHK(d2d(nd.orig + m.p_off, d_.orig + o.p_off, n4));
+    RCHK(d2d(nd.orig[0] + m.p_off, d_.orig[cs] + o.p_off, n4));
+    RCHK(d2d(nd.cellid[0] + m.p_off, d_.cellid[cs] + o.p_off, n4));
+    RCHK(d2d(nd.count + m.c_off, d_.count + o.c_off, (size_t)m.c_cap * 4));
     RCHK(d2d(nd.xb + m.b_off, d_.xb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.yb + m.b_off, d_.yb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.massb + m.b_off, d_.massb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.hq + m.q_off, d_.hq + o.q_off, (size_t)m.q_cap * 8));
     if (m.wind_mode == 1)
       RCHK(d2d(nd.grid + m.g_off, d_.grid + o.g_off, 16 * (size_t)m.g_nx * m.g_ny));
-    RCHK(d2d(nd.dyn + j, d_.dyn + keep[j], sizeof(InstDyn)));
   }
+  if (nk > 0)
+    RCHK(hipMemcpyAsync(nd.dyn, kdyn.data(), sizeof(InstDyn) * nk, hipMemcpyHostToDevice, st_));
   if (na > 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size() && d_.hq) {
     const InstMeta &o = meta_[carry_store_from];
     RCHK(d2d(nd.hq + nm[nk].q_off, d_.hq + o.q_off, (size_t)std::min(o.q_cap, nm[nk].q_cap) * 8));
@@ -405,8 +422,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     RCHK(h2d(nd.A[0] + m.p_off, in.A, n8));
     RCHK(h2d(nd.rho[0] + m.p_off, in.rho, n8));
     RCHK(h2d(nd.hsml[0] + m.p_off, in.hsml, n8));
-    RCHK(h2d(nd.mass + m.p_off, in.mass, n8));
-    RCHK(h2d(nd.orig + m.p_off, keepalive_i.back().data(), (size_t)n * 4));
+    RCHK(h2d(nd.mass[0] + m.p_off, in.mass, n8));
+    RCHK(h2d(nd.orig[0] + m.p_off, keepalive_i.back().data(), (size_t)n * 4));
     if (in.nb > 0) {
       RCHK(h2d(nd.xb + m.b_off, hxb, (size_t)in.nb * 8));
       RCHK(h2d(nd.yb + m.b_off, hyb, (size_t)in.nb * 8));
@@ -443,6 +460,10 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     }
   }
   if (!phys.empty()) RCHK(h2d(nd.blk, phys.data(), sizeof(BlockEntry) * phys.size()));
+  std::vector<int> cinst(ctot + 1, 0);
+  for (int k = 0; k < ninst; ++k)
+    std::fill(cinst.begin() + nm[k].c_off, cinst.begin() + nm[k].c_off + nm[k].c_cap, k);
+  RCHK(h2d(nd.cinst, cinst.data(), sizeof(int) * cinst.size()));
   RCHK(hipStreamSynchronize(st_));
 #undef RCHK
 
@@ -490,7 +511,7 @@ void Engine::L(const char *name, F &&f) {
 void Engine::enqueue_prefix(int buf) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
-  L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 1, d.rebuild, st_); });
+  L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 0, d.rebuild, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
 }
@@ -518,6 +539,7 @@ void Engine::enqueue_step() {
 
 int Engine::eval_single(bool want_diag) {
   enqueue_eval(0, 2, want_diag);
+  launch_fold(d_, st_);
   return sync_and_check();
 }
 
@@ -623,30 +645,41 @@ struct SlotOrder {
   }
 };
 
+// The set holding instance k's state (-1: the device could not be read).
+int Engine::cur_set(int k) {
+  InstDyn y;
+  if (hipMemcpy(&y, d_.dyn + k, sizeof(y), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return y.sel ^ y.rb;
+}
+
 int Engine::download_state(int k, double *x, double *vx, double *rho, double *hsml, double *hth,
                            double *A) {
   if (k < 0 || k >= d_.ninst) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n;
+  const int cs = cur_set(k);
+  if (cs < 0) return -2;
   SlotOrder so;
-  if (so.fetch(d_.orig, m.p_off, n)) return -2;
+  if (so.fetch(d_.orig[cs], m.p_off, n)) return -2;
   std::vector<double> a(n), b(n);
   auto get = [&](std::vector<double> &dst, const double *src) {
     return hipMemcpy(dst.data(), src + m.p_off, (size_t)n * 8, hipMemcpyDeviceToHost);
   };
   if (x) {
-    HIPCHK(get(a, d_.x[0]));
-    HIPCHK(get(b, d_.y[0]));
+    HIPCHK(get(a, d_.x[2 * cs]));
+    HIPCHK(get(b, d_.y[2 * cs]));
     so.put_pairs(a, b, x);
   }
   if (vx) {
-    HIPCHK(get(a, d_.vx[0]));
-    HIPCHK(get(b, d_.vy[0]));
+    HIPCHK(get(a, d_.vx[2 * cs]));
+    HIPCHK(get(b, d_.vy[2 * cs]));
     so.put_pairs(a, b, vx);
   }
-  const std::pair<double *, const double *> one[] = {
-      {rho, d_.rho[0]}, {hsml, d_.hsml[0]}, {hth, d_.h[0]}, {A, d_.A[0]}};
+  const std::pair<double *, const double *> one[] = {{rho, d_.rho[2 * cs]},
+                                                     {hsml, d_.hsml[2 * cs]},
+                                                     {hth, d_.h[2 * cs]},
+                                                     {A, d_.A[2 * cs]}};
   for (const auto &f : one) {
     if (!f.first) continue;
     HIPCHK(get(a, f.second));
@@ -660,8 +693,10 @@ int Engine::download_tend(int k, double *dxdt, double *dvxdt, double *dhdt, doub
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n, N = d_.ntot;
+  const int cs = cur_set(k);
+  if (cs < 0) return -2;
   SlotOrder so;
-  if (so.fetch(d_.orig, m.p_off, n)) return -2;
+  if (so.fetch(d_.orig[cs], m.p_off, n)) return -2;
   std::vector<double> a(n), b(n);
   auto get = [&](std::vector<double> &dst, int comp) {
     return hipMemcpy(dst.data(), d_.tend + (size_t)comp * N + m.p_off, (size_t)n * 8,
@@ -694,8 +729,10 @@ int Engine::download_diag(int k, double *eps3, double *zeta, double *eta, double
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
   const int n = m.n, N = d_.ntot;
+  const int cs = cur_set(k);
+  if (cs < 0) return -2;
   SlotOrder so;
-  if (so.fetch(d_.orig, m.p_off, n)) return -2;
+  if (so.fetch(d_.orig[cs], m.p_off, n)) return -2;
   std::vector<double> c0(n), c1(n), c2(n);
   auto get = [&](std::vector<double> &dst, const double *src, int comp) {
     return hipMemcpy(dst.data(), src + (size_t)comp * N + m.p_off, (size_t)n * 8,
@@ -739,6 +776,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   if (any_boundary_) launch_ranks(d, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);
+  launch_fold(d, st_);
   int rc = sync_and_check();
   if (rc) return rc;
   int n_ip = 0, n_bp = 0, err = 0;

@@ -101,6 +101,39 @@ __device__ __forceinline__ bool particle_of(const DevState &d, int &p, int &k) {
   return true;
 }
 
+// An instance's state is in set sel at the start of an evaluation and in
+// set sel ^ rb once its lists are rebuilt: buffer b of set cs is array
+// 2 cs + b of x ... hsml; mass, orig and cellid are by set.
+__device__ __forceinline__ int cur_set(const InstDyn &y) { return y.sel ^ y.rb; }
+
+// Per wavefront, the lanes holding equal keys in consecutive lanes form
+// runs; one atomic add of the run's total per run.  Returns each lane's
+// offset: the counter's old value plus the increments of the run's
+// earlier lanes (lanes with key < 0 are not counted).
+__device__ __forceinline__ int run_add(int *base, int key, int inc) {
+  const int lane = (int)threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  // first lane of this lane's run, first lane of the next run
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(below);
+  const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+  const int next = above ? __ffsll((long long)above) - 1 : 64;
+  // increments before this lane in its run (inclusive scan within runs)
+  int incl = inc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane - o >= start) incl += t;
+  }
+  const int total = __shfl(incl, next - 1, 64);
+  int old = 0;
+  if (head && key >= 0 && total != 0) old = atomicAdd(base + key, total);
+  old = __shfl(old, start, 64);
+  return old + incl - inc;
+}
+
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -298,13 +331,25 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   if (blockIdx.x >= (unsigned)d.nblk) return;
   int p = 0, k = 0;
   bool ok = particle_of(d, p, k);
-  double xv = ok ? d.x[buf][p] : 0.0, yv = ok ? d.y[buf][p] : 0.0;
-  double hv = ok ? d.hsml[buf][p] : 1.0;
+  const int cs = cur_set(d.dyn[k]);
+  double xv = ok ? d.x[2 * cs + buf][p] : 0.0, yv = ok ? d.y[2 * cs + buf][p] : 0.0;
+  double hv = ok ? d.hsml[2 * cs + buf][p] : 1.0;
   block_partials(d, k, ok, xv, yv, hv);
 }
 
 __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
                              double ymn, double ymx, double hmn);
+
+// end of an evaluation: the current set becomes the instance's set
+__device__ __forceinline__ void fold_set(InstDyn &y) {
+  y.sel ^= y.rb;
+  y.rb = 0;
+}
+
+__global__ void __launch_bounds__(kTPB) k_fold(DevState d) {
+  const int k = blockIdx.x * kTPB + threadIdx.x;
+  if (k < d.ninst) fold_set(d.dyn[k]);
+}
 
 // One block per instance: fold the instance's block partials into the grid
 // of the next evaluation (bucket_search, nnps.f90:211-224).  With
@@ -316,6 +361,7 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
+  if (threadIdx.x == 0) fold_set(d.dyn[k]);
   const double big = __builtin_huge_val();
   double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big;
   for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
@@ -374,7 +420,7 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
   const int xlo = nint(xmn / bin), xhi = nint(xmx / bin);
   const int ylo = nint(ymn / bin), yhi = nint(ymx / bin);
   const long long W = (long long)xhi - xlo + 3, H = (long long)yhi - ylo + 3;
-  if (!(hmn > 0.0) || W * H > (long long)m.c_cap - 1 || W * H <= 0) {
+  if (!(hmn > 0.0) || W * H > (long long)m.c_cap || W * H <= 0) {
     y.err |= kErrCapacity;
     return 0;
   }
@@ -401,34 +447,50 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
 // are in cell order, but for particles that just changed cell), then one
 // 64-bit atomic max per run of (evaluation stamp, value) -- a stale stamp
 // reads as an empty cell, so nothing needs clearing.
-// The cell lists are rebuilt only when some key differs from the one its
-// slot was sorted with (or on the first evaluation after an upload); the
-// rebuild kernels below return at once otherwise.  Between the predictor
-// and the corrector evaluation of a step the grid and the keys normally
-// stay the same, so the corrector reuses the lists.
+// The member counts of the cells follow the keys that change.  An
+// instance's lists are rebuilt only when one of its keys differs from the
+// one its slot was sorted with (or on its first evaluation); the rebuild
+// kernels below return at once for the others.  Between the predictor and
+// the corrector evaluation of a step the grid and the keys normally stay
+// the same, so the corrector reuses the lists.
 __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   int p = 0, k = 0;
   bool changed = false;
   const int lane = (int)threadIdx.x & 63;
-  int key = -1 - lane;
+  int key = -1 - lane, oldk = -1 - lane;
   unsigned long long hv = 0;
   const bool valid = particle_of(d, p, k) && !inst_off(d.dyn[k]);
+  int coff = 0;
   if (valid) {
     const InstDyn &y = d.dyn[k];
+    const int cs = y.sel;
+    coff = d.meta[k].c_off;
     const double bin = y.bin;
-    const double xq = d.x[buf][p], yq = d.y[buf][p], hq = d.hsml[buf][p];
+    const double xq = d.x[2 * cs + buf][p], yq = d.y[2 * cs + buf][p], hq = d.hsml[2 * cs + buf][p];
     const int cx = nint(xq / bin) - y.xlo + 1;
     const int cy = nint(yq / bin) - y.ylo + 1;
     const int c = cy * y.W + cx;
-    changed = d.cellid[p] != c;
-    d.cellid[p] = c;
+    const int old = d.cellid[cs][p];
+    changed = old != c;
+    if (changed) {
+      d.cellid[cs][p] = c;
+      oldk = old;   // < 0 on the first evaluation
+    }
     const double ox = (y.xlo - 1) * bin, oy = (y.ylo - 1) * bin;
     d.pf[p] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
     key = c;
     hv = ((unsigned long long)(unsigned)y.stamp << 32) |
          __float_as_uint((float)hq * 1.000001f);
   }
-  if (__any(changed) && lane == 0) atomicOr(d.rebuild, 1);
+  // blocks never straddle instances: one instance per wavefront
+  if (__any(changed) && lane == 0) {
+    atomicOr(&d.dyn[k].rb, 1);
+    atomicOr(d.rebuild, 1);
+  }
+  if (__any(changed)) {
+    run_add(d.count + coff, changed ? key : -1 - lane, 1);
+    run_add(d.count + coff, oldk, -1);
+  }
   for (int o = 1; o < 64; o <<= 1) {
     const unsigned lo = __shfl_down((unsigned)hv, o, 64);
     const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
@@ -437,22 +499,25 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
     if (lane + o < 64 && ok == key && ov > hv) hv = ov;
   }
   const int prev = __shfl_up(key, 1, 64);
-  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[d.meta[k].c_off + key], hv);
+  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[coff + key], hv);
 }
 
-// Histogram of the cell keys (slot of each particle in its cell).
-// A frozen instance keeps its slots where they are: all of them counted in
-// its last cell, so every instance's lists start at its p_off.
+// Slot of each particle of a rebuilt instance in its new cell: consecutive
+// slots mostly share a cell, one atomic per run.  fill is zero again after
+// the cell sort.
 __global__ void __launch_bounds__(kTPB) k_hist(DevState d) {
-  if (*d.rebuild == 0) return;
+  if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
+  if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
   int p = 0, k = 0;
-  if (!particle_of(d, p, k)) return;
-  const InstMeta &m = d.meta[k];
-  if (inst_off(d.dyn[k])) {
-    if (p == m.p_off) d.count[m.c_off + m.c_cap - 1] = m.n;
-    return;
+  const int lane = (int)threadIdx.x & 63;
+  int key = -1 - lane, coff = 0;
+  if (particle_of(d, p, k)) {
+    const InstDyn &y = d.dyn[k];
+    coff = d.meta[k].c_off;
+    key = d.cellid[y.sel][p];
   }
-  d.slot[p] = atomicAdd(&d.count[m.c_off + d.cellid[p]], 1);
+  const int sl = run_add(d.fill + coff, key, 1);
+  if (key >= 0) d.slot[p] = sl;
 }
 
 // ------------------------------------------------------------------- scan
@@ -505,17 +570,14 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 
 // ------------------------------------------------------- scatter/cellsort
 
-// cl[new slot] = old slot; a frozen instance's last cell is marked unfilled
-// (never sorted, never moved)
+// cl[new slot] = old slot
 __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
   if (*d.rebuild == 0) return;
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
-  if (inst_off(d.dyn[k])) {
-    d.cl[p] = -1;
-    return;
-  }
-  d.cl[d.cell_start[d.meta[k].c_off + d.cellid[p]] + d.slot[p]] = p;
+  const InstDyn &y = d.dyn[k];
+  if (y.rb == 0) return;
+  d.cl[d.cell_start[d.meta[k].c_off + d.cellid[y.sel][p]] + d.slot[p]] = p;
 }
 
 // members of each cell in ascending particle id: the order in which the
@@ -528,17 +590,20 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   if (*d.rebuild == 0) return;
   int c = blockIdx.x * kTPB + threadIdx.x;
   if (c >= d.ctot) return;
+  const InstDyn &y = d.dyn[d.cinst[c]];
+  if (y.rb == 0) return;
   const int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
   const int n = s1 - s0;
-  if (n <= 1) return;
-  for (int s = s0; s < s1; ++s)
-    if ((unsigned)d.cl[s] >= (unsigned)d.ntot) return;  // not filled: frozen instance
+  if (n == 0) return;
+  d.fill[c] = 0;
+  if (n == 1) return;
+  const int *orig = d.orig[y.sel];
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
       v[i] = i < n ? d.cl[s0 + i] : 0;
-      key[i] = i < n ? d.orig[v[i]] : 0x7fffffff;
+      key[i] = i < n ? orig[v[i]] : 0x7fffffff;
     }
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
@@ -553,11 +618,11 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   }
   for (int a = s0 + 1; a < s1; ++a) {
     int v = d.cl[a];
-    int key = d.orig[v];
+    int key = orig[v];
     int b = a - 1;
     while (b >= s0) {
       int u = d.cl[b];
-      if (d.orig[u] <= key) break;
+      if (orig[u] <= key) break;
       d.cl[b + 1] = u;
       --b;
     }
@@ -565,51 +630,33 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
   }
 }
 
-// The state moves to the new slot order of a rebuild: each slot gathers
-// the particle the sorted lists put there (cl[slot] = its old slot) into
-// perm_d / perm_i, with the prefilter positions of the evaluation (buffer
-// buf); k_unpermute copies it back.
-__device__ __forceinline__ void perm_arrays(const DevState &d, double *(&a)[kPermDoubles]) {
-  for (int b = 0; b < 2; ++b) {
-    a[8 * b + 0] = d.x[b]; a[8 * b + 1] = d.y[b];
-    a[8 * b + 2] = d.vx[b]; a[8 * b + 3] = d.vy[b];
-    a[8 * b + 4] = d.h[b]; a[8 * b + 5] = d.A[b];
-    a[8 * b + 6] = d.rho[b]; a[8 * b + 7] = d.hsml[b];
-  }
-  a[16] = d.mass;
-}
-
+// The state of a rebuilt instance moves to the new slot order, from its set
+// sel to set sel ^ 1: each slot gathers the particle the sorted lists put
+// there (cl[slot] = its old slot), and writes the prefilter positions of
+// the evaluation (buffer buf).  The half-step buffer moves only when it is
+// live (the corrector's evaluation).
 __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
   if (*d.rebuild == 0) return;
   int s = 0, k = 0;
   if (!particle_of(d, s, k)) return;
   const InstDyn &y = d.dyn[k];
-  if (inst_off(y)) return;
-  const size_t N = (size_t)d.ntot;
+  if (y.rb == 0) return;
+  const int f = y.sel, t = y.sel ^ 1;
   const int q = d.cl[s];
-  double *a[kPermDoubles];
-  perm_arrays(d, a);
-#pragma unroll
-  for (int j = 0; j < kPermDoubles; ++j) d.perm_d[j * N + s] = a[j][q];
-  d.perm_i[s] = d.orig[q];
-  d.perm_i[N + s] = d.cellid[q];
-  const double xq = d.x[buf][q], yq = d.y[buf][q], hq = d.hsml[buf][q];
+  for (int b = 0; b <= buf; ++b) {
+    const int i = 2 * f + b, o = 2 * t + b;
+    d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
+    d.vx[o][s] = d.vx[i][q]; d.vy[o][s] = d.vy[i][q];
+    d.h[o][s] = d.h[i][q]; d.A[o][s] = d.A[i][q];
+    d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
+  }
+  d.mass[t][s] = d.mass[f][q];
+  d.orig[t][s] = d.orig[f][q];
+  d.cellid[t][s] = d.cellid[f][q];
+  const int i = 2 * f + buf;
+  const double xq = d.x[i][q], yq = d.y[i][q], hq = d.hsml[i][q];
   const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
   d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
-}
-
-__global__ void __launch_bounds__(kTPB) k_unpermute(DevState d) {
-  if (*d.rebuild == 0) return;
-  int s = 0, k = 0;
-  if (!particle_of(d, s, k)) return;
-  if (inst_off(d.dyn[k])) return;
-  const size_t N = (size_t)d.ntot;
-  double *a[kPermDoubles];
-  perm_arrays(d, a);
-#pragma unroll
-  for (int j = 0; j < kPermDoubles; ++j) a[j][s] = d.perm_d[j * N + s];
-  d.orig[s] = d.perm_i[s];
-  d.cellid[s] = d.perm_i[N + s];
 }
 
 // -------------------------------------------------------------- boundary
@@ -733,8 +780,9 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const int W = y.W;
   const int *cs0 = d.cell_start + m.c_off;
   // the evaluated state, by slot (= list position)
-  const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RS = d.rho[buf];
-  const double *VX = d.vx[buf], *VY = d.vy[buf], *MS = d.mass;
+  const int cs = cur_set(y);
+  const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
+  const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
   const int pos0 = be.pos;                // first slot of the block
   const int nloc = be.nloc;
   const bool valid = (int)threadIdx.x < nloc;
@@ -747,7 +795,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // takes the exact rescan path instead.
   int As, Ae, Bs, Be, Cs, Ce;
   {
-    const int cf = d.cellid[pos0], cl = d.cellid[pos0 + nloc - 1];
+    const int cf = d.cellid[cs][pos0], cl = d.cellid[cs][pos0 + nloc - 1];
     As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
     Bs = cs0[cf - 1];     Be = cs0[cl + 2];
     Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
@@ -768,7 +816,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // waves with large supports (more neighbours) get issue priority: they
   // set the kernel's length
   if (__any(HS[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
-  const int c = d.cellid[self];
+  const int c = d.cellid[cs][self];
   const int cy = c / W, cx = c - cy * W;
   const double xi = X[self], yi = Y[self], hi = HS[self];
   const double vxi = VX[self], vyi = VY[self];
@@ -902,7 +950,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // walk finds it.  It must be known before the density sums start.
   double kzi = 0.;
   if (sumd) {
-    kzi = d.kz[m.p_off + d.orig[self]];
+    kzi = d.kz[m.p_off + d.orig[cs][self]];
     bool found = false;
     if (nc <= kNbrMax) {
       for (int j = nc - 1; j >= 0 && !found; --j) {
@@ -995,7 +1043,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
   }
   if (sumd) {
-    d.kz[m.p_off + d.orig[self]] = kzi;
+    d.kz[m.p_off + d.orig[cs][self]] = kzi;
     d.dnum[self] = num;
     d.dsum[self] = num / den;
   }
@@ -1023,7 +1071,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // replacement pressure (eos.f90:141-147), stress tensor
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
-  const double hth = d.h[buf][p], Ac = d.A[buf][p];
+  const double hth = d.h[2 * cs + buf][p], Ac = d.A[2 * cs + buf][p];
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1170,7 +1218,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
         // the last position whose prefix is below lim: its cell is the
         // last one needed
         if (gI + gX < lim && gI + gX + vI[u] + vX[u] >= lim)
-          sh_stop = cs[d.cellid[p0 + t] + 1] - p0;
+          sh_stop = cs[d.cellid[cur_set(y)][p0 + t] + 1] - p0;
       }
       gI += vI[u];
       gX += vX[u];
@@ -1196,6 +1244,7 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || inst_off(y)) return;
+  const int ss = cur_set(y);
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n, t = s - p0;
   const int lim = y.rlim;
@@ -1203,7 +1252,7 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   const int2 *G = d.gpre + p0;
   const size_t N = (size_t)d.ntot;
   const int W = y.W;
-  const int c = d.cellid[s];
+  const int c = d.cellid[ss][s];
   const int f = cs[c] - p0, e = cs[c + 1] - p0;
   const int2 Gf = G[f], Gt = G[t];
   const int2 Ge = e < np ? G[e] : make_int2(y.gendI, y.gendX);
@@ -1214,26 +1263,26 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   int rI = base + (Gt.x - Gf.x);
   int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
   if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
-  const double hi = d.hsml[buf][s];
+  const double hi = d.hsml[2 * ss + buf][s];
   const int nn = d.nnb[s];
   if (nn <= kNbrMax) {
     // the owned pairs are the tail of the list: nI intra, then nX cross
     for (int j = nn - nI - nX; j < nn; ++j) {
       const int q = d.nbr[j * N + s];
       int &rank = (j < nn - nX) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hsml[buf][q]) / 2.;
+      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hsml[2 * ss + buf][q]) / 2.;
       ++rank;
     }
     return;
   }
   const int cy = c / W, cx = c - cy * W;
-  const double xi = d.x[buf][s], yi = d.y[buf][s];
+  const double xi = d.x[2 * ss + buf][s], yi = d.y[2 * ss + buf][s];
   const Ranges R = neighbour_ranges(cs, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int q = R.a[g]; q < R.b[g]; ++q) {
       if (g == 2 && q <= s) continue;
-      const double dx = xi - d.x[buf][q], dy = yi - d.y[buf][q];
-      const double mh = (hi + d.hsml[buf][q]) / 2.;
+      const double dx = xi - d.x[2 * ss + buf][q], dy = yi - d.y[2 * ss + buf][q];
+      const double mh = (hi + d.hsml[2 * ss + buf][q]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
       int &rank = (g == 2) ? rI : rX;
@@ -1298,9 +1347,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // particle id, the cell and those entries index.  A wave so waits for
     // two memory round trips before its first neighbour arithmetic.
     // the evaluated state, by slot (= list position)
-    const double *X = d.x[buf], *Y = d.y[buf], *HS = d.hsml[buf], *RS = d.rho[buf];
-    const double *VX = d.vx[buf], *VY = d.vy[buf], *MS = d.mass;
-    const int c = d.cellid[self];
+    const int cs = cur_set(y);
+    const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
+    const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
+    const int c = d.cellid[cs][self];
     const int p = self;
     const double xi = X[self], yi = Y[self], hi = HS[self];
     const double rhoi = RS[self], mi = MS[self];
@@ -1318,14 +1368,14 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     if (sd && phase != 2) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
     // round 2
-    const double hthi = d.h[buf][p], Ai = d.A[buf][p];
+    const double hthi = d.h[2 * cs + buf][p], Ai = d.A[2 * cs + buf][p];
     const int bfl = d.bflag[m.c_off + c];
     double x1 = 0., y1 = 0., h1 = 0., A1 = 0., kzi = 0.;
     if (phase == 1) {
-      x1 = d.x[0][p]; y1 = d.y[0][p];
-      h1 = d.h[0][p]; A1 = d.A[0][p];
+      x1 = d.x[2 * cs + 0][p]; y1 = d.y[2 * cs + 0][p];
+      h1 = d.h[2 * cs + 0][p]; A1 = d.A[2 * cs + 0][p];
     }
-    if (sumd3) kzi = d.kz[m.p_off + d.orig[self]];
+    if (sumd3) kzi = d.kz[m.p_off + d.orig[cs][self]];
 
     // Everything of the update but the velocities is known before the
     // stress sums (single_step.f90:104-106 tendencies; predictor
@@ -1338,36 +1388,36 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       px_new = xi + vxi * dt / 2;
       py_new = yi + vyi * dt / 2;
       hsml_new = hi;
-      d.x[1][p] = px_new;
-      d.y[1][p] = py_new;
+      d.x[2 * cs + 1][p] = px_new;
+      d.y[2 * cs + 1][p] = py_new;
       const double hh = hthi + dhdt * dt / 2;
       double Ah = Ai + dAdt * dt / 2;
       if (Ah > 1) Ah = 1;
-      d.h[1][p] = hh;
-      d.A[1][p] = Ah;
+      d.h[2 * cs + 1][p] = hh;
+      d.A[2 * cs + 1][p] = Ah;
       // density (time_integration.f90:218): diagnostic, or the summation
       // pass A computed on the step-start kernel values
-      d.rho[1][p] = sd ? dsi : hh * kRhoIce;
-      d.hsml[1][p] = hi;
+      d.rho[2 * cs + 1][p] = sd ? dsi : hh * kRhoIce;
+      d.hsml[2 * cs + 1][p] = hi;
     } else if (phase == 1) {
       const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
       const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
       px_new = 2 * x12 - x1;
       py_new = 2 * y12 - y1;
-      d.x[0][p] = px_new;
-      d.y[0][p] = py_new;
+      d.x[2 * cs + 0][p] = px_new;
+      d.y[2 * cs + 0][p] = py_new;
       const double hn = 2 * h12 - h1;
       double An = 2 * A12 - A1;
       if (An > 1) An = 1;
-      d.h[0][p] = hn;
-      d.A[0][p] = An;
+      d.h[2 * cs + 0][p] = hn;
+      d.A[2 * cs + 0][p] = An;
       if (!sumd3) {
         const double rn = hn * kRhoIce;
-        d.rho[0][p] = rn;
+        d.rho[2 * cs + 0][p] = rn;
         // Marquis_hsml (smoothing_length.f90:136-142)
         double hs = sqrt(mi / rn) * kHsmlMult;
         if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[0][p] = hs;
+        d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
       }
     }
@@ -1529,20 +1579,20 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     }
     // velocities (time_integration.f90:210-211, 240-241) or tendencies
     if (phase == 0) {
-      d.vx[1][p] = vxi + ax * dt / 2;
-      d.vy[1][p] = vyi + ay * dt / 2;
+      d.vx[2 * cs + 1][p] = vxi + ax * dt / 2;
+      d.vy[2 * cs + 1][p] = vyi + ay * dt / 2;
     } else if (phase == 1) {
-      const double u1 = d.vx[0][p], v1 = d.vy[0][p];
+      const double u1 = d.vx[2 * cs + 0][p], v1 = d.vy[2 * cs + 0][p];
       const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
-      d.vx[0][p] = 2 * u12 - u1;
-      d.vy[0][p] = 2 * v12 - v1;
+      d.vx[2 * cs + 0][p] = 2 * u12 - u1;
+      d.vy[2 * cs + 0][p] = 2 * v12 - v1;
       if (sumd3) {
         // summation density of the corrector, then Marquis_hsml
         const double rn = dnumi / den3;
-        d.rho[0][p] = rn;
+        d.rho[2 * cs + 0][p] = rn;
         double hs = sqrt(mi / rn) * kHsmlMult;
         if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[0][p] = hs;
+        d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
       }
     } else {
@@ -1570,29 +1620,30 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (inst_off(y)) return;
+  const int cs = cur_set(y);
   const int s = p;
   const int oc = d.own_cnt[s];
   if ((oc & 0xfffff) == 0) return;
-  const int c = d.cellid[p];
+  const int c = d.cellid[cs][p];
   const int pbase = d.pair_start[m.c_off];
   const int base = d.pair_start[m.c_off + c] - pbase;
   int rI = base + (d.moff[s] & 0xffff);
   int rX = base + d.cell_intra[m.c_off + c] + (d.moff[s] >> 16);
   const int W = y.W, cy = c / W, cx = c - cy * W;
-  const double xi = d.x[buf][s], yi = d.y[buf][s], hi = d.hsml[buf][s];
+  const double xi = d.x[2 * cs + buf][s], yi = d.y[2 * cs + buf][s], hi = d.hsml[2 * cs + buf][s];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int t = R.a[g]; t < R.b[g]; ++t) {
       if (g == 2 && t <= s) continue;
-      const double dx = xi - d.x[buf][t], dy = yi - d.y[buf][t];
-      const double mh = (hi + d.hsml[buf][t]) / 2.;
+      const double dx = xi - d.x[2 * cs + buf][t], dy = yi - d.y[2 * cs + buf][t];
+      const double mh = (hi + d.hsml[2 * cs + buf][t]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
       const int q = t;
       int &rank = (g == 2) ? rI : rX;
       if (rank < cap) {
-        pi[rank] = d.orig[p] + 1;
-        pj[rank] = d.orig[q] + 1;
+        pi[rank] = d.orig[cs][p] + 1;
+        pj[rank] = d.orig[cs][q] + 1;
         rij[rank] = r;
         hij[rank] = mh;
         if (w) w[rank] = c6_w(r, mh);
@@ -1609,6 +1660,7 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
   const int k = 0;
   const InstMeta &m = d.meta[k];
   const InstDyn &y = d.dyn[k];
+  const int ss = cur_set(y);
   int b = blockIdx.x * kTPB + threadIdx.x;
   if (b >= m.nb || inst_off(y)) return;
   const int bg = m.b_off + b;
@@ -1622,12 +1674,12 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
     int c = (cyb + s9 / 3 - 1) * W + (cxb + s9 % 3 - 1);
     for (int s = cs[c]; s < cs[c + 1]; ++s) {
       const int q = s;
-      double dx = bx - d.x[buf][s], dy = by - d.y[buf][s];
+      double dx = bx - d.x[2 * ss + buf][s], dy = by - d.y[2 * ss + buf][s];
       double dd = sqrt(dx * dx + dy * dy);
       if (dd <= kBLength) {
         if (rank < bcap) {
           bpi[rank] = b + 1;
-          bpj[rank] = d.orig[q] + 1;
+          bpj[rank] = d.orig[ss][q] + 1;
           brij[rank] = dd;
         }
         ++rank;
@@ -1648,6 +1700,9 @@ void launch_partial(const DevState &d, int buf, hipStream_t st) {
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
   k_finalize<<<g8(d.ninst), kTPB, 0, st>>>(d, end_of_step);
 }
+void launch_fold(const DevState &d, hipStream_t st) {
+  k_fold<<<cdiv(std::max(d.ninst, 1), kTPB), kTPB, 0, st>>>(d);
+}
 void launch_count(const DevState &d, int buf, hipStream_t st) {
   k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
   k_hist<<<g8(d.nblk), kTPB, 0, st>>>(d);
@@ -1665,7 +1720,6 @@ void launch_scatter(const DevState &d, hipStream_t st) {
 void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
   k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
   k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
-  k_unpermute<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
                   hipStream_t st) {
