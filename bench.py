@@ -186,11 +186,12 @@ def bench_dropin(sc, calls=10):
             "note": "PCIe-inclusive, FP64; one call = one right-hand side (half a step)"}
 
 
-def bench_ensemble(steps=200, baseline_steps=50):
+def bench_ensemble(steps=500, baseline_steps=50):
     """Config 4: 64 closed boxes, particle counts lognormal in [2k, 60k],
-    box 50-300 km, random uniform wind; all stepped in one packed launch per
-    kernel (mixed precision), against the plain FP64 path stepping each
-    instance with its own launches (one instance after the other)."""
+    box 50-300 km, random uniform wind, 500 steps; all stepped in one packed
+    launch per kernel (mixed precision), against the plain FP64 path
+    stepping each instance with its own launches (one instance after the
+    other)."""
     from simp_amd import _lib
     from simp_amd import scenarios as S
     from simp_amd.api import Simp

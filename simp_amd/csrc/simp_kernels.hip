@@ -710,6 +710,15 @@ __device__ __forceinline__ bool in_support(double d2, double mh, double &r) {
   return r <= mh;
 }
 
+// The same predicate without d: below (1 - 1e-13) mhsml^2 the rounded
+// square root cannot exceed mhsml, so only the band between takes it.
+__device__ __forceinline__ bool in_support_nr(double d2, double mh) {
+  const double m2 = mh * mh;
+  if (d2 > m2 * (1.0 + 1e-13)) return false;
+  if (d2 < m2 * (1.0 - 1e-13)) return true;
+  return sqrt(d2) <= mh;
+}
+
 // Wendland C6 gradient factor (kernel.f90:499-508):
 // dW/dx = g * dx / r with g = factor * (-22 q (1-q)^7 (16 q^2 + 7 q + 1)) / hsml_ij
 __device__ __forceinline__ double c6_grad(double r, double mh) {
@@ -761,8 +770,12 @@ struct KSum {
 
 // ---------------------------------------------------------------- pass A
 
-// LDS tile of the block's candidate region (FP32 prefilter data)
+// LDS tile of the block's candidate region (FP32 prefilter data; hsml
+// pre-multiplied by kHalfPad = sqrt(1.00001) / 2, the prefilter's relative
+// margin folded in)
 constexpr int kTile = 1280;
+constexpr float kPad = 1.000005f;
+constexpr float kHalfPad = 0.5f * kPad;
 
 // SUMD: instances with summation density may be present (the density
 // sums are compiled in; each instance still follows its own den_alg)
@@ -808,7 +821,9 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   if (tiled) {
     for (int i = threadIdx.x; i < T; i += kTPB) {
       const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
-      tile[i] = d.pf[s];
+      float4 v = d.pf[s];
+      v.z *= kHalfPad;
+      tile[i] = v;
     }
   }
   __syncthreads();
@@ -833,6 +848,10 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
   double num = 0., den = 0.;
   KSum fdiv, f1, f2, f3, f4;
+  // the pair predicate; d itself only where the FP64 sums use it
+  auto support = [&](double d2, double mh, double &r) {
+    return (MIXED && !sumd) ? in_support_nr(d2, mh) : in_support(d2, mh, r);
+  };
   // kernel gradient, vector_divergence and vector_gradient of one pair
   auto pair = [&](double dx, double dy, double r, double mh, double mq, double rq, double vxq,
                   double vyq) {
@@ -880,6 +899,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     auto tix = [&](int sp) { return sp >= Cs ? sp - Cs + tC : (sp >= Bs ? sp - Bs + tB : sp - As); };
     const int tself = tix(self);
     const float4 pi4 = tile[tself];
+    const float hif = (float)hi;
     const float slack = y.pf_slack;
     // Cell culling: a neighbour cell is scanned only if its nearest point
     // is within the largest support a pair with one of its members can
@@ -899,7 +919,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     auto need = [&](int c, float ex, float ey) {
       const unsigned long long hc = ch[c];
       const float hmax = (unsigned)(hc >> 32) == stampu ? __uint_as_float((unsigned)hc) : 0.0f;
-      const float bnd = __builtin_fmaf(pi4.z + hmax, 0.5f, slack);
+      const float bnd = __builtin_fmaf(hif + hmax, 0.5f, slack);
       return __builtin_fmaf(ex, ex, ey * ey) <= bnd * bnd * 1.00001f;
     };
     // The four segments, in visiting order: row below (cells rm-1, rm,
@@ -927,13 +947,16 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     const int o1 = tix(a1) - e0;
     const int o2 = (b2 > a2 ? tix(a2) : 0) - e1;
     const int o3 = (b3 > a3 ? tix(a3) : 0) - e2;
+    // (mhsml + slack) sqrt(1.00001) = hzi + the candidate's scaled hsml
+    const float hzi = __builtin_fmaf(slack, kPad, pi4.z);
     for (int j = 0; j < e3; ++j) {
       const int ti = j + (j < e0 ? o0 : (j < e1 ? o1 : (j < e2 ? o2 : o3)));
       const float4 q = tile[ti];
       // rejects only candidates certainly beyond mhsml
       const float fx = pi4.x - q.x, fy = pi4.y - q.y;
-      const float fm = __builtin_fmaf(pi4.z + q.z, 0.5f, slack);
-      const bool hit = (ti != tself) & (__builtin_fmaf(fx, fx, fy * fy) <= fm * fm * 1.00001f);
+      const float fm = hzi + q.z;
+      const bool hit =
+          (ti != tself) & (__builtin_fmaf(-fm, fm, __builtin_fmaf(fx, fx, fy * fy)) <= 0.0f);
       if (hit) {
         if (nc < kNbrMax) lst[nc * kTPB + threadIdx.x] = (unsigned short)ti;
         ++nc;
@@ -1005,8 +1028,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         if (j0 + u >= nc) continue;
         const double dx = xi - xq[u], dy = yi - yq[u];
         const double mh = (hi + hq[u]) / 2.;
-        double r;
-        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        double r = 0.;
+        if (!support(dx * dx + dy * dy, mh, r)) continue;
         const int kind = pair_kind(R, sj[u], self);
         nI += kind == 1;
         nX += kind == 2;
@@ -1024,8 +1047,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         if (s == self) continue;
         const double dx = xi - X[s], dy = yi - Y[s];
         const double mh = (hi + HS[s]) / 2.;
-        double r;
-        if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+        double r = 0.;
+        if (!support(dx * dx + dy * dy, mh, r)) continue;
         const int kind = pair_kind(R, s, self);
         nI += kind == 1;
         nX += kind == 2;
