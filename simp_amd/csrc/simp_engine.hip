@@ -511,7 +511,7 @@ void Engine::L(const char *name, F &&f) {
 void Engine::enqueue_prefix(int buf) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
-  L("scan", [&] { launch_scan(d.count, d.ctot, scan_tmp_, d.cell_start, 0, d.rebuild, st_); });
+  L("scan", [&] { launch_rebuild_scan(d, scan_tmp_, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
   L("cellsort", [&] { launch_cellsort(d, buf, st_); });
 }

@@ -14,6 +14,8 @@ void launch_count(const DevState &d, int buf, hipStream_t st);
 // gate: device flag; the scan does nothing while it reads 0 (nullptr: always runs)
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st);
+// rebuilds: slots and the starts of the cell lists (counts are current)
+void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_cellsort(const DevState &d, int buf, hipStream_t st);
 void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,

@@ -505,8 +505,8 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
 // Slot of each particle of a rebuilt instance in its new cell: consecutive
 // slots mostly share a cell, one atomic per run.  fill is zero again after
 // the cell sort.
-__global__ void __launch_bounds__(kTPB) k_hist(DevState d) {
-  if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
+__device__ void hist_block(const DevState &d) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
   if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
   int p = 0, k = 0;
   const int lane = (int)threadIdx.x & 63;
@@ -535,6 +535,23 @@ __global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum,
   }
   s = block_sum(s);
   if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+// First launch of a rebuild: the slots (k_hist's work) and the block sums of
+// the cell counts (k_scan1's), by the same blocks.
+__global__ void __launch_bounds__(kTPB) k_hist_sum(DevState d, int *bsum, int nsb) {
+  if (*d.rebuild == 0) return;
+  if (blockIdx.x < (unsigned)nsb) {
+    constexpr int IPT = kScanItems / kTPB;
+    const int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+      if (base + i < d.ctot) sum += d.count[base + i];
+    sum = block_sum(sum);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sum;
+  }
+  hist_block(d);
 }
 
 // out[i] = sum(in[0..i-1]); out[L] = total.  Optionally clears in[].
@@ -1167,6 +1184,9 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
 // pairs' hsml_ij.
 constexpr int kRankThreads = 1024;
 
+__device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &m, int ss,
+                           int buf, int t, int lim, int gendI, int gendX);
+
 __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
   __shared__ int sh_stop;
   if (blockIdx.x >= (unsigned)d.ninst) return;
@@ -1267,18 +1287,24 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || inst_off(y)) return;
-  const int ss = cur_set(y);
+  const int t = s - m.p_off;
+  if (t >= y.rstop || y.rlim <= 0) return;
+  rank_store(d, y, m, cur_set(y), buf, t, y.rlim, y.gendI, y.gendX);
+}
+
+// The hsml_ij of the pairs list position t owns, where they rank below lim
+// (G: pair-rank prefixes, valid up to the end of the last needed cell).
+__device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &m, int ss,
+                           int buf, int t, int lim, int gendI, int gendX) {
   const int *cs = d.cell_start + m.c_off;
-  const int p0 = cs[0], np = m.n, t = s - p0;
-  const int lim = y.rlim;
-  if (t >= y.rstop || lim <= 0) return;
+  const int p0 = m.p_off, np = m.n, s = p0 + t;
   const int2 *G = d.gpre + p0;
   const size_t N = (size_t)d.ntot;
   const int W = y.W;
   const int c = d.cellid[ss][s];
   const int f = cs[c] - p0, e = cs[c + 1] - p0;
   const int2 Gf = G[f], Gt = G[t];
-  const int2 Ge = e < np ? G[e] : make_int2(y.gendI, y.gendX);
+  const int2 Ge = e < np ? G[e] : make_int2(gendI, gendX);
   const int base = Gf.x + Gf.y;
   if (base >= lim) return;
   const int oc = d.own_cnt[s];
@@ -1286,26 +1312,28 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   int rI = base + (Gt.x - Gf.x);
   int rX = base + (Ge.x - Gf.x) + (Gt.y - Gf.y);
   if (!((nI > 0 && rI < lim) || (nX > 0 && rX < lim))) return;
-  const double hi = d.hsml[2 * ss + buf][s];
+  const double *HS = d.hsml[2 * ss + buf];
+  const double hi = HS[s];
   const int nn = d.nnb[s];
   if (nn <= kNbrMax) {
     // the owned pairs are the tail of the list: nI intra, then nX cross
     for (int j = nn - nI - nX; j < nn; ++j) {
       const int q = d.nbr[j * N + s];
       int &rank = (j < nn - nX) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = (hi + d.hsml[2 * ss + buf][q]) / 2.;
+      if (rank < lim) d.hq[m.q_off + rank] = (hi + HS[q]) / 2.;
       ++rank;
     }
     return;
   }
+  const double *X = d.x[2 * ss + buf], *Y = d.y[2 * ss + buf];
   const int cy = c / W, cx = c - cy * W;
-  const double xi = d.x[2 * ss + buf][s], yi = d.y[2 * ss + buf][s];
+  const double xi = X[s], yi = Y[s];
   const Ranges R = neighbour_ranges(cs, W, cx, cy);
   for (int g = 2; g < 6; ++g) {
     for (int q = R.a[g]; q < R.b[g]; ++q) {
       if (g == 2 && q <= s) continue;
-      const double dx = xi - d.x[2 * ss + buf][q], dy = yi - d.y[2 * ss + buf][q];
-      const double mh = (hi + d.hsml[2 * ss + buf][q]) / 2.;
+      const double dx = xi - X[q], dy = yi - Y[q];
+      const double mh = (hi + HS[q]) / 2.;
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
       int &rank = (g == 2) ? rI : rX;
@@ -1728,7 +1756,11 @@ void launch_fold(const DevState &d, hipStream_t st) {
 }
 void launch_count(const DevState &d, int buf, hipStream_t st) {
   k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
-  k_hist<<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
+void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st) {
+  const int nb = std::max(cdiv(d.ctot, kScanItems), 1);
+  k_hist_sum<<<g8(std::max(nb, d.nblk)), kTPB, 0, st>>>(d, tmp, nb);
+  k_scan2<<<g8(nb), kTPB, 0, st>>>(d.count, d.ctot, tmp, d.cell_start, 0, nb, d.rebuild);
 }
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st) {
