@@ -73,6 +73,13 @@ struct InstDyn {
   int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
   double hmin;          // MINVAL(hsml) of the current evaluation
+  // candidate lists (kept from the evaluation that built them while they
+  // still cover the pair set): built, their skin, the skin of lists built
+  // in this evaluation (0: none), the bound of 2 max|r - r_built| +
+  // max|hsml - hsml_built| for the state the next evaluation sees, the
+  // largest speed of the last evaluated state
+  int vbuilt;
+  float vskin, vskin_next, vmoved, vprev;
   int sel;              // state set holding the instance at the start of an evaluation
   int rb;               // 1: its lists are rebuilt in this evaluation (the state
                         //    then moves to set sel ^ 1: the current set is sel ^ rb)
@@ -113,7 +120,11 @@ struct DevState {
   // j-th neighbour of position s; nnb[s] = count (> kNbrMax: not stored,
   // the gathers rescan the 3x3 block)
   int *nbr, *nnb;
-  double *part;          // per-block partial reductions (5 x nblk)
+  // candidate lists: per slot, the list positions of the 3x3 block within
+  // mhsml + skin at the evaluation that built them (j-major like nbr, at
+  // most kNbrMax; vnc > kNbrMax: not stored)
+  int *vnb, *vnc;
+  double *part;          // per-block partial reductions (7 x nblk)
   int *bflag;            // per cell: stamp of the last evaluation in which a
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;

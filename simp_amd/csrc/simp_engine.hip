@@ -85,10 +85,17 @@ static int store_capacity(const InstanceInput &in) {
 }
 
 // bytes per particle / boundary particle / cell of the device layout
-constexpr size_t kBytesPerParticle = 2 * 8 * 8 + 8 + 4 + 6 * 4 + 7 * 8 + 4 + 16 +
-                                     kNbrMax * 4 + 4 + 4 * 8 + 6 * 8 + 3 * 8 + 6 * 8 + 3 * 8 + 8;
+constexpr size_t kBytesPerParticle =
+    4 * 8 * 8 +             // x .. hsml, two sets of S and T
+    2 * (8 + 4 + 4) +       // mass, orig, cellid per set
+    2 * 4 + 2 * 4 + 8 +     // slot, cl; own_cnt, moff; gpre
+    16 +                    // pf
+    kNbrMax * 4 + 4 +       // nbr, nnb
+    kNbrMax * 4 + 4 +       // vnb, vnc
+    4 * 8 + 3 * 8 +         // sig11/12/22, div; kz, dnum, dsum
+    6 * 8 + 3 * 8 + 6 * 8;  // diag, sig_out, tend
 constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4 + 9 * 4 + 4;
-constexpr size_t kBytesPerCell = 11 * 4;
+constexpr size_t kBytesPerCell = 9 * 4 + 8 + 8;
 
 size_t instance_footprint(const InstanceInput &in) {
   long c = cell_capacity(in);
@@ -146,7 +153,10 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.pf, N);
   rc |= dalloc(list, &d.nbr, (size_t)kNbrMax * N);
   rc |= dalloc(list, &d.nnb, N);
-  rc |= dalloc(list, &d.part, 5 * BLK);
+  rc |= dalloc(list, &d.vnb, (size_t)kNbrMax * N);
+  rc |= dalloc(list, &d.vnc, N);
+
+  rc |= dalloc(list, &d.part, 7 * BLK);
   rc |= dalloc(list, &d.bflag, C);
   rc |= dalloc(list, &d.count, C);
   rc |= dalloc(list, &d.fill, C);
@@ -359,6 +369,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     const int cs = kdyn[j].sel ^ kdyn[j].rb;
     kdyn[j].sel = 0;
     kdyn[j].rb = 0;
+    kdyn[j].vbuilt = 0;     // candidate lists are built again
+    kdyn[j].vskin_next = 0.f;
     RCHK(d2d(nd.x[0] + m.p_off, d_.x[2 * cs] + o.p_off, n8));
     RCHK(d2d(nd.y[0] + m.p_off, d_.y[2 * cs] + o.p_off, n8));
     RCHK(d2d(nd.vx[0] + m.p_off, d_.vx[2 * cs] + o.p_off, n8));

@@ -193,29 +193,33 @@ __device__ __forceinline__ bool inst_off(const InstDyn &y) { return y.err != 0 |
 
 // ------------------------------------------------------ reductions / grid
 
-// Per-block min/max of x, y and min of hsml (invalid lanes give neutral
-// values), stored at the block's logical id.  Non-finite input raises the
-// instance's NaN flag.
+// Per-block min/max of x, y, min of hsml, and max of the speed and of the
+// hsml change (the candidate lists' movement bound; invalid lanes give
+// neutral values), stored at the block's logical id.  Non-finite input
+// raises the instance's NaN flag.
+constexpr int kParts = 7;
+
 __device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok, double xv,
-                                               double yv, double hv) {
-  __shared__ double sh[5][kTPB / 64];
+                                               double yv, double hv, double sp, double dh) {
+  __shared__ double sh[kParts][kTPB / 64];
   bool bad = ok && (!(xv == xv) || !(yv == yv) || !(hv > 0.0) || __builtin_isinf(xv) ||
                     __builtin_isinf(yv) || __builtin_isinf(hv));
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.dyn[k].err, kErrNan);
   const double big = __builtin_huge_val();
-  double v0 = wave_min(ok ? xv : big), v1 = wave_max(ok ? xv : -big);
-  double v2 = wave_min(ok ? yv : big), v3 = wave_max(ok ? yv : -big);
-  double v4 = wave_min(ok ? hv : big);
+  double v[kParts];
+  v[0] = wave_min(ok ? xv : big); v[1] = wave_max(ok ? xv : -big);
+  v[2] = wave_min(ok ? yv : big); v[3] = wave_max(ok ? yv : -big);
+  v[4] = wave_min(ok ? hv : big);
+  v[5] = wave_max(ok ? sp : 0.0); v[6] = wave_max(ok ? dh : 0.0);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
-  }
+  if ((threadIdx.x & 63) == 0)
+    for (int f = 0; f < kParts; ++f) sh[f][w] = v[f];
   __syncthreads();
-  if (threadIdx.x < 5) {
+  if (threadIdx.x < kParts) {
     const int f = threadIdx.x;
+    const bool mx = f == 1 || f == 3 || f >= 5;
     double r = sh[f][0];
-    for (int i = 1; i < kTPB / 64; ++i)
-      r = (f == 1 || f == 3) ? fmax(r, sh[f][i]) : fmin(r, sh[f][i]);
+    for (int i = 1; i < kTPB / 64; ++i) r = mx ? fmax(r, sh[f][i]) : fmin(r, sh[f][i]);
     d.part[(size_t)f * d.nblk + d.blk[blockIdx.x].lid] = r;
   }
 }
@@ -334,11 +338,12 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   const int cs = cur_set(d.dyn[k]);
   double xv = ok ? d.x[2 * cs + buf][p] : 0.0, yv = ok ? d.y[2 * cs + buf][p] : 0.0;
   double hv = ok ? d.hsml[2 * cs + buf][p] : 1.0;
-  block_partials(d, k, ok, xv, yv, hv);
+  block_partials(d, k, ok, xv, yv, hv, 0.0, 0.0);
 }
 
 __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
                              double ymn, double ymx, double hmn);
+
 
 // end of an evaluation: the current set becomes the instance's set
 __device__ __forceinline__ void fold_set(InstDyn &y) {
@@ -346,9 +351,34 @@ __device__ __forceinline__ void fold_set(InstDyn &y) {
   y.rb = 0;
 }
 
+// end of an evaluation of a step: lists pass A built in it are valid from
+// now on (a finalize that follows no pass A -- the first grid of a new
+// layout -- changes nothing)
+__device__ __forceinline__ void lists_built(InstDyn &y) {
+  if (y.vskin_next > 0.f) {
+    y.vskin = y.vskin_next;
+    y.vskin_next = 0.f;
+    y.vmoved = 0.f;          // built at the state just evaluated
+    y.vbuilt = 1;
+  }
+}
+
+// The next state moved from the evaluated one by at most 1.5 dt times the
+// largest speed of this or the previous evaluation's state (predictor: dt/2
+// v_S; corrector: dt v_T - dt/2 v_S), hsml by at most the largest change;
+// 2 |dr| + |dh| adds to the lists' movement bound (rounded up).
+__device__ __forceinline__ void move_bound(InstDyn &y, double vmax, double dhmax) {
+  const double v = fmax(vmax, (double)y.vprev);
+  y.vmoved = (float)((double)y.vmoved + 3.0 * y.dt * v + dhmax) * 1.00001f;
+  y.vprev = (float)vmax * 1.00001f;
+}
+
 __global__ void __launch_bounds__(kTPB) k_fold(DevState d) {
   const int k = blockIdx.x * kTPB + threadIdx.x;
-  if (k < d.ninst) fold_set(d.dyn[k]);
+  if (k < d.ninst) {
+    fold_set(d.dyn[k]);
+    d.dyn[k].vbuilt = 0;    // no movement measured for the next state
+  }
 }
 
 // One block per instance: fold the instance's block partials into the grid
@@ -357,25 +387,33 @@ __global__ void __launch_bounds__(kTPB) k_fold(DevState d) {
 // (time_step.f90:155), itimestep + 1 and current_time + dt
 // (time_integration.f90:68-71).
 __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) {
-  __shared__ double sh[5][kTPB / 64];
+  __shared__ double sh[kParts][kTPB / 64];
   if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
-  if (threadIdx.x == 0) fold_set(d.dyn[k]);
+  if (threadIdx.x == 0) {
+    fold_set(d.dyn[k]);
+    if (!inst_off(d.dyn[k])) lists_built(d.dyn[k]);
+  }
+
   const double big = __builtin_huge_val();
-  double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big;
+  double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big, v5 = 0.0, v6 = 0.0;
+  const size_t nb = (size_t)d.nblk;
   for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
     v0 = fmin(v0, d.part[b]);
-    v1 = fmax(v1, d.part[(size_t)d.nblk + b]);
-    v2 = fmin(v2, d.part[2 * (size_t)d.nblk + b]);
-    v3 = fmax(v3, d.part[3 * (size_t)d.nblk + b]);
-    v4 = fmin(v4, d.part[4 * (size_t)d.nblk + b]);
+    v1 = fmax(v1, d.part[nb + b]);
+    v2 = fmin(v2, d.part[2 * nb + b]);
+    v3 = fmax(v3, d.part[3 * nb + b]);
+    v4 = fmin(v4, d.part[4 * nb + b]);
+    v5 = fmax(v5, d.part[5 * nb + b]);
+    v6 = fmax(v6, d.part[6 * nb + b]);
   }
   v0 = wave_min(v0); v1 = wave_max(v1); v2 = wave_min(v2); v3 = wave_max(v3);
-  v4 = wave_min(v4);
+  v4 = wave_min(v4); v5 = wave_max(v5); v6 = wave_max(v6);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
+    sh[5][w] = v5; sh[6][w] = v6;
   }
   __syncthreads();
   __shared__ int go;
@@ -386,7 +424,10 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
       v2 = fmin(sh[2][0], sh[2][i]); sh[2][0] = v2;
       v3 = fmax(sh[3][0], sh[3][i]); sh[3][0] = v3;
       v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
+      v5 = fmax(sh[5][0], sh[5][i]); sh[5][0] = v5;
+      v6 = fmax(sh[6][0], sh[6][i]); sh[6][0] = v6;
     }
+    move_bound(d.dyn[k], sh[5][0], sh[6][0]);
     go = finalize_grid(d, k, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0]);
   }
   __syncthreads();
@@ -792,6 +833,8 @@ struct KSum {
 // margin folded in)
 constexpr int kTile = 1280;
 constexpr float kPad = 1.000005f;
+// candidate-list skin, relative to MINVAL(hsml)
+constexpr float kSkin = 0.01f;
 constexpr float kHalfPad = 0.5f * kPad;
 
 // SUMD: instances with summation density may be present (the density
@@ -810,45 +853,61 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const int W = y.W;
   const int *cs0 = d.cell_start + m.c_off;
   // the evaluated state, by slot (= list position)
-  const int cs = cur_set(y);
-  const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
-  const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
+  const int ss = cur_set(y), sb = 2 * ss + buf;
+  const double *X = d.x[sb], *Y = d.y[sb], *HS = d.hsml[sb], *RS = d.rho[sb];
+  const double *VX = d.vx[sb], *VY = d.vy[sb], *MS = d.mass[ss];
   const int pos0 = be.pos;                // first slot of the block
   const int nloc = be.nloc;
   const bool valid = (int)threadIdx.x < nloc;
   const int self = pos0 + min((int)threadIdx.x, nloc - 1);
+  const size_t N = (size_t)d.ntot;
+  // Candidate lists: built with the prefilter below (every candidate within
+  // mhsml + skin), they still cover the pair set while the cell lists are
+  // unchanged (same cells, same visiting order) and every pair's distance
+  // and mhsml moved by less than the skin: 2 max|r - r_built| +
+  // max|hsml - hsml_built| < skin, bounded by finalize from the largest
+  // speed and hsml change of every evaluation since.  Then the prefilter
+  // is skipped.
+  const bool vok = y.vbuilt && y.rb == 0 && y.vmoved < 0.9f * y.vskin;
+  const float skin = kSkin * (float)y.hmin;
   // The candidates of every particle of the block lie in three runs of the
   // cell lists: its cell range shifted by -W, 0 and +W cells, widened by one
   // cell on each side.  Overlapping runs (a block spanning most of a row)
   // are merged.  The runs' prefilter data goes to LDS in one round of
   // coalesced loads by the whole block; a block whose region does not fit
   // takes the exact rescan path instead.
-  int As, Ae, Bs, Be, Cs, Ce;
-  {
-    const int cf = d.cellid[cs][pos0], cl = d.cellid[cs][pos0 + nloc - 1];
+  int As = 0, Bs = 0, Cs = 0, tB = 0, tC = 0;
+  bool tiled = false;
+  if (!vok) {
+    int Ae, Be, Ce;
+    const int cf = d.cellid[ss][pos0], cl = d.cellid[ss][pos0 + nloc - 1];
     As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
     Bs = cs0[cf - 1];     Be = cs0[cl + 2];
     Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
     if (Ae >= Bs) { Bs = As; Be = max(Ae, Be); As = Ae = 0x7fffffff; }
     if (Be >= Cs) { Cs = min(Bs, Cs); Ce = max(Be, Ce); Bs = Be = 0x7fffffff; }
-  }
-  const int lA = Ae - As, lB = Be - Bs, lC = Ce - Cs;
-  const int tB = lA, tC = lA + lB, T = lA + lB + lC;
-  const bool tiled = T <= kTile;      // list entries are tile indices (< kTile)
-  if (tiled) {
-    for (int i = threadIdx.x; i < T; i += kTPB) {
-      const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
-      float4 v = d.pf[s];
-      v.z *= kHalfPad;
-      tile[i] = v;
+    const int lA = Ae - As, lB = Be - Bs, lC = Ce - Cs;
+    tB = lA;
+    tC = lA + lB;
+    const int T = lA + lB + lC;
+    tiled = T <= kTile;      // list entries are tile indices (< kTile)
+    if (tiled) {
+      for (int i = threadIdx.x; i < T; i += kTPB) {
+        const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
+        float4 v = d.pf[s];
+        v.z *= kHalfPad;
+        tile[i] = v;
+      }
     }
+    __syncthreads();
+    // lists built in this evaluation carry this skin (any block may say so)
+    if (threadIdx.x == 0) d.dyn[k].vskin_next = skin;
   }
-  __syncthreads();
   if (!valid) return;
   // waves with large supports (more neighbours) get issue priority: they
   // set the kernel's length
   if (__any(HS[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
-  const int c = d.cellid[cs][self];
+  const int c = d.cellid[ss][self];
   const int cy = c / W, cx = c - cy * W;
   const double xi = X[self], yi = Y[self], hi = HS[self];
   const double vxi = VX[self], vyi = VY[self];
@@ -908,7 +967,9 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   //    candidates, which also writes the neighbour list.  Splitting the two
   //    keeps the expensive arithmetic off the ~90 % of candidates that fail.
   int nI = 0, nX = 0, nn = 0, nc = 0;
-  if (!tiled) {
+  if (vok) {
+    nc = d.vnc[self];
+  } else if (!tiled) {
     nc = kNbrMax + 1;
   } else {
     // tile index of a list position (every position of the block's 3x3
@@ -936,7 +997,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     auto need = [&](int c, float ex, float ey) {
       const unsigned long long hc = ch[c];
       const float hmax = (unsigned)(hc >> 32) == stampu ? __uint_as_float((unsigned)hc) : 0.0f;
-      const float bnd = __builtin_fmaf(hif + hmax, 0.5f, slack);
+      const float bnd = __builtin_fmaf(hif + hmax, 0.5f, slack + skin);
       return __builtin_fmaf(ex, ex, ey * ey) <= bnd * bnd * 1.00001f;
     };
     // The four segments, in visiting order: row below (cells rm-1, rm,
@@ -964,8 +1025,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     const int o1 = tix(a1) - e0;
     const int o2 = (b2 > a2 ? tix(a2) : 0) - e1;
     const int o3 = (b3 > a3 ? tix(a3) : 0) - e2;
-    // (mhsml + slack) sqrt(1.00001) = hzi + the candidate's scaled hsml
-    const float hzi = __builtin_fmaf(slack, kPad, pi4.z);
+    // (mhsml + slack + skin) sqrt(1.00001) = hzi + the candidate's scaled hsml
+    const float hzi = __builtin_fmaf(slack + skin, kPad, pi4.z);
     for (int j = 0; j < e3; ++j) {
       const int ti = j + (j < e0 ? o0 : (j < e1 ? o1 : (j < e2 ? o2 : o3)));
       const float4 q = tile[ti];
@@ -982,7 +1043,16 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   }
   // list entry (tile index) -> list position
   auto lpos = [&](int ti) { return ti < tB ? As + ti : (ti < tC ? Bs + ti - tB : Cs + ti - tC); };
-  const size_t N = (size_t)d.ntot;
+  // list position of candidate j
+  auto cand = [&](int j) {
+    return vok ? d.vnb[j * N + self] : lpos((int)lst[j * kTPB + threadIdx.x]);
+  };
+  if (!vok) {
+    // this evaluation's candidate lists, kept for the next ones
+    d.vnc[self] = nc;
+    if (nc <= kNbrMax)
+      for (int j = 0; j < nc; ++j) d.vnb[j * N + self] = cand(j);
+  }
   // kernel_zero(i) (kernel.f90:503): the factor of the last pair in the
   // reference's order that has this particle first, i.e. its last owned
   // pair; particles that own none keep the value of an earlier evaluation.
@@ -990,11 +1060,11 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // walk finds it.  It must be known before the density sums start.
   double kzi = 0.;
   if (sumd) {
-    kzi = d.kz[m.p_off + d.orig[cs][self]];
+    kzi = d.kz[m.p_off + d.orig[ss][self]];
     bool found = false;
     if (nc <= kNbrMax) {
       for (int j = nc - 1; j >= 0 && !found; --j) {
-        const int t = lpos((int)lst[j * kTPB + threadIdx.x]);
+        const int t = cand(j);
         if (pair_kind(R, t, self) == 0) break;
         const double dx = xi - X[t], dy = yi - Y[t];
         const double mh = (hi + HS[t]) / 2.;
@@ -1027,7 +1097,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       int sj[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u)
-        sj[u] = j0 + u < nc ? lpos((int)lst[(j0 + u) * kTPB + threadIdx.x]) : self;
+        sj[u] = j0 + u < nc ? cand(j0 + u) : self;
       // positions and pair data of the batch in one round of loads
       double xq[kBatch], yq[kBatch], hq[kBatch], mq[kBatch], rq[kBatch], uq[kBatch], wq[kBatch];
 #pragma unroll
@@ -1083,7 +1153,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     g1 = f1.value(); g2 = f2.value(); g3 = f3.value(); g4 = f4.value();
   }
   if (sumd) {
-    d.kz[m.p_off + d.orig[cs][self]] = kzi;
+    d.kz[m.p_off + d.orig[ss][self]] = kzi;
     d.dnum[self] = num;
     d.dsum[self] = num / den;
   }
@@ -1111,7 +1181,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // replacement pressure (eos.f90:141-147), stress tensor
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
-  const double hth = d.h[2 * cs + buf][p], Ac = d.A[2 * cs + buf][p];
+  const double hth = d.h[sb][p], Ac = d.A[sb][p];
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1381,6 +1451,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   const bool live = ok && !inst_off(y);
   const int buf = phase == 1 ? 1 : 0;
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
+  double speed = 0.0, dhs = 0.0;
   if (live) {
     const int W = y.W;
     const size_t N = (size_t)d.ntot;
@@ -1412,6 +1483,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     int nx[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) nx[u] = d.nbr[u * N + self];
+
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
     const bool sd = SUMD && m.den_alg == 1;
@@ -1628,6 +1700,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
       ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
     }
+    // the candidate lists' movement bound: speed of the evaluated state and
+    // hsml change of the new one
+    speed = sqrt(vxi * vxi + vyi * vyi);
+    dhs = fabs(hsml_new - hi);
     // velocities (time_integration.f90:210-211, 240-241) or tendencies
     if (phase == 0) {
       d.vx[2 * cs + 1][p] = vxi + ax * dt / 2;
@@ -1656,7 +1732,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       d.tend[5 * N + p] = dAdt;
     }
   }
-  if (phase != 2) block_partials(d, k, live, px_new, py_new, hsml_new);
+  if (phase != 2) block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
 }
 
 // ------------------------------------------------------- pair-list export

@@ -113,6 +113,27 @@ def test_time_integration_fp64_vs_oracle_gridded_wind():
         assert e <= 1e-12, (k, e)
 
 
+def test_candidate_lists_across_their_skin():
+    """Fast random motion (10 m/s, fixed dt 0.05 s): the candidate lists'
+    movement bound (3 dt max|v| per evaluation against a skin of
+    0.01 MINVAL(hsml) = 26 m) lets them be reused for a few evaluations and
+    rebuilt in between, while pairs cross mhsml every step; the pair sets,
+    and so the fields, must stay those of the oracle."""
+    say("candidate lists: box2500, fast random motion, 10 steps")
+    sc = S.box(seed=31)
+    rng = np.random.default_rng(5)
+    sc.vx = rng.normal(0.0, 10.0, size=sc.vx.shape)
+    sc.tse = 0
+    sc.dt = 0.05
+    sc.time = 1.0
+    a, b = sc.copy(), sc.copy()
+    Simp.for_scenario(a).time_integration(a, 10)
+    capi.Oracle().run_pc(b, 10)
+    for k in ("x", "vx", "rho", "hsml", "A", "h"):
+        e = nrel(getattr(a, k), getattr(b, k))
+        assert e <= 1e-12, (k, e)
+
+
 def test_graph_replay_equals_eager_launches():
     say("graph vs eager, box2500 6 steps")
     sc = S.box(seed=21)
