@@ -904,6 +904,14 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     if (threadIdx.x == 0) d.dyn[k].vskin_next = skin;
   }
   if (!valid) return;
+  // kept lists: the count and the first kBatch entries go out with the
+  // particle's own loads (entries past the count are never dereferenced)
+  int vcount = 0, nx[kBatch];
+  if (vok) {
+    vcount = d.vnc[self];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[u * N + self];
+  }
   // waves with large supports (more neighbours) get issue priority: they
   // set the kernel's length
   if (__any(HS[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
@@ -912,6 +920,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const double xi = X[self], yi = Y[self], hi = HS[self];
   const double vxi = VX[self], vyi = VY[self];
   const double rhoi = SUMD ? RS[self] : 0.0;
+  // the rheology's thickness and concentration, loaded with the rest
+  const double hth_i = d.h[sb][self], A_i = d.A[sb][self];
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   const int p = self;
@@ -968,7 +978,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   //    keeps the expensive arithmetic off the ~90 % of candidates that fail.
   int nI = 0, nX = 0, nn = 0, nc = 0;
   if (vok) {
-    nc = d.vnc[self];
+    nc = vcount;
   } else if (!tiled) {
     nc = kNbrMax + 1;
   } else {
@@ -1097,7 +1107,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       int sj[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u)
-        sj[u] = j0 + u < nc ? cand(j0 + u) : self;
+        sj[u] = j0 + u < nc ? (vok ? nx[u] : cand(j0 + u)) : self;
       // positions and pair data of the batch in one round of loads
       double xq[kBatch], yq[kBatch], hq[kBatch], mq[kBatch], rq[kBatch], uq[kBatch], wq[kBatch];
 #pragma unroll
@@ -1109,6 +1119,12 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         rq[u] = RS[sj[u]];
         uq[u] = VX[sj[u]];
         wq[u] = VY[sj[u]];
+      }
+      // kept lists: the next batch's entries go out with this batch's data
+      // (past the end, the last entry again)
+      if (vok) {
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[min(j0 + kBatch + u, nc - 1) * N + self];
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
@@ -1181,7 +1197,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // replacement pressure (eos.f90:141-147), stress tensor
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
-  const double hth = d.h[sb][p], Ac = d.A[sb][p];
+  const double hth = hth_i, Ac = A_i;
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
