@@ -27,17 +27,18 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per particle of one launch of each heavy kernel
-# (compulsory unique traffic, neighbour reads counted once).
-#   passA : state x y hsml vx vy mass rho h A (72) + cellid clpos (8)
-#           -> sigma (3 x 8 | 3 x 4) + div (8) + own_cnt (4)
-#   passB : RHS state x y hsml rho mass h A vx vy (72) + sigma/div (32 | 20)
-#           + cellid clpos own_cnt (12) + step-start x y vx vy h A (48, corrector)
-#           -> x y vx vy h A rho hsml (64)
+# (compulsory unique traffic, neighbour reads counted once, the per-stage
+# figures of the step's byte budget below):
+#   passA : x vx (32) mass hsml h A (32) -> sigma (3 x 8 | 3 x 4) + div (8)
+#   passB : x vx mass h A (56) + sigma (24 | 12) + div (8) -> x vx h A (48),
+#           the corrector also reading the step-start x vx h A (48) and
+#           writing hsml (8); one launch of each phase per step, so the
+#           per-launch figure is their mean
 ALG_BYTES = {
-    ("passA", 0): 72 + 8 + 24 + 8 + 4,
-    ("passA", 1): 72 + 8 + 12 + 8 + 4,
-    ("passB", 0): 72 + 32 + 12 + 48 + 64,
-    ("passB", 1): 72 + 20 + 12 + 48 + 64,
+    ("passA", 0): 64 + 24 + 8,
+    ("passA", 1): 64 + 12 + 8,
+    ("passB", 0): (136 + 192) / 2,
+    ("passB", 1): (124 + 180) / 2,
 }
 # whole step: 576 B (FP64) / 512 B (mixed) per particle-step
 STEP_BYTES = {0: 576, 1: 512}
@@ -348,7 +349,8 @@ def main():
     if not args.no_serve:
         line["serve"] = bench_serve()
     if not args.no_traffic:
-        kname = "k_passB<true, false>" if name == "passB" else "k_passA<true, false>"
+        # (pass B: both phase kernels, k_passB<true, false, 0|1>)
+        kname = "k_passB<true, false," if name == "passB" else "k_passA<true, false>"
         traffic, info = hbm_traffic(kname)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_note"] = info
