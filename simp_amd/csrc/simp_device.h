@@ -72,6 +72,7 @@ struct InstDyn {
   int lstamp;           // stamp of the evaluation that built the boundary cell lists
   int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
+  int rchg;             // 1: the pair-rank prefixes must be recomputed
   double hmin;          // MINVAL(hsml) of the current evaluation
   // candidate lists (kept from the evaluation that built them while they
   // still cover the pair set): built, their skin, the skin of lists built

@@ -370,6 +370,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     kdyn[j].sel = 0;
     kdyn[j].rb = 0;
     kdyn[j].vbuilt = 0;     // candidate lists are built again
+    kdyn[j].rchg = 1;       // and the pair-rank prefixes
     kdyn[j].vskin_next = 0.f;
     RCHK(d2d(nd.x[0] + m.p_off, d_.x[2 * cs] + o.p_off, n8));
     RCHK(d2d(nd.y[0] + m.p_off, d_.y[2 * cs] + o.p_off, n8));
@@ -453,6 +454,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     y.itimestep = in.itimestep;
     y.active = 1;
     y.steps_left = in.nsteps > 0 ? in.nsteps : -1;
+    y.rchg = 1;
   }
   if (na > 0) RCHK(h2d(nd.dyn + nk, ndyn.data(), sizeof(InstDyn) * na));
   RCHK(h2d(nd.meta, nm.data(), sizeof(InstMeta) * ninst));

@@ -920,8 +920,13 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const double xi = X[self], yi = Y[self], hi = HS[self];
   const double vxi = VX[self], vyi = VY[self];
   const double rhoi = SUMD ? RS[self] : 0.0;
-  // the rheology's thickness and concentration, loaded with the rest
-  const double hth_i = d.h[sb][self], A_i = d.A[sb][self];
+  // the rheology's thickness and concentration, loaded with the rest (the
+  // FP64 variant, short of registers, loads them where they are used)
+  double hth_i = 0., A_i = 0.;
+  if (MIXED) {
+    hth_i = d.h[sb][self];
+    A_i = d.A[sb][self];
+  }
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   const int p = self;
@@ -1188,7 +1193,14 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     }
   }
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
-  d.own_cnt[self] = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
+  const int oc_new = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
+  const int oc_old = d.own_cnt[self];     // the previous evaluation's pair counts
+  // the pair ranks k_ranks keeps depend on the owned-pair counts of the
+  // list positions up to its stop: a change there has them recomputed
+  if (__any(((oc_new ^ oc_old) & 0xfffff) != 0 && self - m.p_off <= y.rstop) &&
+      ((int)threadIdx.x & 63) == 0)
+    atomicOr(&d.dyn[k].rchg, 1);
+  d.own_cnt[self] = oc_new;
 
   // strain rate (single_step.f90:80-84)
   const double e11 = g1, e22 = g4, e12 = 0.5 * (g2 + g3);
@@ -1197,7 +1209,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // replacement pressure (eos.f90:141-147), stress tensor
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
-  const double hth = hth_i, Ac = A_i;
+  const double hth = MIXED ? hth_i : d.h[sb][self], Ac = MIXED ? A_i : d.A[sb][self];
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1316,6 +1328,10 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
     if (threadIdx.x == 0) d.dyn[k].rlim = 0;
     return;
   }
+  // the prefixes of the last evaluation still hold when the limit, the
+  // slots (no rebuild) and the owned-pair counts before its stop are the
+  // same (pass A flags a change there)
+  if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) return;
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n;
   int2 *G = d.gpre + p0;       // per position: exclusive prefixes of intra, cross counts
@@ -1358,6 +1374,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
   }
   if (threadIdx.x == 0) {
     InstDyn &yw = d.dyn[k];
+    yw.rchg = 0;
     yw.rlim = lim;
     yw.rstop = sh_stop;
     yw.gendI = baseI;   // prefixes at np, when the loop reached it
