@@ -73,6 +73,7 @@ struct InstDyn {
   int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
   int rchg;             // 1: the pair-rank prefixes must be recomputed
+  int rput;             // 1: k_rankput writes the hsml_ij store in this evaluation
   double hmin;          // MINVAL(hsml) of the current evaluation
   // candidate lists (kept from the evaluation that built them while they
   // still cover the pair set): built, their skin, the skin of lists built

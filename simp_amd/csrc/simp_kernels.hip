@@ -1285,7 +1285,7 @@ constexpr int kRankThreads = 1024;
 __device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &m, int ss,
                            int buf, int t, int lim, int gendI, int gendX);
 
-__global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
+__global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
   __shared__ int sh_stop;
   if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
@@ -1325,13 +1325,21 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
   }
   const int lim = min(bnip, m.q_cap);
   if (lim <= 0) {
-    if (threadIdx.x == 0) d.dyn[k].rlim = 0;
+    if (threadIdx.x == 0) {
+      d.dyn[k].rlim = 0;
+      d.dyn[k].rput = 0;
+    }
     return;
   }
   // the prefixes of the last evaluation still hold when the limit, the
   // slots (no rebuild) and the owned-pair counts before its stop are the
-  // same (pass A flags a change there)
-  if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) return;
+  // same (pass A flags a change there); the corrector's half-step state
+  // then also has the predictor's hsml, so the store the predictor's
+  // evaluation wrote holds too
+  if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) {
+    if (threadIdx.x == 0) d.dyn[k].rput = buf == 0;
+    return;
+  }
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n;
   int2 *G = d.gpre + p0;       // per position: exclusive prefixes of intra, cross counts
@@ -1375,6 +1383,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d) {
   if (threadIdx.x == 0) {
     InstDyn &yw = d.dyn[k];
     yw.rchg = 0;
+    yw.rput = 1;
     yw.rlim = lim;
     yw.rstop = sh_stop;
     yw.gendI = baseI;   // prefixes at np, when the loop reached it
@@ -1391,7 +1400,7 @@ __global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
   const InstMeta &m = d.meta[k];
   if (m.nb == 0 || inst_off(y)) return;
   const int t = s - m.p_off;
-  if (t >= y.rstop || y.rlim <= 0) return;
+  if (t >= y.rstop || y.rlim <= 0 || !y.rput) return;
   rank_store(d, y, m, cur_set(y), buf, t, y.rlim, y.gendI, y.gendX);
 }
 
@@ -1900,7 +1909,7 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
   k_pairtot<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d);
 }
 void launch_ranks(const DevState &d, int buf, hipStream_t st) {
-  k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d);
+  k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d, buf);
   k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 template <bool MIXED, bool SUMD>
