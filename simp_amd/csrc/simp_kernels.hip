@@ -481,13 +481,46 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
 
 // ------------------------------------------------------------------ count
 
-// Cell key of every slot (bucket_search, nnps.f90:249-250), the FP32
-// prefilter positions of the evaluation, and per cell the largest member
-// smoothing length (rounded up to FP32), the bound the gathers' cell
-// culling uses: a max over each run of equal keys in the wavefront (slots
-// are in cell order, but for particles that just changed cell), then one
-// 64-bit atomic max per run of (evaluation stamp, value) -- a stale stamp
-// reads as an empty cell, so nothing needs clearing.
+// The inputs of pass A's prefilter for slot p (lanes with want; every lane
+// of the wavefront takes part): its FP32 position relative to the grid
+// origin and hsml, and per cell the largest member smoothing length
+// (rounded up to FP32), the bound the prefilter's cell culling uses: a max
+// over each run of equal keys in the wavefront (slots are in cell order,
+// but for particles that just changed cell), then one 64-bit atomic max
+// per run of (evaluation stamp, value) -- a stale stamp reads as an empty
+// cell, so nothing needs clearing.
+__device__ __forceinline__ void prefilter_inputs(const DevState &d, bool want, int k, int p,
+                                                 int key, double xq, double yq, double hq) {
+  const int lane = (int)threadIdx.x & 63;
+  int kk = -1 - lane, coff = 0;
+  unsigned long long hv = 0;
+  if (want) {
+    const InstDyn &y = d.dyn[k];
+    const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
+    d.pf[p] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
+    kk = key;
+    coff = d.meta[k].c_off;
+    hv = ((unsigned long long)(unsigned)y.stamp << 32) | __float_as_uint((float)hq * 1.000001f);
+  }
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned lo = __shfl_down((unsigned)hv, o, 64);
+    const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
+    const int ok = __shfl_down(kk, o, 64);
+    const unsigned long long ov = ((unsigned long long)hi << 32) | lo;
+    if (lane + o < 64 && ok == kk && ov > hv) hv = ov;
+  }
+  const int prev = __shfl_up(kk, 1, 64);
+  if (want && (lane == 0 || prev != kk)) atomicMax(&d.cellh[coff + kk], hv);
+}
+
+// pass A runs its prefilter in this evaluation unless the cell lists are
+// rebuilt (then the state move supplies the prefilter's inputs)
+__device__ __forceinline__ bool lists_stale(const InstDyn &y) {
+  return !y.vbuilt || !(y.vmoved < 0.9f * y.vskin);
+}
+
+// Cell key of every slot (bucket_search, nnps.f90:249-250), and the
+// prefilter's inputs when pass A will prefilter.
 // The member counts of the cells follow the keys that change.  An
 // instance's lists are rebuilt only when one of its keys differs from the
 // one its slot was sorted with (or on its first evaluation); the rebuild
@@ -499,15 +532,17 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   bool changed = false;
   const int lane = (int)threadIdx.x & 63;
   int key = -1 - lane, oldk = -1 - lane;
-  unsigned long long hv = 0;
   const bool valid = particle_of(d, p, k) && !inst_off(d.dyn[k]);
   int coff = 0;
+  double xq = 0., yq = 0., hq = 0.;
   if (valid) {
     const InstDyn &y = d.dyn[k];
     const int cs = y.sel;
     coff = d.meta[k].c_off;
     const double bin = y.bin;
-    const double xq = d.x[2 * cs + buf][p], yq = d.y[2 * cs + buf][p], hq = d.hsml[2 * cs + buf][p];
+    xq = d.x[2 * cs + buf][p];
+    yq = d.y[2 * cs + buf][p];
+    hq = d.hsml[2 * cs + buf][p];
     const int cx = nint(xq / bin) - y.xlo + 1;
     const int cy = nint(yq / bin) - y.ylo + 1;
     const int c = cy * y.W + cx;
@@ -517,11 +552,7 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
       d.cellid[cs][p] = c;
       oldk = old;   // < 0 on the first evaluation
     }
-    const double ox = (y.xlo - 1) * bin, oy = (y.ylo - 1) * bin;
-    d.pf[p] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
     key = c;
-    hv = ((unsigned long long)(unsigned)y.stamp << 32) |
-         __float_as_uint((float)hq * 1.000001f);
   }
   // blocks never straddle instances: one instance per wavefront
   if (__any(changed) && lane == 0) {
@@ -532,15 +563,7 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
     run_add(d.count + coff, changed ? key : -1 - lane, 1);
     run_add(d.count + coff, oldk, -1);
   }
-  for (int o = 1; o < 64; o <<= 1) {
-    const unsigned lo = __shfl_down((unsigned)hv, o, 64);
-    const unsigned hi = __shfl_down((unsigned)(hv >> 32), o, 64);
-    const int ok = __shfl_down(key, o, 64);
-    const unsigned long long ov = ((unsigned long long)hi << 32) | lo;
-    if (lane + o < 64 && ok == key && ov > hv) hv = ov;
-  }
-  const int prev = __shfl_up(key, 1, 64);
-  if (valid && (lane == 0 || prev != key)) atomicMax(&d.cellh[coff + key], hv);
+  prefilter_inputs(d, valid && lists_stale(d.dyn[k]), k, p, key, xq, yq, hq);
 }
 
 // Slot of each particle of a rebuilt instance in its new cell: consecutive
@@ -694,27 +717,34 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
 // the evaluation (buffer buf).  The half-step buffer moves only when it is
 // live (the corrector's evaluation).
 __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
-  if (*d.rebuild == 0) return;
+  if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
+  if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
   int s = 0, k = 0;
-  if (!particle_of(d, s, k)) return;
-  const InstDyn &y = d.dyn[k];
-  if (y.rb == 0) return;
-  const int f = y.sel, t = y.sel ^ 1;
-  const int q = d.cl[s];
-  for (int b = 0; b <= buf; ++b) {
-    const int i = 2 * f + b, o = 2 * t + b;
-    d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
-    d.vx[o][s] = d.vx[i][q]; d.vy[o][s] = d.vy[i][q];
-    d.h[o][s] = d.h[i][q]; d.A[o][s] = d.A[i][q];
-    d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
+  const bool act = particle_of(d, s, k);
+  int key = 0;
+  double xq = 0., yq = 0., hq = 0.;
+  if (act) {
+    const InstDyn &y = d.dyn[k];
+    const int f = y.sel, t = y.sel ^ 1;
+    const int q = d.cl[s];
+    for (int b = 0; b <= buf; ++b) {
+      const int i = 2 * f + b, o = 2 * t + b;
+      d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
+      d.vx[o][s] = d.vx[i][q]; d.vy[o][s] = d.vy[i][q];
+      d.h[o][s] = d.h[i][q]; d.A[o][s] = d.A[i][q];
+      d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
+    }
+    d.mass[t][s] = d.mass[f][q];
+    d.orig[t][s] = d.orig[f][q];
+    key = d.cellid[f][q];
+    d.cellid[t][s] = key;
+    const int i = 2 * f + buf;
+    xq = d.x[i][q];
+    yq = d.y[i][q];
+    hq = d.hsml[i][q];
   }
-  d.mass[t][s] = d.mass[f][q];
-  d.orig[t][s] = d.orig[f][q];
-  d.cellid[t][s] = d.cellid[f][q];
-  const int i = 2 * f + buf;
-  const double xq = d.x[i][q], yq = d.y[i][q], hq = d.hsml[i][q];
-  const double ox = (y.xlo - 1) * y.bin, oy = (y.ylo - 1) * y.bin;
-  d.pf[s] = make_float4((float)(xq - ox), (float)(yq - oy), (float)hq, 0.0f);
+  // pass A prefilters after a rebuild: its inputs in the new order
+  prefilter_inputs(d, act, k, s, key, xq, yq, hq);
 }
 
 // -------------------------------------------------------------- boundary
@@ -868,7 +898,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // max|hsml - hsml_built| < skin, bounded by finalize from the largest
   // speed and hsml change of every evaluation since.  Then the prefilter
   // is skipped.
-  const bool vok = y.vbuilt && y.rb == 0 && y.vmoved < 0.9f * y.vskin;
+  const bool vok = y.rb == 0 && !lists_stale(y);
   const float skin = kSkin * (float)y.hmin;
   // The candidates of every particle of the block lie in three runs of the
   // cell lists: its cell range shifted by -W, 0 and +W cells, widened by one
