@@ -131,7 +131,10 @@ struct DevState {
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
   int2 *gpre;            // per list position: pair prefixes (intra, cross) of k_ranks
-  void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 or FP32)
+  void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 variant: the stress)
+  // mixed variant: pass A's record of each slot for pass B's stress sums,
+  // FP32 {sig11, sig12, sig22} / rho^2 and the mass (in sig11's allocation)
+  float4 *srec;
   // summation density (den_alg = 1): kernel_zero of each particle (by
   // p_off + reference index; persists across evaluations like the
   // reference's module array), and per slot the numerator and the density

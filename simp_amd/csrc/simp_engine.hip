@@ -92,7 +92,7 @@ constexpr size_t kBytesPerParticle =
     16 +                    // pf
     kNbrMax * 4 + 4 +       // nbr, nnb
     kNbrMax * 4 + 4 +       // vnb, vnc
-    4 * 8 + 3 * 8 +         // sig11/12/22, div; kz, dnum, dsum
+    5 * 8 + 3 * 8 +         // sig11 (x2: srec)/12/22, div; kz, dnum, dsum
     6 * 8 + 3 * 8 + 6 * 8;  // diag, sig_out, tend
 constexpr size_t kBytesPerBoundary = 3 * 8 + 9 * 4 + 2 * 4 + 9 * 4 + 9 * 4 + 4;
 constexpr size_t kBytesPerCell = 9 * 4 + 8 + 8;
@@ -168,7 +168,8 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.cellh, C);
   rc |= dalloc(list, &d.rebuild, 1);
   double *sg = nullptr;
-  rc |= dalloc(list, &sg, N); d.sig11 = sg;
+  rc |= dalloc(list, &sg, 2 * N); d.sig11 = sg;
+  d.srec = (float4 *)sg;
   rc |= dalloc(list, &sg, N); d.sig12 = sg;
   rc |= dalloc(list, &sg, N); d.sig22 = sg;
   rc |= dalloc(list, &sg, N); d.div = sg;

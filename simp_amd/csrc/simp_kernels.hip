@@ -780,6 +780,7 @@ __device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx,
 // top row 2, in the middle row 0 / 1 / 2 by left-or-earlier / later own-cell
 // member / right.
 constexpr int kBatch = 4;    // neighbour loads issued together (exact test, pass B)
+constexpr int kBatchBMixed = 4;
 
 __device__ __forceinline__ int pair_kind(const Ranges &R, int s, int self) {
   if (s < R.a[1]) return 0;
@@ -1240,6 +1241,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
   const double hth = MIXED ? hth_i : d.h[sb][self], Ac = MIXED ? A_i : d.A[sb][self];
+  const double rho_i = MIXED ? RS[self] : 0.0, m_i = MIXED ? MS[self] : 0.0;
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1256,9 +1258,10 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     ((double *)d.sig12)[self] = s12;
     ((double *)d.sig22)[self] = s22;
   } else {
-    ((float *)d.sig11)[self] = (float)s11;
-    ((float *)d.sig12)[self] = (float)s12;
-    ((float *)d.sig22)[self] = (float)s22;
+    // pass B's FP32 stress sums read one 16-byte record per neighbour
+    const double ir2 = 1.0 / (rho_i * rho_i);
+    d.srec[self] = make_float4((float)(s11 * ir2), (float)(s12 * ir2), (float)(s22 * ir2),
+                               (float)m_i);
   }
   ((double *)d.div)[self] = div;
   if (want_diag) {
@@ -1516,6 +1519,8 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 template <bool MIXED, bool SUMD, int phase>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
+  // neighbours per batch (the mixed variant's batch holds fewer registers)
+  constexpr int kB = MIXED ? kBatchBMixed : kBatch;
   int self = 0, k = 0;
   const bool ok = particle_of(d, self, k);
   const InstDyn &y = d.dyn[k];
@@ -1527,8 +1532,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (live) {
     const int W = y.W;
     const size_t N = (size_t)d.ntot;
-    using ST = typename std::conditional<MIXED, float, double>::type;
-    const ST *S11 = (const ST *)d.sig11, *S12 = (const ST *)d.sig12, *S22 = (const ST *)d.sig22;
+    // FP64: pass A's stress; mixed: its per-slot record (d.srec)
+    const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
+                 *S22 = (const double *)d.sig22;
     // third density call of the step (time_integration.f90:262): the
     // corrector's sum on the half-step kernel values with the densities
     // the second call produced (pass A's dsum)
@@ -1536,7 +1542,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     const bool sumd3 = SUMD && phase == 1 && m.den_alg == 1;
     // The loads go out in two rounds, each issued whole before any of its
     // values is used: (1) everything indexed by the list position,
-    // including the first kBatch neighbour-list entries (read whole: the
+    // including the first kB neighbour-list entries (read whole: the
     // entries past the count are never dereferenced); (2) everything the
     // particle id, the cell and those entries index.  A wave so waits for
     // two memory round trips before its first neighbour arithmetic.
@@ -1551,10 +1557,16 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     const double vxi = VX[self], vyi = VY[self];
     const int nn = d.nnb[self];
     const double divi = ((const double *)d.div)[self];
-    const double s11d = (double)S11[self], s12d = (double)S12[self], s22d = (double)S22[self];
-    int nx[kBatch];
+    double s11d = 0., s12d = 0., s22d = 0.;
+    float4 reci = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (MIXED) {
+      reci = d.srec[self];
+    } else {
+      s11d = S11[self]; s12d = S12[self]; s22d = S22[self];
+    }
+    int nx[kB];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) nx[u] = d.nbr[u * N + self];
+    for (int u = 0; u < kB; ++u) nx[u] = d.nbr[u * N + self];
 
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
@@ -1617,18 +1629,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       }
     }
     const bool listed = nn <= kNbrMax;
-    int sj[kBatch];
+    int sj[kB];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) sj[u] = (listed && u < nn) ? nx[u] : self;
+    for (int u = 0; u < kB; ++u) sj[u] = (listed && u < nn) ? nx[u] : self;
 
     double ax = 0., ay = 0.;
     // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
-    // in batches of kBatch neighbours whose loads are issued together
+    // in batches of kB neighbours whose loads are issued together
     // waves with many or boundary neighbours get issue priority
     if (__any(nn > 10 || (m.nb > 0 && bfl == y.lstamp))) __builtin_amdgcn_s_setprio(2);
     const double ri2 = rhoi * rhoi;
-    const float ir2 = (float)(1.0 / ri2);
-    const float s11f = (float)s11d * ir2, s12f = (float)s12d * ir2, s22f = (float)s22d * ir2;
+    const float s11f = reci.x, s12f = reci.y, s22f = reci.z;
     KSum sx, sy;
     double den3 = 0.;
     if (sumd3) den3 = kzi * mi / dsi;
@@ -1637,63 +1648,67 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       den3 = den3 + mq * w / rho2;
     };
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
-                           ST q11, ST q12, ST q22) {
-      if (!MIXED) {
-        const double gg = c6_grad(r, mh);
-        const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
-        const double rq2 = rq * rq;
-        const double coef = rhoi * mq / kRhoIce / hthi;
-        double cj = ((double)q11 * dw0 + (double)q12 * dw1) / rq2;
-        double ci = (s11d * dw0 + s12d * dw1) / ri2;
-        ax = ax + coef * (ci + cj);
-        cj = ((double)q12 * dw0 + (double)q22 * dw1) / rq2;
-        ci = (s12d * dw0 + s22d * dw1) / ri2;
-        ay = ay + coef * (ci + cj);
-      } else {
-        const float fx = (float)dx, fy = (float)dy;
-        const float rf = __builtin_sqrtf(fx * fx + fy * fy);
-        const float gg = c6_grad_f(rf, (float)mh);
-        const float ir = __builtin_amdgcn_rcpf(rf);
-        const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
-        const float rqf = (float)rq;
-        const float irq2 = __builtin_amdgcn_rcpf(rqf * rqf);
-        const float t11 = __builtin_fmaf((float)q11, irq2, s11f);
-        const float t12 = __builtin_fmaf((float)q12, irq2, s12f);
-        const float t22 = __builtin_fmaf((float)q22, irq2, s22f);
-        const float mf = (float)mq;
-        sx.add(mf * (t11 * dw0 + t12 * dw1));
-        sy.add(mf * (t12 * dw0 + t22 * dw1));
-      }
+                           double q11, double q12, double q22) {
+      const double gg = c6_grad(r, mh);
+      const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+      const double rq2 = rq * rq;
+      const double coef = rhoi * mq / kRhoIce / hthi;
+      double cj = (q11 * dw0 + q12 * dw1) / rq2;
+      double ci = (s11d * dw0 + s12d * dw1) / ri2;
+      ax = ax + coef * (ci + cj);
+      cj = (q12 * dw0 + q22 * dw1) / rq2;
+      ci = (s12d * dw0 + s22d * dw1) / ri2;
+      ay = ay + coef * (ci + cj);
+    };
+    // mixed: q = the neighbour's record {sig / rho^2, mass}
+    auto stress_pair_f = [&](double dx, double dy, double mh, float4 q) {
+      const float fx = (float)dx, fy = (float)dy;
+      const float rf = __builtin_sqrtf(fx * fx + fy * fy);
+      const float gg = c6_grad_f(rf, (float)mh);
+      const float ir = __builtin_amdgcn_rcpf(rf);
+      const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
+      const float t11 = q.x + s11f, t12 = q.y + s12f, t22 = q.z + s22f;
+      sx.add(q.w * (t11 * dw0 + t12 * dw1));
+      sy.add(q.w * (t12 * dw0 + t22 * dw1));
     };
     if (listed) {
-      for (int j0 = 0; j0 < nn; j0 += kBatch) {
-        double xq[kBatch], yq[kBatch], hq[kBatch], rq[kBatch], mq[kBatch], r2[kBatch];
-        ST q11[kBatch], q12[kBatch], q22[kBatch];
+      for (int j0 = 0; j0 < nn; j0 += kB) {
+        double xq[kB], yq[kB], hq[kB], rq[kB], mq[kB], r2[kB];
+        double q11[kB], q12[kB], q22[kB];
+        float4 qr[kB];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
+        for (int u = 0; u < kB; ++u) {
           const int t = sj[u];
           xq[u] = X[t]; yq[u] = Y[t]; hq[u] = HS[t];
-          rq[u] = RS[t]; mq[u] = MS[t];
-          q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
+          if (MIXED) {
+            qr[u] = d.srec[t];
+            mq[u] = sumd3 ? MS[t] : 0.0;
+          } else {
+            rq[u] = RS[t]; mq[u] = MS[t];
+            q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
+          }
           r2[u] = sumd3 ? d.dsum[t] : 1.0;
         }
         // the next batch's list entries go out with this batch's data
         // (past the end of the list, the last entry again: no line of the
         // list is read that the particle does not use)
-        const int j1 = j0 + kBatch;
+        const int j1 = j0 + kB;
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) nx[u] = d.nbr[min(j1 + u, nn - 1) * N + self];
+        for (int u = 0; u < kB; ++u) nx[u] = d.nbr[min(j1 + u, nn - 1) * N + self];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
+        for (int u = 0; u < kB; ++u) {
           if (j0 + u >= nn) break;
           const double dx = xi - xq[u], dy = yi - yq[u];
           const double mh = (hi + hq[u]) / 2.;
-          stress_pair(dx, dy, MIXED ? 0.0 : sqrt(dx * dx + dy * dy), mh, rq[u], mq[u], q11[u],
-                      q12[u], q22[u]);
+          if (MIXED)
+            stress_pair_f(dx, dy, mh, qr[u]);
+          else
+            stress_pair(dx, dy, sqrt(dx * dx + dy * dy), mh, rq[u], mq[u], q11[u], q12[u],
+                        q22[u]);
           if (sumd3) dens_pair(dx, dy, mh, mq[u], r2[u]);
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) sj[u] = j1 + u < nn ? nx[u] : self;
+        for (int u = 0; u < kB; ++u) sj[u] = j1 + u < nn ? nx[u] : self;
       }
     } else {
       const int cy = c / W, cx = c - cy * W;
@@ -1705,7 +1720,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           const double mh = (hi + HS[s]) / 2.;
           double r;
           if (!in_support(dx * dx + dy * dy, mh, r)) continue;
-          stress_pair(dx, dy, r, mh, RS[s], MS[s], S11[s], S12[s], S22[s]);
+          if (MIXED)
+            stress_pair_f(dx, dy, mh, d.srec[s]);
+          else
+            stress_pair(dx, dy, r, mh, RS[s], MS[s], S11[s], S12[s], S22[s]);
           if (sumd3) dens_pair(dx, dy, mh, MS[s], d.dsum[s]);
         }
       }
