@@ -134,16 +134,34 @@ __device__ __forceinline__ int run_add(int *base, int key, int inc) {
   return old + incl - inc;
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
+// Wave-wide min / max of a double (every lane active): DPP row shifts by
+// 1, 2, 4, 8 leave each row's extreme in its lane 15, the row broadcasts
+// of lanes 15 and 31 carry them into lane 63, which is read back to every
+// lane.  A lane with no source keeps its own value (min and max are
+// idempotent).  No LDS round trips, unlike a shuffle butterfly.
+template <int ctrl, int row_mask = 0xf>
+__device__ __forceinline__ double dpp_d(double v) {
+  return __builtin_amdgcn_update_dpp(v, v, ctrl, row_mask, 0xf, false);
 }
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+__device__ __forceinline__ double readlane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+template <bool MAX>
+__device__ __forceinline__ double wave_ext(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : fmin(a, b); };
+  v = op(v, dpp_d<0x111>(v));        // row_shr:1
+  v = op(v, dpp_d<0x112>(v));        // row_shr:2
+  v = op(v, dpp_d<0x114>(v));        // row_shr:4
+  v = op(v, dpp_d<0x118>(v));        // row_shr:8
+  v = op(v, dpp_d<0x142, 0xa>(v));   // row_bcast:15 into rows 1 and 3
+  v = op(v, dpp_d<0x143, 0xc>(v));   // row_bcast:31 into rows 2 and 3
+  return readlane63(v);
+}
+__device__ __forceinline__ double wave_min(double v) { return wave_ext<false>(v); }
+__device__ __forceinline__ double wave_max(double v) { return wave_ext<true>(v); }
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -780,7 +798,7 @@ __device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx,
 // top row 2, in the middle row 0 / 1 / 2 by left-or-earlier / later own-cell
 // member / right.
 constexpr int kBatch = 4;    // neighbour loads issued together (exact test, pass B)
-constexpr int kBatchBMixed = 4;
+constexpr int kBatchBMixed = 3;   // pass B, mixed variant
 
 __device__ __forceinline__ int pair_kind(const Ranges &R, int s, int self) {
   if (s < R.a[1]) return 0;
@@ -834,15 +852,18 @@ __device__ __forceinline__ double c6_w(double r, double mh) {
   return factor * ipow<8>(omq) * (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
 }
 
-__device__ __forceinline__ float c6_grad_f(float r, float mh) {
-  float ih = __builtin_amdgcn_rcpf(mh);
-  float q = r * ih;
-  float factor = kC6f * (ih * ih);
-  float omq = 1.0f - q;
-  float o2 = omq * omq, o4 = o2 * o2;
-  float o7 = o4 * o2 * omq;
-  float poly = __builtin_fmaf(__builtin_fmaf(16.0f, q, 7.0f), q, 1.0f);
-  return -(factor * 22.0f * q * o7 * poly) * ih;
+// FP32 kernel gradient over the distance, from the squared distance:
+// grad W = (dx, dy) * c6_gradr_f(dx^2 + dy^2, mh), the same polynomial
+// with the 1/r folded in (hardware sqrt and reciprocal, ~1 ulp)
+__device__ __forceinline__ float c6_gradr_f(float d2, float mh) {
+  const float ih = __builtin_amdgcn_rcpf(mh);
+  const float q = __builtin_amdgcn_sqrtf(d2) * ih;
+  const float ih2 = ih * ih;
+  const float omq = 1.0f - q;
+  const float o2 = omq * omq, o4 = o2 * o2;
+  const float o7 = o4 * o2 * omq;
+  const float poly = __builtin_fmaf(__builtin_fmaf(16.0f, q, 7.0f), q, 1.0f);
+  return (-22.0f * kC6f) * (ih2 * ih2) * o7 * poly;
 }
 
 // Compensated (TwoSum) FP32 accumulator.
@@ -994,10 +1015,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
       g4 = g4 - mq * w4 / rq;
     } else {
       const float fx = (float)dx, fy = (float)dy;
-      const float rf = __builtin_sqrtf(fx * fx + fy * fy);
-      const float gg = c6_grad_f(rf, (float)mh);
-      const float ir = __builtin_amdgcn_rcpf(rf);
-      const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
+      const float gr = c6_gradr_f(fx * fx + fy * fy, (float)mh);
+      const float dw0 = fx * gr, dw1 = fy * gr;
       const float a0 = (float)dv0, a1 = (float)dv1;
       const float w1 = a0 * dw0, w2 = a1 * dw0, w3 = a0 * dw1, w4 = a1 * dw1;
       const float mf = (float)mq, mr = mf * __builtin_amdgcn_rcpf((float)rq);
@@ -1663,10 +1682,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // mixed: q = the neighbour's record {sig / rho^2, mass}
     auto stress_pair_f = [&](double dx, double dy, double mh, float4 q) {
       const float fx = (float)dx, fy = (float)dy;
-      const float rf = __builtin_sqrtf(fx * fx + fy * fy);
-      const float gg = c6_grad_f(rf, (float)mh);
-      const float ir = __builtin_amdgcn_rcpf(rf);
-      const float dw0 = gg * (fx * ir), dw1 = gg * (fy * ir);
+      const float gr = c6_gradr_f(fx * fx + fy * fy, (float)mh);
+      const float dw0 = fx * gr, dw1 = fy * gr;
       const float t11 = q.x + s11f, t12 = q.y + s12f, t22 = q.z + s22f;
       sx.add(q.w * (t11 * dw0 + t12 * dw1));
       sy.add(q.w * (t12 * dw0 + t22 * dw1));
@@ -1697,7 +1714,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         for (int u = 0; u < kB; ++u) nx[u] = d.nbr[min(j1 + u, nn - 1) * N + self];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-          if (j0 + u >= nn) break;
+          // (mixed: continue keeps the batch's loads issued together; the
+          // FP64 variant, short of registers, schedules better with break)
+          if (j0 + u >= nn) {
+            if (MIXED) continue;
+            break;
+          }
           const double dx = xi - xq[u], dy = yi - yq[u];
           const double mh = (hi + hq[u]) / 2.;
           if (MIXED)
