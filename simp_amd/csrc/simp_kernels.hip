@@ -884,6 +884,34 @@ struct KSum {
 // pre-multiplied by kHalfPad = sqrt(1.00001) / 2, the prefilter's relative
 // margin folded in)
 constexpr int kTile = 1280;
+
+// The cell-list runs holding every candidate of a block's particles: the
+// block's cell range [cf, cl] shifted by -W, 0 and +W cells, widened by one
+// cell on each side; overlapping runs (a block spanning most of a row) are
+// merged.  Tile index i < T covers the runs in order.
+struct TileRuns {
+  int As, Bs, Cs, tB, tC, T;
+};
+__device__ __forceinline__ TileRuns tile_runs(const int *cs0, int cf, int cl, int W) {
+  TileRuns r;
+  int Ae, Be, Ce;
+  r.As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
+  r.Bs = cs0[cf - 1];     Be = cs0[cl + 2];
+  r.Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
+  if (Ae >= r.Bs) { r.Bs = r.As; Be = max(Ae, Be); r.As = Ae = 0x7fffffff; }
+  if (Be >= r.Cs) { r.Cs = min(r.Bs, r.Cs); Ce = max(Be, Ce); r.Bs = Be = 0x7fffffff; }
+  const int lA = Ae - r.As, lB = Be - r.Bs, lC = Ce - r.Cs;
+  r.tB = lA;
+  r.tC = lA + lB;
+  r.T = lA + lB + lC;
+  return r;
+}
+__device__ __forceinline__ int tile_slot(const TileRuns &r, int i) {
+  return i < r.tB ? r.As + i : (i < r.tC ? r.Bs + (i - r.tB) : r.Cs + (i - r.tC));
+}
+__device__ __forceinline__ int tile_index(const TileRuns &r, int sp) {
+  return sp >= r.Cs ? sp - r.Cs + r.tC : (sp >= r.Bs ? sp - r.Bs + r.tB : sp - r.As);
+}
 constexpr float kPad = 1.000005f;
 // candidate-list skin, relative to MINVAL(hsml)
 constexpr float kSkin = 0.01f;
@@ -928,24 +956,14 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // are merged.  The runs' prefilter data goes to LDS in one round of
   // coalesced loads by the whole block; a block whose region does not fit
   // takes the exact rescan path instead.
-  int As = 0, Bs = 0, Cs = 0, tB = 0, tC = 0;
+  TileRuns tr{};
   bool tiled = false;
   if (!vok) {
-    int Ae, Be, Ce;
-    const int cf = d.cellid[ss][pos0], cl = d.cellid[ss][pos0 + nloc - 1];
-    As = cs0[cf - W - 1]; Ae = cs0[cl - W + 2];
-    Bs = cs0[cf - 1];     Be = cs0[cl + 2];
-    Cs = cs0[cf + W - 1]; Ce = cs0[cl + W + 2];
-    if (Ae >= Bs) { Bs = As; Be = max(Ae, Be); As = Ae = 0x7fffffff; }
-    if (Be >= Cs) { Cs = min(Bs, Cs); Ce = max(Be, Ce); Bs = Be = 0x7fffffff; }
-    const int lA = Ae - As, lB = Be - Bs, lC = Ce - Cs;
-    tB = lA;
-    tC = lA + lB;
-    const int T = lA + lB + lC;
-    tiled = T <= kTile;      // list entries are tile indices (< kTile)
+    tr = tile_runs(cs0, d.cellid[ss][pos0], d.cellid[ss][pos0 + nloc - 1], W);
+    tiled = tr.T <= kTile;      // list entries are tile indices (< kTile)
     if (tiled) {
-      for (int i = threadIdx.x; i < T; i += kTPB) {
-        const int s = i < tB ? As + i : (i < tC ? Bs + (i - tB) : Cs + (i - tC));
+      for (int i = threadIdx.x; i < tr.T; i += kTPB) {
+        const int s = tile_slot(tr, i);
         float4 v = d.pf[s];
         v.z *= kHalfPad;
         tile[i] = v;
@@ -1039,7 +1057,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   } else {
     // tile index of a list position (every position of the block's 3x3
     // blocks lies in one of the runs)
-    auto tix = [&](int sp) { return sp >= Cs ? sp - Cs + tC : (sp >= Bs ? sp - Bs + tB : sp - As); };
+    auto tix = [&](int sp) { return tile_index(tr, sp); };
     const int tself = tix(self);
     const float4 pi4 = tile[tself];
     const float hif = (float)hi;
@@ -1107,7 +1125,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     }
   }
   // list entry (tile index) -> list position
-  auto lpos = [&](int ti) { return ti < tB ? As + ti : (ti < tC ? Bs + ti - tB : Cs + ti - tC); };
+  auto lpos = [&](int ti) { return tile_slot(tr, ti); };
   // list position of candidate j
   auto cand = [&](int j) {
     return vok ? d.vnb[j * N + self] : lpos((int)lst[j * kTPB + threadIdx.x]);
@@ -1759,39 +1777,74 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // boundary_kernel -> b_Wendland_C6 -> bf_Monaghan, boundary pairs in
     // ascending boundary index (boundary_kernel.f90:477-502)
     if (m.nb > 0 && bfl == y.lstamp) {
-      // the cell's boundary particles in ascending index, each with the
-      // slot this cell takes in its 3x3 block
+      // The cell's boundary particles in ascending index, each with the
+      // slot this cell takes in its 3x3 block.  A pair's rank counts the
+      // cell's particles before this one (list order) in range of the same
+      // boundary particle.  The lanes of one cell share the cell's entry
+      // run, so the entries are walked wave-synchronously (entry u of every
+      // lane's run at once): the in-range flags of the cell's lanes are one
+      // ballot, and the count over the cell's earlier lanes is a popcount;
+      // only the members in an earlier wave are tested one by one.  The
+      // loads of kBU entries go out together.
+      constexpr int kBU = 1;
       const int own0 = d.cell_start[m.c_off + c];   // first member of the own cell
+      const int lane = (int)threadIdx.x & 63;
+      const int wave0 = self - lane;                 // slot of the wave's lane 0
+      const int first = max(own0 - wave0, 0);        // the cell's first lane here
+      const unsigned long long below =
+          (((unsigned long long)1 << lane) - 1) & ~(((unsigned long long)1 << first) - 1);
       int nbn = 0;
       const int2 run = d.brun[m.c_off + c];
       const int *bl = d.blist + 9 * (size_t)m.b_off;
-      for (int e = run.x; e < run.y; ++e) {
-        const int ent = bl[e];
-        const int bg = m.b_off + (ent & 0xffffff);
-        const int slot9 = ent >> 24;
-        const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
-        const double br = sqrt(bdx * bdx + bdy * bdy);
-        if (!(br <= kBLength)) continue;
-        ++nbn;
-        int before = 0;
-        for (int s = own0; s < self; ++s) {
-          const double ex = d.xb[bg] - X[s], ey = d.yb[bg] - Y[s];
-          if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
+      for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
+        int ent[kBU], rb[kBU];
+        double bx[kBU], by[kBU], mb[kBU], hk[kBU];
+        bool inr[kBU];
+#pragma unroll
+        for (int j = 0; j < kBU; ++j) {
+          const int e = run.x + u0 + j;
+          ent[j] = e < run.y ? bl[e] : -1;
         }
-        const int rank = d.bbase[bg] + d.bpre[bg * 9 + slot9] + before;
-        const double q = br / kBLength * 1.0;
-        const double hk = rank < m.q_cap ? d.hq[m.q_off + rank] : 0.0;
-        const double ih = 1.0 / hk;
-        const double factor = kC6 * (ih * ih);
-        const double omq = 1 - q;
-        const double bw = factor * ipow<8>(omq) *
-                          (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
-        const double mb = d.massb[bg];
-        const double den = (mi + mb) * (br * br);
-        const double F0 = kKappaN * bdx * bw * 2 * mb / den;
-        const double F1 = kKappaN * bdy * bw * 2 * mb / den;
-        ax = ax + (-F0) / rhoi;
-        ay = ay + (-F1) / rhoi;
+#pragma unroll
+        for (int j = 0; j < kBU; ++j) {
+          const int bg = m.b_off + (ent[j] < 0 ? 0 : (ent[j] & 0xffffff));
+          bx[j] = d.xb[bg];
+          by[j] = d.yb[bg];
+          mb[j] = d.massb[bg];
+          rb[j] = d.bbase[bg] + d.bpre[bg * 9 + (ent[j] < 0 ? 0 : (ent[j] >> 24))];
+        }
+#pragma unroll
+        for (int j = 0; j < kBU; ++j) {
+          const double bdx = bx[j] - xi, bdy = by[j] - yi;
+          inr[j] = ent[j] >= 0 && sqrt(bdx * bdx + bdy * bdy) <= kBLength;
+          int before = __popcll(__ballot(inr[j]) & below);
+          if (inr[j]) {
+            for (int s = own0; s < wave0; ++s) {
+              const double ex = bx[j] - X[s], ey = by[j] - Y[s];
+              if (sqrt(ex * ex + ey * ey) <= kBLength) ++before;
+            }
+          }
+          const int rank = rb[j] + before;
+          hk[j] = (inr[j] && rank < m.q_cap) ? d.hq[m.q_off + rank] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kBU; ++j) {
+          if (!inr[j]) continue;
+          ++nbn;
+          const double bdx = bx[j] - xi, bdy = by[j] - yi;
+          const double br = sqrt(bdx * bdx + bdy * bdy);
+          const double q = br / kBLength * 1.0;
+          const double ih = 1.0 / hk[j];
+          const double factor = kC6 * (ih * ih);
+          const double omq = 1 - q;
+          const double bw = factor * ipow<8>(omq) *
+                            (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
+          const double den = (mi + mb[j]) * (br * br);
+          const double F0 = kKappaN * bdx * bw * 2 * mb[j] / den;
+          const double F1 = kKappaN * bdy * bw * 2 * mb[j] / den;
+          ax = ax + (-F0) / rhoi;
+          ay = ay + (-F1) / rhoi;
+        }
       }
       // interaction_stats capacity check (nnps.f90:660-668)
       if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) + nbn > kMaxIndInt)
