@@ -94,7 +94,10 @@ struct BlockEntry {
   int lid;           // logical block id (index of the block's partials)
   int nloc;          // slots in the block (min(n - first, kTPB))
   int pos;           // packed index of the first slot (p_off + first)
-  int pad[3];
+  int cs;            // the instance's current state set (sel ^ rb): 0 after a
+                     // repack, moved by k_permute when the instance rebuilds,
+                     // unchanged by the fold that ends the evaluation
+  int pad[2];
 };
 
 // Device pointers of one engine.

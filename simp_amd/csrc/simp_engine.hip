@@ -465,7 +465,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   for (int k = 0; k < ninst; ++k)
     for (int f = 0; f < nm[k].n; f += kTPB)
       lg.push_back(BlockEntry{k, f, (int)lg.size(), std::min(nm[k].n - f, kTPB),
-                              nm[k].p_off + f, {0, 0, 0}});
+                              nm[k].p_off + f, 0, {0, 0}});
   std::vector<BlockEntry> phys(lg.size());
   {
     const int nb_ = (int)lg.size(), q = nb_ / 8, r = nb_ % 8;

@@ -737,6 +737,7 @@ __global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
 __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
   if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
   if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
+  if (threadIdx.x == 0) d.blk[blockIdx.x].cs = d.dyn[d.blk[blockIdx.x].inst].sel ^ 1;
   int s = 0, k = 0;
   const bool act = particle_of(d, s, k);
   int key = 0;
@@ -924,16 +925,13 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   __shared__ float4 tile[kTile];
   __shared__ unsigned short lst[kNbrMax * kTPB];
   if (blockIdx.x >= (unsigned)d.nblk) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // the rebuild kernels have run
+  // The block entry carries the instance's current state set: the
+  // particle's own loads go out right after it, together with the loads of
+  // the instance records, instead of after them.
   const BlockEntry be = d.blk[blockIdx.x];
   const int k = be.inst;
-  const InstDyn &y = d.dyn[k];
-  if (inst_off(y)) return;                // whole block: blocks never straddle instances
-  const InstMeta &m = d.meta[k];
-  const int W = y.W;
-  const int *cs0 = d.cell_start + m.c_off;
   // the evaluated state, by slot (= list position)
-  const int ss = cur_set(y), sb = 2 * ss + buf;
+  const int ss = be.cs, sb = 2 * ss + buf;
   const double *X = d.x[sb], *Y = d.y[sb], *HS = d.hsml[sb], *RS = d.rho[sb];
   const double *VX = d.vx[sb], *VY = d.vy[sb], *MS = d.mass[ss];
   const int pos0 = be.pos;                // first slot of the block
@@ -941,6 +939,28 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const bool valid = (int)threadIdx.x < nloc;
   const int self = pos0 + min((int)threadIdx.x, nloc - 1);
   const size_t N = (size_t)d.ntot;
+  const int c = d.cellid[ss][self];
+  const double xi = X[self], yi = Y[self], hi = HS[self];
+  const double vxi = VX[self], vyi = VY[self];
+  // kept candidate lists: the count and the first kBatch entries (entries
+  // past the count are never dereferenced; unused when the lists are
+  // rebuilt in this evaluation)
+  int vcount = d.vnc[self], nx[kBatch];
+#pragma unroll
+  for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[u * N + self];
+  // the rheology's thickness and concentration, loaded with the rest (the
+  // FP64 variant, short of registers, loads them where they are used)
+  double hth_i = 0., A_i = 0.;
+  if (MIXED) {
+    hth_i = d.h[sb][self];
+    A_i = d.A[sb][self];
+  }
+  const InstDyn &y = d.dyn[k];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // the rebuild kernels have run
+  if (inst_off(y)) return;                // whole block: blocks never straddle instances
+  const InstMeta &m = d.meta[k];
+  const int W = y.W;
+  const int *cs0 = d.cell_start + m.c_off;
   // Candidate lists: built with the prefilter below (every candidate within
   // mhsml + skin), they still cover the pair set while the cell lists are
   // unchanged (same cells, same visiting order) and every pair's distance
@@ -974,29 +994,11 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     if (threadIdx.x == 0) d.dyn[k].vskin_next = skin;
   }
   if (!valid) return;
-  // kept lists: the count and the first kBatch entries go out with the
-  // particle's own loads (entries past the count are never dereferenced)
-  int vcount = 0, nx[kBatch];
-  if (vok) {
-    vcount = d.vnc[self];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[u * N + self];
-  }
   // waves with large supports (more neighbours) get issue priority: they
   // set the kernel's length
-  if (__any(HS[self] > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
-  const int c = d.cellid[ss][self];
+  if (__any(hi > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
   const int cy = c / W, cx = c - cy * W;
-  const double xi = X[self], yi = Y[self], hi = HS[self];
-  const double vxi = VX[self], vyi = VY[self];
   const double rhoi = SUMD ? RS[self] : 0.0;
-  // the rheology's thickness and concentration, loaded with the rest (the
-  // FP64 variant, short of registers, loads them where they are used)
-  double hth_i = 0., A_i = 0.;
-  if (MIXED) {
-    hth_i = d.h[sb][self];
-    A_i = d.A[sb][self];
-  }
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   const int p = self;
@@ -1558,53 +1560,62 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
   // neighbours per batch (the mixed variant's batch holds fewer registers)
   constexpr int kB = MIXED ? kBatchBMixed : kBatch;
-  int self = 0, k = 0;
-  const bool ok = particle_of(d, self, k);
+  // The loads go out in two rounds, each issued whole before any of its
+  // values is used: (1) everything indexed by the list position, including
+  // the first kB neighbour-list entries (read whole: the entries past the
+  // count are never dereferenced), with the instance records -- the block
+  // entry carries the current state set, so the particle's loads need not
+  // wait for them; (2) everything the cell, the instance and those entries
+  // index.  A wave so waits for two memory round trips before its first
+  // neighbour arithmetic.
+  const BlockEntry be = d.blk[blockIdx.x];
+  const int k = be.inst;
+  const bool ok = (int)threadIdx.x < be.nloc;
+  const int self = be.pos + (ok ? (int)threadIdx.x : 0);   // padding lanes: a real slot
+  const int p = self;
+  const int buf = phase == 1 ? 1 : 0;
+  // the evaluated state, by slot (= list position)
+  const int cs = be.cs;
+  const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
+  const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
+  // FP64: pass A's stress; mixed: its per-slot record (d.srec)
+  const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
+               *S22 = (const double *)d.sig22;
+  const size_t N = (size_t)d.ntot;
+  const int c = d.cellid[cs][self];
+  const double xi = X[self], yi = Y[self], hi = HS[self];
+  const double rhoi = RS[self], mi = MS[self];
+  const double vxi = VX[self], vyi = VY[self];
+  const int nn = d.nnb[self];
+  const double divi = ((const double *)d.div)[self];
+  double s11d = 0., s12d = 0., s22d = 0.;
+  float4 reci = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (MIXED) {
+    reci = d.srec[self];
+  } else {
+    s11d = S11[self]; s12d = S12[self]; s22d = S22[self];
+  }
+  int nx[kB];
+#pragma unroll
+  for (int u = 0; u < kB; ++u) nx[u] = d.nbr[u * N + self];
+  const double hthi = d.h[2 * cs + buf][p], Ai = d.A[2 * cs + buf][p];
+  double x1 = 0., y1 = 0., h1 = 0., A1 = 0.;
+  if (phase == 1) {
+    x1 = d.x[2 * cs + 0][p]; y1 = d.y[2 * cs + 0][p];
+    h1 = d.h[2 * cs + 0][p]; A1 = d.A[2 * cs + 0][p];
+  }
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
   const bool live = ok && !inst_off(y);
-  const int buf = phase == 1 ? 1 : 0;
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   double speed = 0.0, dhs = 0.0;
   if (live) {
     const int W = y.W;
-    const size_t N = (size_t)d.ntot;
-    // FP64: pass A's stress; mixed: its per-slot record (d.srec)
-    const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
-                 *S22 = (const double *)d.sig22;
     // third density call of the step (time_integration.f90:262): the
     // corrector's sum on the half-step kernel values with the densities
     // the second call produced (pass A's dsum)
     // (FP64 in both precision modes, as in pass A)
     const bool sumd3 = SUMD && phase == 1 && m.den_alg == 1;
-    // The loads go out in two rounds, each issued whole before any of its
-    // values is used: (1) everything indexed by the list position,
-    // including the first kB neighbour-list entries (read whole: the
-    // entries past the count are never dereferenced); (2) everything the
-    // particle id, the cell and those entries index.  A wave so waits for
-    // two memory round trips before its first neighbour arithmetic.
-    // the evaluated state, by slot (= list position)
-    const int cs = cur_set(y);
-    const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
-    const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
-    const int c = d.cellid[cs][self];
-    const int p = self;
-    const double xi = X[self], yi = Y[self], hi = HS[self];
-    const double rhoi = RS[self], mi = MS[self];
-    const double vxi = VX[self], vyi = VY[self];
-    const int nn = d.nnb[self];
-    const double divi = ((const double *)d.div)[self];
-    double s11d = 0., s12d = 0., s22d = 0.;
-    float4 reci = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (MIXED) {
-      reci = d.srec[self];
-    } else {
-      s11d = S11[self]; s12d = S12[self]; s22d = S22[self];
-    }
-    int nx[kB];
-#pragma unroll
-    for (int u = 0; u < kB; ++u) nx[u] = d.nbr[u * N + self];
-
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
     const bool sd = SUMD && m.den_alg == 1;
@@ -1612,13 +1623,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     if (sd && phase != 2) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
     // round 2
-    const double hthi = d.h[2 * cs + buf][p], Ai = d.A[2 * cs + buf][p];
     const int bfl = d.bflag[m.c_off + c];
-    double x1 = 0., y1 = 0., h1 = 0., A1 = 0., kzi = 0.;
-    if (phase == 1) {
-      x1 = d.x[2 * cs + 0][p]; y1 = d.y[2 * cs + 0][p];
-      h1 = d.h[2 * cs + 0][p]; A1 = d.A[2 * cs + 0][p];
-    }
+    double kzi = 0.;
     if (sumd3) kzi = d.kz[m.p_off + d.orig[cs][self]];
 
     // Everything of the update but the velocities is known before the
