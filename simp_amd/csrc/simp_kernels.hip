@@ -1800,6 +1800,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       const unsigned long long below =
           (((unsigned long long)1 << lane) - 1) & ~(((unsigned long long)1 << first) - 1);
       int nbn = 0;
+      float fbx = 0.0f, fby = 0.0f;   // mixed: the forces' sum
       const int2 run = d.brun[m.c_off + c];
       const int *bl = d.blist + 9 * (size_t)m.b_off;
       for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
@@ -1838,6 +1839,23 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           if (!inr[j]) continue;
           ++nbn;
           const double bdx = bx[j] - xi, bdy = by[j] - yi;
+          if (MIXED) {
+            // mixed: the force in FP32 (hardware sqrt and reciprocals);
+            // the pair set and its ranks stay those of the FP64 test
+            const float fx = (float)bdx, fy = (float)bdy;
+            const float br2 = fx * fx + fy * fy;
+            const float q = __builtin_amdgcn_sqrtf(br2) * (float)(1.0 / kBLength);
+            const float ih = __builtin_amdgcn_rcpf((float)hk[j]);
+            const float omq = 1.0f - q, o2 = omq * omq, o4 = o2 * o2;
+            const float poly = __builtin_fmaf(
+                __builtin_fmaf(__builtin_fmaf(32.0f, q, 25.0f), q, 8.0f), q, 1.0f);
+            const float bw = kC6f * (ih * ih) * (o4 * o4) * poly;
+            const float cf = (float)(2.0 * kKappaN) * bw * (float)mb[j] *
+                             __builtin_amdgcn_rcpf((float)(mi + mb[j]) * br2);
+            fbx += cf * fx;
+            fby += cf * fy;
+            continue;
+          }
           const double br = sqrt(bdx * bdx + bdy * bdy);
           const double q = br / kBLength * 1.0;
           const double ih = 1.0 / hk[j];
@@ -1851,6 +1869,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           ax = ax + (-F0) / rhoi;
           ay = ay + (-F1) / rhoi;
         }
+      }
+      if (MIXED) {
+        ax = ax - (double)fbx / rhoi;
+        ay = ay - (double)fby / rhoi;
       }
       // interaction_stats capacity check (nnps.f90:660-668)
       if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) + nbn > kMaxIndInt)
