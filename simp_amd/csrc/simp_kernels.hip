@@ -1631,8 +1631,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // stress sums (single_step.f90:104-106 tendencies; predictor
     // time_integration.f90:207-218, corrector :236-280 on the step-start
     // state x1 ... A1) and is stored now: the registers stay free.
-    const double dhdt = -(hthi * divi / rhoi);
-    const double dAdt = -(Ai * divi / rhoi);
+    // (mixed: one division)
+    const double dri = MIXED ? divi / rhoi : 0.0;
+    const double dhdt = MIXED ? -(hthi * dri) : -(hthi * divi / rhoi);
+    const double dAdt = MIXED ? -(Ai * dri) : -(Ai * divi / rhoi);
     const double dt = y.dt;
     if (phase == 0) {
       px_new = xi + vxi * dt / 2;
@@ -1887,11 +1889,18 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       wind_at(d, m, xi, yi, wu, wv);
       const double du = wu - vxi, dv = wv - vyi;
       const double hr = hthi * kRhoIce;
-      ax = ax + (kCWind * kRhoAir * du * fabs(du)) / hr;
-      ay = ay + (kCWind * kRhoAir * dv * fabs(dv)) / hr;
       const double cu = m.cur_u - vxi, cv = m.cur_v - vyi;
-      ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
-      ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
+      if (MIXED) {
+        // one division for the four terms
+        const double ihr = 1.0 / hr;
+        ax = ax + (kCWind * kRhoAir * du * fabs(du) + kCWater * kRhoWater * cu * fabs(cu)) * ihr;
+        ay = ay + (kCWind * kRhoAir * dv * fabs(dv) + kCWater * kRhoWater * cv * fabs(cv)) * ihr;
+      } else {
+        ax = ax + (kCWind * kRhoAir * du * fabs(du)) / hr;
+        ay = ay + (kCWind * kRhoAir * dv * fabs(dv)) / hr;
+        ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
+        ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
+      }
     }
     // the candidate lists' movement bound: speed of the evaluated state and
     // hsml change of the new one
