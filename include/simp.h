@@ -162,6 +162,44 @@ int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf,
 
 const char *simp_strerror(int status);
 
+/* Exact snapshot of a device-resident run: everything simp_upload takes
+ * (state in reference order, masses, boundary particles, dt, current_time,
+ * itimestep) plus the persistent arrays the next steps read -- the
+ * hsml_ij store that the boundary kernel reads stale and kernel_zero -- so
+ * that a handle restored with simp_snapshot_load (same configuration)
+ * continues bit for bit as the original would have.  The reference has no
+ * counterpart: its only resume path is the CSV spin-up reader (8 digits,
+ * in_out/initialize.f90:234-330), which cannot resume exactly.
+ * simp_snapshot_size returns the blob's bytes (or -status). */
+long long simp_snapshot_size(simp_handle *h);
+int simp_snapshot_save(simp_handle *h, void *buf, long long cap);
+int simp_snapshot_load(simp_handle *h, const void *buf, long long len);
+
+/* ---- snapshots in the reference's CSV format (host only, no device) ----
+ * The files of a run folder: particle_momentum.csv, particle_state.csv and
+ * particle_other.csv.  simp_output_create empties (or creates) them, as
+ * the reference's initialize does (in_out/initialize.f90:83-94).
+ * simp_output_append writes one snapshot exactly as the reference's output
+ * routine does (in_out/output.f90:47-78, formats :61-63): per particle i
+ * the position in km and the velocity in cm/s; mass, p, hsml, A, h,
+ * epsilon_11/12/22, sigma_11/12/22; then n, nb and current_time in hours.
+ * Arrays are in the Fortran layout (x, vx, eps3, sig3: component-fastest).
+ * simp_output_read_last reads the last snapshot of a folder back: the
+ * reference's spin-up reader (in_out/initialize.f90:234-330) with the
+ * column layout the writer produces (the reference's reader expects 14
+ * state columns and reads the first snapshot); positions and velocities
+ * come back in m and m/s, current_time in s.  Returns SIMP_E_CAPACITY (n
+ * set) when the snapshot holds more than cap particles. */
+int simp_output_create(const char *folder);
+int simp_output_append(const char *folder, int n, const double *x, const double *vx,
+                       const double *mass, const double *p, const double *hsml,
+                       const double *A, const double *hth, const double *eps3,
+                       const double *sig3, int nb, double current_time);
+int simp_output_read_last(const char *folder, int cap, int *n, int *nb,
+                          double *current_time, double *x, double *vx, double *mass,
+                          double *p, double *hsml, double *A, double *hth,
+                          double *eps3, double *sig3);
+
 /* ---- ensembles: many independent simulations in one packed launch ----
  * No reference counterpart (the reference runs one simulation per process,
  * modules/globals.f90:21-25).  Every step of a batch advances all its active

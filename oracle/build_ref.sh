@@ -46,6 +46,12 @@ done
 "$FC" $FFLAGS -o "$OUT/simp_ref_harness" "$OUT/obj/ref_harness.o" "${objs[@]}" \
     -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
 echo "build_ref: $OUT/simp_ref_harness"
+# the reference's CSV output routine (in_out/output.f90) behind a small
+# driver, for the golden snapshots of the CSV writer
+"$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$HERE/ref_output.f90" -o "$OUT/obj/ref_output.o"
+"$FC" $FFLAGS -o "$OUT/simp_ref_output" "$OUT/obj/ref_output.o" "${objs[@]}" \
+    -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+echo "build_ref: $OUT/simp_ref_output"
 
 # Summation-density variant (den_alg = 1, density.f90:41-100).  Run
 # parameters are compile-time PARAMETERs that the reference configures by

@@ -34,6 +34,8 @@ EXPORTS = (
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
     "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
+    "simp_output_create", "simp_output_append", "simp_output_read_last",
+    "simp_snapshot_size", "simp_snapshot_save", "simp_snapshot_load",
 )
 
 
@@ -121,6 +123,13 @@ def lib():
     L.simp_batch_list.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_instance_footprint.argtypes = [C.POINTER(SimpInstanceDesc)]
     L.simp_instance_footprint.restype = C.c_longlong
+    L.simp_snapshot_size.argtypes = [_vp]
+    L.simp_snapshot_size.restype = C.c_longlong
+    L.simp_snapshot_save.argtypes = [_vp, _vp, C.c_longlong]
+    L.simp_snapshot_load.argtypes = [_vp, _vp, C.c_longlong]
+    L.simp_output_create.argtypes = [C.c_char_p]
+    L.simp_output_append.argtypes = [C.c_char_p, C.c_int] + [_vp] * 9 + [C.c_int, C.c_double]
+    L.simp_output_read_last.argtypes = [C.c_char_p, C.c_int, _vp, _vp, _vp] + [_vp] * 9
     _lib = L
     return L
 

@@ -184,6 +184,24 @@ class Simp:
             sc.dt, sc.time = out["dt"], out["time"]
         return out
 
+    def snapshot(self) -> bytes:
+        """Exact snapshot of the device-resident run (``simp_snapshot_save``):
+        restored with :meth:`restore` on a handle of the same configuration
+        it continues bit for bit."""
+        size = lib().simp_snapshot_size(self._h)
+        if size < 0:
+            check(int(-size), "simp_snapshot_size")
+        buf = C.create_string_buffer(size)
+        check(lib().simp_snapshot_save(self._h, C.cast(buf, C.c_void_p), size),
+              "simp_snapshot_save")
+        return buf.raw
+
+    def restore(self, blob: bytes) -> None:
+        """Upload a run from :meth:`snapshot` (``simp_snapshot_load``)."""
+        check(lib().simp_snapshot_load(self._h, C.c_char_p(blob), len(blob)),
+              "simp_snapshot_load")
+        self._n = int(np.frombuffer(blob[8:12], dtype="<i4")[0])
+
     def download_diagnostics(self) -> dict:
         n = self._n
         out = {"eps": np.zeros((n, 3)), "zeta": np.zeros(n), "eta": np.zeros(n),

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <new>
 #include <string>
@@ -250,6 +251,75 @@ int simp_sync(simp_handle *h) {
 }
 
 void *simp_stream(simp_handle *h) { return h ? (void *)h->eng->stream() : nullptr; }
+
+// ---- exact snapshots ----
+// Blob: "SIMPSNP1", int32 n, nb, itimestep, pad; int64 nq; real8 dt, time;
+// real8 x(2n), vx(2n), mass(n), rho(n), hsml(n), h(n), A(n), xb(2nb),
+// massb(nb), kernel_zero(n), hsml_ij store(nq); little endian.
+static const char kSnapMagic[8] = {'S', 'I', 'M', 'P', 'S', 'N', 'P', '1'};
+static long long snap_bytes(long n, long nb, long nq) {
+  return 8 + 16 + 8 + 16 + 8 * (10 * n + 3 * nb + nq);
+}
+
+long long simp_snapshot_size(simp_handle *h) {
+  if (!h) return -SIMP_E_ARG;
+  if (!h->uploaded) return -SIMP_E_STATE;
+  const long nq = h->eng->store_cap(0);
+  if (nq < 0) return -SIMP_E_STATE;
+  return snap_bytes(h->n, h->nb, nq);
+}
+
+int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
+  if (!h || !buf) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  const long n = h->n, nb = h->nb, nq = h->eng->store_cap(0);
+  if (nq < 0) return SIMP_E_STATE;
+  if (cap < snap_bytes(n, nb, nq)) return SIMP_E_CAPACITY;
+  int rc = h->eng->sync_and_check();
+  if (rc) return map_rc(rc);
+  char *b = (char *)buf;
+  std::memcpy(b, kSnapMagic, 8);
+  double dt = 0, t = 0;
+  int it = 0;
+  rc = h->eng->download_scalars(0, &dt, &t, &it, nullptr, nullptr, nullptr);
+  if (rc) return map_rc(rc);
+  const int32_t hdr[4] = {(int32_t)n, (int32_t)nb, (int32_t)it, 0};
+  const int64_t q64 = nq;
+  std::memcpy(b + 8, hdr, 16);
+  std::memcpy(b + 24, &q64, 8);
+  std::memcpy(b + 32, &dt, 8);
+  std::memcpy(b + 40, &t, 8);
+  double *p = (double *)(b + 48);
+  double *x = p, *vx = x + 2 * n, *mass = vx + 2 * n, *rho = mass + n, *hsml = rho + n;
+  double *hth = hsml + n, *A = hth + n, *xb = A + n, *massb = xb + 2 * nb, *kz = massb + nb;
+  double *hq = kz + n;
+  rc = h->eng->download_state(0, x, vx, rho, hsml, hth, A);
+  if (!rc) rc = h->eng->download_persist(0, mass, xb, massb, kz, hq);
+  return map_rc(rc);
+}
+
+int simp_snapshot_load(simp_handle *h, const void *buf, long long len) {
+  if (!h || !buf || len < 48) return SIMP_E_ARG;
+  const char *b = (const char *)buf;
+  if (std::memcmp(b, kSnapMagic, 8) != 0) return SIMP_E_ARG;
+  int32_t hdr[4];
+  int64_t nq;
+  double dt, t;
+  std::memcpy(hdr, b + 8, 16);
+  std::memcpy(&nq, b + 24, 8);
+  std::memcpy(&dt, b + 32, 8);
+  std::memcpy(&t, b + 40, 8);
+  const long n = hdr[0], nb = hdr[1];
+  if (n < 1 || nb < 0 || nq < 0 || len < snap_bytes(n, nb, nq)) return SIMP_E_ARG;
+  const double *p = (const double *)(b + 48);
+  const double *x = p, *vx = x + 2 * n, *mass = vx + 2 * n, *rho = mass + n, *hsml = rho + n;
+  const double *hth = hsml + n, *A = hth + n, *xb = A + n, *massb = xb + 2 * nb, *kz = massb + nb;
+  const double *hq = kz + n;
+  int rc = simp_upload(h, (int)n, x, vx, mass, rho, hsml, hth, A, (int)nb, nb ? xb : nullptr,
+                       nb ? massb : nullptr, dt, t, hdr[2]);
+  if (rc) return rc;
+  return map_rc(h->eng->upload_persist(0, kz, hq, (long)nq));
+}
 
 int simp_download(simp_handle *h, double *x, double *vx, double *rho, double *hsml, double *hth,
                   double *A, double *dt, double *current_time, int *itimestep) {

@@ -78,6 +78,13 @@ class Engine {
                    int *bnip, int *bpi, int *bpj, double *brij, double *w = nullptr);
   // kernel_zero of instance inst (n values)
   int download_kernel_zero(int inst, double *kz);
+  // snapshots: the arrays of instance k a restart needs besides its state
+  // -- masses (reference order), boundary particles, kernel_zero and the
+  // hsml_ij store (store_cap entries) -- and their restore after an upload
+  long store_cap(int inst) const;
+  int download_persist(int inst, double *mass, double *xb, double *massb, double *kz,
+                       double *hq);
+  int upload_persist(int inst, const double *kz, const double *hq, long nq);
   int cur_set(int inst);
 
   void set_diag(bool on) { diag_ = on; }

@@ -774,6 +774,53 @@ int Engine::download_diag(int k, double *eps3, double *zeta, double *eta, double
   return 0;
 }
 
+long Engine::store_cap(int k) const {
+  return (k < 0 || k >= (int)meta_.size()) ? -1 : (long)meta_[k].q_cap;
+}
+
+int Engine::download_persist(int k, double *mass, double *xb, double *massb, double *kz,
+                             double *hq) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  const InstMeta &m = meta_[k];
+  const int cs = cur_set(k);
+  if (cs < 0) return -2;
+  if (mass) {
+    SlotOrder so;
+    if (so.fetch(d_.orig[cs], m.p_off, m.n)) return -2;
+    std::vector<double> a(m.n);
+    HIPCHK(hipMemcpy(a.data(), d_.mass[cs] + m.p_off, (size_t)m.n * 8, hipMemcpyDeviceToHost));
+    so.put(a, mass);
+  }
+  if (m.nb > 0 && (xb || massb)) {
+    std::vector<double> bx(m.nb), by(m.nb);
+    HIPCHK(hipMemcpy(bx.data(), d_.xb + m.b_off, (size_t)m.nb * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(by.data(), d_.yb + m.b_off, (size_t)m.nb * 8, hipMemcpyDeviceToHost));
+    if (xb)
+      for (int b = 0; b < m.nb; ++b) {
+        xb[2 * b] = bx[b];
+        xb[2 * b + 1] = by[b];
+      }
+    if (massb)
+      HIPCHK(hipMemcpy(massb, d_.massb + m.b_off, (size_t)m.nb * 8, hipMemcpyDeviceToHost));
+  }
+  if (kz) HIPCHK(hipMemcpy(kz, d_.kz + m.p_off, (size_t)m.n * 8, hipMemcpyDeviceToHost));
+  if (hq && m.q_cap > 0)
+    HIPCHK(hipMemcpy(hq, d_.hq + m.q_off, (size_t)m.q_cap * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int Engine::upload_persist(int k, const double *kz, const double *hq, long nq) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  const InstMeta &m = meta_[k];
+  if (kz) HIPCHK(hipMemcpyAsync(d_.kz + m.p_off, kz, (size_t)m.n * 8, hipMemcpyHostToDevice, st_));
+  const long q = std::min<long>(nq, m.q_cap);
+  if (hq && q > 0)
+    HIPCHK(hipMemcpyAsync(d_.hq + m.q_off, hq, (size_t)q * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
 int Engine::download_kernel_zero(int k, double *kz) {
   if (k < 0 || k >= d_.ninst) return -1;
   HIPCHK(hipStreamSynchronize(st_));
