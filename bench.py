@@ -228,6 +228,42 @@ def bench_ensemble(steps=500, baseline_steps=50):
     return out
 
 
+def bench_snapshots(sc, steps=10):
+    """Snapshots of the 250k run after a few device-resident steps: one CSV
+    snapshot in the reference's format (the host writer, to a temporary
+    folder) and one exact snapshot saved and restored."""
+    import tempfile
+    from simp_amd import _lib, output
+    from simp_amd.api import Simp
+    sim = Simp.for_scenario(sc, precision=_lib.SIMP_MIXED)
+    sim.upload(sc)
+    sim.set_diagnostics(True)
+    sim.step(steps)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        output.create(tmp)
+        t0 = time.perf_counter()
+        output.snapshot(sim, tmp, sc.mass, sc.nb)
+        out["csv_ms"] = 1e3 * (time.perf_counter() - t0)
+        size = sum(os.path.getsize(os.path.join(tmp, f)) for f in os.listdir(tmp))
+    out["csv_bytes"] = size
+    out["csv_mb_per_s"] = size / 1e6 / (out["csv_ms"] / 1e3)
+    t0 = time.perf_counter()
+    blob = sim.snapshot()
+    out["exact_save_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["exact_bytes"] = len(blob)
+    other = Simp.for_scenario(sc, precision=_lib.SIMP_MIXED)
+    t0 = time.perf_counter()
+    other.restore(blob)
+    other.sync()
+    out["exact_restore_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["note"] = ("CSV: download + host formatting + write (includes PCIe); "
+                   "exact: state, masses, boundary, kernel_zero and hsml_ij store")
+    sim.close()
+    other.close()
+    return out
+
+
 def bench_serve(count=80, budget_gib=16.0, chunk=10):
     """Config 5: Poisson arrivals (20/s) of closed-box simulations, N in
     [5k, 250k], 100-2000 steps each, HBM-budget admission, continuous
@@ -344,6 +380,7 @@ def main():
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
     line["dropin_single_step"] = bench_dropin(sc)
+    line["snapshots"] = bench_snapshots(sc)
     if not args.no_ensemble:
         line["ensemble"] = bench_ensemble()
     if not args.no_serve:

@@ -93,6 +93,46 @@ module simp_binding
             integer(c_int), intent(out) :: itimestep
         end function simp_download
 
+        ! snapshots in the reference's CSV format (output.f90) and exact
+        ! binary snapshots; folder: NUL-terminated (trim(path)//c_null_char)
+        integer(c_int) function simp_output_create(folder)                  &
+                bind(C, name='simp_output_create')
+            import :: c_int, c_char
+            character(kind=c_char), intent(in) :: folder(*)
+        end function simp_output_create
+
+        integer(c_int) function simp_output_append(folder, n, x, vx, mass,  &
+                p, hsml, A, hth, eps3, sig3, nb, current_time)              &
+                bind(C, name='simp_output_append')
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: folder(*)
+            integer(c_int), value :: n, nb
+            real(c_double), intent(in) :: x(*), vx(*), mass(*), p(*),       &
+                                          hsml(*), A(*), hth(*), eps3(*),   &
+                                          sig3(*)
+            real(c_double), value :: current_time
+        end function simp_output_append
+
+        integer(c_long_long) function simp_snapshot_size(h)                &
+                bind(C, name='simp_snapshot_size')
+            import :: c_ptr, c_long_long
+            type(c_ptr), value :: h
+        end function simp_snapshot_size
+
+        integer(c_int) function simp_snapshot_save(h, buf, cap)             &
+                bind(C, name='simp_snapshot_save')
+            import :: c_ptr, c_int, c_long_long
+            type(c_ptr), value :: h, buf
+            integer(c_long_long), value :: cap
+        end function simp_snapshot_save
+
+        integer(c_int) function simp_snapshot_load(h, buf, len)             &
+                bind(C, name='simp_snapshot_load')
+            import :: c_ptr, c_int, c_long_long
+            type(c_ptr), value :: h, buf
+            integer(c_long_long), value :: len
+        end function simp_snapshot_load
+
         type(c_ptr) function simp_strerror(status) bind(C, name='simp_strerror')
             import :: c_ptr, c_int
             integer(c_int), value :: status
