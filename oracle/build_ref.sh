@@ -52,6 +52,12 @@ echo "build_ref: $OUT/simp_ref_harness"
 "$FC" $FFLAGS -o "$OUT/simp_ref_output" "$OUT/obj/ref_output.o" "${objs[@]}" \
     -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
 echo "build_ref: $OUT/simp_ref_output"
+# the reference's initialize (scenario from its domain and boundary maps)
+# behind a small driver, for the golden vectors of the scenario generator
+"$FC" $FFLAGS -I"$OUT/mod" -J"$OUT/mod" -c "$HERE/ref_init.f90" -o "$OUT/obj/ref_init.o"
+"$FC" $FFLAGS -o "$OUT/simp_ref_init" "$OUT/obj/ref_init.o" "${objs[@]}" \
+    -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+echo "build_ref: $OUT/simp_ref_init"
 
 # Summation-density variant (den_alg = 1, density.f90:41-100).  Run
 # parameters are compile-time PARAMETERs that the reference configures by

@@ -278,3 +278,107 @@ def subdomain(sc: Scenario, x0: float, x1: float, y0: float, y1: float) -> Scena
     if out.tse == 2:
         out.dt = vp_dt(out.hsml)
     return out
+
+
+@dataclasses.dataclass
+class ReferenceMaps:
+    """The inputs of the reference's scenario setup: the domain map
+    (``modules/domain.f90``, 1 = ice cell) and the boundary map
+    (``modules/boundary_domain.f90``, 1 = ocean/ice side), both indexed
+    (x cell, y cell) with the first row at the top, their geometry, and the
+    particle resolution and boundary length of ``modules/globals.f90`` /
+    ``data_variables.f90``."""
+    map: np.ndarray            # (x_numcell, y_numcell) int
+    bmap: np.ndarray           # (bx_numcell, by_numcell) int
+    x_mingeom: float
+    y_maxgeom: float
+    map_resolution: float
+    bx_mingeom: float
+    by_maxgeom: float
+    bmap_resolution: float
+    map_factor: int
+    resolution: float
+    b_length: float = B_LENGTH
+    hsml_multiplicity: int = 3
+
+
+def _f32(v):
+    return np.float32(v)
+
+
+def from_maps(mp: ReferenceMaps, wind=REFERENCE_WIND, tse: int = 2) -> Scenario:
+    """The reference's default initial state, built as its initialize does
+    (``in_out/initialize.f90:102-210`` particle_placing, then
+    ``in_out/boundaries.f90:164-312`` boundary_placing_1 and the boundary
+    masses of ``boundaries.f90:46``), with the reference's default-REAL
+    (single precision) intermediate arithmetic kept where its literals make
+    it so: ``-1./2. + m + (k + 1./2.) / res_ratio`` is evaluated in single
+    precision and only then scaled by the (double) map resolution."""
+    res_ratio = int(np.rint(mp.map_resolution / mp.resolution))   # NINT
+    if res_ratio < 1:
+        raise ValueError("particle resolution too coarse for the domain map")
+    nx, ny = mp.map.shape
+    xs = []
+    half = _f32(0.5)
+    rr = _f32(res_ratio)
+    for n in range(ny):                  # initialize.f90:166-185, y cells outer
+        for m in range(nx):
+            if mp.map[m, n] != 1:
+                continue
+            for k in range(res_ratio):
+                for l in range(res_ratio):
+                    ax = (-half + _f32(m)) + (_f32(k) + half) / rr
+                    ay = (half - _f32(n)) - (_f32(l) + half) / rr
+                    xs.append((mp.x_mingeom + float(ax) * mp.map_resolution,
+                               mp.y_maxgeom + float(ay) * mp.map_resolution))
+    x = np.array(xs, dtype=np.float64).reshape(-1, 2)
+    n_p = x.shape[0]
+    hsml = np.full(n_p, mp.hsml_multiplicity * mp.map_resolution / res_ratio)
+    max_hsml = float(np.max(hsml)) * 10.0
+    A = np.ones(n_p)
+    h = np.ones(n_p)
+    rho = RHO_ICE * h
+    mass = rho * (hsml / mp.hsml_multiplicity) ** 2
+
+    # boundary_placing_1: one boundary particle row along every edge between
+    # an outside cell (0) and an inside cell (1) of the boundary map, bppg
+    # particles per edge
+    bppg = int(np.rint(mp.bmap_resolution / (mp.b_length / 3)))
+    if bppg < 1:
+        raise ValueError("boundary map resolution too fine for the boundary length")
+    bx, by = mp.bmap.shape
+    mf = _f32(half * _f32(mp.map_factor))     # 1./2. * map_factor
+    bp = _f32(bppg)
+    xb = []
+    res = mp.bmap_resolution
+    for n in range(1, bx + 1):                # DO CONCURRENT (n, m): m fastest,
+        for m in range(1, by + 1):            # the compiled reference's order
+            if mp.bmap[n - 1, m - 1] != 0:
+                continue
+            if n + 1 <= bx and mp.bmap[n, m - 1] == 1:
+                for k in range(bppg):
+                    ax = _f32(n) - mf
+                    ay = (_f32(-m + 1) + mf) - (_f32(k) + half) / bp
+                    xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+            if n - 1 >= 1 and mp.bmap[n - 2, m - 1] == 1:
+                for k in range(bppg):
+                    ax = _f32(n - 1) - mf
+                    ay = (_f32(-m + 1) + mf) - (_f32(k) + half) / bp
+                    xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+            if m + 1 <= by and mp.bmap[n - 1, m] == 1:
+                for k in range(bppg):
+                    ax = (_f32(n - 1) - mf) + (_f32(k) + half) / bp
+                    ay = _f32(-m) + mf
+                    xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+            if m - 1 >= 1 and mp.bmap[n - 1, m - 2] == 1:
+                for k in range(bppg):
+                    ax = (_f32(n - 1) - mf) + (_f32(k) + half) / bp
+                    ay = _f32(-m + 1) + mf
+                    xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+    xb = np.array(xb, dtype=np.float64).reshape(-1, 2)
+    massb = np.full(xb.shape[0], float(np.sum(mass)) / n_p)
+    sc = Scenario(x=x, vx=np.zeros((n_p, 2)), mass=mass, rho=rho, hsml=hsml, h=h, A=A,
+                  xb=xb, massb=massb, max_hsml=max_hsml, wind=wind, tse=tse,
+                  name="reference_default")
+    sc.dt = vp_dt(hsml) if tse == 2 else 0.0
+    return sc

@@ -34,3 +34,19 @@ def run_pc_like_reference(o, sc, integration_time):
         out = o.run_pc(sc, 1, itimestep=it)
         it += 1
     return it, out
+
+
+def reference_maps(name="reference_default_init.npz"):
+    """The generator inputs the reference's initialize used (its two maps
+    and their geometry, written by tests/golden/make_init_golden.py), and
+    the golden arrays."""
+    from simp_amd.scenarios import ReferenceMaps
+    g = load(name)
+    geo = g["geometry"]
+    mp = ReferenceMaps(map=g["map"], bmap=g["bmap"], x_mingeom=float(geo[0]),
+                       y_maxgeom=float(geo[1]), map_resolution=float(geo[2]),
+                       bx_mingeom=float(geo[3]), by_maxgeom=float(geo[4]),
+                       bmap_resolution=float(geo[5]), map_factor=int(geo[6]),
+                       resolution=float(geo[7]), b_length=float(geo[8]),
+                       hsml_multiplicity=int(geo[9]))
+    return mp, g
