@@ -56,7 +56,9 @@ typedef struct simp_config {
     int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96)       */
     int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119)     */
     int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127)         */
-    int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134)      */
+    int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134),     */
+                            /* internal_stress.f90:105-136 (stress_1);         */
+                            /* 2: (sigma_i + sigma_j)/(rho_i rho_j), :189-216  */
     int32_t skf;            /* 6: Wendland C6 (globals.f90:147)                 */
     int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155);    */
                             /* 1: summation density (density.f90:41-100)        */
@@ -224,7 +226,7 @@ typedef struct simp_instance_desc {
     double  grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u, *grid_v;
     int32_t den_alg;                     /* 0 or 3: diagnostic; 1: summation   */
-    int32_t reserved;
+    int32_t pa_sph;                      /* 0 or 1: stress_1; 2: stress_2      */
 } simp_instance_desc;
 
 int simp_batch_create(int precision, simp_batch **out);

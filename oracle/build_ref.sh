@@ -59,33 +59,42 @@ echo "build_ref: $OUT/simp_ref_output"
     -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
 echo "build_ref: $OUT/simp_ref_init"
 
-# Summation-density variant (den_alg = 1, density.f90:41-100).  Run
-# parameters are compile-time PARAMETERs that the reference configures by
-# editing globals.f90 (README.md:68-70); the edit is applied on the way into
-# the compiler (stdin), so no copy of the source is written, and everything
-# that uses the module is rebuilt against it.
-D1="$OUT/den1"
-mkdir -p "$D1/obj" "$D1/mod"
-d1objs=()
-grep -q 'parameter(den_alg = 3)' "$SRC/modules/globals.f90"   # the line the edit targets
-sed 's/parameter(den_alg = 3)/parameter(den_alg = 1)/' "$SRC/modules/globals.f90" \
-    | "$FC" $FFLAGS -x f95 -J"$D1/mod" -c - -o "$D1/obj/globals.o"
-d1objs+=("$D1/obj/globals.o")
-for m in data_variables domain boundary_domain mpi_variables tree_module interfaces; do
-    "$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$SRC/modules/$m.f90" -o "$D1/obj/$m.o"
-    d1objs+=("$D1/obj/$m.o")
-done
-for d in ext_force in_out int_force integration neighbor phy_quantities sph; do
-    for f in "$SRC/$d"/*.f90; do
-        b="$(basename "$f" .f90)"
-        "$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$f" -o "$D1/obj/$b.o"
-        d1objs+=("$D1/obj/$b.o")
+# Switch variants.  Run parameters are compile-time PARAMETERs that the
+# reference configures by editing globals.f90 (README.md:68-70); the edit is
+# applied on the way into the compiler (stdin), so no copy of the source is
+# written, and everything that uses the module is rebuilt against it.
+variant() {   # <tag> <PARAMETER line in globals.f90> <replacement>; sets vobjs
+    local V="$OUT/$1" from="$2" to="$3"
+    mkdir -p "$V/obj" "$V/mod"
+    vobjs=()
+    grep -q "$from" "$SRC/modules/globals.f90"   # the line the edit targets
+    sed "s/$from/$to/" "$SRC/modules/globals.f90" \
+        | "$FC" $FFLAGS -x f95 -J"$V/mod" -c - -o "$V/obj/globals.o"
+    vobjs+=("$V/obj/globals.o")
+    for m in data_variables domain boundary_domain mpi_variables tree_module interfaces; do
+        "$FC" $FFLAGS -I"$V/mod" -J"$V/mod" -c "$SRC/modules/$m.f90" -o "$V/obj/$m.o"
+        vobjs+=("$V/obj/$m.o")
     done
-done
-"$FC" $FFLAGS -I"$D1/mod" -J"$D1/mod" -c "$HERE/ref_harness.f90" -o "$D1/obj/ref_harness.o"
-"$FC" $FFLAGS -o "$OUT/simp_ref_harness_den1" "$D1/obj/ref_harness.o" "${d1objs[@]}" \
-    -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
-echo "build_ref: $OUT/simp_ref_harness_den1"
+    for d in ext_force in_out int_force integration neighbor phy_quantities sph; do
+        for f in "$SRC/$d"/*.f90; do
+            b="$(basename "$f" .f90)"
+            "$FC" $FFLAGS -I"$V/mod" -J"$V/mod" -c "$f" -o "$V/obj/$b.o"
+            vobjs+=("$V/obj/$b.o")
+        done
+    done
+    "$FC" $FFLAGS -I"$V/mod" -J"$V/mod" -c "$HERE/ref_harness.f90" -o "$V/obj/ref_harness.o"
+    "$FC" $FFLAGS -o "$OUT/simp_ref_harness_$1" "$V/obj/ref_harness.o" "${vobjs[@]}" \
+        -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+    echo "build_ref: $OUT/simp_ref_harness_$1"
+}
+D1="$OUT/den1"
+# summation density (den_alg = 1, density.f90:41-100)
+variant den1 'parameter(den_alg = 3)' 'parameter(den_alg = 1)'
+d1objs=("${vobjs[@]}")
+# stress_2 particle approximation (pa_sph = 2, internal_stress.f90:144-216)
+variant pa2 'parameter(pa_sph = 1)' 'parameter(pa_sph = 2)'
+P2="$OUT/pa2"
+p2objs=("${vobjs[@]}")
 
 # The drop-in check: the same harness and the reference's own
 # time_integration, with single_step replaced by the iso_c_binding shim of
@@ -111,4 +120,5 @@ gpu_harness() {   # <obj dir> <mod dir> <output> <objects...>
 if [ -f "$LIB" ]; then
     gpu_harness "$OUT/obj" "$OUT/mod" "$OUT/simp_ref_harness_gpu" "${objs[@]}"
     gpu_harness "$D1/obj" "$D1/mod" "$OUT/simp_ref_harness_gpu_den1" "${d1objs[@]}"
+    gpu_harness "$P2/obj" "$P2/mod" "$OUT/simp_ref_harness_gpu_pa2" "${p2objs[@]}"
 fi

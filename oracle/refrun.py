@@ -23,6 +23,9 @@ HARNESS_GPU = os.path.join(HERE, "_ref", "simp_ref_harness_gpu")
 #: the harness built with summation density (den_alg = 1)
 HARNESS_DEN1 = os.path.join(HERE, "_ref", "simp_ref_harness_den1")
 HARNESS_GPU_DEN1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den1")
+#: the harness built with the stress_2 particle approximation (pa_sph = 2)
+HARNESS_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_pa2")
+HARNESS_GPU_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_pa2")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
 

@@ -443,8 +443,23 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         sig3[3 * i + 1] = 2 * et * e12;
         sig3[3 * i + 2] = (z + et) * e22 + (z - et) * e11 - pr * (1 - K_T) / 2;
     }
+    /* stress_2 (internal_stress.f90:189-216): (sigma_i + sigma_j) / (rho_i rho_j) */
+    for (int k = 0; prm->pa_sph == 2 && k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        double dw0 = c->dwdx[2 * k], dw1 = c->dwdx[2 * k + 1];
+        double s11i = sig3[3 * i], s12i = sig3[3 * i + 1], s22i = sig3[3 * i + 2];
+        double s11j = sig3[3 * j], s12j = sig3[3 * j + 1], s22j = sig3[3 * j + 2];
+        double cj = s11j * dw0 + s12j * dw1;
+        double ci = s11i * dw0 + s12i * dw1;
+        dvxdt[2 * i] = dvxdt[2 * i] + mass[j] / RHO_ICE / h[i] / rho[j] * (ci + cj);
+        dvxdt[2 * j] = dvxdt[2 * j] - mass[i] / RHO_ICE / h[j] / rho[i] * (ci + cj);
+        cj = s12j * dw0 + s22j * dw1;
+        ci = s12i * dw0 + s22i * dw1;
+        dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + mass[j] / RHO_ICE / h[i] / rho[j] * (ci + cj);
+        dvxdt[2 * j + 1] = dvxdt[2 * j + 1] - mass[i] / RHO_ICE / h[j] / rho[i] * (ci + cj);
+    }
     /* stress_1 (internal_stress.f90:103-136) */
-    for (int k = 0; k < c->nip; ++k) {
+    for (int k = 0; prm->pa_sph != 2 && k < c->nip; ++k) {
         int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
         double dw0 = c->dwdx[2 * k], dw1 = c->dwdx[2 * k + 1];
         double s11i = sig3[3 * i], s12i = sig3[3 * i + 1], s22i = sig3[3 * i + 2];

@@ -32,6 +32,8 @@ typedef struct orc_params {
     double grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u;  /* grid_ny * grid_nx, row-major (y outer) */
     const double *grid_v;
+    int    pa_sph;       /* 1 (or 0): stress_1 (internal_stress.f90:105-136);
+                            2: stress_2 (:189-216) */
 } orc_params;
 
 orc_ctx *orc_create(void);

@@ -52,6 +52,7 @@ struct InstMeta {
   int tse, wind_mode;
   int g_off, g_nx, g_ny;  // wind grid range
   int den_alg;           // 3: rho = h rho_ice; 1: summation density
+  int pa_sph;            // 1: stress_1; 2: stress_2
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;
   double max_hsml;

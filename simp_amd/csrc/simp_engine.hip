@@ -284,6 +284,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.cur_u = in.current[0]; m.cur_v = in.current[1];
     m.max_hsml = in.max_hsml;
     m.den_alg = in.den_alg;
+    m.pa_sph = in.pa_sph;
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;

@@ -1297,10 +1297,15 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
     ((double *)d.sig12)[self] = s12;
     ((double *)d.sig22)[self] = s22;
   } else {
-    // pass B's FP32 stress sums read one 16-byte record per neighbour
-    const double ir2 = 1.0 / (rho_i * rho_i);
-    d.srec[self] = make_float4((float)(s11 * ir2), (float)(s12 * ir2), (float)(s22 * ir2),
-                               (float)m_i);
+    // pass B's FP32 stress sums read one 16-byte record per neighbour:
+    // stress_1 {sigma / rho^2, m}, stress_2 {sigma, m / rho}
+    if (m.pa_sph != 2) {
+      const double ir2 = 1.0 / (rho_i * rho_i);
+      d.srec[self] = make_float4((float)(s11 * ir2), (float)(s12 * ir2), (float)(s22 * ir2),
+                                 (float)m_i);
+    } else {
+      d.srec[self] = make_float4((float)s11, (float)s12, (float)s22, (float)(m_i / rho_i));
+    }
   }
   ((double *)d.div)[self] = div;
   if (want_diag) {
@@ -1692,10 +1697,24 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       const double w = c6_w(sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
     };
+    const bool pa2 = m.pa_sph == 2;
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
                            double q11, double q12, double q22) {
       const double gg = c6_grad(r, mh);
       const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+      if (pa2) {
+        // stress_2 (internal_stress.f90:189-216); as the pair's second
+        // particle the reference subtracts the term of the opposite gradient,
+        // the same value
+        const double coef = mq / kRhoIce / hthi / rq;
+        double cj = q11 * dw0 + q12 * dw1;
+        double ci = s11d * dw0 + s12d * dw1;
+        ax = ax + coef * (ci + cj);
+        cj = q12 * dw0 + q22 * dw1;
+        ci = s12d * dw0 + s22d * dw1;
+        ay = ay + coef * (ci + cj);
+        return;
+      }
       const double rq2 = rq * rq;
       const double coef = rhoi * mq / kRhoIce / hthi;
       double cj = (q11 * dw0 + q12 * dw1) / rq2;
@@ -1777,7 +1796,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       }
     }
     if (MIXED) {
-      const double cf = rhoi / kRhoIce / hthi;
+      const double cf = (pa2 ? 1.0 : rhoi) / kRhoIce / hthi;
       ax = cf * sx.value();
       ay = cf * sy.value();
     }

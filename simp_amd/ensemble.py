@@ -33,6 +33,7 @@ def _desc(sc, nsteps: int, keep: list) -> SimpInstanceDesc:
     d.nsteps = int(nsteps)
     d.tse = sc.tse
     d.den_alg = getattr(sc, "den_alg", 3)
+    d.pa_sph = getattr(sc, "pa_sph", 1)
     d.wind[0], d.wind[1] = sc.wind
     d.current[0], d.current[1] = sc.current
     d.max_hsml = sc.max_hsml

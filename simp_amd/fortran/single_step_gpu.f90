@@ -28,6 +28,7 @@ subroutine single_step
         call simp_config_default(cfg)
         cfg%max_hsml = max_hsml
         cfg%den_alg = den_alg
+        cfg%pa_sph = pa_sph
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

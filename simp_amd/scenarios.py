@@ -61,6 +61,7 @@ class Scenario:
     current: tuple = (0.0, 0.0)
     tse: int = 2
     den_alg: int = 3         # 3 diagnostic density, 1 summation density
+    pa_sph: int = 1          # 1 stress_1: sigma/rho^2 sums; 2 stress_2: over rho_i rho_j
     dt: float = 0.0
     time: float = 0.0
     name: str = ""

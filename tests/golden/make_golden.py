@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1]
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2]
 """
 from __future__ import annotations
 
@@ -176,11 +176,46 @@ def stale_kernel_zero():
           **_outputs(r, ("p_i", "p_j", "w", "kernel_zero")), out_nip=r["nip"], den_alg=1)
 
 
+def stress_2():
+    """8. the stress_2 particle approximation (pa_sph = 2,
+    internal_stress.f90:189-216), from the harness built with that
+    PARAMETER: one single_step of the closed box at t > 0 and ten
+    predictor-corrector steps of the ice edge."""
+    sc = S.box(nside=50, spacing=2000.0, seed=1)
+    sc.vx = np.random.default_rng(101).normal(0.0, 0.1, size=(sc.n, 2))
+    sc.time = 10.0
+    sc.pa_sph = 2
+    r = refrun.run(refrun.MODE_SINGLE_STEP, sc, harness=refrun.HARNESS_PA2)
+    _save("box2500_single_step_pa2.npz", **_inputs(sc),
+          **_outputs(r, ("dxdt", "dvxdt", "dhdt", "dAdt", "sigma")),
+          out_nip=r["nip"], out_bnip=r["bnip"], pa_sph=2)
+    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
+    sc.wind_grid = None
+    sc.wind = S.REFERENCE_WIND
+    sc.pa_sph = 2
+    K = 10
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
+                   harness=refrun.HARNESS_PA2)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save("edge3600_pc10_pa2.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end, pa_sph=2)
+
+
 if __name__ == "__main__":
-    # no argument: every fixture; otherwise the named groups ("main", "den1")
-    todo = sys.argv[1:] or ["main", "den1"]
+    # no argument: every fixture; otherwise the named groups ("main", "den1", "pa2")
+    todo = sys.argv[1:] or ["main", "den1", "pa2"]
     if "main" in todo:
         main()
     if "den1" in todo:
         summation_density()
         stale_kernel_zero()
+    if "pa2" in todo:
+        stress_2()

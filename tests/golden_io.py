@@ -23,6 +23,7 @@ def scenario(g, prefix="in_") -> Scenario:
                   massb=g[prefix + "massb"].copy(), max_hsml=float(max_hsml),
                   wind=REFERENCE_WIND, tse=2, dt=float(dt), time=float(t))
     sc.den_alg = int(g.get("den_alg", 3))
+    sc.pa_sph = int(g.get("pa_sph", 1))
     return sc
 
 

@@ -19,7 +19,9 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          ("edge3600_pc10.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          # summation density: the reference's sum_density runs on the host on
          # the kernel values and kernel_zero the shim exports
-         ("edge3600_pc10_den1.npz", refrun.HARNESS_DEN1, refrun.HARNESS_GPU_DEN1)]
+         ("edge3600_pc10_den1.npz", refrun.HARNESS_DEN1, refrun.HARNESS_GPU_DEN1),
+         # stress_2 (pa_sph = 2): the shim passes the module's pa_sph
+         ("edge3600_pc10_pa2.npz", refrun.HARNESS_PA2, refrun.HARNESS_GPU_PA2)]
 
 
 @pytest.mark.parametrize("name,cpu_exe,gpu_exe", CASES)

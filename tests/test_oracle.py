@@ -33,8 +33,25 @@ def test_single_step_bit_exact_vs_reference():
         assert _same(out[k], g["out_" + k]), k
 
 
+def test_stress_2_single_step_bit_exact_vs_reference():
+    """pa_sph = 2 (internal_stress.f90:189-216), against the reference built
+    with that PARAMETER; the two approximations differ on the same input."""
+    g = load("box2500_single_step_pa2.npz")
+    sc = scenario(g)
+    assert sc.pa_sph == 2
+    o = capi.Oracle()
+    out = o.single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    sc.pa_sph = 1
+    other = o.single_step(sc)
+    assert not _same(other["dvxdt"], g["out_dvxdt"])
+    assert _same(other["sigma"], g["out_sigma"])
+
+
 @pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz",
-                                  "edge3600_pc10_den1.npz"])
+                                  "edge3600_pc10_den1.npz", "edge3600_pc10_pa2.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
