@@ -55,7 +55,9 @@ typedef struct simp_config {
     int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89)            */
     int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96)       */
     int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119)     */
-    int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127)         */
+    int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127),        */
+                            /* eos.f90:141-147; 1: Hibler P (eos.f90:59);     */
+                            /* 2: Gutfraind (eos.f90:94-102)                  */
     int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134),     */
                             /* internal_stress.f90:105-136 (stress_1);         */
                             /* 2: (sigma_i + sigma_j)/(rho_i rho_j), :189-216  */
@@ -227,6 +229,9 @@ typedef struct simp_instance_desc {
     const double *grid_u, *grid_v;
     int32_t den_alg;                     /* 0 or 3: diagnostic; 1: summation   */
     int32_t pa_sph;                      /* 0 or 1: stress_1; 2: stress_2      */
+    int32_t eos_alg;                     /* 0 or 3: Kreyscher; 1: Hibler; 2:   */
+                                         /* Gutfraind                          */
+    int32_t reserved;
 } simp_instance_desc;
 
 int simp_batch_create(int precision, simp_batch **out);

@@ -95,6 +95,11 @@ d1objs=("${vobjs[@]}")
 variant pa2 'parameter(pa_sph = 1)' 'parameter(pa_sph = 2)'
 P2="$OUT/pa2"
 p2objs=("${vobjs[@]}")
+# Hibler and Gutfraind pressure (eos_alg = 1, 2; eos.f90:59, 94-102)
+variant eos1 'parameter(eos_alg = 3)' 'parameter(eos_alg = 1)'
+variant eos2 'parameter(eos_alg = 3)' 'parameter(eos_alg = 2)'
+E2="$OUT/eos2"
+e2objs=("${vobjs[@]}")
 
 # The drop-in check: the same harness and the reference's own
 # time_integration, with single_step replaced by the iso_c_binding shim of
@@ -121,4 +126,5 @@ if [ -f "$LIB" ]; then
     gpu_harness "$OUT/obj" "$OUT/mod" "$OUT/simp_ref_harness_gpu" "${objs[@]}"
     gpu_harness "$D1/obj" "$D1/mod" "$OUT/simp_ref_harness_gpu_den1" "${d1objs[@]}"
     gpu_harness "$P2/obj" "$P2/mod" "$OUT/simp_ref_harness_gpu_pa2" "${p2objs[@]}"
+    gpu_harness "$E2/obj" "$E2/mod" "$OUT/simp_ref_harness_gpu_eos2" "${e2objs[@]}"
 fi

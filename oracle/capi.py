@@ -22,7 +22,8 @@ class OrcParams(C.Structure):
                 ("grid_nx", C.c_int), ("grid_ny", C.c_int),
                 ("grid_x0", C.c_double), ("grid_y0", C.c_double),
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
-                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p), ("pa_sph", C.c_int)]
+                ("grid_u", C.c_void_p), ("grid_v", C.c_void_p), ("pa_sph", C.c_int),
+                ("eos_alg", C.c_int)]
 
 
 def build() -> None:
@@ -78,6 +79,7 @@ def make_params(sc) -> tuple:
     p.max_hsml = sc.max_hsml
     p.den_alg = getattr(sc, "den_alg", 3)
     p.pa_sph = getattr(sc, "pa_sph", 1)
+    p.eos_alg = getattr(sc, "eos_alg", 3)
     keep = ()
     if sc.wind_grid is not None:
         g = sc.wind_grid

@@ -26,6 +26,10 @@ HARNESS_GPU_DEN1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den1")
 #: the harness built with the stress_2 particle approximation (pa_sph = 2)
 HARNESS_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_pa2")
 HARNESS_GPU_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_pa2")
+#: the harnesses built with the Hibler (eos_alg = 1) and Gutfraind (2) pressure
+HARNESS_EOS1 = os.path.join(HERE, "_ref", "simp_ref_harness_eos1")
+HARNESS_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_eos2")
+HARNESS_GPU_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_eos2")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
 

@@ -435,7 +435,16 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double dcl = fabs(delta) < DELTA_MIN ? DELTA_MIN : delta;
         double z = 0.5 * P * (1 + K_T) / dcl;
         double et = z / (E_ELL * E_ELL);
-        double pr = z * 2 * delta / (1 + K_T);
+        double pr = z * 2 * delta / (1 + K_T);               /* kreyscher_eos */
+        if (prm->eos_alg == 1) {
+            pr = P;                                           /* hibler_eos (eos.f90:59) */
+        } else if (prm->eos_alg == 2) {
+            /* gutfraind_eos (eos.f90:94-102): rho_max = 934, rho_0 = rho_ice,
+             * negative pressures set to zero */
+            double gam = 1.9e5 * h[i] * exp(-(20.0 * (1 - A[i])));
+            pr = gam * (rho[i] - RHO_ICE) / (934.0 - rho[i]);
+            if (pr < 0.) pr = 0.;
+        }
         zeta[i] = z;
         eta[i] = et;
         p[i] = pr;

@@ -34,6 +34,8 @@ typedef struct orc_params {
     const double *grid_v;
     int    pa_sph;       /* 1 (or 0): stress_1 (internal_stress.f90:105-136);
                             2: stress_2 (:189-216) */
+    int    eos_alg;      /* 3 (or 0): kreyscher_eos (eos.f90:141-147);
+                            1: hibler_eos (:59); 2: gutfraind_eos (:94-102) */
 } orc_params;
 
 orc_ctx *orc_create(void);

@@ -63,7 +63,8 @@ class SimpInstanceDesc(C.Structure):
                 ("grid_x0", C.c_double), ("grid_y0", C.c_double),
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p),
-                ("den_alg", C.c_int32), ("pa_sph", C.c_int32)]
+                ("den_alg", C.c_int32), ("pa_sph", C.c_int32), ("eos_alg", C.c_int32),
+                ("reserved2", C.c_int32)]
 
 
 class SimpError(RuntimeError):

@@ -28,13 +28,14 @@ from ._lib import SimpConfig, check, lib, ptr
 
 def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(np.float32(-5.3))),
                 current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False,
-                den_alg: int = 3, pa_sph: int = 1) -> SimpConfig:
+                den_alg: int = 3, pa_sph: int = 1, eos_alg: int = 3) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
     cfg.tse = tse
     cfg.den_alg = den_alg
     cfg.pa_sph = pa_sph
+    cfg.eos_alg = eos_alg
     cfg.wind[0], cfg.wind[1] = wind
     cfg.current[0], cfg.current[1] = current
     cfg.max_hsml = max_hsml
@@ -58,9 +59,10 @@ class Simp:
 
     def __init__(self, precision: int = _lib.SIMP_FP64, tse: int = 2,
                  wind=(0.0, float(np.float32(-5.3))), current=(0.0, 0.0),
-                 max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3, pa_sph: int = 1):
+                 max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3, pa_sph: int = 1,
+                 eos_alg: int = 3):
         self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
-                               den_alg, pa_sph)
+                               den_alg, pa_sph, eos_alg)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -76,7 +78,8 @@ class Simp:
     def for_scenario(cls, sc, precision: int = _lib.SIMP_FP64) -> "Simp":
         return cls(precision=precision, tse=sc.tse, wind=sc.wind, current=sc.current,
                    max_hsml=sc.max_hsml, wind_grid=sc.wind_grid,
-                   den_alg=getattr(sc, "den_alg", 3), pa_sph=getattr(sc, "pa_sph", 1))
+                   den_alg=getattr(sc, "den_alg", 3), pa_sph=getattr(sc, "pa_sph", 1),
+                   eos_alg=getattr(sc, "eos_alg", 3))
 
     def close(self):
         if getattr(self, "_h", None):

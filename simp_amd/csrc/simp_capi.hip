@@ -78,7 +78,8 @@ static int validate(const simp_config *c) {
   if (c->precision != SIMP_FP64 && c->precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
   if (c->int_algorithm != 2 || c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
   if (c->tse != 0 && c->tse != 2) return SIMP_E_UNSUPPORTED;
-  if (c->rheology != 1 || c->bfa != 1 || c->eos_alg != 3) return SIMP_E_UNSUPPORTED;
+  if (c->rheology != 1 || c->bfa != 1) return SIMP_E_UNSUPPORTED;
+  if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
   if (c->skf != 6 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
   if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
@@ -131,6 +132,7 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
   in.max_hsml = c.max_hsml;
   in.den_alg = c.den_alg;
   in.pa_sph = c.pa_sph;
+  in.eos_alg = c.eos_alg;
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
     in.g_nx = h->g_nx; in.g_ny = h->g_ny;
@@ -400,6 +402,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.wind_mode != SIMP_WIND_UNIFORM && d.wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   if (d.den_alg != 0 && d.den_alg != 1 && d.den_alg != 3) return SIMP_E_UNSUPPORTED;
   if (d.pa_sph != 0 && d.pa_sph != 1 && d.pa_sph != 2) return SIMP_E_UNSUPPORTED;
+  if (d.eos_alg < 0 || d.eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))
@@ -412,6 +415,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.tse = d.tse;
   in.den_alg = d.den_alg == 1 ? 1 : 3;
   in.pa_sph = d.pa_sph == 2 ? 2 : 1;
+  in.eos_alg = d.eos_alg == 0 ? 3 : d.eos_alg;
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
   in.current[0] = d.current[0]; in.current[1] = d.current[1];

@@ -20,6 +20,7 @@ constexpr double kPi = 3.1415926535897932;
 constexpr double kPStar = 2.75e4;
 constexpr double kCEos = 20.0;
 constexpr double kRhoIce = 900.0;
+constexpr double kRhoMaxEos = 934.0;   // gutfraind_eos rho_max (eos.f90:88)
 constexpr double kRhoWater = 1026.0;
 constexpr double kRhoAir = (double)1.3f;
 constexpr double kCWind = 1.2e-3;
@@ -53,6 +54,8 @@ struct InstMeta {
   int g_off, g_nx, g_ny;  // wind grid range
   int den_alg;           // 3: rho = h rho_ice; 1: summation density
   int pa_sph;            // 1: stress_1; 2: stress_2
+  int eos_alg;           // 1: Hibler P; 2: Gutfraind; 3: Kreyscher
+  int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;
   double max_hsml;

@@ -24,6 +24,7 @@ struct InstanceInput {
   double max_hsml = 0.0;
   int den_alg = 3;       // 3: diagnostic density; 1: summation density
   int pa_sph = 1;        // 1: stress_1 (sigma / rho^2); 2: stress_2 (over rho_i rho_j)
+  int eos_alg = 3;       // 1: Hibler P; 2: Gutfraind; 3: Kreyscher replacement pressure
   // optional wind grid
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;

@@ -285,6 +285,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.max_hsml = in.max_hsml;
     m.den_alg = in.den_alg;
     m.pa_sph = in.pa_sph;
+    m.eos_alg = in.eos_alg;
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;

@@ -1280,7 +1280,8 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
   const double hth = MIXED ? hth_i : d.h[sb][self], Ac = MIXED ? A_i : d.A[sb][self];
-  const double rho_i = MIXED ? RS[self] : 0.0, m_i = MIXED ? MS[self] : 0.0;
+  const int eos = m.eos_alg;
+  const double rho_i = (MIXED || eos == 2) ? RS[self] : 0.0, m_i = MIXED ? MS[self] : 0.0;
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1288,7 +1289,18 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   const double dcl = fabs(delta) < kDeltaMin ? kDeltaMin : delta;
   const double z = 0.5 * P * (1 + kKT) / dcl;
   const double et = z / (kE * kE);
-  const double pr = z * 2 * delta / (1 + kKT);
+  double pr;
+  if (eos == 1) {
+    pr = P;                                    // hibler_eos (eos.f90:59)
+  } else if (eos == 2) {
+    // gutfraind_eos (eos.f90:94-102), rho_0 = rho_ice, negative
+    // pressures set to zero
+    const double gam = 1.9e5 * hth * exp(-(20.0 * (1 - Ac)));
+    pr = gam * (rho_i - kRhoIce) / (kRhoMaxEos - rho_i);
+    if (pr < 0.) pr = 0.;
+  } else {
+    pr = z * 2 * delta / (1 + kKT);            // kreyscher_eos (eos.f90:147)
+  }
   const double s11 = (z + et) * e11 + (z - et) * e22 - pr * (1 - kKT) / 2;
   const double s12 = 2 * et * e12;
   const double s22 = (z + et) * e22 + (z - et) * e11 - pr * (1 - kKT) / 2;

@@ -29,6 +29,7 @@ subroutine single_step
         cfg%max_hsml = max_hsml
         cfg%den_alg = den_alg
         cfg%pa_sph = pa_sph
+        cfg%eos_alg = eos_alg
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

@@ -62,6 +62,7 @@ class Scenario:
     tse: int = 2
     den_alg: int = 3         # 3 diagnostic density, 1 summation density
     pa_sph: int = 1          # 1 stress_1: sigma/rho^2 sums; 2 stress_2: over rho_i rho_j
+    eos_alg: int = 3         # 3 Kreyscher, 1 Hibler P, 2 Gutfraind pressure
     dt: float = 0.0
     time: float = 0.0
     name: str = ""

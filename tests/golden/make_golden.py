@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2]
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos]
 """
 from __future__ import annotations
 
@@ -209,9 +209,48 @@ def stress_2():
           integration_time=t_end, pa_sph=2)
 
 
+def pressures():
+    """9. the Hibler and Gutfraind pressures (eos_alg = 1, 2; eos.f90:59,
+    94-102), from harnesses built with those PARAMETERs: one single_step of
+    the closed box with thickness spread around 1 (rho = h rho_ice, so the
+    Gutfraind pressure is not zero) for each, and ten predictor-corrector
+    steps of the ice edge with eos_alg = 2."""
+    for eos, exe in ((1, refrun.HARNESS_EOS1), (2, refrun.HARNESS_EOS2)):
+        sc = S.box(nside=50, spacing=2000.0, seed=1)
+        rng = np.random.default_rng(103)
+        sc.vx = rng.normal(0.0, 0.1, size=(sc.n, 2))
+        sc.h = rng.uniform(0.9, 1.1, size=sc.n)
+        sc.rho = sc.h * 900.0
+        sc.time = 10.0
+        sc.eos_alg = eos
+        r = refrun.run(refrun.MODE_SINGLE_STEP, sc, harness=exe)
+        _save(f"box2500_single_step_eos{eos}.npz", **_inputs(sc),
+              **_outputs(r, ("dxdt", "dvxdt", "dhdt", "dAdt", "p", "sigma")),
+              out_nip=r["nip"], out_bnip=r["bnip"], eos_alg=eos)
+    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
+    sc.wind_grid = None
+    sc.wind = S.REFERENCE_WIND
+    sc.eos_alg = 2
+    K = 10
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
+                   harness=refrun.HARNESS_EOS2)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save("edge3600_pc10_eos2.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end, eos_alg=2)
+
+
 if __name__ == "__main__":
-    # no argument: every fixture; otherwise the named groups ("main", "den1", "pa2")
-    todo = sys.argv[1:] or ["main", "den1", "pa2"]
+    # no argument: every fixture; otherwise the named groups ("main", "den1",
+    # "pa2", "eos")
+    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos"]
     if "main" in todo:
         main()
     if "den1" in todo:
@@ -219,3 +258,5 @@ if __name__ == "__main__":
         stale_kernel_zero()
     if "pa2" in todo:
         stress_2()
+    if "eos" in todo:
+        pressures()

@@ -24,6 +24,7 @@ def scenario(g, prefix="in_") -> Scenario:
                   wind=REFERENCE_WIND, tse=2, dt=float(dt), time=float(t))
     sc.den_alg = int(g.get("den_alg", 3))
     sc.pa_sph = int(g.get("pa_sph", 1))
+    sc.eos_alg = int(g.get("eos_alg", 3))
     return sc
 
 

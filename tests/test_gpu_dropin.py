@@ -21,7 +21,9 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          # the kernel values and kernel_zero the shim exports
          ("edge3600_pc10_den1.npz", refrun.HARNESS_DEN1, refrun.HARNESS_GPU_DEN1),
          # stress_2 (pa_sph = 2): the shim passes the module's pa_sph
-         ("edge3600_pc10_pa2.npz", refrun.HARNESS_PA2, refrun.HARNESS_GPU_PA2)]
+         ("edge3600_pc10_pa2.npz", refrun.HARNESS_PA2, refrun.HARNESS_GPU_PA2),
+         # Gutfraind pressure (eos_alg = 2)
+         ("edge3600_pc10_eos2.npz", refrun.HARNESS_EOS2, refrun.HARNESS_GPU_EOS2)]
 
 
 @pytest.mark.parametrize("name,cpu_exe,gpu_exe", CASES)

@@ -50,8 +50,26 @@ def test_stress_2_single_step_bit_exact_vs_reference():
     assert _same(other["sigma"], g["out_sigma"])
 
 
+@pytest.mark.parametrize("eos", [1, 2])
+def test_pressure_switch_single_step_bit_exact_vs_reference(eos):
+    """eos_alg = 1 (hibler_eos, eos.f90:59) and 2 (gutfraind_eos, :94-102),
+    against the reference built with each PARAMETER; the Hibler pressure is
+    the ice strength P itself."""
+    g = load(f"box2500_single_step_eos{eos}.npz")
+    sc = scenario(g)
+    assert sc.eos_alg == eos
+    out = capi.Oracle().single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    if eos == 1:
+        P = 2.75e4 * sc.h * np.exp(-(20.0 * (1 - sc.A)))
+        assert _same(out["p"], P)
+
+
 @pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz",
-                                  "edge3600_pc10_den1.npz", "edge3600_pc10_pa2.npz"])
+                                  "edge3600_pc10_den1.npz", "edge3600_pc10_pa2.npz",
+                                  "edge3600_pc10_eos2.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
