@@ -118,7 +118,7 @@ class Engine {
   int next_id_ = 0;
   bool diag_ = false;
   bool any_boundary_ = false;
-  bool any_sumd_ = false;             // some instance uses summation density
+  bool any_xsw_ = false;              // some instance uses a non-default switch value
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
 };

@@ -498,7 +498,12 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   meta_ = nm;
   ids_ = nids;
   any_boundary_ = nbtot > 0;
-  any_sumd_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.den_alg == 1; });
+  // kernels with the switch values beyond the reference's defaults are
+  // launched only when some instance uses one (each instance then follows
+  // its own values)
+  any_xsw_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) {
+    return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3;
+  });
   if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
   if (ninst > 0) {
     // grid of the next evaluation of every instance
@@ -536,7 +541,7 @@ void Engine::enqueue_prefix(int buf) {
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const DevState &d = d_;
   enqueue_prefix(buf);
-  L("passA", [&] { launch_passA(d, buf, precision_, any_sumd_, want_diag ? 1 : 0, st_); });
+  L("passA", [&] { launch_passA(d, buf, precision_, any_xsw_, want_diag ? 1 : 0, st_); });
   if (phase == 2) {
     // all pair ranks: nip is reported
     L("pairtot", [&] { launch_pairtot(d, st_); });
@@ -544,7 +549,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   }
   // boundary-pair ranks and the head of the pair ranks the boundary kernel reads
   if (any_boundary_) L("ranks", [&] { launch_ranks(d, buf, st_); });
-  L("passB", [&] { launch_passB(d, precision_, any_sumd_, phase, st_); });
+  L("passB", [&] { launch_passB(d, precision_, any_xsw_, phase, st_); });
 }
 
 void Engine::enqueue_step() {
@@ -836,7 +841,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
   enqueue_prefix(0);
-  launch_passA(d, 0, precision_, any_sumd_, 0, st_);
+  launch_passA(d, 0, precision_, any_xsw_, 0, st_);
   if (any_boundary_) launch_ranks(d, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);

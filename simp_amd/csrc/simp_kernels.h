@@ -18,11 +18,11 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
 void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_cellsort(const DevState &d, int buf, hipStream_t st);
-void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
+void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);
 void launch_ranks(const DevState &d, int buf, hipStream_t st);
-void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st);
+void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,

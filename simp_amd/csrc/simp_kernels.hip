@@ -918,9 +918,10 @@ constexpr float kPad = 1.000005f;
 constexpr float kSkin = 0.01f;
 constexpr float kHalfPad = 0.5f * kPad;
 
-// SUMD: instances with summation density may be present (the density
-// sums are compiled in; each instance still follows its own den_alg)
-template <bool MIXED, bool SUMD>
+// XSW: instances with switch values beyond the reference's defaults may be
+// present (summation density, stress_2, the Hibler or Gutfraind pressure:
+// their code is compiled in; each instance still follows its own values)
+template <bool MIXED, bool XSW>
 __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
   __shared__ float4 tile[kTile];
   __shared__ unsigned short lst[kNbrMax * kTPB];
@@ -998,7 +999,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // set the kernel's length
   if (__any(hi > 1.4 * y.hmin)) __builtin_amdgcn_s_setprio(2);
   const int cy = c / W, cx = c - cy * W;
-  const double rhoi = SUMD ? RS[self] : 0.0;
+  const double rhoi = XSW ? RS[self] : 0.0;
   const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
 
   const int p = self;
@@ -1007,7 +1008,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // FP64 in both precision modes: the density sets the next smoothing
   // length, and through it the pair predicate d <= mhsml, which is not
   // continuous -- an FP32 error there flips borderline pairs.
-  const bool sumd = SUMD && m.den_alg == 1;
+  const bool sumd = XSW && m.den_alg == 1;
   double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
   double num = 0., den = 0.;
   KSum fdiv, f1, f2, f3, f4;
@@ -1280,7 +1281,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // (stress_tensor.f90:31-39).  FP64 in both precision modes: the
   // delta -> delta_min switch is decided here.
   const double hth = MIXED ? hth_i : d.h[sb][self], Ac = MIXED ? A_i : d.A[sb][self];
-  const int eos = m.eos_alg;
+  const int eos = XSW ? m.eos_alg : 3;
   const double rho_i = (MIXED || eos == 2) ? RS[self] : 0.0, m_i = MIXED ? MS[self] : 0.0;
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
@@ -1311,7 +1312,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   } else {
     // pass B's FP32 stress sums read one 16-byte record per neighbour:
     // stress_1 {sigma / rho^2, m}, stress_2 {sigma, m / rho}
-    if (m.pa_sph != 2) {
+    if (!XSW || m.pa_sph != 2) {
       const double ir2 = 1.0 / (rho_i * rho_i);
       d.srec[self] = make_float4((float)(s11 * ir2), (float)(s12 * ir2), (float)(s22 * ir2),
                                  (float)m_i);
@@ -1572,7 +1573,7 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 
 // phase: 0 = predictor (state S -> T), 1 = corrector (RHS on T, update S in
 // place), 2 = single evaluation (tendencies to d.tend)
-template <bool MIXED, bool SUMD, int phase>
+template <bool MIXED, bool XSW, int phase>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
   // neighbours per batch (the mixed variant's batch holds fewer registers)
@@ -1632,10 +1633,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // corrector's sum on the half-step kernel values with the densities
     // the second call produced (pass A's dsum)
     // (FP64 in both precision modes, as in pass A)
-    const bool sumd3 = SUMD && phase == 1 && m.den_alg == 1;
+    const bool sumd3 = XSW && phase == 1 && m.den_alg == 1;
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
-    const bool sd = SUMD && m.den_alg == 1;
+    const bool sd = XSW && m.den_alg == 1;
     double dsi = 1.0, dnumi = 0.;
     if (sd && phase != 2) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
@@ -1709,7 +1710,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       const double w = c6_w(sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
     };
-    const bool pa2 = m.pa_sph == 2;
+    const bool pa2 = XSW && m.pa_sph == 2;
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
                            double q11, double q12, double q22) {
       const double gg = c6_grad(r, mh);
@@ -2085,13 +2086,13 @@ void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
   k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
   k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
-void launch_passA(const DevState &d, int buf, int mixed, int sumd, int want_diag,
+void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
                   hipStream_t st) {
-  if (mixed && sumd)
+  if (mixed && xsw)
     k_passA<true, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else if (mixed)
     k_passA<true, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
-  else if (sumd)
+  else if (xsw)
     k_passA<false, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else
     k_passA<false, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
@@ -2103,21 +2104,21 @@ void launch_ranks(const DevState &d, int buf, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d, buf);
   k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
-template <bool MIXED, bool SUMD>
+template <bool MIXED, bool XSW>
 static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
   if (phase == 0)
-    k_passB<MIXED, SUMD, 0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    k_passB<MIXED, XSW, 0><<<g8(d.nblk), kTPB, 0, st>>>(d);
   else if (phase == 1)
-    k_passB<MIXED, SUMD, 1><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    k_passB<MIXED, XSW, 1><<<g8(d.nblk), kTPB, 0, st>>>(d);
   else
-    k_passB<MIXED, SUMD, 2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    k_passB<MIXED, XSW, 2><<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
-void launch_passB(const DevState &d, int mixed, int sumd, int phase, hipStream_t st) {
-  if (mixed && sumd)
+void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st) {
+  if (mixed && xsw)
     launch_passB_t<true, true>(d, phase, st);
   else if (mixed)
     launch_passB_t<true, false>(d, phase, st);
-  else if (sumd)
+  else if (xsw)
     launch_passB_t<false, true>(d, phase, st);
   else
     launch_passB_t<false, false>(d, phase, st);
