@@ -48,7 +48,8 @@ def test_config_defaults_are_the_reference_switches():
 
 @pytest.mark.parametrize("field,value", [("skf", 4), ("rheology", 2), ("tse", 1),
                                          ("den_alg", 2), ("int_algorithm", 1),
-                                         ("precision", 7)])
+                                         ("precision", 7), ("pa_sph", 3), ("eos_alg", 0),
+                                         ("eos_alg", 4)])
 def test_unsupported_switches_are_rejected_before_any_device_work(field, value):
     cfg = make_config()
     setattr(cfg, field, value)
