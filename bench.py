@@ -169,6 +169,28 @@ def hbm_traffic(kernel_substr, timeout=240):
                                                  "fetch_kib": fetch, "write_kib": write}
 
 
+def bench_field_error(sc):
+    """Mixed-precision error per field against the FP64 path (which the
+    parity tests pin to the oracle) at the workloads' own step counts: the
+    open ice edge (250k particles, 1,000 steps) and the Rankine vortex (40k
+    particles, 500 steps).  norm = max|a - b| / max|b|; elem = largest
+    |a_i - b_i| / (|b_i| + 1e-3 max|b|); delta = normwise error of the change
+    over the run (rho, vx, h, A); disp = displacement x - x0."""
+    from simp_amd import accuracy
+    from simp_amd import scenarios as S
+
+    def rounded(tab):
+        return {k: {m: float(f"{v:.3e}") for m, v in r.items() if m in ("norm", "elem", "delta")}
+                for k, r in tab.items()}
+    out = {"reference": "FP64 GPU path", "target_norm": 1e-4}
+    t_edge = accuracy.report(sc, 1000)
+    t_vort = accuracy.report(S.vortex(), 500)
+    out["ice_edge_250k_1000_steps"] = rounded(t_edge)
+    out["vortex_40k_500_steps"] = rounded(t_vort)
+    out["worst_norm"] = max(accuracy.worst(t_edge), accuracy.worst(t_vort))
+    return out
+
+
 def bench_dropin(sc, calls=10):
     """The per-call drop-in (simp_single_step): one right-hand side on host
     arrays per call, the state copied in and the tendencies and diagnostics
@@ -379,6 +401,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    line["field_error"] = bench_field_error(sc)
     line["dropin_single_step"] = bench_dropin(sc)
     line["snapshots"] = bench_snapshots(sc)
     if not args.no_ensemble:

@@ -59,6 +59,7 @@ def lib():
                                  C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]
         L.orc_sum_density.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_get_kernel.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_wind_at.argtypes = [C.POINTER(OrcParams), C.c_int, _dp, _dp]
         L.orc_vp_dt.argtypes = [C.c_int, _dp]
         L.orc_vp_dt.restype = C.c_double
         _lib = L
@@ -188,6 +189,15 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_sum_density failed ({rc})")
         return rho
+
+    def wind_at(self, sc, pts) -> np.ndarray:
+        """The wind (u, v) the forcing term of ``sc`` sees at points
+        ``pts`` (k, 2)."""
+        prm, keep = make_params(sc)
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 2)
+        uv = np.zeros_like(pts)
+        self._L.orc_wind_at(C.byref(prm), pts.shape[0], pts, uv)
+        return uv
 
     def run_pc(self, sc, nsteps: int, itimestep: int = 0) -> dict:
         """Advance ``sc`` in place by ``nsteps`` predictor-corrector steps."""

@@ -342,6 +342,13 @@ static void wind_at(const orc_params *prm, double px, double py, double *u,
     *v = (1.0 - ty) * v0 + ty * v1;
 }
 
+/* The wind the forcing term sees at n points (x interleaved; uv receives
+ * (u, v) interleaved): the sampler wind_forcing's term uses. */
+void orc_wind_at(const orc_params *prm, int n, const double *x, double *uv) {
+    for (int i = 0; i < n; ++i)
+        wind_at(prm, x[2 * i], x[2 * i + 1], uv + 2 * i, uv + 2 * i + 1);
+}
+
 static int ensure_scratch(orc_ctx *c, int n) {
     if (n <= c->nscr) return 0;
     free(c->div); free(c->grad);

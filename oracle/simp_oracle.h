@@ -87,6 +87,10 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n,
  * of the last orc_single_step on this context; rho is updated in place. */
 int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho);
 
+/* Wind at n points (x interleaved, uv interleaved out): uniform, or the
+ * bilinear sample of the grid clamped at its edge. */
+void orc_wind_at(const orc_params *prm, int n, const double *x, double *uv);
+
 /* VP_time_step (time_step.f90:155). */
 double orc_vp_dt(int n, const double *hsml);
 

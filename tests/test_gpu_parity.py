@@ -113,6 +113,30 @@ def test_time_integration_fp64_vs_oracle_gridded_wind():
         assert e <= 1e-12, (k, e)
 
 
+@pytest.mark.parametrize("case", ["vortex", "ice_edge"])
+def test_single_step_gridded_wind_vs_oracle(case):
+    """One right-hand side with the wind on (current_time > 0) and moving
+    ice: the wind term of dvxdt samples the grid at every particle, so the
+    sampler (pinned on the oracle side by tests/test_wind.py) is compared
+    particle by particle."""
+    say(f"single_step gridded wind {case}")
+    if case == "vortex":
+        sc = S.vortex(nside=80, spacing=1000.0, seed=7)
+    else:
+        sc = S.ice_edge(nside=80, spacing=1000.0, seed=7)
+    sc.vx = np.random.default_rng(9).normal(0.0, 0.3, size=(sc.n, 2))
+    sc.time = 3.0
+    got = Simp.for_scenario(sc).single_step(sc)
+    want = capi.Oracle().single_step(sc)
+    for k in ("dvxdt", "dhdt", "sigma"):
+        if case == "vortex":
+            assert np.array_equal(got[k], want[k]), (k, nrel(got[k], want[k]))
+        else:
+            # A < 1 in the ramp: the ice strength's exp may differ from the
+            # host's in the last bit
+            assert nrel(got[k], want[k]) <= 1e-12, (k, nrel(got[k], want[k]))
+
+
 def test_candidate_lists_across_their_skin():
     """Fast random motion (10 m/s, fixed dt 0.05 s): the candidate lists'
     movement bound (3 dt max|v| per evaluation against a skin of
