@@ -8,11 +8,12 @@
 #  * modules are compiled in dependency order (globals, data_variables,
 #    domain, boundary_domain, mpi_variables, tree_module, interfaces); the
 #    reference Makefile's order (Makefile:114-121) does not build from clean.
-#  * no -fopenmp: flang rejects the REDUCTION(-:...) clause at
-#    single_step.f90:55, and the sources are not edited.  Without OpenMP the
-#    directives are comments and the build is the serial reference, the same
-#    arithmetic order as OMP_NUM_THREADS=1.  The omp_get_* calls that remain
-#    are ordinary calls resolved by libomp at link time.
+#  * the golden-vector builds use no -fopenmp: without OpenMP the directives
+#    are comments and the build is the serial reference, the same arithmetic
+#    order as OMP_NUM_THREADS=1.  The omp_get_* calls that remain are
+#    ordinary calls resolved by libomp at link time.  The OpenMP build
+#    (omp_build below, timings only) rewrites the REDUCTION(-:...) clause at
+#    single_step.f90:55, which flang rejects, on the way into the compiler.
 #  * the shipped compiled module files (fortran/src/modules/*.mod) are never
 #    put on the include path; modules are regenerated from source.
 set -euo pipefail
@@ -100,6 +101,52 @@ variant eos1 'parameter(eos_alg = 3)' 'parameter(eos_alg = 1)'
 variant eos2 'parameter(eos_alg = 3)' 'parameter(eos_alg = 2)'
 E2="$OUT/eos2"
 e2objs=("${vobjs[@]}")
+
+# The OpenMP build, for the CPU baseline: the reference compiled as its
+# Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
+# Two edits go in on the way into the compiler (stdin, as above; no source
+# copy is written):
+#  * maxn = 300000 (globals.f90:22; max_interaction = 40 maxn follows);
+#  * single_step.f90:55 REDUCTION(-:...) written REDUCTION(+:...): flang
+#    rejects the '-' reduction identifier (deprecated since OpenMP 5.2), and
+#    OpenMP combines the private copies of a '-' reduction by addition, so
+#    the two clauses are the same reduction.
+# Timings only: with more than one thread the pair lists are merged in a
+# thread-dependent order (nnps.f90:426-433), so no golden vector comes from
+# this build.
+omp_build() {
+    local V="$OUT/omp" FO="-O2 -ffp-contract=off -w -fopenmp"
+    mkdir -p "$V/obj" "$V/mod"
+    local oobjs=()
+    grep -q 'maxn = 80000' "$SRC/modules/globals.f90"
+    grep -q 'REDUCTION(-:' "$SRC/integration/single_step.f90"
+    sed 's/maxn = 80000/maxn = 300000/' "$SRC/modules/globals.f90" \
+        | "$FC" $FO -x f95 -J"$V/mod" -c - -o "$V/obj/globals.o"
+    oobjs+=("$V/obj/globals.o")
+    for m in data_variables domain boundary_domain mpi_variables tree_module interfaces; do
+        "$FC" $FO -I"$V/mod" -J"$V/mod" -c "$SRC/modules/$m.f90" -o "$V/obj/$m.o"
+        oobjs+=("$V/obj/$m.o")
+    done
+    for d in ext_force in_out int_force integration neighbor phy_quantities sph; do
+        for f in "$SRC/$d"/*.f90; do
+            b="$(basename "$f" .f90)"
+            if [ "$d/$b" = "integration/single_step" ]; then
+                sed 's/REDUCTION(-:/REDUCTION(+:/' "$f" \
+                    | "$FC" $FO -I"$V/mod" -J"$V/mod" -x f95 -c - -o "$V/obj/$b.o"
+            else
+                "$FC" $FO -I"$V/mod" -J"$V/mod" -c "$f" -o "$V/obj/$b.o"
+            fi
+            oobjs+=("$V/obj/$b.o")
+        done
+    done
+    "$FC" $FO -I"$V/mod" -J"$V/mod" -c "$HERE/ref_harness.f90" -o "$V/obj/ref_harness.o"
+    "$FC" $FO -o "$OUT/simp_ref_harness_omp" "$V/obj/ref_harness.o" "${oobjs[@]}" \
+        -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
+    echo "build_ref: $OUT/simp_ref_harness_omp"
+}
+omp_build
+# the serial build with the same capacity, for the one-core figure at full size
+variant big 'maxn = 80000' 'maxn = 300000'
 
 # The drop-in check: the same harness and the reference's own
 # time_integration, with single_step replaced by the iso_c_binding shim of

@@ -30,8 +30,13 @@ HARNESS_GPU_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_pa2")
 HARNESS_EOS1 = os.path.join(HERE, "_ref", "simp_ref_harness_eos1")
 HARNESS_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_eos2")
 HARNESS_GPU_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_eos2")
+#: the OpenMP build sized for the 250k workload (maxn = 300000), timings only
+HARNESS_OMP = os.path.join(HERE, "_ref", "simp_ref_harness_omp")
+#: the serial build with the same capacity
+HARNESS_BIG = os.path.join(HERE, "_ref", "simp_ref_harness_big")
 MAXN = 80000    # globals.f90:22
 MAXNB = 5000    # globals.f90:23
+MAXN_OMP = 300000
 
 MODE_NNPS, MODE_SINGLE_STEP, MODE_TIME_INTEGRATION, MODE_TWO_STATES = 1, 2, 3, 4
 
@@ -112,12 +117,13 @@ def run(mode, sc, reps=1, integration_time=0.0, threads=1, timeout=3600, second=
     exe = harness or HARNESS
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
-    if sc.n > MAXN or sc.nb > MAXNB:
+    maxn = MAXN_OMP if exe in (HARNESS_OMP, HARNESS_BIG) else MAXN
+    if sc.n > maxn or sc.nb > MAXNB:
         raise ValueError("scenario exceeds the reference's static capacity")
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         _write_input(fin, mode, sc, reps, integration_time, second)
-        env = dict(os.environ, PWD=d, OMP_NUM_THREADS=str(threads))
+        env = dict(os.environ, PWD=d, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="256M")
         subprocess.run([exe, fin, fout], cwd=d, env=env, check=True,
                        stdout=subprocess.DEVNULL, timeout=timeout,
                        preexec_fn=_big_stack)
