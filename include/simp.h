@@ -123,6 +123,21 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
 int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
                        double *w, double *kernel_zero);
 
+/* ---- interaction_stats (neighbor/nnps.f90:613-658) ----
+ * The neighbour statistics the reference prints from NNPS at every step that
+ * is a multiple of print_step (nnps.f90:512-515): the largest and smallest
+ * number of interactions of one particle (particle pairs plus boundary pairs
+ * it is in), their average over the particles (REAL(4) division as
+ * nnps.f90:651), the number of particles with none, and the number of
+ * particle pairs, of the last evaluation that collected them -- every
+ * evaluation of such a step and every simp_single_step call; itimestep
+ * receives that evaluation's step.  Counts above 4095 for one particle are
+ * reported as 4095.  SIMP_E_STATE before any collecting evaluation.  The
+ * >200-interactions STOP of nnps.f90:660-668 is the SIMP_E_CAPACITY status
+ * of the stepping calls. */
+int simp_interaction_stats(simp_handle *h, int *maxip, int *minip, double *avgip,
+                           int *noip, long long *nip, int *itimestep);
+
 /* ---- NNPS (neighbor/nnps.f90:21-30 -> bucket_search :155-517) ----
  * The pair lists in the reference's exact order (single-thread order):
  * p_i, p_j 1-based, r_ij, hsml_ij; boundary pairs bp_i (boundary particle),
@@ -239,6 +254,11 @@ int simp_batch_destroy(simp_batch *b);
 /* Add count instances; ids[k] receives the id of desc[k]. */
 int simp_batch_add(simp_batch *b, int count, const simp_instance_desc *desc, int *ids);
 int simp_batch_remove(simp_batch *b, int count, const int *ids);
+/* Remove nremove instances and add nadd new ones (ids receives their ids)
+ * with one rebuild of the packed layout: what a request loop does at a
+ * step boundary where some simulations finish and others are admitted. */
+int simp_batch_update(simp_batch *b, int nremove, const int *remove_ids, int nadd,
+                      const simp_instance_desc *desc, int *ids);
 /* Advance every active instance by nsteps predictor-corrector steps. */
 int simp_batch_step(simp_batch *b, int nsteps, int sync);
 int simp_batch_sync(simp_batch *b);
@@ -254,6 +274,10 @@ int simp_batch_count(simp_batch *b, int *instances, long long *particles);
  * to cap entries and returns the number of instances (or -status). */
 int simp_batch_list(simp_batch *b, int cap, int *ids, int *itimestep, int *steps_left,
                     int *active, int *status);
+/* simp_interaction_stats of one instance of the batch. */
+int simp_batch_interaction_stats(simp_batch *b, int id, int *maxip, int *minip,
+                                 double *avgip, int *noip, long long *nip,
+                                 int *itimestep);
 /* Device bytes an instance would occupy (for admission against a budget). */
 long long simp_instance_footprint(const simp_instance_desc *desc);
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "simp_download", "simp_set_diagnostics", "simp_download_diagnostics",
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
+    "simp_batch_update", "simp_interaction_stats", "simp_batch_interaction_stats",
     "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
@@ -116,6 +117,9 @@ def lib():
     L.simp_batch_destroy.argtypes = [_vp]
     L.simp_batch_add.argtypes = [_vp, C.c_int, C.POINTER(SimpInstanceDesc), _vp]
     L.simp_batch_remove.argtypes = [_vp, C.c_int, _vp]
+    L.simp_interaction_stats.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.simp_batch_interaction_stats.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.simp_batch_update.argtypes = [_vp, C.c_int, _vp, C.c_int, C.POINTER(SimpInstanceDesc), _vp]
     L.simp_batch_step.argtypes = [_vp, C.c_int, C.c_int]
     L.simp_batch_sync.argtypes = [_vp]
     L.simp_batch_status.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]

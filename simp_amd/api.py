@@ -54,6 +54,15 @@ def _state_arrays(sc):
             _c(sc.xb), _c(sc.massb)]
 
 
+def _stats(call) -> dict:
+    mx, mn, nz, it = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    avg, nip = C.c_double(), C.c_longlong()
+    check(call(C.byref(mx), C.byref(mn), C.byref(avg), C.byref(nz), C.byref(nip), C.byref(it)),
+          "simp_interaction_stats")
+    return {"max": mx.value, "min": mn.value, "average": avg.value, "no_interaction": nz.value,
+            "nip": nip.value, "itimestep": it.value}
+
+
 class Simp:
     """One simulation context on the current GPU."""
 
@@ -136,6 +145,12 @@ class Simp:
         check(st, "simp_single_step")
         out["nip"], out["bnip"] = nip.value, bnip.value
         return out
+
+    def interaction_stats(self) -> dict:
+        """``interaction_stats`` (``neighbor/nnps.f90:613-658``) of the last
+        evaluation that collected it: every ``single_step`` and both
+        evaluations of a step that is a multiple of print_step (2000)."""
+        return _stats(lambda *a: lib().simp_interaction_stats(self._h, *a))
 
     def kernel_values(self, n: int, cap: Optional[int] = None) -> dict:
         """The last single_step's pairs with their kernel values w and the

@@ -30,6 +30,7 @@ static int map_rc(int rc) {
     case -1: return SIMP_E_ARG;
     case -3: return SIMP_E_CAPACITY;
     case -4: return SIMP_E_NAN;
+    case -5: return SIMP_E_STATE;
     case -6: return SIMP_E_NOMEM;
     default: return SIMP_E_HIP;
   }
@@ -217,6 +218,23 @@ int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j, do
   if (rc) return map_rc(rc);
   if (kernel_zero) rc = h->eng->download_kernel_zero(0, kernel_zero);
   return map_rc(rc);
+}
+
+static int stats_out(Engine *e, int k, int n, int *maxip, int *minip, double *avgip,
+                     int *noip, long long *nip, int *itimestep) {
+  long long sum = 0;
+  int rc = e->interaction_stats(k, maxip, minip, &sum, noip, nip, itimestep);
+  if (rc) return map_rc(rc);
+  // real(sumip) / real(ntotal): a REAL(4) division (nnps.f90:651)
+  if (avgip) *avgip = (double)((float)sum / (float)n);
+  return SIMP_OK;
+}
+
+int simp_interaction_stats(simp_handle *h, int *maxip, int *minip, double *avgip, int *noip,
+                           long long *nip, int *itimestep) {
+  if (!h) return SIMP_E_ARG;
+  if (h->eng->ninst() != 1) return SIMP_E_STATE;
+  return stats_out(h->eng, 0, h->eng->meta(0).n, maxip, minip, avgip, noip, nip, itimestep);
 }
 
 int simp_upload(simp_handle *h, int n, const double *x, const double *vx, const double *mass,
@@ -478,6 +496,26 @@ int simp_batch_remove(simp_batch *b, int count, const int *ids) {
   return map_rc(b->eng->remove(v));
 }
 
+int simp_batch_update(simp_batch *b, int nremove, const int *remove_ids, int nadd,
+                      const simp_instance_desc *desc, int *ids) {
+  if (!b || nremove < 0 || nadd < 0 || (nremove > 0 && !remove_ids) ||
+      (nadd > 0 && (!desc || !ids)))
+    return SIMP_E_ARG;
+  std::vector<int> rm(remove_ids, remove_ids + nremove);
+  for (int id : rm)
+    if (b->eng->index_of(id) < 0) return SIMP_E_ARG;
+  std::vector<InstanceInput> ins(nadd);
+  for (int k = 0; k < nadd; ++k) {
+    int rc = fill_from_desc(desc[k], ins[k]);
+    if (rc) return rc;
+  }
+  std::vector<int> got;
+  int rc = b->eng->update(rm, ins, &got);
+  if (rc) return map_rc(rc);
+  for (int k = 0; k < nadd; ++k) ids[k] = got[k];
+  return SIMP_OK;
+}
+
 int simp_batch_step(simp_batch *b, int nsteps, int sync) {
   if (!b || nsteps < 0) return SIMP_E_ARG;
   int rc = b->eng->steps(nsteps, true);
@@ -519,6 +557,14 @@ int simp_batch_download(simp_batch *b, int id, double *x, double *vx, double *rh
   if (rc) return map_rc(rc);
   return map_rc(b->eng->download_scalars(k, dt, current_time, itimestep, nullptr, nullptr,
                                          nullptr));
+}
+
+int simp_batch_interaction_stats(simp_batch *b, int id, int *maxip, int *minip, double *avgip,
+                                 int *noip, long long *nip, int *itimestep) {
+  if (!b) return SIMP_E_ARG;
+  const int k = b->eng->index_of(id);
+  if (k < 0) return SIMP_E_ARG;
+  return stats_out(b->eng, k, b->eng->meta(k).n, maxip, minip, avgip, noip, nip, itimestep);
 }
 
 int simp_batch_count(simp_batch *b, int *instances, long long *particles) {

@@ -86,6 +86,13 @@ struct InstDyn {
   // largest speed of the last evaluated state
   int vbuilt;
   float vskin, vskin_next, vmoved, vprev;
+  // interaction statistics (interaction_stats, nnps.f90:613-658) of the last
+  // evaluation that collected them: every evaluation of a step that is a
+  // multiple of print_step, and every single evaluation.  ipstep: its
+  // itimestep (-1: none yet); per-particle interactions = pairs + boundary
+  // pairs of the particle
+  int ipstep, ipmax, ipmin, ipzero;
+  unsigned long long ipsum, ipown;    // sum of interactions; particle pairs
   int sel;              // state set holding the instance at the start of an evaluation
   int rb;               // 1: its lists are rebuilt in this evaluation (the state
                         //    then moves to set sel ^ 1: the current set is sel ^ rb)

@@ -57,6 +57,9 @@ class Engine {
   // layout is rebuilt on the device, live state is copied device to device.
   int add(const std::vector<InstanceInput> &ins, std::vector<int> *ids);
   int remove(const std::vector<int> &ids);
+  // Remove instances by id and add new ones in one layout rebuild.
+  int update(const std::vector<int> &remove_ids, const std::vector<InstanceInput> &ins,
+             std::vector<int> *ids);
   int index_of(int id) const;
   const std::vector<int> &ids() const { return ids_; }
 
@@ -73,6 +76,11 @@ class Engine {
                        int *err);
   int download_status(int inst, int *itimestep, int *steps_left, int *active, int *err);
   int download_all_status(std::vector<InstDyn> &out);
+  // interaction statistics of instance k's last collecting evaluation
+  // (-5: none yet): max, min, sum over particles, particles with none,
+  // particle pairs, and the evaluation's itimestep
+  int interaction_stats(int k, int *maxip, int *minip, long long *sumip, int *noip,
+                        long long *nip, int *step);
   int download_tend(int inst, double *dxdt, double *dvxdt, double *dhdt, double *dAdt);
   int download_diag(int inst, double *eps3, double *zeta, double *eta, double *p,
                     double *sig3);

@@ -243,6 +243,15 @@ int Engine::remove(const std::vector<int> &ids) {
   return repack(keep, {}, -1, nullptr);
 }
 
+int Engine::update(const std::vector<int> &remove_ids, const std::vector<InstanceInput> &ins,
+                   std::vector<int> *ids) {
+  std::vector<int> keep;
+  for (size_t k = 0; k < meta_.size(); ++k)
+    if (std::find(remove_ids.begin(), remove_ids.end(), ids_[k]) == remove_ids.end())
+      keep.push_back((int)k);
+  return repack(keep, ins, -1, ids);
+}
+
 // Rebuild the packed layout from the kept instances (old packed indices,
 // their live state copied device to device) followed by the added ones
 // (copied from the host).  carry_store_from: old index whose hsml_ij store
@@ -456,6 +465,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     y.time = in.time;
     y.itimestep = in.itimestep;
     y.active = 1;
+    y.ipstep = -1;
     y.steps_left = in.nsteps > 0 ? in.nsteps : -1;
     y.rchg = 1;
   }
@@ -560,6 +570,7 @@ void Engine::enqueue_step() {
 }
 
 int Engine::eval_single(bool want_diag) {
+  launch_stats_reset(d_, st_);
   enqueue_eval(0, 2, want_diag);
   launch_fold(d_, st_);
   return sync_and_check();
@@ -628,6 +639,22 @@ int Engine::download_status(int k, int *itimestep, int *steps_left, int *active,
   if (steps_left) *steps_left = y.steps_left;
   if (active) *active = y.active;
   if (err) *err = y.err;
+  return 0;
+}
+
+int Engine::interaction_stats(int k, int *maxip, int *minip, long long *sumip, int *noip,
+                              long long *nip, int *step) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  InstDyn y;
+  HIPCHK(hipMemcpy(&y, d_.dyn + k, sizeof(y), hipMemcpyDeviceToHost));
+  if (y.ipstep < 0 || y.ipmin == 0x7fffffff) return -5;
+  if (maxip) *maxip = y.ipmax;
+  if (minip) *minip = y.ipmin;
+  if (sumip) *sumip = (long long)y.ipsum;
+  if (noip) *noip = y.ipzero;
+  if (nip) *nip = (long long)y.ipown;
+  if (step) *step = y.ipstep;
   return 0;
 }
 

@@ -10,6 +10,8 @@ void launch_partial(const DevState &d, int buf, hipStream_t st);
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st);
 // end of an evaluation outside a step (single evaluation, pair export)
 void launch_fold(const DevState &d, hipStream_t st);
+// every instance collects interaction statistics in the next evaluation
+void launch_stats_reset(const DevState &d, hipStream_t st);
 void launch_count(const DevState &d, int buf, hipStream_t st);
 // gate: device flag; the scan does nothing while it reads 0 (nullptr: always runs)
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,

@@ -362,6 +362,39 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
 __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
                              double ymn, double ymx, double hmn);
 
+__device__ __forceinline__ void stats_reset(InstDyn &y) {
+  y.ipstep = y.itimestep;
+  y.ipmax = 0;
+  y.ipmin = 0x7fffffff;
+  y.ipzero = 0;
+  y.ipsum = 0;
+  y.ipown = 0;
+}
+
+// Interaction statistics of one wave's particles (every lane takes part;
+// lanes without a particle pass live = false): max, min, sum and the count
+// of particles with no interaction, then the owned particle pairs.
+__device__ __forceinline__ void stats_add(InstDyn &y, bool live, int cnt, int own) {
+  int mx = live ? cnt : 0, mn = live ? cnt : 0x7fffffff, z = (live && cnt == 0) ? 1 : 0;
+  unsigned long long sm = live ? (unsigned long long)cnt : 0ull;
+  unsigned long long ow = live ? (unsigned long long)own : 0ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = max(mx, __shfl_xor(mx, o, 64));
+    mn = min(mn, __shfl_xor(mn, o, 64));
+    z += __shfl_xor(z, o, 64);
+    sm += __shfl_xor(sm, o, 64);
+    ow += __shfl_xor(ow, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && __any(live)) {
+    atomicMax(&y.ipmax, mx);
+    atomicMin(&y.ipmin, mn);
+    atomicAdd(&y.ipzero, z);
+    atomicAdd(&y.ipsum, sm);
+    atomicAdd(&y.ipown, ow);
+  }
+}
+
 
 // end of an evaluation: the current set becomes the instance's set
 __device__ __forceinline__ void fold_set(InstDyn &y) {
@@ -389,6 +422,12 @@ __device__ __forceinline__ void move_bound(InstDyn &y, double vmax, double dhmax
   const double v = fmax(vmax, (double)y.vprev);
   y.vmoved = (float)((double)y.vmoved + 3.0 * y.dt * v + dhmax) * 1.00001f;
   y.vprev = (float)vmax * 1.00001f;
+}
+
+// a single evaluation collects the interaction statistics whatever its step
+__global__ void __launch_bounds__(kTPB) k_stats_reset(DevState d) {
+  const int k = blockIdx.x * kTPB + threadIdx.x;
+  if (k < d.ninst) stats_reset(d.dyn[k]);
 }
 
 __global__ void __launch_bounds__(kTPB) k_fold(DevState d) {
@@ -494,6 +533,8 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
   y.W = (int)W;
   y.H = (int)H;
   y.stamp += 1;
+  // the evaluations of a print step collect the interaction statistics
+  if (y.itimestep % kPrintStep == 0) stats_reset(y);
   return 1;
 }
 
@@ -1627,6 +1668,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   const bool live = ok && !inst_off(y);
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   double speed = 0.0, dhs = 0.0;
+  int st_cnt = 0, st_own = 0;                          // interaction statistics
   if (live) {
     const int W = y.W;
     // third density call of the step (time_integration.f90:262): the
@@ -1697,6 +1739,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     for (int u = 0; u < kB; ++u) sj[u] = (listed && u < nn) ? nx[u] : self;
 
     double ax = 0., ay = 0.;
+    int nbnd = 0;          // boundary pairs of this particle
     // stress_1 (internal_stress.f90:103-136), over pass A's neighbour list
     // in batches of kB neighbours whose loads are issued together
     // waves with many or boundary neighbours get issue priority
@@ -1908,11 +1951,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         ax = ax - (double)fbx / rhoi;
         ay = ay - (double)fby / rhoi;
       }
-      // interaction_stats capacity check (nnps.f90:660-668)
-      if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) + nbn > kMaxIndInt)
+      nbnd = nbn;
+    }
+
+    // interaction_stats (nnps.f90:613-668) at the steps the reference runs
+    // it, and in every single evaluation: the capacity STOP, the statistics
+    if (phase == 2 || y.itimestep % kPrintStep == 0) {
+      const int oc = d.own_cnt[self];
+      st_cnt = (oc >> 20) + nbnd;
+      st_own = (oc & 1023) + ((oc >> 10) & 1023);
+      if (y.itimestep % kPrintStep == 0 && st_cnt > kMaxIndInt)
         atomicOr(&d.dyn[k].err, kErrCapacity);
-    } else if (y.itimestep % kPrintStep == 0 && (d.own_cnt[self] >> 20) > kMaxIndInt) {
-      atomicOr(&d.dyn[k].err, kErrCapacity);
     }
 
     // wind_forcing / water_drag (wind_forcing.f90:35-50, water_drag.f90:35-51)
@@ -1966,6 +2015,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       d.tend[5 * N + p] = dAdt;
     }
   }
+  // (with every lane of the wave: a wave holds one instance)
+  if (phase == 2 || y.itimestep % kPrintStep == 0) stats_add(d.dyn[k], live, st_cnt, st_own);
   if (phase != 2) block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
 }
 
@@ -2060,6 +2111,9 @@ void launch_partial(const DevState &d, int buf, hipStream_t st) {
 }
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
   k_finalize<<<g8(d.ninst), kTPB, 0, st>>>(d, end_of_step);
+}
+void launch_stats_reset(const DevState &d, hipStream_t st) {
+  k_stats_reset<<<cdiv(std::max(d.ninst, 1), kTPB), kTPB, 0, st>>>(d);
 }
 void launch_fold(const DevState &d, hipStream_t st) {
   k_fold<<<cdiv(std::max(d.ninst, 1), kTPB), kTPB, 0, st>>>(d);

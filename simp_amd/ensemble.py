@@ -20,7 +20,7 @@ from . import _lib
 from ._lib import SimpInstanceDesc, check, lib
 
 
-def _desc(sc, nsteps: int, keep: list) -> SimpInstanceDesc:
+def _desc(sc, nsteps: int, keep: list, itimestep: int = 0) -> SimpInstanceDesc:
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
             (sc.x, sc.vx, sc.mass, sc.rho, sc.hsml, sc.h, sc.A, sc.xb, sc.massb)]
     keep.extend(arrs)
@@ -29,7 +29,7 @@ def _desc(sc, nsteps: int, keep: list) -> SimpInstanceDesc:
     (d.x, d.vx, d.mass, d.rho, d.hsml, d.hth, d.A, d.xb, d.massb) = [
         a.ctypes.data if a.size else None for a in arrs]
     d.dt, d.current_time = sc.dt, sc.time
-    d.itimestep = 0
+    d.itimestep = int(itimestep)
     d.nsteps = int(nsteps)
     d.tse = sc.tse
     d.den_alg = getattr(sc, "den_alg", 3)
@@ -77,11 +77,15 @@ class Ensemble:
         except Exception:
             pass
 
-    def add(self, scenarios: Sequence, nsteps: Optional[Iterable[int]] = None) -> List[int]:
+    def add(self, scenarios: Sequence, nsteps: Optional[Iterable[int]] = None,
+            itimestep: int = 0) -> List[int]:
+        """Add instances (each may carry a step budget); their step counters
+        start at ``itimestep``."""
         scs = list(scenarios)
         ns = list(nsteps) if nsteps is not None else [0] * len(scs)
         keep: list = []
-        arr = (SimpInstanceDesc * len(scs))(*[_desc(sc, k, keep) for sc, k in zip(scs, ns)])
+        arr = (SimpInstanceDesc * len(scs))(*[_desc(sc, k, keep, itimestep)
+                                              for sc, k in zip(scs, ns)])
         ids = np.zeros(len(scs), np.int32)
         check(lib().simp_batch_add(self._h, len(scs), arr, ids.ctypes.data), "simp_batch_add")
         for sc, i in zip(scs, ids):
@@ -93,6 +97,24 @@ class Ensemble:
         check(lib().simp_batch_remove(self._h, len(v), v.ctypes.data), "simp_batch_remove")
         for i in v:
             self._n.pop(int(i), None)
+
+    def update(self, remove: Iterable[int], scenarios: Sequence,
+               nsteps: Optional[Iterable[int]] = None) -> List[int]:
+        """Remove instances and add new ones with one layout rebuild; the
+        ids of the added instances."""
+        rm = np.asarray(list(remove), np.int32)
+        scs = list(scenarios)
+        ns = list(nsteps) if nsteps is not None else [0] * len(scs)
+        keep: list = []
+        arr = (SimpInstanceDesc * max(len(scs), 1))(*[_desc(sc, k, keep) for sc, k in zip(scs, ns)])
+        ids = np.zeros(max(len(scs), 1), np.int32)
+        check(lib().simp_batch_update(self._h, len(rm), rm.ctypes.data if len(rm) else None,
+                                      len(scs), arr, ids.ctypes.data), "simp_batch_update")
+        for i in rm:
+            self._n.pop(int(i), None)
+        for sc, i in zip(scs, ids):
+            self._n[int(i)] = sc.n
+        return [int(i) for i in ids[:len(scs)]]
 
     def step(self, nsteps: int, sync: bool = True) -> None:
         check(lib().simp_batch_step(self._h, nsteps, 1 if sync else 0), "simp_batch_step")
@@ -134,6 +156,11 @@ class Ensemble:
         return {int(a[0][j]): {"itimestep": int(a[1][j]), "steps_left": int(a[2][j]),
                                "active": bool(a[3][j]), "status": int(a[4][j])}
                 for j in range(min(k, cap))}
+
+    def interaction_stats(self, id_: int) -> dict:
+        """Neighbour statistics of one instance (see ``Simp.interaction_stats``)."""
+        from .api import _stats
+        return _stats(lambda *a: lib().simp_batch_interaction_stats(self._h, id_, *a))
 
     def count(self) -> tuple:
         ni, npart = C.c_int(), C.c_longlong()

@@ -11,7 +11,9 @@ the GPU therefore join the running batch instead of waiting for it to drain.
 
 Reported: throughput, p50/p99 step latency (wall time of one step of the
 running batch, as seen by every simulation in it), p50/p99 queueing delay and
-completion latency (arrival to final state on the host).
+completion latency (arrival to final state on the host).  The GPU-busy time
+counts the step launches and the layout updates of admission and retirement
+(one per step boundary that has any).
 """
 from __future__ import annotations
 
@@ -71,10 +73,14 @@ def poisson_requests(count: int = 400, rate: float = 20.0, nmin: int = 5000,
 
 
 class Server:
+    """``ensemble``: the packed batch to serve from (default: a new
+    :class:`Ensemble` of the given precision); any object with the same
+    ``update`` / ``step`` / ``list`` / ``download`` methods will do."""
+
     def __init__(self, precision: int = _lib.SIMP_MIXED, budget_bytes: int = 8 << 30,
                  chunk: int = 10, keep_states: bool = False,
-                 clock: Callable[[], float] = time.perf_counter):
-        self.ens = Ensemble(precision)
+                 clock: Callable[[], float] = time.perf_counter, ensemble=None):
+        self.ens = ensemble if ensemble is not None else Ensemble(precision)
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.keep = keep_states
@@ -84,9 +90,12 @@ class Server:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         results = {r.rid: Result(r.rid, r.scenario.n, r.steps, r.arrival) for r in requests}
         active = {}                  # ensemble id -> (request, footprint)
+        retire = []                  # finished ids, removed with the next layout update
         used = 0
         step_lat = []
-        busy = 0.0
+        busy = 0.0                   # time in step launches
+        relayout = 0.0               # time in layout updates (admission / retirement)
+        updates = 0
         work = 0
         t0 = self.clock()
         while pending or active:
@@ -105,8 +114,15 @@ class Server:
                 admit.append((r, fp))
                 used += fp
                 pending.popleft()
-            if admit:
-                ids = self.ens.add([r.scenario for r, _ in admit], [r.steps for r, _ in admit])
+            if admit or (retire and (active or pending)):
+                # one rebuild of the packed layout per step boundary: the
+                # finished simulations leave and the admitted ones join
+                ts = self.clock()
+                ids = self.ens.update(retire, [r.scenario for r, _ in admit],
+                                      [r.steps for r, _ in admit])
+                relayout += self.clock() - ts
+                updates += 1
+                retire = []
                 tadm = self.clock() - t0
                 for i, (r, fp) in zip(ids, admit):
                     active[i] = (r, fp)
@@ -121,7 +137,7 @@ class Server:
             busy += dt
             step_lat.append(dt / self.chunk)
             status = self.ens.list()
-            done = [i for i, st in status.items() if not st["active"]]
+            done = [i for i, st in status.items() if not st["active"] and i in active]
             for i in done:
                 st = status[i]
                 r, fp = active.pop(i)
@@ -132,10 +148,12 @@ class Server:
                 res.finished = self.clock() - t0
                 work += r.scenario.n * (st["itimestep"])
                 used -= fp
-            if done:
-                self.ens.remove(done)
+                retire.append(i)
         wall = self.clock() - t0
-        return self._metrics(results, step_lat, busy, work, wall)
+        out = self._metrics(results, step_lat, busy + relayout, work, wall)
+        out.update(step_s=busy, relayout_s=relayout, relayouts=updates,
+                   step_throughput=work / busy if busy > 0 else 0.0)
+        return out
 
     @staticmethod
     def _metrics(results, step_lat, busy, work, wall) -> dict:
