@@ -74,10 +74,9 @@ struct InstDyn {
   int active;           // 0: frozen (finished or not started), kernels skip it
   int steps_left;       // steps until the instance freezes; < 0: no limit
   int lstamp;           // stamp of the evaluation that built the boundary cell lists
-  int rlim, rstop;      // k_ranks -> k_rankput: rank limit, list positions needed
+  int rlim, rstop;      // k_ranks: rank limit, list positions whose pairs rank below it
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
-  int rchg;             // 1: the pair-rank prefixes must be recomputed
-  int rput;             // 1: k_rankput writes the hsml_ij store in this evaluation
+  int rchg;             // 1: the pair ranks below the limit must be found again
   double hmin;          // MINVAL(hsml) of the current evaluation
   // candidate lists (kept from the evaluation that built them while they
   // still cover the pair set): built, their skin, the skin of lists built
@@ -145,6 +144,10 @@ struct DevState {
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;
   int2 *gpre;            // per list position: pair prefixes (intra, cross) of k_ranks
+  int *osig;             // per list position below the ranks' stop: a hash of the
+                         // slots of the particle's owned neighbours
+  int2 *hpair;           // per rank below the limit: the pair's owner and partner
+                         // slots (-1: no pair, the hsml_ij entry keeps its value)
   void *sig11, *sig12, *sig22, *div;   // pass-A outputs (FP64 variant: the stress)
   // mixed variant: pass A's record of each slot for pass B's stress sums,
   // FP32 {sig11, sig12, sig22} / rho^2 and the mass (in sig11's allocation)

@@ -88,7 +88,7 @@ static int store_capacity(const InstanceInput &in) {
 constexpr size_t kBytesPerParticle =
     4 * 8 * 8 +             // x .. hsml, two sets of S and T
     2 * (8 + 4 + 4) +       // mass, orig, cellid per set
-    2 * 4 + 2 * 4 + 8 +     // slot, cl; own_cnt, moff; gpre
+    2 * 4 + 2 * 4 + 8 + 4 + // slot, cl; own_cnt, moff; gpre, osig
     16 +                    // pf
     kNbrMax * 4 + 4 +       // nbr, nnb
     kNbrMax * 4 + 4 +       // vnb, vnc
@@ -101,7 +101,7 @@ size_t instance_footprint(const InstanceInput &in) {
   long c = cell_capacity(in);
   if (c < 0) c = 0;
   return kBytesPerParticle * (size_t)in.n + kBytesPerBoundary * (size_t)in.nb +
-         kBytesPerCell * (size_t)c + 8 * (size_t)store_capacity(in) +
+         kBytesPerCell * (size_t)c + 16 * (size_t)store_capacity(in) +
          16 * (size_t)in.g_nx * (size_t)in.g_ny + 4096;
 }
 
@@ -150,6 +150,7 @@ static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, s
   rc |= dalloc(list, &d.own_cnt, N);
   rc |= dalloc(list, &d.moff, N);
   rc |= dalloc(list, &d.gpre, N);
+  rc |= dalloc(list, &d.osig, N);
   rc |= dalloc(list, &d.pf, N);
   rc |= dalloc(list, &d.nbr, (size_t)kNbrMax * N);
   rc |= dalloc(list, &d.nnb, N);
@@ -328,6 +329,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   int rc = alloc_state(nd, nlist, &nscan, ntot, nbtot, ctot, nblk, ninst, gtot);
   if (rc == 0 && hipMalloc((void **)&nd.hq, std::max<long>(qtot, 1) * sizeof(double)) != hipSuccess)
     rc = -6;
+  if (rc == 0) rc = dalloc(nlist, &nd.hpair, std::max<long>(qtot, 1));
   if (rc) {
     free_list(nlist);
     if (nscan) hipFree(nscan);
@@ -358,6 +360,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   RCHK(hipMemsetAsync(nd.bkey, 0xff, std::max<long>(nbtot, 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.bcnt, 0, 9 * std::max<long>(nbtot, 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.cellh, 0, (ctot + 1) * sizeof(unsigned long long), st_));
+  RCHK(hipMemsetAsync(nd.osig, 0, std::max<long>(ntot, 1) * sizeof(int), st_));
   {
     // the first evaluation of a new layout builds the cell lists
     static const int one = 1;

@@ -13,21 +13,19 @@
 //                boundary particle's cell changes
 //   count        cell key NINT(x/bin) (nnps.f90:246-254); flags a rebuild
 //                of the cell lists when any key changed
-//   hist, scan x2, scatter, cellsort
+//   hist, scan x2, scatter, cellsort, permute
 //                the cell lists, members in ascending particle id (the
-//                reference's bin order); all return at once when no key
-//                changed (the corrector evaluation normally reuses the
-//                predictor's lists)
-//   listcopy     the right-hand-side state copied in list order, FP32
-//                prefilter data, per-cell largest hsml
+//                reference's bin order), and the state moved to the new
+//                list order; all return at once when no key changed (the
+//                corrector evaluation normally reuses the predictor's lists)
 //   passA        neighbour gather: Wendland C6 gradient, velocity divergence
 //                and gradient, strain rate and the VP rheology epilogue
 //                (kernel.f90:471-520, vector_divergence.f90:27-28,
 //                vector_gradient.f90:27-40, single_step.f90:80-84,
 //                internal_stress.f90:21-53 and its callees); also the
 //                boundary-pair counts per (boundary particle, cell)
-//   ranks        boundary-pair ranks and bnip; hsml_ij of the first bnip
-//                particle pairs (read by b_Wendland_C6,
+//   ranks        boundary-pair ranks and bnip; hsml_ij of the particle
+//                pairs ranked below bnip (read by b_Wendland_C6,
 //                boundary_kernel.f90:491)
 //   (pairtot, scan: all pair ranks, for the reported nip of single_step)
 //   passB        neighbour gather: stress divergence (internal_stress.f90:
@@ -963,7 +961,8 @@ constexpr float kHalfPad = 0.5f * kPad;
 // present (summation density, stress_2, the Hibler or Gutfraind pressure:
 // their code is compiled in; each instance still follows its own values)
 template <bool MIXED, bool XSW>
-__global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_diag) {
+__global__ void __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(4)))
+k_passA(DevState d, int buf, int want_diag) {
   __shared__ float4 tile[kTile];
   __shared__ unsigned short lst[kNbrMax * kTPB];
   if (blockIdx.x >= (unsigned)d.nblk) return;
@@ -992,17 +991,28 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[u * N + self];
   // the rheology's thickness and concentration, loaded with the rest (the
   // FP64 variant, short of registers, loads them where they are used)
-  double hth_i = 0., A_i = 0.;
+  double hth_i = 0., A_i = 0., rho_own = 0., m_own = 0.;
   if (MIXED) {
     hth_i = d.h[sb][self];
     A_i = d.A[sb][self];
+    rho_own = RS[self];
+    m_own = MS[self];
   }
+  // the previous evaluation's pair counts, loaded here with the rest: the
+  // compiler cannot move a load above the list stores below
+  const int oc_old = d.own_cnt[self];
   const InstDyn &y = d.dyn[k];
+  // particles whose owned pairs can rank below the boundary kernel's limit
+  // (the head of the pair list) also compare the owned neighbours themselves
+  const bool head = self - be.pos + be.first <= y.rstop;
+  const unsigned osig_old = head ? (unsigned)d.osig[self] : 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) *d.rebuild = 0;   // the rebuild kernels have run
   if (inst_off(y)) return;                // whole block: blocks never straddle instances
   const InstMeta &m = d.meta[k];
   const int W = y.W;
   const int *cs0 = d.cell_start + m.c_off;
+  // the cell's boundary stamp (with boundaries): issued before the gathers
+  const int bfl = m.nb > 0 ? d.bflag[m.c_off + c] : -1;
   // Candidate lists: built with the prefilter below (every candidate within
   // mhsml + skin), they still cover the pair set while the cell lists are
   // unchanged (same cells, same visiting order) and every pair's distance
@@ -1094,6 +1104,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   //    candidates, which also writes the neighbour list.  Splitting the two
   //    keeps the expensive arithmetic off the ~90 % of candidates that fail.
   int nI = 0, nX = 0, nn = 0, nc = 0;
+  unsigned osg = 0u;          // hash of the owned neighbours' slots, in order
   if (vok) {
     nc = vcount;
   } else if (!tiled) {
@@ -1253,6 +1264,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         const int kind = pair_kind(R, sj[u], self);
         nI += kind == 1;
         nX += kind == 2;
+        if (kind != 0) osg = osg * 0x9e3779b1u + (unsigned)sj[u];
         d.nbr[nn * N + self] = sj[u];
         ++nn;
         pair(dx, dy, r, mh, mq[u], rq[u], uq[u], wq[u]);
@@ -1272,6 +1284,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
         const int kind = pair_kind(R, s, self);
         nI += kind == 1;
         nX += kind == 2;
+        if (kind != 0) osg = osg * 0x9e3779b1u + (unsigned)s;
         ++ntrue;
         pair(dx, dy, r, mh, MS[s], RS[s], VX[s], VY[s]);
       }
@@ -1293,23 +1306,39 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // boundary pairs per (boundary particle, cell of its 3x3 block): the
   // counts bucket_search's boundary loop finds there (nnps.f90:453-492),
   // gathered from the particle side; k_ranks turns them into ranks
-  if (m.nb > 0 && d.bflag[m.c_off + c] == y.lstamp) {
+  if (m.nb > 0 && bfl == y.lstamp) {
     const int2 run = d.brun[m.c_off + c];
     const int *bl = d.blist + 9 * (size_t)m.b_off;
-    for (int e = run.x; e < run.y; ++e) {
-      const int ent = bl[e];
-      const int bg = m.b_off + (ent & 0xffffff);
-      const double bdx = d.xb[bg] - xi, bdy = d.yb[bg] - yi;
-      if (sqrt(bdx * bdx + bdy * bdy) <= kBLength)
-        atomicAdd(&d.bcnt[(size_t)bg * 9 + (ent >> 24)], 1);
+    // entries in groups of four: their loads, then the boundary particles'
+    // positions, go out together
+    for (int e0 = run.x; e0 < run.y; e0 += 4) {
+      int ent[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ent[u] = e0 + u < run.y ? bl[e0 + u] : -1;
+      double bx[4], by[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int bg = m.b_off + (ent[u] < 0 ? 0 : (ent[u] & 0xffffff));
+        bx[u] = d.xb[bg];
+        by[u] = d.yb[bg];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ent[u] < 0) continue;
+        const int bg = m.b_off + (ent[u] & 0xffffff);
+        const double bdx = bx[u] - xi, bdy = by[u] - yi;
+        if (sqrt(bdx * bdx + bdy * bdy) <= kBLength)
+          atomicAdd(&d.bcnt[(size_t)bg * 9 + (ent[u] >> 24)], 1);
+      }
     }
   }
   if (nI > 1023 || nX > 1023) atomicOr(&d.dyn[k].err, kErrCapacity);
   const int oc_new = (nI & 1023) | ((nX & 1023) << 10) | (min(nn, 4095) << 20);
-  const int oc_old = d.own_cnt[self];     // the previous evaluation's pair counts
-  // the pair ranks k_ranks keeps depend on the owned-pair counts of the
-  // list positions up to its stop: a change there has them recomputed
-  if (__any(((oc_new ^ oc_old) & 0xfffff) != 0 && self - m.p_off <= y.rstop) &&
+  // the pairs k_ranks keeps for the ranks below its limit are the owned
+  // pairs of the list positions up to its stop: a change of their counts or
+  // of the owned neighbours has them found again
+  if (head) d.osig[self] = (int)osg;
+  if (__any(head && (((oc_new ^ oc_old) & 0xfffff) != 0 || osg != osig_old)) &&
       ((int)threadIdx.x & 63) == 0)
     atomicOr(&d.dyn[k].rchg, 1);
   d.own_cnt[self] = oc_new;
@@ -1323,7 +1352,7 @@ __global__ void __launch_bounds__(kTPB) k_passA(DevState d, int buf, int want_di
   // delta -> delta_min switch is decided here.
   const double hth = MIXED ? hth_i : d.h[sb][self], Ac = MIXED ? A_i : d.A[sb][self];
   const int eos = XSW ? m.eos_alg : 3;
-  const double rho_i = (MIXED || eos == 2) ? RS[self] : 0.0, m_i = MIXED ? MS[self] : 0.0;
+  const double rho_i = MIXED ? rho_own : (eos == 2 ? RS[self] : 0.0), m_i = m_own;
   const double em2 = 1.0 / (kE * kE);
   const double P = kPStar * hth * exp(-(kCEos * (1 - Ac)));
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
@@ -1405,12 +1434,15 @@ __global__ void __launch_bounds__(kTPB) k_pairtot(DevState d) {
 // entries b_Wendland_C6 reads at the boundary-pair index
 // (boundary_kernel.f90:491); entries at or above nip keep what an earlier
 // evaluation wrote, as the reference's module array does.  The ranks are
-// the pair prefix over the cell lists
-// (per cell: all intra pairs, then all cross pairs, each in member order),
-// and only the pairs ranked below min(bnip, store size) are read by the
-// boundary kernel -- a short head of the list.  One block streams the
-// prefix over list positions until it passes that limit, then writes those
-// pairs' hsml_ij.
+// the pair prefix over the cell lists (per cell: all intra pairs, then all
+// cross pairs, each in member order), and only the pairs ranked below
+// min(bnip, store size) are read by the boundary kernel -- a short head of
+// the list.  One block per instance finds them, streaming the prefix over
+// list positions until it passes that limit, and keeps each such pair's
+// owner and partner slots.  While no rebuild moves the slots and the head
+// particles keep their owned neighbours (pass A compares them), the pairs
+// at those ranks stay the same, and each step only refreshes their hsml_ij
+// from the new smoothing lengths.
 constexpr int kRankThreads = 1024;
 
 __device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &m, int ss,
@@ -1434,12 +1466,15 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
     int tsum = 0;
     for (int b = b0; b < b0 + per && b < m.nb; ++b) {
       const size_t bg = (size_t)(m.b_off + b);
+      int c9[9];
+#pragma unroll
+      for (int s9 = 0; s9 < 9; ++s9) c9[s9] = d.bcnt[bg * 9 + s9];
       int run = 0;
+#pragma unroll
       for (int s9 = 0; s9 < 9; ++s9) {
-        const int c9 = d.bcnt[bg * 9 + s9];
         d.bcnt[bg * 9 + s9] = 0;
         d.bpre[bg * 9 + s9] = run;
-        run += c9;
+        run += c9[s9];
       }
       d.bcount[bg] = run;
       tsum += run;
@@ -1456,28 +1491,48 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
   }
   const int lim = min(bnip, m.q_cap);
   if (lim <= 0) {
-    if (threadIdx.x == 0) {
-      d.dyn[k].rlim = 0;
-      d.dyn[k].rput = 0;
+    if (threadIdx.x == 0) d.dyn[k].rlim = 0;
+    return;
+  }
+  const int ss = cur_set(y);
+  int2 *HP = d.hpair + m.q_off;
+  // 1) the pairs below the limit are those of the last search while the
+  //    slots, the limit and the head's owned neighbours are unchanged (pass
+  //    A flags a change); their hsml_ij changes only with hsml, at the end of
+  //    a step -- the corrector's half-step state has the predictor's hsml
+  if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) {
+    if (buf == 0) {
+      const double *HS = d.hsml[2 * ss + buf];
+      double *HQ = d.hq + m.q_off;
+      constexpr int kU = 4;
+      for (int r0 = threadIdx.x; r0 < lim; r0 += kRankThreads * kU) {
+        int2 hp[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int r = r0 + u * kRankThreads;
+          hp[u] = r < lim ? HP[r] : make_int2(-1, -1);
+        }
+        double ha[kU], hb[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          ha[u] = hp[u].x >= 0 ? HS[hp[u].x] : 0.0;
+          hb[u] = hp[u].x >= 0 ? HS[hp[u].y] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (hp[u].x >= 0) HQ[r0 + u * kRankThreads] = (ha[u] + hb[u]) / 2.;
+      }
     }
     return;
   }
-  // the prefixes of the last evaluation still hold when the limit, the
-  // slots (no rebuild) and the owned-pair counts before its stop are the
-  // same (pass A flags a change there); the corrector's half-step state
-  // then also has the predictor's hsml, so the store the predictor's
-  // evaluation wrote holds too
-  if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) {
-    if (threadIdx.x == 0) d.dyn[k].rput = buf == 0;
-    return;
-  }
+  // 2) otherwise search again: no pair at any rank yet
+  for (int r = threadIdx.x; r < lim; r += kRankThreads) HP[r] = make_int2(-1, -1);
   const int *cs = d.cell_start + m.c_off;
   const int p0 = cs[0], np = m.n;
   int2 *G = d.gpre + p0;       // per position: exclusive prefixes of intra, cross counts
   if (threadIdx.x == 0) sh_stop = np;
   __syncthreads();
-  // 1) prefixes up to the end of the last cell whose first pair ranks
-  //    below lim
+  //    prefixes up to the end of the last cell whose first pair ranks below lim
   constexpr int kPer = 4;      // consecutive positions per thread
   int baseI = 0, baseX = 0;
   for (int c0 = 0; c0 < np && c0 <= sh_stop; c0 += kRankThreads * kPer) {
@@ -1502,7 +1557,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
         // the last position whose prefix is below lim: its cell is the
         // last one needed
         if (gI + gX < lim && gI + gX + vI[u] + vX[u] >= lim)
-          sh_stop = cs[d.cellid[cur_set(y)][p0 + t] + 1] - p0;
+          sh_stop = cs[d.cellid[ss][p0 + t] + 1] - p0;
       }
       gI += vI[u];
       gX += vX[u];
@@ -1511,28 +1566,18 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
     baseX += totX;
     __syncthreads();
   }
+  //    then each position's owned pairs below lim: their slots and hsml_ij
+  const int stop = sh_stop;
+  for (int t = threadIdx.x; t < stop; t += kRankThreads)
+    rank_store(d, y, m, ss, buf, t, lim, baseI, baseX);
   if (threadIdx.x == 0) {
     InstDyn &yw = d.dyn[k];
     yw.rchg = 0;
-    yw.rput = 1;
     yw.rlim = lim;
-    yw.rstop = sh_stop;
+    yw.rstop = stop;
     yw.gendI = baseI;   // prefixes at np, when the loop reached it
     yw.gendX = baseX;
   }
-}
-
-// 2) one thread per list position of the cells k_ranks found needed: the
-//    owned pairs that rank below lim get their hsml_ij
-__global__ void __launch_bounds__(kTPB) k_rankput(DevState d, int buf) {
-  int s = 0, k = 0;
-  if (!particle_of(d, s, k)) return;
-  const InstDyn &y = d.dyn[k];
-  const InstMeta &m = d.meta[k];
-  if (m.nb == 0 || inst_off(y)) return;
-  const int t = s - m.p_off;
-  if (t >= y.rstop || y.rlim <= 0 || !y.rput) return;
-  rank_store(d, y, m, cur_set(y), buf, t, y.rlim, y.gendI, y.gendX);
 }
 
 // The hsml_ij of the pairs list position t owns, where they rank below lim
@@ -1563,7 +1608,10 @@ __device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &
     for (int j = nn - nI - nX; j < nn; ++j) {
       const int q = d.nbr[j * N + s];
       int &rank = (j < nn - nX) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = (hi + HS[q]) / 2.;
+      if (rank < lim) {
+        d.hq[m.q_off + rank] = (hi + HS[q]) / 2.;
+        d.hpair[m.q_off + rank] = make_int2(s, q);
+      }
       ++rank;
     }
     return;
@@ -1580,7 +1628,10 @@ __device__ void rank_store(const DevState &d, const InstDyn &y, const InstMeta &
       double r;
       if (!in_support(dx * dx + dy * dy, mh, r)) continue;
       int &rank = (g == 2) ? rI : rX;
-      if (rank < lim) d.hq[m.q_off + rank] = mh;
+      if (rank < lim) {
+        d.hq[m.q_off + rank] = mh;
+        d.hpair[m.q_off + rank] = make_int2(s, q);
+      }
       ++rank;
     }
   }
@@ -1658,10 +1709,11 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
 #pragma unroll
   for (int u = 0; u < kB; ++u) nx[u] = d.nbr[u * N + self];
   const double hthi = d.h[2 * cs + buf][p], Ai = d.A[2 * cs + buf][p];
-  double x1 = 0., y1 = 0., h1 = 0., A1 = 0.;
+  double x1 = 0., y1 = 0., h1 = 0., A1 = 0., u1 = 0., v1 = 0.;
   if (phase == 1) {
     x1 = d.x[2 * cs + 0][p]; y1 = d.y[2 * cs + 0][p];
     h1 = d.h[2 * cs + 0][p]; A1 = d.A[2 * cs + 0][p];
+    u1 = d.vx[2 * cs + 0][p]; v1 = d.vy[2 * cs + 0][p];
   }
   const InstDyn &y = d.dyn[k];
   const InstMeta &m = d.meta[k];
@@ -1682,8 +1734,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     double dsi = 1.0, dnumi = 0.;
     if (sd && phase != 2) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
-    // round 2
+    // round 2: also the wind at the particle and the cell's run of boundary
+    // entries, which the stores below would otherwise hold back
     const int bfl = d.bflag[m.c_off + c];
+    double wu, wv;
+    wind_at(d, m, xi, yi, wu, wv);
+    const int2 run = m.nb > 0 ? d.brun[m.c_off + c] : make_int2(0, 0);
     double kzi = 0.;
     if (sumd3) kzi = d.kz[m.p_off + d.orig[cs][self]];
 
@@ -1869,7 +1925,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       // ballot, and the count over the cell's earlier lanes is a popcount;
       // only the members in an earlier wave are tested one by one.  The
       // loads of kBU entries go out together.
-      constexpr int kBU = 1;
+      constexpr int kBU = 2;
       const int own0 = d.cell_start[m.c_off + c];   // first member of the own cell
       const int lane = (int)threadIdx.x & 63;
       const int wave0 = self - lane;                 // slot of the wave's lane 0
@@ -1878,7 +1934,6 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           (((unsigned long long)1 << lane) - 1) & ~(((unsigned long long)1 << first) - 1);
       int nbn = 0;
       float fbx = 0.0f, fby = 0.0f;   // mixed: the forces' sum
-      const int2 run = d.brun[m.c_off + c];
       const int *bl = d.blist + 9 * (size_t)m.b_off;
       for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
         int ent[kBU], rb[kBU];
@@ -1966,8 +2021,6 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
 
     // wind_forcing / water_drag (wind_forcing.f90:35-50, water_drag.f90:35-51)
     if (y.time > 0.) {
-      double wu, wv;
-      wind_at(d, m, xi, yi, wu, wv);
       const double du = wu - vxi, dv = wv - vyi;
       const double hr = hthi * kRhoIce;
       const double cu = m.cur_u - vxi, cv = m.cur_v - vyi;
@@ -1992,7 +2045,6 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       d.vx[2 * cs + 1][p] = vxi + ax * dt / 2;
       d.vy[2 * cs + 1][p] = vyi + ay * dt / 2;
     } else if (phase == 1) {
-      const double u1 = d.vx[2 * cs + 0][p], v1 = d.vy[2 * cs + 0][p];
       const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
       d.vx[2 * cs + 0][p] = 2 * u12 - u1;
       d.vy[2 * cs + 0][p] = 2 * v12 - v1;
@@ -2156,7 +2208,6 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
 }
 void launch_ranks(const DevState &d, int buf, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d, buf);
-  k_rankput<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 template <bool MIXED, bool XSW>
 static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
