@@ -331,15 +331,13 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     rc = -6;
   if (rc == 0) rc = dalloc(nlist, &nd.hpair, std::max<long>(qtot, 1));
   if (rc) {
-    free_list(nlist);
-    if (nscan) hipFree(nscan);
+    free_list(nlist);           // (scan_tmp is on the list)
     if (nd.hq) hipFree(nd.hq);
     return rc;
   }
   auto fail = [&](int code) {
     hipStreamSynchronize(st_);
     free_list(nlist);
-    hipFree(nscan);
     hipFree(nd.hq);
     return code;
   };
@@ -499,8 +497,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
 
   // swap in the new layout
   graph_drop();
-  free_list(allocs_);
-  if (scan_tmp_) hipFree(scan_tmp_);
+  free_list(allocs_);           // (scan_tmp_ is on the list)
   if (d_.hq) hipFree(d_.hq);
   d_ = nd;
   allocs_.swap(nlist);
@@ -548,7 +545,7 @@ void Engine::enqueue_prefix(int buf) {
   L("count", [&] { launch_count(d, buf, st_); });
   L("scan", [&] { launch_rebuild_scan(d, scan_tmp_, st_); });
   L("scatter", [&] { launch_scatter(d, st_); });
-  L("cellsort", [&] { launch_cellsort(d, buf, st_); });
+  L("permute", [&] { launch_permute(d, buf, st_); });
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {

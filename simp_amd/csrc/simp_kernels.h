@@ -19,7 +19,7 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
 // rebuilds: slots and the starts of the cell lists (counts are current)
 void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
-void launch_cellsort(const DevState &d, int buf, hipStream_t st);
+void launch_permute(const DevState &d, int buf, hipStream_t st);
 void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);

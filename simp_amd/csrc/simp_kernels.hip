@@ -718,61 +718,47 @@ __global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
   d.cl[d.cell_start[d.meta[k].c_off + d.cellid[y.sel][p]] + d.slot[p]] = p;
 }
 
-// members of each cell in ascending particle id: the order in which the
-// reference fills its bins (nnps.f90:246-254).  Cells of up to kRankMax
-// members (the common case) are ranked in registers; larger ones are
-// insertion-sorted in place.
+// The member of cell c (scattered into the cell's list range [s0, s0 + n)
+// in arbitrary order by k_scatter) whose particle id ranks r among the
+// cell's members: the cell's members in ascending particle id are the order
+// in which the reference fills its bins (nnps.f90:246-254).  Cells of up to
+// kRankMax members (the common case) are ranked in registers.
 constexpr int kRankMax = 16;
 
-__global__ void __launch_bounds__(kTPB) k_cellsort(DevState d, int buf) {
-  if (*d.rebuild == 0) return;
-  int c = blockIdx.x * kTPB + threadIdx.x;
-  if (c >= d.ctot) return;
-  const InstDyn &y = d.dyn[d.cinst[c]];
-  if (y.rb == 0) return;
-  const int s0 = d.cell_start[c], s1 = d.cell_start[c + 1];
-  const int n = s1 - s0;
-  if (n == 0) return;
-  d.fill[c] = 0;
-  if (n == 1) return;
-  const int *orig = d.orig[y.sel];
+__device__ __forceinline__ int cell_member_at(const int *cl, const int *orig, int s0, int n,
+                                              int r) {
+  if (n == 1) return cl[s0];
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
 #pragma unroll
-    for (int i = 0; i < kRankMax; ++i) {
-      v[i] = i < n ? d.cl[s0 + i] : 0;
-      key[i] = i < n ? orig[v[i]] : 0x7fffffff;
-    }
+    for (int i = 0; i < kRankMax; ++i) v[i] = i < n ? cl[s0 + i] : 0;
+#pragma unroll
+    for (int i = 0; i < kRankMax; ++i) key[i] = i < n ? orig[v[i]] : 0x7fffffff;
+    int q = v[0];
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) {
-      if (i < n) {
-        int r = 0;
+      int rk = 0;
 #pragma unroll
-        for (int j = 0; j < kRankMax; ++j) r += key[j] < key[i];
-        d.cl[s0 + r] = v[i];
-      }
+      for (int j = 0; j < kRankMax; ++j) rk += key[j] < key[i];
+      if (i < n && rk == r) q = v[i];
     }
-    return;
+    return q;
   }
-  for (int a = s0 + 1; a < s1; ++a) {
-    int v = d.cl[a];
-    int key = orig[v];
-    int b = a - 1;
-    while (b >= s0) {
-      int u = d.cl[b];
-      if (orig[u] <= key) break;
-      d.cl[b + 1] = u;
-      --b;
-    }
-    d.cl[b + 1] = v;
+  for (int i = 0; i < n; ++i) {
+    const int vi = cl[s0 + i], ki = orig[vi];
+    int rk = 0;
+    for (int j = 0; j < n; ++j) rk += orig[cl[s0 + j]] < ki;
+    if (rk == r) return vi;
   }
+  return cl[s0];
 }
 
 // The state of a rebuilt instance moves to the new slot order, from its set
-// sel to set sel ^ 1: each slot gathers the particle the sorted lists put
-// there (cl[slot] = its old slot), and writes the prefilter positions of
-// the evaluation (buffer buf).  The half-step buffer moves only when it is
-// live (the corrector's evaluation).
+// sel to set sel ^ 1: each slot takes the member of its cell that ranks there
+// by particle id (its old slot), and writes the prefilter positions of the
+// evaluation (buffer buf).  The half-step buffer moves only when it is live
+// (the corrector's evaluation).  The first slot of each cell clears the
+// cell's fill counter for the next rebuild.
 __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
   if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
   if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
@@ -784,7 +770,12 @@ __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
   if (act) {
     const InstDyn &y = d.dyn[k];
     const int f = y.sel, t = y.sel ^ 1;
-    const int q = d.cl[s];
+    const int coff = d.meta[k].c_off;
+    // the cell of slot s: that of any particle scattered into its range
+    const int c = d.cellid[f][d.cl[s]];
+    const int s0 = d.cell_start[coff + c], s1 = d.cell_start[coff + c + 1];
+    if (s == s0) d.fill[coff + c] = 0;
+    const int q = cell_member_at(d.cl, d.orig[f], s0, s1 - s0, s - s0);
     for (int b = 0; b <= buf; ++b) {
       const int i = 2 * f + b, o = 2 * t + b;
       d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
@@ -794,7 +785,7 @@ __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
     }
     d.mass[t][s] = d.mass[f][q];
     d.orig[t][s] = d.orig[f][q];
-    key = d.cellid[f][q];
+    key = c;
     d.cellid[t][s] = key;
     const int i = 2 * f + buf;
     xq = d.x[i][q];
@@ -2188,8 +2179,7 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
 void launch_scatter(const DevState &d, hipStream_t st) {
   k_scatter<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
-void launch_cellsort(const DevState &d, int buf, hipStream_t st) {
-  k_cellsort<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d, buf);
+void launch_permute(const DevState &d, int buf, hipStream_t st) {
   k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
