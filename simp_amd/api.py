@@ -126,13 +126,17 @@ class Simp:
                 "bnip": kb, "bp_i": bi[:kb], "bp_j": bj[:kb], "br_ij": br[:kb]}
 
     # --------------------------------------------------------- single_step
-    def single_step(self, sc, current_time: Optional[float] = None, itimestep: int = 1) -> dict:
+    def single_step(self, sc, current_time: Optional[float] = None, itimestep: int = 1,
+                    out: Optional[dict] = None) -> dict:
+        """``out``: the output arrays of an earlier call of the same size, to
+        be filled again (as a Fortran caller passes its module arrays)."""
         n = sc.n
         t = sc.time if current_time is None else current_time
-        out = {k: np.zeros((n, 2)) for k in ("dxdt", "dvxdt")}
-        out.update({k: np.zeros(n) for k in ("dhdt", "dAdt", "zeta", "eta", "p")})
-        out["eps"] = np.zeros((n, 3))
-        out["sigma"] = np.zeros((n, 3))
+        if out is None or out["dhdt"].shape[0] != n:
+            out = {k: np.zeros((n, 2)) for k in ("dxdt", "dvxdt")}
+            out.update({k: np.zeros(n) for k in ("dhdt", "dAdt", "zeta", "eta", "p")})
+            out["eps"] = np.zeros((n, 3))
+            out["sigma"] = np.zeros((n, 3))
         nip, bnip = C.c_int(), C.c_int()
         a = _state_arrays(sc)
         st = lib().simp_single_step(

@@ -22,6 +22,8 @@ struct simp_handle {
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
   std::vector<double> g_u, g_v;
+  long grid_ver = 0;          // simp_set_wind_grid calls so far
+  long grid_ver_loaded = -1;  // the version the resident instance was uploaded with
 };
 
 static int map_rc(int rc) {
@@ -121,6 +123,7 @@ int simp_set_wind_grid(simp_handle *h, int nx, int ny, double x0, double y0, dou
   h->g_x0 = x0; h->g_y0 = y0; h->g_dx = dx; h->g_dy = dy;
   h->g_u.assign(u, u + (size_t)nx * ny);
   h->g_v.assign(v, v + (size_t)nx * ny);
+  h->grid_ver += 1;
   return SIMP_OK;
 }
 
@@ -164,14 +167,16 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
   in.itimestep = itimestep;
   int rc = fill_input(h, in);
   if (rc) return rc;
-  rc = h->eng->upload({in}, true);
+  // the resident layout of the last call takes the new state when it can
+  // (same sizes and switches); otherwise a full upload
+  rc = h->eng->reload(in, h->grid_ver == h->grid_ver_loaded);
+  if (rc == 1) rc = h->eng->upload({in}, true);
   if (rc) return map_rc(rc);
+  h->grid_ver_loaded = h->grid_ver;
   h->uploaded = false;  // the device state now holds this call's inputs
   rc = h->eng->eval_single(eps3 || zeta || eta || p || sig3);
   if (rc) return map_rc(rc);
-  rc = h->eng->download_tend(0, dxdt, dvxdt, dhdt, dAdt);
-  if (rc) return map_rc(rc);
-  rc = h->eng->download_diag(0, eps3, zeta, eta, p, sig3);
+  rc = h->eng->download_single(dxdt, dvxdt, dhdt, dAdt, eps3, zeta, eta, p, sig3);
   if (rc) return map_rc(rc);
   if (nip || bnip) {
     int a = 0, b = 0;
@@ -251,6 +256,7 @@ int simp_upload(simp_handle *h, int n, const double *x, const double *vx, const 
   if (rc) return rc;
   rc = h->eng->upload({in}, false);
   if (rc) return map_rc(rc);
+  h->grid_ver_loaded = h->grid_ver;
   h->uploaded = true;
   h->n = n;
   h->nb = nb;

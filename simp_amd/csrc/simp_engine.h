@@ -56,6 +56,19 @@ class Engine {
   // Add instances (ids returned) / remove instances by id: the packed
   // layout is rebuilt on the device, live state is copied device to device.
   int add(const std::vector<InstanceInput> &ins, std::vector<int> *ids);
+  // A new state for the single instance already resident -- same particle
+  // and boundary counts and switches, its cells and boundary-pair store
+  // large enough -- copied in through pinned staging; the layout, the
+  // hsml_ij store and kernel_zero stay (they persist between single_step
+  // calls as the reference's module arrays do).  Returns 1 when a full
+  // upload is needed instead.
+  // grid_same: a wind grid in `in` is the one of the resident instance
+  int reload(const InstanceInput &in, bool grid_same);
+  // tendencies and diagnostics of the single instance after eval_single, in
+  // reference order, in one batch of copies through pinned staging (any
+  // pointer may be null)
+  int download_single(double *dxdt, double *dvxdt, double *dhdt, double *dAdt, double *eps3,
+                      double *zeta, double *eta, double *p, double *sig3);
   int remove(const std::vector<int> &ids);
   // Remove instances by id and add new ones in one layout rebuild.
   int update(const std::vector<int> &remove_ids, const std::vector<InstanceInput> &ins,
@@ -116,8 +129,13 @@ class Engine {
   template <class F>
   void L(const char *name, F &&f);
 
+  int stage_reserve(size_t bytes);
+
   int precision_;
   hipStream_t st_ = nullptr;
+  std::vector<BlockEntry> blk_host_;  // the block table as uploaded (state set 0)
+  void *stage_ = nullptr;             // pinned host staging (reload, download_single)
+  size_t stage_bytes_ = 0;
   DevState d_{};
   std::vector<void *> allocs_;        // device buffers of d_
   int *scan_tmp_ = nullptr;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 
 #include "simp_kernels.h"
@@ -206,6 +207,7 @@ Engine::Engine(int precision) : precision_(precision) {
 Engine::~Engine() {
   graph_drop();
   if (st_) hipStreamSynchronize(st_);
+  if (stage_) hipHostFree(stage_);
   free_list(allocs_);
   if (d_.hq) hipFree(d_.hq);
   if (st_) hipStreamDestroy(st_);
@@ -488,6 +490,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     }
   }
   if (!phys.empty()) RCHK(h2d(nd.blk, phys.data(), sizeof(BlockEntry) * phys.size()));
+  blk_host_ = phys;
   std::vector<int> cinst(ctot + 1, 0);
   for (int k = 0; k < ninst; ++k)
     std::fill(cinst.begin() + nm[k].c_off, cinst.begin() + nm[k].c_off + nm[k].c_cap, k);
@@ -521,6 +524,169 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     launch_finalize(d_, 0, st_);
     if (hipStreamSynchronize(st_) != hipSuccess) return -2;
   }
+  return 0;
+}
+
+// Host loops over n items split over a few threads (the staging copies of
+// the per-call drop-in are host-memory bound).
+template <class F>
+static void host_for(int n, F &&f) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int T = (int)std::min<unsigned>(std::min(hw, 8u), (unsigned)std::max(n / 16384, 1));
+  if (T <= 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int per = (n + T - 1) / T;
+  for (int t = 1; t < T; ++t)
+    th.emplace_back([&, t] { f(std::min(n, t * per), std::min(n, (t + 1) * per)); });
+  f(0, std::min(n, per));
+  for (auto &x : th) x.join();
+}
+
+int Engine::stage_reserve(size_t bytes) {
+  if (bytes <= stage_bytes_) return 0;
+  if (stage_) hipHostFree(stage_);
+  stage_ = nullptr;
+  stage_bytes_ = 0;
+  if (hipHostMalloc(&stage_, bytes, hipHostMallocDefault) != hipSuccess) {
+    stage_ = nullptr;
+    return -6;
+  }
+  stage_bytes_ = bytes;
+  return 0;
+}
+
+int Engine::reload(const InstanceInput &in, bool grid_same) {
+  if (meta_.size() != 1 || d_.ninst != 1 || blk_host_.empty()) return 1;
+  const InstMeta &o = meta_[0];
+  if (in.n != o.n || in.nb != o.nb || in.tse != o.tse || in.wind_mode != o.wind_mode ||
+      in.den_alg != o.den_alg || in.pa_sph != o.pa_sph || in.eos_alg != o.eos_alg)
+    return 1;
+  if (in.wind_mode == 1 && (!grid_same || in.g_nx != o.g_nx || in.g_ny != o.g_ny)) return 1;
+  if (!in.x || !in.vx || !in.mass || !in.rho || !in.hsml || !in.hth || !in.A) return -1;
+  if (in.nb > 0 && (!in.xb || !in.massb)) return -1;
+  const long cells = grid_cells(in.n, in.x, in.hsml, in.nb, in.xb, 1.0);
+  if (cells < 0 || cells > (long)o.c_cap) return 1;
+  HIPCHK(hipStreamSynchronize(st_));
+  InstDyn y;
+  HIPCHK(hipMemcpy(&y, d_.dyn, sizeof(y), hipMemcpyDeviceToHost));
+  // the boundary-pair store must hold the new state's pairs: the last
+  // evaluation's count with room to spare (a full upload counts them)
+  if (in.nb > 0 && 3L * y.bnip / 2 + 512 > (long)o.q_cap) return 1;
+  const int n = in.n, nb = in.nb;
+  const size_t n8 = (size_t)n * 8, b8 = (size_t)nb * 8;
+  int rc = stage_reserve(9 * n8 + 3 * b8 + (size_t)n * 4);
+  if (rc) return rc;
+  double *hx = (double *)stage_, *hy = hx + n, *hvx = hy + n, *hvy = hvx + n;
+  double *hh = hvy + n, *hA = hh + n, *hr = hA + n, *hs = hr + n, *hm = hs + n;
+  double *hxb = hm + n, *hyb = hxb + nb, *hmb = hyb + nb;
+  int *horig = (int *)(hmb + nb);
+  host_for(n, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      hx[i] = in.x[2 * i]; hy[i] = in.x[2 * i + 1];
+      hvx[i] = in.vx[2 * i]; hvy[i] = in.vx[2 * i + 1];
+      horig[i] = i;
+    }
+    const size_t l = (size_t)(i1 - i0) * 8;
+    std::memcpy(hh + i0, in.hth + i0, l);
+    std::memcpy(hA + i0, in.A + i0, l);
+    std::memcpy(hr + i0, in.rho + i0, l);
+    std::memcpy(hs + i0, in.hsml + i0, l);
+    std::memcpy(hm + i0, in.mass + i0, l);
+  });
+  InstMeta m = o;
+  m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
+  m.cur_u = in.current[0]; m.cur_v = in.current[1];
+  m.max_hsml = in.max_hsml;
+  m.bxmin = m.bymin = 1e300;
+  m.bxmax = m.bymax = -1e300;
+  for (int b = 0; b < nb; ++b) {
+    hxb[b] = in.xb[2 * b]; hyb[b] = in.xb[2 * b + 1]; hmb[b] = in.massb[b];
+    m.bxmin = std::min(m.bxmin, hxb[b]); m.bxmax = std::max(m.bxmax, hxb[b]);
+    m.bymin = std::min(m.bymin, hyb[b]); m.bymax = std::max(m.bymax, hyb[b]);
+  }
+  // the instance record: a new state in set 0, lists to be built, the
+  // evaluation stamps continuing (stamped cells of earlier evaluations stay
+  // stale)
+  y.dt = in.dt; y.time = in.time; y.itimestep = in.itimestep;
+  y.err = 0; y.active = 1; y.steps_left = -1;
+  y.rchg = 1; y.vbuilt = 0; y.vskin_next = 0.f; y.vmoved = 0.f; y.vprev = 0.f;
+  y.sel = 0; y.rb = 0; y.ipstep = -1;
+  auto h2d = [&](void *dst, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st_) : hipSuccess;
+  };
+  HIPCHK(h2d(d_.x[0], hx, n8)); HIPCHK(h2d(d_.y[0], hy, n8));
+  HIPCHK(h2d(d_.vx[0], hvx, n8)); HIPCHK(h2d(d_.vy[0], hvy, n8));
+  HIPCHK(h2d(d_.h[0], hh, n8)); HIPCHK(h2d(d_.A[0], hA, n8));
+  HIPCHK(h2d(d_.rho[0], hr, n8)); HIPCHK(h2d(d_.hsml[0], hs, n8));
+  HIPCHK(h2d(d_.mass[0], hm, n8));
+  HIPCHK(h2d(d_.orig[0], horig, (size_t)n * 4));
+  if (nb > 0) {
+    HIPCHK(h2d(d_.xb, hxb, b8)); HIPCHK(h2d(d_.yb, hyb, b8)); HIPCHK(h2d(d_.massb, hmb, b8));
+  }
+  // no key yet: the first evaluation builds the lists
+  HIPCHK(hipMemsetAsync(d_.cellid[0], 0xff, (size_t)n * 4, st_));
+  HIPCHK(hipMemsetAsync(d_.count, 0, ((size_t)o.c_cap + 1) * 4, st_));
+  HIPCHK(hipMemsetAsync(d_.fill, 0, ((size_t)o.c_cap + 1) * 4, st_));
+  if (nb > 0) {
+    HIPCHK(hipMemsetAsync(d_.bkey, 0xff, (size_t)nb * 4, st_));
+    HIPCHK(hipMemsetAsync(d_.bcnt, 0, 9 * (size_t)nb * 4, st_));
+  }
+  static const int one = 1;
+  HIPCHK(h2d(d_.rebuild, &one, sizeof(int)));
+  HIPCHK(h2d(d_.blk, blk_host_.data(), sizeof(BlockEntry) * blk_host_.size()));
+  HIPCHK(h2d(d_.meta, &m, sizeof(m)));
+  HIPCHK(h2d(d_.dyn, &y, sizeof(y)));
+  meta_[0] = m;
+  // grid of the first evaluation
+  launch_partial(d_, 0, st_);
+  launch_finalize(d_, 0, st_);
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+int Engine::download_single(double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                            double *eps3, double *zeta, double *eta, double *p, double *sig3) {
+  if (d_.ninst < 1) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  const int n = meta_[0].n, N = d_.ntot, off = meta_[0].p_off;
+  const int cs = cur_set(0);
+  if (cs < 0) return -2;
+  // 15 fields by slot and the slot order, in one batch
+  const size_t n8 = (size_t)n * 8;
+  int rc = stage_reserve(15 * n8 + (size_t)n * 4);
+  if (rc) return rc;
+  double *st = (double *)stage_;
+  int *ord = (int *)(st + 15 * (size_t)n);
+  auto d2h = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st_);
+  };
+  HIPCHK(d2h(ord, d_.orig[cs] + off, (size_t)n * 4));
+  const bool want_t = dxdt || dvxdt || dhdt || dAdt, want_d = eps3 || zeta || eta || p || sig3;
+  for (int f = 0; f < 6 && want_t; ++f)
+    HIPCHK(d2h(st + f * (size_t)n, d_.tend + (size_t)f * N + off, n8));
+  for (int f = 0; f < 6 && want_d; ++f)
+    HIPCHK(d2h(st + (6 + f) * (size_t)n, d_.diag + (size_t)f * N + off, n8));
+  for (int f = 0; f < 3 && sig3; ++f)
+    HIPCHK(d2h(st + (12 + f) * (size_t)n, d_.sig_out + (size_t)f * N + off, n8));
+  HIPCHK(hipStreamSynchronize(st_));
+  auto F = [&](int f, int i) { return st[f * (size_t)n + i]; };
+  host_for(n, [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      const int r = ord[i];
+      if (dxdt) { dxdt[2 * r] = F(0, i); dxdt[2 * r + 1] = F(1, i); }
+      if (dvxdt) { dvxdt[2 * r] = F(2, i); dvxdt[2 * r + 1] = F(3, i); }
+      if (dhdt) dhdt[r] = F(4, i);
+      if (dAdt) dAdt[r] = F(5, i);
+      if (eps3) { eps3[3 * r] = F(6, i); eps3[3 * r + 1] = F(7, i); eps3[3 * r + 2] = F(8, i); }
+      if (zeta) zeta[r] = F(9, i);
+      if (eta) eta[r] = F(10, i);
+      if (p) p[r] = F(11, i);
+      if (sig3) { sig3[3 * r] = F(12, i); sig3[3 * r + 1] = F(13, i); sig3[3 * r + 2] = F(14, i); }
+    }
+  });
   return 0;
 }
 
