@@ -110,6 +110,39 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
                      double *eps3, double *zeta, double *eta, double *p,
                      double *sig3, int *nip, int *bnip);
 
+/* ---- simp_single_step without allocation ----
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and status codes as
+ * simp_single_step, but the call allocates nothing, on the device or on
+ * the host: the device state lives in one array per GPU that the library
+ * holds from its first simp_create on that GPU until the process ends
+ * (SIMP_NOALLOC_SCRATCH_BYTES: one instance of up to about 250,000
+ * particles at the reference's spacing; the hsml_ij store holds 262,144
+ * boundary pairs).  An input that needs more returns SIMP_E_CAPACITY
+ * before anything is touched.
+ * Every handle on one GPU uses the same array, which keeps the last call's
+ * contents: the hsml_ij store and kernel_zero persist from call to call as
+ * the reference's module arrays do (a handle's first call starts them from
+ * zero, as a new handle's simp_single_step does), shared by all the
+ * handles on that GPU like a module array shared by all callers in one
+ * process.  Calls that overlap on one GPU (threads, several handles) race.
+ * simp_kernel_values and simp_interaction_stats read the last call back
+ * while no other handle's call has followed it (SIMP_E_STATE otherwise).
+ * A handle that holds a layout from the allocating entry points returns
+ * SIMP_E_STATE here. */
+#define SIMP_NOALLOC_SCRATCH_BYTES (256u << 20)
+int simp_single_step_noalloc(simp_handle *h, int n, const double *x,
+                             const double *vx, const double *mass,
+                             const double *rho, const double *hsml,
+                             const double *hth, const double *A, int nb,
+                             const double *xb, const double *massb,
+                             double current_time, int itimestep,
+                             double *dxdt, double *dvxdt, double *dhdt,
+                             double *dAdt, double *eps3, double *zeta,
+                             double *eta, double *p, double *sig3, int *nip,
+                             int *bnip);
+
 /* ---- kernel values of the last single_step, for a host-side density ----
  * Replaces the module arrays single_step leaves for sum_density
  * (phy_quantities/density.f90:41-100; written at sph/kernel.f90:503,511):

@@ -28,7 +28,7 @@ SIMP_CONFIG_VERSION = 1
 #: every entry point declared in include/simp.h
 EXPORTS = (
     "simp_config_default", "simp_create", "simp_destroy", "simp_set_wind_grid",
-    "simp_single_step", "simp_kernel_values", "simp_nnps", "simp_upload", "simp_step_pc",
+    "simp_single_step", "simp_single_step_noalloc", "simp_kernel_values", "simp_nnps", "simp_upload", "simp_step_pc",
     "simp_download", "simp_set_diagnostics", "simp_download_diagnostics",
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
@@ -100,6 +100,7 @@ def lib():
                                      C.c_double, C.c_double, _vp, _vp]
     L.simp_single_step.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp,
                                    C.c_double, C.c_int] + [_vp] * 9 + [_vp, _vp])
+    L.simp_single_step_noalloc.argtypes = L.simp_single_step.argtypes
     L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
                             _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]

@@ -127,9 +127,11 @@ class Simp:
 
     # --------------------------------------------------------- single_step
     def single_step(self, sc, current_time: Optional[float] = None, itimestep: int = 1,
-                    out: Optional[dict] = None) -> dict:
+                    out: Optional[dict] = None, noalloc: bool = False) -> dict:
         """``out``: the output arrays of an earlier call of the same size, to
-        be filled again (as a Fortran caller passes its module arrays)."""
+        be filled again (as a Fortran caller passes its module arrays).
+        ``noalloc``: through ``simp_single_step_noalloc``, which allocates
+        nothing (its device state lives in the library's per-GPU array)."""
         n = sc.n
         t = sc.time if current_time is None else current_time
         if out is None or out["dhdt"].shape[0] != n:
@@ -139,14 +141,15 @@ class Simp:
             out["sigma"] = np.zeros((n, 3))
         nip, bnip = C.c_int(), C.c_int()
         a = _state_arrays(sc)
-        st = lib().simp_single_step(
+        fn = lib().simp_single_step_noalloc if noalloc else lib().simp_single_step
+        st = fn(
             self._h, n, ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(a[5]),
             ptr(a[6]), sc.nb, ptr(a[7]) if sc.nb else None, ptr(a[8]) if sc.nb else None,
             t, itimestep,
             ptr(out["dxdt"]), ptr(out["dvxdt"]), ptr(out["dhdt"]), ptr(out["dAdt"]),
             ptr(out["eps"]), ptr(out["zeta"]), ptr(out["eta"]), ptr(out["p"]),
             ptr(out["sigma"]), C.byref(nip), C.byref(bnip))
-        check(st, "simp_single_step")
+        check(st, "simp_single_step_noalloc" if noalloc else "simp_single_step")
         out["nip"], out["bnip"] = nip.value, bnip.value
         return out
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -17,6 +18,7 @@ using simp::InstanceInput;
 struct simp_handle {
   simp_config cfg;
   Engine *eng = nullptr;
+  int dev = 0;                // the HIP device current at simp_create
   bool uploaded = false;
   int n = 0, nb = 0;
   int g_nx = 0, g_ny = 0;
@@ -25,6 +27,49 @@ struct simp_handle {
   long grid_ver = 0;          // simp_set_wind_grid calls so far
   long grid_ver_loaded = -1;  // the version the resident instance was uploaded with
 };
+
+// ---- device memory of simp_single_step_noalloc ----
+// One array per GPU the library runs on, held until the process ends: the
+// hsml_ij store and its pair slots, kernel_zero, then the layout of the
+// call's instance and the staging of its inputs and results
+// (Engine::single_in_memory).  In host code `scratch` names a host-side
+// stand-in, not device memory: only the address hipGetSymbolAddress
+// returns for the current device is ever passed on.
+constexpr size_t kScratchBytes = (size_t)SIMP_NOALLOC_SCRATCH_BYTES;
+constexpr long kScratchStore = 1L << 18;   // hsml_ij store entries
+constexpr long kScratchKz = 1L << 19;      // kernel_zero entries (particles)
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct ScratchDevice {
+  bool looked_up = false;
+  void *ptr = nullptr;                // device address of `scratch` (null: lookup failed)
+  const simp_handle *last = nullptr;  // the handle whose call laid out its content last
+  long kz_valid = 0;                  // particles of that call (kernel_zero entries kept)
+};
+ScratchDevice g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// Once per device, with that device current (at simp_create).
+void scratch_lookup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchDevice &sd = g_scratch[dev];
+  if (sd.looked_up) return;
+  sd.looked_up = true;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) sd.ptr = p;
+  else (void)hipGetLastError();   // the no-allocation entry point reports it
+}
+
+// A handle whose device state lies in its GPU's array can read it back only
+// while no other handle's call has laid out its own there.
+bool scratch_state_lost(const simp_handle *h) {
+  return h->eng->in_caller_memory() &&
+         (h->dev < 0 || h->dev >= kMaxDevices || g_scratch[h->dev].last != h);
+}
+}  // namespace
 
 static int map_rc(int rc) {
   switch (rc) {
@@ -99,6 +144,11 @@ int simp_create(const simp_config *cfg, simp_handle **out) {
   simp_handle *h = new (std::nothrow) simp_handle;
   if (!h) return SIMP_E_NOMEM;
   h->cfg = *cfg;
+  if (hipGetDevice(&h->dev) != hipSuccess) {
+    delete h;
+    return SIMP_E_HIP;
+  }
+  scratch_lookup(h->dev);
   h->eng = new (std::nothrow) Engine(cfg->precision);
   if (!h->eng || !h->eng->stream()) {
     delete h->eng;
@@ -111,6 +161,8 @@ int simp_create(const simp_config *cfg, simp_handle **out) {
 
 int simp_destroy(simp_handle *h) {
   if (!h) return SIMP_E_ARG;
+  if (h->dev >= 0 && h->dev < kMaxDevices && g_scratch[h->dev].last == h)
+    g_scratch[h->dev].last = nullptr;
   delete h->eng;
   delete h;
   return SIMP_OK;
@@ -188,6 +240,49 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
   return SIMP_OK;
 }
 
+int simp_single_step_noalloc(simp_handle *h, int n, const double *x, const double *vx,
+                             const double *mass, const double *rho, const double *hsml,
+                             const double *hth, const double *A, int nb, const double *xb,
+                             const double *massb, double current_time, int itimestep,
+                             double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                             double *eps3, double *zeta, double *eta, double *p, double *sig3,
+                             int *nip, int *bnip) {
+  if (!h || n < 1 || nb < 0) return SIMP_E_ARG;
+  if (h->dev < 0 || h->dev >= kMaxDevices) return SIMP_E_HIP;
+  ScratchDevice &sd = g_scratch[h->dev];
+  if (!sd.ptr) return SIMP_E_HIP;
+  InstanceInput in;
+  in.n = n; in.nb = nb;
+  in.x = x; in.vx = vx; in.mass = mass; in.rho = rho; in.hsml = hsml; in.hth = hth; in.A = A;
+  in.xb = xb; in.massb = massb;
+  in.time = current_time;
+  in.itimestep = itimestep;
+  int rc = fill_input(h, in);
+  if (rc) return rc;
+  // a handle's first call starts the persistent arrays from zero, as a new
+  // handle's simp_single_step does; later calls find them as the last call
+  // on this GPU left them
+  const bool first = !h->eng->in_caller_memory();
+  rc = h->eng->single_in_memory((char *)sd.ptr, kScratchBytes, kScratchStore, kScratchKz, in,
+                                first, sd.kz_valid);
+  if (rc) return map_rc(rc);
+  sd.last = h;
+  sd.kz_valid = n;
+  h->uploaded = false;
+  rc = h->eng->eval_single(eps3 || zeta || eta || p || sig3);
+  if (rc) return map_rc(rc);
+  rc = h->eng->single_in_memory_out(dxdt, dvxdt, dhdt, dAdt, eps3, zeta, eta, p, sig3);
+  if (rc) return map_rc(rc);
+  if (nip || bnip) {
+    int a = 0, b = 0;
+    rc = h->eng->download_scalars(0, nullptr, nullptr, nullptr, &a, &b, nullptr);
+    if (rc) return map_rc(rc);
+    if (nip) *nip = a;
+    if (bnip) *bnip = nb > 0 ? b : 0;
+  }
+  return SIMP_OK;
+}
+
 int simp_nnps(simp_handle *h, int n, const double *x, const double *hsml, int nb,
               const double *xb, int cap, int *nip, int *p_i, int *p_j, double *r_ij,
               double *hsml_ij, int bcap, int *bnip, int *bp_i, int *bp_j, double *br_ij) {
@@ -212,7 +307,7 @@ int simp_nnps(simp_handle *h, int n, const double *x, const double *hsml, int nb
 int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j, double *w,
                        double *kernel_zero) {
   if (!h || cap < 0 || !nip) return SIMP_E_ARG;
-  if (h->eng->ninst() != 1) return SIMP_E_STATE;
+  if (h->eng->ninst() != 1 || scratch_state_lost(h)) return SIMP_E_STATE;
   if (kernel_zero && h->cfg.den_alg != 1) return SIMP_E_STATE;
   int bnip = 0;
   int rc = h->eng->export_pairs(cap, nip, p_i, p_j, nullptr, nullptr, 0, &bnip, nullptr,
@@ -238,7 +333,7 @@ static int stats_out(Engine *e, int k, int n, int *maxip, int *minip, double *av
 int simp_interaction_stats(simp_handle *h, int *maxip, int *minip, double *avgip, int *noip,
                            long long *nip, int *itimestep) {
   if (!h) return SIMP_E_ARG;
-  if (h->eng->ninst() != 1) return SIMP_E_STATE;
+  if (h->eng->ninst() != 1 || scratch_state_lost(h)) return SIMP_E_STATE;
   return stats_out(h->eng, 0, h->eng->meta(0).n, maxip, minip, avgip, noip, nip, itimestep);
 }
 

@@ -69,6 +69,24 @@ class Engine {
   // pointer may be null)
   int download_single(double *dxdt, double *dvxdt, double *dhdt, double *dAdt, double *eps3,
                       double *zeta, double *eta, double *p, double *sig3);
+  // ---- one instance laid out in caller-provided device memory: nothing is
+  // allocated, on the device or on the host.  The memory holds, in this
+  // order, the persistent arrays -- the hsml_ij store (q_cap doubles), its
+  // pair slots (q_cap int2) and kernel_zero (kz_cap doubles, by particle
+  // index) -- then the layout of `in` and the staging of the call's inputs
+  // and results.  The persistent arrays keep their contents between calls:
+  // clear_persist clears them; otherwise kernel_zero entries from kz_valid
+  // up to n start at zero (particles the last call did not have).  -3 when
+  // the memory is too small (nothing is touched then), -5 when the engine
+  // holds a layout it allocated.
+  static size_t single_bytes(const InstanceInput &in, long q_cap, long kz_cap);
+  int single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
+                       const InstanceInput &in, bool clear_persist, long kz_valid);
+  // after eval_single: the results in the caller's order and layout, through
+  // the staging area (any pointer may be null)
+  int single_in_memory_out(double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                           double *eps3, double *zeta, double *eta, double *p, double *sig3);
+  bool in_caller_memory() const { return arena_; }
   int remove(const std::vector<int> &ids);
   // Remove instances by id and add new ones in one layout rebuild.
   int update(const std::vector<int> &remove_ids, const std::vector<InstanceInput> &ins,
@@ -147,6 +165,8 @@ class Engine {
   bool any_xsw_ = false;              // some instance uses a non-default switch value
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
+  bool arena_ = false;                // d_ lies in caller memory (single_in_memory)
+  double *arena_stage_ = nullptr;     // its staging area
 };
 
 }  // namespace simp

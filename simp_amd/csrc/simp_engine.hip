@@ -108,15 +108,11 @@ size_t instance_footprint(const InstanceInput &in) {
 
 // ------------------------------------------------------------ allocation
 
-template <class T>
-static int dalloc(std::vector<void *> &list, T **p, size_t count) {
-  if (count == 0) count = 1;
-  if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
-    *p = nullptr;
-    return -6;
-  }
-  list.push_back(*p);
-  return 0;
+// An allocator is called as alloc(&ptr, bytes) and returns nonzero when it
+// cannot provide the bytes.
+template <class T, class A>
+static int dalloc(A &alloc, T **p, size_t count) {
+  return alloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
 }
 
 static void free_list(std::vector<void *> &list) {
@@ -124,84 +120,90 @@ static void free_list(std::vector<void *> &list) {
   list.clear();
 }
 
-// Every buffer of a layout with the given totals (the hsml_ij store apart).
-static int alloc_state(DevState &d, std::vector<void *> &list, int **scan_tmp, size_t N,
-                       size_t NB, size_t C, size_t BLK, size_t INST, size_t G) {
+// Every buffer of a layout with the given totals (the hsml_ij store, its
+// pair slots and kernel_zero apart).  Returns nonzero when the allocator
+// fails.
+template <class A>
+static int alloc_state(DevState &d, A &&alloc, int **scan_tmp, size_t N, size_t NB, size_t C,
+                       size_t BLK, size_t INST, size_t G) {
   int rc = 0;
   N = std::max<size_t>(N, 1);
   NB = std::max<size_t>(NB, 1);
   C = std::max<size_t>(C, 1) + 1;
   for (int b = 0; b < 4; ++b) {
-    rc |= dalloc(list, &d.x[b], N);
-    rc |= dalloc(list, &d.y[b], N);
-    rc |= dalloc(list, &d.vx[b], N);
-    rc |= dalloc(list, &d.vy[b], N);
-    rc |= dalloc(list, &d.h[b], N);
-    rc |= dalloc(list, &d.A[b], N);
-    rc |= dalloc(list, &d.rho[b], N);
-    rc |= dalloc(list, &d.hsml[b], N);
+    rc |= dalloc(alloc, &d.x[b], N);
+    rc |= dalloc(alloc, &d.y[b], N);
+    rc |= dalloc(alloc, &d.vx[b], N);
+    rc |= dalloc(alloc, &d.vy[b], N);
+    rc |= dalloc(alloc, &d.h[b], N);
+    rc |= dalloc(alloc, &d.A[b], N);
+    rc |= dalloc(alloc, &d.rho[b], N);
+    rc |= dalloc(alloc, &d.hsml[b], N);
   }
   for (int b = 0; b < 2; ++b) {
-    rc |= dalloc(list, &d.mass[b], N);
-    rc |= dalloc(list, &d.orig[b], N);
-    rc |= dalloc(list, &d.cellid[b], N);
+    rc |= dalloc(alloc, &d.mass[b], N);
+    rc |= dalloc(alloc, &d.orig[b], N);
+    rc |= dalloc(alloc, &d.cellid[b], N);
   }
-  rc |= dalloc(list, &d.slot, N);
-  rc |= dalloc(list, &d.cl, N);
-  rc |= dalloc(list, &d.own_cnt, N);
-  rc |= dalloc(list, &d.moff, N);
-  rc |= dalloc(list, &d.gpre, N);
-  rc |= dalloc(list, &d.osig, N);
-  rc |= dalloc(list, &d.pf, N);
-  rc |= dalloc(list, &d.nbr, (size_t)kNbrMax * N);
-  rc |= dalloc(list, &d.nnb, N);
-  rc |= dalloc(list, &d.vnb, (size_t)kNbrMax * N);
-  rc |= dalloc(list, &d.vnc, N);
+  rc |= dalloc(alloc, &d.slot, N);
+  rc |= dalloc(alloc, &d.cl, N);
+  rc |= dalloc(alloc, &d.own_cnt, N);
+  rc |= dalloc(alloc, &d.moff, N);
+  rc |= dalloc(alloc, &d.gpre, N);
+  rc |= dalloc(alloc, &d.osig, N);
+  rc |= dalloc(alloc, &d.pf, N);
+  rc |= dalloc(alloc, &d.nbr, (size_t)kNbrMax * N);
+  rc |= dalloc(alloc, &d.nnb, N);
+  rc |= dalloc(alloc, &d.vnb, (size_t)kNbrMax * N);
+  rc |= dalloc(alloc, &d.vnc, N);
 
-  rc |= dalloc(list, &d.part, 7 * BLK);
-  rc |= dalloc(list, &d.bflag, C);
-  rc |= dalloc(list, &d.count, C);
-  rc |= dalloc(list, &d.fill, C);
-  rc |= dalloc(list, &d.cinst, C);
-  rc |= dalloc(list, &d.cell_start, C);
-  rc |= dalloc(list, &d.pair_count, C);
-  rc |= dalloc(list, &d.pair_start, C);
-  rc |= dalloc(list, &d.cell_intra, C);
-  rc |= dalloc(list, &d.cellh, C);
-  rc |= dalloc(list, &d.rebuild, 1);
+  rc |= dalloc(alloc, &d.part, 7 * BLK);
+  rc |= dalloc(alloc, &d.bflag, C);
+  rc |= dalloc(alloc, &d.count, C);
+  rc |= dalloc(alloc, &d.fill, C);
+  rc |= dalloc(alloc, &d.cinst, C);
+  rc |= dalloc(alloc, &d.cell_start, C);
+  rc |= dalloc(alloc, &d.pair_count, C);
+  rc |= dalloc(alloc, &d.pair_start, C);
+  rc |= dalloc(alloc, &d.cell_intra, C);
+  rc |= dalloc(alloc, &d.cellh, C);
+  rc |= dalloc(alloc, &d.rebuild, 1);
   double *sg = nullptr;
-  rc |= dalloc(list, &sg, 2 * N); d.sig11 = sg;
+  rc |= dalloc(alloc, &sg, 2 * N); d.sig11 = sg;
   d.srec = (float4 *)sg;
-  rc |= dalloc(list, &sg, N); d.sig12 = sg;
-  rc |= dalloc(list, &sg, N); d.sig22 = sg;
-  rc |= dalloc(list, &sg, N); d.div = sg;
-  rc |= dalloc(list, &d.kz, N);
-  rc |= dalloc(list, &d.dnum, N);
-  rc |= dalloc(list, &d.dsum, N);
-  rc |= dalloc(list, &d.diag, 6 * N);
-  rc |= dalloc(list, &d.sig_out, 3 * N);
-  rc |= dalloc(list, &d.tend, 6 * N);
-  rc |= dalloc(list, &d.xb, NB);
-  rc |= dalloc(list, &d.yb, NB);
-  rc |= dalloc(list, &d.massb, NB);
-  rc |= dalloc(list, &d.bpre, 9 * NB);
-  rc |= dalloc(list, &d.bbase, NB);
-  rc |= dalloc(list, &d.bcount, NB);
-  rc |= dalloc(list, &d.bcnt, 9 * NB);
-  rc |= dalloc(list, &d.bkey, NB);
-  rc |= dalloc(list, &d.brun, C);
-  rc |= dalloc(list, &d.bcellc, C);
-  rc |= dalloc(list, &d.blist, 9 * NB);
-  rc |= dalloc(list, &d.grid, std::max<size_t>(G, 1));
-  rc |= dalloc(list, &d.meta, std::max<size_t>(INST, 1));
-  rc |= dalloc(list, &d.dyn, std::max<size_t>(INST, 1));
-  rc |= dalloc(list, &d.blk, std::max<size_t>(BLK, 1));
-  rc |= dalloc(list, scan_tmp, (size_t)cdiv((long)C + 1, kScanItems) + 1);
-  return rc ? -6 : 0;
+  rc |= dalloc(alloc, &sg, N); d.sig12 = sg;
+  rc |= dalloc(alloc, &sg, N); d.sig22 = sg;
+  rc |= dalloc(alloc, &sg, N); d.div = sg;
+  rc |= dalloc(alloc, &d.dnum, N);
+  rc |= dalloc(alloc, &d.dsum, N);
+  rc |= dalloc(alloc, &d.diag, 6 * N);
+  rc |= dalloc(alloc, &d.sig_out, 3 * N);
+  rc |= dalloc(alloc, &d.tend, 6 * N);
+  rc |= dalloc(alloc, &d.xb, NB);
+  rc |= dalloc(alloc, &d.yb, NB);
+  rc |= dalloc(alloc, &d.massb, NB);
+  rc |= dalloc(alloc, &d.bpre, 9 * NB);
+  rc |= dalloc(alloc, &d.bbase, NB);
+  rc |= dalloc(alloc, &d.bcount, NB);
+  rc |= dalloc(alloc, &d.bcnt, 9 * NB);
+  rc |= dalloc(alloc, &d.bkey, NB);
+  rc |= dalloc(alloc, &d.brun, C);
+  rc |= dalloc(alloc, &d.bcellc, C);
+  rc |= dalloc(alloc, &d.blist, 9 * NB);
+  rc |= dalloc(alloc, &d.grid, std::max<size_t>(G, 1));
+  rc |= dalloc(alloc, &d.meta, std::max<size_t>(INST, 1));
+  rc |= dalloc(alloc, &d.dyn, std::max<size_t>(INST, 1));
+  rc |= dalloc(alloc, &d.blk, std::max<size_t>(BLK, 1));
+  rc |= dalloc(alloc, scan_tmp, (size_t)cdiv((long)C + 1, kScanItems) + 1);
+  return rc;
 }
 
 Engine::Engine(int precision) : precision_(precision) {
   hipStreamCreate(&st_);
+  // room for one instance's records, so that laying one out in caller
+  // memory allocates nothing on the host either
+  meta_.reserve(1);
+  ids_.reserve(1);
 }
 
 Engine::~Engine() {
@@ -209,7 +211,7 @@ Engine::~Engine() {
   if (st_) hipStreamSynchronize(st_);
   if (stage_) hipHostFree(stage_);
   free_list(allocs_);
-  if (d_.hq) hipFree(d_.hq);
+  if (d_.hq && !arena_) hipFree(d_.hq);
   if (st_) hipStreamDestroy(st_);
 }
 
@@ -327,11 +329,20 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // new buffers
   DevState nd{};
   std::vector<void *> nlist;
+  auto hip_alloc = [&nlist](void **p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      return 1;
+    }
+    nlist.push_back(*p);
+    return 0;
+  };
   int *nscan = nullptr;
-  int rc = alloc_state(nd, nlist, &nscan, ntot, nbtot, ctot, nblk, ninst, gtot);
+  int rc = alloc_state(nd, hip_alloc, &nscan, ntot, nbtot, ctot, nblk, ninst, gtot) ? -6 : 0;
+  if (rc == 0 && dalloc(hip_alloc, &nd.kz, ntot)) rc = -6;
   if (rc == 0 && hipMalloc((void **)&nd.hq, std::max<long>(qtot, 1) * sizeof(double)) != hipSuccess)
     rc = -6;
-  if (rc == 0) rc = dalloc(nlist, &nd.hpair, std::max<long>(qtot, 1));
+  if (rc == 0 && dalloc(hip_alloc, &nd.hpair, qtot)) rc = -6;
   if (rc) {
     free_list(nlist);           // (scan_tmp is on the list)
     if (nd.hq) hipFree(nd.hq);
@@ -501,7 +512,9 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // swap in the new layout
   graph_drop();
   free_list(allocs_);           // (scan_tmp_ is on the list)
-  if (d_.hq) hipFree(d_.hq);
+  if (d_.hq && !arena_) hipFree(d_.hq);
+  arena_ = false;
+  arena_stage_ = nullptr;
   d_ = nd;
   allocs_.swap(nlist);
   scan_tmp_ = nscan;
@@ -690,6 +703,206 @@ int Engine::download_single(double *dxdt, double *dvxdt, double *dhdt, double *d
   return 0;
 }
 
+// ------------------------------------------- one instance in caller memory
+
+namespace {
+// Bump allocation from [base, base + cap) at 256-byte alignment; with a
+// null base it only measures.
+struct Carve {
+  char *base;
+  size_t cap, used = 0;
+  int operator()(void **p, size_t bytes) {
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off > cap || bytes > cap - off) {
+      *p = nullptr;
+      return 1;
+    }
+    *p = base ? (void *)(base + off) : nullptr;
+    used = off + bytes;
+    return 0;
+  }
+};
+
+// The part of the caller memory after the persistent arrays: the layout of
+// one instance of `in` with c_cap cells, then the staging of the call's
+// inputs and results (max(15 n, 4 n + 2 nb) doubles).
+int carve_single(Carve &cv, DevState &nd, int **scan, double **stage, const InstanceInput &in,
+                 long c_cap) {
+  const size_t n = (size_t)in.n, nb = (size_t)in.nb;
+  const size_t g = in.wind_mode == 1 ? 2 * (size_t)in.g_nx * in.g_ny : 0;
+  const size_t nblk = (size_t)cdiv(in.n, kTPB);
+  if (alloc_state(nd, cv, scan, n, nb, (size_t)c_cap, nblk, 1, g)) return 1;
+  return dalloc(cv, stage, std::max(15 * n, 4 * n + 2 * nb));
+}
+}  // namespace
+
+size_t Engine::single_bytes(const InstanceInput &in, long q_cap, long kz_cap) {
+  const long cells = cell_capacity(in);
+  if (cells < 0 || in.n < 1 || in.nb < 0) return 0;
+  Carve cv{nullptr, ~(size_t)0};
+  DevState nd{};
+  int *scan = nullptr;
+  double *stage = nullptr, *hq = nullptr, *kz = nullptr;
+  int2 *hp = nullptr;
+  dalloc(cv, &hq, (size_t)q_cap);
+  dalloc(cv, &hp, (size_t)q_cap);
+  dalloc(cv, &kz, (size_t)kz_cap);
+  carve_single(cv, nd, &scan, &stage, in, cells);
+  return cv.used;
+}
+
+int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
+                             const InstanceInput &in, bool clear_persist, long kz_valid) {
+  if (!meta_.empty() && !arena_) return -5;
+  if (in.n < 1 || in.nb < 0 || in.nb > kMaxBoundaryPerInst) return -1;
+  if (!in.x || !in.vx || !in.mass || !in.rho || !in.hsml || !in.hth || !in.A) return -1;
+  if (in.nb > 0 && (!in.xb || !in.massb)) return -1;
+  if (in.wind_mode == 1 && (in.g_nx < 2 || in.g_ny < 2 || !in.g_u || !in.g_v)) return -1;
+  if (in.n > kz_cap || q_cap < 1 || q_cap > INT32_MAX) return -3;
+  const long cells = cell_capacity(in);
+  if (cells < 0) return -4;
+  if (cells > (1L << 28) || (long)in.n > INT32_MAX / kNbrMax) return -3;
+  // every pointer first: nothing is touched unless the whole layout fits
+  Carve cv{base, bytes};
+  DevState nd{};
+  int *nscan = nullptr;
+  double *stage = nullptr;
+  if (dalloc(cv, &nd.hq, (size_t)q_cap) || dalloc(cv, &nd.hpair, (size_t)q_cap) ||
+      dalloc(cv, &nd.kz, (size_t)kz_cap))
+    return -3;
+  if (carve_single(cv, nd, &nscan, &stage, in, cells)) return -3;
+
+  InstMeta m;
+  std::memset(&m, 0, sizeof(m));
+  m.n = in.n;
+  m.nb = in.nb;
+  m.nblk = cdiv(in.n, kTPB);
+  m.c_cap = (int)cells;
+  m.q_cap = (int)q_cap;
+  m.tse = in.tse;
+  m.wind_mode = in.wind_mode;
+  m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
+  m.cur_u = in.current[0]; m.cur_v = in.current[1];
+  m.max_hsml = in.max_hsml;
+  m.den_alg = in.den_alg;
+  m.pa_sph = in.pa_sph;
+  m.eos_alg = in.eos_alg;
+  if (in.wind_mode == 1) {
+    m.g_nx = in.g_nx; m.g_ny = in.g_ny;
+    m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
+  }
+  m.bxmin = m.bymin = 1e300;
+  m.bxmax = m.bymax = -1e300;
+  for (int b = 0; b < in.nb; ++b) {
+    m.bxmin = std::min(m.bxmin, in.xb[2 * b]); m.bxmax = std::max(m.bxmax, in.xb[2 * b]);
+    m.bymin = std::min(m.bymin, in.xb[2 * b + 1]);
+    m.bymax = std::max(m.bymax, in.xb[2 * b + 1]);
+  }
+  InstDyn y;
+  std::memset(&y, 0, sizeof(y));
+  y.dt = in.dt;
+  y.time = in.time;
+  y.itimestep = in.itimestep;
+  y.active = 1;
+  y.ipstep = -1;
+  y.steps_left = -1;
+  y.rchg = 1;
+  nd.ntot = in.n; nd.nbtot = in.nb; nd.ctot = (int)cells; nd.qtot = (int)q_cap;
+  nd.ninst = 1;
+  nd.nblk = m.nblk;
+
+  const size_t n = (size_t)in.n, nb = (size_t)in.nb, C1 = (size_t)cells + 1;
+  auto h2d = [&](void *dst, const void *src, size_t b) {
+    return b ? hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, st_) : hipSuccess;
+  };
+  auto zero = [&](void *dst, int v, size_t b) {
+    return b ? hipMemsetAsync(dst, v, b, st_) : hipSuccess;
+  };
+  // the persistent arrays: cleared when the caller starts them afresh,
+  // otherwise kernel_zero of particles beyond the last call's count starts
+  // at zero (the hsml_ij store keeps every entry)
+  if (clear_persist) {
+    HIPCHK(zero(nd.hq, 0, (size_t)q_cap * 8));
+    HIPCHK(zero(nd.kz, 0, (size_t)kz_cap * 8));
+  } else if ((long)n > kz_valid) {
+    const long from = std::max(kz_valid, 0L);
+    HIPCHK(zero(nd.kz + from, 0, (size_t)(n - from) * 8));
+  }
+  // what a new layout starts from (as a packed layout does)
+  HIPCHK(zero(nd.count, 0, C1 * 4));
+  HIPCHK(zero(nd.fill, 0, C1 * 4));
+  HIPCHK(zero(nd.cinst, 0, C1 * 4));
+  HIPCHK(zero(nd.cellid[0], 0xff, n * 4));
+  HIPCHK(zero(nd.bflag, 0xff, C1 * 4));
+  HIPCHK(zero(nd.bcellc, 0xff, C1 * 4));
+  HIPCHK(zero(nd.bkey, 0xff, std::max<size_t>(nb, 1) * 4));
+  HIPCHK(zero(nd.bcnt, 0, 9 * std::max<size_t>(nb, 1) * 4));
+  HIPCHK(zero(nd.cellh, 0, C1 * 8));
+  HIPCHK(zero(nd.osig, 0, n * 4));
+  static const int one = 1;
+  HIPCHK(h2d(nd.rebuild, &one, sizeof(int)));
+  // the state: (2, n) arrays through the staging area, the others directly
+  double *sx = stage, *sv = stage + 2 * n, *sb = stage + 4 * n;
+  HIPCHK(h2d(sx, in.x, 2 * n * 8));
+  HIPCHK(h2d(sv, in.vx, 2 * n * 8));
+  HIPCHK(h2d(sb, in.xb, 2 * nb * 8));
+  HIPCHK(h2d(nd.h[0], in.hth, n * 8));
+  HIPCHK(h2d(nd.A[0], in.A, n * 8));
+  HIPCHK(h2d(nd.rho[0], in.rho, n * 8));
+  HIPCHK(h2d(nd.hsml[0], in.hsml, n * 8));
+  HIPCHK(h2d(nd.mass[0], in.mass, n * 8));
+  HIPCHK(h2d(nd.massb, in.massb, nb * 8));
+  if (in.wind_mode == 1) {
+    const size_t gg = (size_t)in.g_nx * in.g_ny;
+    HIPCHK(h2d(nd.grid, in.g_u, gg * 8));
+    HIPCHK(h2d(nd.grid + gg, in.g_v, gg * 8));
+  }
+  HIPCHK(h2d(nd.meta, &m, sizeof(m)));
+  HIPCHK(h2d(nd.dyn, &y, sizeof(y)));
+  launch_single_unpack(nd, sx, sv, sb, in.n, in.nb, st_);
+  launch_single_blocks(nd, in.n, st_);
+
+  d_ = nd;
+  scan_tmp_ = nscan;
+  arena_ = true;
+  arena_stage_ = stage;
+  blk_host_.clear();
+  meta_.assign(1, m);
+  ids_.assign(1, next_id_++);
+  any_boundary_ = in.nb > 0;
+  any_xsw_ = m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3;
+  // grid of the first evaluation
+  launch_partial(d_, 0, st_);
+  launch_finalize(d_, 0, st_);
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+int Engine::single_in_memory_out(double *dxdt, double *dvxdt, double *dhdt, double *dAdt,
+                                 double *eps3, double *zeta, double *eta, double *p,
+                                 double *sig3) {
+  if (!arena_ || d_.ninst != 1) return -5;
+  const size_t n = (size_t)meta_[0].n;
+  const bool want_t = dxdt || dvxdt || dhdt || dAdt, want_d = eps3 || zeta || eta || p || sig3;
+  launch_single_pack(d_, arena_stage_, (int)n, want_t, want_d, st_);
+  auto d2h = [&](double *dst, size_t off, size_t cnt) {
+    return dst ? hipMemcpyAsync(dst, arena_stage_ + off * n, cnt * n * 8, hipMemcpyDeviceToHost,
+                                st_)
+               : hipSuccess;
+  };
+  HIPCHK(d2h(dxdt, 0, 2));
+  HIPCHK(d2h(dvxdt, 2, 2));
+  HIPCHK(d2h(dhdt, 4, 1));
+  HIPCHK(d2h(dAdt, 5, 1));
+  HIPCHK(d2h(eps3, 6, 3));
+  HIPCHK(d2h(zeta, 9, 1));
+  HIPCHK(d2h(eta, 10, 1));
+  HIPCHK(d2h(p, 11, 1));
+  HIPCHK(d2h(sig3, 12, 3));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
 template <class F>
 void Engine::L(const char *name, F &&f) {
   if (!prof) {
@@ -767,6 +980,11 @@ int Engine::steps(int nsteps, bool use_graph) {
 
 int Engine::sync_and_check() {
   HIPCHK(hipStreamSynchronize(st_));
+  if (d_.ninst == 1) {
+    InstDyn y;
+    HIPCHK(hipMemcpy(&y, d_.dyn, sizeof(y), hipMemcpyDeviceToHost));
+    return (y.err & kErrNan) ? -4 : (y.err ? -3 : 0);
+  }
   std::vector<InstDyn> hd(d_.ninst);
   HIPCHK(hipMemcpy(hd.data(), d_.dyn, sizeof(InstDyn) * d_.ninst, hipMemcpyDeviceToHost));
   int worst = 0;

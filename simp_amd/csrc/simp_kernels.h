@@ -29,5 +29,13 @@ void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, 
                          double *hij, double *w, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st);
+// one instance laid out in caller memory: its block table (d.blk, d.nblk),
+// its state from the caller's (2, n) arrays staged on the device, and the
+// results of a single evaluation into the caller's order (15 n doubles)
+void launch_single_blocks(const DevState &d, int n, hipStream_t st);
+void launch_single_unpack(const DevState &d, const double *x2, const double *v2,
+                          const double *xb2, int n, int nb, hipStream_t st);
+void launch_single_pack(const DevState &d, double *out, int n, int want_t, int want_d,
+                        hipStream_t st);
 
 }  // namespace simp

@@ -2143,6 +2143,83 @@ __global__ void __launch_bounds__(kTPB) k_export_bpairs(DevState d, int buf, int
   }
 }
 
+// ------------------------------------------ one instance in caller memory
+
+// The block table of one instance of n particles at packed offset 0, built
+// on the device (the table the host builds for a packed layout): physical
+// block j takes logical block x q + min(x, r) + j / 8, x = j mod 8, q and
+// r the quotient and remainder of nblk / 8, so that the workgroups each XCD
+// receives round-robin cover a contiguous run of slots.
+__global__ void __launch_bounds__(kTPB) k_single_blocks(BlockEntry *blk, int n, int nblk) {
+  const int j = blockIdx.x * kTPB + threadIdx.x;
+  if (j >= nblk) return;
+  const int q = nblk / 8, r = nblk % 8, x = j % 8;
+  const int lb = x * q + min(x, r) + j / 8;
+  BlockEntry e;
+  e.inst = 0;
+  e.first = lb * kTPB;
+  e.lid = lb;
+  e.nloc = min(n - e.first, kTPB);
+  e.pos = e.first;
+  e.cs = 0;
+  e.pad[0] = e.pad[1] = 0;
+  blk[j] = e;
+}
+
+// A call's state in the caller's layout (positions, velocities and boundary
+// positions as (x, y) pairs per particle, copied into device memory as they
+// are) into state set 0, slot = particle index.
+__global__ void __launch_bounds__(kTPB) k_single_unpack(DevState d, const double2 *x2,
+                                                        const double2 *v2, const double2 *xb2,
+                                                        int n, int nb) {
+  const int i = blockIdx.x * kTPB + threadIdx.x;
+  if (i < n) {
+    const double2 p = x2[i], v = v2[i];
+    d.x[0][i] = p.x;
+    d.y[0][i] = p.y;
+    d.vx[0][i] = v.x;
+    d.vy[0][i] = v.y;
+    d.orig[0][i] = i;
+  }
+  if (i < nb) {
+    const double2 b = xb2[i];
+    d.xb[i] = b.x;
+    d.yb[i] = b.y;
+  }
+}
+
+// The results of a single evaluation from slot order into the caller's
+// particle order and layout, at fixed offsets of out (in units of n
+// doubles): dxdt (2, n) at 0, dvxdt (2, n) at 2, dhdt at 4, dAdt at 5,
+// eps3 (3, n) at 6, zeta at 9, eta at 10, p at 11, sig3 (3, n) at 12.
+__global__ void __launch_bounds__(kTPB) k_single_pack(DevState d, double *out, int n,
+                                                      int want_t, int want_d) {
+  const int s = blockIdx.x * kTPB + threadIdx.x;
+  if (s >= n) return;
+  const InstDyn &y = d.dyn[0];
+  const int r = d.orig[y.sel ^ y.rb][s];
+  const size_t N = (size_t)d.ntot, m = (size_t)n;
+  if (want_t) {
+    const double *T = d.tend;
+    out[2 * r] = T[s];
+    out[2 * r + 1] = T[N + s];
+    out[2 * m + 2 * r] = T[2 * N + s];
+    out[2 * m + 2 * r + 1] = T[3 * N + s];
+    out[4 * m + r] = T[4 * N + s];
+    out[5 * m + r] = T[5 * N + s];
+  }
+  if (want_d) {
+    const double *D = d.diag, *S = d.sig_out;
+    for (int c = 0; c < 3; ++c) {
+      out[6 * m + 3 * r + c] = D[c * N + s];
+      out[12 * m + 3 * r + c] = S[c * N + s];
+    }
+    out[9 * m + r] = D[3 * N + s];
+    out[10 * m + r] = D[4 * N + s];
+    out[11 * m + r] = D[5 * N + s];
+  }
+}
+
 // --------------------------------------------------------------- launches
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -2225,6 +2302,18 @@ void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, 
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st) {
   if (nb > 0) k_export_bpairs<<<g8(cdiv(nb, kTPB)), kTPB, 0, st>>>(d, buf, bcap, bpi, bpj, brij);
+}
+void launch_single_blocks(const DevState &d, int n, hipStream_t st) {
+  k_single_blocks<<<cdiv(std::max(d.nblk, 1), kTPB), kTPB, 0, st>>>(d.blk, n, d.nblk);
+}
+void launch_single_unpack(const DevState &d, const double *x2, const double *v2,
+                          const double *xb2, int n, int nb, hipStream_t st) {
+  k_single_unpack<<<cdiv(std::max(std::max(n, nb), 1), kTPB), kTPB, 0, st>>>(
+      d, (const double2 *)x2, (const double2 *)v2, (const double2 *)xb2, n, nb);
+}
+void launch_single_pack(const DevState &d, double *out, int n, int want_t, int want_d,
+                        hipStream_t st) {
+  k_single_pack<<<cdiv(std::max(n, 1), kTPB), kTPB, 0, st>>>(d, out, n, want_t, want_d);
 }
 
 }  // namespace simp
