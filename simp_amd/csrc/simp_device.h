@@ -77,6 +77,8 @@ struct InstDyn {
   int rlim, rstop;      // k_ranks: rank limit, list positions whose pairs rank below it
   int gendI, gendX;     //   and the pair prefixes at the end of the lists
   int rchg;             // 1: the pair ranks below the limit must be found again
+  int rfresh;           // 1: pass B takes the hsml_ij of the ranks below the limit
+                        //    from their pair slots (and stores it)
   double hmin;          // MINVAL(hsml) of the current evaluation
   // candidate lists (kept from the evaluation that built them while they
   // still cover the pair set): built, their skin, the skin of lists built

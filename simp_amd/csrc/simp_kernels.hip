@@ -1482,7 +1482,10 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
   }
   const int lim = min(bnip, m.q_cap);
   if (lim <= 0) {
-    if (threadIdx.x == 0) d.dyn[k].rlim = 0;
+    if (threadIdx.x == 0) {
+      d.dyn[k].rlim = 0;
+      d.dyn[k].rfresh = 0;
+    }
     return;
   }
   const int ss = cur_set(y);
@@ -1490,30 +1493,12 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
   // 1) the pairs below the limit are those of the last search while the
   //    slots, the limit and the head's owned neighbours are unchanged (pass
   //    A flags a change); their hsml_ij changes only with hsml, at the end of
-  //    a step -- the corrector's half-step state has the predictor's hsml
+  //    a step -- the corrector's half-step state has the predictor's hsml.
+  //    Every rank below the limit is the index of exactly one boundary pair,
+  //    so pass B, which reads each of them once, refreshes the entry from
+  //    the pair's slots (rfresh) in the evaluations of a step's start state.
   if (y.rb == 0 && y.rchg == 0 && lim == y.rlim) {
-    if (buf == 0) {
-      const double *HS = d.hsml[2 * ss + buf];
-      double *HQ = d.hq + m.q_off;
-      constexpr int kU = 4;
-      for (int r0 = threadIdx.x; r0 < lim; r0 += kRankThreads * kU) {
-        int2 hp[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int r = r0 + u * kRankThreads;
-          hp[u] = r < lim ? HP[r] : make_int2(-1, -1);
-        }
-        double ha[kU], hb[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          ha[u] = hp[u].x >= 0 ? HS[hp[u].x] : 0.0;
-          hb[u] = hp[u].x >= 0 ? HS[hp[u].y] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-          if (hp[u].x >= 0) HQ[r0 + u * kRankThreads] = (ha[u] + hb[u]) / 2.;
-      }
-    }
+    if (threadIdx.x == 0) d.dyn[k].rfresh = buf == 0;
     return;
   }
   // 2) otherwise search again: no pair at any rank yet
@@ -1563,6 +1548,7 @@ __global__ void __launch_bounds__(kRankThreads) k_ranks(DevState d, int buf) {
     rank_store(d, y, m, ss, buf, t, lim, baseI, baseX);
   if (threadIdx.x == 0) {
     InstDyn &yw = d.dyn[k];
+    yw.rfresh = 0;      // the search wrote the entries
     yw.rchg = 0;
     yw.rlim = lim;
     yw.rstop = stop;
@@ -1925,6 +1911,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           (((unsigned long long)1 << lane) - 1) & ~(((unsigned long long)1 << first) - 1);
       int nbn = 0;
       float fbx = 0.0f, fby = 0.0f;   // mixed: the forces' sum
+      const bool rfresh = phase != 1 && y.rfresh;
       const int *bl = d.blist + 9 * (size_t)m.b_off;
       for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
         int ent[kBU], rb[kBU];
@@ -1955,7 +1942,18 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             }
           }
           const int rank = rb[j] + before;
-          hk[j] = (inr[j] && rank < m.q_cap) ? d.hq[m.q_off + rank] : 0.0;
+          hk[j] = 0.0;
+          if (inr[j] && rank < m.q_cap) {
+            // the entry of a pair below the rank limit is refreshed from the
+            // pair's smoothing lengths (and kept for later stale reads)
+            const int2 hp = rfresh ? d.hpair[m.q_off + rank] : make_int2(-1, -1);
+            if (hp.x >= 0) {
+              hk[j] = (HS[hp.x] + HS[hp.y]) / 2.;
+              d.hq[m.q_off + rank] = hk[j];
+            } else {
+              hk[j] = d.hq[m.q_off + rank];
+            }
+          }
         }
 #pragma unroll
         for (int j = 0; j < kBU; ++j) {
