@@ -209,6 +209,27 @@ def hbm_traffic(kernel_substr, timeout=240):
                                                  "fetch_kib": fetch, "write_kib": write}
 
 
+def bench_forced_search(sc, steps):
+    """The same step with the cell lists and candidate lists rebuilt at every
+    evaluation, as the reference's bucket_search searches (same results)."""
+    from simp_amd import _lib
+    from simp_amd.api import Simp
+    out = {"note": "cell lists and candidate lists rebuilt at every evaluation "
+                   "(simp_set_search(h, 1)); same results as the headline",
+           "steps": steps}
+    for name, prec in (("mixed", _lib.SIMP_MIXED), ("fp64", _lib.SIMP_FP64)):
+        dev = Simp.for_scenario(sc, precision=prec)
+        dev.set_search(True)
+        dev.upload(sc)
+        dev.step(10, sync=True)
+        wall = _timed_steps(dev, steps)
+        st = dev.search_stats()
+        assert st["cell_rebuilds"] == st["evaluations"]
+        out[name] = {"value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps}
+        dev.close()
+    return out
+
+
 def bench_field_error(sc):
     """Mixed-precision error per field against the FP64 path (which the
     parity tests pin to the oracle) at the workloads' own step counts: the
@@ -399,7 +420,8 @@ def main():
         lat = _latencies(dev, args.latency_steps)
         res[prec] = {"rate": n * args.steps / wall, "ms": 1e3 * wall / args.steps,
                      "p50": 1e3 * float(np.percentile(lat, 50)),
-                     "p99": 1e3 * float(np.percentile(lat, 99))}
+                     "p99": 1e3 * float(np.percentile(lat, 99)),
+                     "search": dev.search_stats()}
         prof[prec] = dev.profile_steps(20)
         dev.close()
 
@@ -448,6 +470,15 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    line["search"] = {
+        "note": "evaluations since the upload (warm-up, timed and latency steps) and the "
+                "share that rebuilt the cell lists / the candidate lists",
+        **{name: {"evaluations": r["search"]["evaluations"],
+                  "cell_rebuild_rate": r["search"]["cell_rebuilds"] / r["search"]["evaluations"],
+                  "candidate_rebuild_rate":
+                      r["search"]["candidate_rebuilds"] / r["search"]["evaluations"]}
+           for name, r in (("mixed", mixed), ("fp64", fp64))}}
+    line["search_every_evaluation"] = bench_forced_search(sc, min(args.steps, 300))
     line["field_error"] = bench_field_error(sc)
     line["dropin_single_step"] = bench_dropin(sc)
     line["snapshots"] = bench_snapshots(sc)

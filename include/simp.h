@@ -156,6 +156,22 @@ int simp_single_step_noalloc(simp_handle *h, int n, const double *x,
 int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
                        double *w, double *kernel_zero);
 
+/* ---- how often the neighbour search runs ----
+ * The reference's bucket_search (nnps.f90:155-517) sorts the particles into
+ * cells and tests the 3x3 block of every particle at every evaluation.  This
+ * library rebuilds its cell lists only in evaluations where a particle
+ * changed cell, and its candidate lists (the 3x3 block within mhsml plus a
+ * skin) only when the particles may have moved by more than the skin; the
+ * pair sets and their order are the same either way.
+ * simp_search_stats: the evaluations of the handle's instance since its
+ * upload, and of them those that rebuilt the cell lists and those that
+ * rebuilt the candidate lists.  simp_set_search(h, 1) rebuilds both at
+ * every evaluation (same results; a measurement of the full search),
+ * simp_set_search(h, 0) returns to the default. */
+int simp_set_search(simp_handle *h, int every_evaluation);
+int simp_search_stats(simp_handle *h, long long *evaluations,
+                      long long *cell_rebuilds, long long *candidate_rebuilds);
+
 /* ---- interaction_stats (neighbor/nnps.f90:613-658) ----
  * The neighbour statistics the reference prints from NNPS at every step that
  * is a multiple of print_step (nnps.f90:512-515): the largest and smallest

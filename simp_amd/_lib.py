@@ -33,6 +33,7 @@ EXPORTS = (
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
     "simp_batch_update", "simp_interaction_stats", "simp_batch_interaction_stats",
+    "simp_set_search", "simp_search_stats",
     "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
@@ -101,6 +102,8 @@ def lib():
     L.simp_single_step.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp,
                                    C.c_double, C.c_int] + [_vp] * 9 + [_vp, _vp])
     L.simp_single_step_noalloc.argtypes = L.simp_single_step.argtypes
+    L.simp_set_search.argtypes = [_vp, C.c_int]
+    L.simp_search_stats.argtypes = [_vp, _vp, _vp, _vp]
     L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
                             _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]

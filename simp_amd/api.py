@@ -153,6 +153,21 @@ class Simp:
         out["nip"], out["bnip"] = nip.value, bnip.value
         return out
 
+    def set_search(self, every_evaluation: bool) -> None:
+        """Rebuild the cell and candidate lists at every evaluation (as the
+        reference's bucket_search does) instead of when needed; the results
+        are the same."""
+        check(lib().simp_set_search(self._h, 1 if every_evaluation else 0), "simp_set_search")
+
+    def search_stats(self) -> dict:
+        """Evaluations since the upload, and of them those that rebuilt the
+        cell lists and the candidate lists."""
+        e, c, v = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        check(lib().simp_search_stats(self._h, C.byref(e), C.byref(c), C.byref(v)),
+              "simp_search_stats")
+        return {"evaluations": e.value, "cell_rebuilds": c.value,
+                "candidate_rebuilds": v.value}
+
     def interaction_stats(self) -> dict:
         """``interaction_stats`` (``neighbor/nnps.f90:613-658``) of the last
         evaluation that collected it: every ``single_step`` and both

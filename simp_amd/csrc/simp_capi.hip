@@ -330,6 +330,19 @@ static int stats_out(Engine *e, int k, int n, int *maxip, int *minip, double *av
   return SIMP_OK;
 }
 
+int simp_set_search(simp_handle *h, int every_evaluation) {
+  if (!h || (every_evaluation != 0 && every_evaluation != 1)) return SIMP_E_ARG;
+  h->eng->set_force_search(every_evaluation != 0);
+  return SIMP_OK;
+}
+
+int simp_search_stats(simp_handle *h, long long *evaluations, long long *cell_rebuilds,
+                      long long *candidate_rebuilds) {
+  if (!h) return SIMP_E_ARG;
+  if (h->eng->ninst() != 1 || scratch_state_lost(h)) return SIMP_E_STATE;
+  return map_rc(h->eng->search_stats(0, evaluations, cell_rebuilds, candidate_rebuilds));
+}
+
 int simp_interaction_stats(simp_handle *h, int *maxip, int *minip, double *avgip, int *noip,
                            long long *nip, int *itimestep) {
   if (!h) return SIMP_E_ARG;

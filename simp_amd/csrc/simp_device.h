@@ -94,6 +94,8 @@ struct InstDyn {
   // pairs of the particle
   int ipstep, ipmax, ipmin, ipzero;
   unsigned long long ipsum, ipown;    // sum of interactions; particle pairs
+  unsigned long long n_eval, n_cell, n_cand;   // evaluations; of them, those that
+                        // rebuilt the cell lists / the candidate lists
   int sel;              // state set holding the instance at the start of an evaluation
   int rb;               // 1: its lists are rebuilt in this evaluation (the state
                         //    then moves to set sel ^ 1: the current set is sel ^ rb)
@@ -180,6 +182,7 @@ struct DevState {
   BlockEntry *blk;       // particle blocks
   int nblk;
   int ntot, nbtot, ctot, qtot, ninst;
+  int force_search;      // 1: rebuild the cell and candidate lists at every evaluation
 };
 
 }  // namespace simp

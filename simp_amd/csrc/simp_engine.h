@@ -128,6 +128,12 @@ class Engine {
   int upload_persist(int inst, const double *kz, const double *hq, long nq);
   int cur_set(int inst);
 
+  // 1: rebuild the cell and candidate lists at every evaluation (the
+  // reference searches anew every time; the results are the same)
+  void set_force_search(bool on);
+  // instance k's evaluations since its upload, and of them those that
+  // rebuilt the cell lists / the candidate lists
+  int search_stats(int k, long long *evals, long long *cell, long long *cand);
   void set_diag(bool on) { diag_ = on; }
   bool diag() const { return diag_; }
   hipStream_t stream() const { return st_; }
@@ -165,6 +171,7 @@ class Engine {
   bool any_xsw_ = false;              // some instance uses a non-default switch value
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
+  bool force_search_ = false;
   bool arena_ = false;                // d_ lies in caller memory (single_in_memory)
   double *arena_stage_ = nullptr;     // its staging area
 };

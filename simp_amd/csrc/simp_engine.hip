@@ -516,6 +516,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   arena_ = false;
   arena_stage_ = nullptr;
   d_ = nd;
+  d_.force_search = force_search_;
   allocs_.swap(nlist);
   scan_tmp_ = nscan;
   d_.ntot = (int)ntot; d_.nbtot = (int)nbtot; d_.ctot = (int)ctot; d_.qtot = (int)qtot;
@@ -863,6 +864,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   launch_single_blocks(nd, in.n, st_);
 
   d_ = nd;
+  d_.force_search = force_search_;
   scan_tmp_ = nscan;
   arena_ = true;
   arena_stage_ = stage;
@@ -993,6 +995,24 @@ int Engine::sync_and_check() {
     if (y.err) worst = -3;
   }
   return worst;
+}
+
+void Engine::set_force_search(bool on) {
+  if (force_search_ == on) return;
+  force_search_ = on;
+  d_.force_search = on;
+  graph_drop();           // the captured launches hold the old value
+}
+
+int Engine::search_stats(int k, long long *evals, long long *cell, long long *cand) {
+  if (k < 0 || k >= d_.ninst) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  InstDyn y;
+  HIPCHK(hipMemcpy(&y, d_.dyn + k, sizeof(y), hipMemcpyDeviceToHost));
+  if (evals) *evals = (long long)y.n_eval;
+  if (cell) *cell = (long long)y.n_cell;
+  if (cand) *cand = (long long)y.n_cand;
+  return 0;
 }
 
 int Engine::download_scalars(int k, double *dt, double *time, int *itimestep, int *nip,

@@ -611,10 +611,14 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
     }
     key = c;
   }
-  // blocks never straddle instances: one instance per wavefront
-  if (__any(changed) && lane == 0) {
-    atomicOr(&d.dyn[k].rb, 1);
-    atomicOr(d.rebuild, 1);
+  // blocks never straddle instances: one instance per wavefront.  With
+  // force_search every evaluation rebuilds the lists (the same order: the
+  // measurement of the reference's search at every evaluation).
+  // (plain stores of the same value: atomics from every wavefront of a
+  // rebuild would serialize on two addresses)
+  if ((d.force_search ? __any(valid) : __any(changed)) && lane == 0) {
+    d.dyn[k].rb = 1;
+    *d.rebuild = 1;
   }
   if (__any(changed)) {
     run_add(d.count + coff, changed ? key : -1 - lane, 1);
@@ -1012,6 +1016,14 @@ k_passA(DevState d, int buf, int want_diag) {
   // speed and hsml change of every evaluation since.  Then the prefilter
   // is skipped.
   const bool vok = y.rb == 0 && !lists_stale(y);
+  // search statistics of the instance: evaluations, and those that rebuilt
+  // the cell lists / the candidate lists
+  if (be.first == 0 && threadIdx.x == 0) {
+    InstDyn &yw = d.dyn[k];
+    yw.n_eval += 1;
+    if (y.rb) yw.n_cell += 1;
+    if (!vok) yw.n_cand += 1;
+  }
   const float skin = kSkin * (float)y.hmin;
   // The candidates of every particle of the block lie in three runs of the
   // cell lists: its cell range shifted by -W, 0 and +W cells, widened by one

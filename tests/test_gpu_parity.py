@@ -192,6 +192,34 @@ def test_candidate_lists_across_their_skin():
         assert e <= 1e-12, (k, e)
 
 
+@pytest.mark.parametrize("precision", [_lib.SIMP_FP64, _lib.SIMP_MIXED])
+def test_search_at_every_evaluation_changes_nothing(precision):
+    """simp_set_search(1) rebuilds the cell and candidate lists at every
+    evaluation, as the reference searches; the kept lists must give the
+    same bits.  The search statistics count the evaluations and rebuilds."""
+    say(f"search every evaluation vs kept lists, box2500, precision={precision}")
+    sc = S.box(seed=33)
+    sc.vx = np.random.default_rng(6).normal(0.0, 3.0, size=sc.vx.shape)
+    sc.tse = 0
+    sc.dt = 0.05
+    sc.time = 1.0
+    out, stats = [], []
+    for every in (False, True):
+        dev = Simp.for_scenario(sc, precision=precision)
+        dev.set_search(every)
+        dev.upload(sc)
+        dev.step(12)
+        out.append(dev.download())
+        stats.append(dev.search_stats())
+        dev.close()
+    for k in ("x", "vx", "rho", "hsml", "h", "A"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    kept, full = stats
+    assert kept["evaluations"] == full["evaluations"] == 24
+    assert full["cell_rebuilds"] == full["candidate_rebuilds"] == 24
+    assert 1 <= kept["cell_rebuilds"] < 24 and 1 <= kept["candidate_rebuilds"] < 24
+
+
 def test_graph_replay_equals_eager_launches():
     say("graph vs eager, box2500 6 steps")
     sc = S.box(seed=21)
