@@ -101,6 +101,27 @@ variant eos1 'parameter(eos_alg = 3)' 'parameter(eos_alg = 1)'
 variant eos2 'parameter(eos_alg = 3)' 'parameter(eos_alg = 2)'
 E2="$OUT/eos2"
 e2objs=("${vobjs[@]}")
+# Mohr-Coulomb rheology (rheology = 2; bulk_viscosity.f90:86-145,
+# shear_viscosity.f90:67-124)
+variant mc 'parameter(rheology = 1)' 'parameter(rheology = 2)'
+MC="$OUT/mc"
+mcobjs=("${vobjs[@]}")
+# Wendland C2 and C4 kernels (skf = 4, 5; kernel.f90:359-464,
+# boundary_kernel.f90:290-431)
+variant c2 'parameter(skf = 6)' 'parameter(skf = 4)'
+C2="$OUT/c2"
+c2objs=("${vobjs[@]}")
+variant c4 'parameter(skf = 6)' 'parameter(skf = 5)'
+C4="$OUT/c4"
+c4objs=("${vobjs[@]}")
+# Euler and leap-frog time integration (int_algorithm = 1, 3;
+# time_integration.f90:86-148, 289-391)
+variant euler 'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 1)'
+EU="$OUT/euler"
+euobjs=("${vobjs[@]}")
+variant leap 'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 3)'
+LF="$OUT/leap"
+lfobjs=("${vobjs[@]}")
 
 # The OpenMP build, for the CPU baseline: the reference compiled as its
 # Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
@@ -174,4 +195,9 @@ if [ -f "$LIB" ]; then
     gpu_harness "$D1/obj" "$D1/mod" "$OUT/simp_ref_harness_gpu_den1" "${d1objs[@]}"
     gpu_harness "$P2/obj" "$P2/mod" "$OUT/simp_ref_harness_gpu_pa2" "${p2objs[@]}"
     gpu_harness "$E2/obj" "$E2/mod" "$OUT/simp_ref_harness_gpu_eos2" "${e2objs[@]}"
+    gpu_harness "$MC/obj" "$MC/mod" "$OUT/simp_ref_harness_gpu_mc" "${mcobjs[@]}"
+    gpu_harness "$C2/obj" "$C2/mod" "$OUT/simp_ref_harness_gpu_c2" "${c2objs[@]}"
+    gpu_harness "$C4/obj" "$C4/mod" "$OUT/simp_ref_harness_gpu_c4" "${c4objs[@]}"
+    gpu_harness "$EU/obj" "$EU/mod" "$OUT/simp_ref_harness_gpu_euler" "${euobjs[@]}"
+    gpu_harness "$LF/obj" "$LF/mod" "$OUT/simp_ref_harness_gpu_leap" "${lfobjs[@]}"
 fi

@@ -23,7 +23,8 @@ class OrcParams(C.Structure):
                 ("grid_x0", C.c_double), ("grid_y0", C.c_double),
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p), ("pa_sph", C.c_int),
-                ("eos_alg", C.c_int)]
+                ("eos_alg", C.c_int), ("rheology", C.c_int), ("skf", C.c_int),
+                ("int_algorithm", C.c_int)]
 
 
 def build() -> None:
@@ -81,6 +82,9 @@ def make_params(sc) -> tuple:
     p.den_alg = getattr(sc, "den_alg", 3)
     p.pa_sph = getattr(sc, "pa_sph", 1)
     p.eos_alg = getattr(sc, "eos_alg", 3)
+    p.rheology = getattr(sc, "rheology", 1)
+    p.skf = getattr(sc, "skf", 6)
+    p.int_algorithm = getattr(sc, "int_algorithm", 2)
     keep = ()
     if sc.wind_grid is not None:
         g = sc.wind_grid

@@ -30,6 +30,19 @@ HARNESS_GPU_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_pa2")
 HARNESS_EOS1 = os.path.join(HERE, "_ref", "simp_ref_harness_eos1")
 HARNESS_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_eos2")
 HARNESS_GPU_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_eos2")
+#: the harnesses built with Mohr-Coulomb rheology (rheology = 2), the
+#: Wendland C2 / C4 kernels (skf = 4, 5) and the Euler / leap-frog schemes
+#: (int_algorithm = 1, 3), and their GPU drop-in counterparts
+HARNESS_MC = os.path.join(HERE, "_ref", "simp_ref_harness_mc")
+HARNESS_C2 = os.path.join(HERE, "_ref", "simp_ref_harness_c2")
+HARNESS_C4 = os.path.join(HERE, "_ref", "simp_ref_harness_c4")
+HARNESS_EULER = os.path.join(HERE, "_ref", "simp_ref_harness_euler")
+HARNESS_LEAP = os.path.join(HERE, "_ref", "simp_ref_harness_leap")
+HARNESS_GPU_MC = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_mc")
+HARNESS_GPU_C2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_c2")
+HARNESS_GPU_C4 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_c4")
+HARNESS_GPU_EULER = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_euler")
+HARNESS_GPU_LEAP = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_leap")
 #: the OpenMP build sized for the 250k workload (maxn = 300000), timings only
 HARNESS_OMP = os.path.join(HERE, "_ref", "simp_ref_harness_omp")
 #: the serial build with the same capacity

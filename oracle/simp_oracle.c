@@ -44,6 +44,13 @@ static const double K_T = 0.0;            /* T_star / P_star */
 static const double E_ELL = 2.0;          /* 0.5 (P*+T*) / S*, exact */
 static const double HSML_MULT = 3.0;      /* hsml_multiplicity */
 #define RHO_AIR ((double)1.3f)            /* rho_air = 1.3 (REAL(4) literal) */
+/* Mohr-Coulomb (data_variables.f90:153-180): mu = TAN(phi pi / 180.) with
+ * phi = 45, 1 - 2^-53 as the reference build folds it; zeta_max and eta_max
+ * are declared there but never given a value, and the compiled reference's
+ * module storage holds zero, so the caps are zero times the pressure. */
+static const double MU_MC = 0x1.fffffffffffffp-1;
+static const double ZETA_MAX = 0.0, ETA_MAX = 0.0;
+static const double ZERO_MC = 1.e-18;      /* zero = 1.d-18 */
 
 enum { ORC_OK = 0, ORC_E_ARG = 1, ORC_E_NOMEM = 2 };
 
@@ -360,6 +367,74 @@ static int ensure_scratch(orc_ctx *c, int n) {
 }
 
 /* ---- single_step, single_step.f90:5-108 ------------------------------ */
+/* One pair's kernel (kernel.f90): kernel_zero(cp_i), w(int_id) and the
+ * gradient factor g with dwdx = g (dx / r) scale_h.  Fortran's x**k is
+ * expanded as repeated products (ipow), default-REAL literals are exact. */
+static void kernel_pair(int skf, double r, double hij, double *kz, double *w, double *g) {
+    double q = r / hij * 1.0;                       /* scale_h = 1 */
+    double ih = 1.0 / hij;
+    if (skf == 4) {
+        /* Wendland_C2 (kernel.f90:387-396) */
+        double factor = 3. / (2. * PI_) * (ih * ih);
+        *kz = factor;
+        *w = factor * (ipow(1. - q, 4) * (1 + 4 * q));
+        *g = factor * (20. * (ipow(q, 4) - 3 * ipow(q, 3) + 3 * ipow(q, 2) - q)) / hij;
+    } else if (skf == 5) {
+        /* Wendland_C4 (kernel.f90:443-452) */
+        double factor = 3. / PI_ * (ih * ih);
+        *kz = factor * 3;
+        *w = factor * (ipow(1 - q, 6) * (35 * ipow(q, 2) + 18 * q + 3));
+        *g = factor * (-(56 * q * (5 * q + 1) * ipow(1 - q, 5))) / hij;
+    } else {
+        /* Wendland_C6 (kernel.f90:499-508) */
+        double factor = 78. / (7. * PI_) * (ih * ih);
+        double omq = 1 - q;
+        *kz = factor;
+        *w = factor * ipow(omq, 8) * (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
+        *g = factor * (-(22. * q * ipow(omq, 7) * (16 * ipow(q, 2) + 7. * q + 1))) / hij;
+    }
+}
+
+/* The boundary kernel's value bw of one boundary pair (boundary_kernel.f90):
+ * b_Wendland_C6 reads hsml_ij at the boundary-pair index (:491), C2 and C4
+ * use b_length (:343, :417). */
+static double boundary_w(int skf, double br, double hk) {
+    double q = br / B_LENGTH * 1.0;
+    if (skf == 4) {
+        double t = 1.0 / B_LENGTH;
+        double factor = 3. / (2. * PI_) * (t * t);
+        return factor * (ipow(1. - q, 4) * (1 + 4 * q));
+    }
+    if (skf == 5) {
+        double t = 1.0 / B_LENGTH;
+        double factor = 3. / PI_ * (t * t);
+        return factor * (ipow(1 - q, 6) * (35 * ipow(q, 2) + 18 * q + 3));
+    }
+    double ih = 1.0 / hk;
+    double factor = 78. / (7. * PI_) * (ih * ih);
+    double omq = 1 - q;
+    return factor * ipow(omq, 8) * (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
+}
+
+/* MC_bulk_viscosity and MC_shear_viscosity (bulk_viscosity.f90:113-143,
+ * shear_viscosity.f90:91-122) at one particle. */
+static void mc_viscosity(double e11, double e12, double e22, double P, double *z,
+                         double *et) {
+    double sq = sqrt((e11 - e22) * (e11 - e22) + 4 * (e12 * e12));
+    double e1 = 0.5 * ((e11 + e22) + sq);
+    double e2 = 0.5 * ((e11 + e22) - sq);
+    double eI = fabs(e1 + e2);
+    if (eI < ZERO_MC) eI = ZERO_MC;
+    double zz = 0.5 * P * (1 + K_T) / eI;
+    if (zz > ZETA_MAX * P) zz = ZETA_MAX * P;
+    double eII = e1 - e2;
+    if (fabs(eII) < ZERO_MC) eII = ZERO_MC;
+    double ee = MU_MC * (0.5 * P * (1 + K_T) - zz * (e1 + e2)) / eII;
+    if (ee > ETA_MAX * P) ee = ETA_MAX * P;
+    *z = zz;
+    *et = ee;
+}
+
 int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
                     const double *vx, const double *mass, const double *rho,
                     const double *hsml, const double *h, const double *A,
@@ -388,24 +463,18 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
     /* NNPS (:47) */
     int rc = orc_nnps(c, n, x, hsml, nb, xb);
     if (rc) return rc;
-    /* pair loop (:56-75): kernel (Wendland_C6, kernel.f90:471-520),
+    /* pair loop (:56-75): kernel (Wendland C6 / C2 / C4, kernel.f90:471-520,
+     * :359-408, :415-464; scale_h = 1 for all three, initialize.f90:208-209),
      * vector_divergence (vector_divergence.f90:27-28),
      * vector_gradient (vector_gradient.f90:27-40) */
-    const double c6 = 78. / (7. * PI_);
+    const int skf = prm->skf ? prm->skf : 6;
     for (int k = 0; k < c->nip; ++k) {
         int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
         double dx0 = x[2 * i] - x[2 * j], dx1 = x[2 * i + 1] - x[2 * j + 1];
         double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
         double r = c->r_ij[k], hij = c->hsml_ij[k];
-        double q = r / hij * 1.0;                       /* scale_h = 1 */
-        double ih = 1.0 / hij;
-        double factor = c6 * (ih * ih);
-        double omq = 1 - q;
-        double o7 = ipow(omq, 7);
-        /* kernel_zero(cp_i) = factor; w = factor (1-q)^8 (32q^3 + 25q^2 + 8q + 1) */
-        c->kz[i] = factor;
-        c->w[k] = factor * ipow(omq, 8) * (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
-        double g = factor * (-(22. * q * o7 * (16 * ipow(q, 2) + 7. * q + 1))) / hij;
+        double g;
+        kernel_pair(skf, r, hij, &c->kz[i], &c->w[k], &g);
         double dw0 = g * (dx0 / r) * 1.0;
         double dw1 = g * (dx1 / r) * 1.0;
         c->dwdx[2 * k] = dw0;
@@ -442,6 +511,7 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double dcl = fabs(delta) < DELTA_MIN ? DELTA_MIN : delta;
         double z = 0.5 * P * (1 + K_T) / dcl;
         double et = z / (E_ELL * E_ELL);
+        if (prm->rheology == 2) mc_viscosity(e11, e12, e22, P, &z, &et);
         double pr = z * 2 * delta / (1 + K_T);               /* kreyscher_eos */
         if (prm->eos_alg == 1) {
             pr = P;                                           /* hibler_eos (eos.f90:59) */
@@ -489,21 +559,17 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + rho[i] * mass[j] / RHO_ICE / h[i] * (ci + cj);
         dvxdt[2 * j + 1] = dvxdt[2 * j + 1] + rho[j] * mass[i] / RHO_ICE / h[j] * (-ci - cj);
     }
-    /* boundary_kernel -> b_Wendland_C6 (boundary_kernel.f90:477-502) ->
-     * bf_Monaghan (boundary_force.f90:78-82).  The kernel factor reads
-     * hsml_ij at the boundary-pair index, i.e. an entry of the particle-pair
-     * list (:491); stale entries beyond nip are read as they are. */
+    /* boundary_kernel -> b_Wendland_C6 / C2 / C4 (boundary_kernel.f90:438-504,
+     * :290-357, :364-431) -> bf_Monaghan (boundary_force.f90:78-82).  The C6
+     * kernel factor reads hsml_ij at the boundary-pair index, i.e. an entry
+     * of the particle-pair list (:491); stale entries beyond nip are read as
+     * they are. */
     for (int k = 0; k < c->bnip; ++k) {
         int i = c->bp_j[k] - 1, b = c->bp_i[k] - 1;
         double bdx0 = xb[2 * b] - x[2 * i], bdx1 = xb[2 * b + 1] - x[2 * i + 1];
         double br = c->br_ij[k];
-        double q = br / B_LENGTH * 1.0;
         double hk = k < c->cap ? c->hsml_ij[k] : 0.0;
-        double ih = 1.0 / hk;
-        double factor = c6 * (ih * ih);
-        double omq = 1 - q;
-        double bw = factor * ipow(omq, 8) *
-                    (32 * ipow(q, 3) + 25 * ipow(q, 2) + 8 * q + 1);
+        double bw = boundary_w(skf, br, hk);
         double den = (mass[i] + massb[b]) * (br * br);
         double F0 = KAPPA_N * bdx0 * bw * 2 * massb[b] / den;
         double F1 = KAPPA_N * bdx1 * bw * 2 * massb[b] / den;
@@ -585,8 +651,9 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
     double dt = *dt_io, t = *time_io;
     int it = *itimestep;
     int rc = ORC_OK;
+    const int scheme = prm->int_algorithm ? prm->int_algorithm : 2;
     for (int s = 0; s < nsteps && rc == ORC_OK; ++s) {
-        /* monitor current quantities (:197-201) */
+        /* monitor current quantities (:197-201, :323-327) */
         memcpy(x1, x, n2 * sizeof(double));
         memcpy(v1, vx, n2 * sizeof(double));
         memcpy(h1, h, (size_t)n * sizeof(double));
@@ -595,6 +662,21 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
         if (rc) break;
+        if (scheme == 1) {
+            /* euler_scheme (:114-146): full step, density, Marquis_hsml */
+            for (size_t i = 0; i < n2; ++i) {
+                x[i] = x[i] + dxdt[i] * dt;
+                vx[i] = vx[i] + dvxdt[i] * dt;
+            }
+            for (int i = 0; i < n; ++i) {
+                h[i] = h[i] + dhdt[i] * dt;
+                A[i] = A[i] + dAdt[i] * dt;
+                if (A[i] > 1) A[i] = 1;
+                if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+            }
+            if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+            goto end_of_step;
+        }
         /* predict half step (:207-215), density (:218) */
         for (size_t i = 0; i < n2; ++i) {
             x[i] = x[i] + dxdt[i] * dt / 2;
@@ -611,6 +693,22 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
         if (rc) break;
+        if (scheme == 3) {
+            /* leapfrog_scheme (:361-372): velocity, thickness, position from
+             * the step-start values and the half-step tendencies, density */
+            for (size_t i = 0; i < n2; ++i) {
+                vx[i] = v1[i] + dvxdt[i] * dt;
+                x[i] = x1[i] + (v1[i] + dxdt[i]) * dt / 2;
+            }
+            for (int i = 0; i < n; ++i) {
+                h[i] = h1[i] + dhdt[i] * dt;
+                A[i] = A1[i] + dAdt[i] * dt;
+                if (A[i] > 1) A[i] = 1;
+                if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+            }
+            if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+            goto end_of_step;
+        }
         /* corrector (:236-262); with summation density, density() after
          * the half-step values (:244) already changes rho */
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
@@ -629,6 +727,7 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
         }
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+    end_of_step:
         for (int i = 0; i < n; ++i) {
             /* Marquis_hsml (smoothing_length.f90:136-142) */
             hsml[i] = sqrt(mass[i] / rho[i]) * HSML_MULT;

@@ -36,6 +36,18 @@ typedef struct orc_params {
                             2: stress_2 (:189-216) */
     int    eos_alg;      /* 3 (or 0): kreyscher_eos (eos.f90:141-147);
                             1: hibler_eos (:59); 2: gutfraind_eos (:94-102) */
+    int    rheology;     /* 1 (or 0): elliptical yield curve
+                            (bulk_viscosity.f90:35-79, shear_viscosity.f90:36-60);
+                            2: Mohr-Coulomb (bulk_viscosity.f90:86-145,
+                            shear_viscosity.f90:67-124) */
+    int    skf;          /* 6 (or 0): Wendland C6 (kernel.f90:471-520,
+                            boundary_kernel.f90:438-504); 4: Wendland C2
+                            (kernel.f90:359-408, boundary_kernel.f90:290-357);
+                            5: Wendland C4 (kernel.f90:415-464,
+                            boundary_kernel.f90:364-431) */
+    int    int_algorithm; /* 2 (or 0): predictor_corrector
+                            (time_integration.f90:155-282); 1: euler_scheme
+                            (:86-148); 3: leapfrog_scheme (:289-391) */
 } orc_params;
 
 orc_ctx *orc_create(void);
@@ -72,7 +84,7 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n,
                     double *sig3);
 
 /* nsteps of time_integration's loop (time_integration.f90:38-73) with the
- * predictor_corrector scheme (:155-282).  State arrays are updated in place;
+ * scheme prm->int_algorithm selects (predictor_corrector by default).  State arrays are updated in place;
  * dt, current_time and itimestep advance as in the reference.  The
  * diagnostic outputs (eps3, zeta, eta, p, sig3) hold the last single_step's
  * values and may be NULL. */

@@ -30,6 +30,9 @@ subroutine single_step
         cfg%den_alg = den_alg
         cfg%pa_sph = pa_sph
         cfg%eos_alg = eos_alg
+        cfg%rheology = rheology
+        cfg%skf = skf
+        cfg%int_algorithm = int_algorithm
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

@@ -63,6 +63,9 @@ class Scenario:
     den_alg: int = 3         # 3 diagnostic density, 1 summation density
     pa_sph: int = 1          # 1 stress_1: sigma/rho^2 sums; 2 stress_2: over rho_i rho_j
     eos_alg: int = 3         # 3 Kreyscher, 1 Hibler P, 2 Gutfraind pressure
+    rheology: int = 1        # 1 elliptical yield curve, 2 Mohr-Coulomb
+    skf: int = 6             # 6 Wendland C6, 4 Wendland C2, 5 Wendland C4
+    int_algorithm: int = 2   # 2 predictor-corrector, 1 Euler, 3 leap-frog
     dt: float = 0.0
     time: float = 0.0
     name: str = ""

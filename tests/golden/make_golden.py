@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos]
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos] [switches]
 """
 from __future__ import annotations
 
@@ -247,10 +247,57 @@ def pressures():
           integration_time=t_end, eos_alg=2)
 
 
+def _pc10_edge(tag, harness, **switches):
+    """Ten steps of the 3,600-particle ice edge (reference wind) with the
+    harness built for one switch value."""
+    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
+    sc.wind_grid = None
+    sc.wind = S.REFERENCE_WIND
+    for k, v in switches.items():
+        setattr(sc, k, v)
+    K = 10
+    probe = sc.copy()
+    o = capi.Oracle()
+    times = []
+    for k in range(K):
+        o.run_pc(probe, 1, itimestep=k)
+        times.append(probe.time)
+    t_end = 0.5 * (times[-2] + times[-1])
+    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end, harness=harness)
+    assert r["itimestep"] == K, r["itimestep"]
+    _save(f"edge3600_pc10_{tag}.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
+          integration_time=t_end, **switches)
+
+
+def switches():
+    """10. Mohr-Coulomb rheology (rheology = 2), the Wendland C2 and C4
+    kernels (skf = 4, 5) and the Euler and leap-frog schemes
+    (int_algorithm = 1, 3), each from the harness built with that PARAMETER:
+    one single_step of the closed box at t > 0 with its pairs' kernel values
+    (rheology and kernels), and ten steps of the ice edge (all five)."""
+    for tag, exe, sw in (("mc", refrun.HARNESS_MC, {"rheology": 2}),
+                         ("c2", refrun.HARNESS_C2, {"skf": 4}),
+                         ("c4", refrun.HARNESS_C4, {"skf": 5})):
+        sc = S.box(nside=50, spacing=2000.0, seed=1)
+        sc.vx = np.random.default_rng(101).normal(0.0, 0.1, size=(sc.n, 2))
+        sc.time = 10.0
+        for k, v in sw.items():
+            setattr(sc, k, v)
+        r = refrun.run(refrun.MODE_SINGLE_STEP, sc, harness=exe)
+        _save(f"box2500_single_step_{tag}.npz", **_inputs(sc),
+              **_outputs(r, ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p",
+                             "sigma", "w", "kernel_zero")),
+              out_nip=r["nip"], out_bnip=r["bnip"], **sw)
+        _pc10_edge(tag, exe, **sw)
+    _pc10_edge("euler", refrun.HARNESS_EULER, int_algorithm=1)
+    _pc10_edge("leap", refrun.HARNESS_LEAP, int_algorithm=3)
+
+
 if __name__ == "__main__":
     # no argument: every fixture; otherwise the named groups ("main", "den1",
-    # "pa2", "eos")
-    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos"]
+    # "pa2", "eos", "switches")
+    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches"]
     if "main" in todo:
         main()
     if "den1" in todo:
@@ -260,3 +307,5 @@ if __name__ == "__main__":
         stress_2()
     if "eos" in todo:
         pressures()
+    if "switches" in todo:
+        switches()

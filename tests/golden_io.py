@@ -25,6 +25,9 @@ def scenario(g, prefix="in_") -> Scenario:
     sc.den_alg = int(g.get("den_alg", 3))
     sc.pa_sph = int(g.get("pa_sph", 1))
     sc.eos_alg = int(g.get("eos_alg", 3))
+    sc.rheology = int(g.get("rheology", 1))
+    sc.skf = int(g.get("skf", 6))
+    sc.int_algorithm = int(g.get("int_algorithm", 2))
     return sc
 
 

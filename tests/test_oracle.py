@@ -67,9 +67,36 @@ def test_pressure_switch_single_step_bit_exact_vs_reference(eos):
         assert _same(out["p"], P)
 
 
+@pytest.mark.parametrize("tag", ["mc", "c2", "c4"])
+def test_rheology_and_kernel_switches_single_step_bit_exact_vs_reference(tag):
+    """Mohr-Coulomb rheology (rheology = 2, bulk_viscosity.f90:86-145,
+    shear_viscosity.f90:67-124) and the Wendland C2 / C4 kernels (skf = 4, 5,
+    kernel.f90:359-464, boundary_kernel.f90:290-431), against the reference
+    built with each PARAMETER, kernel values included.  The reference's MC
+    caps zeta_max and eta_max are never assigned (data_variables.f90:153-155)
+    and hold zero in the compiled reference, so its viscosities are zero."""
+    g = load(f"box2500_single_step_{tag}.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    out = o.single_step(sc)
+    kv = o.kernel_values(sc.n)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    assert _same(kv["w"], g["out_w"]) and _same(kv["kernel_zero"], g["out_kernel_zero"])
+    if tag == "mc":
+        assert not np.any(g["out_zeta"]) and not np.any(g["out_eta"])
+    # the switch changes the result
+    default = scenario(g)
+    default.rheology, default.skf = 1, 6
+    assert not _same(capi.Oracle().single_step(default)["dvxdt"], g["out_dvxdt"])
+
+
 @pytest.mark.parametrize("name", ["box2500_pc20.npz", "edge3600_pc10.npz",
                                   "edge3600_pc10_den1.npz", "edge3600_pc10_pa2.npz",
-                                  "edge3600_pc10_eos2.npz"])
+                                  "edge3600_pc10_eos2.npz", "edge3600_pc10_mc.npz",
+                                  "edge3600_pc10_c2.npz", "edge3600_pc10_c4.npz",
+                                  "edge3600_pc10_euler.npz", "edge3600_pc10_leap.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
@@ -210,6 +237,90 @@ def test_kernel_gradient_is_the_derivative_of_wendland_c6():
         # divergence: div_1 = -m2 (dv . gradW), dhdt = -h div / rho
         div = -m2 * dWdx
         assert abs(out["dhdt"][0] / (-(sc.h[0] * div / sc.rho[0])) - 1) < 1e-12
+
+
+# Wendland's C2 and C4 profiles in two dimensions (Wendland 1995) and the
+# factors the reference multiplies them by (kernel.f90:388, :444)
+_WENDLAND = {
+    4: (lambda q: (1 - q) ** 4 * (1 + 4 * q), lambda h: 3.0 / (2.0 * math.pi * h * h)),
+    5: (lambda q: (1 - q) ** 6 * (35 * q ** 2 + 18 * q + 3), lambda h: 3.0 / (math.pi * h * h)),
+}
+
+
+@pytest.mark.parametrize("skf", [4, 5])
+def test_wendland_c2_c4_value_gradient_and_kernel_zero(skf):
+    """For one pair: w is the factor times Wendland's profile at q = r / h;
+    the gradient the strain rate carries is the r-derivative of that w (a
+    central difference of the oracle's own w); kernel_zero is w at r = 0."""
+    prof, fac = _WENDLAND[skf]
+    h = 2000.0
+    o = capi.Oracle()
+
+    def w_at(r):
+        sc = _two_particles(r, (1.0, 0.0), (1.0, 0.0), h)
+        sc.skf = skf
+        out = o.single_step(sc, current_time=0.0)
+        return sc, out, o.kernel_values(sc.n)
+
+    for r in (150.0, 700.0, 1234.5, 1800.0):
+        sc, out, kv = w_at(r)
+        assert abs(kv["w"][0] / (fac(h) * prof(r / h)) - 1) < 1e-13
+        m2, rho2 = sc.mass[1], sc.rho[1]
+        dWdx = -out["eps"][0, 0] * rho2 / m2           # dx / r = +1
+        e = 1e-3
+        dwdr = (w_at(r + e)[2]["w"][0] - w_at(r - e)[2]["w"][0]) / (2 * e)
+        assert abs(dWdx / dwdr - 1) < 1e-6, (r, dWdx, dwdr)
+    sc, out, kv = w_at(1e-6)
+    kz = kv["kernel_zero"].max()        # set for the pair's first particle only
+    assert abs(kz / (fac(h) * prof(0.0)) - 1) < 1e-13
+    assert abs(kv["w"][0] / kz - 1) < 1e-8
+
+
+def test_mohr_coulomb_principal_rates_and_zero_caps():
+    """MC_bulk_viscosity / MC_shear_viscosity with the caps the compiled
+    reference holds (zeta_max = eta_max = 0): the viscosities are zero, so the
+    Kreyscher pressure and the stress vanish; with the Hibler pressure the
+    stress is the isotropic -P / 2.  The principal rates e1 + e2 = trace are
+    checked on a state with shear."""
+    sc = S.box(nside=12, spacing=1000.0, seed=9, hsml_factor=1.6)
+    rng = np.random.default_rng(4)
+    sc.vx = rng.normal(0.0, 0.1, size=(sc.n, 2))
+    sc.time = 0.0
+    sc.rheology = 2
+    out = capi.Oracle().single_step(sc)
+    assert np.any(out["eps"][:, 1] != 0)
+    assert not np.any(out["zeta"]) and not np.any(out["eta"])
+    assert not np.any(out["p"]) and not np.any(out["sigma"])
+    sc.eos_alg = 1
+    out = capi.Oracle().single_step(sc)
+    P = 2.75e4 * sc.h * np.exp(-(20.0 * (1 - sc.A)))
+    assert _same(out["p"], P)
+    assert np.allclose(out["sigma"][:, 0], -P / 2, rtol=0, atol=0)
+    assert not np.any(out["sigma"][:, 1])
+
+
+def test_euler_and_leapfrog_orders():
+    """One free particle under wind and water drag (no neighbours, no
+    boundary): Euler converges at first order, leap-frog and the
+    predictor-corrector at second order, to the same velocity."""
+    def run(scheme, nsteps, T=2000.0):
+        sc = _two_particles(1.0e6, (1.0, 0.0), (0.0, 0.0))
+        sc.x = sc.x[:1].copy(); sc.vx = np.zeros((1, 2))
+        for k in ("mass", "rho", "hsml", "h", "A"):
+            setattr(sc, k, getattr(sc, k)[:1].copy())
+        sc.tse = 0
+        sc.dt = T / nsteps
+        sc.time = 1.0
+        sc.wind = (8.0, -3.0)
+        sc.int_algorithm = scheme
+        capi.Oracle().run_pc(sc, nsteps)
+        return sc.vx[0].copy()
+
+    ref = run(2, 4096)
+    for scheme, order in ((1, 1), (3, 2), (2, 2)):
+        e1 = np.abs(run(scheme, 32) - ref).max()
+        e2 = np.abs(run(scheme, 64) - ref).max()
+        assert abs(math.log2(e1 / e2) - order) < 0.3, (scheme, e1, e2)
 
 
 def test_uniform_velocity_gives_zero_strain_and_stress():
