@@ -44,16 +44,20 @@ extern "C" {
 #define SIMP_CONFIG_VERSION 1
 
 /* Runtime form of the compile-time switches of modules/globals.f90:58-187
- * that reach the hot path.  Only the reference's active values are
- * implemented; any other value is rejected with SIMP_E_UNSUPPORTED. */
+ * that reach the hot path.  The values each field lists are implemented;
+ * any other value is rejected with SIMP_E_UNSUPPORTED. */
 typedef struct simp_config {
     int32_t version;        /* SIMP_CONFIG_VERSION                              */
     int32_t precision;      /* SIMP_FP64 or SIMP_MIXED                          */
-    int32_t int_algorithm;  /* 2: predictor-corrector (globals.f90:64)          */
+    int32_t int_algorithm;  /* 2: predictor-corrector (globals.f90:64),         */
+                            /* time_integration.f90:155-282; 1: Euler (:86-148) */
+                            /* 3: leap-frog (:289-391); batches run 2           */
     int32_t nnps_algorithm; /* 2: bucket search (globals.f90:73)                */
     int32_t sle;            /* 2: hsml = 3 sqrt(m/rho) (globals.f90:81)         */
     int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89)            */
-    int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96)       */
+    int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96);      */
+                            /* 2: Mohr-Coulomb (bulk_viscosity.f90:86-145,      */
+                            /* shear_viscosity.f90:67-124)                      */
     int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119)     */
     int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127),        */
                             /* eos.f90:141-147; 1: Hibler P (eos.f90:59);     */
@@ -61,7 +65,9 @@ typedef struct simp_config {
     int32_t pa_sph;         /* 1: sigma_i/rho_i^2 + sigma_j/rho_j^2 (:134),     */
                             /* internal_stress.f90:105-136 (stress_1);         */
                             /* 2: (sigma_i + sigma_j)/(rho_i rho_j), :189-216  */
-    int32_t skf;            /* 6: Wendland C6 (globals.f90:147)                 */
+    int32_t skf;            /* 6: Wendland C6 (globals.f90:147); 4: Wendland    */
+                            /* C2, 5: Wendland C4 (kernel.f90:359-464,          */
+                            /* boundary_kernel.f90:290-431)                     */
     int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155);    */
                             /* 1: summation density (density.f90:41-100)        */
     int32_t wind_mode;      /* SIMP_WIND_UNIFORM or SIMP_WIND_GRID              */
@@ -295,6 +301,8 @@ typedef struct simp_instance_desc {
     int32_t pa_sph;                      /* 0 or 1: stress_1; 2: stress_2      */
     int32_t eos_alg;                     /* 0 or 3: Kreyscher; 1: Hibler; 2:   */
                                          /* Gutfraind                          */
+    int32_t rheology;                    /* 0 or 1: elliptical; 2: Mohr-Coulomb*/
+    int32_t skf;                         /* 0 or 6: Wendland C6; 4: C2; 5: C4  */
     int32_t reserved;
 } simp_instance_desc;
 

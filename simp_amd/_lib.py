@@ -66,7 +66,7 @@ class SimpInstanceDesc(C.Structure):
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p),
                 ("den_alg", C.c_int32), ("pa_sph", C.c_int32), ("eos_alg", C.c_int32),
-                ("reserved2", C.c_int32)]
+                ("rheology", C.c_int32), ("skf", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class SimpError(RuntimeError):

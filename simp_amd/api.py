@@ -28,7 +28,8 @@ from ._lib import SimpConfig, check, lib, ptr
 
 def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(np.float32(-5.3))),
                 current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False,
-                den_alg: int = 3, pa_sph: int = 1, eos_alg: int = 3) -> SimpConfig:
+                den_alg: int = 3, pa_sph: int = 1, eos_alg: int = 3, rheology: int = 1,
+                skf: int = 6, int_algorithm: int = 2) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
@@ -36,6 +37,9 @@ def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(
     cfg.den_alg = den_alg
     cfg.pa_sph = pa_sph
     cfg.eos_alg = eos_alg
+    cfg.rheology = rheology
+    cfg.skf = skf
+    cfg.int_algorithm = int_algorithm
     cfg.wind[0], cfg.wind[1] = wind
     cfg.current[0], cfg.current[1] = current
     cfg.max_hsml = max_hsml
@@ -69,9 +73,9 @@ class Simp:
     def __init__(self, precision: int = _lib.SIMP_FP64, tse: int = 2,
                  wind=(0.0, float(np.float32(-5.3))), current=(0.0, 0.0),
                  max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3, pa_sph: int = 1,
-                 eos_alg: int = 3):
+                 eos_alg: int = 3, rheology: int = 1, skf: int = 6, int_algorithm: int = 2):
         self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
-                               den_alg, pa_sph, eos_alg)
+                               den_alg, pa_sph, eos_alg, rheology, skf, int_algorithm)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -87,8 +91,8 @@ class Simp:
     def for_scenario(cls, sc, precision: int = _lib.SIMP_FP64) -> "Simp":
         return cls(precision=precision, tse=sc.tse, wind=sc.wind, current=sc.current,
                    max_hsml=sc.max_hsml, wind_grid=sc.wind_grid,
-                   den_alg=getattr(sc, "den_alg", 3), pa_sph=getattr(sc, "pa_sph", 1),
-                   eos_alg=getattr(sc, "eos_alg", 3))
+                   den_alg=sc.den_alg, pa_sph=sc.pa_sph, eos_alg=sc.eos_alg,
+                   rheology=sc.rheology, skf=sc.skf, int_algorithm=sc.int_algorithm)
 
     def close(self):
         if getattr(self, "_h", None):

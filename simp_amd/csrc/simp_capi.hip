@@ -124,12 +124,13 @@ const char *simp_strerror(int s) {
 static int validate(const simp_config *c) {
   if (!c || c->version != SIMP_CONFIG_VERSION) return SIMP_E_ARG;
   if (c->precision != SIMP_FP64 && c->precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
-  if (c->int_algorithm != 2 || c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
+  if (c->int_algorithm < 1 || c->int_algorithm > 3) return SIMP_E_UNSUPPORTED;
+  if (c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
   if (c->tse != 0 && c->tse != 2) return SIMP_E_UNSUPPORTED;
-  if (c->rheology != 1 || c->bfa != 1) return SIMP_E_UNSUPPORTED;
+  if ((c->rheology != 1 && c->rheology != 2) || c->bfa != 1) return SIMP_E_UNSUPPORTED;
   if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
-  if (c->skf != 6 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
+  if (c->skf < 4 || c->skf > 6 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
   if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   return SIMP_OK;
 }
@@ -155,6 +156,7 @@ int simp_create(const simp_config *cfg, simp_handle **out) {
     delete h;
     return SIMP_E_HIP;
   }
+  h->eng->set_scheme(cfg->int_algorithm);
   *out = h;
   return SIMP_OK;
 }
@@ -189,6 +191,8 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
   in.den_alg = c.den_alg;
   in.pa_sph = c.pa_sph;
   in.eos_alg = c.eos_alg;
+  in.rheology = c.rheology;
+  in.skf = c.skf;
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
     in.g_nx = h->g_nx; in.g_ny = h->g_ny;
@@ -535,6 +539,8 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.den_alg != 0 && d.den_alg != 1 && d.den_alg != 3) return SIMP_E_UNSUPPORTED;
   if (d.pa_sph != 0 && d.pa_sph != 1 && d.pa_sph != 2) return SIMP_E_UNSUPPORTED;
   if (d.eos_alg < 0 || d.eos_alg > 3) return SIMP_E_UNSUPPORTED;
+  if (d.rheology < 0 || d.rheology > 2) return SIMP_E_UNSUPPORTED;
+  if (d.skf != 0 && (d.skf < 4 || d.skf > 6)) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))
@@ -548,6 +554,8 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.den_alg = d.den_alg == 1 ? 1 : 3;
   in.pa_sph = d.pa_sph == 2 ? 2 : 1;
   in.eos_alg = d.eos_alg == 0 ? 3 : d.eos_alg;
+  in.rheology = d.rheology == 2 ? 2 : 1;
+  in.skf = d.skf == 0 ? 6 : d.skf;
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
   in.current[0] = d.current[0]; in.current[1] = d.current[1];

@@ -55,6 +55,8 @@ struct InstMeta {
   int den_alg;           // 3: rho = h rho_ice; 1: summation density
   int pa_sph;            // 1: stress_1; 2: stress_2
   int eos_alg;           // 1: Hibler P; 2: Gutfraind; 3: Kreyscher
+  int rheology;          // 1: elliptical yield curve; 2: Mohr-Coulomb
+  int skf;               // 6: Wendland C6; 4: C2; 5: C4
   int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;

@@ -25,6 +25,8 @@ struct InstanceInput {
   int den_alg = 3;       // 3: diagnostic density; 1: summation density
   int pa_sph = 1;        // 1: stress_1 (sigma / rho^2); 2: stress_2 (over rho_i rho_j)
   int eos_alg = 3;       // 1: Hibler P; 2: Gutfraind; 3: Kreyscher replacement pressure
+  int rheology = 1;      // 1: elliptical yield curve; 2: Mohr-Coulomb
+  int skf = 6;           // 6: Wendland C6; 4: Wendland C2; 5: Wendland C4
   // optional wind grid
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
@@ -134,6 +136,9 @@ class Engine {
   // instance k's evaluations since its upload, and of them those that
   // rebuilt the cell lists / the candidate lists
   int search_stats(int k, long long *evals, long long *cell, long long *cand);
+  // time integration of steps(): 2 predictor_corrector, 1 euler_scheme,
+  // 3 leapfrog_scheme (time_integration.f90:155-282, 86-148, 289-391)
+  void set_scheme(int int_algorithm);
   void set_diag(bool on) { diag_ = on; }
   bool diag() const { return diag_; }
   hipStream_t stream() const { return st_; }
@@ -172,6 +177,7 @@ class Engine {
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   bool force_search_ = false;
+  int scheme_ = 2;
   bool arena_ = false;                // d_ lies in caller memory (single_in_memory)
   double *arena_stage_ = nullptr;     // its staging area
 };

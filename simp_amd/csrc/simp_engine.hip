@@ -19,6 +19,12 @@ namespace simp {
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// an instance with a switch value beyond the reference's defaults: the
+// kernels then run their variants that read each instance's values
+static inline bool nondefault(const InstMeta &m) {
+  return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6;
+}
+
 // ------------------------------------------------------------ host sizing
 
 // Host copy of the bucket_search grid size (nnps.f90:211-224) for sizing.
@@ -300,6 +306,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.den_alg = in.den_alg;
     m.pa_sph = in.pa_sph;
     m.eos_alg = in.eos_alg;
+    m.rheology = in.rheology;
+    m.skf = in.skf;
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -528,9 +536,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // kernels with the switch values beyond the reference's defaults are
   // launched only when some instance uses one (each instance then follows
   // its own values)
-  any_xsw_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) {
-    return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3;
-  });
+  any_xsw_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return nondefault(m); });
   if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
   if (ninst > 0) {
     // grid of the next evaluation of every instance
@@ -576,7 +582,8 @@ int Engine::reload(const InstanceInput &in, bool grid_same) {
   if (meta_.size() != 1 || d_.ninst != 1 || blk_host_.empty()) return 1;
   const InstMeta &o = meta_[0];
   if (in.n != o.n || in.nb != o.nb || in.tse != o.tse || in.wind_mode != o.wind_mode ||
-      in.den_alg != o.den_alg || in.pa_sph != o.pa_sph || in.eos_alg != o.eos_alg)
+      in.den_alg != o.den_alg || in.pa_sph != o.pa_sph || in.eos_alg != o.eos_alg ||
+      in.rheology != o.rheology || in.skf != o.skf)
     return 1;
   if (in.wind_mode == 1 && (!grid_same || in.g_nx != o.g_nx || in.g_ny != o.g_ny)) return 1;
   if (!in.x || !in.vx || !in.mass || !in.rho || !in.hsml || !in.hth || !in.A) return -1;
@@ -788,6 +795,8 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   m.den_alg = in.den_alg;
   m.pa_sph = in.pa_sph;
   m.eos_alg = in.eos_alg;
+  m.rheology = in.rheology;
+  m.skf = in.skf;
   if (in.wind_mode == 1) {
     m.g_nx = in.g_nx; m.g_ny = in.g_ny;
     m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -872,7 +881,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   meta_.assign(1, m);
   ids_.assign(1, next_id_++);
   any_boundary_ = in.nb > 0;
-  any_xsw_ = m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3;
+  any_xsw_ = nondefault(m);
   // grid of the first evaluation
   launch_partial(d_, 0, st_);
   launch_finalize(d_, 0, st_);
@@ -944,10 +953,24 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
 }
 
 void Engine::enqueue_step() {
+  if (scheme_ == 1) {
+    // euler_scheme: one evaluation of the step's state, then the update
+    enqueue_eval(0, 3, diag_);
+    L("euler", [&] { launch_euler(d_, st_); });
+    L("finalize", [&] { launch_finalize(d_, 1, st_); });
+    return;
+  }
+  // predictor_corrector and leapfrog_scheme share the half step
   enqueue_eval(0, 0, false);
   L("finalize", [&] { launch_finalize(d_, 0, st_); });
-  enqueue_eval(1, 1, diag_);
+  enqueue_eval(1, scheme_ == 3 ? 4 : 1, diag_);
   L("finalize", [&] { launch_finalize(d_, 1, st_); });
+}
+
+void Engine::set_scheme(int int_algorithm) {
+  if (scheme_ == int_algorithm) return;
+  scheme_ = int_algorithm;
+  graph_drop();
 }
 
 int Engine::eval_single(bool want_diag) {

@@ -24,7 +24,10 @@ void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);
 void launch_ranks(const DevState &d, int buf, hipStream_t st);
+// phase: 0 predictor, 1 corrector, 2 single evaluation, 3 Euler evaluation
+// (then launch_euler), 4 second leap-frog stage
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st);
+void launch_euler(const DevState &d, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,

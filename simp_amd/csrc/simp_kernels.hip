@@ -901,6 +901,113 @@ __device__ __forceinline__ float c6_gradr_f(float d2, float mh) {
   return (-22.0f * kC6f) * (ih2 * ih2) * o7 * poly;
 }
 
+// The Wendland C2 and C4 kernels (skf = 4, 5; kernel.f90:387-396,
+// :443-452; scale_h = 1) beside C6, in the reference's evaluation order:
+// the gradient factor g (dW/dx = g dx / r), the value w, kernel_zero (the
+// value at r = 0) and the FP32 gradient over the distance.
+constexpr double kC2 = 3.0 / (2.0 * kPi);    // 3. / (2. * pi)
+constexpr double kC4 = 3.0 / kPi;            // 3. / pi
+constexpr float kC2f = (float)(3.0 / (2.0 * kPi));
+constexpr float kC4f = (float)(3.0 / kPi);
+
+__device__ __forceinline__ double kern_grad(int skf, double r, double mh) {
+  if (skf == 4) {
+    const double q = r / mh * 1.0, ih = 1.0 / mh;
+    const double factor = kC2 * (ih * ih);
+    return factor * (20. * (ipow<4>(q) - 3 * ipow<3>(q) + 3 * ipow<2>(q) - q)) / mh;
+  }
+  if (skf == 5) {
+    const double q = r / mh * 1.0, ih = 1.0 / mh;
+    const double factor = kC4 * (ih * ih);
+    return factor * (-(56 * q * (5 * q + 1) * ipow<5>(1 - q))) / mh;
+  }
+  return c6_grad(r, mh);
+}
+
+__device__ __forceinline__ double kern_w(int skf, double r, double mh) {
+  if (skf == 4) {
+    const double q = r / mh * 1.0, ih = 1.0 / mh;
+    return kC2 * (ih * ih) * (ipow<4>(1. - q) * (1 + 4 * q));
+  }
+  if (skf == 5) {
+    const double q = r / mh * 1.0, ih = 1.0 / mh;
+    return kC4 * (ih * ih) * (ipow<6>(1 - q) * (35 * ipow<2>(q) + 18 * q + 3));
+  }
+  return c6_w(r, mh);
+}
+
+__device__ __forceinline__ double kern_zero(int skf, double mh) {
+  const double ih = 1.0 / mh;
+  if (skf == 4) return kC2 * (ih * ih);
+  if (skf == 5) return kC4 * (ih * ih) * 3;
+  return c6_factor(mh);
+}
+
+__device__ __forceinline__ float kern_gradr_f(int skf, float d2, float mh) {
+  if (skf == 4 || skf == 5) {
+    const float ih = __builtin_amdgcn_rcpf(mh);
+    const float q = __builtin_amdgcn_sqrtf(d2) * ih;
+    const float ih2 = ih * ih;
+    const float omq = 1.0f - q, o2 = omq * omq;
+    if (skf == 4) return (-20.0f * kC2f) * (ih2 * ih2) * (o2 * omq);
+    return (-56.0f * kC4f) * (ih2 * ih2) * (o2 * o2 * omq) * __builtin_fmaf(5.0f, q, 1.0f);
+  }
+  return c6_gradr_f(d2, mh);
+}
+
+// The boundary kernels' values (boundary_kernel.f90): b_Wendland_C6 reads
+// hsml_ij at the boundary-pair index (:491), C2 and C4 scale with b_length
+// (:343-346, :417-420).
+__device__ __forceinline__ double bkern_w(int skf, double br, double hk) {
+  const double q = br / kBLength * 1.0;
+  const double ih = (skf == 4 || skf == 5) ? 1.0 / kBLength : 1.0 / hk;
+  if (skf == 4) return kC2 * (ih * ih) * (ipow<4>(1. - q) * (1 + 4 * q));
+  if (skf == 5) return kC4 * (ih * ih) * (ipow<6>(1 - q) * (35 * ipow<2>(q) + 18 * q + 3));
+  const double omq = 1 - q;
+  return kC6 * (ih * ih) * ipow<8>(omq) * (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
+}
+
+__device__ __forceinline__ float bkern_w_f(int skf, float q, float hk) {
+  const float omq = 1.0f - q, o2 = omq * omq, o4 = o2 * o2;
+  if (skf == 4 || skf == 5) {
+    const float ih = (float)(1.0 / kBLength);
+    if (skf == 4) return kC2f * (ih * ih) * o4 * __builtin_fmaf(4.0f, q, 1.0f);
+    return kC4f * (ih * ih) * (o4 * o2) *
+           __builtin_fmaf(__builtin_fmaf(35.0f, q, 18.0f), q, 3.0f);
+  }
+  const float ih = __builtin_amdgcn_rcpf(hk);
+  const float poly = __builtin_fmaf(
+      __builtin_fmaf(__builtin_fmaf(32.0f, q, 25.0f), q, 8.0f), q, 1.0f);
+  return kC6f * (ih * ih) * (o4 * o4) * poly;
+}
+
+// MC_bulk_viscosity and MC_shear_viscosity (bulk_viscosity.f90:113-143,
+// shear_viscosity.f90:91-122): principal strain rates, zeta from their sum
+// and eta from their difference, each floored at 1e-18 and capped at
+// zeta_max P, eta_max P.  The reference declares zeta_max and eta_max but
+// never assigns them (data_variables.f90:153-155); its compiled module
+// storage holds zero, and so do these caps.  mu = TAN(45 pi / 180.) as the
+// reference build folds it, 1 - 2^-53.
+constexpr double kMuMC = 0x1.fffffffffffffp-1;
+constexpr double kZetaMax = 0.0, kEtaMax = 0.0, kZeroMC = 1.e-18;
+
+__device__ __forceinline__ void mc_viscosity(double e11, double e12, double e22, double P,
+                                             double &z, double &et) {
+  const double sq = sqrt((e11 - e22) * (e11 - e22) + 4 * (e12 * e12));
+  const double e1 = 0.5 * ((e11 + e22) + sq);
+  const double e2 = 0.5 * ((e11 + e22) - sq);
+  double eI = fabs(e1 + e2);
+  if (eI < kZeroMC) eI = kZeroMC;
+  double zz = 0.5 * P * (1 + kKT) / eI;
+  if (zz > kZetaMax * P) zz = kZetaMax * P;
+  double eII = e1 - e2;
+  if (fabs(eII) < kZeroMC) eII = kZeroMC;
+  double ee = kMuMC * (0.5 * P * (1 + kKT) - zz * (e1 + e2)) / eII;
+  if (ee > kEtaMax * P) ee = kEtaMax * P;
+  z = zz;
+  et = ee;
+}
+
 // Compensated (TwoSum) FP32 accumulator.
 struct KSum {
   float s = 0.0f, c = 0.0f;
@@ -1071,16 +1178,17 @@ k_passA(DevState d, int buf, int want_diag) {
     return (MIXED && !sumd) ? in_support_nr(d2, mh) : in_support(d2, mh, r);
   };
   // kernel gradient, vector_divergence and vector_gradient of one pair
+  const int skf = XSW ? m.skf : 6;
   auto pair = [&](double dx, double dy, double r, double mh, double mq, double rq, double vxq,
                   double vyq) {
     const double dv0 = vxi - vxq, dv1 = vyi - vyq;
     if (sumd) {
-      const double w = c6_w(r, mh);
+      const double w = kern_w(skf, r, mh);
       den = den + mq * w / rq;
       num = num + mq * w;
     }
     if (!MIXED) {
-      const double gg = c6_grad(r, mh);
+      const double gg = kern_grad(skf, r, mh);
       const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
       const double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
       div = div - mq * (w1 + w4);
@@ -1090,7 +1198,7 @@ k_passA(DevState d, int buf, int want_diag) {
       g4 = g4 - mq * w4 / rq;
     } else {
       const float fx = (float)dx, fy = (float)dy;
-      const float gr = c6_gradr_f(fx * fx + fy * fy, (float)mh);
+      const float gr = kern_gradr_f(skf, fx * fx + fy * fy, (float)mh);
       const float dw0 = fx * gr, dw1 = fy * gr;
       const float a0 = (float)dv0, a1 = (float)dv1;
       const float w1 = a0 * dw0, w2 = a1 * dw0, w3 = a0 * dw1, w4 = a1 * dw1;
@@ -1211,7 +1319,7 @@ k_passA(DevState d, int buf, int want_diag) {
         const double mh = (hi + HS[t]) / 2.;
         double r;
         if (in_support(dx * dx + dy * dy, mh, r)) {
-          kzi = c6_factor(mh);
+          kzi = kern_zero(skf, mh);
           found = true;
         }
       }
@@ -1223,7 +1331,7 @@ k_passA(DevState d, int buf, int want_diag) {
           const double mh = (hi + HS[t]) / 2.;
           double r;
           if (in_support(dx * dx + dy * dy, mh, r)) {
-            kzi = c6_factor(mh);
+            kzi = kern_zero(skf, mh);
             found = true;
           }
         }
@@ -1361,8 +1469,9 @@ k_passA(DevState d, int buf, int want_diag) {
   const double delta = sqrt((e11 * e11 + e22 * e22) * (1 + em2) + 4 * em2 * (e12 * e12) +
                             2 * e11 * e22 * (1 - em2));
   const double dcl = fabs(delta) < kDeltaMin ? kDeltaMin : delta;
-  const double z = 0.5 * P * (1 + kKT) / dcl;
-  const double et = z / (kE * kE);
+  double z = 0.5 * P * (1 + kKT) / dcl;
+  double et = z / (kE * kE);
+  if (XSW && m.rheology == 2) mc_viscosity(e11, e12, e22, P, z, et);
   double pr;
   if (eos == 1) {
     pr = P;                                    // hibler_eos (eos.f90:59)
@@ -1653,7 +1762,9 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 }
 
 // phase: 0 = predictor (state S -> T), 1 = corrector (RHS on T, update S in
-// place), 2 = single evaluation (tendencies to d.tend)
+// place), 2 = single evaluation (tendencies to d.tend), 3 = Euler step (the
+// tendencies to d.tend, which k_euler applies), 4 = second stage of the
+// leap-frog step (RHS on T, update S in place)
 template <bool MIXED, bool XSW, int phase>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
@@ -1672,7 +1783,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   const bool ok = (int)threadIdx.x < be.nloc;
   const int self = be.pos + (ok ? (int)threadIdx.x : 0);   // padding lanes: a real slot
   const int p = self;
-  const int buf = phase == 1 ? 1 : 0;
+  const int buf = (phase == 1 || phase == 4) ? 1 : 0;
   // the evaluated state, by slot (= list position)
   const int cs = be.cs;
   const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
@@ -1699,7 +1810,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   for (int u = 0; u < kB; ++u) nx[u] = d.nbr[u * N + self];
   const double hthi = d.h[2 * cs + buf][p], Ai = d.A[2 * cs + buf][p];
   double x1 = 0., y1 = 0., h1 = 0., A1 = 0., u1 = 0., v1 = 0.;
-  if (phase == 1) {
+  if (phase == 1 || phase == 4) {
     x1 = d.x[2 * cs + 0][p]; y1 = d.y[2 * cs + 0][p];
     h1 = d.h[2 * cs + 0][p]; A1 = d.A[2 * cs + 0][p];
     u1 = d.vx[2 * cs + 0][p]; v1 = d.vy[2 * cs + 0][p];
@@ -1721,7 +1832,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // numerator and density (corrector)
     const bool sd = XSW && m.den_alg == 1;
     double dsi = 1.0, dnumi = 0.;
-    if (sd && phase != 2) dsi = d.dsum[self];
+    if (sd && phase != 2 && phase != 3) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
     // round 2: also the wind at the particle and the cell's run of boundary
     // entries, which the stores below would otherwise hold back
@@ -1777,6 +1888,26 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
       }
+    } else if (phase == 4) {
+      // leapfrog_scheme (time_integration.f90:361-391): thickness,
+      // position and concentration from the step-start state and this
+      // evaluation's tendencies (dxdt = the half-step velocity), density on
+      // this evaluation (diagnostic, or pass A's summation), Marquis_hsml
+      const double hn = h1 + dhdt * dt;
+      px_new = x1 + (u1 + vxi) * dt / 2;
+      py_new = y1 + (v1 + vyi) * dt / 2;
+      d.x[2 * cs + 0][p] = px_new;
+      d.y[2 * cs + 0][p] = py_new;
+      double An = A1 + dAdt * dt;
+      if (An > 1) An = 1;
+      d.h[2 * cs + 0][p] = hn;
+      d.A[2 * cs + 0][p] = An;
+      const double rn = sd ? dsi : hn * kRhoIce;
+      d.rho[2 * cs + 0][p] = rn;
+      double hs = sqrt(mi / rn) * kHsmlMult;
+      if (hs > m.max_hsml) hs = m.max_hsml;
+      d.hsml[2 * cs + 0][p] = hs;
+      hsml_new = hs;
     }
     const bool listed = nn <= kNbrMax;
     int sj[kB];
@@ -1794,14 +1925,15 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     KSum sx, sy;
     double den3 = 0.;
     if (sumd3) den3 = kzi * mi / dsi;
+    const int skf = XSW ? m.skf : 6;
     auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
-      const double w = c6_w(sqrt(dx * dx + dy * dy), mh);
+      const double w = kern_w(skf, sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
     };
     const bool pa2 = XSW && m.pa_sph == 2;
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
                            double q11, double q12, double q22) {
-      const double gg = c6_grad(r, mh);
+      const double gg = kern_grad(skf, r, mh);
       const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
       if (pa2) {
         // stress_2 (internal_stress.f90:189-216); as the pair's second
@@ -1828,7 +1960,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // mixed: q = the neighbour's record {sig / rho^2, mass}
     auto stress_pair_f = [&](double dx, double dy, double mh, float4 q) {
       const float fx = (float)dx, fy = (float)dy;
-      const float gr = c6_gradr_f(fx * fx + fy * fy, (float)mh);
+      const float gr = kern_gradr_f(skf, fx * fx + fy * fy, (float)mh);
       const float dw0 = fx * gr, dw1 = fy * gr;
       const float t11 = q.x + s11f, t12 = q.y + s12f, t22 = q.z + s22f;
       sx.add(q.w * (t11 * dw0 + t12 * dw1));
@@ -1978,11 +2110,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             const float fx = (float)bdx, fy = (float)bdy;
             const float br2 = fx * fx + fy * fy;
             const float q = __builtin_amdgcn_sqrtf(br2) * (float)(1.0 / kBLength);
-            const float ih = __builtin_amdgcn_rcpf((float)hk[j]);
-            const float omq = 1.0f - q, o2 = omq * omq, o4 = o2 * o2;
-            const float poly = __builtin_fmaf(
-                __builtin_fmaf(__builtin_fmaf(32.0f, q, 25.0f), q, 8.0f), q, 1.0f);
-            const float bw = kC6f * (ih * ih) * (o4 * o4) * poly;
+            const float bw = bkern_w_f(skf, q, (float)hk[j]);
             const float cf = (float)(2.0 * kKappaN) * bw * (float)mb[j] *
                              __builtin_amdgcn_rcpf((float)(mi + mb[j]) * br2);
             fbx += cf * fx;
@@ -1990,12 +2118,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             continue;
           }
           const double br = sqrt(bdx * bdx + bdy * bdy);
-          const double q = br / kBLength * 1.0;
-          const double ih = 1.0 / hk[j];
-          const double factor = kC6 * (ih * ih);
-          const double omq = 1 - q;
-          const double bw = factor * ipow<8>(omq) *
-                            (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
+          const double bw = bkern_w(skf, br, hk[j]);
           const double den = (mi + mb[j]) * (br * br);
           const double F0 = kKappaN * bdx * bw * 2 * mb[j] / den;
           const double F1 = kKappaN * bdy * bw * 2 * mb[j] / den;
@@ -2058,6 +2181,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
       }
+    } else if (phase == 4) {
+      d.vx[2 * cs + 0][p] = u1 + ax * dt;
+      d.vy[2 * cs + 0][p] = v1 + ay * dt;
     } else {
       const int N = d.ntot;
       d.tend[p] = vxi;
@@ -2070,7 +2196,50 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   }
   // (with every lane of the wave: a wave holds one instance)
   if (phase == 2 || y.itimestep % kPrintStep == 0) stats_add(d.dyn[k], live, st_cnt, st_own);
-  if (phase != 2) block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
+  if (phase != 2 && phase != 3) block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
+}
+
+// euler_scheme's update (time_integration.f90:117-146) after the Euler
+// evaluation: the state advanced in place by dt with the tendencies pass B
+// wrote, the density of the new state (diagnostic, or pass A's summation
+// on this evaluation's kernel values and densities), Marquis_hsml, and the
+// block's partials for the next grid.  Nothing here reads a neighbour, so
+// the update is in place.
+__global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
+  int p = 0, k = 0;
+  const bool ok = particle_of(d, p, k);
+  const InstDyn &y = d.dyn[k];
+  const bool live = ok && !inst_off(y);
+  double xn = 0., yn = 0., hs = 1.0, sp = 0., dh = 0.;
+  if (live) {
+    const InstMeta &m = d.meta[k];
+    const int cs = d.blk[blockIdx.x].cs;
+    const size_t N = (size_t)d.ntot;
+    const double *T = d.tend;
+    const double dt = y.dt;
+    const double vx = d.vx[2 * cs][p], vy = d.vy[2 * cs][p];
+    xn = d.x[2 * cs][p] + T[p] * dt;
+    yn = d.y[2 * cs][p] + T[N + p] * dt;
+    const double un = vx + T[2 * N + p] * dt, vn = vy + T[3 * N + p] * dt;
+    const double hn = d.h[2 * cs][p] + T[4 * N + p] * dt;
+    double An = d.A[2 * cs][p] + T[5 * N + p] * dt;
+    if (An > 1) An = 1;
+    const double rn = m.den_alg == 1 ? d.dsum[p] : hn * kRhoIce;
+    hs = sqrt(d.mass[cs][p] / rn) * kHsmlMult;
+    if (hs > m.max_hsml) hs = m.max_hsml;
+    sp = sqrt(vx * vx + vy * vy);
+    dh = fabs(hs - d.hsml[2 * cs][p]);
+    d.x[2 * cs][p] = xn;
+    d.y[2 * cs][p] = yn;
+    d.vx[2 * cs][p] = un;
+    d.vy[2 * cs][p] = vn;
+    d.h[2 * cs][p] = hn;
+    d.A[2 * cs][p] = An;
+    d.rho[2 * cs][p] = rn;
+    d.hsml[2 * cs][p] = hs;
+  }
+  block_partials(d, k, live, xn, yn, hs, sp, dh);
 }
 
 // ------------------------------------------------------- pair-list export
@@ -2111,7 +2280,7 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
         pj[rank] = d.orig[cs][q] + 1;
         rij[rank] = r;
         hij[rank] = mh;
-        if (w) w[rank] = c6_w(r, mh);
+        if (w) w[rank] = kern_w(m.skf, r, mh);
       }
       ++rank;
     }
@@ -2292,8 +2461,15 @@ static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
     k_passB<MIXED, XSW, 0><<<g8(d.nblk), kTPB, 0, st>>>(d);
   else if (phase == 1)
     k_passB<MIXED, XSW, 1><<<g8(d.nblk), kTPB, 0, st>>>(d);
-  else
+  else if (phase == 2)
     k_passB<MIXED, XSW, 2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else if (phase == 3)
+    k_passB<MIXED, XSW, 3><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else
+    k_passB<MIXED, XSW, 4><<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
+void launch_euler(const DevState &d, hipStream_t st) {
+  k_euler<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st) {
   if (mixed && xsw)

@@ -35,6 +35,8 @@ def _desc(sc, nsteps: int, keep: list, itimestep: int = 0) -> SimpInstanceDesc:
     d.den_alg = getattr(sc, "den_alg", 3)
     d.pa_sph = getattr(sc, "pa_sph", 1)
     d.eos_alg = getattr(sc, "eos_alg", 3)
+    d.rheology = getattr(sc, "rheology", 1)
+    d.skf = getattr(sc, "skf", 6)
     d.wind[0], d.wind[1] = sc.wind
     d.current[0], d.current[1] = sc.current
     d.max_hsml = sc.max_hsml

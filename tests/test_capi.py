@@ -46,8 +46,9 @@ def test_config_defaults_are_the_reference_switches():
     assert cfg.wind[1] == float(np.float32(-5.3))
 
 
-@pytest.mark.parametrize("field,value", [("skf", 4), ("rheology", 2), ("tse", 1),
-                                         ("den_alg", 2), ("int_algorithm", 1),
+@pytest.mark.parametrize("field,value", [("skf", 3), ("skf", 7), ("rheology", 3), ("tse", 1),
+                                         ("den_alg", 2), ("int_algorithm", 0),
+                                         ("int_algorithm", 4),
                                          ("precision", 7), ("pa_sph", 3), ("eos_alg", 0),
                                          ("eos_alg", 4)])
 def test_unsupported_switches_are_rejected_before_any_device_work(field, value):

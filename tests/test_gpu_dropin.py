@@ -23,7 +23,16 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          # stress_2 (pa_sph = 2): the shim passes the module's pa_sph
          ("edge3600_pc10_pa2.npz", refrun.HARNESS_PA2, refrun.HARNESS_GPU_PA2),
          # Gutfraind pressure (eos_alg = 2)
-         ("edge3600_pc10_eos2.npz", refrun.HARNESS_EOS2, refrun.HARNESS_GPU_EOS2)]
+         ("edge3600_pc10_eos2.npz", refrun.HARNESS_EOS2, refrun.HARNESS_GPU_EOS2),
+         # Mohr-Coulomb rheology, Wendland C2 / C4 kernels: the shim passes the
+         # module's rheology and skf
+         ("edge3600_pc10_mc.npz", refrun.HARNESS_MC, refrun.HARNESS_GPU_MC),
+         ("edge3600_pc10_c2.npz", refrun.HARNESS_C2, refrun.HARNESS_GPU_C2),
+         ("edge3600_pc10_c4.npz", refrun.HARNESS_C4, refrun.HARNESS_GPU_C4),
+         # the reference's own Euler and leap-frog loops calling the GPU
+         # single_step
+         ("edge3600_pc10_euler.npz", refrun.HARNESS_EULER, refrun.HARNESS_GPU_EULER),
+         ("edge3600_pc10_leap.npz", refrun.HARNESS_LEAP, refrun.HARNESS_GPU_LEAP)]
 
 
 @pytest.mark.parametrize("name,cpu_exe,gpu_exe", CASES)
