@@ -67,7 +67,9 @@ typedef struct simp_config {
                             /* 2: (sigma_i + sigma_j)/(rho_i rho_j), :189-216  */
     int32_t skf;            /* 6: Wendland C6 (globals.f90:147); 4: Wendland    */
                             /* C2, 5: Wendland C4 (kernel.f90:359-464,          */
-                            /* boundary_kernel.f90:290-431)                     */
+                            /* boundary_kernel.f90:290-431); 1: quadratic,      */
+                            /* 2: Gaussian, 3: quintic spline, 7, 8: Marquis    */
+                            /* (kernel.f90:108-290, 527-633)                    */
     int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155);    */
                             /* 1: summation density (density.f90:41-100)        */
     int32_t wind_mode;      /* SIMP_WIND_UNIFORM or SIMP_WIND_GRID              */
@@ -302,7 +304,8 @@ typedef struct simp_instance_desc {
     int32_t eos_alg;                     /* 0 or 3: Kreyscher; 1: Hibler; 2:   */
                                          /* Gutfraind                          */
     int32_t rheology;                    /* 0 or 1: elliptical; 2: Mohr-Coulomb*/
-    int32_t skf;                         /* 0 or 6: Wendland C6; 4: C2; 5: C4  */
+    int32_t skf;                         /* 0 or 6: Wendland C6; 1-5, 7, 8 as  */
+                                         /* simp_config.skf                    */
     int32_t reserved;
 } simp_instance_desc;
 

@@ -114,6 +114,12 @@ c2objs=("${vobjs[@]}")
 variant c4 'parameter(skf = 6)' 'parameter(skf = 5)'
 C4="$OUT/c4"
 c4objs=("${vobjs[@]}")
+# the quadratic, Gaussian, quintic-spline and Marquis kernels (skf = 1, 2,
+# 3, 7, 8; kernel.f90:108-290, 527-633; boundary_kernel.f90:54-283)
+for k in 1 2 3 7 8; do
+    variant "k$k" 'parameter(skf = 6)' "parameter(skf = $k)"
+    eval "kobjs_$k=(\"\${vobjs[@]}\")"
+done
 # Euler and leap-frog time integration (int_algorithm = 1, 3;
 # time_integration.f90:86-148, 289-391)
 variant euler 'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 1)'
@@ -200,4 +206,7 @@ if [ -f "$LIB" ]; then
     gpu_harness "$C4/obj" "$C4/mod" "$OUT/simp_ref_harness_gpu_c4" "${c4objs[@]}"
     gpu_harness "$EU/obj" "$EU/mod" "$OUT/simp_ref_harness_gpu_euler" "${euobjs[@]}"
     gpu_harness "$LF/obj" "$LF/mod" "$OUT/simp_ref_harness_gpu_leap" "${lfobjs[@]}"
+    for k in 1 2 3 7 8; do
+        eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
+    done
 fi

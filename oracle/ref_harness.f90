@@ -55,7 +55,16 @@ program simp_ref_harness
 
     ntotal = n
     ntotalb = nb
-    scale_h = 1
+    ! the kernel's scale factor as initialize sets it (initialize.f90:207-217):
+    ! 1 for the Wendland kernels, 2 for the quadratic, 3 for the others
+    select case (skf)
+    case (4, 5, 6)
+        scale_h = 1
+    case (1)
+        scale_h = 2
+    case default
+        scale_h = 3
+    end select
     itimestep = 1
 
     ! the CSV files time_integration's output routine appends to

@@ -37,6 +37,10 @@ HARNESS_MC = os.path.join(HERE, "_ref", "simp_ref_harness_mc")
 HARNESS_C2 = os.path.join(HERE, "_ref", "simp_ref_harness_c2")
 HARNESS_C4 = os.path.join(HERE, "_ref", "simp_ref_harness_c4")
 HARNESS_EULER = os.path.join(HERE, "_ref", "simp_ref_harness_euler")
+#: the harnesses built with skf = 1, 2, 3, 7, 8
+HARNESS_SKF = {k: os.path.join(HERE, "_ref", f"simp_ref_harness_k{k}") for k in (1, 2, 3, 7, 8)}
+HARNESS_GPU_SKF = {k: os.path.join(HERE, "_ref", f"simp_ref_harness_gpu_k{k}")
+                   for k in (1, 2, 3, 7, 8)}
 HARNESS_LEAP = os.path.join(HERE, "_ref", "simp_ref_harness_leap")
 HARNESS_GPU_MC = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_mc")
 HARNESS_GPU_C2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_c2")

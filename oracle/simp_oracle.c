@@ -367,12 +367,80 @@ static int ensure_scratch(orc_ctx *c, int n) {
 }
 
 /* ---- single_step, single_step.f90:5-108 ------------------------------ */
+/* scale_h of each kernel (initialize.f90:208-217): 1 for the Wendland
+ * kernels, 2 for the quadratic, 3 for the Gaussian, quintic and Marquis
+ * kernels.  The search radius stays mhsml; q runs up to scale_h. */
+static int scale_h(int skf) {
+    if (skf == 1) return 2;
+    if (skf == 2 || skf == 3 || skf == 7 || skf == 8) return 3;
+    return 1;
+}
+
 /* One pair's kernel (kernel.f90): kernel_zero(cp_i), w(int_id) and the
  * gradient factor g with dwdx = g (dx / r) scale_h.  Fortran's x**k is
- * expanded as repeated products (ipow), default-REAL literals are exact. */
+ * expanded as repeated products (ipow), default-REAL literals are exact
+ * unless noted. */
 static void kernel_pair(int skf, double r, double hij, double *kz, double *w, double *g) {
-    double q = r / hij * 1.0;                       /* scale_h = 1 */
+    const int sh = scale_h(skf);
+    double q = r / hij * sh;
     double ih = 1.0 / hij;
+    if (skf == 1) {
+        /* Quadratic (kernel.f90:269-278) */
+        double t = sh / hij;
+        double factor = 2. / PI_ * (t * t);
+        *kz = factor * 3. / 4.;
+        *w = factor * (3. / 16. * ipow(q, 2) - 3. / 4. * q + 3. / 4.);
+        *g = factor * (3. / 8. * q - 3. / 4.) / hij;
+        return;
+    }
+    if (skf == 2) {
+        /* Gaussian (kernel.f90:136-145) */
+        double t = sh / hij;
+        double factor = 1. / PI_ * (t * t);
+        *kz = factor;
+        *w = factor * exp(-ipow(q, 2));
+        *g = factor * (-(2 * q * exp(-ipow(q, 2)))) / hij;
+        return;
+    }
+    if (skf == 3) {
+        /* quintic_spline (kernel.f90:193-221), three pieces of q */
+        double t = sh / hij;
+        double factor = 7.e0 / (478.e0 * PI_) * (t * t);
+        *kz = factor * 66;        /* (3)**5 - 6 * (2)**5 + 15 */
+        if (q >= 0. && q <= 1.) {
+            *w = factor * (ipow(3 - q, 5) - 6 * ipow(2 - q, 5) + 15 * ipow(1 - q, 5));
+            *g = factor * (-(5 * ipow(3 - q, 4)) + 30 * ipow(2 - q, 4) -
+                           75 * ipow(1 - q, 4)) / hij;
+        } else if (q > 1. && q <= 2.) {
+            *w = factor * (ipow(3 - q, 5) - 6 * ipow(2 - q, 5));
+            *g = factor * (-(5 * ipow(3 - q, 4)) + 30 * ipow(2 - q, 4)) / hij;
+        } else {
+            *w = factor * ipow(3 - q, 5);
+            *g = factor * (-(5 * ipow(3 - q, 4))) / hij;
+        }
+        return;
+    }
+    if (skf == 7) {
+        /* Marquis (kernel.f90:555-564): a smoothed step at q = 1 */
+        double t = sh / hij;
+        double factor = 1. / PI_ * (t * t);
+        *kz = factor * (0.5 - 0.5 * tanh(0.));
+        *w = factor * (0.5 - 0.5 * tanh(300 * (q - 1)));
+        double ch = cosh(300 * (-1 + q));
+        *g = factor * (-(150 * (1.0 / (ch * ch)))) / hij;
+        return;
+    }
+    if (skf == 8) {
+        /* Marquis2 (kernel.f90:604-621): C1 = 10, C2 = 50; 34.1558 and
+         * log(2.) are default-REAL */
+        const double C1 = 10., C2 = 50.;
+        const double ln2 = (double)logf(2.0f);
+        double factor = 1. / ((double)34.1558f * PI_ * ipow(hij, 2));
+        *kz = factor * C2 * (log(cosh(C1)) / C1 + 1 + ln2 / C1);
+        *w = factor * C2 * (log(cosh(C1 * (1 - q))) / C1 - q + 1 + ln2 / C1);
+        *g = factor * (-(C2 * tanh(C1 * (1 - q))) - 1) / hij;
+        return;
+    }
     if (skf == 4) {
         /* Wendland_C2 (kernel.f90:387-396) */
         double factor = 3. / (2. * PI_) * (ih * ih);
@@ -399,7 +467,30 @@ static void kernel_pair(int skf, double r, double hij, double *kz, double *w, do
  * b_Wendland_C6 reads hsml_ij at the boundary-pair index (:491), C2 and C4
  * use b_length (:343, :417). */
 static double boundary_w(int skf, double br, double hk) {
-    double q = br / B_LENGTH * 1.0;
+    const int sh = scale_h(skf);
+    double q = br / B_LENGTH * sh;
+    if (skf == 1) {
+        /* b_Quadratic (boundary_kernel.f90:106-110) */
+        double t = sh / B_LENGTH;
+        double factor = 2. / PI_ * (t * t);
+        return factor * (3. / 16. * ipow(q, 2) - 3. / 4. * q + 3. / 4.);
+    }
+    if (skf == 2 || skf == 7 || skf == 8) {
+        /* b_Gaussian (boundary_kernel.f90:177-181), also for both Marquis
+         * kernels (:38-43) */
+        double t = sh / B_LENGTH;
+        double factor = 1. / PI_ * (t * t);
+        return factor * exp(-ipow(q, 2));
+    }
+    if (skf == 3) {
+        /* b_quintic_spline (boundary_kernel.f90:250-272) */
+        double t = sh / B_LENGTH;
+        double factor = 15. / (478. * PI_) * (t * t);
+        if (q >= 0. && q <= 1.)
+            return factor * (ipow(3 - q, 5) - 6 * ipow(2 - q, 5) + 15 * ipow(1 - q, 5));
+        if (q > 1. && q <= 2.) return factor * (ipow(3 - q, 5) - 6 * ipow(2 - q, 5));
+        return factor * ipow(3 - q, 5);
+    }
     if (skf == 4) {
         double t = 1.0 / B_LENGTH;
         double factor = 3. / (2. * PI_) * (t * t);
@@ -475,8 +566,9 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double r = c->r_ij[k], hij = c->hsml_ij[k];
         double g;
         kernel_pair(skf, r, hij, &c->kz[i], &c->w[k], &g);
-        double dw0 = g * (dx0 / r) * 1.0;
-        double dw1 = g * (dx1 / r) * 1.0;
+        const double sh = scale_h(skf);
+        double dw0 = g * (dx0 / r) * sh;
+        double dw1 = g * (dx1 / r) * sh;
         c->dwdx[2 * k] = dw0;
         c->dwdx[2 * k + 1] = dw1;
         double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;

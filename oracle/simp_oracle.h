@@ -44,7 +44,13 @@ typedef struct orc_params {
                             boundary_kernel.f90:438-504); 4: Wendland C2
                             (kernel.f90:359-408, boundary_kernel.f90:290-357);
                             5: Wendland C4 (kernel.f90:415-464,
-                            boundary_kernel.f90:364-431) */
+                            boundary_kernel.f90:364-431); 1: quadratic
+                            (kernel.f90:241-290, boundary_kernel.f90:54-120);
+                            2: Gaussian (kernel.f90:108-157,
+                            boundary_kernel.f90:127-192); 3: quintic spline
+                            (kernel.f90:165-233, boundary_kernel.f90:199-283);
+                            7, 8: the Marquis kernels (kernel.f90:527-633)
+                            with b_Gaussian (boundary_kernel.f90:38-43) */
     int    int_algorithm; /* 2 (or 0): predictor_corrector
                             (time_integration.f90:155-282); 1: euler_scheme
                             (:86-148); 3: leapfrog_scheme (:289-391) */

@@ -130,7 +130,7 @@ static int validate(const simp_config *c) {
   if ((c->rheology != 1 && c->rheology != 2) || c->bfa != 1) return SIMP_E_UNSUPPORTED;
   if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
-  if (c->skf < 4 || c->skf > 6 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
+  if (c->skf < 1 || c->skf > 8 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
   if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   return SIMP_OK;
 }
@@ -540,7 +540,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.pa_sph != 0 && d.pa_sph != 1 && d.pa_sph != 2) return SIMP_E_UNSUPPORTED;
   if (d.eos_alg < 0 || d.eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (d.rheology < 0 || d.rheology > 2) return SIMP_E_UNSUPPORTED;
-  if (d.skf != 0 && (d.skf < 4 || d.skf > 6)) return SIMP_E_UNSUPPORTED;
+  if (d.skf < 0 || d.skf > 8) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))

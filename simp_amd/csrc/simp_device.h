@@ -56,7 +56,8 @@ struct InstMeta {
   int pa_sph;            // 1: stress_1; 2: stress_2
   int eos_alg;           // 1: Hibler P; 2: Gutfraind; 3: Kreyscher
   int rheology;          // 1: elliptical yield curve; 2: Mohr-Coulomb
-  int skf;               // 6: Wendland C6; 4: C2; 5: C4
+  int skf;               // 6: Wendland C6; 4: C2; 5: C4; 1 quadratic, 2 Gaussian,
+                         // 3 quintic spline, 7 / 8 Marquis
   int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;

@@ -910,7 +910,76 @@ constexpr double kC4 = 3.0 / kPi;            // 3. / pi
 constexpr float kC2f = (float)(3.0 / (2.0 * kPi));
 constexpr float kC4f = (float)(3.0 / kPi);
 
+// scale_h of each kernel (initialize.f90:207-217): q = scale_h r / h and
+// dW/dx = g (dx / r) scale_h; the support stays mhsml
+__device__ __forceinline__ double kern_scale(int skf) {
+  return skf == 1 ? 2.0 : (skf == 2 || skf == 3 || skf == 7 || skf == 8) ? 3.0 : 1.0;
+}
+
+// The quadratic (skf = 1, kernel.f90:269-278), Gaussian (2, :136-145),
+// quintic spline (3, :193-221) and Marquis kernels (7, :555-564; 8,
+// :604-621): value w, gradient factor g and kernel_zero of one pair, FP64
+// in both precision modes.  Marquis2's 34.1558 and log(2.) are default-REAL.
+constexpr double kLn2f = (double)0x1.62e43p-1f;       // log(2.) in REAL(4)
+constexpr double kMarquis2 = (double)0x1.113f14p+5f;  // 34.1558 in REAL(4)
+
+__device__ void kern_other(int skf, double r, double mh, double *w, double *g, double *kz) {
+  const double sh = kern_scale(skf);
+  const double q = r / mh * sh;
+  const double t = sh / mh;
+  if (skf == 1) {
+    const double factor = 2. / kPi * (t * t);
+    if (kz) *kz = factor * 3. / 4.;
+    if (w) *w = factor * (3. / 16. * ipow<2>(q) - 3. / 4. * q + 3. / 4.);
+    if (g) *g = factor * (3. / 8. * q - 3. / 4.) / mh;
+  } else if (skf == 2) {
+    const double factor = 1. / kPi * (t * t);
+    const double e = exp(-ipow<2>(q));
+    if (kz) *kz = factor;
+    if (w) *w = factor * e;
+    if (g) *g = factor * (-(2 * q * e)) / mh;
+  } else if (skf == 3) {
+    const double factor = 7.e0 / (478.e0 * kPi) * (t * t);
+    if (kz) *kz = factor * 66;
+    const double a = 3 - q, b = 2 - q, c = 1 - q;
+    double ww, gg;
+    if (q >= 0. && q <= 1.) {
+      ww = factor * (ipow<5>(a) - 6 * ipow<5>(b) + 15 * ipow<5>(c));
+      gg = factor * (-(5 * ipow<4>(a)) + 30 * ipow<4>(b) - 75 * ipow<4>(c)) / mh;
+    } else if (q > 1. && q <= 2.) {
+      ww = factor * (ipow<5>(a) - 6 * ipow<5>(b));
+      gg = factor * (-(5 * ipow<4>(a)) + 30 * ipow<4>(b)) / mh;
+    } else {
+      ww = factor * ipow<5>(a);
+      gg = factor * (-(5 * ipow<4>(a))) / mh;
+    }
+    if (w) *w = ww;
+    if (g) *g = gg;
+  } else if (skf == 7) {
+    const double factor = 1. / kPi * (t * t);
+    if (kz) *kz = factor * 0.5;                       // (0.5 - 0.5 * tanh(0.))
+    if (w) *w = factor * (0.5 - 0.5 * tanh(300 * (q - 1)));
+    if (g) {
+      const double ch = cosh(300 * (-1 + q));
+      *g = factor * (-(150 * (1.0 / (ch * ch)))) / mh;
+    }
+  } else {
+    const double C1 = 10., C2 = 50.;
+    const double factor = 1. / (kMarquis2 * kPi * ipow<2>(mh));
+    if (kz) *kz = factor * C2 * (log(cosh(C1)) / C1 + 1 + kLn2f / C1);
+    if (w) *w = factor * C2 * (log(cosh(C1 * (1 - q))) / C1 - q + 1 + kLn2f / C1);
+    if (g) *g = factor * (-(C2 * tanh(C1 * (1 - q))) - 1) / mh;
+  }
+}
+
+__device__ __forceinline__ bool kern_wendland(int skf) { return skf >= 4 && skf <= 6; }
+
 __device__ __forceinline__ double kern_grad(int skf, double r, double mh) {
+  if (!kern_wendland(skf)) {
+    double g;
+    kern_other(skf, r, mh, nullptr, &g, nullptr);
+    return g;
+  }
   if (skf == 4) {
     const double q = r / mh * 1.0, ih = 1.0 / mh;
     const double factor = kC2 * (ih * ih);
@@ -925,6 +994,11 @@ __device__ __forceinline__ double kern_grad(int skf, double r, double mh) {
 }
 
 __device__ __forceinline__ double kern_w(int skf, double r, double mh) {
+  if (!kern_wendland(skf)) {
+    double w;
+    kern_other(skf, r, mh, &w, nullptr, nullptr);
+    return w;
+  }
   if (skf == 4) {
     const double q = r / mh * 1.0, ih = 1.0 / mh;
     return kC2 * (ih * ih) * (ipow<4>(1. - q) * (1 + 4 * q));
@@ -937,6 +1011,11 @@ __device__ __forceinline__ double kern_w(int skf, double r, double mh) {
 }
 
 __device__ __forceinline__ double kern_zero(int skf, double mh) {
+  if (!kern_wendland(skf)) {
+    double kz;
+    kern_other(skf, 0.0, mh, nullptr, nullptr, &kz);
+    return kz;
+  }
   const double ih = 1.0 / mh;
   if (skf == 4) return kC2 * (ih * ih);
   if (skf == 5) return kC4 * (ih * ih) * 3;
@@ -944,6 +1023,11 @@ __device__ __forceinline__ double kern_zero(int skf, double mh) {
 }
 
 __device__ __forceinline__ float kern_gradr_f(int skf, float d2, float mh) {
+  if (!kern_wendland(skf)) {
+    // (the other kernels in FP64: their gradients need not vanish at r = 0)
+    const double r = sqrt((double)d2);
+    return (float)(kern_grad(skf, r, (double)mh) * kern_scale(skf) / r);
+  }
   if (skf == 4 || skf == 5) {
     const float ih = __builtin_amdgcn_rcpf(mh);
     const float q = __builtin_amdgcn_sqrtf(d2) * ih;
@@ -959,6 +1043,20 @@ __device__ __forceinline__ float kern_gradr_f(int skf, float d2, float mh) {
 // hsml_ij at the boundary-pair index (:491), C2 and C4 scale with b_length
 // (:343-346, :417-420).
 __device__ __forceinline__ double bkern_w(int skf, double br, double hk) {
+  if (!kern_wendland(skf)) {
+    // b_Quadratic (boundary_kernel.f90:106-110), b_quintic_spline
+    // (:250-272), b_Gaussian (:177-181, also for the Marquis kernels)
+    const double sh = kern_scale(skf), q = br / kBLength * sh, t = sh / kBLength;
+    if (skf == 1) return 2. / kPi * (t * t) * (3. / 16. * ipow<2>(q) - 3. / 4. * q + 3. / 4.);
+    if (skf == 3) {
+      const double factor = 15. / (478. * kPi) * (t * t);
+      const double a = 3 - q, b = 2 - q, c = 1 - q;
+      if (q >= 0. && q <= 1.) return factor * (ipow<5>(a) - 6 * ipow<5>(b) + 15 * ipow<5>(c));
+      if (q > 1. && q <= 2.) return factor * (ipow<5>(a) - 6 * ipow<5>(b));
+      return factor * ipow<5>(a);
+    }
+    return 1. / kPi * (t * t) * exp(-ipow<2>(q));
+  }
   const double q = br / kBLength * 1.0;
   const double ih = (skf == 4 || skf == 5) ? 1.0 / kBLength : 1.0 / hk;
   if (skf == 4) return kC2 * (ih * ih) * (ipow<4>(1. - q) * (1 + 4 * q));
@@ -1179,6 +1277,7 @@ k_passA(DevState d, int buf, int want_diag) {
   };
   // kernel gradient, vector_divergence and vector_gradient of one pair
   const int skf = XSW ? m.skf : 6;
+  const double ksc = XSW ? kern_scale(skf) : 1.0;
   auto pair = [&](double dx, double dy, double r, double mh, double mq, double rq, double vxq,
                   double vyq) {
     const double dv0 = vxi - vxq, dv1 = vyi - vyq;
@@ -1189,7 +1288,7 @@ k_passA(DevState d, int buf, int want_diag) {
     }
     if (!MIXED) {
       const double gg = kern_grad(skf, r, mh);
-      const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
       const double w1 = dv0 * dw0, w2 = dv1 * dw0, w3 = dv0 * dw1, w4 = dv1 * dw1;
       div = div - mq * (w1 + w4);
       g1 = g1 - mq * w1 / rq;
@@ -1926,6 +2025,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     double den3 = 0.;
     if (sumd3) den3 = kzi * mi / dsi;
     const int skf = XSW ? m.skf : 6;
+    const double ksc = XSW ? kern_scale(skf) : 1.0;
     auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
       const double w = kern_w(skf, sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
@@ -1934,7 +2034,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
                            double q11, double q12, double q22) {
       const double gg = kern_grad(skf, r, mh);
-      const double dw0 = gg * (dx / r) * 1.0, dw1 = gg * (dy / r) * 1.0;
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
       if (pa2) {
         // stress_2 (internal_stress.f90:189-216); as the pair's second
         // particle the reference subtracts the term of the opposite gradient,
@@ -2110,7 +2210,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             const float fx = (float)bdx, fy = (float)bdy;
             const float br2 = fx * fx + fy * fy;
             const float q = __builtin_amdgcn_sqrtf(br2) * (float)(1.0 / kBLength);
-            const float bw = bkern_w_f(skf, q, (float)hk[j]);
+            const float bw = kern_wendland(skf) ? bkern_w_f(skf, q, (float)hk[j])
+                                                : (float)bkern_w(skf, sqrt(bdx * bdx + bdy * bdy),
+                                                                 hk[j]);
             const float cf = (float)(2.0 * kKappaN) * bw * (float)mb[j] *
                              __builtin_amdgcn_rcpf((float)(mi + mb[j]) * br2);
             fbx += cf * fx;

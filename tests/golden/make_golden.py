@@ -272,13 +272,15 @@ def _pc10_edge(tag, harness, **switches):
 
 def switches():
     """10. Mohr-Coulomb rheology (rheology = 2), the Wendland C2 and C4
-    kernels (skf = 4, 5) and the Euler and leap-frog schemes
+    kernels (skf = 4, 5), the quadratic, Gaussian, quintic-spline and
+    Marquis kernels (skf = 1, 2, 3, 7, 8) and the Euler and leap-frog schemes
     (int_algorithm = 1, 3), each from the harness built with that PARAMETER:
     one single_step of the closed box at t > 0 with its pairs' kernel values
-    (rheology and kernels), and ten steps of the ice edge (all five)."""
-    for tag, exe, sw in (("mc", refrun.HARNESS_MC, {"rheology": 2}),
+    (rheology and kernels), and ten steps of the ice edge (all of them)."""
+    kernels = [(f"k{k}", refrun.HARNESS_SKF[k], {"skf": k}) for k in (1, 2, 3, 7, 8)]
+    for tag, exe, sw in [("mc", refrun.HARNESS_MC, {"rheology": 2}),
                          ("c2", refrun.HARNESS_C2, {"skf": 4}),
-                         ("c4", refrun.HARNESS_C4, {"skf": 5})):
+                         ("c4", refrun.HARNESS_C4, {"skf": 5})] + kernels:
         sc = S.box(nside=50, spacing=2000.0, seed=1)
         sc.vx = np.random.default_rng(101).normal(0.0, 0.1, size=(sc.n, 2))
         sc.time = 10.0

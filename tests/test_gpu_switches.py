@@ -35,7 +35,12 @@ def nrel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("tag", ["mc", "c2", "c4"])
+# kernels whose value uses exp, tanh, cosh or log: the device's libm may
+# differ from the host's in the last bit, so they are compared within 1e-12
+TRANSCENDENTAL = ("k2", "k7", "k8")
+
+
+@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8"])
 def test_switch_single_step_fp64_bit_exact_vs_reference_golden(tag):
     say(f"single_step fp64 {tag} golden box2500")
     g = load(f"box2500_single_step_{tag}.npz")
@@ -43,13 +48,26 @@ def test_switch_single_step_fp64_bit_exact_vs_reference_golden(tag):
     dev = Simp.for_scenario(sc)
     out = dev.single_step(sc)
     assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
-    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
-        assert np.array_equal(out[k], g["out_" + k]), (k, nrel(out[k], g["out_" + k]))
     kv = dev.kernel_values(sc.n)
-    assert np.array_equal(kv["w"], g["out_w"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        if tag in TRANSCENDENTAL:
+            assert nrel(out[k], g["out_" + k]) <= 1e-12, (k, nrel(out[k], g["out_" + k]))
+        else:
+            assert np.array_equal(out[k], g["out_" + k]), (k, nrel(out[k], g["out_" + k]))
+    if tag == "k8":
+        # Marquis2's value cancels: for q in (2, 2.5) its terms of about 1
+        # sum to about 1e-3, so a last-bit difference of log(cosh(.)) is
+        # about 1e-6 of the value (the gradient, which the fields use, does
+        # not cancel)
+        w, want = kv["w"], g["out_w"]
+        assert np.all(np.abs(w - want) <= 5e-6 * np.abs(want)), nrel(w, want)
+    elif tag in TRANSCENDENTAL:
+        assert nrel(kv["w"], g["out_w"]) <= 1e-12
+    else:
+        assert np.array_equal(kv["w"], g["out_w"])
 
 
-@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "euler", "leap"])
+@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8", "euler", "leap"])
 def test_switch_time_integration_fp64_vs_reference_golden(tag):
     say(f"time_integration fp64 {tag} golden edge3600")
     g = load(f"edge3600_pc10_{tag}.npz")
@@ -88,8 +106,8 @@ def test_schemes_with_both_densities_vs_oracle(scheme, den_alg):
         assert e <= 1e-12, (k, e)
 
 
-@pytest.mark.parametrize("sw", [{"rheology": 2}, {"skf": 4}, {"skf": 5},
-                                {"int_algorithm": 1}, {"int_algorithm": 3}])
+@pytest.mark.parametrize("sw", [{"rheology": 2}, {"skf": 4}, {"skf": 5}, {"skf": 1},
+                                {"skf": 3}, {"int_algorithm": 1}, {"int_algorithm": 3}])
 def test_switch_mixed_precision_within_tolerance(sw):
     say(f"time_integration mixed {sw} vortex 40k, 20 steps")
     sc = S.vortex()
@@ -109,9 +127,10 @@ def test_switch_mixed_precision_within_tolerance(sw):
 
 @pytest.mark.parametrize("precision", [_lib.SIMP_FP64, _lib.SIMP_MIXED])
 def test_rheology_and_kernels_per_instance_in_an_ensemble(precision):
-    say(f"ensemble with default, MC, C2, C4 members, precision={precision}")
-    scs = S.ensemble(count=4, nmin=1500, nmax=4000, seed=57)
-    for sc, sw in zip(scs, ({}, {"rheology": 2}, {"skf": 4}, {"skf": 5})):
+    say(f"ensemble with default, MC, C2, C4, Gaussian, Marquis2 members, precision={precision}")
+    scs = S.ensemble(count=6, nmin=1500, nmax=4000, seed=57)
+    for sc, sw in zip(scs, ({}, {"rheology": 2}, {"skf": 4}, {"skf": 5}, {"skf": 2},
+                            {"skf": 8})):
         sc.time = 1e-6
         for k, v in sw.items():
             setattr(sc, k, v)

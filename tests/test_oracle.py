@@ -67,7 +67,7 @@ def test_pressure_switch_single_step_bit_exact_vs_reference(eos):
         assert _same(out["p"], P)
 
 
-@pytest.mark.parametrize("tag", ["mc", "c2", "c4"])
+@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8"])
 def test_rheology_and_kernel_switches_single_step_bit_exact_vs_reference(tag):
     """Mohr-Coulomb rheology (rheology = 2, bulk_viscosity.f90:86-145,
     shear_viscosity.f90:67-124) and the Wendland C2 / C4 kernels (skf = 4, 5,
@@ -96,7 +96,10 @@ def test_rheology_and_kernel_switches_single_step_bit_exact_vs_reference(tag):
                                   "edge3600_pc10_den1.npz", "edge3600_pc10_pa2.npz",
                                   "edge3600_pc10_eos2.npz", "edge3600_pc10_mc.npz",
                                   "edge3600_pc10_c2.npz", "edge3600_pc10_c4.npz",
-                                  "edge3600_pc10_euler.npz", "edge3600_pc10_leap.npz"])
+                                  "edge3600_pc10_euler.npz", "edge3600_pc10_leap.npz",
+                                  "edge3600_pc10_k1.npz", "edge3600_pc10_k2.npz",
+                                  "edge3600_pc10_k3.npz", "edge3600_pc10_k7.npz",
+                                  "edge3600_pc10_k8.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
@@ -239,20 +242,35 @@ def test_kernel_gradient_is_the_derivative_of_wendland_c6():
         assert abs(out["dhdt"][0] / (-(sc.h[0] * div / sc.rho[0])) - 1) < 1e-12
 
 
-# Wendland's C2 and C4 profiles in two dimensions (Wendland 1995) and the
-# factors the reference multiplies them by (kernel.f90:388, :444)
-_WENDLAND = {
-    4: (lambda q: (1 - q) ** 4 * (1 + 4 * q), lambda h: 3.0 / (2.0 * math.pi * h * h)),
-    5: (lambda q: (1 - q) ** 6 * (35 * q ** 2 + 18 * q + 3), lambda h: 3.0 / (math.pi * h * h)),
+def _quintic(q):
+    return ((3 - q) ** 5 - 6 * np.clip(2 - q, 0, None) ** 5
+            + 15 * np.clip(1 - q, 0, None) ** 5)
+
+
+# The published profiles of the kernels in two dimensions as functions of
+# q = scale_h r / h -- Wendland C2 and C4 (Wendland 1995), the quadratic
+# (Johnson 1996), Gaussian (Gingold & Monaghan 1977), quintic spline
+# (Morris 1997) and the smoothed step of the Marquis kernel -- with the
+# factors and scale_h the reference uses (kernel.f90:270, 137, 194, 388, 444,
+# 556; initialize.f90:207-217).
+_KERNELS = {
+    4: (lambda q: (1 - q) ** 4 * (1 + 4 * q), lambda h: 3.0 / (2.0 * math.pi * h * h), 1),
+    5: (lambda q: (1 - q) ** 6 * (35 * q ** 2 + 18 * q + 3), lambda h: 3.0 / (math.pi * h * h), 1),
+    1: (lambda q: 3 / 16 * q ** 2 - 3 / 4 * q + 3 / 4, lambda h: 2.0 / math.pi * (2 / h) ** 2, 2),
+    2: (lambda q: np.exp(-q ** 2), lambda h: 1.0 / math.pi * (3 / h) ** 2, 3),
+    3: (_quintic, lambda h: 7.0 / (478.0 * math.pi) * (3 / h) ** 2, 3),
+    7: (lambda q: 0.5 - 0.5 * np.tanh(300 * (q - 1)), lambda h: 1.0 / math.pi * (3 / h) ** 2, 3),
 }
 
 
-@pytest.mark.parametrize("skf", [4, 5])
-def test_wendland_c2_c4_value_gradient_and_kernel_zero(skf):
-    """For one pair: w is the factor times Wendland's profile at q = r / h;
-    the gradient the strain rate carries is the r-derivative of that w (a
-    central difference of the oracle's own w); kernel_zero is w at r = 0."""
-    prof, fac = _WENDLAND[skf]
+@pytest.mark.parametrize("skf", [4, 5, 1, 2, 3, 7])
+def test_kernel_value_gradient_and_kernel_zero(skf):
+    """For one pair: w is the factor times the kernel's profile at
+    q = scale_h r / h; the gradient the strain rate carries is the
+    r-derivative of that w (a central difference of the oracle's own w);
+    kernel_zero is w at r = 0 -- except for the Marquis kernel, whose
+    kernel_zero takes tanh(0.), the profile at q = 1 (kernel.f90:559)."""
+    prof, fac, _ = _KERNELS[skf]
     h = 2000.0
     o = capi.Oracle()
 
@@ -262,9 +280,11 @@ def test_wendland_c2_c4_value_gradient_and_kernel_zero(skf):
         out = o.single_step(sc, current_time=0.0)
         return sc, out, o.kernel_values(sc.n)
 
-    for r in (150.0, 700.0, 1234.5, 1800.0):
+    sh = _KERNELS[skf][2]
+    radii = (150.0, 700.0, 1234.5, 1800.0) if skf != 7 else (664.0, 666.0, 667.5, 669.0)
+    for r in radii:
         sc, out, kv = w_at(r)
-        assert abs(kv["w"][0] / (fac(h) * prof(r / h)) - 1) < 1e-13
+        assert abs(kv["w"][0] / (fac(h) * prof(sh * r / h)) - 1) < 1e-12
         m2, rho2 = sc.mass[1], sc.rho[1]
         dWdx = -out["eps"][0, 0] * rho2 / m2           # dx / r = +1
         e = 1e-3
@@ -272,8 +292,10 @@ def test_wendland_c2_c4_value_gradient_and_kernel_zero(skf):
         assert abs(dWdx / dwdr - 1) < 1e-6, (r, dWdx, dwdr)
     sc, out, kv = w_at(1e-6)
     kz = kv["kernel_zero"].max()        # set for the pair's first particle only
-    assert abs(kz / (fac(h) * prof(0.0)) - 1) < 1e-13
-    assert abs(kv["w"][0] / kz - 1) < 1e-8
+    q0 = 1.0 if skf == 7 else 0.0
+    assert abs(kz / (fac(h) * prof(q0)) - 1) < 1e-13
+    if skf != 7:
+        assert abs(kv["w"][0] / kz - 1) < 1e-8
 
 
 def test_mohr_coulomb_principal_rates_and_zero_caps():
