@@ -51,11 +51,14 @@ def _timed_steps(dev, steps):
     return time.perf_counter() - t0
 
 
-def _latencies(dev, count):
+def _latencies(dev, count, eager=False):
     lat = []
-    for _ in range(count):
+    for _ in range(max(count, 1)):
         t0 = time.perf_counter()
-        dev.step(1, sync=True)
+        if eager:
+            dev.profile_steps(1)
+        else:
+            dev.step(1, sync=True)
         lat.append(time.perf_counter() - t0)
     return np.array(lat)
 
@@ -384,6 +387,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes for roofline.traffic")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the step's kernels directly instead of replaying the captured "
+                         "step graph (for kernel tracing; slower)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--workload", choices=("ice_edge", "ensemble", "serve"), default="ice_edge",
                     help="ensemble / serve: print only that workload's line")
@@ -415,9 +421,18 @@ def main():
     for prec in precs:
         dev = Simp.for_scenario(sc, precision=prec)
         dev.upload(sc)
-        dev.step(args.warmup, sync=True)
-        wall = _timed_steps(dev, args.steps)
-        lat = _latencies(dev, args.latency_steps)
+        if args.eager:
+            # every launch of the step direct (timed with events), no captured
+            # graph: for rocprofv3's kernel tracing, which fails on the
+            # graph launches of this build (host segfault in the tracer)
+            dev.profile_steps(args.warmup)
+            t0 = time.perf_counter()
+            dev.profile_steps(args.steps)
+            wall = time.perf_counter() - t0
+        else:
+            dev.step(args.warmup, sync=True)
+            wall = _timed_steps(dev, args.steps)
+        lat = _latencies(dev, args.latency_steps, args.eager)
         res[prec] = {"rate": n * args.steps / wall, "ms": 1e3 * wall / args.steps,
                      "p50": 1e3 * float(np.percentile(lat, 50)),
                      "p99": 1e3 * float(np.percentile(lat, 99)),
@@ -478,6 +493,11 @@ def main():
                   "candidate_rebuild_rate":
                       r["search"]["candidate_rebuilds"] / r["search"]["evaluations"]}
            for name, r in (("mixed", mixed), ("fp64", fp64))}}
+    if args.eager:
+        # (the measurements below replay captured graphs)
+        line["launch"] = "eager (kernel tracing); ms_per_step includes the event records"
+        print(json.dumps(line), flush=True)
+        return
     line["search_every_evaluation"] = bench_forced_search(sc, min(args.steps, 300))
     line["field_error"] = bench_field_error(sc)
     line["dropin_single_step"] = bench_dropin(sc)
