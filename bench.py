@@ -298,6 +298,7 @@ def bench_ensemble(steps=500, baseline_steps=50):
         ens = Ensemble(prec)
         ens.add(scs)
         ens.step(5)
+        out[key + "_tuning"] = ens.tune(10)
         t0 = time.perf_counter()
         ens.step(steps)
         dt = time.perf_counter() - t0
@@ -387,6 +388,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes for roofline.traffic")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tuning instead of running the autotuner")
     ap.add_argument("--eager", action="store_true",
                     help="launch the step's kernels directly instead of replaying the captured "
                          "step graph (for kernel tracing; slower)")
@@ -416,6 +419,7 @@ def main():
     n = sc.n
     res = {}
     prof = {}
+    tuned = {}
     precs = {"both": (_lib.SIMP_FP64, _lib.SIMP_MIXED), "mixed": (_lib.SIMP_MIXED,),
              "fp64": (_lib.SIMP_FP64,)}[args.precision]
     for prec in precs:
@@ -431,6 +435,9 @@ def main():
             wall = time.perf_counter() - t0
         else:
             dev.step(args.warmup, sync=True)
+            # the autotuner picks the kernels' batch sizes and the list skin
+            # on this workload (its trial steps are steps of the simulation)
+            tuned[prec] = dev.tune(30) if not args.no_tune else dev.get_tuning()
             wall = _timed_steps(dev, args.steps)
         lat = _latencies(dev, args.latency_steps, args.eager)
         res[prec] = {"rate": n * args.steps / wall, "ms": 1e3 * wall / args.steps,
@@ -485,6 +492,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": kms},
     }
+    if tuned:
+        line["autotune"] = {
+            "note": "chosen by simp_tune on this workload after the warm-up (pass A / pass B "
+                    "neighbour batches, candidate-list skin as a fraction of MINVAL(hsml)); "
+                    "us_start: the default setting, us_best: the chosen one",
+            **{name: tuned[p] for name, p in (("mixed", _lib.SIMP_MIXED), ("fp64", _lib.SIMP_FP64))
+               if p in tuned}}
     line["search"] = {
         "note": "evaluations since the upload (warm-up, timed and latency steps) and the "
                 "share that rebuilt the cell lists / the candidate lists",

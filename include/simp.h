@@ -180,6 +180,30 @@ int simp_set_search(simp_handle *h, int every_evaluation);
 int simp_search_stats(simp_handle *h, long long *evaluations,
                       long long *cell_rebuilds, long long *candidate_rebuilds);
 
+/* ---- tuning and the autotuner (no counterpart in the reference) ----
+ * Parameters of the step kernels that change no result, only the speed: the
+ * neighbours whose loads pass A's exact pair test (passA_batch, 1-4) and
+ * pass B's sums (passB_batch, 2-4; 0: 3 in mixed precision, 4 in FP64)
+ * issue together, and the skin of the candidate lists as a fraction of
+ * MINVAL(hsml) (0 < skin <= 0.2).  The cell size stays the reference's
+ * 2 MINVAL(hsml) (nnps.f90:211): the cells fix the pair order.
+ * simp_tune / simp_batch_tune time each candidate setting over
+ * steps_per_trial steps of the resident content, one parameter at a time
+ * (pass A's batch, pass B's, the skin), and keep the fastest; the trial
+ * steps advance the simulation (about 11 trials).  us_start and us_best
+ * receive the microseconds per step of the setting it started from and of
+ * the one it keeps; either may be NULL. */
+typedef struct simp_tuning {
+    int32_t passA_batch;
+    int32_t passB_batch;
+    int32_t reserved;
+    double  skin;
+} simp_tuning;
+int simp_set_tuning(simp_handle *h, const simp_tuning *t);
+int simp_get_tuning(simp_handle *h, simp_tuning *t);
+int simp_tune(simp_handle *h, int steps_per_trial, simp_tuning *chosen,
+              double *us_start, double *us_best);
+
 /* ---- interaction_stats (neighbor/nnps.f90:613-658) ----
  * The neighbour statistics the reference prints from NNPS at every step that
  * is a multiple of print_step (nnps.f90:512-515): the largest and smallest
@@ -334,6 +358,10 @@ int simp_batch_count(simp_batch *b, int *instances, long long *particles);
  * to cap entries and returns the number of instances (or -status). */
 int simp_batch_list(simp_batch *b, int cap, int *ids, int *itimestep, int *steps_left,
                     int *active, int *status);
+/* simp_set_tuning / simp_tune for a batch (all its instances at once). */
+int simp_batch_set_tuning(simp_batch *b, const simp_tuning *t);
+int simp_batch_tune(simp_batch *b, int steps_per_trial, simp_tuning *chosen,
+                    double *us_start, double *us_best);
 /* simp_interaction_stats of one instance of the batch. */
 int simp_batch_interaction_stats(simp_batch *b, int id, int *maxip, int *minip,
                                  double *avgip, int *noip, long long *nip,

@@ -33,7 +33,8 @@ EXPORTS = (
     "simp_sync", "simp_stream", "simp_profile_steps", "simp_strerror",
     "simp_batch_create", "simp_batch_destroy", "simp_batch_add", "simp_batch_remove",
     "simp_batch_update", "simp_interaction_stats", "simp_batch_interaction_stats",
-    "simp_set_search", "simp_search_stats",
+    "simp_set_search", "simp_search_stats", "simp_set_tuning", "simp_get_tuning",
+    "simp_tune", "simp_batch_set_tuning", "simp_batch_tune",
     "simp_batch_step", "simp_batch_sync", "simp_batch_status", "simp_batch_download",
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
@@ -67,6 +68,11 @@ class SimpInstanceDesc(C.Structure):
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p),
                 ("den_alg", C.c_int32), ("pa_sph", C.c_int32), ("eos_alg", C.c_int32),
                 ("rheology", C.c_int32), ("skf", C.c_int32), ("reserved2", C.c_int32)]
+
+
+class SimpTuning(C.Structure):
+    _fields_ = [("passA_batch", C.c_int32), ("passB_batch", C.c_int32),
+                ("reserved", C.c_int32), ("skin", C.c_double)]
 
 
 class SimpError(RuntimeError):
@@ -103,6 +109,11 @@ def lib():
                                    C.c_double, C.c_int] + [_vp] * 9 + [_vp, _vp])
     L.simp_single_step_noalloc.argtypes = L.simp_single_step.argtypes
     L.simp_set_search.argtypes = [_vp, C.c_int]
+    L.simp_set_tuning.argtypes = [_vp, _vp]
+    L.simp_get_tuning.argtypes = [_vp, _vp]
+    L.simp_tune.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+    L.simp_batch_set_tuning.argtypes = [_vp, _vp]
+    L.simp_batch_tune.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
     L.simp_search_stats.argtypes = [_vp, _vp, _vp, _vp]
     L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,

@@ -157,6 +157,27 @@ class Simp:
         out["nip"], out["bnip"] = nip.value, bnip.value
         return out
 
+    def set_tuning(self, passA_batch: int = 2, passB_batch: int = 0, skin: float = 0.01) -> None:
+        """Kernel tuning parameters (no result depends on them)."""
+        t = _lib.SimpTuning(passA_batch, passB_batch, 0, skin)
+        check(lib().simp_set_tuning(self._h, C.byref(t)), "simp_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = _lib.SimpTuning()
+        check(lib().simp_get_tuning(self._h, C.byref(t)), "simp_get_tuning")
+        return {"passA_batch": t.passA_batch, "passB_batch": t.passB_batch, "skin": t.skin}
+
+    def tune(self, steps_per_trial: int = 30) -> dict:
+        """The autotuner on the uploaded state (its trial steps advance the
+        simulation): the setting it keeps and the microseconds per step of
+        the setting it started from and of the kept one."""
+        t = _lib.SimpTuning()
+        a, b = C.c_double(), C.c_double()
+        check(lib().simp_tune(self._h, steps_per_trial, C.byref(t), C.byref(a), C.byref(b)),
+              "simp_tune")
+        return {"passA_batch": t.passA_batch, "passB_batch": t.passB_batch, "skin": t.skin,
+                "us_start": a.value, "us_best": b.value}
+
     def set_search(self, every_evaluation: bool) -> None:
         """Rebuild the cell and candidate lists at every evaluation (as the
         reference's bucket_search does) instead of when needed; the results

@@ -334,6 +334,49 @@ static int stats_out(Engine *e, int k, int n, int *maxip, int *minip, double *av
   return SIMP_OK;
 }
 
+static int tuning_in(const simp_tuning *t, simp::Tuning *out) {
+  if (!t || t->passA_batch < 1 || t->passA_batch > 4) return SIMP_E_ARG;
+  if (t->passB_batch != 0 && (t->passB_batch < 2 || t->passB_batch > 4)) return SIMP_E_ARG;
+  if (!(t->skin > 0.0) || t->skin > 0.2) return SIMP_E_ARG;
+  out->passA_batch = t->passA_batch;
+  out->passB_batch = t->passB_batch;
+  out->skin = (float)t->skin;
+  return SIMP_OK;
+}
+
+static void tuning_out(const simp::Tuning &t, simp_tuning *out) {
+  if (!out) return;
+  out->passA_batch = t.passA_batch;
+  out->passB_batch = t.passB_batch;
+  out->reserved = 0;
+  out->skin = (double)t.skin;
+}
+
+int simp_set_tuning(simp_handle *h, const simp_tuning *t) {
+  if (!h) return SIMP_E_ARG;
+  simp::Tuning v;
+  int rc = tuning_in(t, &v);
+  if (rc) return rc;
+  h->eng->set_tuning(v);
+  return SIMP_OK;
+}
+
+int simp_get_tuning(simp_handle *h, simp_tuning *t) {
+  if (!h || !t) return SIMP_E_ARG;
+  tuning_out(h->eng->tuning(), t);
+  return SIMP_OK;
+}
+
+int simp_tune(simp_handle *h, int steps_per_trial, simp_tuning *chosen, double *us_start,
+              double *us_best) {
+  if (!h || steps_per_trial < 1) return SIMP_E_ARG;
+  if (!h->uploaded) return SIMP_E_STATE;
+  int rc = h->eng->autotune(steps_per_trial, us_start, us_best);
+  if (rc == 0) rc = h->eng->sync_and_check();
+  tuning_out(h->eng->tuning(), chosen);
+  return map_rc(rc);
+}
+
 int simp_set_search(simp_handle *h, int every_evaluation) {
   if (!h || (every_evaluation != 0 && every_evaluation != 1)) return SIMP_E_ARG;
   h->eng->set_force_search(every_evaluation != 0);
@@ -635,6 +678,25 @@ int simp_batch_update(simp_batch *b, int nremove, const int *remove_ids, int nad
   int rc = b->eng->update(rm, ins, &got);
   if (rc) return map_rc(rc);
   for (int k = 0; k < nadd; ++k) ids[k] = got[k];
+  return SIMP_OK;
+}
+
+int simp_batch_tune(simp_batch *b, int steps_per_trial, simp_tuning *chosen, double *us_start,
+                    double *us_best) {
+  if (!b || steps_per_trial < 1) return SIMP_E_ARG;
+  if (b->eng->ninst() == 0) return SIMP_E_STATE;
+  int rc = b->eng->autotune(steps_per_trial, us_start, us_best);
+  if (rc == 0 && hipStreamSynchronize(b->eng->stream()) != hipSuccess) rc = -2;
+  tuning_out(b->eng->tuning(), chosen);
+  return map_rc(rc);
+}
+
+int simp_batch_set_tuning(simp_batch *b, const simp_tuning *t) {
+  if (!b) return SIMP_E_ARG;
+  simp::Tuning v;
+  int rc = tuning_in(t, &v);
+  if (rc) return rc;
+  b->eng->set_tuning(v);
   return SIMP_OK;
 }
 

@@ -186,6 +186,7 @@ struct DevState {
   int nblk;
   int ntot, nbtot, ctot, qtot, ninst;
   int force_search;      // 1: rebuild the cell and candidate lists at every evaluation
+  float skin;            // candidate-list skin, a fraction of MINVAL(hsml) (tuned)
 };
 
 }  // namespace simp

@@ -41,6 +41,15 @@ struct Profiler {
   std::vector<Rec> recs;
 };
 
+// Tuning parameters of the step kernels; none of them changes a result:
+// the neighbours whose loads pass A's exact test and pass B issue together,
+// and the candidate lists' skin as a fraction of MINVAL(hsml).
+struct Tuning {
+  int passA_batch = 2;   // 1 .. 4
+  int passB_batch = 0;   // 2 .. 4; 0: 3 (mixed), 4 (FP64)
+  float skin = 0.01f;    // (0, 0.2]
+};
+
 // Device bytes one instance occupies (state, per-evaluation buffers, cells,
 // boundary-pair store), for admission against an HBM budget.
 size_t instance_footprint(const InstanceInput &in);
@@ -136,6 +145,15 @@ class Engine {
   // instance k's evaluations since its upload, and of them those that
   // rebuilt the cell lists / the candidate lists
   int search_stats(int k, long long *evals, long long *cell, long long *cand);
+  void set_tuning(const Tuning &t);
+  const Tuning &tuning() const { return tune_; }
+  // The autotuner: times candidate settings over `steps` steps of the
+  // resident content each and keeps the fastest -- pass A's batch, then pass
+  // B's, then the skin, one coordinate at a time.  The trial steps are steps
+  // of the simulation (the results do not depend on the setting).  us_start:
+  // microseconds per step of the setting it started from; us_best: of the
+  // one it keeps.
+  int autotune(int nsteps, double *us_start, double *us_best);
   // time integration of steps(): 2 predictor_corrector, 1 euler_scheme,
   // 3 leapfrog_scheme (time_integration.f90:155-282, 86-148, 289-391)
   void set_scheme(int int_algorithm);
@@ -178,6 +196,7 @@ class Engine {
   hipGraph_t graph_ = nullptr;
   bool force_search_ = false;
   int scheme_ = 2;
+  Tuning tune_;
   bool arena_ = false;                // d_ lies in caller memory (single_in_memory)
   double *arena_stage_ = nullptr;     // its staging area
 };

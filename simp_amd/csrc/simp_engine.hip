@@ -525,6 +525,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   arena_stage_ = nullptr;
   d_ = nd;
   d_.force_search = force_search_;
+  d_.skin = tune_.skin;
   allocs_.swap(nlist);
   scan_tmp_ = nscan;
   d_.ntot = (int)ntot; d_.nbtot = (int)nbtot; d_.ctot = (int)ctot; d_.qtot = (int)qtot;
@@ -874,6 +875,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
 
   d_ = nd;
   d_.force_search = force_search_;
+  d_.skin = tune_.skin;
   scan_tmp_ = nscan;
   arena_ = true;
   arena_stage_ = stage;
@@ -941,7 +943,9 @@ void Engine::enqueue_prefix(int buf) {
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   const DevState &d = d_;
   enqueue_prefix(buf);
-  L("passA", [&] { launch_passA(d, buf, precision_, any_xsw_, want_diag ? 1 : 0, st_); });
+  L("passA", [&] {
+    launch_passA(d, buf, precision_, any_xsw_, want_diag ? 1 : 0, tune_.passA_batch, st_);
+  });
   if (phase == 2) {
     // all pair ranks: nip is reported
     L("pairtot", [&] { launch_pairtot(d, st_); });
@@ -949,7 +953,7 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   }
   // boundary-pair ranks and the head of the pair ranks the boundary kernel reads
   if (any_boundary_) L("ranks", [&] { launch_ranks(d, buf, st_); });
-  L("passB", [&] { launch_passB(d, precision_, any_xsw_, phase, st_); });
+  L("passB", [&] { launch_passB(d, precision_, any_xsw_, phase, tune_.passB_batch, st_); });
 }
 
 void Engine::enqueue_step() {
@@ -965,6 +969,69 @@ void Engine::enqueue_step() {
   L("finalize", [&] { launch_finalize(d_, 0, st_); });
   enqueue_eval(1, scheme_ == 3 ? 4 : 1, diag_);
   L("finalize", [&] { launch_finalize(d_, 1, st_); });
+}
+
+void Engine::set_tuning(const Tuning &t) {
+  tune_ = t;
+  d_.skin = t.skin;
+  graph_drop();           // the captured launches hold the old variants and values
+}
+
+int Engine::autotune(int nsteps, double *us_start, double *us_best) {
+  if (nsteps < 1) return -1;
+  if (d_.ninst == 0) return -5;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  auto trial = [&](const Tuning &t, double *us) {
+    set_tuning(t);
+    int rc = this->steps(1, true);        // builds the step graph
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(a, st_));
+    rc = this->steps(nsteps, true);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b, st_));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    *us = 1e3 * (double)ms / nsteps;
+    return 0;
+  };
+  Tuning best = tune_;
+  double tb = 0.;
+  int rc = trial(best, &tb);
+  if (rc == 0 && us_start) *us_start = tb;
+  auto consider = [&](Tuning t) {
+    double us = 0.;
+    if (rc == 0) rc = trial(t, &us);
+    if (rc == 0 && us < tb) {
+      tb = us;
+      best = t;
+    }
+  };
+  for (int ba : {1, 2, 3, 4}) {
+    if (ba == best.passA_batch) continue;
+    Tuning t = best;
+    t.passA_batch = ba;
+    consider(t);
+  }
+  for (int bb : {2, 3, 4}) {
+    if (bb == best.passB_batch) continue;
+    Tuning t = best;
+    t.passB_batch = bb;
+    consider(t);
+  }
+  for (float sk : {0.005f, 0.01f, 0.02f, 0.04f}) {
+    if (sk == best.skin) continue;
+    Tuning t = best;
+    t.skin = sk;
+    consider(t);
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  set_tuning(best);
+  if (us_best) *us_best = tb;
+  return rc;
 }
 
 void Engine::set_scheme(int int_algorithm) {
@@ -1295,7 +1362,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
   enqueue_prefix(0);
-  launch_passA(d, 0, precision_, any_xsw_, 0, st_);
+  launch_passA(d, 0, precision_, any_xsw_, 0, tune_.passA_batch, st_);
   if (any_boundary_) launch_ranks(d, 0, st_);
   launch_pairtot(d, st_);
   launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_);

@@ -20,13 +20,15 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
 void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st);
 void launch_scatter(const DevState &d, hipStream_t st);
 void launch_permute(const DevState &d, int buf, hipStream_t st);
-void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
+// ba / bb: neighbours whose loads go out together in pass A / pass B (tuning
+// parameters; 0 or an untuned value: the default)
+void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag, int ba,
                   hipStream_t st);
 void launch_pairtot(const DevState &d, hipStream_t st);
 void launch_ranks(const DevState &d, int buf, hipStream_t st);
 // phase: 0 predictor, 1 corrector, 2 single evaluation, 3 Euler evaluation
 // (then launch_euler), 4 second leap-frog stage
-void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st);
+void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st);
 void launch_euler(const DevState &d, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st);

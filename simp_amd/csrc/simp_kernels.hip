@@ -832,8 +832,11 @@ __device__ __forceinline__ Ranges neighbour_ranges(const int *cs, int W, int cx,
 // so a list position alone tells its kind: below the middle row 0, in the
 // top row 2, in the middle row 0 / 1 / 2 by left-or-earlier / later own-cell
 // member / right.
-constexpr int kBatch = 4;    // neighbour loads issued together (exact test, pass B)
-constexpr int kBatchBMixed = 3;   // pass B, mixed variant
+// neighbour loads issued together, the defaults of the tuning parameters
+// (pass A's exact test; pass B FP64 and mixed)
+constexpr int kBatchA = 2;
+constexpr int kBatchBFP64 = 4;
+constexpr int kBatchBMixed = 3;
 
 __device__ __forceinline__ int pair_kind(const Ranges &R, int s, int self) {
   if (s < R.a[1]) return 0;
@@ -1154,15 +1157,17 @@ __device__ __forceinline__ int tile_index(const TileRuns &r, int sp) {
 }
 constexpr float kPad = 1.000005f;
 // candidate-list skin, relative to MINVAL(hsml)
-constexpr float kSkin = 0.01f;
 constexpr float kHalfPad = 0.5f * kPad;
 
 // XSW: instances with switch values beyond the reference's defaults may be
 // present (summation density, stress_2, the Hibler or Gutfraind pressure:
 // their code is compiled in; each instance still follows its own values)
-template <bool MIXED, bool XSW>
+// BA: candidates whose loads go out together in the exact test (a tuning
+// parameter: the results do not depend on it)
+template <bool MIXED, bool XSW, int BA>
 __global__ void __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(4)))
 k_passA(DevState d, int buf, int want_diag) {
+  constexpr int kBatch = BA;
   __shared__ float4 tile[kTile];
   __shared__ unsigned short lst[kNbrMax * kTPB];
   if (blockIdx.x >= (unsigned)d.nblk) return;
@@ -1229,7 +1234,7 @@ k_passA(DevState d, int buf, int want_diag) {
     if (y.rb) yw.n_cell += 1;
     if (!vok) yw.n_cand += 1;
   }
-  const float skin = kSkin * (float)y.hmin;
+  const float skin = d.skin * (float)y.hmin;
   // The candidates of every particle of the block lie in three runs of the
   // cell lists: its cell range shifted by -W, 0 and +W cells, widened by one
   // cell on each side.  Overlapping runs (a block spanning most of a row)
@@ -1864,11 +1869,11 @@ __device__ __forceinline__ void wind_at(const DevState &d, const InstMeta &m, do
 // place), 2 = single evaluation (tendencies to d.tend), 3 = Euler step (the
 // tendencies to d.tend, which k_euler applies), 4 = second stage of the
 // leap-frog step (RHS on T, update S in place)
-template <bool MIXED, bool XSW, int phase>
+template <bool MIXED, bool XSW, int phase, int BB>
 __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if (blockIdx.x >= (unsigned)d.nblk) return;   // padding block (no partials)
-  // neighbours per batch (the mixed variant's batch holds fewer registers)
-  constexpr int kB = MIXED ? kBatchBMixed : kBatch;
+  // neighbours per batch (a tuning parameter: the results do not depend on it)
+  constexpr int kB = BB;
   // The loads go out in two rounds, each issued whole before any of its
   // values is used: (1) everything indexed by the list position, including
   // the first kB neighbour-list entries (read whole: the entries past the
@@ -2540,16 +2545,29 @@ void launch_scatter(const DevState &d, hipStream_t st) {
 void launch_permute(const DevState &d, int buf, hipStream_t st) {
   k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
-void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag,
+// The tuned variants exist for the reference's default switch values; the
+// variants for other values run the default batch sizes.
+template <bool MIXED>
+static void launch_passA_t(const DevState &d, int buf, int want_diag, int ba, hipStream_t st) {
+  if (ba == 1)
+    k_passA<MIXED, false, 1><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+  else if (ba == 3)
+    k_passA<MIXED, false, 3><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+  else if (ba == 4)
+    k_passA<MIXED, false, 4><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+  else
+    k_passA<MIXED, false, 2><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+}
+void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag, int ba,
                   hipStream_t st) {
   if (mixed && xsw)
-    k_passA<true, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<true, true, kBatchA><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else if (mixed)
-    k_passA<true, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_passA_t<true>(d, buf, want_diag, ba, st);
   else if (xsw)
-    k_passA<false, true><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    k_passA<false, true, kBatchA><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
   else
-    k_passA<false, false><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_passA_t<false>(d, buf, want_diag, ba, st);
 }
 void launch_pairtot(const DevState &d, hipStream_t st) {
   k_pairtot<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d);
@@ -2557,31 +2575,44 @@ void launch_pairtot(const DevState &d, hipStream_t st) {
 void launch_ranks(const DevState &d, int buf, hipStream_t st) {
   k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d, buf);
 }
-template <bool MIXED, bool XSW>
-static void launch_passB_t(const DevState &d, int phase, hipStream_t st) {
-  if (phase == 0)
-    k_passB<MIXED, XSW, 0><<<g8(d.nblk), kTPB, 0, st>>>(d);
-  else if (phase == 1)
-    k_passB<MIXED, XSW, 1><<<g8(d.nblk), kTPB, 0, st>>>(d);
-  else if (phase == 2)
-    k_passB<MIXED, XSW, 2><<<g8(d.nblk), kTPB, 0, st>>>(d);
-  else if (phase == 3)
-    k_passB<MIXED, XSW, 3><<<g8(d.nblk), kTPB, 0, st>>>(d);
+template <bool MIXED, bool XSW, int phase>
+static void launch_passB_b(const DevState &d, int bb, hipStream_t st) {
+  constexpr int def = MIXED ? kBatchBMixed : kBatchBFP64;
+  // tuned variants: the predictor and corrector of the default switches
+  if (!XSW && phase <= 1 && bb == 2)
+    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 2 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else if (!XSW && phase <= 1 && bb == 3)
+    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 3 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else if (!XSW && phase <= 1 && bb == 4)
+    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 4 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
   else
-    k_passB<MIXED, XSW, 4><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    k_passB<MIXED, XSW, phase, def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
+template <bool MIXED, bool XSW>
+static void launch_passB_t(const DevState &d, int phase, int bb, hipStream_t st) {
+  if (phase == 0)
+    launch_passB_b<MIXED, XSW, 0>(d, bb, st);
+  else if (phase == 1)
+    launch_passB_b<MIXED, XSW, 1>(d, bb, st);
+  else if (phase == 2)
+    launch_passB_b<MIXED, XSW, 2>(d, bb, st);
+  else if (phase == 3)
+    launch_passB_b<MIXED, XSW, 3>(d, bb, st);
+  else
+    launch_passB_b<MIXED, XSW, 4>(d, bb, st);
 }
 void launch_euler(const DevState &d, hipStream_t st) {
   k_euler<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
-void launch_passB(const DevState &d, int mixed, int xsw, int phase, hipStream_t st) {
+void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st) {
   if (mixed && xsw)
-    launch_passB_t<true, true>(d, phase, st);
+    launch_passB_t<true, true>(d, phase, bb, st);
   else if (mixed)
-    launch_passB_t<true, false>(d, phase, st);
+    launch_passB_t<true, false>(d, phase, bb, st);
   else if (xsw)
-    launch_passB_t<false, true>(d, phase, st);
+    launch_passB_t<false, true>(d, phase, bb, st);
   else
-    launch_passB_t<false, false>(d, phase, st);
+    launch_passB_t<false, false>(d, phase, bb, st);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, hipStream_t st) {

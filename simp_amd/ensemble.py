@@ -124,6 +124,17 @@ class Ensemble:
     def sync(self) -> None:
         check(lib().simp_batch_sync(self._h), "simp_batch_sync")
 
+    def tune(self, steps_per_trial: int = 20) -> dict:
+        """The autotuner on the packed content (its trial steps are steps of
+        every active instance)."""
+        from ._lib import SimpTuning
+        t = SimpTuning()
+        a, b = C.c_double(), C.c_double()
+        check(lib().simp_batch_tune(self._h, steps_per_trial, C.byref(t), C.byref(a),
+                                    C.byref(b)), "simp_batch_tune")
+        return {"passA_batch": t.passA_batch, "passB_batch": t.passB_batch, "skin": t.skin,
+                "us_start": a.value, "us_best": b.value}
+
     def status(self, id_: int) -> dict:
         it, left, act, st = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().simp_batch_status(self._h, id_, C.byref(it), C.byref(left), C.byref(act),

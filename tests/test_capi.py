@@ -69,3 +69,12 @@ def test_bad_arguments():
     assert L.simp_create(C.byref(cfg), C.byref(h)) == _lib.SIMP_E_ARG
     assert L.simp_step_pc(None, 1, 1) == _lib.SIMP_E_ARG
     assert L.simp_strerror(_lib.SIMP_E_CAPACITY) == b"capacity exceeded"
+
+
+def test_tuning_calls_reject_bad_arguments_without_a_device():
+    L = _lib.lib()
+    t = _lib.SimpTuning(2, 0, 0, 0.01)
+    assert L.simp_set_tuning(None, C.byref(t)) == _lib.SIMP_E_ARG
+    assert L.simp_get_tuning(None, C.byref(t)) == _lib.SIMP_E_ARG
+    assert L.simp_tune(None, 10, None, None, None) == _lib.SIMP_E_ARG
+    assert L.simp_batch_tune(None, 10, None, None, None) == _lib.SIMP_E_ARG
