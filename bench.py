@@ -7,9 +7,12 @@ Gaussian-random-field wind), synthetic, state resident in HBM.  A step is one
 full predictor-corrector step (two right-hand-side evaluations + update).
 
 Prints one JSON line: mixed-precision particle-steps/s (``value``), the FP64
-rate, p50/p99 step latency, the dominant kernel's roofline (HIP events on the
-library's stream), and the host-CPU baseline (the compiled reference on a
-bounded sub-domain sample of the same workload).
+rate, p50/p99 step latency, the autotuner's choice, the search statistics,
+the dominant kernel's roofline (HIP events on the library's stream) and HBM
+traffic, the mixed-precision error per field, the per-call drop-in,
+snapshots, the smaller box and vortex workloads, the packed ensemble and the
+served request stream, and the host-CPU baseline (the compiled reference on
+the same workload, bounded number of steps).
 """
 from __future__ import annotations
 
@@ -358,6 +361,47 @@ def bench_snapshots(sc, steps=10):
     return out
 
 
+def bench_configs(cpu=True):
+    """The two smaller single-simulation workloads at their own sizes and
+    step counts: the closed box (2,500 particles, 200 steps) and the Rankine
+    vortex (40,000 particles, 500 steps), each in FP64 and mixed precision
+    with p50/p99 step latency, and the compiled reference on the host cores
+    (OpenMP build, bounded sample) beside them."""
+    from simp_amd import _lib
+    from simp_amd import scenarios as S
+    from simp_amd.api import Simp
+    out = {}
+    for key, sc, steps in (("box_2500_200_steps", S.box(), 200),
+                           ("vortex_40k_500_steps", S.vortex(), 500)):
+        r = {"particles": sc.n, "boundary_particles": sc.nb, "steps": steps}
+        for name, prec in (("fp64", _lib.SIMP_FP64), ("mixed", _lib.SIMP_MIXED)):
+            dev = Simp.for_scenario(sc, precision=prec)
+            dev.upload(sc)
+            dev.step(10, sync=True)
+            dev.tune(10)
+            dev.upload(sc)
+            wall = _timed_steps(dev, steps)
+            lat = _latencies(dev, 100, False)
+            r[name] = {"value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                       "p50_step_ms": 1e3 * float(np.percentile(lat, 50)),
+                       "p99_step_ms": 1e3 * float(np.percentile(lat, 99))}
+            dev.close()
+        if cpu:
+            from oracle import refrun
+            if os.path.exists(refrun.HARNESS_OMP):
+                c = sc.copy()
+                c.wind_grid = None
+                c.wind = S.REFERENCE_WIND
+                c.time = 0.0
+                T = _cpu_threads()
+                best = max((_ref_rate(refrun, refrun.HARNESS_OMP, c, t, 2.0) for t in (T, 1)),
+                           key=lambda v: v["value"])
+                r["cpu_reference"] = best
+                r["gpu_fp64_over_cpu"] = r["fp64"]["value"] / best["value"]
+        out[key] = r
+    return out
+
+
 def bench_serve(count=80, budget_gib=16.0, chunk=10):
     """Config 5: Poisson arrivals (20/s) of closed-box simulations, N in
     [5k, 250k], 100-2000 steps each, HBM-budget admission, continuous
@@ -394,6 +438,8 @@ def main():
                     help="launch the step's kernels directly instead of replaying the captured "
                          "step graph (for kernel tracing; slower)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 2,500-particle box and 40k vortex workloads")
     ap.add_argument("--workload", choices=("ice_edge", "ensemble", "serve"), default="ice_edge",
                     help="ensemble / serve: print only that workload's line")
     args = ap.parse_args()
@@ -516,13 +562,15 @@ def main():
     line["field_error"] = bench_field_error(sc)
     line["dropin_single_step"] = bench_dropin(sc)
     line["snapshots"] = bench_snapshots(sc)
+    if not args.no_configs:
+        line["configs"] = bench_configs(cpu=not args.no_cpu)
     if not args.no_ensemble:
         line["ensemble"] = bench_ensemble()
     if not args.no_serve:
         line["serve"] = bench_serve()
     if not args.no_traffic:
         # (pass B: both phase kernels, k_passB<true, false, 0|1>)
-        kname = "k_passB<true, false," if name == "passB" else "k_passA<true, false>"
+        kname = "k_passB<true, false," if name == "passB" else "k_passA<true, false,"
         traffic, info = hbm_traffic(kname)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_note"] = info

@@ -274,6 +274,14 @@ const char *simp_strerror(int status);
 long long simp_snapshot_size(simp_handle *h);
 int simp_snapshot_save(simp_handle *h, void *buf, long long cap);
 int simp_snapshot_load(simp_handle *h, const void *buf, long long len);
+/* Header of a blob (host only, no device): its particle and boundary
+ * counts, step and time, after checking that the blob of len bytes holds
+ * everything the header announces -- what simp_snapshot_load checks first;
+ * a caller can size max_particles / max_boundary from it.  SIMP_E_ARG for a
+ * blob that is not a snapshot, is truncated, or announces impossible
+ * counts.  Any output pointer may be NULL. */
+int simp_snapshot_info(const void *buf, long long len, int *n, int *nb, int *itimestep,
+                       double *current_time);
 
 /* ---- snapshots in the reference's CSV format (host only, no device) ----
  * The files of a run folder: particle_momentum.csv, particle_state.csv and

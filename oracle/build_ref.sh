@@ -98,6 +98,8 @@ P2="$OUT/pa2"
 p2objs=("${vobjs[@]}")
 # Hibler and Gutfraind pressure (eos_alg = 1, 2; eos.f90:59, 94-102)
 variant eos1 'parameter(eos_alg = 3)' 'parameter(eos_alg = 1)'
+E1="$OUT/eos1"
+e1objs=("${vobjs[@]}")
 variant eos2 'parameter(eos_alg = 3)' 'parameter(eos_alg = 2)'
 E2="$OUT/eos2"
 e2objs=("${vobjs[@]}")
@@ -200,6 +202,7 @@ if [ -f "$LIB" ]; then
     gpu_harness "$OUT/obj" "$OUT/mod" "$OUT/simp_ref_harness_gpu" "${objs[@]}"
     gpu_harness "$D1/obj" "$D1/mod" "$OUT/simp_ref_harness_gpu_den1" "${d1objs[@]}"
     gpu_harness "$P2/obj" "$P2/mod" "$OUT/simp_ref_harness_gpu_pa2" "${p2objs[@]}"
+    gpu_harness "$E1/obj" "$E1/mod" "$OUT/simp_ref_harness_gpu_eos1" "${e1objs[@]}"
     gpu_harness "$E2/obj" "$E2/mod" "$OUT/simp_ref_harness_gpu_eos2" "${e2objs[@]}"
     gpu_harness "$MC/obj" "$MC/mod" "$OUT/simp_ref_harness_gpu_mc" "${mcobjs[@]}"
     gpu_harness "$C2/obj" "$C2/mod" "$OUT/simp_ref_harness_gpu_c2" "${c2objs[@]}"

@@ -29,6 +29,7 @@ HARNESS_GPU_PA2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_pa2")
 #: the harnesses built with the Hibler (eos_alg = 1) and Gutfraind (2) pressure
 HARNESS_EOS1 = os.path.join(HERE, "_ref", "simp_ref_harness_eos1")
 HARNESS_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_eos2")
+HARNESS_GPU_EOS1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_eos1")
 HARNESS_GPU_EOS2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_eos2")
 #: the harnesses built with Mohr-Coulomb rheology (rheology = 2), the
 #: Wendland C2 / C4 kernels (skf = 4, 5) and the Euler / leap-frog schemes

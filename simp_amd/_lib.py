@@ -39,6 +39,7 @@ EXPORTS = (
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
     "simp_snapshot_size", "simp_snapshot_save", "simp_snapshot_load",
+    "simp_snapshot_info",
 )
 
 
@@ -147,6 +148,7 @@ def lib():
     L.simp_snapshot_size.restype = C.c_longlong
     L.simp_snapshot_save.argtypes = [_vp, _vp, C.c_longlong]
     L.simp_snapshot_load.argtypes = [_vp, _vp, C.c_longlong]
+    L.simp_snapshot_info.argtypes = [_vp, C.c_longlong, _ip, _ip, _ip, _dp]
     L.simp_output_create.argtypes = [C.c_char_p]
     L.simp_output_append.argtypes = [C.c_char_p, C.c_int] + [_vp] * 9 + [C.c_int, C.c_double]
     L.simp_output_read_last.argtypes = [C.c_char_p, C.c_int, _vp, _vp, _vp] + [_vp] * 9

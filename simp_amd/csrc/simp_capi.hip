@@ -481,25 +481,60 @@ int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
   return map_rc(rc);
 }
 
-int simp_snapshot_load(simp_handle *h, const void *buf, long long len) {
-  if (!h || !buf || len < 48) return SIMP_E_ARG;
+// Parses and checks a blob's header; the counts are bounded by the doubles
+// the blob holds before any size arithmetic, so a forged or truncated
+// header cannot overflow it.
+static int snap_header(const void *buf, long long len, long *n, long *nb, long *nq, int *it,
+                       double *dt, double *t) {
+  if (!buf || len < 48) return SIMP_E_ARG;
   const char *b = (const char *)buf;
   if (std::memcmp(b, kSnapMagic, 8) != 0) return SIMP_E_ARG;
   int32_t hdr[4];
-  int64_t nq;
-  double dt, t;
+  int64_t q;
   std::memcpy(hdr, b + 8, 16);
-  std::memcpy(&nq, b + 24, 8);
-  std::memcpy(&dt, b + 32, 8);
-  std::memcpy(&t, b + 40, 8);
-  const long n = hdr[0], nb = hdr[1];
-  if (n < 1 || nb < 0 || nq < 0 || len < snap_bytes(n, nb, nq)) return SIMP_E_ARG;
+  std::memcpy(&q, b + 24, 8);
+  std::memcpy(dt, b + 32, 8);
+  std::memcpy(t, b + 40, 8);
+  const long long avail = (len - 48) / 8;
+  if (hdr[0] < 1 || hdr[1] < 0 || q < 0) return SIMP_E_ARG;
+  const long long fixed = 10LL * hdr[0] + 3LL * hdr[1];
+  if (fixed > avail || q > avail - fixed) return SIMP_E_ARG;
+  if (len < snap_bytes(hdr[0], hdr[1], (long)q)) return SIMP_E_ARG;
+  *n = hdr[0];
+  *nb = hdr[1];
+  *nq = (long)q;
+  *it = hdr[2];
+  return SIMP_OK;
+}
+
+int simp_snapshot_info(const void *buf, long long len, int *n, int *nb, int *itimestep,
+                       double *current_time) {
+  long n_, nb_, nq_;
+  int it;
+  double dt, t;
+  const int rc = snap_header(buf, len, &n_, &nb_, &nq_, &it, &dt, &t);
+  if (rc) return rc;
+  if (n) *n = (int)n_;
+  if (nb) *nb = (int)nb_;
+  if (itimestep) *itimestep = it;
+  if (current_time) *current_time = t;
+  return SIMP_OK;
+}
+
+int simp_snapshot_load(simp_handle *h, const void *buf, long long len) {
+  if (!h) return SIMP_E_ARG;
+  long n, nb, nq;
+  int it;
+  double dt, t;
+  const int hrc = snap_header(buf, len, &n, &nb, &nq, &it, &dt, &t);
+  if (hrc) return hrc;
+  const char *b = (const char *)buf;
   const double *p = (const double *)(b + 48);
   const double *x = p, *vx = x + 2 * n, *mass = vx + 2 * n, *rho = mass + n, *hsml = rho + n;
   const double *hth = hsml + n, *A = hth + n, *xb = A + n, *massb = xb + 2 * nb, *kz = massb + nb;
   const double *hq = kz + n;
   int rc = simp_upload(h, (int)n, x, vx, mass, rho, hsml, hth, A, (int)nb, nb ? xb : nullptr,
-                       nb ? massb : nullptr, dt, t, hdr[2]);
+                       nb ? massb : nullptr, dt, t, it);
   if (rc) return rc;
   return map_rc(h->eng->upload_persist(0, kz, hq, (long)nq));
 }

@@ -214,7 +214,7 @@ def pressures():
     94-102), from harnesses built with those PARAMETERs: one single_step of
     the closed box with thickness spread around 1 (rho = h rho_ice, so the
     Gutfraind pressure is not zero) for each, and ten predictor-corrector
-    steps of the ice edge with eos_alg = 2."""
+    steps of the ice edge with each."""
     for eos, exe in ((1, refrun.HARNESS_EOS1), (2, refrun.HARNESS_EOS2)):
         sc = S.box(nside=50, spacing=2000.0, seed=1)
         rng = np.random.default_rng(103)
@@ -227,24 +227,8 @@ def pressures():
         _save(f"box2500_single_step_eos{eos}.npz", **_inputs(sc),
               **_outputs(r, ("dxdt", "dvxdt", "dhdt", "dAdt", "p", "sigma")),
               out_nip=r["nip"], out_bnip=r["bnip"], eos_alg=eos)
-    sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
-    sc.wind_grid = None
-    sc.wind = S.REFERENCE_WIND
-    sc.eos_alg = 2
-    K = 10
-    probe = sc.copy()
-    o = capi.Oracle()
-    times = []
-    for k in range(K):
-        o.run_pc(probe, 1, itimestep=k)
-        times.append(probe.time)
-    t_end = 0.5 * (times[-2] + times[-1])
-    r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end,
-                   harness=refrun.HARNESS_EOS2)
-    assert r["itimestep"] == K, r["itimestep"]
-    _save("edge3600_pc10_eos2.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
-          out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
-          integration_time=t_end, eos_alg=2)
+    _pc10_edge("eos1", refrun.HARNESS_EOS1, eos_alg=1)
+    _pc10_edge("eos2", refrun.HARNESS_EOS2, eos_alg=2)
 
 
 def _pc10_edge(tag, harness, **switches):

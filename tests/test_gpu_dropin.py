@@ -23,6 +23,7 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          # stress_2 (pa_sph = 2): the shim passes the module's pa_sph
          ("edge3600_pc10_pa2.npz", refrun.HARNESS_PA2, refrun.HARNESS_GPU_PA2),
          # Gutfraind pressure (eos_alg = 2)
+         ("edge3600_pc10_eos1.npz", refrun.HARNESS_EOS1, refrun.HARNESS_GPU_EOS1),
          ("edge3600_pc10_eos2.npz", refrun.HARNESS_EOS2, refrun.HARNESS_GPU_EOS2),
          # Mohr-Coulomb rheology, Wendland C2 / C4 kernels: the shim passes the
          # module's rheology and skf
