@@ -47,6 +47,13 @@ ALG_BYTES = {
 STEP_BYTES = {0: 576, 1: 512}
 
 
+def _done(section, t0):
+    """A line on stderr when a section of the run has finished (the JSON
+    result goes to stdout at the end)."""
+    print(f"[bench] {section} done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+          flush=True)
+
+
 def _timed_steps(dev, steps):
     dev.sync()
     t0 = time.perf_counter()
@@ -469,6 +476,7 @@ def main():
     precs = {"both": (_lib.SIMP_FP64, _lib.SIMP_MIXED), "mixed": (_lib.SIMP_MIXED,),
              "fp64": (_lib.SIMP_FP64,)}[args.precision]
     for prec in precs:
+        t_start = time.perf_counter()
         dev = Simp.for_scenario(sc, precision=prec)
         dev.upload(sc)
         if args.eager:
@@ -492,6 +500,7 @@ def main():
                      "search": dev.search_stats()}
         prof[prec] = dev.profile_steps(20)
         dev.close()
+        _done("headline, " + ("mixed" if prec == _lib.SIMP_MIXED else "fp64"), t_start)
 
     if args.precision != "both":
         r = res[precs[0]]
@@ -558,24 +567,40 @@ def main():
         line["launch"] = "eager (kernel tracing); ms_per_step includes the event records"
         print(json.dumps(line), flush=True)
         return
+    t0 = time.perf_counter()
     line["search_every_evaluation"] = bench_forced_search(sc, min(args.steps, 300))
+    _done("search at every evaluation", t0)
+    t0 = time.perf_counter()
     line["field_error"] = bench_field_error(sc)
+    _done("field error", t0)
+    t0 = time.perf_counter()
     line["dropin_single_step"] = bench_dropin(sc)
     line["snapshots"] = bench_snapshots(sc)
+    _done("drop-in and snapshots", t0)
     if not args.no_configs:
+        t0 = time.perf_counter()
         line["configs"] = bench_configs(cpu=not args.no_cpu)
+        _done("box and vortex workloads", t0)
     if not args.no_ensemble:
+        t0 = time.perf_counter()
         line["ensemble"] = bench_ensemble()
+        _done("ensemble", t0)
     if not args.no_serve:
+        t0 = time.perf_counter()
         line["serve"] = bench_serve()
+        _done("serve", t0)
     if not args.no_traffic:
         # (pass B: both phase kernels, k_passB<true, false, 0|1>)
         kname = "k_passB<true, false," if name == "passB" else "k_passA<true, false,"
+        t0 = time.perf_counter()
         traffic, info = hbm_traffic(kname)
+        _done("HBM traffic counters", t0)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_note"] = info
     if not args.no_cpu:
+        t0 = time.perf_counter()
         cb = cpu_baseline(sc)
+        _done("CPU baseline", t0)
         cb["gpu_fp64_over_cpu"] = fp64["rate"] / cb["value"]
         cb["gpu_mixed_over_cpu"] = mixed["rate"] / cb["value"]
         line["cpu_baseline"] = cb

@@ -54,7 +54,9 @@ typedef struct simp_config {
                             /* 3: leap-frog (:289-391); batches run 2           */
     int32_t nnps_algorithm; /* 2: bucket search (globals.f90:73)                */
     int32_t sle;            /* 2: hsml = 3 sqrt(m/rho) (globals.f90:81)         */
-    int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89)            */
+    int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89); 1: the    */
+                            /* Monaghan dt (time_step.f90:39-125) without      */
+                            /* artificial viscosity                             */
     int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96);      */
                             /* 2: Mohr-Coulomb (bulk_viscosity.f90:86-145,      */
                             /* shear_viscosity.f90:67-124)                      */
@@ -325,7 +327,7 @@ typedef struct simp_instance_desc {
     double  dt, current_time;
     int32_t itimestep;
     int32_t nsteps;                      /* steps to run, then freeze; 0: none */
-    int32_t tse;                         /* 0 (keep dt) or 2 (VP dt)           */
+    int32_t tse;                         /* 0 (keep dt), 1 (Monaghan), 2 (VP)   */
     int32_t wind_mode;                   /* SIMP_WIND_UNIFORM / SIMP_WIND_GRID */
     double  wind[2], current[2], max_hsml;
     int32_t grid_nx, grid_ny;            /* wind grid (wind_mode = GRID)       */

@@ -130,6 +130,10 @@ euobjs=("${vobjs[@]}")
 variant leap 'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 3)'
 LF="$OUT/leap"
 lfobjs=("${vobjs[@]}")
+# Monaghan time step (tse = 1; time_step.f90:39-125)
+variant tse1 'parameter(tse = 2)' 'parameter(tse = 1)'
+T1="$OUT/tse1"
+t1objs=("${vobjs[@]}")
 
 # The OpenMP build, for the CPU baseline: the reference compiled as its
 # Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
@@ -209,6 +213,7 @@ if [ -f "$LIB" ]; then
     gpu_harness "$C4/obj" "$C4/mod" "$OUT/simp_ref_harness_gpu_c4" "${c4objs[@]}"
     gpu_harness "$EU/obj" "$EU/mod" "$OUT/simp_ref_harness_gpu_euler" "${euobjs[@]}"
     gpu_harness "$LF/obj" "$LF/mod" "$OUT/simp_ref_harness_gpu_leap" "${lfobjs[@]}"
+    gpu_harness "$T1/obj" "$T1/mod" "$OUT/simp_ref_harness_gpu_tse1" "${t1objs[@]}"
     for k in 1 2 3 7 8; do
         eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
     done

@@ -63,6 +63,8 @@ def lib():
         L.orc_wind_at.argtypes = [C.POINTER(OrcParams), C.c_int, _dp, _dp]
         L.orc_vp_dt.argtypes = [C.c_int, _dp]
         L.orc_vp_dt.restype = C.c_double
+        L.orc_monaghan_dt.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_monaghan_dt.restype = C.c_double
         _lib = L
     return _lib
 
@@ -193,6 +195,15 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_sum_density failed ({rc})")
         return rho
+
+    @staticmethod
+    def monaghan_dt(hsml, rho, dvxdt) -> float:
+        """Monaghan_time_step without artificial viscosity
+        (time_step.f90:39-125)."""
+        hsml = np.ascontiguousarray(hsml, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        dvxdt = np.ascontiguousarray(dvxdt, dtype=np.float64).reshape(-1, 2)
+        return float(lib().orc_monaghan_dt(hsml.size, _ptr(hsml), _ptr(rho), _ptr(dvxdt)))
 
     def wind_at(self, sc, pts) -> np.ndarray:
         """The wind (u, v) the forcing term of ``sc`` sees at points

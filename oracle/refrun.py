@@ -48,6 +48,9 @@ HARNESS_GPU_C2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_c2")
 HARNESS_GPU_C4 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_c4")
 HARNESS_GPU_EULER = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_euler")
 HARNESS_GPU_LEAP = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_leap")
+#: the harness built with the Monaghan time step (tse = 1)
+HARNESS_TSE1 = os.path.join(HERE, "_ref", "simp_ref_harness_tse1")
+HARNESS_GPU_TSE1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_tse1")
 #: the OpenMP build sized for the 250k workload (maxn = 300000), timings only
 HARNESS_OMP = os.path.join(HERE, "_ref", "simp_ref_harness_omp")
 #: the serial build with the same capacity

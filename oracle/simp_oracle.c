@@ -725,6 +725,29 @@ double orc_vp_dt(int n, const double *hsml) {
     return dt;
 }
 
+/* Monaghan_time_step (time_step.f90:39-125) without artificial viscosity
+ * (artificial_visc = .FALSE., globals.f90:186): fF MINVAL(hsml / |a|)**0.5
+ * with fF = 0.25 and a the last evaluation's dvxdt (:79-83), against
+ * MINVAL(hsml / c) (:117), the smaller of the two (:121); c from c_yang
+ * (sound_speed.f90), sqrt(Y (1 - nu) / (rho (1 + nu) (1 - 2 nu))), at the
+ * step's final density.  Y = 1.D9, nu = 0.3 (a default-REAL literal,
+ * data_variables.f90:159-160). */
+static const double Y_MOD = 1.0e9;
+static const double NU = (double)0.3f;
+
+double orc_monaghan_dt(int n, const double *hsml, const double *rho, const double *dvxdt) {
+    double mf = 0.0, mc = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double acc = sqrt(dvxdt[2 * i] * dvxdt[2 * i] + dvxdt[2 * i + 1] * dvxdt[2 * i + 1]);
+        double c = sqrt(Y_MOD * (1 - NU) / (rho[i] * (1 + NU) * (1 - 2 * NU)));
+        double f = hsml[i] / acc, g = hsml[i] / c;
+        if (i == 0 || f < mf) mf = f;
+        if (i == 0 || g < mc) mc = g;
+    }
+    double dt_force = 0.25 * sqrt(mf);
+    return dt_force < mc ? dt_force : mc;
+}
+
 /* ---- time_integration + predictor_corrector -------------------------- */
 int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                double *vx, const double *mass, double *rho, double *hsml,
@@ -827,6 +850,7 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
         }
         /* time_step (:280) */
         if (prm->tse == 2) dt = orc_vp_dt(n, hsml);
+        else if (prm->tse == 1) dt = orc_monaghan_dt(n, hsml, rho, dvxdt);
         /* loop bookkeeping (time_integration.f90:68-71) */
         it += 1;
         t = t + dt;

@@ -21,7 +21,8 @@ typedef struct orc_ctx orc_ctx;
 /* Forcing and switches of one simulation (the runtime form of the
  * compile-time PARAMETERs the reference uses on this path). */
 typedef struct orc_params {
-    int    tse;          /* 0: keep dt (time_step.f90:21-22); 2: VP dt (:132-157) */
+    int    tse;          /* 0: keep dt (time_step.f90:21-22); 1: Monaghan dt (:39-125);
+                            2: VP dt (:132-157) */
     int    wind_mode;    /* 0: uniform wind; 1: gridded wind, bilinear sampling */
     double wind[2];      /* uniform wind (m/s); the reference hard-codes (0, -5.3f) */
     double current[2];   /* uniform ocean current (m/s); reference: (0, 0) */
@@ -111,6 +112,9 @@ void orc_wind_at(const orc_params *prm, int n, const double *x, double *uv);
 
 /* VP_time_step (time_step.f90:155). */
 double orc_vp_dt(int n, const double *hsml);
+/* Monaghan_time_step without artificial viscosity (time_step.f90:39-125)
+ * from hsml, rho and the last evaluation's dvxdt (2, n). */
+double orc_monaghan_dt(int n, const double *hsml, const double *rho, const double *dvxdt);
 
 #ifdef __cplusplus
 }

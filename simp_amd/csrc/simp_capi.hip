@@ -126,7 +126,7 @@ static int validate(const simp_config *c) {
   if (c->precision != SIMP_FP64 && c->precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
   if (c->int_algorithm < 1 || c->int_algorithm > 3) return SIMP_E_UNSUPPORTED;
   if (c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
-  if (c->tse != 0 && c->tse != 2) return SIMP_E_UNSUPPORTED;
+  if (c->tse < 0 || c->tse > 2) return SIMP_E_UNSUPPORTED;
   if ((c->rheology != 1 && c->rheology != 2) || c->bfa != 1) return SIMP_E_UNSUPPORTED;
   if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
@@ -612,7 +612,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.n < 1 || d.nb < 0 || !d.x || !d.vx || !d.mass || !d.rho || !d.hsml || !d.hth || !d.A)
     return SIMP_E_ARG;
   if (d.nb > 0 && (!d.xb || !d.massb)) return SIMP_E_ARG;
-  if (d.tse != 0 && d.tse != 2) return SIMP_E_UNSUPPORTED;
+  if (d.tse < 0 || d.tse > 2) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode != SIMP_WIND_UNIFORM && d.wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   if (d.den_alg != 0 && d.den_alg != 1 && d.den_alg != 3) return SIMP_E_UNSUPPORTED;
   if (d.pa_sph != 0 && d.pa_sph != 1 && d.pa_sph != 2) return SIMP_E_UNSUPPORTED;

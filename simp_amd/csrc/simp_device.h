@@ -31,6 +31,10 @@ constexpr double kKappaN = 5.4e12;
 constexpr double kKT = 0.0;
 constexpr double kE = 2.0;
 constexpr double kHsmlMult = 3.0;
+// c_yang's material constants (sound_speed.f90; data_variables.f90:159-160):
+// Young modulus and Poisson ratio, the latter a default-REAL literal
+constexpr double kYoung = 1.0e9;
+constexpr double kNu = (double)0.3f;
 constexpr int kMaxIndInt = 200;     // globals.f90:24
 constexpr int kPrintStep = 2000;    // globals.f90:55
 
@@ -38,6 +42,9 @@ constexpr int kTPB = 256;           // threads per block of particle kernels
 constexpr int kScanItems = 2048;    // elements per block of the scan
 constexpr int kMaxBoundaryPerInst = 8192;
 constexpr int kNbrMax = 32;         // neighbours kept per particle in the list
+// per-block partial reductions: min/max x, min/max y, min hsml, max speed,
+// max hsml change, and for the Monaghan time step min hsml/|a|, min hsml/c
+constexpr int kParts = 9;
 
 // Error bits (OR-ed into InstDyn::err); translated to SIMP_E_* on the host.
 constexpr int kErrCapacity = 1;
@@ -146,7 +153,7 @@ struct DevState {
   // mhsml + skin at the evaluation that built them (j-major like nbr, at
   // most kNbrMax; vnc > kNbrMax: not stored)
   int *vnb, *vnc;
-  double *part;          // per-block partial reductions (7 x nblk)
+  double *part;          // per-block partial reductions (kParts x nblk)
   int *bflag;            // per cell: stamp of the last evaluation in which a
                          // boundary particle's 3x3 block covered the cell
   int *own_cnt, *moff, *pair_count, *pair_start, *cell_intra;

@@ -22,7 +22,8 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // an instance with a switch value beyond the reference's defaults: the
 // kernels then run their variants that read each instance's values
 static inline bool nondefault(const InstMeta &m) {
-  return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6;
+  return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
+         m.tse == 1;
 }
 
 // ------------------------------------------------------------ host sizing
@@ -163,7 +164,7 @@ static int alloc_state(DevState &d, A &&alloc, int **scan_tmp, size_t N, size_t 
   rc |= dalloc(alloc, &d.vnb, (size_t)kNbrMax * N);
   rc |= dalloc(alloc, &d.vnc, N);
 
-  rc |= dalloc(alloc, &d.part, 7 * BLK);
+  rc |= dalloc(alloc, &d.part, kParts * BLK);
   rc |= dalloc(alloc, &d.bflag, C);
   rc |= dalloc(alloc, &d.count, C);
   rc |= dalloc(alloc, &d.fill, C);

@@ -211,33 +211,53 @@ __device__ __forceinline__ bool inst_off(const InstDyn &y) { return y.err != 0 |
 
 // Per-block min/max of x, y, min of hsml, and max of the speed and of the
 // hsml change (the candidate lists' movement bound; invalid lanes give
-// neutral values), stored at the block's logical id.  Non-finite input
-// raises the instance's NaN flag.
-constexpr int kParts = 7;
-
+// neutral values), stored at the block's logical id; with NP = kParts also
+// the minima of hsml/|a| and hsml/c the Monaghan time step takes (only the
+// kernels that end a step of such an instance compute them, and only the
+// finalize of such an instance reads them).  Non-finite input raises the
+// instance's NaN flag.
+template <int NP = 7>
 __device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok, double xv,
-                                               double yv, double hv, double sp, double dh) {
-  __shared__ double sh[kParts][kTPB / 64];
+                                               double yv, double hv, double sp, double dh,
+                                               double fo = 0.0, double fc = 0.0) {
+  static_assert(NP == 7 || NP == kParts, "partials: the grid's, or the grid's and dt's");
+  __shared__ double sh[NP][kTPB / 64];
   bool bad = ok && (!(xv == xv) || !(yv == yv) || !(hv > 0.0) || __builtin_isinf(xv) ||
                     __builtin_isinf(yv) || __builtin_isinf(hv));
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.dyn[k].err, kErrNan);
   const double big = __builtin_huge_val();
-  double v[kParts];
+  double v[NP];
   v[0] = wave_min(ok ? xv : big); v[1] = wave_max(ok ? xv : -big);
   v[2] = wave_min(ok ? yv : big); v[3] = wave_max(ok ? yv : -big);
   v[4] = wave_min(ok ? hv : big);
   v[5] = wave_max(ok ? sp : 0.0); v[6] = wave_max(ok ? dh : 0.0);
+  if (NP == kParts) {
+    v[NP - 2] = wave_min(ok ? fo : big);
+    v[NP - 1] = wave_min(ok ? fc : big);
+  }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
-    for (int f = 0; f < kParts; ++f) sh[f][w] = v[f];
+    for (int f = 0; f < NP; ++f) sh[f][w] = v[f];
   __syncthreads();
-  if (threadIdx.x < kParts) {
+  if (threadIdx.x < NP) {
     const int f = threadIdx.x;
-    const bool mx = f == 1 || f == 3 || f >= 5;
+    const bool mx = f == 1 || f == 3 || f == 5 || f == 6;
     double r = sh[f][0];
     for (int i = 1; i < kTPB / 64; ++i) r = mx ? fmax(r, sh[f][i]) : fmin(r, sh[f][i]);
     d.part[(size_t)f * d.nblk + d.blk[blockIdx.x].lid] = r;
   }
+}
+
+// The terms of Monaghan_time_step (time_step.f90:79-83, :117) at one
+// particle without artificial viscosity: hsml / |a| with a the evaluation's
+// dvxdt, and hsml / c with c from c_yang (sound_speed.f90) at the step's
+// final density.
+__device__ __forceinline__ void monaghan_terms(double hs, double rho, double ax, double ay,
+                                               double &fo, double &fc) {
+  const double acc = sqrt(ax * ax + ay * ay);
+  const double c = sqrt(kYoung * (1 - kNu) / (rho * (1 + kNu) * (1 - 2 * kNu)));
+  fo = hs / acc;
+  fc = hs / c;
 }
 
 // For one instance, by one block: the cells of the boundary particles (whose
@@ -358,7 +378,7 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
 }
 
 __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
-                             double ymn, double ymx, double hmn);
+                             double ymn, double ymx, double hmn, double mforce, double mcfl);
 
 __device__ __forceinline__ void stats_reset(InstDyn &y) {
   y.ipstep = y.itimestep;
@@ -453,7 +473,10 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
 
   const double big = __builtin_huge_val();
   double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big, v5 = 0.0, v6 = 0.0;
+  double v7 = big, v8 = big;
   const size_t nb = (size_t)d.nblk;
+  // the Monaghan time step's minima exist at the end of such a step only
+  const bool mdt = end_of_step && m.tse == 1;
   for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
     v0 = fmin(v0, d.part[b]);
     v1 = fmax(v1, d.part[nb + b]);
@@ -462,13 +485,18 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
     v4 = fmin(v4, d.part[4 * nb + b]);
     v5 = fmax(v5, d.part[5 * nb + b]);
     v6 = fmax(v6, d.part[6 * nb + b]);
+    if (mdt) {
+      v7 = fmin(v7, d.part[7 * nb + b]);
+      v8 = fmin(v8, d.part[8 * nb + b]);
+    }
   }
   v0 = wave_min(v0); v1 = wave_max(v1); v2 = wave_min(v2); v3 = wave_max(v3);
   v4 = wave_min(v4); v5 = wave_max(v5); v6 = wave_max(v6);
+  v7 = wave_min(v7); v8 = wave_min(v8);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sh[0][w] = v0; sh[1][w] = v1; sh[2][w] = v2; sh[3][w] = v3; sh[4][w] = v4;
-    sh[5][w] = v5; sh[6][w] = v6;
+    sh[5][w] = v5; sh[6][w] = v6; sh[7][w] = v7; sh[8][w] = v8;
   }
   __syncthreads();
   __shared__ int go;
@@ -481,9 +509,12 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
       v4 = fmin(sh[4][0], sh[4][i]); sh[4][0] = v4;
       v5 = fmax(sh[5][0], sh[5][i]); sh[5][0] = v5;
       v6 = fmax(sh[6][0], sh[6][i]); sh[6][0] = v6;
+      v7 = fmin(sh[7][0], sh[7][i]); sh[7][0] = v7;
+      v8 = fmin(sh[8][0], sh[8][i]); sh[8][0] = v8;
     }
     move_bound(d.dyn[k], sh[5][0], sh[6][0]);
-    go = finalize_grid(d, k, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0]);
+    go = finalize_grid(d, k, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0],
+                       sh[7][0], sh[8][0]);
   }
   __syncthreads();
   // the boundary particles' cell lists for the next evaluation
@@ -493,7 +524,7 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
 // Thread 0 of k_finalize: close the step, then the grid of the next
 // evaluation.  Returns whether the instance evaluates next.
 __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
-                             double ymn, double ymx, double hmn) {
+                             double ymn, double ymx, double hmn, double mforce, double mcfl) {
   const InstMeta &m = d.meta[k];
   InstDyn &y = d.dyn[k];
   if (inst_off(y)) return 0;
@@ -501,6 +532,12 @@ __device__ int finalize_grid(const DevState &d, int k, int end_of_step, double x
     // VP_time_step: MINVAL(rho_ice hsml^2 delta_min / (P_star (1 + k_t))) is
     // the same expression at MINVAL(hsml), each operation being monotone
     if (m.tse == 2) y.dt = kRhoIce * (hmn * hmn) * kDeltaMin / (kPStar * (1 + kKT));
+    // Monaghan_time_step (time_step.f90:79-121): fF = 0.25 times the root of
+    // MINVAL(hsml / |a|), or MINVAL(hsml / c) if smaller
+    if (m.tse == 1) {
+      const double dt_force = 0.25 * sqrt(mforce);
+      y.dt = dt_force < mcfl ? dt_force : mcfl;
+    }
     y.itimestep += 1;
     y.time = y.time + y.dt;
     if (y.steps_left > 0 && --y.steps_left == 0) {
@@ -1924,6 +1961,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   const bool live = ok && !inst_off(y);
   double hsml_new = 1.0, px_new = 0.0, py_new = 0.0;   // for the partials
   double speed = 0.0, dhs = 0.0;
+  double rho_new = 1.0, mfo = 0.0, mfc = 0.0;          // Monaghan time step terms
   int st_cnt = 0, st_own = 0;                          // interaction statistics
   if (live) {
     const int W = y.W;
@@ -1991,6 +2029,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         if (hs > m.max_hsml) hs = m.max_hsml;
         d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
+        rho_new = rn;
       }
     } else if (phase == 4) {
       // leapfrog_scheme (time_integration.f90:361-391): thickness,
@@ -2012,6 +2051,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       if (hs > m.max_hsml) hs = m.max_hsml;
       d.hsml[2 * cs + 0][p] = hs;
       hsml_new = hs;
+      rho_new = rn;
     }
     const bool listed = nn <= kNbrMax;
     int sj[kB];
@@ -2287,6 +2327,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         if (hs > m.max_hsml) hs = m.max_hsml;
         d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
+        rho_new = rn;
       }
     } else if (phase == 4) {
       d.vx[2 * cs + 0][p] = u1 + ax * dt;
@@ -2300,10 +2341,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       d.tend[4 * N + p] = dhdt;
       d.tend[5 * N + p] = dAdt;
     }
+    // time_step at the end of the step (time_integration.f90:280, :389):
+    // the Monaghan step's terms from this evaluation's dvxdt
+    if (XSW && (phase == 1 || phase == 4) && m.tse == 1)
+      monaghan_terms(hsml_new, rho_new, ax, ay, mfo, mfc);
   }
   // (with every lane of the wave: a wave holds one instance)
   if (phase == 2 || y.itimestep % kPrintStep == 0) stats_add(d.dyn[k], live, st_cnt, st_own);
-  if (phase != 2 && phase != 3) block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
+  if constexpr (XSW && (phase == 1 || phase == 4))
+    block_partials<kParts>(d, k, live, px_new, py_new, hsml_new, speed, dhs, mfo, mfc);
+  else if constexpr (phase != 2 && phase != 3)
+    block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
 }
 
 // euler_scheme's update (time_integration.f90:117-146) after the Euler
@@ -2318,7 +2366,7 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
   const bool ok = particle_of(d, p, k);
   const InstDyn &y = d.dyn[k];
   const bool live = ok && !inst_off(y);
-  double xn = 0., yn = 0., hs = 1.0, sp = 0., dh = 0.;
+  double xn = 0., yn = 0., hs = 1.0, sp = 0., dh = 0., fo = 0., fc = 0.;
   if (live) {
     const InstMeta &m = d.meta[k];
     const int cs = d.blk[blockIdx.x].cs;
@@ -2337,6 +2385,8 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
     if (hs > m.max_hsml) hs = m.max_hsml;
     sp = sqrt(vx * vx + vy * vy);
     dh = fabs(hs - d.hsml[2 * cs][p]);
+    // time_step (time_integration.f90:146): the Monaghan step's terms
+    if (m.tse == 1) monaghan_terms(hs, rn, T[2 * N + p], T[3 * N + p], fo, fc);
     d.x[2 * cs][p] = xn;
     d.y[2 * cs][p] = yn;
     d.vx[2 * cs][p] = un;
@@ -2346,7 +2396,7 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
     d.rho[2 * cs][p] = rn;
     d.hsml[2 * cs][p] = hs;
   }
-  block_partials(d, k, live, xn, yn, hs, sp, dh);
+  block_partials<kParts>(d, k, live, xn, yn, hs, sp, dh, fo, fc);
 }
 
 // ------------------------------------------------------- pair-list export

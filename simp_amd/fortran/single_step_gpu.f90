@@ -33,6 +33,7 @@ subroutine single_step
         cfg%rheology = rheology
         cfg%skf = skf
         cfg%int_algorithm = int_algorithm
+        cfg%tse = tse
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

@@ -257,8 +257,9 @@ def _pc10_edge(tag, harness, **switches):
 def switches():
     """10. Mohr-Coulomb rheology (rheology = 2), the Wendland C2 and C4
     kernels (skf = 4, 5), the quadratic, Gaussian, quintic-spline and
-    Marquis kernels (skf = 1, 2, 3, 7, 8) and the Euler and leap-frog schemes
-    (int_algorithm = 1, 3), each from the harness built with that PARAMETER:
+    Marquis kernels (skf = 1, 2, 3, 7, 8), the Euler and leap-frog schemes
+    (int_algorithm = 1, 3) and the Monaghan time step (tse = 1), each from
+    the harness built with that PARAMETER:
     one single_step of the closed box at t > 0 with its pairs' kernel values
     (rheology and kernels), and ten steps of the ice edge (all of them)."""
     kernels = [(f"k{k}", refrun.HARNESS_SKF[k], {"skf": k}) for k in (1, 2, 3, 7, 8)]
@@ -278,6 +279,7 @@ def switches():
         _pc10_edge(tag, exe, **sw)
     _pc10_edge("euler", refrun.HARNESS_EULER, int_algorithm=1)
     _pc10_edge("leap", refrun.HARNESS_LEAP, int_algorithm=3)
+    _pc10_edge("tse1", refrun.HARNESS_TSE1, tse=1)
 
 
 if __name__ == "__main__":

@@ -21,7 +21,7 @@ def scenario(g, prefix="in_") -> Scenario:
                   hsml=g[prefix + "hsml"].copy(), h=g[prefix + "h"].copy(),
                   A=g[prefix + "A"].copy(), xb=g[prefix + "xb"].reshape(-1, 2).copy(),
                   massb=g[prefix + "massb"].copy(), max_hsml=float(max_hsml),
-                  wind=REFERENCE_WIND, tse=2, dt=float(dt), time=float(t))
+                  wind=REFERENCE_WIND, tse=int(g.get("tse", 2)), dt=float(dt), time=float(t))
     sc.den_alg = int(g.get("den_alg", 3))
     sc.pa_sph = int(g.get("pa_sph", 1))
     sc.eos_alg = int(g.get("eos_alg", 3))

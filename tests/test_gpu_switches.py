@@ -67,7 +67,8 @@ def test_switch_single_step_fp64_bit_exact_vs_reference_golden(tag):
         assert np.array_equal(kv["w"], g["out_w"])
 
 
-@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8", "euler", "leap"])
+@pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8", "euler", "leap",
+                                 "tse1"])
 def test_switch_time_integration_fp64_vs_reference_golden(tag):
     say(f"time_integration fp64 {tag} golden edge3600")
     g = load(f"edge3600_pc10_{tag}.npz")
@@ -78,6 +79,7 @@ def test_switch_time_integration_fp64_vs_reference_golden(tag):
     out = dev.time_integration(sc, K)
     assert out["itimestep"] == K
     assert abs(out["time"] - float(g["out_time"])) <= 1e-12 * abs(float(g["out_time"]))
+    assert abs(out["dt"] - float(g["out_dt"])) <= 1e-12 * abs(float(g["out_dt"]))
     for k in FIELDS:
         e = nrel(getattr(sc, k), g["out_" + k])
         assert e <= 1e-12, (k, e)

@@ -99,7 +99,8 @@ def test_rheology_and_kernel_switches_single_step_bit_exact_vs_reference(tag):
                                   "edge3600_pc10_euler.npz", "edge3600_pc10_leap.npz",
                                   "edge3600_pc10_k1.npz", "edge3600_pc10_k2.npz",
                                   "edge3600_pc10_k3.npz", "edge3600_pc10_k7.npz",
-                                  "edge3600_pc10_k8.npz"])
+                                  "edge3600_pc10_k8.npz", "edge3600_pc10_eos1.npz",
+                                  "edge3600_pc10_tse1.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
