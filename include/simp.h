@@ -73,7 +73,8 @@ typedef struct simp_config {
                             /* 2: Gaussian, 3: quintic spline, 7, 8: Marquis    */
                             /* (kernel.f90:108-290, 527-633)                    */
     int32_t den_alg;        /* 3: diagnostic density rho = h rho_ice (:155);    */
-                            /* 1: summation density (density.f90:41-100)        */
+                            /* 1: summation density (density.f90:41-100);       */
+                            /* 2: continuity density (density.f90:107-165)      */
     int32_t wind_mode;      /* SIMP_WIND_UNIFORM or SIMP_WIND_GRID              */
     int32_t max_particles;  /* capacity of particles (0: size at upload)        */
     int32_t max_boundary;   /* capacity of boundary particles (0: at upload)    */
@@ -165,6 +166,17 @@ int simp_single_step_noalloc(simp_handle *h, int n, const double *x,
  * nip is always written; SIMP_E_CAPACITY when it exceeds cap. */
 int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
                        double *w, double *kernel_zero);
+
+/* ---- kernel gradients of the last single_step, for a host-side density --
+ * Replaces the module arrays single_step leaves for con_density
+ * (phy_quantities/density.f90:107-165): the particle pairs of the last
+ * simp_single_step in the reference's order (p_i, p_j 1-based) and the
+ * kernel gradient dwdx of each (written by the kernel routines of
+ * sph/kernel.f90, e.g. :507), in the pair's orientation dx = x(p_i) -
+ * x(p_j), (2, nip) interleaved.  Any array may be NULL; nip is always
+ * written; SIMP_E_CAPACITY when it exceeds cap (pairs). */
+int simp_kernel_gradients(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
+                          double *dwdx);
 
 /* ---- how often the neighbour search runs ----
  * The reference's bucket_search (nnps.f90:155-517) sorts the particles into
@@ -333,7 +345,8 @@ typedef struct simp_instance_desc {
     int32_t grid_nx, grid_ny;            /* wind grid (wind_mode = GRID)       */
     double  grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u, *grid_v;
-    int32_t den_alg;                     /* 0 or 3: diagnostic; 1: summation   */
+    int32_t den_alg;                     /* 0 or 3: diagnostic; 1: summation;  */
+                                         /* 2: continuity                      */
     int32_t pa_sph;                      /* 0 or 1: stress_1; 2: stress_2      */
     int32_t eos_alg;                     /* 0 or 3: Kreyscher; 1: Hibler; 2:   */
                                          /* Gutfraind                          */

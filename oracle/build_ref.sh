@@ -130,6 +130,10 @@ euobjs=("${vobjs[@]}")
 variant leap 'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 3)'
 LF="$OUT/leap"
 lfobjs=("${vobjs[@]}")
+# continuity density (den_alg = 2; density.f90:107-165)
+variant den2 'parameter(den_alg = 3)' 'parameter(den_alg = 2)'
+D2="$OUT/den2"
+d2objs=("${vobjs[@]}")
 # Monaghan time step (tse = 1; time_step.f90:39-125)
 variant tse1 'parameter(tse = 2)' 'parameter(tse = 1)'
 T1="$OUT/tse1"
@@ -214,6 +218,7 @@ if [ -f "$LIB" ]; then
     gpu_harness "$EU/obj" "$EU/mod" "$OUT/simp_ref_harness_gpu_euler" "${euobjs[@]}"
     gpu_harness "$LF/obj" "$LF/mod" "$OUT/simp_ref_harness_gpu_leap" "${lfobjs[@]}"
     gpu_harness "$T1/obj" "$T1/mod" "$OUT/simp_ref_harness_gpu_tse1" "${t1objs[@]}"
+    gpu_harness "$D2/obj" "$D2/mod" "$OUT/simp_ref_harness_gpu_den2" "${d2objs[@]}"
     for k in 1 2 3 7 8; do
         eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
     done

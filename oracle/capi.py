@@ -63,6 +63,8 @@ def lib():
         L.orc_wind_at.argtypes = [C.POINTER(OrcParams), C.c_int, _dp, _dp]
         L.orc_vp_dt.argtypes = [C.c_int, _dp]
         L.orc_vp_dt.restype = C.c_double
+        L.orc_con_density.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.orc_get_dwdx.argtypes = [C.c_void_p, _dp]
         L.orc_monaghan_dt.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_monaghan_dt.restype = C.c_double
         _lib = L
@@ -196,6 +198,23 @@ class Oracle:
             raise RuntimeError(f"orc_sum_density failed ({rc})")
         return rho
 
+    def kernel_gradients(self) -> np.ndarray:
+        """dwdx (nip, 2) of the last single_step."""
+        g = np.zeros((self._L.orc_nip(self._c), 2))
+        self._L.orc_get_dwdx(self._c, g)
+        return g
+
+    def con_density(self, mass, vx) -> np.ndarray:
+        """con_density's drhodt with the last single_step's pairs and
+        gradients and the velocities ``vx`` (n, 2)."""
+        vx = np.ascontiguousarray(vx, dtype=np.float64)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        out = np.zeros(mass.shape[0])
+        rc = self._L.orc_con_density(self._c, mass.shape[0], vx, mass, out)
+        if rc:
+            raise RuntimeError(f"orc_con_density failed ({rc})")
+        return out
+
     @staticmethod
     def monaghan_dt(hsml, rho, dvxdt) -> float:
         """Monaghan_time_step without artificial viscosity
@@ -203,7 +222,7 @@ class Oracle:
         hsml = np.ascontiguousarray(hsml, dtype=np.float64)
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         dvxdt = np.ascontiguousarray(dvxdt, dtype=np.float64).reshape(-1, 2)
-        return float(lib().orc_monaghan_dt(hsml.size, _ptr(hsml), _ptr(rho), _ptr(dvxdt)))
+        return float(lib().orc_monaghan_dt(hsml.size, hsml, rho, dvxdt))
 
     def wind_at(self, sc, pts) -> np.ndarray:
         """The wind (u, v) the forcing term of ``sc`` sees at points

@@ -51,6 +51,9 @@ HARNESS_GPU_LEAP = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_leap")
 #: the harness built with the Monaghan time step (tse = 1)
 HARNESS_TSE1 = os.path.join(HERE, "_ref", "simp_ref_harness_tse1")
 HARNESS_GPU_TSE1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_tse1")
+#: the harness built with continuity density (den_alg = 2)
+HARNESS_DEN2 = os.path.join(HERE, "_ref", "simp_ref_harness_den2")
+HARNESS_GPU_DEN2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den2")
 #: the OpenMP build sized for the 250k workload (maxn = 300000), timings only
 HARNESS_OMP = os.path.join(HERE, "_ref", "simp_ref_harness_omp")
 #: the serial build with the same capacity

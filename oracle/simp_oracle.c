@@ -20,6 +20,9 @@
  *   - sum_density (den_alg = 1): golden vectors of the reference built with
  *     that PARAMETER, including the stale kernel_zero reads, plus the
  *     uniform-density fixed point.
+ *   - con_density (den_alg = 2): golden vectors of the reference built with
+ *     that PARAMETER, plus a zero rate for a uniform translation and
+ *     -rho div(v) for a uniform expansion on a lattice.
  */
 #include "simp_oracle.h"
 
@@ -715,6 +718,36 @@ int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho) {
     return ORC_OK;
 }
 
+/* ---- con_density, density.f90:107-165 ---------------------------------
+ * drhodt(i) = sum over the pairs of the last single_step, in list order,
+ * of mass(j) (vx_i - vx_j) . dwdx_k (and mass(i) times the same for j),
+ * with the velocities given (the module's at the time density() is
+ * called) and the gradients the last single_step left. */
+static int con_density(orc_ctx *c, int n, const double *vx, const double *mass,
+                       double *drhodt) {
+    if (n < 1) return ORC_E_ARG;
+    for (int i = 0; i < n; ++i) drhodt[i] = 0.;
+    for (int k = 0; k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+        double w1 = dv0 * c->dwdx[2 * k], w4 = dv1 * c->dwdx[2 * k + 1];
+        double v_ij = w1 + w4;
+        drhodt[i] = drhodt[i] + mass[j] * v_ij;
+        drhodt[j] = drhodt[j] + mass[i] * v_ij;
+    }
+    return ORC_OK;
+}
+
+int orc_con_density(orc_ctx *c, int n, const double *vx, const double *mass,
+                    double *drhodt) {
+    return con_density(c, n, vx, mass, drhodt);
+}
+
+/* dwdx(2, nip) of the last single_step (kernel.f90, the kernel routines) */
+void orc_get_dwdx(const orc_ctx *c, double *dwdx) {
+    memcpy(dwdx, c->dwdx, sizeof(double) * 2 * (size_t)c->nip);
+}
+
 /* VP_time_step (time_step.f90:155) */
 double orc_vp_dt(int n, const double *hsml) {
     double dt = 0.0;
@@ -757,12 +790,14 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                double *p_o, double *sig3_o) {
     if (n < 1 || nsteps < 0) return ORC_E_ARG;
     size_t n2 = (size_t)n * 2;
-    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 29);
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 31);
     if (!buf) return ORC_E_NOMEM;
     double *x1 = buf, *v1 = x1 + n2, *h1 = v1 + n2, *A1 = h1 + n;
     double *dxdt = A1 + n, *dvxdt = dxdt + n2, *dhdt = dvxdt + n2, *dAdt = dhdt + n;
     double *eps3 = dAdt + n, *sig3 = eps3 + 3 * (size_t)n;
     double *zeta = sig3 + 3 * (size_t)n, *eta = zeta + n, *pr = eta + n;
+    double *rho1 = pr + n, *drhodt = rho1 + n;
+    const int cden = prm->den_alg == 2;   /* con_density */
     double dt = *dt_io, t = *time_io;
     int it = *itimestep;
     int rc = ORC_OK;
@@ -773,6 +808,7 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
         memcpy(v1, vx, n2 * sizeof(double));
         memcpy(h1, h, (size_t)n * sizeof(double));
         memcpy(A1, A, (size_t)n * sizeof(double));
+        memcpy(rho1, rho, (size_t)n * sizeof(double));
         rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
@@ -787,9 +823,14 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                 h[i] = h[i] + dhdt[i] * dt;
                 A[i] = A[i] + dAdt[i] * dt;
                 if (A[i] > 1) A[i] = 1;
-                if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+                if (prm->den_alg == 3) rho[i] = h[i] * RHO_ICE;
             }
             if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+            if (cden) {
+                /* (:131-134) on the updated velocities */
+                if ((rc = con_density(c, n, vx, mass, drhodt))) break;
+                for (int i = 0; i < n; ++i) rho[i] = rho[i] + drhodt[i] * dt;
+            }
             goto end_of_step;
         }
         /* predict half step (:207-215), density (:218) */
@@ -801,9 +842,15 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             h[i] = h[i] + dhdt[i] * dt / 2;
             A[i] = A[i] + dAdt[i] * dt / 2;
             if (A[i] > 1) A[i] = 1;
-            if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;   /* diagnostic_density */
+            if (prm->den_alg == 3) rho[i] = h[i] * RHO_ICE;   /* diagnostic_density */
         }
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+        if (cden) {
+            /* (:221-224, :347-349): the half-step velocities, the step-start
+             * gradients */
+            if ((rc = con_density(c, n, vx, mass, drhodt))) break;
+            for (int i = 0; i < n; ++i) rho[i] = rho[i] + drhodt[i] * dt / 2;
+        }
         rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
@@ -819,14 +866,24 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
                 h[i] = h1[i] + dhdt[i] * dt;
                 A[i] = A1[i] + dAdt[i] * dt;
                 if (A[i] > 1) A[i] = 1;
-                if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+                if (prm->den_alg == 3) rho[i] = h[i] * RHO_ICE;
             }
             if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+            if (cden) {
+                /* (:375-377): the new velocities, the half-step gradients */
+                if ((rc = con_density(c, n, vx, mass, drhodt))) break;
+                for (int i = 0; i < n; ++i) rho[i] = rho1[i] + drhodt[i] * dt;
+            }
             goto end_of_step;
         }
         /* corrector (:236-262); with summation density, density() after
-         * the half-step values (:244) already changes rho */
+         * the half-step values (:244) already changes rho; with continuity
+         * density it takes the half-step velocities (vx is not updated yet)
+         * and the half-step gradients, rho_12 = rho_1 + drhodt dt/2
+         * (:245-248) and rho = 2 rho_12 - rho_1 (:265-268; the drhodt of
+         * the third density call is not used) */
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
+        if (cden && (rc = con_density(c, n, vx, mass, drhodt))) break;
         for (size_t i = 0; i < n2; ++i) {
             double x12 = x1[i] + dxdt[i] * dt / 2;
             double v12 = v1[i] + dvxdt[i] * dt / 2;
@@ -839,7 +896,11 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             h[i] = 2 * h12 - h1[i];
             A[i] = 2 * A12 - A1[i];
             if (A[i] > 1) A[i] = 1;
-            if (prm->den_alg != 1) rho[i] = h[i] * RHO_ICE;
+            if (prm->den_alg == 3) rho[i] = h[i] * RHO_ICE;
+            if (cden) {
+                double rho12 = rho1[i] + drhodt[i] * dt / 2;
+                rho[i] = 2 * rho12 - rho1[i];
+            }
         }
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
     end_of_step:

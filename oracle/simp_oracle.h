@@ -28,7 +28,8 @@ typedef struct orc_params {
     double current[2];   /* uniform ocean current (m/s); reference: (0, 0) */
     double max_hsml;     /* smoothing-length cap (initialize.f90:191) */
     int    den_alg;      /* 3: diagnostic rho = h rho_ice (density.f90:172-194);
-                            1: summation (density.f90:41-100) */
+                            1: summation (density.f90:41-100);
+                            2: continuity (density.f90:107-165) */
     int    grid_nx, grid_ny;
     double grid_x0, grid_y0, grid_dx, grid_dy;
     const double *grid_u;  /* grid_ny * grid_nx, row-major (y outer) */
@@ -111,6 +112,12 @@ int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho);
 void orc_wind_at(const orc_params *prm, int n, const double *x, double *uv);
 
 /* VP_time_step (time_step.f90:155). */
+/* con_density (density.f90:107-165): drhodt from the last single_step's
+ * pairs and gradients with the velocities vx (2, n). */
+int orc_con_density(orc_ctx *c, int n, const double *vx, const double *mass,
+                    double *drhodt);
+/* the pair gradients dwdx (2, nip) of the last single_step */
+void orc_get_dwdx(const orc_ctx *c, double *dwdx);
 double orc_vp_dt(int n, const double *hsml);
 /* Monaghan_time_step without artificial viscosity (time_step.f90:39-125)
  * from hsml, rho and the last evaluation's dvxdt (2, n). */

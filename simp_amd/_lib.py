@@ -39,7 +39,7 @@ EXPORTS = (
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
     "simp_snapshot_size", "simp_snapshot_save", "simp_snapshot_load",
-    "simp_snapshot_info",
+    "simp_snapshot_info", "simp_kernel_gradients",
 )
 
 
@@ -117,6 +117,7 @@ def lib():
     L.simp_batch_tune.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
     L.simp_search_stats.argtypes = [_vp, _vp, _vp, _vp]
     L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.simp_kernel_gradients.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
                             _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]
     L.simp_upload.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp, C.c_double,

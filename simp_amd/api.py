@@ -217,6 +217,20 @@ class Simp:
             out["kernel_zero"] = kz
         return out
 
+    def kernel_gradients(self, cap: Optional[int] = None) -> dict:
+        """The last single_step's pairs with their kernel gradients dwdx
+        (nip, 2) in the pair's orientation (what con_density reads on the
+        host)."""
+        cap = cap if cap is not None else 16 * max(getattr(self, "_n", 0), 4096) + 64
+        nip = C.c_int()
+        pi = np.zeros(cap, np.int32); pj = np.zeros(cap, np.int32); g = np.zeros((cap, 2))
+        st = lib().simp_kernel_gradients(self._h, cap, C.byref(nip), ptr(pi), ptr(pj), ptr(g))
+        if st == _lib.SIMP_E_CAPACITY and nip.value > cap:
+            return self.kernel_gradients(cap=nip.value)
+        check(st, "simp_kernel_gradients")
+        k = nip.value
+        return {"nip": k, "p_i": pi[:k], "p_j": pj[:k], "dwdx": g[:k]}
+
     # --------------------------------------------- device-resident stepping
     def upload(self, sc, itimestep: int = 0) -> None:
         self._n = sc.n

@@ -130,7 +130,7 @@ static int validate(const simp_config *c) {
   if ((c->rheology != 1 && c->rheology != 2) || c->bfa != 1) return SIMP_E_UNSUPPORTED;
   if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
-  if (c->skf < 1 || c->skf > 8 || (c->den_alg != 3 && c->den_alg != 1)) return SIMP_E_UNSUPPORTED;
+  if (c->skf < 1 || c->skf > 8 || c->den_alg < 1 || c->den_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->wind_mode != SIMP_WIND_UNIFORM && c->wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
   return SIMP_OK;
 }
@@ -321,6 +321,18 @@ int simp_kernel_values(simp_handle *h, int cap, int *nip, int *p_i, int *p_j, do
   if (rc == -3 && *nip <= cap) rc = 0;
   if (rc) return map_rc(rc);
   if (kernel_zero) rc = h->eng->download_kernel_zero(0, kernel_zero);
+  return map_rc(rc);
+}
+
+int simp_kernel_gradients(simp_handle *h, int cap, int *nip, int *p_i, int *p_j,
+                          double *dwdx) {
+  if (!h || cap < 0 || !nip) return SIMP_E_ARG;
+  if (h->eng->ninst() != 1 || scratch_state_lost(h)) return SIMP_E_STATE;
+  int bnip = 0;
+  int rc = h->eng->export_pairs(cap, nip, p_i, p_j, nullptr, nullptr, 0, &bnip, nullptr,
+                                nullptr, nullptr, nullptr, dwdx);
+  // only a particle-pair overflow is a capacity error here
+  if (rc == -3 && *nip <= cap) rc = 0;
   return map_rc(rc);
 }
 
@@ -614,7 +626,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.nb > 0 && (!d.xb || !d.massb)) return SIMP_E_ARG;
   if (d.tse < 0 || d.tse > 2) return SIMP_E_UNSUPPORTED;
   if (d.wind_mode != SIMP_WIND_UNIFORM && d.wind_mode != SIMP_WIND_GRID) return SIMP_E_UNSUPPORTED;
-  if (d.den_alg != 0 && d.den_alg != 1 && d.den_alg != 3) return SIMP_E_UNSUPPORTED;
+  if (d.den_alg < 0 || d.den_alg > 3) return SIMP_E_UNSUPPORTED;
   if (d.pa_sph != 0 && d.pa_sph != 1 && d.pa_sph != 2) return SIMP_E_UNSUPPORTED;
   if (d.eos_alg < 0 || d.eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (d.rheology < 0 || d.rheology > 2) return SIMP_E_UNSUPPORTED;
@@ -629,7 +641,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.dt = d.dt; in.time = d.current_time; in.itimestep = d.itimestep;
   in.nsteps = d.nsteps > 0 ? d.nsteps : 0;
   in.tse = d.tse;
-  in.den_alg = d.den_alg == 1 ? 1 : 3;
+  in.den_alg = d.den_alg == 0 ? 3 : d.den_alg;
   in.pa_sph = d.pa_sph == 2 ? 2 : 1;
   in.eos_alg = d.eos_alg == 0 ? 3 : d.eos_alg;
   in.rheology = d.rheology == 2 ? 2 : 1;

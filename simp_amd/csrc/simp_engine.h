@@ -127,7 +127,8 @@ class Engine {
   int download_diag(int inst, double *eps3, double *zeta, double *eta, double *p,
                     double *sig3);
   int export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, double *hij, int bcap,
-                   int *bnip, int *bpi, int *bpj, double *brij, double *w = nullptr);
+                   int *bnip, int *bpi, int *bpj, double *brij, double *w = nullptr,
+                   double *dwdx = nullptr);
   // kernel_zero of instance inst (n values)
   int download_kernel_zero(int inst, double *kz);
   // snapshots: the arrays of instance k a restart needs besides its state
@@ -192,6 +193,7 @@ class Engine {
   bool diag_ = false;
   bool any_boundary_ = false;
   bool any_xsw_ = false;              // some instance uses a non-default switch value
+  bool any_cden_ = false;             // some instance uses continuity density
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   bool force_search_ = false;

@@ -22,7 +22,7 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // an instance with a switch value beyond the reference's defaults: the
 // kernels then run their variants that read each instance's values
 static inline bool nondefault(const InstMeta &m) {
-  return m.den_alg == 1 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
+  return m.den_alg != 3 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
          m.tse == 1;
 }
 
@@ -539,6 +539,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // launched only when some instance uses one (each instance then follows
   // its own values)
   any_xsw_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return nondefault(m); });
+  any_cden_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.den_alg == 2; });
   if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
   if (ninst > 0) {
     // grid of the next evaluation of every instance
@@ -885,6 +886,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   ids_.assign(1, next_id_++);
   any_boundary_ = in.nb > 0;
   any_xsw_ = nondefault(m);
+  any_cden_ = m.den_alg == 2;
   // grid of the first evaluation
   launch_partial(d_, 0, st_);
   launch_finalize(d_, 0, st_);
@@ -961,14 +963,17 @@ void Engine::enqueue_step() {
   if (scheme_ == 1) {
     // euler_scheme: one evaluation of the step's state, then the update
     enqueue_eval(0, 3, diag_);
+    if (any_cden_) L("condens", [&] { launch_condens(d_, 1, st_); });
     L("euler", [&] { launch_euler(d_, st_); });
     L("finalize", [&] { launch_finalize(d_, 1, st_); });
     return;
   }
   // predictor_corrector and leapfrog_scheme share the half step
   enqueue_eval(0, 0, false);
+  if (any_cden_) L("condens", [&] { launch_condens(d_, 0, st_); });
   L("finalize", [&] { launch_finalize(d_, 0, st_); });
   enqueue_eval(1, scheme_ == 3 ? 4 : 1, diag_);
+  if (any_cden_ && scheme_ == 3) L("condens", [&] { launch_condens(d_, 2, st_); });
   L("finalize", [&] { launch_finalize(d_, 1, st_); });
 }
 
@@ -1359,7 +1364,8 @@ int Engine::download_kernel_zero(int k, double *kz) {
 }
 
 int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, double *hij,
-                         int bcap, int *bnip, int *bpi, int *bpj, double *brij, double *w) {
+                         int bcap, int *bnip, int *bpi, int *bpj, double *brij, double *w,
+                         double *dwdx) {
   // requires a prefix + passA (own counts) + pair ranks for state S
   const DevState &d = d_;
   enqueue_prefix(0);
@@ -1377,18 +1383,19 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
   *nip = n_ip;
   *bnip = n_bp;
   int *dpi = nullptr, *dpj = nullptr, *dbi = nullptr, *dbj = nullptr;
-  double *dr = nullptr, *dh = nullptr, *dbr = nullptr, *dw = nullptr;
+  double *dr = nullptr, *dh = nullptr, *dbr = nullptr, *dw = nullptr, *dg = nullptr;
   const int c1 = std::max(std::min(cap, n_ip), 1), c2 = std::max(std::min(bcap, n_bp), 1);
   bool ok = hipMalloc((void **)&dpi, c1 * 4) == hipSuccess &&
             hipMalloc((void **)&dpj, c1 * 4) == hipSuccess &&
             hipMalloc((void **)&dr, c1 * 8) == hipSuccess &&
             hipMalloc((void **)&dh, c1 * 8) == hipSuccess &&
             (!w || hipMalloc((void **)&dw, c1 * 8) == hipSuccess) &&
+            (!dwdx || hipMalloc((void **)&dg, c1 * 16) == hipSuccess) &&
             hipMalloc((void **)&dbi, c2 * 4) == hipSuccess &&
             hipMalloc((void **)&dbj, c2 * 4) == hipSuccess &&
             hipMalloc((void **)&dbr, c2 * 8) == hipSuccess;
   if (ok) {
-    launch_export_pairs(d, 0, std::min(cap, n_ip), dpi, dpj, dr, dh, dw, st_);
+    launch_export_pairs(d, 0, std::min(cap, n_ip), dpi, dpj, dr, dh, dw, dg, st_);
     launch_export_bpairs(d, 0, any_boundary_ ? meta_[0].nb : 0, std::min(bcap, n_bp), dbi, dbj,
                          dbr, st_);
     ok = hipStreamSynchronize(st_) == hipSuccess;
@@ -1399,6 +1406,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
       if (rij) ok &= hipMemcpy(rij, dr, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
       if (hij) ok &= hipMemcpy(hij, dh, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
       if (w) ok &= hipMemcpy(w, dw, k1 * 8, hipMemcpyDeviceToHost) == hipSuccess;
+      if (dwdx) ok &= hipMemcpy(dwdx, dg, k1 * 16, hipMemcpyDeviceToHost) == hipSuccess;
     }
     if (ok && k2 > 0) {
       if (bpi) ok &= hipMemcpy(bpi, dbi, k2 * 4, hipMemcpyDeviceToHost) == hipSuccess;
@@ -1406,7 +1414,7 @@ int Engine::export_pairs(int cap, int *nip, int *pi, int *pj, double *rij, doubl
       if (brij) ok &= hipMemcpy(brij, dbr, k2 * 8, hipMemcpyDeviceToHost) == hipSuccess;
     }
   }
-  hipFree(dpi); hipFree(dpj); hipFree(dr); hipFree(dh); hipFree(dw);
+  hipFree(dpi); hipFree(dpj); hipFree(dr); hipFree(dh); hipFree(dw); hipFree(dg);
   hipFree(dbi); hipFree(dbj); hipFree(dbr);
   if (!ok) return -2;
   if (n_ip > cap || n_bp > bcap) return -3;

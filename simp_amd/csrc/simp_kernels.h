@@ -30,8 +30,12 @@ void launch_ranks(const DevState &d, int buf, hipStream_t st);
 // (then launch_euler), 4 second leap-frog stage
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st);
 void launch_euler(const DevState &d, hipStream_t st);
+// continuity density after an update (instances with den_alg = 2): mode 0
+// after the predictor, 1 before the Euler update, 2 after the leap-frog's
+// second evaluation
+void launch_condens(const DevState &d, int mode, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
-                         double *hij, double *w, hipStream_t st);
+                         double *hij, double *w, double *dwdx, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st);
 // one instance laid out in caller memory: its block table (d.blk, d.nblk),

@@ -1970,6 +1970,10 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // the second call produced (pass A's dsum)
     // (FP64 in both precision modes, as in pass A)
     const bool sumd3 = XSW && phase == 1 && m.den_alg == 1;
+    // continuity density in the corrector (time_integration.f90:245-248,
+    // 265-268): drhodt on the half-step velocities and gradients, summed
+    // with the stress (FP64 in both precision modes)
+    const bool cd = XSW && phase == 1 && m.den_alg == 2;
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
     const bool sd = XSW && m.den_alg == 1;
@@ -2021,7 +2025,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       if (An > 1) An = 1;
       d.h[2 * cs + 0][p] = hn;
       d.A[2 * cs + 0][p] = An;
-      if (!sumd3) {
+      if (!sumd3 && !cd) {
         const double rn = hn * kRhoIce;
         d.rho[2 * cs + 0][p] = rn;
         // Marquis_hsml (smoothing_length.f90:136-142)
@@ -2045,13 +2049,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       if (An > 1) An = 1;
       d.h[2 * cs + 0][p] = hn;
       d.A[2 * cs + 0][p] = An;
-      const double rn = sd ? dsi : hn * kRhoIce;
-      d.rho[2 * cs + 0][p] = rn;
-      double hs = sqrt(mi / rn) * kHsmlMult;
-      if (hs > m.max_hsml) hs = m.max_hsml;
-      d.hsml[2 * cs + 0][p] = hs;
-      hsml_new = hs;
-      rho_new = rn;
+      // (continuity density: k_condens takes the density, hsml and the
+      // partials from here, on the new velocities)
+      if (!(XSW && m.den_alg == 2)) {
+        const double rn = sd ? dsi : hn * kRhoIce;
+        d.rho[2 * cs + 0][p] = rn;
+        double hs = sqrt(mi / rn) * kHsmlMult;
+        if (hs > m.max_hsml) hs = m.max_hsml;
+        d.hsml[2 * cs + 0][p] = hs;
+        hsml_new = hs;
+        rho_new = rn;
+      }
     }
     const bool listed = nn <= kNbrMax;
     int sj[kB];
@@ -2071,6 +2079,14 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     if (sumd3) den3 = kzi * mi / dsi;
     const int skf = XSW ? m.skf : 6;
     const double ksc = XSW ? kern_scale(skf) : 1.0;
+    double drho = 0.;
+    auto cden_pair = [&](double dx, double dy, double mh, double mq, double uq, double vq) {
+      const double r = sqrt(dx * dx + dy * dy);
+      const double gg = kern_grad(skf, r, mh);
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
+      const double w1 = (vxi - uq) * dw0, w4 = (vyi - vq) * dw1;
+      drho = drho + mq * (w1 + w4);
+    };
     auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
       const double w = kern_w(skf, sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
@@ -2114,15 +2130,17 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     if (listed) {
       for (int j0 = 0; j0 < nn; j0 += kB) {
         double xq[kB], yq[kB], hq[kB], rq[kB], mq[kB], r2[kB];
-        double q11[kB], q12[kB], q22[kB];
+        double q11[kB], q12[kB], q22[kB], uq[kB], vq[kB];
         float4 qr[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
           const int t = sj[u];
           xq[u] = X[t]; yq[u] = Y[t]; hq[u] = HS[t];
+          uq[u] = cd ? VX[t] : 0.0;
+          vq[u] = cd ? VY[t] : 0.0;
           if (MIXED) {
             qr[u] = d.srec[t];
-            mq[u] = sumd3 ? MS[t] : 0.0;
+            mq[u] = (sumd3 || cd) ? MS[t] : 0.0;
           } else {
             rq[u] = RS[t]; mq[u] = MS[t];
             q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
@@ -2151,6 +2169,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             stress_pair(dx, dy, sqrt(dx * dx + dy * dy), mh, rq[u], mq[u], q11[u], q12[u],
                         q22[u]);
           if (sumd3) dens_pair(dx, dy, mh, mq[u], r2[u]);
+          if (cd) cden_pair(dx, dy, mh, mq[u], uq[u], vq[u]);
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) sj[u] = j1 + u < nn ? nx[u] : self;
@@ -2170,6 +2189,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           else
             stress_pair(dx, dy, r, mh, RS[s], MS[s], S11[s], S12[s], S22[s]);
           if (sumd3) dens_pair(dx, dy, mh, MS[s], d.dsum[s]);
+          if (cd) cden_pair(dx, dy, mh, MS[s], VX[s], VY[s]);
         }
       }
     }
@@ -2328,10 +2348,26 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         d.hsml[2 * cs + 0][p] = hs;
         hsml_new = hs;
         rho_new = rn;
+      } else if (cd) {
+        // rho_12 = rho_1 + drhodt dt/2, rho = 2 rho_12 - rho_1, Marquis_hsml
+        const double r1 = d.rho[2 * cs + 0][p];
+        const double r12 = r1 + drho * dt / 2;
+        const double rn = 2 * r12 - r1;
+        d.rho[2 * cs + 0][p] = rn;
+        double hs = sqrt(mi / rn) * kHsmlMult;
+        if (hs > m.max_hsml) hs = m.max_hsml;
+        d.hsml[2 * cs + 0][p] = hs;
+        hsml_new = hs;
+        rho_new = rn;
       }
     } else if (phase == 4) {
       d.vx[2 * cs + 0][p] = u1 + ax * dt;
       d.vy[2 * cs + 0][p] = v1 + ay * dt;
+      if (XSW && m.den_alg == 2) {
+        // for k_condens's Monaghan time step terms
+        d.tend[2 * (size_t)d.ntot + p] = ax;
+        d.tend[3 * (size_t)d.ntot + p] = ay;
+      }
     } else {
       const int N = d.ntot;
       d.tend[p] = vxi;
@@ -2348,8 +2384,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   }
   // (with every lane of the wave: a wave holds one instance)
   if (phase == 2 || y.itimestep % kPrintStep == 0) stats_add(d.dyn[k], live, st_cnt, st_own);
-  if constexpr (XSW && (phase == 1 || phase == 4))
-    block_partials<kParts>(d, k, live, px_new, py_new, hsml_new, speed, dhs, mfo, mfc);
+  if constexpr (XSW && (phase == 1 || phase == 4)) {
+    // (block-uniform: k_condens writes a leap-frog continuity-density
+    // instance's partials)
+    if (!(phase == 4 && m.den_alg == 2))
+      block_partials<kParts>(d, k, live, px_new, py_new, hsml_new, speed, dhs, mfo, mfc);
+  }
   else if constexpr (phase != 2 && phase != 3)
     block_partials(d, k, live, px_new, py_new, hsml_new, speed, dhs);
 }
@@ -2380,7 +2420,8 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
     const double hn = d.h[2 * cs][p] + T[4 * N + p] * dt;
     double An = d.A[2 * cs][p] + T[5 * N + p] * dt;
     if (An > 1) An = 1;
-    const double rn = m.den_alg == 1 ? d.dsum[p] : hn * kRhoIce;
+    // (summation density: pass A's sum; continuity density: k_condens's)
+    const double rn = m.den_alg != 3 ? d.dsum[p] : hn * kRhoIce;
     hs = sqrt(d.mass[cs][p] / rn) * kHsmlMult;
     if (hs > m.max_hsml) hs = m.max_hsml;
     sp = sqrt(vx * vx + vy * vy);
@@ -2399,13 +2440,119 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
   block_partials<kParts>(d, k, live, xn, yn, hs, sp, dh, fo, fc);
 }
 
+// con_density (density.f90:107-165) after an update, for the instances with
+// den_alg = 2: drhodt = sum over the evaluation's pairs, in the order the
+// reference's list reaches the particle, of m_j (v_i - v_j) . dwdx_ij, with
+// the evaluation's gradients and the velocities the update produced (FP64 in
+// both precision modes; the product does not depend on which particle of the
+// pair comes first).
+//   MODE 0, after the predictor (time_integration.f90:221-224, :347-349):
+//     gradients at the step-start state, half-step velocities;
+//     rho_T = rho_S + drhodt dt / 2.
+//   MODE 1, after the Euler evaluation and before its in-place update
+//     (:131-134): velocities v + dvxdt dt from the evaluation's tendencies
+//     (the update's own expression); dsum = rho + drhodt dt for k_euler.
+//   MODE 2, after the leap-frog's second evaluation (:375-377): gradients at
+//     the half-step state, the new velocities; rho = rho_1 + drhodt dt, then
+//     Marquis_hsml and the block's partials (which pass B leaves to this
+//     kernel for such instances).
+template <int MODE>
+__global__ void __launch_bounds__(kTPB) k_condens(DevState d) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
+  const BlockEntry be = d.blk[blockIdx.x];
+  const int k = be.inst;
+  const InstMeta &m = d.meta[k];
+  if (m.den_alg != 2) return;   // (block-uniform)
+  const InstDyn &y = d.dyn[k];
+  const bool ok = (int)threadIdx.x < be.nloc;
+  const bool live = ok && !inst_off(y);
+  const int self = be.pos + (ok ? (int)threadIdx.x : 0);
+  const int cs = be.cs;
+  const int eb = MODE == 2 ? 1 : 0;   // buffer of the evaluated state
+  const int vb = MODE == 0 ? 1 : 0;   // buffer of the velocities
+  const double *X = d.x[2 * cs + eb], *Y = d.y[2 * cs + eb], *HS = d.hsml[2 * cs + eb];
+  const double *VX = d.vx[2 * cs + vb], *VY = d.vy[2 * cs + vb], *MS = d.mass[cs];
+  const size_t N = (size_t)d.ntot;
+  const double dt = y.dt;
+  double px = 0., py = 0., hs = 1.0, sp = 0., dh = 0., fo = 0., fc = 0.;
+  if (live) {
+    // the velocity of slot t after the update
+    auto vel = [&](int t, double &u, double &v) {
+      u = VX[t];
+      v = VY[t];
+      if (MODE == 1) {
+        u = u + d.tend[2 * N + t] * dt;
+        v = v + d.tend[3 * N + t] * dt;
+      }
+    };
+    const int skf = m.skf;
+    const double ksc = kern_scale(skf);
+    const double xi = X[self], yi = Y[self], hi = HS[self];
+    double ui, vi;
+    vel(self, ui, vi);
+    double drho = 0.;
+    auto pair = [&](int t, double dx, double dy, double r, double mh) {
+      double uq, vq;
+      vel(t, uq, vq);
+      const double gg = kern_grad(skf, r, mh);
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
+      const double w1 = (ui - uq) * dw0, w4 = (vi - vq) * dw1;
+      drho = drho + MS[t] * (w1 + w4);
+    };
+    const int nn = d.nnb[self];
+    if (nn <= kNbrMax) {
+      for (int j = 0; j < nn; ++j) {
+        const int t = d.nbr[j * N + self];
+        const double dx = xi - X[t], dy = yi - Y[t];
+        pair(t, dx, dy, sqrt(dx * dx + dy * dy), (hi + HS[t]) / 2.);
+      }
+    } else {
+      const int W = y.W;
+      const int c = d.cellid[cs][self];
+      const int cy = c / W, cx = c - cy * W;
+      const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+      for (int g = 0; g < 6; ++g) {
+        for (int t = R.a[g]; t < R.b[g]; ++t) {
+          if (t == self) continue;
+          const double dx = xi - X[t], dy = yi - Y[t];
+          const double mh = (hi + HS[t]) / 2.;
+          double r;
+          if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          pair(t, dx, dy, r, mh);
+        }
+      }
+    }
+    const double rho0 = d.rho[2 * cs + 0][self];
+    if (MODE == 0) {
+      d.rho[2 * cs + 1][self] = rho0 + drho * dt / 2;
+    } else if (MODE == 1) {
+      d.dsum[self] = rho0 + drho * dt;
+    } else {
+      const double rn = rho0 + drho * dt;
+      d.rho[2 * cs + 0][self] = rn;
+      // Marquis_hsml (smoothing_length.f90:136-142)
+      hs = sqrt(MS[self] / rn) * kHsmlMult;
+      if (hs > m.max_hsml) hs = m.max_hsml;
+      d.hsml[2 * cs + 0][self] = hs;
+      px = d.x[2 * cs + 0][self];
+      py = d.y[2 * cs + 0][self];
+      const double ue = d.vx[2 * cs + 1][self], ve = d.vy[2 * cs + 1][self];
+      sp = sqrt(ue * ue + ve * ve);
+      dh = fabs(hs - hi);
+      if (m.tse == 1) monaghan_terms(hs, rn, d.tend[2 * N + self], d.tend[3 * N + self], fo, fc);
+    }
+  }
+  if (MODE == 2) block_partials<kParts>(d, k, live, px, py, hs, sp, dh, fo, fc);
+}
+
 // ------------------------------------------------------- pair-list export
 
 // Particle pairs owned by p at their ranks in the reference's order
-// (nnps.f90:259-411): p_i, p_j 1-based, r_ij, hsml_ij.
+// (nnps.f90:259-411): p_i, p_j 1-based, r_ij, hsml_ij, and optionally the
+// kernel values w and gradients dwdx (2 per pair, dx = x(p_i) - x(p_j)).
 __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int cap, int *pi,
                                                        int *pj, double *rij, double *hij,
-                                                       double *w) {
+                                                       double *w, double *dwdx) {
   int p = 0, k = 0;
   if (!particle_of(d, p, k)) return;
   const InstDyn &y = d.dyn[k];
@@ -2438,6 +2585,11 @@ __global__ void __launch_bounds__(kTPB) k_export_pairs(DevState d, int buf, int 
         rij[rank] = r;
         hij[rank] = mh;
         if (w) w[rank] = kern_w(m.skf, r, mh);
+        if (dwdx) {
+          const double gg = kern_grad(m.skf, r, mh), ksc = kern_scale(m.skf);
+          dwdx[2 * rank] = gg * (dx / r) * ksc;
+          dwdx[2 * rank + 1] = gg * (dy / r) * ksc;
+        }
       }
       ++rank;
     }
@@ -2654,6 +2806,14 @@ static void launch_passB_t(const DevState &d, int phase, int bb, hipStream_t st)
 void launch_euler(const DevState &d, hipStream_t st) {
   k_euler<<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
+void launch_condens(const DevState &d, int mode, hipStream_t st) {
+  if (mode == 0)
+    k_condens<0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else if (mode == 1)
+    k_condens<1><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else
+    k_condens<2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st) {
   if (mixed && xsw)
     launch_passB_t<true, true>(d, phase, bb, st);
@@ -2665,8 +2825,8 @@ void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipS
     launch_passB_t<false, false>(d, phase, bb, st);
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
-                         double *hij, double *w, hipStream_t st) {
-  k_export_pairs<<<g8(d.nblk), kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w);
+                         double *hij, double *w, double *dwdx, hipStream_t st) {
+  k_export_pairs<<<g8(d.nblk), kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w, dwdx);
 }
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st) {

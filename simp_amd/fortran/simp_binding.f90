@@ -64,6 +64,15 @@ module simp_binding
             real(c_double), intent(out) :: w(*), kernel_zero(*)
         end function simp_kernel_values
 
+        integer(c_int) function simp_kernel_gradients(h, cap, nip, p_i, p_j, &
+                dwdx) bind(C, name='simp_kernel_gradients')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+            integer(c_int), value :: cap
+            integer(c_int), intent(out) :: nip, p_i(*), p_j(*)
+            real(c_double), intent(out) :: dwdx(*)
+        end function simp_kernel_gradients
+
         integer(c_int) function simp_upload(h, n, x, vx, mass, rho, hsml,   &
                 hth, A, nb, xb, massb, dt, current_time, itimestep)          &
                 bind(C, name='simp_upload')

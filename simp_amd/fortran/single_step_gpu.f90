@@ -9,8 +9,10 @@
 !        current_time, itimestep, max_hsml
 ! Writes dxdt, dvxdt, dhdt, dAdt, epsilon_11/12/22, zeta, eta, p,
 !        sigma_11/12/22, nip, bnip; with den_alg = 1 also p_i, p_j, w and
-!        kernel_zero, which sum_density (density.f90:41-100) reads next
-! The other pair arrays (r_ij, hsml_ij, dwdx, boundary pairs) stay on the
+!        kernel_zero, which sum_density (density.f90:41-100) reads next;
+!        with den_alg = 2 p_i, p_j and dwdx, which con_density
+!        (density.f90:107-165) reads
+! The other pair arrays (r_ij, hsml_ij, boundary pairs) stay on the
 ! device; no active caller reads them on the host.
 !---------------------------------------------------------------------------
 subroutine single_step
@@ -60,5 +62,9 @@ subroutine single_step
         st = simp_kernel_values(handle, size(w), nipc, p_i, p_j, w,       &
                                 kernel_zero)
         call simp_check(st, 'single_step kernel values')
+    elseif (den_alg == 2) then
+        ! con_density reads the pairs and their gradients
+        st = simp_kernel_gradients(handle, size(p_i), nipc, p_i, p_j, dwdx)
+        call simp_check(st, 'single_step kernel gradients')
     endif
 end subroutine single_step

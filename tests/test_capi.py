@@ -47,7 +47,7 @@ def test_config_defaults_are_the_reference_switches():
 
 
 @pytest.mark.parametrize("field,value", [("skf", 0), ("skf", 9), ("rheology", 3), ("tse", 3),
-                                         ("den_alg", 2), ("int_algorithm", 0),
+                                         ("den_alg", 4), ("int_algorithm", 0),
                                          ("int_algorithm", 4),
                                          ("precision", 7), ("pa_sph", 3), ("eos_alg", 0),
                                          ("eos_alg", 4)])

@@ -34,7 +34,8 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          # single_step
          ("edge3600_pc10_euler.npz", refrun.HARNESS_EULER, refrun.HARNESS_GPU_EULER),
          ("edge3600_pc10_leap.npz", refrun.HARNESS_LEAP, refrun.HARNESS_GPU_LEAP),
-         ("edge3600_pc10_tse1.npz", refrun.HARNESS_TSE1, refrun.HARNESS_GPU_TSE1)] + [
+         ("edge3600_pc10_tse1.npz", refrun.HARNESS_TSE1, refrun.HARNESS_GPU_TSE1),
+         ("edge3600_pc10_den2.npz", refrun.HARNESS_DEN2, refrun.HARNESS_GPU_DEN2)] + [
          # the quadratic, Gaussian, quintic-spline and Marquis kernels
          (f"edge3600_pc10_k{k}.npz", refrun.HARNESS_SKF[k], refrun.HARNESS_GPU_SKF[k])
          for k in (1, 2, 3, 7, 8)]

@@ -68,7 +68,7 @@ def test_switch_single_step_fp64_bit_exact_vs_reference_golden(tag):
 
 
 @pytest.mark.parametrize("tag", ["mc", "c2", "c4", "k1", "k2", "k3", "k7", "k8", "euler", "leap",
-                                 "tse1"])
+                                 "tse1", "den2"])
 def test_switch_time_integration_fp64_vs_reference_golden(tag):
     say(f"time_integration fp64 {tag} golden edge3600")
     g = load(f"edge3600_pc10_{tag}.npz")
@@ -92,7 +92,7 @@ def test_switch_time_integration_fp64_vs_reference_golden(tag):
 
 
 @pytest.mark.parametrize("scheme", [1, 3])
-@pytest.mark.parametrize("den_alg", [3, 1])
+@pytest.mark.parametrize("den_alg", [3, 1, 2])
 def test_schemes_with_both_densities_vs_oracle(scheme, den_alg):
     say(f"time_integration fp64 int_algorithm={scheme} den_alg={den_alg} box2500, 12 steps")
     sc = S.box(seed=41)
@@ -109,7 +109,8 @@ def test_schemes_with_both_densities_vs_oracle(scheme, den_alg):
 
 
 @pytest.mark.parametrize("sw", [{"rheology": 2}, {"skf": 4}, {"skf": 5}, {"skf": 1},
-                                {"skf": 3}, {"int_algorithm": 1}, {"int_algorithm": 3}])
+                                {"skf": 3}, {"int_algorithm": 1}, {"int_algorithm": 3},
+                                {"den_alg": 2}, {"den_alg": 2, "int_algorithm": 3}])
 def test_switch_mixed_precision_within_tolerance(sw):
     say(f"time_integration mixed {sw} vortex 40k, 20 steps")
     sc = S.vortex()

@@ -100,7 +100,7 @@ def test_rheology_and_kernel_switches_single_step_bit_exact_vs_reference(tag):
                                   "edge3600_pc10_k1.npz", "edge3600_pc10_k2.npz",
                                   "edge3600_pc10_k3.npz", "edge3600_pc10_k7.npz",
                                   "edge3600_pc10_k8.npz", "edge3600_pc10_eos1.npz",
-                                  "edge3600_pc10_tse1.npz"])
+                                  "edge3600_pc10_tse1.npz", "edge3600_pc10_den2.npz"])
 def test_time_integration_bit_exact_vs_reference(name):
     g = load(name)
     sc = scenario(g)
@@ -462,3 +462,50 @@ def test_summation_density_keeps_a_uniform_density():
     rho2 = o.sum_density(sc.mass, sc.rho)
     assert rho2.min() >= sc.rho.min() and rho2.max() <= sc.rho.max()
     assert np.std(rho2) < np.std(sc.rho)
+
+
+def test_continuity_density_rates_closed_form():
+    """con_density (density.f90:107-165) on the pairs of a uniform lattice:
+    a uniform translation gives drhodt = 0 exactly; a uniform expansion
+    v = a x gives the continuity equation's -rho div(v) = -2 a rho in the
+    lattice's interior, within the SPH quadrature error; reversing the
+    expansion flips the sign."""
+    sc = S.box(nside=30, spacing=1000.0, seed=1)
+    g = np.arange(30) * 1000.0 + 500.0
+    sc.x = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2).copy()
+    sc.hsml[:] = 4000.0                   # ~50 neighbours: a fine quadrature
+    sc.max_hsml = 1e5
+    sc.rho[:] = 900.0
+    sc.mass[:] = 900.0 * 1000.0 ** 2      # rho times the lattice cell
+    o = capi.Oracle()
+    o.single_step(sc)
+    v = np.tile([0.3, -0.2], (sc.n, 1))
+    assert np.all(o.con_density(sc.mass, v) == 0.0)
+    a = 1e-6
+    dr = o.con_density(sc.mass, a * sc.x)
+    interior = np.all((sc.x > 5000.0) & (sc.x < 25000.0), axis=1)
+    rel = dr[interior] / (-2 * a * 900.0)
+    assert np.all(np.abs(rel - 1.0) < 0.01), (rel.min(), rel.max())
+    assert np.array_equal(o.con_density(sc.mass, -a * sc.x), -dr)
+
+
+def test_monaghan_time_step_closed_form():
+    """Monaghan_time_step without artificial viscosity (time_step.f90:79-121)
+    on hand-made values: the force limit 0.25 sqrt(hsml/|a|) at the particle
+    with the smallest hsml/|a| (|a| = 5 from a 3-4-5 triangle), the sound
+    crossing hsml/c with c = sqrt(Y (1 - nu) / (rho (1 + nu) (1 - 2 nu)))
+    = sqrt(1e9 0.7 / (900 1.3 0.4)) ~ 1223 m/s when it is smaller, and a
+    zero acceleration that takes no part (hsml / 0 = inf)."""
+    hsml = np.array([100.0, 400.0, 50.0])
+    rho = np.full(3, 900.0)
+    acc = np.array([[3.0, 4.0], [0.0, 0.0], [30.0, 40.0]])
+    nu = float(np.float32(0.3))
+    c = math.sqrt(1e9 * (1 - nu) / (900.0 * (1 + nu) * (1 - 2 * nu)))
+    assert 1220.0 < c < 1226.0
+    # hsml/|a| = 20, inf, 1 -> 0.25 sqrt(1) = 0.25 s; hsml/c = 50/c ~ 0.041 s
+    assert capi.Oracle.monaghan_dt(hsml, rho, acc) == 50.0 / c
+    # with a much smaller acceleration limit the force term decides
+    acc2 = acc * 1e6
+    assert capi.Oracle.monaghan_dt(hsml, rho, acc2) == 0.25 * math.sqrt(50.0 / 5e7)
+    # all at rest: only the sound crossing remains
+    assert capi.Oracle.monaghan_dt(hsml, rho, np.zeros((3, 2))) == 50.0 / c
