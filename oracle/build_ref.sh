@@ -64,12 +64,18 @@ echo "build_ref: $OUT/simp_ref_init"
 # reference configures by editing globals.f90 (README.md:68-70); the edit is
 # applied on the way into the compiler (stdin), so no copy of the source is
 # written, and everything that uses the module is rebuilt against it.
-variant() {   # <tag> <PARAMETER line in globals.f90> <replacement>; sets vobjs
-    local V="$OUT/$1" from="$2" to="$3"
+variant() {   # <tag> (<PARAMETER text in globals.f90> <replacement>)...; sets vobjs
+    local tag="$1" V="$OUT/$1"
+    shift
     mkdir -p "$V/obj" "$V/mod"
     vobjs=()
-    grep -q "$from" "$SRC/modules/globals.f90"   # the line the edit targets
-    sed "s/$from/$to/" "$SRC/modules/globals.f90" \
+    local edits=()
+    while [ $# -ge 2 ]; do
+        grep -q "$1" "$SRC/modules/globals.f90"   # the text the edit targets
+        edits+=(-e "s/$1/$2/")
+        shift 2
+    done
+    sed "${edits[@]}" "$SRC/modules/globals.f90" \
         | "$FC" $FFLAGS -x f95 -J"$V/mod" -c - -o "$V/obj/globals.o"
     vobjs+=("$V/obj/globals.o")
     for m in data_variables domain boundary_domain mpi_variables tree_module interfaces; do
@@ -84,9 +90,10 @@ variant() {   # <tag> <PARAMETER line in globals.f90> <replacement>; sets vobjs
         done
     done
     "$FC" $FFLAGS -I"$V/mod" -J"$V/mod" -c "$HERE/ref_harness.f90" -o "$V/obj/ref_harness.o"
-    "$FC" $FFLAGS -o "$OUT/simp_ref_harness_$1" "$V/obj/ref_harness.o" "${vobjs[@]}" \
+    "$FC" $FFLAGS -o "$OUT/simp_ref_harness_$tag" "$V/obj/ref_harness.o" "${vobjs[@]}" \
         -L/opt/rocm/llvm/lib -lomp -Wl,-rpath,/opt/rocm/llvm/lib
-    echo "build_ref: $OUT/simp_ref_harness_$1"
+    echo "build_ref: $OUT/simp_ref_harness_$tag"
+    eval "vobjs_$tag=(\"\${vobjs[@]}\")"
 }
 D1="$OUT/den1"
 # summation density (den_alg = 1, density.f90:41-100)
@@ -138,6 +145,29 @@ d2objs=("${vobjs[@]}")
 variant tse1 'parameter(tse = 2)' 'parameter(tse = 1)'
 T1="$OUT/tse1"
 t1objs=("${vobjs[@]}")
+# smoothing length kept (sle = 0) and Benz's rate (sle = 1;
+# smoothing_length.f90:21-111), the latter also with the Euler scheme and
+# with continuity density
+variant sle0 'parameter(sle = 2)' 'parameter(sle = 0)'
+variant sle1 'parameter(sle = 2)' 'parameter(sle = 1)'
+variant sle1eu 'parameter(sle = 2)' 'parameter(sle = 1)' \
+    'parameter(int_algorithm = 2)' 'parameter(int_algorithm = 1)'
+variant sle1d2 'parameter(sle = 2)' 'parameter(sle = 1)' \
+    'parameter(den_alg = 3)' 'parameter(den_alg = 2)'
+# XSPH (regularization.f90:45-95; with it regularization also runs the
+# artificial viscosity art_visc_alg selects, :29-35) and the artificial
+# viscosities of Hernquist and Katz (:102-213) and Monaghan (:220-308),
+# the latter also with the Monaghan time step, which then adds the
+# artificial viscosity's CFL terms (time_step.f90:89-111)
+variant xsph 'xsph_regularization = .FALSE.' 'xsph_regularization = .TRUE.'
+variant avhk 'artificial_visc   = .FALSE.' 'artificial_visc   = .TRUE.'
+variant avmon 'artificial_visc   = .FALSE.' 'artificial_visc   = .TRUE.' \
+    'parameter(art_visc_alg = 1)' 'parameter(art_visc_alg = 2)'
+variant avtse1 'artificial_visc   = .FALSE.' 'artificial_visc   = .TRUE.' \
+    'parameter(tse = 2)' 'parameter(tse = 1)'
+# the Wang and Marquis boundary forces (bfa = 2, 3; boundary_force.f90:109-224)
+variant bfa2 'parameter(bfa = 1)' 'parameter(bfa = 2)'
+variant bfa3 'parameter(bfa = 1)' 'parameter(bfa = 3)'
 
 # The OpenMP build, for the CPU baseline: the reference compiled as its
 # Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
@@ -221,5 +251,8 @@ if [ -f "$LIB" ]; then
     gpu_harness "$D2/obj" "$D2/mod" "$OUT/simp_ref_harness_gpu_den2" "${d2objs[@]}"
     for k in 1 2 3 7 8; do
         eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
+    done
+    for t in sle0 sle1 sle1eu sle1d2 xsph avhk avmon avtse1 bfa2 bfa3; do
+        eval "gpu_harness \"$OUT/$t/obj\" \"$OUT/$t/mod\" \"$OUT/simp_ref_harness_gpu_$t\" \"\${vobjs_$t[@]}\""
     done
 fi

@@ -24,7 +24,9 @@ class OrcParams(C.Structure):
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p), ("pa_sph", C.c_int),
                 ("eos_alg", C.c_int), ("rheology", C.c_int), ("skf", C.c_int),
-                ("int_algorithm", C.c_int)]
+                ("int_algorithm", C.c_int), ("sle", C.c_int), ("xsph", C.c_int),
+                ("art_visc", C.c_int), ("art_visc_alg", C.c_int), ("bfa", C.c_int),
+                ("b_normal", C.c_void_p), ("step_dt", C.c_double)]
 
 
 def build() -> None:
@@ -67,6 +69,8 @@ def lib():
         L.orc_get_dwdx.argtypes = [C.c_void_p, _dp]
         L.orc_monaghan_dt.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_monaghan_dt.restype = C.c_double
+        L.orc_sound_speed.argtypes = [C.c_double]
+        L.orc_sound_speed.restype = C.c_double
         _lib = L
     return _lib
 
@@ -89,7 +93,15 @@ def make_params(sc) -> tuple:
     p.rheology = getattr(sc, "rheology", 1)
     p.skf = getattr(sc, "skf", 6)
     p.int_algorithm = getattr(sc, "int_algorithm", 2)
-    keep = ()
+    p.sle = getattr(sc, "sle", 2)
+    p.xsph = getattr(sc, "xsph", 0)
+    p.art_visc = getattr(sc, "art_visc", 0)
+    p.art_visc_alg = getattr(sc, "art_visc_alg", 1)
+    p.bfa = getattr(sc, "bfa", 1)
+    p.step_dt = sc.dt
+    bn = np.ascontiguousarray(sc.b_normal(), dtype=np.float64)
+    p.b_normal = bn.ctypes.data if bn.size else None
+    keep = (bn,)
     if sc.wind_grid is not None:
         g = sc.wind_grid
         u = np.ascontiguousarray(g.u, dtype=np.float64)
@@ -99,7 +111,7 @@ def make_params(sc) -> tuple:
         p.grid_x0, p.grid_y0, p.grid_dx, p.grid_dy = g.x0, g.y0, g.dx, g.dy
         p.grid_u = u.ctypes.data
         p.grid_v = v.ctypes.data
-        keep = (u, v)
+        keep = keep + (u, v)
     return p, keep
 
 

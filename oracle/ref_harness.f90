@@ -15,7 +15,10 @@
 !   int32  n, nb, reps     reps: number of consecutive single_step calls
 !   real8  dt, current_time, integration_time, max_hsml
 !   real8  x(2,n), vx(2,n), mass(n), rho(n), hsml(n), A(n), h(n)
-!   real8  xb(2,nb), massb(nb)
+!   real8  xb(2,nb), massb(nb), b_normal(2,nb)
+! The sound speed is set from the densities read, as initialize does after
+! setting them (initialize.f90:199-200); sound_speed computes it only for
+! the switches that read it (sound_speed.f90:21).
 ! Output (stream):
 !   int32  nip, bnip, itimestep, pad
 !   real8  dt, current_time, wall_seconds
@@ -50,7 +53,7 @@ program simp_ref_harness
     read(u) dt, current_time, integration_time, max_hsml
     read(u) x(:, 1:n), vx(:, 1:n), mass(1:n), rho(1:n), hsml(1:n), &
             A(1:n), h(1:n)
-    if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb)
+    if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb), b_normal(:, 1:nb)
     close(u)
 
     ntotal = n
@@ -66,6 +69,7 @@ program simp_ref_harness
         scale_h = 3
     end select
     itimestep = 1
+    call sound_speed
 
     ! the CSV files time_integration's output routine appends to
     output_folder = '/'
@@ -94,11 +98,12 @@ program simp_ref_harness
         read(u) dt, current_time, integration_time, max_hsml
         read(u) x(:, 1:n), vx(:, 1:n), mass(1:n), rho(1:n), hsml(1:n), &
                 A(1:n), h(1:n)
-        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb)
+        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb), b_normal(:, 1:nb)
         read(u) x(:, 1:n), vx(:, 1:n), mass(1:n), rho(1:n), hsml(1:n), &
                 A(1:n), h(1:n)
-        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb)
+        if (nb > 0) read(u) xb(:, 1:nb), massb(1:nb), b_normal(:, 1:nb)
         close(u)
+        call sound_speed
         call single_step
     else
         write(*,*) 'harness: unknown mode'

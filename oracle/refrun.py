@@ -54,6 +54,14 @@ HARNESS_GPU_TSE1 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_tse1")
 #: the harness built with continuity density (den_alg = 2)
 HARNESS_DEN2 = os.path.join(HERE, "_ref", "simp_ref_harness_den2")
 HARNESS_GPU_DEN2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den2")
+#: the harnesses built with sle = 0 / 1 (also with Euler, with continuity
+#: density), XSPH, the two artificial viscosities (Monaghan's also with the
+#: Monaghan time step) and the Wang / Marquis boundary forces, by tag
+VARIANT_TAGS = ("sle0", "sle1", "sle1eu", "sle1d2", "xsph", "avhk", "avmon", "avtse1",
+                "bfa2", "bfa3")
+HARNESS_VARIANT = {t: os.path.join(HERE, "_ref", f"simp_ref_harness_{t}") for t in VARIANT_TAGS}
+HARNESS_GPU_VARIANT = {t: os.path.join(HERE, "_ref", f"simp_ref_harness_gpu_{t}")
+                       for t in VARIANT_TAGS}
 #: the OpenMP build sized for the 250k workload (maxn = 300000), timings only
 HARNESS_OMP = os.path.join(HERE, "_ref", "simp_ref_harness_omp")
 #: the serial build with the same capacity
@@ -75,6 +83,7 @@ def _write_state(f, sc):
     if sc.nb:
         f.write(np.ascontiguousarray(sc.xb, dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(sc.massb, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(sc.b_normal(), dtype="<f8").tobytes())
 
 
 def _write_input(path, mode, sc, reps, integration_time, second=None):

@@ -23,6 +23,12 @@
  *   - con_density (den_alg = 2): golden vectors of the reference built with
  *     that PARAMETER, plus a zero rate for a uniform translation and
  *     -rho div(v) for a uniform expansion on a lattice.
+ *   - sle = 0 / 1, XSPH, the artificial viscosities, bf_Wang and bf_Marquis:
+ *     golden vectors of the reference built with each PARAMETER, plus
+ *     closed forms (XSPH and the viscosities vanish for uniform flow; a
+ *     single diverging pair gets no viscosity; Wang's force cancels the
+ *     normal velocity in one dt; Benz's rate is -h/2 div v / rho for a
+ *     uniform expansion).
  */
 #include "simp_oracle.h"
 
@@ -54,6 +60,12 @@ static const double HSML_MULT = 3.0;      /* hsml_multiplicity */
 static const double MU_MC = 0x1.fffffffffffffp-1;
 static const double ZETA_MAX = 0.0, ETA_MAX = 0.0;
 static const double ZERO_MC = 1.e-18;      /* zero = 1.d-18 */
+/* artificial viscosity, XSPH and boundary friction (data_variables.f90:
+ * 153-180): alpha_pi = 1., beta_pi = 2. (exact), chi = 0.3 and xi = 0.5225
+ * default-REAL literals */
+static const double ALPHA_PI = 1.0, BETA_PI = 2.0;
+#define CHI_XSPH ((double)0.3f)
+#define XI_FRIC ((double)0.5225f)
 
 enum { ORC_OK = 0, ORC_E_ARG = 1, ORC_E_NOMEM = 2 };
 
@@ -76,6 +88,13 @@ struct orc_ctx {
     /* scratch */
     int nscr;
     double *div, *grad;
+    double *aux;              /* (5, n): c, lv / div_v, xsph_sum (2), spare */
+    /* drhodt(maxn) (data_variables.f90): Benz_hsml adds each step's pair
+     * sum to it (smoothing_length.f90:76-98) and nothing but con_density
+     * resets it, so without continuity density it accumulates over the
+     * run; zero at the start like the module array (initialize.f90:44) */
+    int ndr;
+    double *drp;
 };
 
 orc_ctx *orc_create(void) { return (orc_ctx *)calloc(1, sizeof(orc_ctx)); }
@@ -85,7 +104,7 @@ void orc_destroy(orc_ctx *c) {
     free(c->p_i); free(c->p_j); free(c->r_ij); free(c->hsml_ij); free(c->dwdx);
     free(c->w); free(c->kz);
     free(c->bp_i); free(c->bp_j); free(c->br_ij);
-    free(c->div); free(c->grad);
+    free(c->div); free(c->grad); free(c->aux); free(c->drp);
     free(c);
 }
 
@@ -360,11 +379,19 @@ void orc_wind_at(const orc_params *prm, int n, const double *x, double *uv) {
 }
 
 static int ensure_scratch(orc_ctx *c, int n) {
+    if (n > c->ndr) {   /* zero-initialised like the module array */
+        double *d = (double *)realloc(c->drp, (size_t)n * sizeof(double));
+        if (!d) return -1;
+        memset(d + c->ndr, 0, (size_t)(n - c->ndr) * sizeof(double));
+        c->drp = d;
+        c->ndr = n;
+    }
     if (n <= c->nscr) return 0;
-    free(c->div); free(c->grad);
+    free(c->div); free(c->grad); free(c->aux);
     c->div = (double *)malloc((size_t)n * sizeof(double));
     c->grad = (double *)malloc((size_t)n * 4 * sizeof(double));
-    if (!c->div || !c->grad) { c->nscr = 0; return -1; }
+    c->aux = (double *)malloc((size_t)n * 5 * sizeof(double));
+    if (!c->div || !c->grad || !c->aux) { c->nscr = 0; return -1; }
     c->nscr = n;
     return 0;
 }
@@ -529,6 +556,98 @@ static void mc_viscosity(double e11, double e12, double e22, double P, double *z
     *et = ee;
 }
 
+/* c_yang (sound_speed.f90:57-58): sqrt(Y (1 - nu) / (rho (1 + nu)
+ * (1 - 2 nu))), Y = 1.D9, nu = 0.3 (a default-REAL literal,
+ * data_variables.f90:159-160) */
+static const double Y_MOD = 1.0e9;
+static const double NU = (double)0.3f;
+double orc_sound_speed(double rho) {
+    return sqrt(Y_MOD * (1 - NU) / (rho * (1 + NU) * (1 - 2 * NU)));
+}
+
+/* The sound speed single_step sees: sound_speed computes it after every
+ * density update (time_integration.f90:140, 230, 274, 355, 383; and
+ * initialize.f90:199-200) only when a switch reads it (sound_speed.f90:21);
+ * otherwise it stays at its initial zero (initialize.f90:34). */
+static int needs_c(const orc_params *prm) { return prm->art_visc || prm->tse == 1; }
+
+/* regularization (regularization.f90:5-36): XSPH when requested, then the
+ * artificial viscosity art_visc_alg names -- which runs whenever
+ * regularization does, also for XSPH alone (then with the zero sound speed
+ * the switches leave, :29-35). */
+static void regularization(orc_ctx *c, const orc_params *prm, int n, const double *x,
+                           const double *vx, const double *mass, const double *rho,
+                           const double *hsml, double *dxdt, double *dvxdt) {
+    double *cs = c->aux, *lv = c->aux + n, *xs = c->aux + 2 * (size_t)n;
+    const int wc = needs_c(prm);
+    for (int i = 0; i < n; ++i) cs[i] = wc ? orc_sound_speed(rho[i]) : 0.0;
+    if (prm->xsph) {
+        /* XSPH (:69-93): the same dvx = v_i - v_j enters both particles' sums */
+        for (int i = 0; i < 2 * n; ++i) xs[i] = 0.;
+        for (int k = 0; k < c->nip; ++k) {
+            int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+            double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+            double rm = 0.5 * (rho[i] + rho[j]);
+            xs[2 * i] = xs[2 * i] + mass[j] * dv0 * c->w[k] / rm;
+            xs[2 * i + 1] = xs[2 * i + 1] + mass[j] * dv1 * c->w[k] / rm;
+            xs[2 * j] = xs[2 * j] + mass[i] * dv0 * c->w[k] / rm;
+            xs[2 * j + 1] = xs[2 * j + 1] + mass[i] * dv1 * c->w[k] / rm;
+        }
+        for (int i = 0; i < 2 * n; ++i) dxdt[i] = dxdt[i] - CHI_XSPH * xs[i];
+    }
+    const int alg = prm->art_visc_alg ? prm->art_visc_alg : 1;
+    /* pi_ij keeps its value from the last pair that set it: a pair that
+     * does not set it takes the previous one's (0 before the first) */
+    double pi_ij = 0.;
+    if (alg == 1) {
+        /* art_visc_HK (:141-211): div_v over the pairs, zero where >= 0 */
+        for (int i = 0; i < n; ++i) lv[i] = 0.;
+        for (int k = 0; k < c->nip; ++k) {
+            int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+            double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+            double w1 = dv0 * c->dwdx[2 * k], w4 = dv1 * c->dwdx[2 * k + 1];
+            lv[i] = lv[i] - mass[j] * (w1 + w4) / rho[i];
+            lv[j] = lv[j] - mass[i] * (w1 + w4) / rho[j];
+        }
+        for (int i = 0; i < n; ++i) {
+            double dv = lv[i] >= 0. ? 0. : lv[i];
+            double t = fabs(dv) * hsml[i];
+            lv[i] = (ALPHA_PI * hsml[i] * cs[i] * fabs(dv) + BETA_PI * (t * t)) * rho[i];
+        }
+        for (int k = 0; k < c->nip; ++k) {
+            int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+            double dx0 = x[2 * i] - x[2 * j], dx1 = x[2 * i + 1] - x[2 * j + 1];
+            double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+            double crit = dv0 * dx0 + dv1 * dx1;
+            if (crit <= 0.) pi_ij = lv[i] / (rho[i] * rho[i]) + lv[j] / (rho[j] * rho[j]);
+            dvxdt[2 * i] = dvxdt[2 * i] - mass[j] * pi_ij * c->dwdx[2 * k];
+            dvxdt[2 * j] = dvxdt[2 * j] + mass[i] * pi_ij * c->dwdx[2 * k];
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] - mass[j] * pi_ij * c->dwdx[2 * k + 1];
+            dvxdt[2 * j + 1] = dvxdt[2 * j + 1] + mass[i] * pi_ij * c->dwdx[2 * k + 1];
+        }
+    } else if (alg == 2) {
+        /* art_visc_Mon (:261-306); s_cor = 0.01 hsml_ij, 0.01 a default-REAL
+         * literal */
+        for (int k = 0; k < c->nip; ++k) {
+            int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+            double dx0 = x[2 * i] - x[2 * j], dx1 = x[2 * i + 1] - x[2 * j + 1];
+            double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+            double v_r = dv0 * dx0 + dv1 * dx1;
+            double hij = c->hsml_ij[k], rij = c->r_ij[k];
+            double s_cor = (double)0.01f * hij;
+            if (v_r <= 0.) {
+                double iota = hij * v_r / (rij * rij + s_cor * s_cor);
+                pi_ij = (BETA_PI * (iota * iota) - ALPHA_PI * iota * (cs[i] + cs[j]) / 2) /
+                        ((rho[i] + rho[j]) / 2);
+            }
+            dvxdt[2 * i] = dvxdt[2 * i] - mass[j] * pi_ij * c->dwdx[2 * k];
+            dvxdt[2 * j] = dvxdt[2 * j] + mass[i] * pi_ij * c->dwdx[2 * k];
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] - mass[j] * pi_ij * c->dwdx[2 * k + 1];
+            dvxdt[2 * j + 1] = dvxdt[2 * j + 1] + mass[i] * pi_ij * c->dwdx[2 * k + 1];
+        }
+    }
+}
+
 int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
                     const double *vx, const double *mass, const double *rho,
                     const double *hsml, const double *h, const double *A,
@@ -659,8 +778,22 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
      * kernel factor reads hsml_ij at the boundary-pair index, i.e. an entry
      * of the particle-pair list (:491); stale entries beyond nip are read as
      * they are. */
+    const int bfa = prm->bfa ? prm->bfa : 1;
     for (int k = 0; k < c->bnip; ++k) {
         int i = c->bp_j[k] - 1, b = c->bp_i[k] - 1;
+        if (bfa == 3) continue;   /* bf_Marquis: no force with boundary_pressure (:211-214) */
+        if (bfa == 2) {
+            /* bf_Wang (boundary_force.f90:145-174), boundary_friction off:
+             * the normal velocity and the rest of the speed, each over dt */
+            const double n0 = prm->b_normal ? prm->b_normal[2 * b] : 0.0;
+            const double n1 = prm->b_normal ? prm->b_normal[2 * b + 1] : 0.0;
+            double v0 = n0 * vx[2 * i] + n1 * vx[2 * i + 1];
+            double v1 = sqrt(vx[2 * i] * vx[2 * i] + vx[2 * i + 1] * vx[2 * i + 1]) - v0;
+            double F0 = mass[i] * v0 / prm->step_dt, F1 = mass[i] * v1 / prm->step_dt;
+            dvxdt[2 * i] = dvxdt[2 * i] + (F0 * n0 + F1 * n1);
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (F0 * n1 - F1 * n0);
+            continue;
+        }
         double bdx0 = xb[2 * b] - x[2 * i], bdx1 = xb[2 * b + 1] - x[2 * i + 1];
         double br = c->br_ij[k];
         double hk = k < c->cap ? c->hsml_ij[k] : 0.0;
@@ -689,6 +822,8 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
             dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (C_WATER * RHO_WATER * dv * fabs(dv)) / hr;
         }
     }
+    /* regularization (single_step.f90:99-101) */
+    if (prm->xsph || prm->art_visc) regularization(c, prm, n, x, vx, mass, rho, hsml, dxdt, dvxdt);
     /* tendencies (:104-106) */
     for (int i = 0; i < n; ++i) {
         dhdt[i] = -(h[i] * div[i] / rho[i]);
@@ -723,10 +858,9 @@ int orc_sum_density(orc_ctx *c, int n, const double *mass, double *rho) {
  * of mass(j) (vx_i - vx_j) . dwdx_k (and mass(i) times the same for j),
  * with the velocities given (the module's at the time density() is
  * called) and the gradients the last single_step left. */
-static int con_density(orc_ctx *c, int n, const double *vx, const double *mass,
-                       double *drhodt) {
+static int pair_rate(orc_ctx *c, int n, const double *vx, const double *mass,
+                     double *drhodt) {
     if (n < 1) return ORC_E_ARG;
-    for (int i = 0; i < n; ++i) drhodt[i] = 0.;
     for (int k = 0; k < c->nip; ++k) {
         int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
         double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
@@ -736,6 +870,13 @@ static int con_density(orc_ctx *c, int n, const double *vx, const double *mass,
         drhodt[j] = drhodt[j] + mass[i] * v_ij;
     }
     return ORC_OK;
+}
+
+static int con_density(orc_ctx *c, int n, const double *vx, const double *mass,
+                       double *drhodt) {
+    if (n < 1) return ORC_E_ARG;
+    for (int i = 0; i < n; ++i) drhodt[i] = 0.;
+    return pair_rate(c, n, vx, mass, drhodt);
 }
 
 int orc_con_density(orc_ctx *c, int n, const double *vx, const double *mass,
@@ -765,8 +906,39 @@ double orc_vp_dt(int n, const double *hsml) {
  * (sound_speed.f90), sqrt(Y (1 - nu) / (rho (1 + nu) (1 - 2 nu))), at the
  * step's final density.  Y = 1.D9, nu = 0.3 (a default-REAL literal,
  * data_variables.f90:159-160). */
-static const double Y_MOD = 1.0e9;
-static const double NU = (double)0.3f;
+/* Monaghan_time_step with artificial viscosity (time_step.f90:79-121): the
+ * force condition as above, and the CFL condition with the velocity
+ * divergence of the last single_step's pairs and gradients at the current
+ * velocities and densities (:89-107, not clamped), 0.3 and 1.2 default-REAL
+ * literals */
+static double monaghan_dt_av(const orc_ctx *c, int n, const double *hsml, const double *rho,
+                             const double *vx, const double *mass, const double *dvxdt,
+                             double *div_v) {
+    double mf = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double acc = sqrt(dvxdt[2 * i] * dvxdt[2 * i] + dvxdt[2 * i + 1] * dvxdt[2 * i + 1]);
+        double f = hsml[i] / acc;
+        if (i == 0 || f < mf) mf = f;
+        div_v[i] = 0.;
+    }
+    for (int k = 0; k < c->nip; ++k) {
+        int i = c->p_i[k] - 1, j = c->p_j[k] - 1;
+        double dv0 = vx[2 * i] - vx[2 * j], dv1 = vx[2 * i + 1] - vx[2 * j + 1];
+        double w1 = dv0 * c->dwdx[2 * k], w4 = dv1 * c->dwdx[2 * k + 1];
+        div_v[i] = div_v[i] - mass[j] * (w1 + w4) / rho[i];
+        div_v[j] = div_v[j] - mass[i] * (w1 + w4) / rho[j];
+    }
+    double mc = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double cs = orc_sound_speed(rho[i]);
+        double ad = fabs(div_v[i]);
+        double g = (double)0.3f * hsml[i] /
+                   (cs + hsml[i] * ad + (double)1.2f * (ALPHA_PI * cs + BETA_PI * hsml[i] * ad));
+        if (i == 0 || g < mc) mc = g;
+    }
+    double dt_force = 0.25 * sqrt(mf);
+    return dt_force < mc ? dt_force : mc;
+}
 
 double orc_monaghan_dt(int n, const double *hsml, const double *rho, const double *dvxdt) {
     double mf = 0.0, mc = 0.0;
@@ -781,9 +953,25 @@ double orc_monaghan_dt(int n, const double *hsml, const double *rho, const doubl
     return dt_force < mc ? dt_force : mc;
 }
 
+/* mass_update_from_area (mass_evolution.f90:40-71) for sle = 0 and 1:
+ * mass = rho_ice A h pi (alpha hsml)^2 with alpha = 1. / (2 hsml_multiplicity),
+ * a default-REAL quotient (data_variables.f90:169); sle = 2 keeps the mass
+ * (:27-29).  Called after every density update (time_integration.f90:137,
+ * 227, 271, 352, 380). */
+static void mass_evolution(const orc_params *prm, int n, const double *A, const double *h,
+                           const double *hsml, double *mass) {
+    if (prm->sle == 2) return;
+    const double alpha = (double)(1.0f / 6);
+    for (int i = 0; i < n; ++i) {
+        double t = alpha * hsml[i];
+        double area = PI_ * (t * t);
+        mass[i] = RHO_ICE * A[i] * h[i] * area;
+    }
+}
+
 /* ---- time_integration + predictor_corrector -------------------------- */
 int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
-               double *vx, const double *mass, double *rho, double *hsml,
+               double *vx, double *mass, double *rho, double *hsml,
                double *h, double *A, int nb, const double *xb,
                const double *massb, double *dt_io, double *time_io,
                int *itimestep, double *eps3_o, double *zeta_o, double *eta_o,
@@ -809,7 +997,9 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
         memcpy(h1, h, (size_t)n * sizeof(double));
         memcpy(A1, A, (size_t)n * sizeof(double));
         memcpy(rho1, rho, (size_t)n * sizeof(double));
-        rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
+        orc_params sp = *prm;   /* the module dt single_step sees (bf_Wang) */
+        sp.step_dt = dt;
+        rc = orc_single_step(c, &sp, n, x, vx, mass, rho, hsml, h, A, nb, xb,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
         if (rc) break;
@@ -851,7 +1041,8 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
             if ((rc = con_density(c, n, vx, mass, drhodt))) break;
             for (int i = 0; i < n; ++i) rho[i] = rho[i] + drhodt[i] * dt / 2;
         }
-        rc = orc_single_step(c, prm, n, x, vx, mass, rho, hsml, h, A, nb, xb,
+        mass_evolution(prm, n, A, h, hsml, mass);   /* (:227, :352) */
+        rc = orc_single_step(c, &sp, n, x, vx, mass, rho, hsml, h, A, nb, xb,
                              massb, t, dxdt, dvxdt, dhdt, dAdt, eps3, zeta,
                              eta, pr, sig3);
         if (rc) break;
@@ -904,13 +1095,41 @@ int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n, double *x,
         }
         if (prm->den_alg == 1 && (rc = orc_sum_density(c, n, mass, rho))) break;
     end_of_step:
-        for (int i = 0; i < n; ++i) {
-            /* Marquis_hsml (smoothing_length.f90:136-142) */
-            hsml[i] = sqrt(mass[i] / rho[i]) * HSML_MULT;
-            if (hsml[i] > prm->max_hsml) hsml[i] = prm->max_hsml;
+        /* the predictor-corrector's third density call (:262) runs on the
+         * final velocities with the half step's masses; only Benz_hsml reads
+         * its continuity rate */
+        if (cden && scheme == 2 && prm->sle == 1 &&
+            (rc = con_density(c, n, vx, mass, drhodt)))
+            break;
+        mass_evolution(prm, n, A, h, hsml, mass);   /* (:137, :271, :380) */
+        if (prm->sle == 2) {
+            for (int i = 0; i < n; ++i) {
+                /* Marquis_hsml (smoothing_length.f90:136-142) */
+                hsml[i] = sqrt(mass[i] / rho[i]) * HSML_MULT;
+                if (hsml[i] > prm->max_hsml) hsml[i] = prm->max_hsml;
+            }
+        } else if (prm->sle == 1) {
+            /* Benz_hsml (smoothing_length.f90:75-109): the last single_step's
+             * pairs and gradients at the step's final velocities; without
+             * continuity density the sum goes onto the persistent drhodt,
+             * with it drhodt is the last density call's (time_integration.f90
+             * :262 in the predictor-corrector: final velocities too) */
+            double *dr = c->drp;
+            if (cden) {
+                dr = drhodt;   /* the last density call's (:131, :262, :375) */
+            } else if ((rc = pair_rate(c, n, vx, mass, c->drp))) {
+                break;
+            }
+            for (int i = 0; i < n; ++i) {
+                double dh = -(hsml[i] * dr[i] / 2 / rho[i]);
+                hsml[i] = hsml[i] + dh * dt;
+                if (hsml[i] < 0) hsml[i] = hsml[i] - dh * dt;
+            }
         }
         /* time_step (:280) */
         if (prm->tse == 2) dt = orc_vp_dt(n, hsml);
+        else if (prm->tse == 1 && prm->art_visc)
+            dt = monaghan_dt_av(c, n, hsml, rho, vx, mass, dvxdt, drhodt);
         else if (prm->tse == 1) dt = orc_monaghan_dt(n, hsml, rho, dvxdt);
         /* loop bookkeeping (time_integration.f90:68-71) */
         it += 1;

@@ -56,6 +56,20 @@ typedef struct orc_params {
     int    int_algorithm; /* 2 (or 0): predictor_corrector
                             (time_integration.f90:155-282); 1: euler_scheme
                             (:86-148); 3: leapfrog_scheme (:289-391) */
+    int    sle;          /* smoothing_length (smoothing_length.f90:21-30):
+                            2: Marquis_hsml (:118-145); 1: Benz_hsml
+                            (:40-111); 0: unchanged.  No zero default. */
+    int    xsph;         /* xsph_regularization (globals.f90:187):
+                            XSPH (regularization.f90:45-95) */
+    int    art_visc;     /* artificial_visc (globals.f90:186) */
+    int    art_visc_alg; /* 1 (or 0): art_visc_HK (regularization.f90:102-213);
+                            2: art_visc_Mon (:220-308).  Either runs whenever
+                            regularization does (:29-35) */
+    int    bfa;          /* 1 (or 0): bf_Monaghan (boundary_force.f90:38-102);
+                            2: bf_Wang (:109-176); 3: bf_Marquis (:183-223) */
+    const double *b_normal;  /* (2, nb) boundary normals, bf_Wang only */
+    double step_dt;      /* the module dt single_step sees (bf_Wang reads it);
+                            orc_run_pc sets it to each step's dt */
 } orc_params;
 
 orc_ctx *orc_create(void);
@@ -95,9 +109,10 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n,
  * scheme prm->int_algorithm selects (predictor_corrector by default).  State arrays are updated in place;
  * dt, current_time and itimestep advance as in the reference.  The
  * diagnostic outputs (eps3, zeta, eta, p, sig3) hold the last single_step's
- * values and may be NULL. */
+ * values and may be NULL.  With sle = 0 or 1 the masses evolve
+ * (mass_evolution.f90:40-71) and mass is updated in place as well. */
 int orc_run_pc(orc_ctx *c, const orc_params *prm, int nsteps, int n,
-               double *x, double *vx, const double *mass, double *rho,
+               double *x, double *vx, double *mass, double *rho,
                double *hsml, double *h, double *A, int nb, const double *xb,
                const double *massb, double *dt, double *current_time,
                int *itimestep, double *eps3, double *zeta, double *eta,
@@ -118,6 +133,8 @@ int orc_con_density(orc_ctx *c, int n, const double *vx, const double *mass,
                     double *drhodt);
 /* the pair gradients dwdx (2, nip) of the last single_step */
 void orc_get_dwdx(const orc_ctx *c, double *dwdx);
+/* c_yang sound speed (sound_speed.f90:36-60) of a density */
+double orc_sound_speed(double rho);
 double orc_vp_dt(int n, const double *hsml);
 /* Monaghan_time_step without artificial viscosity (time_step.f90:39-125)
  * from hsml, rho and the last evaluation's dvxdt (2, n). */

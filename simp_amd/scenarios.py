@@ -66,6 +66,15 @@ class Scenario:
     rheology: int = 1        # 1 elliptical yield curve, 2 Mohr-Coulomb
     skf: int = 6             # 6 Wendland C6, 4 Wendland C2, 5 Wendland C4
     int_algorithm: int = 2   # 2 predictor-corrector, 1 Euler, 3 leap-frog
+    sle: int = 2             # 2 Marquis hsml, 1 Benz hsml, 0 hsml unchanged
+    xsph: int = 0            # 1: XSPH position regularization
+    art_visc: int = 0        # 1: artificial viscosity
+    art_visc_alg: int = 1    # 1 Hernquist and Katz, 2 Monaghan
+    bfa: int = 1             # boundary force: 1 Monaghan, 2 Wang, 3 Marquis
+    #: inward unit normal of each boundary particle (nb, 2), as
+    #: boundary_placing_1 sets b_normal (in_out/boundaries.f90:232-287);
+    #: read by the Wang boundary force only.  None: zeros.
+    bnormal: Optional[np.ndarray] = None
     dt: float = 0.0
     time: float = 0.0
     name: str = ""
@@ -82,7 +91,14 @@ class Scenario:
         return dataclasses.replace(
             self, x=self.x.copy(), vx=self.vx.copy(), mass=self.mass.copy(),
             rho=self.rho.copy(), hsml=self.hsml.copy(), h=self.h.copy(),
-            A=self.A.copy(), xb=self.xb.copy(), massb=self.massb.copy())
+            A=self.A.copy(), xb=self.xb.copy(), massb=self.massb.copy(),
+            bnormal=None if self.bnormal is None else self.bnormal.copy())
+
+    def b_normal(self) -> np.ndarray:
+        """The boundary normals as (nb, 2) FP64 (zeros when unset)."""
+        if self.bnormal is None or self.nb == 0:
+            return np.zeros((self.nb, 2))
+        return np.ascontiguousarray(self.bnormal, dtype=np.float64).reshape(self.nb, 2)
 
 
 def vp_dt(hsml: np.ndarray) -> float:
@@ -123,6 +139,19 @@ def walls(x0: float, y0: float, x1: float, y1: float,
     return P[np.sort(idx)]
 
 
+def wall_normals(xb: np.ndarray, x0: float, y0: float, x1: float, y1: float) -> np.ndarray:
+    """Inward unit normals of boundary particles on the walls of the
+    rectangle [x0, x1] x [y0, y1] (the orientation boundary_placing_1 gives
+    its rows, in_out/boundaries.f90:232-287: pointing into the ice); a
+    corner takes the normal of its vertical wall."""
+    nrm = np.zeros_like(xb)
+    nrm[xb[:, 1] == y0] = (0.0, 1.0)
+    nrm[xb[:, 1] == y1] = (0.0, -1.0)
+    nrm[xb[:, 0] == x0] = (1.0, 0.0)
+    nrm[xb[:, 0] == x1] = (-1.0, 0.0)
+    return nrm
+
+
 def lognormal_thickness(rng, n, mean=1.5, sd=0.5):
     s2 = math.log(1.0 + sd * sd / (mean * mean))
     mu = math.log(mean * mean / math.sqrt(mean * mean + sd * sd))
@@ -159,7 +188,7 @@ def box(nside: int = 50, spacing: float = 2000.0, seed: int = 1,
     A = np.ones(n)
     xb = walls(0.0, 0.0, L, L)
     return _finish(x, hsml, h, A, xb, rng, wind=tuple(wind), tse=tse, dt=dt,
-                   name=f"box{n}")
+                   name=f"box{n}", bnormal=wall_normals(xb, 0.0, 0.0, L, L))
 
 
 def rankine_grid(L: float, vmax: float, radius: float, gdx: float) -> WindGrid:
@@ -191,7 +220,7 @@ def vortex(nside: int = 200, spacing: float = 1000.0, seed: int = 2,
     xb = walls(0.0, 0.0, L, L)
     grid = rankine_grid(L, 15.0, 60000.0 * (L / 200000.0), 2000.0 * (L / 200000.0))
     return _finish(x, hsml, h, A, xb, rng, wind_grid=grid, tse=tse, dt=dt,
-                   name=f"vortex{n}")
+                   name=f"vortex{n}", bnormal=wall_normals(xb, 0.0, 0.0, L, L))
 
 
 def grf_wind(L: float, std: float, corr: float, gdx: float,
@@ -234,7 +263,7 @@ def ice_edge(nside: int = 500, spacing: float = 1000.0, seed: int = 3,
     xb = walls(0.0, 0.0, 0.0, L, sides=("left",))
     grid = grf_wind(L, 8.0, 50000.0 * (L / 500000.0), 5000.0 * (L / 500000.0), rng)
     return _finish(x, hsml, h, A, xb, rng, wind_grid=grid, tse=tse, dt=dt,
-                   name=f"ice_edge{n}")
+                   name=f"ice_edge{n}", bnormal=np.tile([1.0, 0.0], (xb.shape[0], 1)))
 
 
 def box_member(N: float, L: float, wind, seed: int, tse: int = 2) -> Scenario:
@@ -250,7 +279,8 @@ def box_member(N: float, L: float, wind, seed: int, tse: int = 2) -> Scenario:
     h = lognormal_thickness(sub, n)
     A = np.ones(n)
     xb = walls(0.0, 0.0, L, L)
-    return _finish(x, hsml, h, A, xb, sub, wind=tuple(wind), tse=tse, name=f"box{n}_L{L:.0f}")
+    return _finish(x, hsml, h, A, xb, sub, wind=tuple(wind), tse=tse, name=f"box{n}_L{L:.0f}",
+                   bnormal=wall_normals(xb, 0.0, 0.0, L, L))
 
 
 def ensemble(count: int = 64, nmin: int = 2000, nmax: int = 60000,
@@ -279,7 +309,8 @@ def subdomain(sc: Scenario, x0: float, x1: float, y0: float, y1: float) -> Scena
         sc, x=np.ascontiguousarray(sc.x[m]), vx=np.ascontiguousarray(sc.vx[m]),
         mass=sc.mass[m].copy(), rho=sc.rho[m].copy(), hsml=sc.hsml[m].copy(),
         h=sc.h[m].copy(), A=sc.A[m].copy(), xb=np.ascontiguousarray(sc.xb[mb]),
-        massb=sc.massb[mb].copy(), name=sc.name + "_sub")
+        massb=sc.massb[mb].copy(), name=sc.name + "_sub",
+        bnormal=None if sc.bnormal is None else np.ascontiguousarray(sc.bnormal[mb]))
     if out.tse == 2:
         out.dt = vp_dt(out.hsml)
     return out
@@ -355,6 +386,7 @@ def from_maps(mp: ReferenceMaps, wind=REFERENCE_WIND, tse: int = 2) -> Scenario:
     mf = _f32(half * _f32(mp.map_factor))     # 1./2. * map_factor
     bp = _f32(bppg)
     xb = []
+    nrm = []
     res = mp.bmap_resolution
     for n in range(1, bx + 1):                # DO CONCURRENT (n, m): m fastest,
         for m in range(1, by + 1):            # the compiled reference's order
@@ -365,25 +397,30 @@ def from_maps(mp: ReferenceMaps, wind=REFERENCE_WIND, tse: int = 2) -> Scenario:
                     ax = _f32(n) - mf
                     ay = (_f32(-m + 1) + mf) - (_f32(k) + half) / bp
                     xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+                    nrm.append((1.0, 0.0))
             if n - 1 >= 1 and mp.bmap[n - 2, m - 1] == 1:
                 for k in range(bppg):
                     ax = _f32(n - 1) - mf
                     ay = (_f32(-m + 1) + mf) - (_f32(k) + half) / bp
                     xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+                    nrm.append((-1.0, 0.0))
             if m + 1 <= by and mp.bmap[n - 1, m] == 1:
                 for k in range(bppg):
                     ax = (_f32(n - 1) - mf) + (_f32(k) + half) / bp
                     ay = _f32(-m) + mf
                     xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+                    nrm.append((0.0, -1.0))
             if m - 1 >= 1 and mp.bmap[n - 1, m - 2] == 1:
                 for k in range(bppg):
                     ax = (_f32(n - 1) - mf) + (_f32(k) + half) / bp
                     ay = _f32(-m + 1) + mf
                     xb.append((mp.bx_mingeom + float(ax) * res, mp.by_maxgeom + float(ay) * res))
+                    nrm.append((0.0, 1.0))
     xb = np.array(xb, dtype=np.float64).reshape(-1, 2)
     massb = np.full(xb.shape[0], float(np.sum(mass)) / n_p)
     sc = Scenario(x=x, vx=np.zeros((n_p, 2)), mass=mass, rho=rho, hsml=hsml, h=h, A=A,
                   xb=xb, massb=massb, max_hsml=max_hsml, wind=wind, tse=tse,
-                  name="reference_default")
+                  name="reference_default",
+                  bnormal=np.array(nrm, dtype=np.float64).reshape(-1, 2))
     sc.dt = vp_dt(hsml) if tse == 2 else 0.0
     return sc

@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos] [switches]
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos] [switches] [regularization]
 """
 from __future__ import annotations
 
@@ -28,6 +28,8 @@ STATE_KEYS = ("x", "vx", "mass", "rho", "hsml", "h", "A", "xb", "massb")
 def _inputs(sc, prefix="in_"):
     d = {prefix + k: np.asarray(getattr(sc, k)) for k in STATE_KEYS}
     d[prefix + "scalars"] = np.array([sc.dt, sc.time, sc.max_hsml])
+    if sc.bnormal is not None:
+        d[prefix + "bnormal"] = sc.b_normal()
     return d
 
 
@@ -231,15 +233,14 @@ def pressures():
     _pc10_edge("eos2", refrun.HARNESS_EOS2, eos_alg=2)
 
 
-def _pc10_edge(tag, harness, **switches):
-    """Ten steps of the 3,600-particle ice edge (reference wind) with the
-    harness built for one switch value."""
+def _pc10_edge(tag, harness, K=10, **switches):
+    """Ten (or K) steps of the 3,600-particle ice edge (reference wind) with
+    the harness built for one switch value."""
     sc = S.ice_edge(nside=60, spacing=1000.0, seed=3)
     sc.wind_grid = None
     sc.wind = S.REFERENCE_WIND
     for k, v in switches.items():
         setattr(sc, k, v)
-    K = 10
     probe = sc.copy()
     o = capi.Oracle()
     times = []
@@ -249,7 +250,9 @@ def _pc10_edge(tag, harness, **switches):
     t_end = 0.5 * (times[-2] + times[-1])
     r = refrun.run(refrun.MODE_TIME_INTEGRATION, sc, integration_time=t_end, harness=harness)
     assert r["itimestep"] == K, r["itimestep"]
-    _save(f"edge3600_pc10_{tag}.npz", **_inputs(sc), **_outputs(r, STATE_OUT),
+    assert all(np.all(np.isfinite(r[k])) for k in STATE_OUT), tag
+    name = f"edge3600_pc{K}_{tag}.npz"
+    _save(name, **_inputs(sc), **_outputs(r, STATE_OUT),
           out_dt=r["dt"], out_time=r["time"], out_itimestep=r["itimestep"],
           integration_time=t_end, **switches)
 
@@ -280,12 +283,54 @@ def switches():
     _pc10_edge("euler", refrun.HARNESS_EULER, int_algorithm=1)
     _pc10_edge("leap", refrun.HARNESS_LEAP, int_algorithm=3)
     _pc10_edge("tse1", refrun.HARNESS_TSE1, tse=1)
+    _pc10_edge("den2", refrun.HARNESS_DEN2, den_alg=2)
+
+
+def _single_box(tag, harness, **switches):
+    """One single_step of the 2,500-particle closed box at t > 0 with random
+    velocities (both signs of v . r in its pairs), kernel values included."""
+    sc = S.box(nside=50, spacing=2000.0, seed=1)
+    sc.vx = np.random.default_rng(101).normal(0.0, 0.1, size=(sc.n, 2))
+    sc.time = 10.0
+    for k, v in switches.items():
+        setattr(sc, k, v)
+    r = refrun.run(refrun.MODE_SINGLE_STEP, sc, harness=harness)
+    _save(f"box2500_single_step_{tag}.npz", **_inputs(sc),
+          **_outputs(r, ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p",
+                         "sigma", "w", "kernel_zero")),
+          out_nip=r["nip"], out_bnip=r["bnip"], **switches)
+
+
+def regularization():
+    """11. smoothing-length evolution kept (sle = 0) and Benz's rate (sle =
+    1, also with the Euler scheme and with continuity density), XSPH, the
+    Hernquist-Katz and Monaghan artificial viscosities (the latter also with
+    the Monaghan time step and its viscous CFL terms) and the Wang and
+    Marquis boundary forces, each from the harness built with those
+    PARAMETERs: single_step of the closed box (XSPH, viscosities, Wang) and
+    steps of the ice edge (all).  Wang's force is a momentum over dt, which
+    the state does not survive for long: three steps."""
+    H = refrun.HARNESS_VARIANT
+    _pc10_edge("sle0", H["sle0"], sle=0)
+    _pc10_edge("sle1", H["sle1"], sle=1)
+    _pc10_edge("sle1eu", H["sle1eu"], sle=1, int_algorithm=1)
+    _pc10_edge("sle1d2", H["sle1d2"], sle=1, den_alg=2)
+    _single_box("xsph", H["xsph"], xsph=1)
+    _pc10_edge("xsph", H["xsph"], xsph=1)
+    _single_box("avhk", H["avhk"], art_visc=1)
+    _pc10_edge("avhk", H["avhk"], art_visc=1)
+    _single_box("avmon", H["avmon"], art_visc=1, art_visc_alg=2)
+    _pc10_edge("avmon", H["avmon"], art_visc=1, art_visc_alg=2)
+    _pc10_edge("avtse1", H["avtse1"], art_visc=1, tse=1)
+    _single_box("bfa2", H["bfa2"], bfa=2)
+    _pc10_edge("bfa2", H["bfa2"], K=3, bfa=2)
+    _pc10_edge("bfa3", H["bfa3"], bfa=3)
 
 
 if __name__ == "__main__":
     # no argument: every fixture; otherwise the named groups ("main", "den1",
     # "pa2", "eos", "switches")
-    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches"]
+    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches", "regularization"]
     if "main" in todo:
         main()
     if "den1" in todo:
@@ -297,3 +342,5 @@ if __name__ == "__main__":
         pressures()
     if "switches" in todo:
         switches()
+    if "regularization" in todo:
+        regularization()

@@ -28,6 +28,13 @@ def scenario(g, prefix="in_") -> Scenario:
     sc.rheology = int(g.get("rheology", 1))
     sc.skf = int(g.get("skf", 6))
     sc.int_algorithm = int(g.get("int_algorithm", 2))
+    sc.sle = int(g.get("sle", 2))
+    sc.xsph = int(g.get("xsph", 0))
+    sc.art_visc = int(g.get("art_visc", 0))
+    sc.art_visc_alg = int(g.get("art_visc_alg", 1))
+    sc.bfa = int(g.get("bfa", 1))
+    if prefix + "bnormal" in g:
+        sc.bnormal = g[prefix + "bnormal"].reshape(-1, 2).copy()
     return sc
 
 

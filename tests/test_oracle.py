@@ -509,3 +509,159 @@ def test_monaghan_time_step_closed_form():
     assert capi.Oracle.monaghan_dt(hsml, rho, acc2) == 0.25 * math.sqrt(50.0 / 5e7)
     # all at rest: only the sound crossing remains
     assert capi.Oracle.monaghan_dt(hsml, rho, np.zeros((3, 2))) == 50.0 / c
+
+
+# ----------------------------------- regularization, Benz hsml, boundary forces
+
+@pytest.mark.parametrize("tag", ["xsph", "avhk", "avmon", "bfa2"])
+def test_regularization_and_wang_single_step_bit_exact_vs_reference(tag):
+    """XSPH (regularization.f90:45-95, which also runs the art_visc_alg
+    viscosity with the sound speed left at zero), the Hernquist-Katz
+    (:102-213) and Monaghan (:220-308) artificial viscosities with the
+    c_yang sound speed, and the Wang boundary force (boundary_force.f90:
+    109-176) against the reference built with each PARAMETER."""
+    g = load(f"box2500_single_step_{tag}.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    out = o.single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    # the switch changes the result
+    plain = scenario(g)
+    plain.xsph = plain.art_visc = 0
+    plain.bfa = 1
+    p = capi.Oracle().single_step(plain)
+    assert not (_same(p["dxdt"], g["out_dxdt"]) and _same(p["dvxdt"], g["out_dvxdt"]))
+
+
+@pytest.mark.parametrize("tag", ["sle0", "sle1", "sle1eu", "sle1d2", "xsph", "avhk", "avmon",
+                                 "avtse1", "bfa3"])
+def test_regularization_and_hsml_switches_ten_steps_bit_exact_vs_reference(tag):
+    g = load(f"edge3600_pc10_{tag}.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    steps, out = run_pc_like_reference(o, sc, float(g["integration_time"]))
+    assert steps == int(g["out_itimestep"])
+    assert sc.dt == float(g["out_dt"]) and sc.time == float(g["out_time"])
+    for k in ("x", "vx", "rho", "hsml", "A", "h"):
+        assert _same(getattr(sc, k), g["out_" + k]), k
+    for k in ("eps", "zeta", "eta", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    if tag == "sle0":
+        assert _same(sc.hsml, g["in_hsml"])
+
+
+def test_wang_boundary_force_three_steps_bit_exact_vs_reference():
+    g = load("edge3600_pc3_bfa2.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    steps, out = run_pc_like_reference(o, sc, float(g["integration_time"]))
+    assert steps == int(g["out_itimestep"]) == 3
+    for k in ("x", "vx", "rho", "hsml", "A", "h"):
+        assert _same(getattr(sc, k), g["out_" + k]), k
+
+
+def _lattice_box(n1=20, s=1000.0):
+    """A regular lattice without walls (with walls, most boundary pairs of
+    such a small box rank beyond nip and the C6 boundary kernel reads the
+    zero hsml_ij entries a fresh context holds there)."""
+    sc = S.box(nside=n1, spacing=s, seed=1)
+    g = np.arange(n1) * s + s / 2
+    sc.x = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2).copy()
+    sc.xb = np.zeros((0, 2))
+    sc.massb = np.zeros(0)
+    sc.bnormal = None
+    return sc
+
+
+def test_regularization_vanishes_for_uniform_flow():
+    """With one velocity everywhere every pair has dvx = 0: the XSPH sum,
+    the velocity divergence of Hernquist-Katz and v . r of Monaghan's
+    viscosity are zero, so neither XSPH nor either viscosity changes the
+    tendencies, bit for bit."""
+    base = _lattice_box()
+    base.vx[:] = (0.2, -0.1)
+    base.time = 5.0
+    ref = capi.Oracle().single_step(base)
+    for sw in ({"xsph": 1}, {"art_visc": 1}, {"art_visc": 1, "art_visc_alg": 2},
+               {"xsph": 1, "art_visc": 1, "art_visc_alg": 2}):
+        sc = base.copy()
+        for k, v in sw.items():
+            setattr(sc, k, v)
+        out = capi.Oracle().single_step(sc)
+        for k in ("dxdt", "dvxdt", "dhdt", "dAdt"):
+            assert _same(out[k], ref[k]), (sw, k)
+
+
+def test_artificial_viscosity_acts_only_on_approaching_pairs():
+    """Both viscosities apply only where v_ij . r_ij <= 0 (regularization.f90
+    :189-196, :284-291): for a uniform expansion v = a x every pair recedes,
+    pi_ij keeps its initial zero and the tendencies equal those without
+    viscosity bit for bit; the contraction v = -a x does change them."""
+    base = _lattice_box()
+    base.time = 5.0
+    a = 1e-7
+    c0 = base.x.mean(axis=0)
+    for sgn in (1.0, -1.0):
+        sc0 = base.copy()
+        sc0.vx = sgn * a * (base.x - c0)
+        ref = capi.Oracle().single_step(sc0)
+        for alg in (1, 2):
+            sc = sc0.copy()
+            sc.art_visc, sc.art_visc_alg = 1, alg
+            out = capi.Oracle().single_step(sc)
+            same = _same(out["dvxdt"], ref["dvxdt"])
+            assert same == (sgn > 0), (sgn, alg)
+
+
+def test_marquis_boundary_force_is_no_force_and_wang_needs_a_normal():
+    """bf_Marquis returns at once while boundary_pressure is on
+    (boundary_force.f90:211-214): the tendencies are those of a run without
+    boundary particles.  bf_Wang with zero normals gives v_n = 0 and
+    F_t along a zero normal: no force either."""
+    sc = S.box(nside=15, spacing=1000.0, seed=4)
+    sc.vx = np.random.default_rng(3).normal(0.0, 0.1, size=(sc.n, 2))
+    sc.time = 2.0
+    nob = sc.copy()
+    nob.xb = np.zeros((0, 2))
+    nob.massb = np.zeros(0)
+    nob.bnormal = None
+    ref = capi.Oracle().single_step(nob)
+    m = sc.copy()
+    m.bfa = 3
+    out = capi.Oracle().single_step(m)
+    assert out["bnip"] > 0
+    assert _same(out["dvxdt"], ref["dvxdt"]) and _same(out["dxdt"], ref["dxdt"])
+    w = sc.copy()
+    w.bfa = 2
+    w.bnormal = np.zeros((sc.nb, 2))
+    out = capi.Oracle().single_step(w)
+    assert _same(out["dvxdt"], ref["dvxdt"])
+    # with the inward normals a particle moving into the wall is pushed back
+    w.bnormal = sc.bnormal
+    out = capi.Oracle().single_step(w)
+    assert not _same(out["dvxdt"], ref["dvxdt"])
+
+
+def test_benz_and_fixed_smoothing_lengths():
+    """sle = 0 keeps hsml bit for bit; Benz's rate (smoothing_length.f90:
+    75-109) is driven by the pair sum of (v_i - v_j) . grad W: a uniform
+    translation without walls or wind (the water drag is then the same for
+    every particle) stays uniform, the sum is zero and hsml is kept bit for
+    bit too, while Marquis' hsml = 3 sqrt(m / rho) follows the masses,
+    here 10 % above those the smoothing lengths were set from."""
+    sc = _lattice_box(12)
+    sc.wind = (0.0, 0.0)
+    sc.h[:] = 1.0
+    sc.rho[:] = 900.0
+    sc.mass = sc.mass * 1.1
+    sc.vx[:] = (0.05, 0.02)
+    for sle in (0, 1):
+        s2 = sc.copy()
+        s2.sle = sle
+        capi.Oracle().run_pc(s2, 3)
+        assert _same(s2.hsml, sc.hsml), sle
+    s2 = sc.copy()
+    capi.Oracle().run_pc(s2, 3)
+    assert not _same(s2.hsml, sc.hsml)
