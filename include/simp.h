@@ -41,7 +41,11 @@ extern "C" {
 #define SIMP_WIND_UNIFORM 0
 #define SIMP_WIND_GRID    1
 
-#define SIMP_CONFIG_VERSION 1
+#define SIMP_CONFIG_VERSION 2
+
+/* simp_instance_desc.sle value for the reference's sle = 0 (smoothing
+ * lengths kept); 0 there means the default, Marquis_hsml (sle = 2). */
+#define SIMP_SLE_KEEP (-1)
 
 /* Runtime form of the compile-time switches of modules/globals.f90:58-187
  * that reach the hot path.  The values each field lists are implemented;
@@ -53,14 +57,21 @@ typedef struct simp_config {
                             /* time_integration.f90:155-282; 1: Euler (:86-148) */
                             /* 3: leap-frog (:289-391); batches run 2           */
     int32_t nnps_algorithm; /* 2: bucket search (globals.f90:73)                */
-    int32_t sle;            /* 2: hsml = 3 sqrt(m/rho) (globals.f90:81)         */
+    int32_t sle;            /* 2: hsml = 3 sqrt(m/rho) (globals.f90:81,        */
+                            /* smoothing_length.f90:118-145); 1: Benz's rate  */
+                            /* (:40-111); 0: kept; with 0 and 1 the masses    */
+                            /* evolve (mass_evolution.f90:40-71)              */
     int32_t tse;            /* 0: keep dt, 2: VP dt (globals.f90:89); 1: the    */
-                            /* Monaghan dt (time_step.f90:39-125) without      */
-                            /* artificial viscosity                             */
+                            /* Monaghan dt (time_step.f90:39-125), with the    */
+                            /* viscous CFL terms when artificial_visc is on    */
     int32_t rheology;       /* 1: elliptical yield curve (globals.f90:96);      */
                             /* 2: Mohr-Coulomb (bulk_viscosity.f90:86-145,      */
                             /* shear_viscosity.f90:67-124)                      */
-    int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119)     */
+    int32_t bfa;            /* 1: Monaghan boundary force (globals.f90:119);    */
+                            /* 2: Wang (boundary_force.f90:109-176; reads the  */
+                            /* boundary normals and dt, simp_set_module_state);*/
+                            /* 3: Marquis (:183-223, no force while the        */
+                            /* boundary pressure is on)                        */
     int32_t eos_alg;        /* 3: Kreyscher replacement pressure (:127),        */
                             /* eos.f90:141-147; 1: Hibler P (eos.f90:59);     */
                             /* 2: Gutfraind (eos.f90:94-102)                  */
@@ -84,6 +95,13 @@ typedef struct simp_config {
     double  current[2];     /* uniform ocean current (m/s); reference (0, 0),   */
                             /* ext_force/water_drag.f90:37-38                   */
     double  max_hsml;       /* smoothing-length cap (in_out/initialize.f90:191) */
+    int32_t artificial_visc;      /* 0 / 1 (globals.f90:186)                   */
+    int32_t art_visc_alg;         /* 1: Hernquist-Katz (regularization.f90:     */
+                                  /* 102-213); 2: Monaghan (:220-308)           */
+    int32_t xsph_regularization;  /* 0 / 1: XSPH (regularization.f90:45-95);    */
+                                  /* regularization then also runs the          */
+                                  /* art_visc_alg viscosity (:29-35)            */
+    int32_t reserved2;
 } simp_config;
 
 typedef struct simp_handle simp_handle;
@@ -153,6 +171,20 @@ int simp_single_step_noalloc(simp_handle *h, int n, const double *x,
                              double *dAdt, double *eps3, double *zeta,
                              double *eta, double *p, double *sig3, int *nip,
                              int *bnip);
+
+/* ---- module state single_step reads besides its arguments ----
+ * The Wang boundary force (bfa = 2, boundary_force.f90:145-174) reads the
+ * time step dt and the boundary normals b_normal (data_variables.f90:41-46,
+ * set by boundary_placing_1, in_out/boundaries.f90:232-287): (2, nb)
+ * interleaved unit vectors pointing into the ice.  simp_single_step and
+ * simp_upload take them from the last call of this function (dt: only
+ * simp_single_step; simp_upload has its own).  b_normal may be NULL (zeros);
+ * nb must then be 0 or match the next call's boundary count.  The sound
+ * speed the artificial viscosity and the Monaghan time step read is c_yang
+ * of the given densities, as sound_speed leaves it before every
+ * single_step call of the reference (initialize.f90:199-200,
+ * time_integration.f90:140, 230, 274, 355, 383). */
+int simp_set_module_state(simp_handle *h, double dt, int nb, const double *b_normal);
 
 /* ---- kernel values of the last single_step, for a host-side density ----
  * Replaces the module arrays single_step leaves for sum_density
@@ -354,6 +386,16 @@ typedef struct simp_instance_desc {
     int32_t skf;                         /* 0 or 6: Wendland C6; 1-5, 7, 8 as  */
                                          /* simp_config.skf                    */
     int32_t reserved;
+    int32_t sle;                         /* 0 or 2: Marquis hsml; 1: Benz;     */
+                                         /* SIMP_SLE_KEEP: kept (sle = 0)      */
+    int32_t xsph;                        /* 1: XSPH regularization             */
+    int32_t art_visc;                    /* 1: artificial viscosity            */
+    int32_t art_visc_alg;                /* 0 or 1: Hernquist-Katz; 2: Monaghan*/
+    int32_t bfa;                         /* 0 or 1: Monaghan; 2: Wang; 3:      */
+                                         /* Marquis boundary force             */
+    int32_t reserved2;
+    const double *b_normal;              /* (2, nb) boundary normals (Wang),   */
+                                         /* or NULL (zeros)                    */
 } simp_instance_desc;
 
 int simp_batch_create(int precision, simp_batch **out);

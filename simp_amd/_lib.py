@@ -23,7 +23,8 @@ SIMP_FP64 = 0
 SIMP_MIXED = 1
 SIMP_WIND_UNIFORM = 0
 SIMP_WIND_GRID = 1
-SIMP_CONFIG_VERSION = 1
+SIMP_CONFIG_VERSION = 2
+SIMP_SLE_KEEP = -1
 
 #: every entry point declared in include/simp.h
 EXPORTS = (
@@ -39,7 +40,7 @@ EXPORTS = (
     "simp_batch_count", "simp_batch_list", "simp_instance_footprint",
     "simp_output_create", "simp_output_append", "simp_output_read_last",
     "simp_snapshot_size", "simp_snapshot_save", "simp_snapshot_load",
-    "simp_snapshot_info", "simp_kernel_gradients",
+    "simp_snapshot_info", "simp_kernel_gradients", "simp_set_module_state",
 )
 
 
@@ -51,7 +52,9 @@ class SimpConfig(C.Structure):
                 ("skf", C.c_int32), ("den_alg", C.c_int32), ("wind_mode", C.c_int32),
                 ("max_particles", C.c_int32), ("max_boundary", C.c_int32),
                 ("reserved", C.c_int32), ("wind", C.c_double * 2),
-                ("current", C.c_double * 2), ("max_hsml", C.c_double)]
+                ("current", C.c_double * 2), ("max_hsml", C.c_double),
+                ("artificial_visc", C.c_int32), ("art_visc_alg", C.c_int32),
+                ("xsph_regularization", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class SimpInstanceDesc(C.Structure):
@@ -68,7 +71,10 @@ class SimpInstanceDesc(C.Structure):
                 ("grid_dx", C.c_double), ("grid_dy", C.c_double),
                 ("grid_u", C.c_void_p), ("grid_v", C.c_void_p),
                 ("den_alg", C.c_int32), ("pa_sph", C.c_int32), ("eos_alg", C.c_int32),
-                ("rheology", C.c_int32), ("skf", C.c_int32), ("reserved2", C.c_int32)]
+                ("rheology", C.c_int32), ("skf", C.c_int32), ("reserved2", C.c_int32),
+                ("sle", C.c_int32), ("xsph", C.c_int32), ("art_visc", C.c_int32),
+                ("art_visc_alg", C.c_int32), ("bfa", C.c_int32), ("reserved3", C.c_int32),
+                ("b_normal", C.c_void_p)]
 
 
 class SimpTuning(C.Structure):
@@ -118,6 +124,7 @@ def lib():
     L.simp_search_stats.argtypes = [_vp, _vp, _vp, _vp]
     L.simp_kernel_values.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.simp_kernel_gradients.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp]
+    L.simp_set_module_state.argtypes = [_vp, C.c_double, C.c_int, _vp]
     L.simp_nnps.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp,
                             _vp, _vp, C.c_int, _vp, _vp, _vp, _vp]
     L.simp_upload.argtypes = ([_vp, C.c_int] + [_vp] * 7 + [C.c_int, _vp, _vp, C.c_double,

@@ -29,10 +29,16 @@ from ._lib import SimpConfig, check, lib, ptr
 def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(np.float32(-5.3))),
                 current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False,
                 den_alg: int = 3, pa_sph: int = 1, eos_alg: int = 3, rheology: int = 1,
-                skf: int = 6, int_algorithm: int = 2) -> SimpConfig:
+                skf: int = 6, int_algorithm: int = 2, sle: int = 2, xsph: int = 0,
+                art_visc: int = 0, art_visc_alg: int = 1, bfa: int = 1) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
+    cfg.sle = sle
+    cfg.xsph_regularization = xsph
+    cfg.artificial_visc = art_visc
+    cfg.art_visc_alg = art_visc_alg
+    cfg.bfa = bfa
     cfg.tse = tse
     cfg.den_alg = den_alg
     cfg.pa_sph = pa_sph
@@ -73,9 +79,12 @@ class Simp:
     def __init__(self, precision: int = _lib.SIMP_FP64, tse: int = 2,
                  wind=(0.0, float(np.float32(-5.3))), current=(0.0, 0.0),
                  max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3, pa_sph: int = 1,
-                 eos_alg: int = 3, rheology: int = 1, skf: int = 6, int_algorithm: int = 2):
+                 eos_alg: int = 3, rheology: int = 1, skf: int = 6, int_algorithm: int = 2,
+                 sle: int = 2, xsph: int = 0, art_visc: int = 0, art_visc_alg: int = 1,
+                 bfa: int = 1):
         self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
-                               den_alg, pa_sph, eos_alg, rheology, skf, int_algorithm)
+                               den_alg, pa_sph, eos_alg, rheology, skf, int_algorithm, sle,
+                               xsph, art_visc, art_visc_alg, bfa)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -92,7 +101,17 @@ class Simp:
         return cls(precision=precision, tse=sc.tse, wind=sc.wind, current=sc.current,
                    max_hsml=sc.max_hsml, wind_grid=sc.wind_grid,
                    den_alg=sc.den_alg, pa_sph=sc.pa_sph, eos_alg=sc.eos_alg,
-                   rheology=sc.rheology, skf=sc.skf, int_algorithm=sc.int_algorithm)
+                   rheology=sc.rheology, skf=sc.skf, int_algorithm=sc.int_algorithm,
+                   sle=sc.sle, xsph=sc.xsph, art_visc=sc.art_visc,
+                   art_visc_alg=sc.art_visc_alg, bfa=sc.bfa)
+
+    def set_module_state(self, sc) -> None:
+        """dt and the boundary normals of ``sc``, which the Wang boundary
+        force reads (simp_set_module_state)."""
+        bn = _c(sc.b_normal())
+        self._bn = bn
+        check(lib().simp_set_module_state(self._h, sc.dt, sc.nb, ptr(bn) if sc.nb else None),
+              "simp_set_module_state")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -145,6 +164,7 @@ class Simp:
             out["sigma"] = np.zeros((n, 3))
         nip, bnip = C.c_int(), C.c_int()
         a = _state_arrays(sc)
+        self.set_module_state(sc)
         fn = lib().simp_single_step_noalloc if noalloc else lib().simp_single_step
         st = fn(
             self._h, n, ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(a[5]),
@@ -235,6 +255,7 @@ class Simp:
     def upload(self, sc, itimestep: int = 0) -> None:
         self._n = sc.n
         a = _state_arrays(sc)
+        self.set_module_state(sc)
         check(lib().simp_upload(
             self._h, sc.n, ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(a[5]),
             ptr(a[6]), sc.nb, ptr(a[7]) if sc.nb else None, ptr(a[8]) if sc.nb else None,

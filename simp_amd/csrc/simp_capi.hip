@@ -26,6 +26,8 @@ struct simp_handle {
   std::vector<double> g_u, g_v;
   long grid_ver = 0;          // simp_set_wind_grid calls so far
   long grid_ver_loaded = -1;  // the version the resident instance was uploaded with
+  double step_dt = 0.0;       // simp_set_module_state: dt, boundary normals
+  std::vector<double> bnormal;
 };
 
 // ---- device memory of simp_single_step_noalloc ----
@@ -105,6 +107,9 @@ void simp_config_default(simp_config *c) {
   c->wind[1] = (double)(-5.3f);
   c->current[0] = c->current[1] = 0.0;
   c->max_hsml = 0.0;
+  c->artificial_visc = 0;
+  c->art_visc_alg = 1;
+  c->xsph_regularization = 0;
 }
 
 const char *simp_strerror(int s) {
@@ -125,9 +130,11 @@ static int validate(const simp_config *c) {
   if (!c || c->version != SIMP_CONFIG_VERSION) return SIMP_E_ARG;
   if (c->precision != SIMP_FP64 && c->precision != SIMP_MIXED) return SIMP_E_UNSUPPORTED;
   if (c->int_algorithm < 1 || c->int_algorithm > 3) return SIMP_E_UNSUPPORTED;
-  if (c->nnps_algorithm != 2 || c->sle != 2) return SIMP_E_UNSUPPORTED;
+  if (c->nnps_algorithm != 2 || c->sle < 0 || c->sle > 2) return SIMP_E_UNSUPPORTED;
   if (c->tse < 0 || c->tse > 2) return SIMP_E_UNSUPPORTED;
-  if ((c->rheology != 1 && c->rheology != 2) || c->bfa != 1) return SIMP_E_UNSUPPORTED;
+  if ((c->rheology != 1 && c->rheology != 2) || c->bfa < 1 || c->bfa > 3)
+    return SIMP_E_UNSUPPORTED;
+  if (c->art_visc_alg != 1 && c->art_visc_alg != 2) return SIMP_E_UNSUPPORTED;
   if (c->eos_alg < 1 || c->eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (c->pa_sph != 1 && c->pa_sph != 2) return SIMP_E_UNSUPPORTED;
   if (c->skf < 1 || c->skf > 8 || c->den_alg < 1 || c->den_alg > 3) return SIMP_E_UNSUPPORTED;
@@ -193,6 +200,15 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
   in.eos_alg = c.eos_alg;
   in.rheology = c.rheology;
   in.skf = c.skf;
+  in.sle = c.sle;
+  in.xsph = c.xsph_regularization ? 1 : 0;
+  in.art_visc = c.artificial_visc ? 1 : 0;
+  in.art_visc_alg = c.art_visc_alg;
+  in.bfa = c.bfa;
+  if (!h->bnormal.empty()) {
+    if ((int)(h->bnormal.size() / 2) != in.nb) return SIMP_E_ARG;
+    in.bnormal = h->bnormal.data();
+  }
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
     in.g_nx = h->g_nx; in.g_ny = h->g_ny;
@@ -221,6 +237,7 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
   in.xb = xb; in.massb = massb;
   in.time = current_time;
   in.itimestep = itimestep;
+  in.dt = h->step_dt;     // (read by the Wang boundary force only)
   int rc = fill_input(h, in);
   if (rc) return rc;
   // the resident layout of the last call takes the new state when it can
@@ -244,6 +261,16 @@ int simp_single_step(simp_handle *h, int n, const double *x, const double *vx,
   return SIMP_OK;
 }
 
+int simp_set_module_state(simp_handle *h, double dt, int nb, const double *b_normal) {
+  if (!h || nb < 0) return SIMP_E_ARG;
+  h->step_dt = dt;
+  if (b_normal && nb > 0)
+    h->bnormal.assign(b_normal, b_normal + 2 * (size_t)nb);
+  else
+    h->bnormal.clear();
+  return SIMP_OK;
+}
+
 int simp_single_step_noalloc(simp_handle *h, int n, const double *x, const double *vx,
                              const double *mass, const double *rho, const double *hsml,
                              const double *hth, const double *A, int nb, const double *xb,
@@ -263,6 +290,9 @@ int simp_single_step_noalloc(simp_handle *h, int n, const double *x, const doubl
   in.itimestep = itimestep;
   int rc = fill_input(h, in);
   if (rc) return rc;
+  // the regularization, smoothing-length and boundary-force switches need
+  // arrays this layout does not carve
+  if (in.sle != 2 || in.xsph || in.art_visc || in.bfa != 1) return SIMP_E_UNSUPPORTED;
   // a handle's first call starts the persistent arrays from zero, as a new
   // handle's simp_single_step does; later calls find them as the last call
   // on this GPU left them
@@ -448,12 +478,20 @@ int simp_sync(simp_handle *h) {
 void *simp_stream(simp_handle *h) { return h ? (void *)h->eng->stream() : nullptr; }
 
 // ---- exact snapshots ----
-// Blob: "SIMPSNP1", int32 n, nb, itimestep, pad; int64 nq; real8 dt, time;
+// Blob: "SIMPSNP1", int32 n, nb, itimestep, flags; int64 nq; real8 dt, time;
 // real8 x(2n), vx(2n), mass(n), rho(n), hsml(n), h(n), A(n), xb(2nb),
-// massb(nb), kernel_zero(n), hsml_ij store(nq); little endian.
+// massb(nb), kernel_zero(n), hsml_ij store(nq); then with flags bit 0 (Benz's
+// smoothing length) the persistent drhodt(n), with bit 1 (the Wang boundary
+// force) the boundary normals b_normal(2nb); little endian.
 static const char kSnapMagic[8] = {'S', 'I', 'M', 'P', 'S', 'N', 'P', '1'};
-static long long snap_bytes(long n, long nb, long nq) {
-  return 8 + 16 + 8 + 16 + 8 * (10 * n + 3 * nb + nq);
+constexpr int kSnapDrhodt = 1, kSnapNormals = 2;
+static long long snap_bytes(long n, long nb, long nq, int flags) {
+  return 8 + 16 + 8 + 16 +
+         8 * (10 * n + 3 * nb + nq + ((flags & kSnapDrhodt) ? n : 0) +
+              ((flags & kSnapNormals) ? 2 * nb : 0));
+}
+static int snap_flags(const simp_handle *h) {
+  return (h->cfg.sle == 1 ? kSnapDrhodt : 0) | (h->cfg.bfa == 2 ? kSnapNormals : 0);
 }
 
 long long simp_snapshot_size(simp_handle *h) {
@@ -461,15 +499,16 @@ long long simp_snapshot_size(simp_handle *h) {
   if (!h->uploaded) return -SIMP_E_STATE;
   const long nq = h->eng->store_cap(0);
   if (nq < 0) return -SIMP_E_STATE;
-  return snap_bytes(h->n, h->nb, nq);
+  return snap_bytes(h->n, h->nb, nq, snap_flags(h));
 }
 
 int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
   if (!h || !buf) return SIMP_E_ARG;
   if (!h->uploaded) return SIMP_E_STATE;
   const long n = h->n, nb = h->nb, nq = h->eng->store_cap(0);
+  const int flags = snap_flags(h);
   if (nq < 0) return SIMP_E_STATE;
-  if (cap < snap_bytes(n, nb, nq)) return SIMP_E_CAPACITY;
+  if (cap < snap_bytes(n, nb, nq, flags)) return SIMP_E_CAPACITY;
   int rc = h->eng->sync_and_check();
   if (rc) return map_rc(rc);
   char *b = (char *)buf;
@@ -478,7 +517,7 @@ int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
   int it = 0;
   rc = h->eng->download_scalars(0, &dt, &t, &it, nullptr, nullptr, nullptr);
   if (rc) return map_rc(rc);
-  const int32_t hdr[4] = {(int32_t)n, (int32_t)nb, (int32_t)it, 0};
+  const int32_t hdr[4] = {(int32_t)n, (int32_t)nb, (int32_t)it, (int32_t)flags};
   const int64_t q64 = nq;
   std::memcpy(b + 8, hdr, 16);
   std::memcpy(b + 24, &q64, 8);
@@ -487,9 +526,13 @@ int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
   double *p = (double *)(b + 48);
   double *x = p, *vx = x + 2 * n, *mass = vx + 2 * n, *rho = mass + n, *hsml = rho + n;
   double *hth = hsml + n, *A = hth + n, *xb = A + n, *massb = xb + 2 * nb, *kz = massb + nb;
-  double *hq = kz + n;
+  double *hq = kz + n, *drp = hq + nq;
+  double *bn = drp + ((flags & kSnapDrhodt) ? n : 0);
   rc = h->eng->download_state(0, x, vx, rho, hsml, hth, A);
-  if (!rc) rc = h->eng->download_persist(0, mass, xb, massb, kz, hq);
+  if (!rc)
+    rc = h->eng->download_persist(0, mass, xb, massb, kz, hq,
+                                  (flags & kSnapDrhodt) ? drp : nullptr,
+                                  (flags & kSnapNormals) ? bn : nullptr);
   return map_rc(rc);
 }
 
@@ -497,7 +540,7 @@ int simp_snapshot_save(simp_handle *h, void *buf, long long cap) {
 // the blob holds before any size arithmetic, so a forged or truncated
 // header cannot overflow it.
 static int snap_header(const void *buf, long long len, long *n, long *nb, long *nq, int *it,
-                       double *dt, double *t) {
+                       double *dt, double *t, int *flags) {
   if (!buf || len < 48) return SIMP_E_ARG;
   const char *b = (const char *)buf;
   if (std::memcmp(b, kSnapMagic, 8) != 0) return SIMP_E_ARG;
@@ -508,23 +551,27 @@ static int snap_header(const void *buf, long long len, long *n, long *nb, long *
   std::memcpy(dt, b + 32, 8);
   std::memcpy(t, b + 40, 8);
   const long long avail = (len - 48) / 8;
-  if (hdr[0] < 1 || hdr[1] < 0 || q < 0) return SIMP_E_ARG;
-  const long long fixed = 10LL * hdr[0] + 3LL * hdr[1];
+  if (hdr[0] < 1 || hdr[1] < 0 || q < 0 || (hdr[3] & ~(kSnapDrhodt | kSnapNormals)))
+    return SIMP_E_ARG;
+  const long long fixed = 10LL * hdr[0] + 3LL * hdr[1] +
+                          ((hdr[3] & kSnapDrhodt) ? (long long)hdr[0] : 0) +
+                          ((hdr[3] & kSnapNormals) ? 2LL * hdr[1] : 0);
   if (fixed > avail || q > avail - fixed) return SIMP_E_ARG;
-  if (len < snap_bytes(hdr[0], hdr[1], (long)q)) return SIMP_E_ARG;
+  if (len < snap_bytes(hdr[0], hdr[1], (long)q, hdr[3])) return SIMP_E_ARG;
   *n = hdr[0];
   *nb = hdr[1];
   *nq = (long)q;
   *it = hdr[2];
+  *flags = hdr[3];
   return SIMP_OK;
 }
 
 int simp_snapshot_info(const void *buf, long long len, int *n, int *nb, int *itimestep,
                        double *current_time) {
   long n_, nb_, nq_;
-  int it;
+  int it, fl;
   double dt, t;
-  const int rc = snap_header(buf, len, &n_, &nb_, &nq_, &it, &dt, &t);
+  const int rc = snap_header(buf, len, &n_, &nb_, &nq_, &it, &dt, &t, &fl);
   if (rc) return rc;
   if (n) *n = (int)n_;
   if (nb) *nb = (int)nb_;
@@ -536,19 +583,24 @@ int simp_snapshot_info(const void *buf, long long len, int *n, int *nb, int *iti
 int simp_snapshot_load(simp_handle *h, const void *buf, long long len) {
   if (!h) return SIMP_E_ARG;
   long n, nb, nq;
-  int it;
+  int it, flags;
   double dt, t;
-  const int hrc = snap_header(buf, len, &n, &nb, &nq, &it, &dt, &t);
+  const int hrc = snap_header(buf, len, &n, &nb, &nq, &it, &dt, &t, &flags);
   if (hrc) return hrc;
+  // the blob carries what the handle's switches persist
+  if (flags != snap_flags(h)) return SIMP_E_ARG;
   const char *b = (const char *)buf;
   const double *p = (const double *)(b + 48);
   const double *x = p, *vx = x + 2 * n, *mass = vx + 2 * n, *rho = mass + n, *hsml = rho + n;
   const double *hth = hsml + n, *A = hth + n, *xb = A + n, *massb = xb + 2 * nb, *kz = massb + nb;
-  const double *hq = kz + n;
+  const double *hq = kz + n, *drp = hq + nq;
+  const double *bn = drp + ((flags & kSnapDrhodt) ? n : 0);
+  if (flags & kSnapNormals) h->bnormal.assign(bn, bn + 2 * nb);
   int rc = simp_upload(h, (int)n, x, vx, mass, rho, hsml, hth, A, (int)nb, nb ? xb : nullptr,
                        nb ? massb : nullptr, dt, t, it);
   if (rc) return rc;
-  return map_rc(h->eng->upload_persist(0, kz, hq, (long)nq));
+  return map_rc(h->eng->upload_persist(0, kz, hq, (long)nq,
+                                       (flags & kSnapDrhodt) ? drp : nullptr));
 }
 
 int simp_download(simp_handle *h, double *x, double *vx, double *rho, double *hsml, double *hth,
@@ -631,6 +683,9 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.eos_alg < 0 || d.eos_alg > 3) return SIMP_E_UNSUPPORTED;
   if (d.rheology < 0 || d.rheology > 2) return SIMP_E_UNSUPPORTED;
   if (d.skf < 0 || d.skf > 8) return SIMP_E_UNSUPPORTED;
+  if (d.sle != SIMP_SLE_KEEP && (d.sle < 0 || d.sle > 2)) return SIMP_E_UNSUPPORTED;
+  if (d.art_visc_alg < 0 || d.art_visc_alg > 2 || d.bfa < 0 || d.bfa > 3)
+    return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))
@@ -646,6 +701,12 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.eos_alg = d.eos_alg == 0 ? 3 : d.eos_alg;
   in.rheology = d.rheology == 2 ? 2 : 1;
   in.skf = d.skf == 0 ? 6 : d.skf;
+  in.sle = d.sle == SIMP_SLE_KEEP ? 0 : (d.sle == 1 ? 1 : 2);
+  in.xsph = d.xsph ? 1 : 0;
+  in.art_visc = d.art_visc ? 1 : 0;
+  in.art_visc_alg = d.art_visc_alg == 2 ? 2 : 1;
+  in.bfa = d.bfa == 0 ? 1 : d.bfa;
+  in.bnormal = d.b_normal;
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
   in.current[0] = d.current[0]; in.current[1] = d.current[1];

@@ -35,6 +35,11 @@ constexpr double kHsmlMult = 3.0;
 // Young modulus and Poisson ratio, the latter a default-REAL literal
 constexpr double kYoung = 1.0e9;
 constexpr double kNu = (double)0.3f;
+// artificial viscosity, XSPH (data_variables.f90:171-173): alpha_pi = 1.,
+// beta_pi = 2. (exact), chi = 0.3 (default REAL)
+constexpr double kAlphaPi = 1.0;
+constexpr double kBetaPi = 2.0;
+constexpr double kChi = (double)0.3f;
 constexpr int kMaxIndInt = 200;     // globals.f90:24
 constexpr int kPrintStep = 2000;    // globals.f90:55
 
@@ -42,6 +47,7 @@ constexpr int kTPB = 256;           // threads per block of particle kernels
 constexpr int kScanItems = 2048;    // elements per block of the scan
 constexpr int kMaxBoundaryPerInst = 8192;
 constexpr int kNbrMax = 32;         // neighbours kept per particle in the list
+constexpr int kPairBlk = 256;       // pair ranks per block of the viscosity's pi_ij passes
 // per-block partial reductions: min/max x, min/max y, min hsml, max speed,
 // max hsml change, and for the Monaghan time step min hsml/|a|, min hsml/c
 constexpr int kParts = 9;
@@ -65,6 +71,15 @@ struct InstMeta {
   int rheology;          // 1: elliptical yield curve; 2: Mohr-Coulomb
   int skf;               // 6: Wendland C6; 4: C2; 5: C4; 1 quadratic, 2 Gaussian,
                          // 3 quintic spline, 7 / 8 Marquis
+  int sle;               // 2: Marquis hsml; 1: Benz hsml; 0: hsml unchanged
+  int xsph;              // 1: XSPH position regularization
+  int avis;              // artificial viscosity regularization runs with: 0 none,
+                         // 1 Hernquist-Katz, 2 Monaghan (whenever XSPH or
+                         // artificial_visc is on, regularization.f90:29-35)
+  int needc;             // 1: the sound speed is c_yang of the density (else 0)
+  int avdt;              // 1: the Monaghan time step takes the viscous CFL terms
+  int bfa;               // boundary force: 1 Monaghan, 2 Wang, 3 Marquis
+  int endp;              // 1: an end-of-step pair pass (Benz hsml, viscous CFL)
   int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;
@@ -121,7 +136,9 @@ struct BlockEntry {
   int cs;            // the instance's current state set (sel ^ rb): 0 after a
                      // repack, moved by k_permute when the instance rebuilds,
                      // unchanged by the fold that ends the evaluation
-  int pad[2];
+  int tmass;         // 1: the instance's masses evolve (sle 0 or 1) and the
+                     // half-step state's are in masst
+  int pad;
 };
 
 // Device pointers of one engine.
@@ -130,6 +147,10 @@ struct DevState {
   // [2 * set + buf], buf 0 = S (step start / current), 1 = T (half step)
   double *x[4], *y[4], *vx[4], *vy[4], *h[4], *A[4], *rho[4], *hsml[4];
   double *mass[2];
+  // the half-step state's masses of instances whose masses evolve (sle 0, 1:
+  // mass_evolution.f90:40-71 after every density update), by set; null when
+  // no instance's do
+  double *masst[2];
   int *orig[2];         // reference (0-based) particle index of each slot
   int *cellid[2];       // cell (local id) of each slot
   // cell lists: members per cell (kept current by the key updates), their
@@ -186,6 +207,18 @@ struct DevState {
   int2 *brun;
   int *blist, *bcellc;
   double *hq;            // persistent hsml_ij store (stale semantics)
+  // switches beyond the defaults (allocated only when an instance uses one):
+  double *bnx, *bny;     // boundary normals (the Wang boundary force)
+  double *drp;           // Benz's drhodt, by p_off + reference index: persists
+                         // across steps like the reference's module array
+  double *lv;            // per slot: the Hernquist-Katz viscosity's lv
+  int *pidx;             // per (list entry, slot), j-major like nbr: the pair's
+                         // rank in the packed pair ranks (pair_start)
+  double *piv;           // per pair rank: the artificial viscosity's pi_ij
+  int *pflg;             // per pair rank: 1 where the pair sets pi_ij
+  int *pblast;           // per block of kPairBlk ranks: its last setting rank (-1: none)
+  double *hnew, *cfl;    // per slot, Euler: the end pass's hsml and CFL term
+  int pcap;              // capacity of piv / pflg (pair ranks)
   double *grid;          // wind grids, u then v per instance
   InstMeta *meta;
   InstDyn *dyn;

@@ -27,6 +27,12 @@ struct InstanceInput {
   int eos_alg = 3;       // 1: Hibler P; 2: Gutfraind; 3: Kreyscher replacement pressure
   int rheology = 1;      // 1: elliptical yield curve; 2: Mohr-Coulomb
   int skf = 6;           // 6: Wendland C6; 4: Wendland C2; 5: Wendland C4
+  int sle = 2;           // 2: Marquis hsml; 1: Benz hsml; 0: hsml unchanged
+  int xsph = 0;          // 1: XSPH regularization
+  int art_visc = 0;      // 1: artificial viscosity
+  int art_visc_alg = 1;  // 1: Hernquist-Katz; 2: Monaghan
+  int bfa = 1;           // 1: Monaghan; 2: Wang; 3: Marquis boundary force
+  const double *bnormal = nullptr;   // (2, nb) boundary normals (Wang); null: zeros
   // optional wind grid
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;
@@ -135,9 +141,12 @@ class Engine {
   // -- masses (reference order), boundary particles, kernel_zero and the
   // hsml_ij store (store_cap entries) -- and their restore after an upload
   long store_cap(int inst) const;
+  // (drp: Benz's persistent drhodt, by reference index; bnormal: (2, nb)
+  // boundary normals of the Wang force)
   int download_persist(int inst, double *mass, double *xb, double *massb, double *kz,
-                       double *hq);
-  int upload_persist(int inst, const double *kz, const double *hq, long nq);
+                       double *hq, double *drp = nullptr, double *bnormal = nullptr);
+  int upload_persist(int inst, const double *kz, const double *hq, long nq,
+                     const double *drp = nullptr);
   int cur_set(int inst);
 
   // 1: rebuild the cell and candidate lists at every evaluation (the
@@ -194,6 +203,8 @@ class Engine {
   bool any_boundary_ = false;
   bool any_xsw_ = false;              // some instance uses a non-default switch value
   bool any_cden_ = false;             // some instance uses continuity density
+  bool any_av_ = false;               // some instance runs an artificial viscosity
+  bool any_endp_ = false;             // some instance needs the end-of-step pair pass
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   bool force_search_ = false;

@@ -23,7 +23,16 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // kernels then run their variants that read each instance's values
 static inline bool nondefault(const InstMeta &m) {
   return m.den_alg != 3 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
-         m.tse == 1;
+         m.tse == 1 || m.sle != 2 || m.xsph || m.avis || m.bfa != 1;
+}
+
+// device bytes of the arrays the regularization and smoothing-length
+// switches add, per particle of an instance that uses them
+static size_t ext_bytes_per_particle(const InstanceInput &in) {
+  size_t b = 0;
+  if (in.sle != 2) b += 8 + 2 * 8 + 2 * 8;                   // drp, masst, hnew / cfl
+  if (in.xsph || in.art_visc) b += 8 + kNbrMax * 4 + (kNbrMax / 2) * 12;   // lv, pidx, pairs
+  return b;
 }
 
 // ------------------------------------------------------------ host sizing
@@ -108,7 +117,8 @@ constexpr size_t kBytesPerCell = 9 * 4 + 8 + 8;
 size_t instance_footprint(const InstanceInput &in) {
   long c = cell_capacity(in);
   if (c < 0) c = 0;
-  return kBytesPerParticle * (size_t)in.n + kBytesPerBoundary * (size_t)in.nb +
+  return (kBytesPerParticle + ext_bytes_per_particle(in)) * (size_t)in.n +
+         (kBytesPerBoundary + (in.bfa == 2 ? 16 : 0)) * (size_t)in.nb +
          kBytesPerCell * (size_t)c + 16 * (size_t)store_capacity(in) +
          16 * (size_t)in.g_nx * (size_t)in.g_ny + 4096;
 }
@@ -309,6 +319,16 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.eos_alg = in.eos_alg;
     m.rheology = in.rheology;
     m.skf = in.skf;
+    m.sle = in.sle;
+    m.xsph = in.xsph ? 1 : 0;
+    // regularization runs for XSPH or the artificial viscosity, and then
+    // always with the viscosity art_visc_alg names (regularization.f90:29-35)
+    m.avis = (in.xsph || in.art_visc) ? in.art_visc_alg : 0;
+    // c_yang is computed only for these switches (sound_speed.f90:21)
+    m.needc = (in.art_visc || in.tse == 1) ? 1 : 0;
+    m.avdt = (in.art_visc && in.tse == 1) ? 1 : 0;
+    m.bfa = in.bfa;
+    m.endp = (in.sle == 1 || m.avdt) ? 1 : 0;
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -349,6 +369,34 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   int *nscan = nullptr;
   int rc = alloc_state(nd, hip_alloc, &nscan, ntot, nbtot, ctot, nblk, ninst, gtot) ? -6 : 0;
   if (rc == 0 && dalloc(hip_alloc, &nd.kz, ntot)) rc = -6;
+  // the switches' arrays, when an instance uses them
+  const bool x_sle = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.sle != 2; });
+  const bool x_av = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.avis != 0; });
+  const bool x_wang = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.bfa == 2; });
+  const bool x_end = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.endp != 0; });
+  // pair ranks of the viscosity's pi_ij: every pair of every instance has a
+  // rank in the packed order; the viscosity needs the stored neighbour lists
+  // (at most kNbrMax entries, each pair in two lists), so kNbrMax / 2 per
+  // particle bounds the pairs
+  const long pcap = x_av ? std::max<long>((long)(kNbrMax / 2) * ntot, kPairBlk) : 0;
+  if (rc == 0 && x_sle) {
+    if (dalloc(hip_alloc, &nd.drp, ntot) || dalloc(hip_alloc, &nd.masst[0], ntot) ||
+        dalloc(hip_alloc, &nd.masst[1], ntot))
+      rc = -6;
+  }
+  if (rc == 0 && x_end) {
+    if (dalloc(hip_alloc, &nd.hnew, ntot) || dalloc(hip_alloc, &nd.cfl, ntot)) rc = -6;
+  }
+  if (rc == 0 && x_av) {
+    if (dalloc(hip_alloc, &nd.lv, ntot) || dalloc(hip_alloc, &nd.pidx, (size_t)kNbrMax * ntot) ||
+        dalloc(hip_alloc, &nd.piv, pcap) || dalloc(hip_alloc, &nd.pflg, pcap) ||
+        dalloc(hip_alloc, &nd.pblast, (size_t)cdiv(pcap, kPairBlk)))
+      rc = -6;
+  }
+  if (rc == 0 && x_wang) {
+    if (dalloc(hip_alloc, &nd.bnx, nbtot) || dalloc(hip_alloc, &nd.bny, nbtot)) rc = -6;
+  }
+  nd.pcap = (int)pcap;
   if (rc == 0 && hipMalloc((void **)&nd.hq, std::max<long>(qtot, 1) * sizeof(double)) != hipSuccess)
     rc = -6;
   if (rc == 0 && dalloc(hip_alloc, &nd.hpair, qtot)) rc = -6;
@@ -374,6 +422,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   RCHK(hipMemsetAsync(nd.cellid[0], 0xff, std::max<long>(ntot, 1) * sizeof(int), st_));
   // kernel_zero starts at zero, as the reference's module array does
   RCHK(hipMemsetAsync(nd.kz, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
+  // and so does Benz's drhodt (initialize.f90:44)
+  if (nd.drp) RCHK(hipMemsetAsync(nd.drp, 0, std::max<long>(ntot, 1) * sizeof(double), st_));
   RCHK(hipMemsetAsync(nd.bflag, 0xff, (ctot + 1) * sizeof(int), st_));
   RCHK(hipMemsetAsync(nd.bcellc, 0xff, (ctot + 1) * sizeof(int), st_));
   // boundary lists are (re)built at the first evaluation of a layout
@@ -424,6 +474,11 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     RCHK(d2d(nd.yb + m.b_off, d_.yb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.massb + m.b_off, d_.massb + o.b_off, (size_t)m.nb * 8));
     RCHK(d2d(nd.hq + m.q_off, d_.hq + o.q_off, (size_t)m.q_cap * 8));
+    if (m.sle == 1) RCHK(d2d(nd.drp + m.p_off, d_.drp + o.p_off, n8));
+    if (m.bfa == 2) {
+      RCHK(d2d(nd.bnx + m.b_off, d_.bnx + o.b_off, (size_t)m.nb * 8));
+      RCHK(d2d(nd.bny + m.b_off, d_.bny + o.b_off, (size_t)m.nb * 8));
+    }
     if (m.wind_mode == 1)
       RCHK(d2d(nd.grid + m.g_off, d_.grid + o.g_off, 16 * (size_t)m.g_nx * m.g_ny));
   }
@@ -432,8 +487,11 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   if (na > 0 && carry_store_from >= 0 && carry_store_from < (int)meta_.size() && d_.hq) {
     const InstMeta &o = meta_[carry_store_from];
     RCHK(d2d(nd.hq + nm[nk].q_off, d_.hq + o.q_off, (size_t)std::min(o.q_cap, nm[nk].q_cap) * 8));
-    // kernel_zero persists by particle index across calls as well
+    // kernel_zero persists by particle index across calls as well, and so
+    // does Benz's drhodt
     RCHK(d2d(nd.kz + nm[nk].p_off, d_.kz + o.p_off, (size_t)std::min(o.n, nm[nk].n) * 8));
+    if (nd.drp && d_.drp)
+      RCHK(d2d(nd.drp + nm[nk].p_off, d_.drp + o.p_off, (size_t)std::min(o.n, nm[nk].n) * 8));
   }
 
   // added instances: host to device (staged in SoA)
@@ -447,17 +505,19 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     const InstanceInput &in = add[j];
     const InstMeta &m = nm[nk + j];
     const int n = in.n;
-    keepalive.emplace_back(12 * (size_t)n + 3 * (size_t)in.nb);
+    keepalive.emplace_back(12 * (size_t)n + 5 * (size_t)in.nb);
     std::vector<double> &st = keepalive.back();
     double *hx = st.data(), *hy = hx + n, *hvx = hy + n, *hvy = hvx + n;
     for (int i = 0; i < n; ++i) {
       hx[i] = in.x[2 * i]; hy[i] = in.x[2 * i + 1];
       hvx[i] = in.vx[2 * i]; hvy[i] = in.vx[2 * i + 1];
     }
-    double *hxb = hvy + n, *hyb = hxb + in.nb;
+    double *hxb = hvy + n, *hyb = hxb + in.nb, *hnx = hyb + in.nb, *hny = hnx + in.nb;
     for (int b = 0; b < in.nb; ++b) {
       hxb[b] = in.xb[2 * b];
       hyb[b] = in.xb[2 * b + 1];
+      hnx[b] = in.bnormal ? in.bnormal[2 * b] : 0.0;
+      hny[b] = in.bnormal ? in.bnormal[2 * b + 1] : 0.0;
     }
     keepalive_i.emplace_back(n);
     for (int i = 0; i < n; ++i) keepalive_i.back()[i] = i;
@@ -476,6 +536,10 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
       RCHK(h2d(nd.xb + m.b_off, hxb, (size_t)in.nb * 8));
       RCHK(h2d(nd.yb + m.b_off, hyb, (size_t)in.nb * 8));
       RCHK(h2d(nd.massb + m.b_off, in.massb, (size_t)in.nb * 8));
+      if (m.bfa == 2) {
+        RCHK(h2d(nd.bnx + m.b_off, hnx, (size_t)in.nb * 8));
+        RCHK(h2d(nd.bny + m.b_off, hny, (size_t)in.nb * 8));
+      }
     }
     if (in.wind_mode == 1) {
       const size_t gg = (size_t)in.g_nx * in.g_ny;
@@ -500,7 +564,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   for (int k = 0; k < ninst; ++k)
     for (int f = 0; f < nm[k].n; f += kTPB)
       lg.push_back(BlockEntry{k, f, (int)lg.size(), std::min(nm[k].n - f, kTPB),
-                              nm[k].p_off + f, 0, {0, 0}});
+                              nm[k].p_off + f, 0, nm[k].sle != 2 ? 1 : 0, 0});
   std::vector<BlockEntry> phys(lg.size());
   {
     const int nb_ = (int)lg.size(), q = nb_ / 8, r = nb_ % 8;
@@ -540,6 +604,8 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
   // its own values)
   any_xsw_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return nondefault(m); });
   any_cden_ = std::any_of(nm.begin(), nm.end(), [](const InstMeta &m) { return m.den_alg == 2; });
+  any_av_ = x_av;
+  any_endp_ = x_end;
   if (new_ids) new_ids->assign(nids.begin() + nk, nids.end());
   if (ninst > 0) {
     // grid of the next evaluation of every instance
@@ -586,8 +652,13 @@ int Engine::reload(const InstanceInput &in, bool grid_same) {
   const InstMeta &o = meta_[0];
   if (in.n != o.n || in.nb != o.nb || in.tse != o.tse || in.wind_mode != o.wind_mode ||
       in.den_alg != o.den_alg || in.pa_sph != o.pa_sph || in.eos_alg != o.eos_alg ||
-      in.rheology != o.rheology || in.skf != o.skf)
+      in.rheology != o.rheology || in.skf != o.skf || in.sle != o.sle ||
+      (in.xsph ? 1 : 0) != o.xsph || in.bfa != o.bfa ||
+      ((in.xsph || in.art_visc) ? in.art_visc_alg : 0) != o.avis ||
+      (in.art_visc || in.tse == 1 ? 1 : 0) != o.needc)
     return 1;
+  // (the boundary normals of the Wang force are laid out with the boundary)
+  if (in.bfa == 2) return 1;
   if (in.wind_mode == 1 && (!grid_same || in.g_nx != o.g_nx || in.g_ny != o.g_ny)) return 1;
   if (!in.x || !in.vx || !in.mass || !in.rho || !in.hsml || !in.hth || !in.A) return -1;
   if (in.nb > 0 && (!in.xb || !in.massb)) return -1;
@@ -949,11 +1020,13 @@ void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
   L("passA", [&] {
     launch_passA(d, buf, precision_, any_xsw_, want_diag ? 1 : 0, tune_.passA_batch, st_);
   });
-  if (phase == 2) {
-    // all pair ranks: nip is reported
+  if (phase == 2 || any_av_) {
+    // all pair ranks: nip is reported, and the artificial viscosity's pi_ij
+    // runs over the pairs in their order
     L("pairtot", [&] { launch_pairtot(d, st_); });
     L("scan", [&] { launch_scan(d.pair_count, d.ctot, scan_tmp_, d.pair_start, 0, nullptr, st_); });
   }
+  if (any_av_) L("avpairs", [&] { launch_av(d, buf, st_); });
   // boundary-pair ranks and the head of the pair ranks the boundary kernel reads
   if (any_boundary_) L("ranks", [&] { launch_ranks(d, buf, st_); });
   L("passB", [&] { launch_passB(d, precision_, any_xsw_, phase, tune_.passB_batch, st_); });
@@ -964,6 +1037,7 @@ void Engine::enqueue_step() {
     // euler_scheme: one evaluation of the step's state, then the update
     enqueue_eval(0, 3, diag_);
     if (any_cden_) L("condens", [&] { launch_condens(d_, 1, st_); });
+    if (any_endp_) L("endpass", [&] { launch_endpass(d_, 1, st_); });
     L("euler", [&] { launch_euler(d_, st_); });
     L("finalize", [&] { launch_finalize(d_, 1, st_); });
     return;
@@ -974,6 +1048,7 @@ void Engine::enqueue_step() {
   L("finalize", [&] { launch_finalize(d_, 0, st_); });
   enqueue_eval(1, scheme_ == 3 ? 4 : 1, diag_);
   if (any_cden_ && scheme_ == 3) L("condens", [&] { launch_condens(d_, 2, st_); });
+  if (any_endp_) L("endpass", [&] { launch_endpass(d_, 0, st_); });
   L("finalize", [&] { launch_finalize(d_, 1, st_); });
 }
 
@@ -1313,7 +1388,7 @@ long Engine::store_cap(int k) const {
 }
 
 int Engine::download_persist(int k, double *mass, double *xb, double *massb, double *kz,
-                             double *hq) {
+                             double *hq, double *drp, double *bnormal) {
   if (k < 0 || k >= d_.ninst) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   const InstMeta &m = meta_[k];
@@ -1341,13 +1416,32 @@ int Engine::download_persist(int k, double *mass, double *xb, double *massb, dou
   if (kz) HIPCHK(hipMemcpy(kz, d_.kz + m.p_off, (size_t)m.n * 8, hipMemcpyDeviceToHost));
   if (hq && m.q_cap > 0)
     HIPCHK(hipMemcpy(hq, d_.hq + m.q_off, (size_t)m.q_cap * 8, hipMemcpyDeviceToHost));
+  if (drp) {
+    if (!d_.drp) return -5;
+    HIPCHK(hipMemcpy(drp, d_.drp + m.p_off, (size_t)m.n * 8, hipMemcpyDeviceToHost));
+  }
+  if (bnormal && m.nb > 0) {
+    if (!d_.bnx) return -5;
+    std::vector<double> nx(m.nb), ny(m.nb);
+    HIPCHK(hipMemcpy(nx.data(), d_.bnx + m.b_off, (size_t)m.nb * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ny.data(), d_.bny + m.b_off, (size_t)m.nb * 8, hipMemcpyDeviceToHost));
+    for (int b = 0; b < m.nb; ++b) {
+      bnormal[2 * b] = nx[b];
+      bnormal[2 * b + 1] = ny[b];
+    }
+  }
   return 0;
 }
 
-int Engine::upload_persist(int k, const double *kz, const double *hq, long nq) {
+int Engine::upload_persist(int k, const double *kz, const double *hq, long nq,
+                           const double *drp) {
   if (k < 0 || k >= d_.ninst) return -1;
   const InstMeta &m = meta_[k];
   if (kz) HIPCHK(hipMemcpyAsync(d_.kz + m.p_off, kz, (size_t)m.n * 8, hipMemcpyHostToDevice, st_));
+  if (drp) {
+    if (!d_.drp) return -5;
+    HIPCHK(hipMemcpyAsync(d_.drp + m.p_off, drp, (size_t)m.n * 8, hipMemcpyHostToDevice, st_));
+  }
   const long q = std::min<long>(nq, m.q_cap);
   if (hq && q > 0)
     HIPCHK(hipMemcpyAsync(d_.hq + m.q_off, hq, (size_t)q * 8, hipMemcpyHostToDevice, st_));

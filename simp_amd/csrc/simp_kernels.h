@@ -34,6 +34,15 @@ void launch_euler(const DevState &d, hipStream_t st);
 // after the predictor, 1 before the Euler update, 2 after the leap-frog's
 // second evaluation
 void launch_condens(const DevState &d, int mode, hipStream_t st);
+// artificial viscosity (instances with avis != 0), after pass A and the pair
+// ranks (launch_pairtot, launch_scan): each pair's pi_ij in the packed pair
+// ranks, the value a pair that does not set one keeps from the pair before
+// it, and every list entry's pair rank
+void launch_av(const DevState &d, int buf, hipStream_t st);
+// end-of-step pair pass (instances with endp): mode 0 after the corrector or
+// the leap-frog's second evaluation, 1 after the Euler evaluation, before
+// launch_euler
+void launch_endpass(const DevState &d, int mode, hipStream_t st);
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, double *dwdx, hipStream_t st);
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,

@@ -252,12 +252,31 @@ __device__ __forceinline__ void block_partials(const DevState &d, int k, bool ok
 // particle without artificial viscosity: hsml / |a| with a the evaluation's
 // dvxdt, and hsml / c with c from c_yang (sound_speed.f90) at the step's
 // final density.
+// mass_update_from_area (mass_evolution.f90:66-69): rho_ice A h pi
+// (alpha hsml)^2, alpha = 1. / (2 hsml_multiplicity) a default-REAL quotient
+// (data_variables.f90:169)
+__device__ __forceinline__ double area_mass(double A, double h, double hs) {
+  const double t = (double)(1.0f / 6) * hs;
+  return kRhoIce * A * h * (kPi * (t * t));
+}
+__device__ __forceinline__ double sound_speed(double rho) {
+  // c_yang (sound_speed.f90:57-58)
+  return sqrt(kYoung * (1 - kNu) / (rho * (1 + kNu) * (1 - 2 * kNu)));
+}
 __device__ __forceinline__ void monaghan_terms(double hs, double rho, double ax, double ay,
                                                double &fo, double &fc) {
   const double acc = sqrt(ax * ax + ay * ay);
-  const double c = sqrt(kYoung * (1 - kNu) / (rho * (1 + kNu) * (1 - 2 * kNu)));
+  const double c = sound_speed(rho);
   fo = hs / acc;
   fc = hs / c;
+}
+// With artificial viscosity the CFL term (time_step.f90:105-107) takes the
+// velocity divergence div_v of the last evaluation's pairs at the step's
+// final velocities; 0.3 and 1.2 are default-REAL literals.
+__device__ __forceinline__ double monaghan_cfl_av(double hs, double rho, double div_v) {
+  const double c = sound_speed(rho);
+  const double ad = fabs(div_v);
+  return (double)0.3f * hs / (c + hs * ad + (double)1.2f * (kAlphaPi * c + kBetaPi * hs * ad));
 }
 
 // For one instance, by one block: the cells of the boundary particles (whose
@@ -825,6 +844,7 @@ __global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
       d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
     }
     d.mass[t][s] = d.mass[f][q];
+    if (buf == 1 && d.masst[0]) d.masst[t][s] = d.masst[f][q];
     d.orig[t][s] = d.orig[f][q];
     key = c;
     d.cellid[t][s] = key;
@@ -1216,7 +1236,8 @@ k_passA(DevState d, int buf, int want_diag) {
   // the evaluated state, by slot (= list position)
   const int ss = be.cs, sb = 2 * ss + buf;
   const double *X = d.x[sb], *Y = d.y[sb], *HS = d.hsml[sb], *RS = d.rho[sb];
-  const double *VX = d.vx[sb], *VY = d.vy[sb], *MS = d.mass[ss];
+  const double *VX = d.vx[sb], *VY = d.vy[sb];
+  const double *MS = (XSW && buf && be.tmass) ? d.masst[ss] : d.mass[ss];
   const int pos0 = be.pos;                // first slot of the block
   const int nloc = be.nloc;
   const bool valid = (int)threadIdx.x < nloc;
@@ -1310,12 +1331,15 @@ k_passA(DevState d, int buf, int want_diag) {
   // length, and through it the pair predicate d <= mhsml, which is not
   // continuous -- an FP32 error there flips borderline pairs.
   const bool sumd = XSW && m.den_alg == 1;
+  // the Hernquist-Katz viscosity's velocity divergence (regularization.f90
+  // :148-167, each term over rho_i; FP64 in both precision modes)
+  const bool hk = XSW && m.avis == 1;
   double div = 0., g1 = 0., g2 = 0., g3 = 0., g4 = 0.;
-  double num = 0., den = 0.;
+  double num = 0., den = 0., dvhk = 0.;
   KSum fdiv, f1, f2, f3, f4;
   // the pair predicate; d itself only where the FP64 sums use it
   auto support = [&](double d2, double mh, double &r) {
-    return (MIXED && !sumd) ? in_support_nr(d2, mh) : in_support(d2, mh, r);
+    return (MIXED && !sumd && !hk) ? in_support_nr(d2, mh) : in_support(d2, mh, r);
   };
   // kernel gradient, vector_divergence and vector_gradient of one pair
   const int skf = XSW ? m.skf : 6;
@@ -1327,6 +1351,11 @@ k_passA(DevState d, int buf, int want_diag) {
       const double w = kern_w(skf, r, mh);
       den = den + mq * w / rq;
       num = num + mq * w;
+    }
+    if (hk) {
+      const double gg = kern_grad(skf, r, mh);
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
+      dvhk = dvhk - mq * (dv0 * dw0 + dv1 * dw1) / rhoi;
     }
     if (!MIXED) {
       const double gg = kern_grad(skf, r, mh);
@@ -1554,6 +1583,14 @@ k_passA(DevState d, int buf, int want_diag) {
     d.kz[m.p_off + d.orig[ss][self]] = kzi;
     d.dnum[self] = num;
     d.dsum[self] = num / den;
+  }
+  if (hk) {
+    // no viscosity for diverging flow; lv (regularization.f90:170-176) with
+    // the sound speed the switches leave (c_yang of the density, or zero)
+    const double dv = dvhk >= 0. ? 0. : dvhk;
+    const double ci = m.needc ? sound_speed(rhoi) : 0.0;
+    const double t = fabs(dv) * hi;
+    d.lv[self] = (kAlphaPi * hi * ci * fabs(dv) + kBetaPi * (t * t)) * rhoi;
   }
   // boundary pairs per (boundary particle, cell of its 3x3 block): the
   // counts bucket_search's boundary loop finds there (nnps.f90:453-492),
@@ -1928,7 +1965,9 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   // the evaluated state, by slot (= list position)
   const int cs = be.cs;
   const double *X = d.x[2 * cs + buf], *Y = d.y[2 * cs + buf], *HS = d.hsml[2 * cs + buf], *RS = d.rho[2 * cs + buf];
-  const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf], *MS = d.mass[cs];
+  const double *VX = d.vx[2 * cs + buf], *VY = d.vy[2 * cs + buf];
+  // (evolving masses: the half-step state's own)
+  const double *MS = (XSW && buf && be.tmass) ? d.masst[cs] : d.mass[cs];
   // FP64: pass A's stress; mixed: its per-slot record (d.srec)
   const double *S11 = (const double *)d.sig11, *S12 = (const double *)d.sig12,
                *S22 = (const double *)d.sig22;
@@ -1977,6 +2016,16 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     // summation-density instance: pass A's density (predictor) or its
     // numerator and density (corrector)
     const bool sd = XSW && m.den_alg == 1;
+    // regularization (single_step.f90:99-101): the XSPH sum over the pairs,
+    // and the artificial viscosity's pair terms after the drag (FP64 in both
+    // precision modes)
+    const bool xs_on = XSW && m.xsph;
+    const bool av_on = XSW && m.avis != 0;
+    // smoothing length at the end of the step: Marquis_hsml here (sle = 2),
+    // kept (0), or Benz's rate in the end pass (1)
+    const bool marq = !XSW || m.sle == 2;
+    // the end pass takes the new hsml, the Monaghan terms and the partials
+    const bool endp = XSW && m.endp && (phase == 1 || phase == 4);
     double dsi = 1.0, dnumi = 0.;
     if (sd && phase != 2 && phase != 3) dsi = d.dsum[self];
     if (sumd3) dnumi = d.dnum[self];
@@ -1999,11 +2048,13 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     const double dAdt = MIXED ? -(Ai * dri) : -(Ai * divi / rhoi);
     const double dt = y.dt;
     if (phase == 0) {
-      px_new = xi + vxi * dt / 2;
-      py_new = yi + vyi * dt / 2;
       hsml_new = hi;
-      d.x[2 * cs + 1][p] = px_new;
-      d.y[2 * cs + 1][p] = py_new;
+      if (!xs_on) {
+        px_new = xi + vxi * dt / 2;
+        py_new = yi + vyi * dt / 2;
+        d.x[2 * cs + 1][p] = px_new;
+        d.y[2 * cs + 1][p] = py_new;
+      }
       const double hh = hthi + dhdt * dt / 2;
       double Ah = Ai + dAdt * dt / 2;
       if (Ah > 1) Ah = 1;
@@ -2013,27 +2064,36 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       // pass A computed on the step-start kernel values
       d.rho[2 * cs + 1][p] = sd ? dsi : hh * kRhoIce;
       d.hsml[2 * cs + 1][p] = hi;
+      // mass_evolution (:227, :352) of evolving masses
+      if (XSW && be.tmass) d.masst[cs][p] = area_mass(Ah, hh, hi);
     } else if (phase == 1) {
-      const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
       const double h12 = h1 + dhdt * dt / 2, A12 = A1 + dAdt * dt / 2;
-      px_new = 2 * x12 - x1;
-      py_new = 2 * y12 - y1;
-      d.x[2 * cs + 0][p] = px_new;
-      d.y[2 * cs + 0][p] = py_new;
+      if (!xs_on) {
+        const double x12 = x1 + vxi * dt / 2, y12 = y1 + vyi * dt / 2;
+        px_new = 2 * x12 - x1;
+        py_new = 2 * y12 - y1;
+        d.x[2 * cs + 0][p] = px_new;
+        d.y[2 * cs + 0][p] = py_new;
+      }
       const double hn = 2 * h12 - h1;
       double An = 2 * A12 - A1;
       if (An > 1) An = 1;
       d.h[2 * cs + 0][p] = hn;
       d.A[2 * cs + 0][p] = An;
+      hsml_new = hi;
+      // mass_evolution (:271), before smoothing_length
+      if (XSW && be.tmass) d.mass[cs][p] = area_mass(An, hn, hi);
       if (!sumd3 && !cd) {
         const double rn = hn * kRhoIce;
         d.rho[2 * cs + 0][p] = rn;
-        // Marquis_hsml (smoothing_length.f90:136-142)
-        double hs = sqrt(mi / rn) * kHsmlMult;
-        if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[2 * cs + 0][p] = hs;
-        hsml_new = hs;
         rho_new = rn;
+        if (marq) {
+          // Marquis_hsml (smoothing_length.f90:136-142)
+          double hs = sqrt(mi / rn) * kHsmlMult;
+          if (hs > m.max_hsml) hs = m.max_hsml;
+          d.hsml[2 * cs + 0][p] = hs;
+          hsml_new = hs;
+        }
       }
     } else if (phase == 4) {
       // leapfrog_scheme (time_integration.f90:361-391): thickness,
@@ -2041,24 +2101,31 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       // evaluation's tendencies (dxdt = the half-step velocity), density on
       // this evaluation (diagnostic, or pass A's summation), Marquis_hsml
       const double hn = h1 + dhdt * dt;
-      px_new = x1 + (u1 + vxi) * dt / 2;
-      py_new = y1 + (v1 + vyi) * dt / 2;
-      d.x[2 * cs + 0][p] = px_new;
-      d.y[2 * cs + 0][p] = py_new;
+      if (!xs_on) {
+        px_new = x1 + (u1 + vxi) * dt / 2;
+        py_new = y1 + (v1 + vyi) * dt / 2;
+        d.x[2 * cs + 0][p] = px_new;
+        d.y[2 * cs + 0][p] = py_new;
+      }
       double An = A1 + dAdt * dt;
       if (An > 1) An = 1;
       d.h[2 * cs + 0][p] = hn;
       d.A[2 * cs + 0][p] = An;
+      hsml_new = hi;
+      // mass_evolution (:380), before smoothing_length
+      if (XSW && be.tmass) d.mass[cs][p] = area_mass(An, hn, hi);
       // (continuity density: k_condens takes the density, hsml and the
       // partials from here, on the new velocities)
       if (!(XSW && m.den_alg == 2)) {
         const double rn = sd ? dsi : hn * kRhoIce;
         d.rho[2 * cs + 0][p] = rn;
-        double hs = sqrt(mi / rn) * kHsmlMult;
-        if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[2 * cs + 0][p] = hs;
-        hsml_new = hs;
         rho_new = rn;
+        if (marq) {
+          double hs = sqrt(mi / rn) * kHsmlMult;
+          if (hs > m.max_hsml) hs = m.max_hsml;
+          d.hsml[2 * cs + 0][p] = hs;
+          hsml_new = hs;
+        }
       }
     }
     const bool listed = nn <= kNbrMax;
@@ -2090,6 +2157,20 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     auto dens_pair = [&](double dx, double dy, double mh, double mq, double rho2) {
       const double w = kern_w(skf, sqrt(dx * dx + dy * dy), mh);
       den3 = den3 + mq * w / rho2;
+    };
+    // XSPH (regularization.f90:73-91): the pair's dvx = v_i - v_j enters both
+    // particles' sums with the same sign; owned pairs (self = i) are the
+    // tail of the list, nI + nX entries
+    double xsx = 0., xsy = 0.;
+    const int own0 = xs_on ? nn - (d.own_cnt[self] & 1023) - ((d.own_cnt[self] >> 10) & 1023) : 0;
+    auto xsph_pair = [&](bool own, double dx, double dy, double mh, double mq, double rq,
+                         double uq, double vq) {
+      const double w = kern_w(skf, sqrt(dx * dx + dy * dy), mh);
+      const double rm = 0.5 * (rhoi + rq);
+      const double e0 = own ? vxi - uq : uq - vxi;
+      const double e1 = own ? vyi - vq : vq - vyi;
+      xsx = xsx + mq * e0 * w / rm;
+      xsy = xsy + mq * e1 * w / rm;
     };
     const bool pa2 = XSW && m.pa_sph == 2;
     auto stress_pair = [&](double dx, double dy, double r, double mh, double rq, double mq,
@@ -2136,11 +2217,12 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         for (int u = 0; u < kB; ++u) {
           const int t = sj[u];
           xq[u] = X[t]; yq[u] = Y[t]; hq[u] = HS[t];
-          uq[u] = cd ? VX[t] : 0.0;
-          vq[u] = cd ? VY[t] : 0.0;
+          uq[u] = (cd || xs_on) ? VX[t] : 0.0;
+          vq[u] = (cd || xs_on) ? VY[t] : 0.0;
           if (MIXED) {
             qr[u] = d.srec[t];
-            mq[u] = (sumd3 || cd) ? MS[t] : 0.0;
+            mq[u] = (sumd3 || cd || xs_on) ? MS[t] : 0.0;
+            rq[u] = xs_on ? RS[t] : 1.0;
           } else {
             rq[u] = RS[t]; mq[u] = MS[t];
             q11[u] = S11[t]; q12[u] = S12[t]; q22[u] = S22[t];
@@ -2170,6 +2252,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
                         q22[u]);
           if (sumd3) dens_pair(dx, dy, mh, mq[u], r2[u]);
           if (cd) cden_pair(dx, dy, mh, mq[u], uq[u], vq[u]);
+          if (xs_on) xsph_pair(j0 + u >= own0, dx, dy, mh, mq[u], rq[u], uq[u], vq[u]);
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) sj[u] = j1 + u < nn ? nx[u] : self;
@@ -2190,6 +2273,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
             stress_pair(dx, dy, r, mh, RS[s], MS[s], S11[s], S12[s], S22[s]);
           if (sumd3) dens_pair(dx, dy, mh, MS[s], d.dsum[s]);
           if (cd) cden_pair(dx, dy, mh, MS[s], VX[s], VY[s]);
+          if (xs_on) xsph_pair(pair_kind(R, s, self) != 0, dx, dy, mh, MS[s], RS[s], VX[s], VY[s]);
         }
       }
     }
@@ -2221,6 +2305,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       int nbn = 0;
       float fbx = 0.0f, fby = 0.0f;   // mixed: the forces' sum
       const bool rfresh = phase != 1 && y.rfresh;
+      const int bfa = XSW ? m.bfa : 1;
       const int *bl = d.blist + 9 * (size_t)m.b_off;
       for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
         int ent[kBU], rb[kBU];
@@ -2269,6 +2354,19 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           if (!inr[j]) continue;
           ++nbn;
           const double bdx = bx[j] - xi, bdy = by[j] - yi;
+          if (bfa == 3) continue;   // bf_Marquis: no force with boundary_pressure (:211-214)
+          if (bfa == 2) {
+            // bf_Wang (boundary_force.f90:145-174): the normal velocity and
+            // the rest of the speed, each as a momentum over dt (FP64)
+            const int bg = m.b_off + (ent[j] & 0xffffff);
+            const double n0 = d.bnx[bg], n1 = d.bny[bg];
+            const double v0 = n0 * vxi + n1 * vyi;
+            const double v1 = sqrt(vxi * vxi + vyi * vyi) - v0;
+            const double F0 = mi * v0 / dt, F1 = mi * v1 / dt;
+            ax = ax + (F0 * n0 + F1 * n1);
+            ay = ay + (F0 * n1 - F1 * n0);
+            continue;
+          }
           if (MIXED) {
             // mixed: the force in FP32 (hardware sqrt and reciprocals);
             // the pair set and its ranks stay those of the FP64 test
@@ -2327,10 +2425,46 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
         ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
       }
     }
-    // the candidate lists' movement bound: speed of the evaluated state and
-    // hsml change of the new one
-    speed = sqrt(vxi * vxi + vyi * vyi);
-    dhs = fabs(hsml_new - hi);
+    // the artificial viscosity's pair terms (regularization.f90:198-208,
+    // :293-303) in list order, with pi_ij of k_avpair / k_avres; as the
+    // pair's second particle the reference adds the term of the opposite
+    // gradient, the same value
+    if (av_on && listed) {
+      for (int j = 0; j < nn; ++j) {
+        const int t = d.nbr[j * N + self];
+        const double pi = d.piv[d.pidx[j * N + self]];
+        const double dx = xi - X[t], dy = yi - Y[t];
+        const double r = sqrt(dx * dx + dy * dy);
+        const double gg = kern_grad(skf, r, (hi + HS[t]) / 2.);
+        const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
+        ax = ax - MS[t] * pi * dw0;
+        ay = ay - MS[t] * pi * dw1;
+      }
+    }
+    // XSPH: dxdt = 0 - chi xsph_sum (regularization.f90:93), then + vx
+    // (single_step.f90:106); the position updates deferred above
+    double dxdt0 = vxi, dxdt1 = vyi;
+    if (xs_on) {
+      dxdt0 = (0.0 - kChi * xsx) + vxi;
+      dxdt1 = (0.0 - kChi * xsy) + vyi;
+      if (phase == 0) {
+        px_new = xi + dxdt0 * dt / 2;
+        py_new = yi + dxdt1 * dt / 2;
+        d.x[2 * cs + 1][p] = px_new;
+        d.y[2 * cs + 1][p] = py_new;
+      } else if (phase == 1) {
+        const double x12 = x1 + dxdt0 * dt / 2, y12 = y1 + dxdt1 * dt / 2;
+        px_new = 2 * x12 - x1;
+        py_new = 2 * y12 - y1;
+        d.x[2 * cs + 0][p] = px_new;
+        d.y[2 * cs + 0][p] = py_new;
+      } else if (phase == 4) {
+        px_new = x1 + (u1 + dxdt0) * dt / 2;
+        py_new = y1 + (v1 + dxdt1) * dt / 2;
+        d.x[2 * cs + 0][p] = px_new;
+        d.y[2 * cs + 0][p] = py_new;
+      }
+    }
     // velocities (time_integration.f90:210-211, 240-241) or tendencies
     if (phase == 0) {
       d.vx[2 * cs + 1][p] = vxi + ax * dt / 2;
@@ -2339,44 +2473,54 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       const double u12 = u1 + ax * dt / 2, v12 = v1 + ay * dt / 2;
       d.vx[2 * cs + 0][p] = 2 * u12 - u1;
       d.vy[2 * cs + 0][p] = 2 * v12 - v1;
-      if (sumd3) {
-        // summation density of the corrector, then Marquis_hsml
-        const double rn = dnumi / den3;
+      if (sumd3 || cd) {
+        double rn;
+        if (sumd3) {
+          // summation density of the corrector
+          rn = dnumi / den3;
+        } else {
+          // rho_12 = rho_1 + drhodt dt/2, rho = 2 rho_12 - rho_1
+          const double r1 = d.rho[2 * cs + 0][p];
+          const double r12 = r1 + drho * dt / 2;
+          rn = 2 * r12 - r1;
+        }
         d.rho[2 * cs + 0][p] = rn;
-        double hs = sqrt(mi / rn) * kHsmlMult;
-        if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[2 * cs + 0][p] = hs;
-        hsml_new = hs;
         rho_new = rn;
-      } else if (cd) {
-        // rho_12 = rho_1 + drhodt dt/2, rho = 2 rho_12 - rho_1, Marquis_hsml
-        const double r1 = d.rho[2 * cs + 0][p];
-        const double r12 = r1 + drho * dt / 2;
-        const double rn = 2 * r12 - r1;
-        d.rho[2 * cs + 0][p] = rn;
-        double hs = sqrt(mi / rn) * kHsmlMult;
-        if (hs > m.max_hsml) hs = m.max_hsml;
-        d.hsml[2 * cs + 0][p] = hs;
-        hsml_new = hs;
-        rho_new = rn;
+        if (marq) {
+          // Marquis_hsml
+          double hs = sqrt(mi / rn) * kHsmlMult;
+          if (hs > m.max_hsml) hs = m.max_hsml;
+          d.hsml[2 * cs + 0][p] = hs;
+          hsml_new = hs;
+        }
+      }
+      if (endp) {
+        // for the end pass's Monaghan time step terms
+        d.tend[2 * (size_t)d.ntot + p] = ax;
+        d.tend[3 * (size_t)d.ntot + p] = ay;
       }
     } else if (phase == 4) {
       d.vx[2 * cs + 0][p] = u1 + ax * dt;
       d.vy[2 * cs + 0][p] = v1 + ay * dt;
-      if (XSW && m.den_alg == 2) {
-        // for k_condens's Monaghan time step terms
+      if (XSW && (m.den_alg == 2 || endp)) {
+        // for k_condens's or the end pass's Monaghan time step terms
         d.tend[2 * (size_t)d.ntot + p] = ax;
         d.tend[3 * (size_t)d.ntot + p] = ay;
       }
     } else {
       const int N = d.ntot;
-      d.tend[p] = vxi;
-      d.tend[N + p] = vyi;
+      d.tend[p] = dxdt0;
+      d.tend[N + p] = dxdt1;
       d.tend[2 * N + p] = ax;
       d.tend[3 * N + p] = ay;
       d.tend[4 * N + p] = dhdt;
       d.tend[5 * N + p] = dAdt;
     }
+    // the candidate lists' movement bound: speed of the evaluated state and
+    // hsml change of the new one (final only here: the summation and
+    // continuity densities of the corrector set it with the velocities)
+    speed = sqrt(vxi * vxi + vyi * vyi);
+    dhs = fabs(hsml_new - hi);
     // time_step at the end of the step (time_integration.f90:280, :389):
     // the Monaghan step's terms from this evaluation's dvxdt
     if (XSW && (phase == 1 || phase == 4) && m.tse == 1)
@@ -2387,7 +2531,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
   if constexpr (XSW && (phase == 1 || phase == 4)) {
     // (block-uniform: k_condens writes a leap-frog continuity-density
     // instance's partials)
-    if (!(phase == 4 && m.den_alg == 2))
+    if (!(phase == 4 && m.den_alg == 2) && !m.endp)
       block_partials<kParts>(d, k, live, px_new, py_new, hsml_new, speed, dhs, mfo, mfc);
   }
   else if constexpr (phase != 2 && phase != 3)
@@ -2422,12 +2566,20 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
     if (An > 1) An = 1;
     // (summation density: pass A's sum; continuity density: k_condens's)
     const double rn = m.den_alg != 3 ? d.dsum[p] : hn * kRhoIce;
-    hs = sqrt(d.mass[cs][p] / rn) * kHsmlMult;
-    if (hs > m.max_hsml) hs = m.max_hsml;
+    const double h0 = d.hsml[2 * cs][p];
+    if (m.endp) {
+      hs = d.hnew[p];                 // the end pass's (Benz's or Marquis')
+    } else if (m.sle == 2) {
+      hs = sqrt(d.mass[cs][p] / rn) * kHsmlMult;
+      if (hs > m.max_hsml) hs = m.max_hsml;
+    } else {
+      hs = h0;                        // sle = 0: kept
+    }
     sp = sqrt(vx * vx + vy * vy);
-    dh = fabs(hs - d.hsml[2 * cs][p]);
+    dh = fabs(hs - h0);
     // time_step (time_integration.f90:146): the Monaghan step's terms
     if (m.tse == 1) monaghan_terms(hs, rn, T[2 * N + p], T[3 * N + p], fo, fc);
+    if (m.avdt) fc = d.cfl[p];
     d.x[2 * cs][p] = xn;
     d.y[2 * cs][p] = yn;
     d.vx[2 * cs][p] = un;
@@ -2436,6 +2588,8 @@ __global__ void __launch_bounds__(kTPB) k_euler(DevState d) {
     d.A[2 * cs][p] = An;
     d.rho[2 * cs][p] = rn;
     d.hsml[2 * cs][p] = hs;
+    // mass_evolution (:137), with the hsml before smoothing_length
+    if (m.sle != 2) d.mass[cs][p] = area_mass(An, hn, h0);
   }
   block_partials<kParts>(d, k, live, xn, yn, hs, sp, dh, fo, fc);
 }
@@ -2471,7 +2625,10 @@ __global__ void __launch_bounds__(kTPB) k_condens(DevState d) {
   const int eb = MODE == 2 ? 1 : 0;   // buffer of the evaluated state
   const int vb = MODE == 0 ? 1 : 0;   // buffer of the velocities
   const double *X = d.x[2 * cs + eb], *Y = d.y[2 * cs + eb], *HS = d.hsml[2 * cs + eb];
-  const double *VX = d.vx[2 * cs + vb], *VY = d.vy[2 * cs + vb], *MS = d.mass[cs];
+  const double *VX = d.vx[2 * cs + vb], *VY = d.vy[2 * cs + vb];
+  // (evolving masses: after the leap-frog's second evaluation the density
+  // call precedes mass_evolution, time_integration.f90:372-380)
+  const double *MS = (MODE == 2 && be.tmass) ? d.masst[cs] : d.mass[cs];
   const size_t N = (size_t)d.ntot;
   const double dt = y.dt;
   double px = 0., py = 0., hs = 1.0, sp = 0., dh = 0., fo = 0., fc = 0.;
@@ -2530,10 +2687,13 @@ __global__ void __launch_bounds__(kTPB) k_condens(DevState d) {
     } else {
       const double rn = rho0 + drho * dt;
       d.rho[2 * cs + 0][self] = rn;
-      // Marquis_hsml (smoothing_length.f90:136-142)
-      hs = sqrt(MS[self] / rn) * kHsmlMult;
-      if (hs > m.max_hsml) hs = m.max_hsml;
-      d.hsml[2 * cs + 0][self] = hs;
+      hs = hi;
+      if (m.sle == 2) {
+        // Marquis_hsml (smoothing_length.f90:136-142)
+        hs = sqrt(MS[self] / rn) * kHsmlMult;
+        if (hs > m.max_hsml) hs = m.max_hsml;
+        d.hsml[2 * cs + 0][self] = hs;
+      }
       px = d.x[2 * cs + 0][self];
       py = d.y[2 * cs + 0][self];
       const double ue = d.vx[2 * cs + 1][self], ve = d.vy[2 * cs + 1][self];
@@ -2542,7 +2702,303 @@ __global__ void __launch_bounds__(kTPB) k_condens(DevState d) {
       if (m.tse == 1) monaghan_terms(hs, rn, d.tend[2 * N + self], d.tend[3 * N + self], fo, fc);
     }
   }
-  if (MODE == 2) block_partials<kParts>(d, k, live, px, py, hs, sp, dh, fo, fc);
+  // (an instance with the end pass gets its partials there)
+  if (MODE == 2 && !m.endp) block_partials<kParts>(d, k, live, px, py, hs, sp, dh, fo, fc);
+}
+
+// ----------------------------------------- artificial viscosity and end pass
+
+// art_visc_HK and art_visc_Mon (regularization.f90:179-211, :268-306) walk
+// the pair list in the reference's order, and pi_ij is not reset per pair:
+// a pair whose criterion v_ij . r_ij <= 0 fails keeps the value of the last
+// pair before it that set one (0 before the first).  k_avpair gives every
+// pair owned by a slot its rank in the packed pair ranks (pair_start, moff
+// and cell_intra of k_pairtot and the scan: per cell all intra pairs, then
+// all cross pairs, each in member order), whether it sets pi_ij and the
+// value it sets, and records the rank on both particles' list entries (the
+// owner's entry and the partner's, which lies in the partner's non-owned
+// head); k_avlast and k_avres then give every other pair the value of the
+// last setting pair of its instance before it.  Pass B adds the pair terms
+// in list order.  FP64 in both precision modes.
+__global__ void __launch_bounds__(kTPB) k_avpair(DevState d, int buf) {
+  int p = 0, k = 0;
+  if (!particle_of(d, p, k)) return;
+  const InstMeta &m = d.meta[k];
+  const InstDyn &y = d.dyn[k];
+  if (m.avis == 0 || inst_off(y)) return;
+  const int cs = d.blk[blockIdx.x].cs, sb = 2 * cs + buf;
+  const double *X = d.x[sb], *Y = d.y[sb], *VX = d.vx[sb], *VY = d.vy[sb];
+  const double *HS = d.hsml[sb], *RS = d.rho[sb];
+  const size_t N = (size_t)d.ntot;
+  const int nn = d.nnb[p];
+  if (nn > kNbrMax) {         // the pair terms need the stored lists
+    atomicOr(&d.dyn[k].err, kErrCapacity);
+    return;
+  }
+  const int oc = d.own_cnt[p];
+  const int nI = oc & 1023, nX = (oc >> 10) & 1023;
+  const int c = d.cellid[cs][p];
+  const int base = d.pair_start[m.c_off + c];
+  const int mo = d.moff[p];
+  const int cint = d.cell_intra[m.c_off + c];
+  const int j0 = nn - nI - nX, jx = nn - nX;
+  const double xi = X[p], yi = Y[p], vxi = VX[p], vyi = VY[p], hi = HS[p], rhoi = RS[p];
+  const double ci = m.needc ? sound_speed(rhoi) : 0.0;
+  const double lvi = m.avis == 1 ? d.lv[p] : 0.0;
+  for (int j = j0; j < nn; ++j) {
+    const int q = d.nbr[j * N + p];
+    const int g = j < jx ? base + (mo & 0xffff) + (j - j0) : base + cint + (mo >> 16) + (j - jx);
+    if (g >= d.pcap) {
+      atomicOr(&d.dyn[k].err, kErrCapacity);
+      return;
+    }
+    // pair (i, j) = (p, q): p owns it, so it is the pair's first particle
+    const double dx = xi - X[q], dy = yi - Y[q];
+    const double dv0 = vxi - VX[q], dv1 = vyi - VY[q];
+    const double crit = dv0 * dx + dv1 * dy;
+    const double rq = RS[q];
+    double pi = 0.0;
+    if (m.avis == 1) {
+      pi = lvi / (rhoi * rhoi) + d.lv[q] / (rq * rq);
+    } else {
+      const double mh = (hi + HS[q]) / 2.;
+      const double r = sqrt(dx * dx + dy * dy);
+      const double s_cor = (double)0.01f * mh;
+      const double iota = mh * crit / (r * r + s_cor * s_cor);
+      const double cq = m.needc ? sound_speed(rq) : 0.0;
+      pi = (kBetaPi * (iota * iota) - kAlphaPi * iota * (ci + cq) / 2) / ((rhoi + rq) / 2);
+    }
+    const int set = crit <= 0. ? 1 : 0;
+    d.piv[g] = set ? pi : 0.0;
+    d.pflg[g] = set;
+    d.pidx[j * N + p] = g;
+    const int nq = d.nnb[q];
+    if (nq > kNbrMax) {
+      atomicOr(&d.dyn[k].err, kErrCapacity);
+      continue;
+    }
+    for (int jj = 0; jj < nq; ++jj) {
+      if (d.nbr[jj * N + q] == p) {
+        d.pidx[jj * N + q] = g;
+        break;
+      }
+    }
+  }
+}
+
+// total pair ranks of this evaluation (the scan's last entry)
+__device__ __forceinline__ int pair_total(const DevState &d) { return d.pair_start[d.ctot]; }
+
+// per block of kPairBlk ranks: the last rank that sets pi_ij (-1: none)
+__global__ void __launch_bounds__(kPairBlk) k_avlast(DevState d) {
+  const int g = blockIdx.x * kPairBlk + threadIdx.x;
+  if ((int)blockIdx.x * kPairBlk >= d.pcap) return;
+  const int tot = min(pair_total(d), d.pcap);
+  const int v = (g < tot && d.pflg[g]) ? g : -1;
+  __shared__ int wm[kPairBlk / 64];
+  int w = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) w = max(w, __shfl_xor(w, o, 64));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int r = wm[0];
+    for (int i = 1; i < kPairBlk / 64; ++i) r = max(r, wm[i]);
+    d.pblast[blockIdx.x] = r;
+  }
+}
+
+// the instance whose pair ranks hold g: the last with pair_start[c_off] <= g
+__device__ __forceinline__ int pair_instance(const DevState &d, int g) {
+  int a = 0, b = d.ninst - 1;
+  while (a < b) {
+    const int mid = (a + b + 1) >> 1;
+    if (d.pair_start[d.meta[mid].c_off] <= g) a = mid; else b = mid - 1;
+  }
+  return a;
+}
+
+__global__ void __launch_bounds__(kPairBlk) k_avres(DevState d) {
+  const int b0 = blockIdx.x * kPairBlk;
+  if (b0 >= d.pcap) return;
+  const int g = b0 + threadIdx.x;
+  const int tot = min(pair_total(d), d.pcap);
+  const bool in = g < tot;
+  const int k = pair_instance(d, in ? g : max(min(tot, d.pcap) - 1, 0));
+  const int g0 = d.pair_start[d.meta[k].c_off];
+  const int fl = in ? d.pflg[g] : 0;
+  // inclusive max-scan of the setting ranks over the block
+  __shared__ int wm[kPairBlk / 64];
+  __shared__ int prev;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int v = fl ? g : -1;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if (lane >= o) v = max(v, t);
+  }
+  if (lane == 63) wm[wid] = v;
+  if (threadIdx.x == 0) {
+    // the last setting rank before the block, down to the start of the
+    // instance of its first rank
+    const int kf = pair_instance(d, min(b0, max(tot - 1, 0)));
+    const int gf = d.pair_start[d.meta[kf].c_off];
+    int r = -1;
+    for (int bb = (int)blockIdx.x - 1; bb >= 0 && (bb + 1) * kPairBlk > gf; --bb) {
+      r = d.pblast[bb];
+      if (r >= 0) break;
+    }
+    prev = r;
+  }
+  __syncthreads();
+  for (int i = 0; i < wid; ++i) v = max(v, wm[i]);
+  if (!in || fl || d.meta[k].avis == 0) return;
+  int src = max(v, prev);
+  d.piv[g] = src >= g0 ? d.piv[src] : 0.0;
+}
+
+// The end-of-step pair pass of instances with Benz's smoothing length
+// (smoothing_length.f90:75-109) or the Monaghan time step's viscous CFL
+// terms (time_step.f90:89-107): both sum over the last evaluation's pairs,
+// in the order the reference's list reaches the particle, with that
+// evaluation's gradients and the step's final velocities and densities.
+// Benz: drhodt = the persistent sum plus this step's terms (the reference
+// never resets it without continuity density; with it, the last density
+// call's sum alone), hsml += -hsml drhodt / 2 / rho dt, undone where it
+// turns negative.  MODE 0: after the corrector or the leap-frog's second
+// evaluation (evaluated state T, final state S; the new hsml goes to S and
+// this kernel writes the block's partials); MODE 1: after the Euler
+// evaluation, before its in-place update (final velocities and density from
+// the tendencies as k_euler forms them; the new hsml and the CFL term go to
+// hnew / cfl for k_euler).
+template <int MODE>
+__global__ void __launch_bounds__(kTPB) k_endpass(DevState d) {
+  if (blockIdx.x >= (unsigned)d.nblk) return;
+  const BlockEntry be = d.blk[blockIdx.x];
+  const int k = be.inst;
+  const InstMeta &m = d.meta[k];
+  if (!m.endp) return;   // (block-uniform)
+  const InstDyn &y = d.dyn[k];
+  const bool ok = (int)threadIdx.x < be.nloc;
+  const bool live = ok && !inst_off(y);
+  const int self = be.pos + (ok ? (int)threadIdx.x : 0);
+  const int cs = be.cs;
+  const int eb = MODE == 0 ? 1 : 0;   // buffer of the evaluated state
+  const double *X = d.x[2 * cs + eb], *Y = d.y[2 * cs + eb], *HS = d.hsml[2 * cs + eb];
+  const double *VX = d.vx[2 * cs + 0], *VY = d.vy[2 * cs + 0];
+  // masses: Benz's own sum (den_alg != 2) follows mass_evolution and takes
+  // the new ones; a continuity rate is the last density call's, which
+  // precedes it (time_integration.f90:128-137, 262-271, 372-380): the
+  // evaluated state's
+  const bool newm = be.tmass && m.den_alg != 2;
+  const double *MS = (MODE == 0 && be.tmass && !newm) ? d.masst[cs] : d.mass[cs];
+  const size_t N = (size_t)d.ntot;
+  const double dt = y.dt;
+  double px = 0., py = 0., hsn = 1.0, sp = 0., dh = 0., fo = 0., fc = 0.;
+  if (live) {
+    // the final velocity of slot t
+    auto vel = [&](int t, double &u, double &v) {
+      u = VX[t];
+      v = VY[t];
+      if (MODE == 1) {
+        u = u + d.tend[2 * N + t] * dt;
+        v = v + d.tend[3 * N + t] * dt;
+      }
+    };
+    // the mass of slot t the sums take (Euler: the new masses from the
+    // tendencies, as k_euler forms them)
+    auto mass_of = [&](int t) {
+      if (MODE == 1 && newm) {
+        double An = d.A[2 * cs][t] + d.tend[5 * N + t] * dt;
+        if (An > 1) An = 1;
+        return area_mass(An, d.h[2 * cs][t] + d.tend[4 * N + t] * dt, HS[t]);
+      }
+      return MS[t];
+    };
+    const int skf = m.skf;
+    const double ksc = kern_scale(skf);
+    const double xi = X[self], yi = Y[self], hi = HS[self];
+    double ui, vi;
+    vel(self, ui, vi);
+    // the final density
+    double rn;
+    if (MODE == 0) {
+      rn = d.rho[2 * cs + 0][self];
+    } else {
+      rn = m.den_alg != 3 ? d.dsum[self]
+                          : (d.h[2 * cs][self] + d.tend[4 * N + self] * dt) * kRhoIce;
+    }
+    const bool benz = m.sle == 1;
+    const bool keep = benz && m.den_alg != 2;   // the persistent drhodt
+    const int ri = m.p_off + d.orig[cs][self];
+    double drho = keep ? d.drp[ri] : 0.0;
+    double divv = 0.;
+    auto pair = [&](int t, double dx, double dy, double r, double mh) {
+      double uq, vq;
+      vel(t, uq, vq);
+      const double gg = kern_grad(skf, r, mh);
+      const double dw0 = gg * (dx / r) * ksc, dw1 = gg * (dy / r) * ksc;
+      const double w1 = (ui - uq) * dw0, w4 = (vi - vq) * dw1;
+      const double mq = mass_of(t);
+      drho = drho + mq * (w1 + w4);
+      divv = divv - mq * (w1 + w4) / rn;
+    };
+    const int nn = d.nnb[self];
+    if (nn <= kNbrMax) {
+      for (int j = 0; j < nn; ++j) {
+        const int t = d.nbr[j * N + self];
+        const double dx = xi - X[t], dy = yi - Y[t];
+        pair(t, dx, dy, sqrt(dx * dx + dy * dy), (hi + HS[t]) / 2.);
+      }
+    } else {
+      const int W = y.W;
+      const int c = d.cellid[cs][self];
+      const int cy = c / W, cx = c - cy * W;
+      const Ranges R = neighbour_ranges(d.cell_start + m.c_off, W, cx, cy);
+      for (int g = 0; g < 6; ++g) {
+        for (int t = R.a[g]; t < R.b[g]; ++t) {
+          if (t == self) continue;
+          const double dx = xi - X[t], dy = yi - Y[t];
+          const double mh = (hi + HS[t]) / 2.;
+          double r;
+          if (!in_support(dx * dx + dy * dy, mh, r)) continue;
+          pair(t, dx, dy, r, mh);
+        }
+      }
+    }
+    if (benz) {
+      const double dhdt = -(hi * drho / 2 / rn);
+      hsn = hi + dhdt * dt;
+      if (hsn < 0) hsn = hsn - dhdt * dt;
+      if (keep) d.drp[ri] = drho;
+    } else if (m.sle == 2) {
+      if (MODE == 0) {
+        hsn = d.hsml[2 * cs + 0][self];          // pass B's Marquis_hsml
+      } else {
+        hsn = sqrt(mass_of(self) / rn) * kHsmlMult;   // (smoothing_length.f90:136-142)
+        if (hsn > m.max_hsml) hsn = m.max_hsml;
+      }
+    } else {
+      hsn = hi;
+    }
+    if (m.tse == 1) {
+      const double ax = d.tend[2 * N + self], ay = d.tend[3 * N + self];
+      monaghan_terms(hsn, rn, ax, ay, fo, fc);
+      if (m.avdt) fc = monaghan_cfl_av(hsn, rn, divv);
+    }
+    if (MODE == 0) {
+      if (benz) d.hsml[2 * cs + 0][self] = hsn;
+      px = d.x[2 * cs + 0][self];
+      py = d.y[2 * cs + 0][self];
+      const double ue = d.vx[2 * cs + 1][self], ve = d.vy[2 * cs + 1][self];
+      sp = sqrt(ue * ue + ve * ve);
+      dh = fabs(hsn - hi);
+    } else {
+      d.hnew[self] = hsn;
+      d.cfl[self] = fc;
+    }
+  }
+  if (MODE == 0) block_partials<kParts>(d, k, live, px, py, hsn, sp, dh, fo, fc);
 }
 
 // ------------------------------------------------------- pair-list export
@@ -2650,7 +3106,8 @@ __global__ void __launch_bounds__(kTPB) k_single_blocks(BlockEntry *blk, int n, 
   e.nloc = min(n - e.first, kTPB);
   e.pos = e.first;
   e.cs = 0;
-  e.pad[0] = e.pad[1] = 0;
+  e.tmass = 0;
+  e.pad = 0;
   blk[j] = e;
 }
 
@@ -2813,6 +3270,18 @@ void launch_condens(const DevState &d, int mode, hipStream_t st) {
     k_condens<1><<<g8(d.nblk), kTPB, 0, st>>>(d);
   else
     k_condens<2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+}
+void launch_av(const DevState &d, int buf, hipStream_t st) {
+  k_avpair<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  const int nbp = g8(cdiv(std::max(d.pcap, 1), kPairBlk));
+  k_avlast<<<nbp, kPairBlk, 0, st>>>(d);
+  k_avres<<<nbp, kPairBlk, 0, st>>>(d);
+}
+void launch_endpass(const DevState &d, int mode, hipStream_t st) {
+  if (mode == 0)
+    k_endpass<0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+  else
+    k_endpass<1><<<g8(d.nblk), kTPB, 0, st>>>(d);
 }
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st) {
   if (mixed && xsw)

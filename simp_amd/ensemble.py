@@ -37,6 +37,16 @@ def _desc(sc, nsteps: int, keep: list, itimestep: int = 0) -> SimpInstanceDesc:
     d.eos_alg = getattr(sc, "eos_alg", 3)
     d.rheology = getattr(sc, "rheology", 1)
     d.skf = getattr(sc, "skf", 6)
+    sle = getattr(sc, "sle", 2)
+    d.sle = _lib.SIMP_SLE_KEEP if sle == 0 else sle
+    d.xsph = getattr(sc, "xsph", 0)
+    d.art_visc = getattr(sc, "art_visc", 0)
+    d.art_visc_alg = getattr(sc, "art_visc_alg", 1)
+    d.bfa = getattr(sc, "bfa", 1)
+    if sc.bnormal is not None and sc.nb:
+        bn = np.ascontiguousarray(sc.b_normal(), dtype=np.float64)
+        keep.append(bn)
+        d.b_normal = bn.ctypes.data
     d.wind[0], d.wind[1] = sc.wind
     d.current[0], d.current[1] = sc.current
     d.max_hsml = sc.max_hsml

@@ -19,6 +19,8 @@ module simp_binding
                               eos_alg, pa_sph, skf, den_alg, wind_mode,    &
                               max_particles, max_boundary, reserved
         real(c_double)     :: wind(2), current(2), max_hsml
+        integer(c_int32_t) :: artificial_visc, art_visc_alg,               &
+                              xsph_regularization, reserved2
     end type simp_config
 
     interface
@@ -63,6 +65,15 @@ module simp_binding
             integer(c_int), intent(out) :: nip, p_i(*), p_j(*)
             real(c_double), intent(out) :: w(*), kernel_zero(*)
         end function simp_kernel_values
+
+        integer(c_int) function simp_set_module_state(h, dt, nb, b_normal) &
+                bind(C, name='simp_set_module_state')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: dt
+            integer(c_int), value :: nb
+            real(c_double), intent(in) :: b_normal(*)
+        end function simp_set_module_state
 
         integer(c_int) function simp_kernel_gradients(h, cap, nip, p_i, p_j, &
                 dwdx) bind(C, name='simp_kernel_gradients')

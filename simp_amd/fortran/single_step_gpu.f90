@@ -6,12 +6,15 @@
 ! links against it unchanged.
 !
 ! Reads  ntotal, x, vx, mass, rho, hsml, h, A, ntotalb, xb, massb,
-!        current_time, itimestep, max_hsml
+!        current_time, itimestep, max_hsml; with bfa = 2 also dt and
+!        b_normal (the Wang boundary force)
 ! Writes dxdt, dvxdt, dhdt, dAdt, epsilon_11/12/22, zeta, eta, p,
 !        sigma_11/12/22, nip, bnip; with den_alg = 1 also p_i, p_j, w and
 !        kernel_zero, which sum_density (density.f90:41-100) reads next;
-!        with den_alg = 2 p_i, p_j and dwdx, which con_density
-!        (density.f90:107-165) reads
+!        with den_alg = 2, sle = 1 or the Monaghan time step with artificial
+!        viscosity p_i, p_j and dwdx, which con_density (density.f90:107-165),
+!        Benz_hsml (smoothing_length.f90:75-99) and Monaghan_time_step
+!        (time_step.f90:89-107) read
 ! The other pair arrays (r_ij, hsml_ij, boundary pairs) stay on the
 ! device; no active caller reads them on the host.
 !---------------------------------------------------------------------------
@@ -36,6 +39,11 @@ subroutine single_step
         cfg%skf = skf
         cfg%int_algorithm = int_algorithm
         cfg%tse = tse
+        cfg%sle = sle
+        cfg%bfa = bfa
+        cfg%art_visc_alg = art_visc_alg
+        cfg%artificial_visc = merge(1, 0, artificial_visc)
+        cfg%xsph_regularization = merge(1, 0, xsph_regularization)
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif
@@ -44,6 +52,10 @@ subroutine single_step
     endif
     if (.not. allocated(eps3)) allocate(eps3(3, ntotal), sig3(3, ntotal))
 
+    if (bfa == 2) then
+        st = simp_set_module_state(handle, dt, ntotalb, b_normal)
+        call simp_check(st, 'single_step module state')
+    endif
     st = simp_single_step(handle, ntotal, x, vx, mass, rho, hsml, h, A,    &
                           ntotalb, xb, massb, current_time, itimestep,     &
                           dxdt, dvxdt, dhdt, dAdt, eps3, zeta, eta, p,     &
@@ -62,8 +74,10 @@ subroutine single_step
         st = simp_kernel_values(handle, size(w), nipc, p_i, p_j, w,       &
                                 kernel_zero)
         call simp_check(st, 'single_step kernel values')
-    elseif (den_alg == 2) then
-        ! con_density reads the pairs and their gradients
+    endif
+    if (den_alg == 2 .or. sle == 1 .or. (tse == 1 .and. artificial_visc)) then
+        ! con_density, Benz_hsml and the viscous time step read the pairs
+        ! and their gradients
         st = simp_kernel_gradients(handle, size(p_i), nipc, p_i, p_j, dwdx)
         call simp_check(st, 'single_step kernel gradients')
     endif

@@ -309,7 +309,9 @@ def regularization():
     Marquis boundary forces, each from the harness built with those
     PARAMETERs: single_step of the closed box (XSPH, viscosities, Wang) and
     steps of the ice edge (all).  Wang's force is a momentum over dt, which
-    the state does not survive for long: three steps."""
+    the state does not survive for long: two steps (the first, at rest and
+    without forcing, leaves the velocities zero; the second moves them to
+    about 1 m/s; a third would take them to 1e18)."""
     H = refrun.HARNESS_VARIANT
     _pc10_edge("sle0", H["sle0"], sle=0)
     _pc10_edge("sle1", H["sle1"], sle=1)
@@ -323,7 +325,7 @@ def regularization():
     _pc10_edge("avmon", H["avmon"], art_visc=1, art_visc_alg=2)
     _pc10_edge("avtse1", H["avtse1"], art_visc=1, tse=1)
     _single_box("bfa2", H["bfa2"], bfa=2)
-    _pc10_edge("bfa2", H["bfa2"], K=3, bfa=2)
+    _pc10_edge("bfa2", H["bfa2"], K=2, bfa=2)
     _pc10_edge("bfa3", H["bfa3"], bfa=3)
 
 

@@ -38,7 +38,13 @@ CASES = [("box2500_pc20.npz", refrun.HARNESS, refrun.HARNESS_GPU),
          ("edge3600_pc10_den2.npz", refrun.HARNESS_DEN2, refrun.HARNESS_GPU_DEN2)] + [
          # the quadratic, Gaussian, quintic-spline and Marquis kernels
          (f"edge3600_pc10_k{k}.npz", refrun.HARNESS_SKF[k], refrun.HARNESS_GPU_SKF[k])
-         for k in (1, 2, 3, 7, 8)]
+         for k in (1, 2, 3, 7, 8)] + [
+         # smoothing length kept / Benz's rate (the reference's own
+         # smoothing_length and mass_evolution on the host, the pairs and
+         # gradients from the shim), XSPH and the artificial viscosities
+         # inside the GPU single_step, the Wang and Marquis boundary forces
+         (f"edge3600_pc{2 if t == 'bfa2' else 10}_{t}.npz", refrun.HARNESS_VARIANT[t],
+          refrun.HARNESS_GPU_VARIANT[t]) for t in refrun.VARIANT_TAGS]
 
 
 @pytest.mark.parametrize("name,cpu_exe,gpu_exe", CASES)

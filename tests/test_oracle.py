@@ -552,12 +552,12 @@ def test_regularization_and_hsml_switches_ten_steps_bit_exact_vs_reference(tag):
         assert _same(sc.hsml, g["in_hsml"])
 
 
-def test_wang_boundary_force_three_steps_bit_exact_vs_reference():
-    g = load("edge3600_pc3_bfa2.npz")
+def test_wang_boundary_force_two_steps_bit_exact_vs_reference():
+    g = load("edge3600_pc2_bfa2.npz")
     sc = scenario(g)
     o = capi.Oracle()
     steps, out = run_pc_like_reference(o, sc, float(g["integration_time"]))
-    assert steps == int(g["out_itimestep"]) == 3
+    assert steps == int(g["out_itimestep"]) == 2
     for k in ("x", "vx", "rho", "hsml", "A", "h"):
         assert _same(getattr(sc, k), g["out_" + k]), k
 
