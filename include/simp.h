@@ -101,7 +101,18 @@ typedef struct simp_config {
     int32_t xsph_regularization;  /* 0 / 1: XSPH (regularization.f90:45-95);    */
                                   /* regularization then also runs the          */
                                   /* art_visc_alg viscosity (:29-35)            */
-    int32_t reserved2;
+    /* the other logical scenario switches (globals.f90:178-187), 0 / 1;
+     * simp_config_default gives the reference's values (friction 0, the
+     * others 1); viscosity stays on */
+    int32_t boundary_friction;    /* tangential boundary forces (boundary_    */
+                                  /* force.f90:87-100, 157-166, 217-221)       */
+    int32_t boundary_pressure;    /* normal boundary forces; without either    */
+                                  /* boundary force no boundary pairs exist    */
+                                  /* (nnps.f90:440) and the boundary is unused */
+    int32_t pressure_force;       /* 0: eos is not called, p stays the ice     */
+                                  /* strength P (internal_stress.f90:35-39)    */
+    int32_t wind_force;           /* wind_forcing.f90:35                       */
+    int32_t water_force;          /* water_drag.f90:35                         */
 } simp_config;
 
 typedef struct simp_handle simp_handle;
@@ -393,10 +404,19 @@ typedef struct simp_instance_desc {
     int32_t art_visc_alg;                /* 0 or 1: Hernquist-Katz; 2: Monaghan*/
     int32_t bfa;                         /* 0 or 1: Monaghan; 2: Wang; 3:      */
                                          /* Marquis boundary force             */
-    int32_t reserved2;
+    int32_t boundary_friction;           /* 1: boundary friction               */
     const double *b_normal;              /* (2, nb) boundary normals (Wang),   */
                                          /* or NULL (zeros)                    */
+    int32_t forces_off;                  /* SIMP_OFF_* bits: the reference's   */
+                                         /* logical switches set to .FALSE.    */
+    int32_t reserved2;
 } simp_instance_desc;
+
+/* simp_instance_desc.forces_off bits */
+#define SIMP_OFF_BOUNDARY_PRESSURE 1
+#define SIMP_OFF_PRESSURE_FORCE    2
+#define SIMP_OFF_WIND              4
+#define SIMP_OFF_WATER             8
 
 int simp_batch_create(int precision, simp_batch **out);
 int simp_batch_destroy(simp_batch *b);

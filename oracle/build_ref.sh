@@ -168,6 +168,26 @@ variant avtse1 'artificial_visc   = .FALSE.' 'artificial_visc   = .TRUE.' \
 # the Wang and Marquis boundary forces (bfa = 2, 3; boundary_force.f90:109-224)
 variant bfa2 'parameter(bfa = 1)' 'parameter(bfa = 2)'
 variant bfa3 'parameter(bfa = 1)' 'parameter(bfa = 3)'
+# boundary friction (boundary_force.f90:87-100 with the boundary kernels'
+# gradients, boundary_kernel.f90; Wang's :157-166; Marquis' :217-221, which
+# runs only with the boundary pressure off), the boundary pressure off
+# (no boundary pairs at all, nnps.f90:440), and wind, water drag and the
+# replacement pressure off (globals.f90:178-187)
+FRIC='boundary_friction = .FALSE.'
+FRIC_ON='boundary_friction = .TRUE.'
+variant fric "$FRIC" "$FRIC_ON"
+variant fric1 "$FRIC" "$FRIC_ON" 'parameter(skf = 6)' 'parameter(skf = 1)'
+variant fricg "$FRIC" "$FRIC_ON" 'parameter(skf = 6)' 'parameter(skf = 2)'
+variant fricq "$FRIC" "$FRIC_ON" 'parameter(skf = 6)' 'parameter(skf = 3)'
+variant fric4 "$FRIC" "$FRIC_ON" 'parameter(skf = 6)' 'parameter(skf = 4)'
+variant fric5 "$FRIC" "$FRIC_ON" 'parameter(skf = 6)' 'parameter(skf = 5)'
+variant bfa2f "$FRIC" "$FRIC_ON" 'parameter(bfa = 1)' 'parameter(bfa = 2)'
+variant bfa3f "$FRIC" "$FRIC_ON" 'parameter(bfa = 1)' 'parameter(bfa = 3)' \
+    'boundary_pressure = .TRUE.' 'boundary_pressure = .FALSE.'
+variant nobp 'boundary_pressure = .TRUE.' 'boundary_pressure = .FALSE.'
+variant noforce 'wind_force        = .TRUE.' 'wind_force        = .FALSE.' \
+    'water_force       = .TRUE.' 'water_force       = .FALSE.' \
+    'pressure_force    = .TRUE.' 'pressure_force    = .FALSE.'
 
 # The OpenMP build, for the CPU baseline: the reference compiled as its
 # Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
@@ -252,7 +272,8 @@ if [ -f "$LIB" ]; then
     for k in 1 2 3 7 8; do
         eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
     done
-    for t in sle0 sle1 sle1eu sle1d2 xsph avhk avmon avtse1 bfa2 bfa3; do
+    for t in sle0 sle1 sle1eu sle1d2 xsph avhk avmon avtse1 bfa2 bfa3 fric fric1 fricg fricq \
+             fric4 fric5 bfa2f bfa3f nobp noforce; do
         eval "gpu_harness \"$OUT/$t/obj\" \"$OUT/$t/mod\" \"$OUT/simp_ref_harness_gpu_$t\" \"\${vobjs_$t[@]}\""
     done
 fi

@@ -26,7 +26,10 @@ class OrcParams(C.Structure):
                 ("eos_alg", C.c_int), ("rheology", C.c_int), ("skf", C.c_int),
                 ("int_algorithm", C.c_int), ("sle", C.c_int), ("xsph", C.c_int),
                 ("art_visc", C.c_int), ("art_visc_alg", C.c_int), ("bfa", C.c_int),
-                ("b_normal", C.c_void_p), ("step_dt", C.c_double)]
+                ("b_normal", C.c_void_p), ("step_dt", C.c_double),
+                ("boundary_friction", C.c_int), ("boundary_pressure", C.c_int),
+                ("pressure_force", C.c_int), ("wind_force", C.c_int),
+                ("water_force", C.c_int)]
 
 
 def build() -> None:
@@ -98,6 +101,11 @@ def make_params(sc) -> tuple:
     p.art_visc = getattr(sc, "art_visc", 0)
     p.art_visc_alg = getattr(sc, "art_visc_alg", 1)
     p.bfa = getattr(sc, "bfa", 1)
+    p.boundary_friction = getattr(sc, "boundary_friction", 0)
+    p.boundary_pressure = getattr(sc, "boundary_pressure", 1)
+    p.pressure_force = getattr(sc, "pressure_force", 1)
+    p.wind_force = getattr(sc, "wind_force", 1)
+    p.water_force = getattr(sc, "water_force", 1)
     p.step_dt = sc.dt
     bn = np.ascontiguousarray(sc.b_normal(), dtype=np.float64)
     p.b_normal = bn.ctypes.data if bn.size else None

@@ -66,6 +66,7 @@ static const double ZERO_MC = 1.e-18;      /* zero = 1.d-18 */
 static const double ALPHA_PI = 1.0, BETA_PI = 2.0;
 #define CHI_XSPH ((double)0.3f)
 #define XI_FRIC ((double)0.5225f)
+static const double KAPPA_T = 5.e-8;      /* kappa_t = 5.D-8 */
 
 enum { ORC_OK = 0, ORC_E_ARG = 1, ORC_E_NOMEM = 2 };
 
@@ -539,6 +540,42 @@ static double boundary_w(int skf, double br, double hk) {
 
 /* MC_bulk_viscosity and MC_shear_viscosity (bulk_viscosity.f90:113-143,
  * shear_viscosity.f90:91-122) at one particle. */
+/* The boundary kernels' gradient (boundary_kernel.f90, bdwdx of each
+ * b_ routine): bdwdx = -(g (bdx / br) scale_h) with g = factor inner /
+ * b_length returned here, inner the routine's bracket in its evaluation
+ * order. */
+static double boundary_g(int skf, double br, double hk) {
+    const int sh = scale_h(skf);
+    double q = br / B_LENGTH * sh;
+    double t = (double)sh / B_LENGTH, factor, inner;
+    if (skf == 1) {            /* b_Quadratic (:106-112) */
+        factor = 2. / PI_ * (t * t);
+        inner = 3. / 8. * q - 3. / 4.;
+    } else if (skf == 2 || skf == 7 || skf == 8) {   /* b_Gaussian (:177-183) */
+        factor = 1. / PI_ * (t * t);
+        inner = -(2 * q * exp(-ipow(q, 2)));
+    } else if (skf == 3) {     /* b_quintic_spline (:250-275) */
+        factor = 15. / (478. * PI_) * (t * t);
+        if (q >= 0. && q <= 1.)
+            inner = -(5 * ipow(3 - q, 4)) + 30 * ipow(2 - q, 4) - 75 * ipow(1 - q, 4);
+        else if (q > 1. && q <= 2.)
+            inner = -(5 * ipow(3 - q, 4)) + 30 * ipow(2 - q, 4);
+        else
+            inner = -(5 * ipow(3 - q, 4));
+    } else if (skf == 4) {     /* b_Wendland_C2 (:343-349) */
+        factor = 3. / (2. * PI_) * (t * t);
+        inner = 20. * (ipow(q, 4) - 3 * ipow(q, 3) + 3 * ipow(q, 2) - q);
+    } else if (skf == 5) {     /* b_Wendland_C4 (:417-423) */
+        factor = 3. / PI_ * (t * t);
+        inner = -(56 * q * (5 * q + 1) * ipow(1 - q, 5));
+    } else {                   /* b_Wendland_C6 (:490-496): hsml_ij(int_id) */
+        double ih = 1.0 / hk;
+        factor = 78. / (7. * PI_) * (ih * ih);
+        inner = -(22. * q * ipow(1 - q, 7) * (16 * ipow(q, 2) + 7. * q + 1));
+    }
+    return factor * inner / B_LENGTH;
+}
+
 static void mc_viscosity(double e11, double e12, double e22, double P, double *z,
                          double *et) {
     double sq = sqrt((e11 - e22) * (e11 - e22) + 4 * (e12 * e12));
@@ -569,7 +606,9 @@ double orc_sound_speed(double rho) {
  * density update (time_integration.f90:140, 230, 274, 355, 383; and
  * initialize.f90:199-200) only when a switch reads it (sound_speed.f90:21);
  * otherwise it stays at its initial zero (initialize.f90:34). */
-static int needs_c(const orc_params *prm) { return prm->art_visc || prm->tse == 1; }
+static int needs_c(const orc_params *prm) {
+    return prm->art_visc || prm->tse == 1 || prm->boundary_friction;
+}
 
 /* regularization (regularization.f90:5-36): XSPH when requested, then the
  * artificial viscosity art_visc_alg names -- which runs whenever
@@ -673,8 +712,9 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         div[i] = 0.;
         grad[4 * i] = grad[4 * i + 1] = grad[4 * i + 2] = grad[4 * i + 3] = 0.;
     }
-    /* NNPS (:47) */
-    int rc = orc_nnps(c, n, x, hsml, nb, xb);
+    /* NNPS (:47); boundary pairs only with a boundary force (nnps.f90:440) */
+    const int bforce = prm->boundary_pressure || prm->boundary_friction;
+    int rc = orc_nnps(c, n, x, hsml, bforce ? nb : 0, xb);
     if (rc) return rc;
     /* pair loop (:56-75): kernel (Wendland C6 / C2 / C4, kernel.f90:471-520,
      * :359-408, :415-464; scale_h = 1 for all three, initialize.f90:208-209),
@@ -727,7 +767,8 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double et = z / (E_ELL * E_ELL);
         if (prm->rheology == 2) mc_viscosity(e11, e12, e22, P, &z, &et);
         double pr = z * 2 * delta / (1 + K_T);               /* kreyscher_eos */
-        if (prm->eos_alg == 1) {
+        if (prm->eos_alg == 1 || !prm->pressure_force) {
+            /* hibler_eos, or no eos call at all: p keeps the ice strength */
             pr = P;                                           /* hibler_eos (eos.f90:59) */
         } else if (prm->eos_alg == 2) {
             /* gutfraind_eos (eos.f90:94-102): rho_max = 934, rho_0 = rho_ice,
@@ -779,34 +820,62 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
      * of the particle-pair list (:491); stale entries beyond nip are read as
      * they are. */
     const int bfa = prm->bfa ? prm->bfa : 1;
+    const int bp = prm->boundary_pressure, bf = prm->boundary_friction;
     for (int k = 0; k < c->bnip; ++k) {
         int i = c->bp_j[k] - 1, b = c->bp_i[k] - 1;
-        if (bfa == 3) continue;   /* bf_Marquis: no force with boundary_pressure (:211-214) */
+        double br = c->br_ij[k];
+        if (bfa == 3) {
+            /* bf_Marquis (:210-221): nothing with the boundary pressure on;
+             * the friction brakes the particle, less with distance */
+            if (bp || !bf) continue;
+            dvxdt[2 * i] = dvxdt[2 * i] - vx[2 * i] * (1 - (br / B_LENGTH));
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] - vx[2 * i + 1] * (1 - (br / B_LENGTH));
+            continue;
+        }
         if (bfa == 2) {
-            /* bf_Wang (boundary_force.f90:145-174), boundary_friction off:
-             * the normal velocity and the rest of the speed, each over dt */
+            /* bf_Wang (boundary_force.f90:145-174): the normal velocity and
+             * the rest of the speed, each over dt; with friction the
+             * tangential part is capped at xi times the normal force */
+            if (!bp) continue;
             const double n0 = prm->b_normal ? prm->b_normal[2 * b] : 0.0;
             const double n1 = prm->b_normal ? prm->b_normal[2 * b + 1] : 0.0;
             double v0 = n0 * vx[2 * i] + n1 * vx[2 * i + 1];
             double v1 = sqrt(vx[2 * i] * vx[2 * i] + vx[2 * i + 1] * vx[2 * i + 1]) - v0;
             double F0 = mass[i] * v0 / prm->step_dt, F1 = mass[i] * v1 / prm->step_dt;
+            if (bf && fabs(v1) > fabs(XI_FRIC * v0)) {
+                v1 = v1 * XI_FRIC * fabs(F0 / F1);
+                F1 = mass[i] * v1 / prm->step_dt;
+            }
             dvxdt[2 * i] = dvxdt[2 * i] + (F0 * n0 + F1 * n1);
             dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (F0 * n1 - F1 * n0);
             continue;
         }
         double bdx0 = xb[2 * b] - x[2 * i], bdx1 = xb[2 * b + 1] - x[2 * i + 1];
-        double br = c->br_ij[k];
         double hk = k < c->cap ? c->hsml_ij[k] : 0.0;
-        double bw = boundary_w(skf, br, hk);
-        double den = (mass[i] + massb[b]) * (br * br);
-        double F0 = KAPPA_N * bdx0 * bw * 2 * massb[b] / den;
-        double F1 = KAPPA_N * bdx1 * bw * 2 * massb[b] / den;
-        dvxdt[2 * i] = dvxdt[2 * i] + (-F0) / rho[i];
-        dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (-F1) / rho[i];
+        if (bp) {
+            double bw = boundary_w(skf, br, hk);
+            double den = (mass[i] + massb[b]) * (br * br);
+            double F0 = KAPPA_N * bdx0 * bw * 2 * massb[b] / den;
+            double F1 = KAPPA_N * bdx1 * bw * 2 * massb[b] / den;
+            dvxdt[2 * i] = dvxdt[2 * i] + (-F0) / rho[i];
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (-F1) / rho[i];
+        }
+        if (bf) {
+            /* bf_Monaghan's friction (:87-100): kappa_t c v . bdx over br rho
+             * against the boundary kernel's gradient */
+            const double g = boundary_g(skf, br, hk);
+            const double sh = scale_h(skf);
+            const double bw0 = -(g * (bdx0 / br) * sh), bw1 = -(g * (bdx1 / br) * sh);
+            const double ci = orc_sound_speed(rho[i]);
+            const double v_r = vx[2 * i] * bdx0 + vx[2 * i + 1] * bdx1;
+            const double pi_aj = KAPPA_T * ci * v_r / br / rho[i];
+            dvxdt[2 * i] = dvxdt[2 * i] + (-(massb[b] * pi_aj * bw0)) / rho[i];
+            dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (-(massb[b] * pi_aj * bw1)) / rho[i];
+        }
     }
     /* wind_forcing (wind_forcing.f90:35-50) and water_drag
      * (water_drag.f90:35-51): both gated on current_time > 0 */
-    if (current_time > 0.) {
+    if (prm->wind_force && current_time > 0.) {
         for (int i = 0; i < n; ++i) {
             double wu, wv;
             wind_at(prm, x[2 * i], x[2 * i + 1], &wu, &wv);
@@ -815,6 +884,8 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
             dvxdt[2 * i] = dvxdt[2 * i] + (C_WIND * RHO_AIR * du * fabs(du)) / hr;
             dvxdt[2 * i + 1] = dvxdt[2 * i + 1] + (C_WIND * RHO_AIR * dv * fabs(dv)) / hr;
         }
+    }
+    if (prm->water_force && current_time > 0.) {
         for (int i = 0; i < n; ++i) {
             double du = prm->current[0] - vx[2 * i], dv = prm->current[1] - vx[2 * i + 1];
             double hr = h[i] * RHO_ICE;

@@ -70,6 +70,14 @@ typedef struct orc_params {
     const double *b_normal;  /* (2, nb) boundary normals, bf_Wang only */
     double step_dt;      /* the module dt single_step sees (bf_Wang reads it);
                             orc_run_pc sets it to each step's dt */
+    /* the logical scenario switches of globals.f90:178-187 (no zero
+     * default: 1 is the reference's value except for boundary_friction) */
+    int    boundary_friction;  /* tangential boundary forces */
+    int    boundary_pressure;  /* normal boundary forces; with neither, no
+                                  boundary pairs (nnps.f90:440) */
+    int    pressure_force;     /* 0: eos is not called, p stays the ice
+                                  strength P (internal_stress.f90:35-39) */
+    int    wind_force, water_force;
 } orc_params;
 
 orc_ctx *orc_create(void);

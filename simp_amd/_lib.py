@@ -25,6 +25,10 @@ SIMP_WIND_UNIFORM = 0
 SIMP_WIND_GRID = 1
 SIMP_CONFIG_VERSION = 2
 SIMP_SLE_KEEP = -1
+SIMP_OFF_BOUNDARY_PRESSURE = 1
+SIMP_OFF_PRESSURE_FORCE = 2
+SIMP_OFF_WIND = 4
+SIMP_OFF_WATER = 8
 
 #: every entry point declared in include/simp.h
 EXPORTS = (
@@ -54,7 +58,9 @@ class SimpConfig(C.Structure):
                 ("reserved", C.c_int32), ("wind", C.c_double * 2),
                 ("current", C.c_double * 2), ("max_hsml", C.c_double),
                 ("artificial_visc", C.c_int32), ("art_visc_alg", C.c_int32),
-                ("xsph_regularization", C.c_int32), ("reserved2", C.c_int32)]
+                ("xsph_regularization", C.c_int32), ("boundary_friction", C.c_int32),
+                ("boundary_pressure", C.c_int32), ("pressure_force", C.c_int32),
+                ("wind_force", C.c_int32), ("water_force", C.c_int32)]
 
 
 class SimpInstanceDesc(C.Structure):
@@ -73,8 +79,9 @@ class SimpInstanceDesc(C.Structure):
                 ("den_alg", C.c_int32), ("pa_sph", C.c_int32), ("eos_alg", C.c_int32),
                 ("rheology", C.c_int32), ("skf", C.c_int32), ("reserved2", C.c_int32),
                 ("sle", C.c_int32), ("xsph", C.c_int32), ("art_visc", C.c_int32),
-                ("art_visc_alg", C.c_int32), ("bfa", C.c_int32), ("reserved3", C.c_int32),
-                ("b_normal", C.c_void_p)]
+                ("art_visc_alg", C.c_int32), ("bfa", C.c_int32),
+                ("boundary_friction", C.c_int32), ("b_normal", C.c_void_p),
+                ("forces_off", C.c_int32), ("reserved3", C.c_int32)]
 
 
 class SimpTuning(C.Structure):

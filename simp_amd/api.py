@@ -30,10 +30,18 @@ def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(
                 current=(0.0, 0.0), max_hsml: float = 0.0, wind_grid: bool = False,
                 den_alg: int = 3, pa_sph: int = 1, eos_alg: int = 3, rheology: int = 1,
                 skf: int = 6, int_algorithm: int = 2, sle: int = 2, xsph: int = 0,
-                art_visc: int = 0, art_visc_alg: int = 1, bfa: int = 1) -> SimpConfig:
+                art_visc: int = 0, art_visc_alg: int = 1, bfa: int = 1,
+                boundary_friction: int = 0, boundary_pressure: int = 1,
+                pressure_force: int = 1, wind_force: int = 1,
+                water_force: int = 1) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
+    cfg.boundary_friction = boundary_friction
+    cfg.boundary_pressure = boundary_pressure
+    cfg.pressure_force = pressure_force
+    cfg.wind_force = wind_force
+    cfg.water_force = water_force
     cfg.sle = sle
     cfg.xsph_regularization = xsph
     cfg.artificial_visc = art_visc
@@ -81,10 +89,12 @@ class Simp:
                  max_hsml: float = 0.0, wind_grid=None, den_alg: int = 3, pa_sph: int = 1,
                  eos_alg: int = 3, rheology: int = 1, skf: int = 6, int_algorithm: int = 2,
                  sle: int = 2, xsph: int = 0, art_visc: int = 0, art_visc_alg: int = 1,
-                 bfa: int = 1):
+                 bfa: int = 1, boundary_friction: int = 0, boundary_pressure: int = 1,
+                 pressure_force: int = 1, wind_force: int = 1, water_force: int = 1):
         self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
                                den_alg, pa_sph, eos_alg, rheology, skf, int_algorithm, sle,
-                               xsph, art_visc, art_visc_alg, bfa)
+                               xsph, art_visc, art_visc_alg, bfa, boundary_friction,
+                               boundary_pressure, pressure_force, wind_force, water_force)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -103,7 +113,10 @@ class Simp:
                    den_alg=sc.den_alg, pa_sph=sc.pa_sph, eos_alg=sc.eos_alg,
                    rheology=sc.rheology, skf=sc.skf, int_algorithm=sc.int_algorithm,
                    sle=sc.sle, xsph=sc.xsph, art_visc=sc.art_visc,
-                   art_visc_alg=sc.art_visc_alg, bfa=sc.bfa)
+                   art_visc_alg=sc.art_visc_alg, bfa=sc.bfa,
+                   boundary_friction=sc.boundary_friction,
+                   boundary_pressure=sc.boundary_pressure, pressure_force=sc.pressure_force,
+                   wind_force=sc.wind_force, water_force=sc.water_force)
 
     def set_module_state(self, sc) -> None:
         """dt and the boundary normals of ``sc``, which the Wang boundary

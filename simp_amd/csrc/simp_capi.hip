@@ -110,6 +110,11 @@ void simp_config_default(simp_config *c) {
   c->artificial_visc = 0;
   c->art_visc_alg = 1;
   c->xsph_regularization = 0;
+  c->boundary_friction = 0;
+  c->boundary_pressure = 1;
+  c->pressure_force = 1;
+  c->wind_force = 1;
+  c->water_force = 1;
 }
 
 const char *simp_strerror(int s) {
@@ -188,6 +193,14 @@ int simp_set_wind_grid(simp_handle *h, int nx, int ny, double x0, double y0, dou
   return SIMP_OK;
 }
 
+// Without a boundary force the reference finds no boundary pairs
+// (nnps.f90:440) and never reads the boundary particles.
+static void drop_unused_boundary(InstanceInput &in) {
+  if (in.boundary_pressure || in.boundary_friction) return;
+  in.nb = 0;
+  in.xb = in.massb = in.bnormal = nullptr;
+}
+
 static int fill_input(simp_handle *h, InstanceInput &in) {
   const simp_config &c = h->cfg;
   in.tse = c.tse;
@@ -209,6 +222,12 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
     if ((int)(h->bnormal.size() / 2) != in.nb) return SIMP_E_ARG;
     in.bnormal = h->bnormal.data();
   }
+  in.boundary_friction = c.boundary_friction ? 1 : 0;
+  in.boundary_pressure = c.boundary_pressure ? 1 : 0;
+  in.pressure_force = c.pressure_force ? 1 : 0;
+  in.wind_force = c.wind_force ? 1 : 0;
+  in.water_force = c.water_force ? 1 : 0;
+  drop_unused_boundary(in);
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
     in.g_nx = h->g_nx; in.g_ny = h->g_ny;
@@ -292,7 +311,8 @@ int simp_single_step_noalloc(simp_handle *h, int n, const double *x, const doubl
   if (rc) return rc;
   // the regularization, smoothing-length and boundary-force switches need
   // arrays this layout does not carve
-  if (in.sle != 2 || in.xsph || in.art_visc || in.bfa != 1) return SIMP_E_UNSUPPORTED;
+  if (in.sle != 2 || in.xsph || in.art_visc || in.bfa != 1 || in.boundary_friction)
+    return SIMP_E_UNSUPPORTED;
   // a handle's first call starts the persistent arrays from zero, as a new
   // handle's simp_single_step does; later calls find them as the last call
   // on this GPU left them
@@ -686,6 +706,9 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.sle != SIMP_SLE_KEEP && (d.sle < 0 || d.sle > 2)) return SIMP_E_UNSUPPORTED;
   if (d.art_visc_alg < 0 || d.art_visc_alg > 2 || d.bfa < 0 || d.bfa > 3)
     return SIMP_E_UNSUPPORTED;
+  if (d.forces_off & ~(SIMP_OFF_BOUNDARY_PRESSURE | SIMP_OFF_PRESSURE_FORCE | SIMP_OFF_WIND |
+                       SIMP_OFF_WATER))
+    return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
        !(d.grid_dy > 0)))
@@ -707,6 +730,12 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.art_visc_alg = d.art_visc_alg == 2 ? 2 : 1;
   in.bfa = d.bfa == 0 ? 1 : d.bfa;
   in.bnormal = d.b_normal;
+  in.boundary_friction = d.boundary_friction ? 1 : 0;
+  in.boundary_pressure = (d.forces_off & SIMP_OFF_BOUNDARY_PRESSURE) ? 0 : 1;
+  in.pressure_force = (d.forces_off & SIMP_OFF_PRESSURE_FORCE) ? 0 : 1;
+  in.wind_force = (d.forces_off & SIMP_OFF_WIND) ? 0 : 1;
+  in.water_force = (d.forces_off & SIMP_OFF_WATER) ? 0 : 1;
+  drop_unused_boundary(in);
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
   in.current[0] = d.current[0]; in.current[1] = d.current[1];

@@ -40,6 +40,10 @@ constexpr double kNu = (double)0.3f;
 constexpr double kAlphaPi = 1.0;
 constexpr double kBetaPi = 2.0;
 constexpr double kChi = (double)0.3f;
+// boundary friction (data_variables.f90:175-177): kappa_t = 5.D-8, xi =
+// 0.5225 (default REAL)
+constexpr double kKappaT = 5.e-8;
+constexpr double kXi = (double)0.5225f;
 constexpr int kMaxIndInt = 200;     // globals.f90:24
 constexpr int kPrintStep = 2000;    // globals.f90:55
 
@@ -80,6 +84,9 @@ struct InstMeta {
   int avdt;              // 1: the Monaghan time step takes the viscous CFL terms
   int bfa;               // boundary force: 1 Monaghan, 2 Wang, 3 Marquis
   int endp;              // 1: an end-of-step pair pass (Benz hsml, viscous CFL)
+  int fric;              // 1: boundary friction (boundary_friction)
+  int bpres;             // 1: boundary normal force (boundary_pressure)
+  int wind_on, water_on; // wind_force, water_force (globals.f90:182-183)
   int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;

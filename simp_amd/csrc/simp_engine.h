@@ -33,6 +33,10 @@ struct InstanceInput {
   int art_visc_alg = 1;  // 1: Hernquist-Katz; 2: Monaghan
   int bfa = 1;           // 1: Monaghan; 2: Wang; 3: Marquis boundary force
   const double *bnormal = nullptr;   // (2, nb) boundary normals (Wang); null: zeros
+  // the logical scenario switches of globals.f90:178-187 (without a
+  // boundary force no boundary pairs exist: the boundary is dropped)
+  int boundary_friction = 0, boundary_pressure = 1, pressure_force = 1;
+  int wind_force = 1, water_force = 1;
   // optional wind grid
   int g_nx = 0, g_ny = 0;
   double g_x0 = 0, g_y0 = 0, g_dx = 1, g_dy = 1;

@@ -23,7 +23,33 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // kernels then run their variants that read each instance's values
 static inline bool nondefault(const InstMeta &m) {
   return m.den_alg != 3 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
-         m.tse == 1 || m.sle != 2 || m.xsph || m.avis || m.bfa != 1;
+         m.tse == 1 || m.sle != 2 || m.xsph || m.avis || m.bfa != 1 || m.fric || !m.bpres ||
+         !m.wind_on || !m.water_on;
+}
+
+// The switch fields of an instance's record.
+static void meta_switches(InstMeta &m, const InstanceInput &in) {
+  m.den_alg = in.den_alg;
+  m.pa_sph = in.pa_sph;
+  // without the pressure force eos is not called and p stays the ice
+  // strength: the Hibler pressure (internal_stress.f90:21-39)
+  m.eos_alg = in.pressure_force ? in.eos_alg : 1;
+  m.rheology = in.rheology;
+  m.skf = in.skf;
+  m.sle = in.sle;
+  m.fric = in.boundary_friction ? 1 : 0;
+  m.bpres = in.boundary_pressure ? 1 : 0;
+  m.wind_on = in.wind_force ? 1 : 0;
+  m.water_on = in.water_force ? 1 : 0;
+  m.xsph = in.xsph ? 1 : 0;
+  // regularization runs for XSPH or the artificial viscosity, and then
+  // always with the viscosity art_visc_alg names (regularization.f90:29-35)
+  m.avis = (in.xsph || in.art_visc) ? in.art_visc_alg : 0;
+  // c_yang is computed only for these switches (sound_speed.f90:21)
+  m.needc = (in.art_visc || in.tse == 1 || in.boundary_friction) ? 1 : 0;
+  m.avdt = (in.art_visc && in.tse == 1) ? 1 : 0;
+  m.bfa = in.bfa;
+  m.endp = (in.sle == 1 || m.avdt) ? 1 : 0;
 }
 
 // device bytes of the arrays the regularization and smoothing-length
@@ -314,21 +340,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
     m.cur_u = in.current[0]; m.cur_v = in.current[1];
     m.max_hsml = in.max_hsml;
-    m.den_alg = in.den_alg;
-    m.pa_sph = in.pa_sph;
-    m.eos_alg = in.eos_alg;
-    m.rheology = in.rheology;
-    m.skf = in.skf;
-    m.sle = in.sle;
-    m.xsph = in.xsph ? 1 : 0;
-    // regularization runs for XSPH or the artificial viscosity, and then
-    // always with the viscosity art_visc_alg names (regularization.f90:29-35)
-    m.avis = (in.xsph || in.art_visc) ? in.art_visc_alg : 0;
-    // c_yang is computed only for these switches (sound_speed.f90:21)
-    m.needc = (in.art_visc || in.tse == 1) ? 1 : 0;
-    m.avdt = (in.art_visc && in.tse == 1) ? 1 : 0;
-    m.bfa = in.bfa;
-    m.endp = (in.sle == 1 || m.avdt) ? 1 : 0;
+    meta_switches(m, in);
     if (in.wind_mode == 1) {
       m.g_nx = in.g_nx; m.g_ny = in.g_ny;
       m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -651,11 +663,14 @@ int Engine::reload(const InstanceInput &in, bool grid_same) {
   if (meta_.size() != 1 || d_.ninst != 1 || blk_host_.empty()) return 1;
   const InstMeta &o = meta_[0];
   if (in.n != o.n || in.nb != o.nb || in.tse != o.tse || in.wind_mode != o.wind_mode ||
-      in.den_alg != o.den_alg || in.pa_sph != o.pa_sph || in.eos_alg != o.eos_alg ||
+      in.den_alg != o.den_alg || in.pa_sph != o.pa_sph ||
+      (in.pressure_force ? in.eos_alg : 1) != o.eos_alg ||
       in.rheology != o.rheology || in.skf != o.skf || in.sle != o.sle ||
       (in.xsph ? 1 : 0) != o.xsph || in.bfa != o.bfa ||
       ((in.xsph || in.art_visc) ? in.art_visc_alg : 0) != o.avis ||
-      (in.art_visc || in.tse == 1 ? 1 : 0) != o.needc)
+      (in.art_visc || in.tse == 1 || in.boundary_friction ? 1 : 0) != o.needc ||
+      (in.boundary_friction ? 1 : 0) != o.fric || (in.boundary_pressure ? 1 : 0) != o.bpres ||
+      (in.wind_force ? 1 : 0) != o.wind_on || (in.water_force ? 1 : 0) != o.water_on)
     return 1;
   // (the boundary normals of the Wang force are laid out with the boundary)
   if (in.bfa == 2) return 1;
@@ -866,11 +881,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   m.wind_u = in.wind[0]; m.wind_v = in.wind[1];
   m.cur_u = in.current[0]; m.cur_v = in.current[1];
   m.max_hsml = in.max_hsml;
-  m.den_alg = in.den_alg;
-  m.pa_sph = in.pa_sph;
-  m.eos_alg = in.eos_alg;
-  m.rheology = in.rheology;
-  m.skf = in.skf;
+  meta_switches(m, in);
   if (in.wind_mode == 1) {
     m.g_nx = in.g_nx; m.g_ny = in.g_ny;
     m.g_x0 = in.g_x0; m.g_y0 = in.g_y0; m.g_dx = in.g_dx; m.g_dy = in.g_dy;
@@ -958,6 +969,8 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   any_boundary_ = in.nb > 0;
   any_xsw_ = nondefault(m);
   any_cden_ = m.den_alg == 2;
+  any_av_ = false;      // (the caller rejects the switches that need arrays
+  any_endp_ = false;    //  this layout does not carve)
   // grid of the first evaluation
   launch_partial(d_, 0, st_);
   launch_finalize(d_, 0, st_);

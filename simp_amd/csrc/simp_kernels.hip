@@ -1125,6 +1125,42 @@ __device__ __forceinline__ double bkern_w(int skf, double br, double hk) {
   return kC6 * (ih * ih) * ipow<8>(omq) * (32 * ipow<3>(q) + 25 * ipow<2>(q) + 8 * q + 1);
 }
 
+// The boundary kernels' gradient scalar g: bdwdx = -(g (bdx / br)
+// scale_h), g = factor inner / b_length with each b_ routine's bracket in its
+// evaluation order (boundary_kernel.f90:110-112, 182-183, 257-274, 348-349,
+// 422-423, 495-496)
+__device__ double bkern_g(int skf, double br, double hk) {
+  const double sh = kern_scale(skf), q = br / kBLength * sh, t = sh / kBLength;
+  double factor, inner;
+  if (skf == 1) {
+    factor = 2. / kPi * (t * t);
+    inner = 3. / 8. * q - 3. / 4.;
+  } else if (skf == 2 || skf == 7 || skf == 8) {
+    factor = 1. / kPi * (t * t);
+    inner = -(2 * q * exp(-ipow<2>(q)));
+  } else if (skf == 3) {
+    factor = 15. / (478. * kPi) * (t * t);
+    const double a = ipow<4>(3 - q), b = ipow<4>(2 - q), c = ipow<4>(1 - q);
+    if (q >= 0. && q <= 1.)
+      inner = -(5 * a) + 30 * b - 75 * c;
+    else if (q > 1. && q <= 2.)
+      inner = -(5 * a) + 30 * b;
+    else
+      inner = -(5 * a);
+  } else if (skf == 4) {
+    factor = 3. / (2. * kPi) * (t * t);
+    inner = 20. * (ipow<4>(q) - 3 * ipow<3>(q) + 3 * ipow<2>(q) - q);
+  } else if (skf == 5) {
+    factor = 3. / kPi * (t * t);
+    inner = -(56 * q * (5 * q + 1) * ipow<5>(1 - q));
+  } else {
+    const double ih = 1.0 / hk;
+    factor = 78. / (7. * kPi) * (ih * ih);
+    inner = -(22. * q * ipow<7>(1 - q) * (16 * ipow<2>(q) + 7. * q + 1));
+  }
+  return factor * inner / kBLength;
+}
+
 __device__ __forceinline__ float bkern_w_f(int skf, float q, float hk) {
   const float omq = 1.0f - q, o2 = omq * omq, o4 = o2 * o2;
   if (skf == 4 || skf == 5) {
@@ -2306,6 +2342,7 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       float fbx = 0.0f, fby = 0.0f;   // mixed: the forces' sum
       const bool rfresh = phase != 1 && y.rfresh;
       const int bfa = XSW ? m.bfa : 1;
+      const bool bpres = !XSW || m.bpres, fric = XSW && m.fric;
       const int *bl = d.blist + 9 * (size_t)m.b_off;
       for (int u0 = 0; __any(run.x + u0 < run.y); u0 += kBU) {
         int ent[kBU], rb[kBU];
@@ -2354,19 +2391,56 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
           if (!inr[j]) continue;
           ++nbn;
           const double bdx = bx[j] - xi, bdy = by[j] - yi;
-          if (bfa == 3) continue;   // bf_Marquis: no force with boundary_pressure (:211-214)
+          if (bfa == 3) {
+            // bf_Marquis (:210-221): no force with the boundary pressure on;
+            // the friction brakes the particle, less with distance (FP64)
+            if (bpres || !fric) continue;
+            const double br = sqrt(bdx * bdx + bdy * bdy);
+            ax = ax - vxi * (1 - (br / kBLength));
+            ay = ay - vyi * (1 - (br / kBLength));
+            continue;
+          }
           if (bfa == 2) {
             // bf_Wang (boundary_force.f90:145-174): the normal velocity and
-            // the rest of the speed, each as a momentum over dt (FP64)
+            // the rest of the speed, each as a momentum over dt; friction
+            // caps the tangential part at xi times the normal (FP64)
+            if (!bpres) continue;
             const int bg = m.b_off + (ent[j] & 0xffffff);
             const double n0 = d.bnx[bg], n1 = d.bny[bg];
             const double v0 = n0 * vxi + n1 * vyi;
-            const double v1 = sqrt(vxi * vxi + vyi * vyi) - v0;
-            const double F0 = mi * v0 / dt, F1 = mi * v1 / dt;
+            double v1 = sqrt(vxi * vxi + vyi * vyi) - v0;
+            const double F0 = mi * v0 / dt;
+            double F1 = mi * v1 / dt;
+            if (fric && fabs(v1) > fabs(kXi * v0)) {
+              v1 = v1 * kXi * fabs(F0 / F1);
+              F1 = mi * v1 / dt;
+            }
             ax = ax + (F0 * n0 + F1 * n1);
             ay = ay + (F0 * n1 - F1 * n0);
             continue;
           }
+          if (fric) {
+            // bf_Monaghan's normal force (FP64 here), then its friction
+            // (:87-100): kappa_t c v . bdx / br / rho against the boundary
+            // kernel's gradient
+            const double br = sqrt(bdx * bdx + bdy * bdy);
+            if (bpres) {
+              const double bw = bkern_w(skf, br, hk[j]);
+              const double den = (mi + mb[j]) * (br * br);
+              const double F0 = kKappaN * bdx * bw * 2 * mb[j] / den;
+              const double F1 = kKappaN * bdy * bw * 2 * mb[j] / den;
+              ax = ax + (-F0) / rhoi;
+              ay = ay + (-F1) / rhoi;
+            }
+            const double g = bkern_g(skf, br, hk[j]);
+            const double bw0 = -(g * (bdx / br) * ksc), bw1 = -(g * (bdy / br) * ksc);
+            const double v_r = vxi * bdx + vyi * bdy;
+            const double pi_aj = kKappaT * sound_speed(rhoi) * v_r / br / rhoi;
+            ax = ax + (-(mb[j] * pi_aj * bw0)) / rhoi;
+            ay = ay + (-(mb[j] * pi_aj * bw1)) / rhoi;
+            continue;
+          }
+          if (!bpres) continue;
           if (MIXED) {
             // mixed: the force in FP32 (hardware sqrt and reciprocals);
             // the pair set and its ranks stay those of the FP64 test
@@ -2409,6 +2483,8 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
     }
 
     // wind_forcing / water_drag (wind_forcing.f90:35-50, water_drag.f90:35-51)
+    // (each gated by its scenario switch, globals.f90:182-183)
+    const bool won = !XSW || m.wind_on, con = !XSW || m.water_on;
     if (y.time > 0.) {
       const double du = wu - vxi, dv = wv - vyi;
       const double hr = hthi * kRhoIce;
@@ -2416,13 +2492,21 @@ __global__ void __launch_bounds__(kTPB) k_passB(DevState d) {
       if (MIXED) {
         // one division for the four terms
         const double ihr = 1.0 / hr;
-        ax = ax + (kCWind * kRhoAir * du * fabs(du) + kCWater * kRhoWater * cu * fabs(cu)) * ihr;
-        ay = ay + (kCWind * kRhoAir * dv * fabs(dv) + kCWater * kRhoWater * cv * fabs(cv)) * ihr;
+        const double wx = won ? kCWind * kRhoAir * du * fabs(du) : 0.0;
+        const double wy = won ? kCWind * kRhoAir * dv * fabs(dv) : 0.0;
+        const double cx = con ? kCWater * kRhoWater * cu * fabs(cu) : 0.0;
+        const double cy = con ? kCWater * kRhoWater * cv * fabs(cv) : 0.0;
+        ax = ax + (wx + cx) * ihr;
+        ay = ay + (wy + cy) * ihr;
       } else {
-        ax = ax + (kCWind * kRhoAir * du * fabs(du)) / hr;
-        ay = ay + (kCWind * kRhoAir * dv * fabs(dv)) / hr;
-        ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
-        ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
+        if (won) {
+          ax = ax + (kCWind * kRhoAir * du * fabs(du)) / hr;
+          ay = ay + (kCWind * kRhoAir * dv * fabs(dv)) / hr;
+        }
+        if (con) {
+          ax = ax + (kCWater * kRhoWater * cu * fabs(cu)) / hr;
+          ay = ay + (kCWater * kRhoWater * cv * fabs(cv)) / hr;
+        }
       }
     }
     // the artificial viscosity's pair terms (regularization.f90:198-208,

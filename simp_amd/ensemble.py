@@ -43,6 +43,14 @@ def _desc(sc, nsteps: int, keep: list, itimestep: int = 0) -> SimpInstanceDesc:
     d.art_visc = getattr(sc, "art_visc", 0)
     d.art_visc_alg = getattr(sc, "art_visc_alg", 1)
     d.bfa = getattr(sc, "bfa", 1)
+    d.boundary_friction = getattr(sc, "boundary_friction", 0)
+    off = 0
+    for flag, name in ((_lib.SIMP_OFF_BOUNDARY_PRESSURE, "boundary_pressure"),
+                       (_lib.SIMP_OFF_PRESSURE_FORCE, "pressure_force"),
+                       (_lib.SIMP_OFF_WIND, "wind_force"), (_lib.SIMP_OFF_WATER, "water_force")):
+        if not getattr(sc, name, 1):
+            off |= flag
+    d.forces_off = off
     if sc.bnormal is not None and sc.nb:
         bn = np.ascontiguousarray(sc.b_normal(), dtype=np.float64)
         keep.append(bn)

@@ -20,7 +20,9 @@ module simp_binding
                               max_particles, max_boundary, reserved
         real(c_double)     :: wind(2), current(2), max_hsml
         integer(c_int32_t) :: artificial_visc, art_visc_alg,               &
-                              xsph_regularization, reserved2
+                              xsph_regularization, boundary_friction,      &
+                              boundary_pressure, pressure_force,           &
+                              wind_force, water_force
     end type simp_config
 
     interface

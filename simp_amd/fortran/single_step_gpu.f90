@@ -30,6 +30,12 @@ subroutine single_step
     integer(c_int) :: st, nipc, bnipc
 
     if (.not. c_associated(handle)) then
+        if (.not. viscosity) then
+            ! internal_stress would leave p, zeta and eta as earlier calls
+            ! left them (internal_stress.f90:21-31): not built
+            write(*,*) 'single_step (GPU): viscosity = .FALSE. is not supported'
+            stop
+        endif
         call simp_config_default(cfg)
         cfg%max_hsml = max_hsml
         cfg%den_alg = den_alg
@@ -44,6 +50,11 @@ subroutine single_step
         cfg%art_visc_alg = art_visc_alg
         cfg%artificial_visc = merge(1, 0, artificial_visc)
         cfg%xsph_regularization = merge(1, 0, xsph_regularization)
+        cfg%boundary_friction = merge(1, 0, boundary_friction)
+        cfg%boundary_pressure = merge(1, 0, boundary_pressure)
+        cfg%pressure_force = merge(1, 0, pressure_force)
+        cfg%wind_force = merge(1, 0, wind_force)
+        cfg%water_force = merge(1, 0, water_force)
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

@@ -71,6 +71,12 @@ class Scenario:
     art_visc: int = 0        # 1: artificial viscosity
     art_visc_alg: int = 1    # 1 Hernquist and Katz, 2 Monaghan
     bfa: int = 1             # boundary force: 1 Monaghan, 2 Wang, 3 Marquis
+    # the logical scenario switches of globals.f90:178-187 (1 = .TRUE.)
+    boundary_friction: int = 0
+    boundary_pressure: int = 1
+    pressure_force: int = 1
+    wind_force: int = 1
+    water_force: int = 1
     #: inward unit normal of each boundary particle (nb, 2), as
     #: boundary_placing_1 sets b_normal (in_out/boundaries.f90:232-287);
     #: read by the Wang boundary force only.  None: zeros.

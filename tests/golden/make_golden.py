@@ -6,7 +6,7 @@ inputs and stores inputs and outputs as ``.npz`` fixtures next to this
 script.  The reference has no tests or fixtures of its own; these vectors are
 the pin for the oracle.  Regenerate with::
 
-    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos] [switches] [regularization]
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py [main] [den1] [pa2] [eos] [switches] [regularization] [friction]
 """
 from __future__ import annotations
 
@@ -329,10 +329,35 @@ def regularization():
     _pc10_edge("bfa3", H["bfa3"], bfa=3)
 
 
+def friction():
+    """12. boundary friction: bf_Monaghan's tangential term with the
+    gradients of six boundary kernels (C6, quadratic, Gaussian, quintic
+    spline, C2, C4), Wang's friction cap, Marquis' braking (with the
+    boundary pressure off); the boundary pressure off (no boundary pairs);
+    wind, water drag and the replacement pressure off -- single steps of the
+    closed box and steps of the ice edge, each from the harness built with
+    those PARAMETERs."""
+    H = refrun.HARNESS_VARIANT
+    F = {"boundary_friction": 1}
+    for tag, sw in (("fric", {}), ("fric1", {"skf": 1}), ("fricg", {"skf": 2}),
+                    ("fricq", {"skf": 3}), ("fric4", {"skf": 4}), ("fric5", {"skf": 5})):
+        _single_box(tag, H[tag], **F, **sw)
+    _pc10_edge("fric", H["fric"], **F)
+    _single_box("bfa2f", H["bfa2f"], bfa=2, **F)
+    _pc10_edge("bfa2f", H["bfa2f"], K=2, bfa=2, **F)
+    _single_box("bfa3f", H["bfa3f"], bfa=3, boundary_pressure=0, **F)
+    _pc10_edge("bfa3f", H["bfa3f"], bfa=3, boundary_pressure=0, **F)
+    _single_box("nobp", H["nobp"], boundary_pressure=0)
+    _pc10_edge("nobp", H["nobp"], boundary_pressure=0)
+    _single_box("noforce", H["noforce"], wind_force=0, water_force=0, pressure_force=0)
+    _pc10_edge("noforce", H["noforce"], wind_force=0, water_force=0, pressure_force=0)
+
+
 if __name__ == "__main__":
     # no argument: every fixture; otherwise the named groups ("main", "den1",
     # "pa2", "eos", "switches")
-    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches", "regularization"]
+    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches", "regularization",
+                            "friction"]
     if "main" in todo:
         main()
     if "den1" in todo:
@@ -346,3 +371,5 @@ if __name__ == "__main__":
         switches()
     if "regularization" in todo:
         regularization()
+    if "friction" in todo:
+        friction()

@@ -1,0 +1,124 @@
+"""Boundary friction (bf_Monaghan's tangential term with the gradients of
+six boundary kernels, Wang's friction cap, Marquis' braking with the
+boundary pressure off), the boundary pressure off, and wind, water drag and
+the replacement pressure off, on the device through the C ABI: bit for bit
+against the reference built with those PARAMETERs for one evaluation,
+within 1e-12 over ten steps, within the mixed path's tolerance in mixed
+precision, and per instance in a packed ensemble (alone equals mixed)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from golden_io import load, scenario
+from oracle import capi
+from simp_amd import _lib
+from simp_amd import scenarios as S
+from simp_amd.api import Simp
+from simp_amd.ensemble import Ensemble
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "vx", "rho", "hsml", "A", "h")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def nrel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("tag", ["fric", "fric1", "fricg", "fricq", "fric4", "fric5", "bfa2f",
+                                 "bfa3f", "nobp", "noforce"])
+def test_friction_single_step_fp64_vs_reference_golden(tag):
+    say(f"single_step fp64 {tag} golden box2500")
+    g = load(f"box2500_single_step_{tag}.npz")
+    sc = scenario(g)
+    dev = Simp.for_scenario(sc)
+    out = dev.single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        if tag == "fricg":
+            # the Gaussian boundary kernel's exp: the device's may differ
+            # from the host's in the last bit
+            assert nrel(out[k], g["out_" + k]) <= 1e-12, (k, nrel(out[k], g["out_" + k]))
+        else:
+            assert np.array_equal(out[k], g["out_" + k]), (k, nrel(out[k], g["out_" + k]))
+    dev.close()
+
+
+@pytest.mark.parametrize("name", ["edge3600_pc10_fric.npz", "edge3600_pc2_bfa2f.npz",
+                                  "edge3600_pc10_bfa3f.npz", "edge3600_pc10_nobp.npz",
+                                  "edge3600_pc10_noforce.npz"])
+def test_friction_time_integration_fp64_vs_reference_golden(name):
+    say(f"time_integration fp64 {name}")
+    g = load(name)
+    sc = scenario(g)
+    K = int(g["out_itimestep"])
+    dev = Simp.for_scenario(sc)
+    out = dev.time_integration(sc, K)
+    assert out["itimestep"] == K
+    for k in FIELDS:
+        e = nrel(getattr(sc, k), g["out_" + k])
+        assert e <= 1e-12, (k, e)
+    dev.close()
+
+
+@pytest.mark.parametrize("sw", [{"boundary_friction": 1},
+                                {"boundary_friction": 1, "bfa": 3, "boundary_pressure": 0},
+                                {"wind_force": 0, "water_force": 0, "pressure_force": 0}])
+def test_friction_mixed_precision_within_tolerance(sw):
+    say(f"time_integration mixed {sw} vortex 40k, 20 steps")
+    sc = S.vortex()
+    sc.vx = np.random.default_rng(8).normal(0.0, 0.05, size=(sc.n, 2))
+    sc.time = 10.0
+    for k, v in sw.items():
+        setattr(sc, k, v)
+    a, b = sc.copy(), sc.copy()
+    dev = Simp.for_scenario(a, precision=_lib.SIMP_MIXED)
+    dev.time_integration(a, 20)
+    dev.close()
+    capi.Oracle().run_pc(b, 20)
+    for k in FIELDS:
+        e = nrel(getattr(a, k), getattr(b, k))
+        assert e <= 1e-4, (k, e)
+    assert nrel(a.vx - sc.vx, b.vx - sc.vx) <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [_lib.SIMP_FP64, _lib.SIMP_MIXED])
+def test_friction_per_instance_in_an_ensemble(precision):
+    say(f"ensemble with friction and forcing switches, precision={precision}")
+    sws = ({}, {"boundary_friction": 1}, {"boundary_friction": 1, "skf": 3},
+           {"boundary_friction": 1, "bfa": 3, "boundary_pressure": 0},
+           {"boundary_pressure": 0}, {"wind_force": 0}, {"water_force": 0},
+           {"pressure_force": 0})
+    scs = S.ensemble(count=len(sws), nmin=1500, nmax=4000, seed=67)
+    for sc, sw in zip(scs, sws):
+        sc.time = 1e-6
+        sc.vx = np.random.default_rng(sc.n).normal(0.0, 0.01, size=(sc.n, 2))
+        for k, v in sw.items():
+            setattr(sc, k, v)
+    ens = Ensemble(precision)
+    ids = ens.add(scs)
+    ens.step(3)
+    for sc, i, sw in zip(scs, ids, sws):
+        got = ens.download(i)
+        dev = Simp.for_scenario(sc, precision=precision)
+        dev.upload(sc)
+        dev.step(3)
+        want = dev.download()
+        dev.close()
+        for k in FIELDS:
+            assert np.array_equal(got[k], want[k]), (sw, k)
+        if precision == _lib.SIMP_FP64:
+            b = sc.copy()
+            capi.Oracle().run_pc(b, 3)
+            for k in FIELDS:
+                assert nrel(got[k], getattr(b, k)) <= 1e-12, (sw, k)
+    ens.close()

@@ -665,3 +665,67 @@ def test_benz_and_fixed_smoothing_lengths():
     s2 = sc.copy()
     capi.Oracle().run_pc(s2, 3)
     assert not _same(s2.hsml, sc.hsml)
+
+
+# ---------------------------- boundary friction, boundary pressure, forcing off
+
+@pytest.mark.parametrize("tag", ["fric", "fric1", "fricg", "fricq", "fric4", "fric5", "bfa2f",
+                                 "bfa3f", "nobp", "noforce"])
+def test_friction_and_scenario_switches_single_step_bit_exact_vs_reference(tag):
+    """bf_Monaghan's friction with the gradients of six boundary kernels
+    (boundary_force.f90:87-100, boundary_kernel.f90), Wang's friction cap
+    (:157-166), Marquis' braking with the boundary pressure off (:217-221),
+    the boundary pressure off (no boundary pairs, nnps.f90:440), and wind,
+    water drag and the replacement pressure off, against the reference
+    built with those PARAMETERs."""
+    g = load(f"box2500_single_step_{tag}.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    out = o.single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+    if tag == "nobp":
+        assert out["bnip"] == 0
+
+
+@pytest.mark.parametrize("name", ["edge3600_pc10_fric.npz", "edge3600_pc2_bfa2f.npz",
+                                  "edge3600_pc10_bfa3f.npz", "edge3600_pc10_nobp.npz",
+                                  "edge3600_pc10_noforce.npz"])
+def test_friction_and_scenario_switches_time_integration_bit_exact_vs_reference(name):
+    g = load(name)
+    sc = scenario(g)
+    o = capi.Oracle()
+    steps, out = run_pc_like_reference(o, sc, float(g["integration_time"]))
+    assert steps == int(g["out_itimestep"])
+    for k in ("x", "vx", "rho", "hsml", "A", "h"):
+        assert _same(getattr(sc, k), g["out_" + k]), k
+
+
+def test_boundary_friction_opposes_the_tangential_motion_and_pressure_off_is_no_boundary():
+    """Closed forms: Monaghan's friction term is -m_b kappa_t c (v . bdx) /
+    br / rho^2 dW -- for a particle at rest it vanishes, so with the
+    velocities zero the friction run equals the plain run bit for bit; with
+    the boundary pressure and the friction off the boundary particles are
+    never seen: the run equals one without them.  (The 2,500-particle box has
+    fewer boundary pairs than particle pairs: the C6 boundary kernel reads no
+    hsml_ij entry a fresh context leaves at zero.)"""
+    sc = S.box(nside=50, spacing=2000.0, seed=4)
+    sc.time = 2.0
+    ref = capi.Oracle().single_step(sc)
+    f = sc.copy()
+    f.boundary_friction = 1
+    assert _same(capi.Oracle().single_step(f)["dvxdt"], ref["dvxdt"])
+    sc.vx = np.random.default_rng(3).normal(0.0, 0.1, size=(sc.n, 2))
+    f.vx = sc.vx.copy()
+    assert not _same(capi.Oracle().single_step(f)["dvxdt"], capi.Oracle().single_step(sc)["dvxdt"])
+    off = sc.copy()
+    off.boundary_pressure = 0
+    nob = sc.copy()
+    nob.xb = np.zeros((0, 2))
+    nob.massb = np.zeros(0)
+    nob.bnormal = None
+    a, b = capi.Oracle().single_step(off), capi.Oracle().single_step(nob)
+    assert a["bnip"] == 0
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt"):
+        assert _same(a[k], b[k]), k
