@@ -41,7 +41,7 @@ extern "C" {
 #define SIMP_WIND_UNIFORM 0
 #define SIMP_WIND_GRID    1
 
-#define SIMP_CONFIG_VERSION 2
+#define SIMP_CONFIG_VERSION 3
 
 /* simp_instance_desc.sle value for the reference's sle = 0 (smoothing
  * lengths kept); 0 there means the default, Marquis_hsml (sle = 2). */
@@ -103,7 +103,7 @@ typedef struct simp_config {
                                   /* art_visc_alg viscosity (:29-35)            */
     /* the other logical scenario switches (globals.f90:178-187), 0 / 1;
      * simp_config_default gives the reference's values (friction 0, the
-     * others 1); viscosity stays on */
+     * others 1) */
     int32_t boundary_friction;    /* tangential boundary forces (boundary_    */
                                   /* force.f90:87-100, 157-166, 217-221)       */
     int32_t boundary_pressure;    /* normal boundary forces; without either    */
@@ -113,6 +113,9 @@ typedef struct simp_config {
                                   /* strength P (internal_stress.f90:35-39)    */
     int32_t wind_force;           /* wind_forcing.f90:35                       */
     int32_t water_force;          /* water_drag.f90:35                         */
+    int32_t viscosity;            /* 0 with pressure_force 0: the strain rate  */
+                                  /* is never set and stays zero (single_step. */
+                                  /* f90:80); with pressure_force 1 no effect  */
 } simp_config;
 
 typedef struct simp_handle simp_handle;
@@ -417,6 +420,7 @@ typedef struct simp_instance_desc {
 #define SIMP_OFF_PRESSURE_FORCE    2
 #define SIMP_OFF_WIND              4
 #define SIMP_OFF_WATER             8
+#define SIMP_OFF_VISCOSITY        16
 
 int simp_batch_create(int precision, simp_batch **out);
 int simp_batch_destroy(simp_batch *b);

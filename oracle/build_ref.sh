@@ -188,6 +188,10 @@ variant nobp 'boundary_pressure = .TRUE.' 'boundary_pressure = .FALSE.'
 variant noforce 'wind_force        = .TRUE.' 'wind_force        = .FALSE.' \
     'water_force       = .TRUE.' 'water_force       = .FALSE.' \
     'pressure_force    = .TRUE.' 'pressure_force    = .FALSE.'
+variant novisc 'viscosity         = .TRUE.' 'viscosity         = .FALSE.' \
+    'pressure_force    = .TRUE.' 'pressure_force    = .FALSE.'
+variant novisc1 'viscosity         = .TRUE.' 'viscosity         = .FALSE.' \
+    'parameter(eos_alg = 3)' 'parameter(eos_alg = 1)'
 
 # The OpenMP build, for the CPU baseline: the reference compiled as its
 # Makefile intends (-fopenmp) and sized for the 250,000-particle workload.
@@ -273,7 +277,7 @@ if [ -f "$LIB" ]; then
         eval "gpu_harness \"$OUT/k$k/obj\" \"$OUT/k$k/mod\" \"$OUT/simp_ref_harness_gpu_k$k\" \"\${kobjs_$k[@]}\""
     done
     for t in sle0 sle1 sle1eu sle1d2 xsph avhk avmon avtse1 bfa2 bfa3 fric fric1 fricg fricq \
-             fric4 fric5 bfa2f bfa3f nobp noforce; do
+             fric4 fric5 bfa2f bfa3f nobp noforce novisc novisc1; do
         eval "gpu_harness \"$OUT/$t/obj\" \"$OUT/$t/mod\" \"$OUT/simp_ref_harness_gpu_$t\" \"\${vobjs_$t[@]}\""
     done
 fi

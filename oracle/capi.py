@@ -29,7 +29,7 @@ class OrcParams(C.Structure):
                 ("b_normal", C.c_void_p), ("step_dt", C.c_double),
                 ("boundary_friction", C.c_int), ("boundary_pressure", C.c_int),
                 ("pressure_force", C.c_int), ("wind_force", C.c_int),
-                ("water_force", C.c_int)]
+                ("water_force", C.c_int), ("viscosity", C.c_int)]
 
 
 def build() -> None:
@@ -106,6 +106,7 @@ def make_params(sc) -> tuple:
     p.pressure_force = getattr(sc, "pressure_force", 1)
     p.wind_force = getattr(sc, "wind_force", 1)
     p.water_force = getattr(sc, "water_force", 1)
+    p.viscosity = getattr(sc, "viscosity", 1)
     p.step_dt = sc.dt
     bn = np.ascontiguousarray(sc.b_normal(), dtype=np.float64)
     p.b_normal = bn.ctypes.data if bn.size else None

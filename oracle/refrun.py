@@ -58,10 +58,11 @@ HARNESS_GPU_DEN2 = os.path.join(HERE, "_ref", "simp_ref_harness_gpu_den2")
 #: density), XSPH, the two artificial viscosities (Monaghan's also with the
 #: Monaghan time step), the Wang / Marquis boundary forces, boundary
 #: friction (with six kernels, with Wang's and Marquis' forces), the
-#: boundary pressure off, and wind, water drag and pressure off, by tag
+#: boundary pressure off, wind, water drag and pressure off, and viscosity
+#: off (with pressure off; with the Hibler eos), by tag
 VARIANT_TAGS = ("sle0", "sle1", "sle1eu", "sle1d2", "xsph", "avhk", "avmon", "avtse1",
                 "bfa2", "bfa3", "fric", "fric1", "fricg", "fricq", "fric4", "fric5", "bfa2f",
-                "bfa3f", "nobp", "noforce")
+                "bfa3f", "nobp", "noforce", "novisc", "novisc1")
 HARNESS_VARIANT = {t: os.path.join(HERE, "_ref", f"simp_ref_harness_{t}") for t in VARIANT_TAGS}
 HARNESS_GPU_VARIANT = {t: os.path.join(HERE, "_ref", f"simp_ref_harness_gpu_{t}")
                        for t in VARIANT_TAGS}

@@ -745,11 +745,15 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         grad[4 * i + 3] = grad[4 * i + 3] - mass[j] * w4 / rho[j];
         grad[4 * j + 3] = grad[4 * j + 3] - mass[i] * w4 / rho[i];
     }
-    /* strain rate (:80-84) */
+    /* strain rate (:80-84), set only IF (pressure_force .OR. viscosity);
+     * otherwise epsilon keeps its value, the zero initialize gives it
+     * (initialize.f90:61-63) -- the switches are PARAMETERs, so it never
+     * changes */
+    const int strain = prm->pressure_force || prm->viscosity;
     for (int i = 0; i < n; ++i) {
-        eps3[3 * i + 0] = grad[4 * i + 0];
-        eps3[3 * i + 2] = grad[4 * i + 3];
-        eps3[3 * i + 1] = 0.5 * (grad[4 * i + 1] + grad[4 * i + 2]);
+        eps3[3 * i + 0] = strain ? grad[4 * i + 0] : 0.0;
+        eps3[3 * i + 2] = strain ? grad[4 * i + 3] : 0.0;
+        eps3[3 * i + 1] = strain ? 0.5 * (grad[4 * i + 1] + grad[4 * i + 2]) : 0.0;
     }
     /* internal_stress (internal_stress.f90:21-53): Hibler P (:24),
      * elliptical_bulk_viscosity (bulk_viscosity.f90:65-76),
@@ -766,9 +770,19 @@ int orc_single_step(orc_ctx *c, const orc_params *prm, int n, const double *x,
         double z = 0.5 * P * (1 + K_T) / dcl;
         double et = z / (E_ELL * E_ELL);
         if (prm->rheology == 2) mc_viscosity(e11, e12, e22, P, &z, &et);
+        if (!prm->viscosity) {
+            /* IF (viscosity) (internal_stress.f90:20-31) is skipped: zeta and
+             * eta keep initialize's zero (initialize.f90:64-65), p its zero
+             * (:30) unless eos sets it */
+            z = 0.0;
+            et = 0.0;
+        }
         double pr = z * 2 * delta / (1 + K_T);               /* kreyscher_eos */
-        if (prm->eos_alg == 1 || !prm->pressure_force) {
-            /* hibler_eos, or no eos call at all: p keeps the ice strength */
+        if (!prm->pressure_force) {
+            /* no eos call: p keeps the ice strength the viscosity block set
+             * (internal_stress.f90:24), or zero without it */
+            pr = prm->viscosity ? P : 0.0;
+        } else if (prm->eos_alg == 1) {
             pr = P;                                           /* hibler_eos (eos.f90:59) */
         } else if (prm->eos_alg == 2) {
             /* gutfraind_eos (eos.f90:94-102): rho_max = 934, rho_0 = rho_ice,

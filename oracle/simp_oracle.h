@@ -78,6 +78,10 @@ typedef struct orc_params {
     int    pressure_force;     /* 0: eos is not called, p stays the ice
                                   strength P (internal_stress.f90:35-39) */
     int    wind_force, water_force;
+    int    viscosity;          /* 0: zeta, eta stay zero and p is set by eos
+                                  only (internal_stress.f90:20-39); with
+                                  pressure_force 0 also the strain rate is
+                                  never set (single_step.f90:80) */
 } orc_params;
 
 orc_ctx *orc_create(void);

@@ -23,12 +23,13 @@ SIMP_FP64 = 0
 SIMP_MIXED = 1
 SIMP_WIND_UNIFORM = 0
 SIMP_WIND_GRID = 1
-SIMP_CONFIG_VERSION = 2
+SIMP_CONFIG_VERSION = 3
 SIMP_SLE_KEEP = -1
 SIMP_OFF_BOUNDARY_PRESSURE = 1
 SIMP_OFF_PRESSURE_FORCE = 2
 SIMP_OFF_WIND = 4
 SIMP_OFF_WATER = 8
+SIMP_OFF_VISCOSITY = 16
 
 #: every entry point declared in include/simp.h
 EXPORTS = (
@@ -60,7 +61,8 @@ class SimpConfig(C.Structure):
                 ("artificial_visc", C.c_int32), ("art_visc_alg", C.c_int32),
                 ("xsph_regularization", C.c_int32), ("boundary_friction", C.c_int32),
                 ("boundary_pressure", C.c_int32), ("pressure_force", C.c_int32),
-                ("wind_force", C.c_int32), ("water_force", C.c_int32)]
+                ("wind_force", C.c_int32), ("water_force", C.c_int32),
+                ("viscosity", C.c_int32)]
 
 
 class SimpInstanceDesc(C.Structure):

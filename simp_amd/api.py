@@ -33,7 +33,7 @@ def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(
                 art_visc: int = 0, art_visc_alg: int = 1, bfa: int = 1,
                 boundary_friction: int = 0, boundary_pressure: int = 1,
                 pressure_force: int = 1, wind_force: int = 1,
-                water_force: int = 1) -> SimpConfig:
+                water_force: int = 1, viscosity: int = 1) -> SimpConfig:
     cfg = SimpConfig()
     lib().simp_config_default(C.byref(cfg))
     cfg.precision = precision
@@ -42,6 +42,7 @@ def make_config(precision: int = _lib.SIMP_FP64, tse: int = 2, wind=(0.0, float(
     cfg.pressure_force = pressure_force
     cfg.wind_force = wind_force
     cfg.water_force = water_force
+    cfg.viscosity = viscosity
     cfg.sle = sle
     cfg.xsph_regularization = xsph
     cfg.artificial_visc = art_visc
@@ -90,11 +91,13 @@ class Simp:
                  eos_alg: int = 3, rheology: int = 1, skf: int = 6, int_algorithm: int = 2,
                  sle: int = 2, xsph: int = 0, art_visc: int = 0, art_visc_alg: int = 1,
                  bfa: int = 1, boundary_friction: int = 0, boundary_pressure: int = 1,
-                 pressure_force: int = 1, wind_force: int = 1, water_force: int = 1):
+                 pressure_force: int = 1, wind_force: int = 1, water_force: int = 1,
+                 viscosity: int = 1):
         self.cfg = make_config(precision, tse, wind, current, max_hsml, wind_grid is not None,
                                den_alg, pa_sph, eos_alg, rheology, skf, int_algorithm, sle,
                                xsph, art_visc, art_visc_alg, bfa, boundary_friction,
-                               boundary_pressure, pressure_force, wind_force, water_force)
+                               boundary_pressure, pressure_force, wind_force, water_force,
+                               viscosity)
         h = C.c_void_p()
         check(lib().simp_create(C.byref(self.cfg), C.byref(h)), "simp_create")
         self._h = h
@@ -116,7 +119,8 @@ class Simp:
                    art_visc_alg=sc.art_visc_alg, bfa=sc.bfa,
                    boundary_friction=sc.boundary_friction,
                    boundary_pressure=sc.boundary_pressure, pressure_force=sc.pressure_force,
-                   wind_force=sc.wind_force, water_force=sc.water_force)
+                   wind_force=sc.wind_force, water_force=sc.water_force,
+                   viscosity=getattr(sc, "viscosity", 1))
 
     def set_module_state(self, sc) -> None:
         """dt and the boundary normals of ``sc``, which the Wang boundary

@@ -115,6 +115,7 @@ void simp_config_default(simp_config *c) {
   c->pressure_force = 1;
   c->wind_force = 1;
   c->water_force = 1;
+  c->viscosity = 1;
 }
 
 const char *simp_strerror(int s) {
@@ -227,6 +228,7 @@ static int fill_input(simp_handle *h, InstanceInput &in) {
   in.pressure_force = c.pressure_force ? 1 : 0;
   in.wind_force = c.wind_force ? 1 : 0;
   in.water_force = c.water_force ? 1 : 0;
+  in.viscosity = c.viscosity ? 1 : 0;
   drop_unused_boundary(in);
   if (c.wind_mode == SIMP_WIND_GRID) {
     if (h->g_nx < 2) return SIMP_E_STATE;
@@ -707,7 +709,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   if (d.art_visc_alg < 0 || d.art_visc_alg > 2 || d.bfa < 0 || d.bfa > 3)
     return SIMP_E_UNSUPPORTED;
   if (d.forces_off & ~(SIMP_OFF_BOUNDARY_PRESSURE | SIMP_OFF_PRESSURE_FORCE | SIMP_OFF_WIND |
-                       SIMP_OFF_WATER))
+                       SIMP_OFF_WATER | SIMP_OFF_VISCOSITY))
     return SIMP_E_UNSUPPORTED;
   if (d.wind_mode == SIMP_WIND_GRID &&
       (d.grid_nx < 2 || d.grid_ny < 2 || !d.grid_u || !d.grid_v || !(d.grid_dx > 0) ||
@@ -735,6 +737,7 @@ static int fill_from_desc(const simp_instance_desc &d, InstanceInput &in) {
   in.pressure_force = (d.forces_off & SIMP_OFF_PRESSURE_FORCE) ? 0 : 1;
   in.wind_force = (d.forces_off & SIMP_OFF_WIND) ? 0 : 1;
   in.water_force = (d.forces_off & SIMP_OFF_WATER) ? 0 : 1;
+  in.viscosity = (d.forces_off & SIMP_OFF_VISCOSITY) ? 0 : 1;
   drop_unused_boundary(in);
   in.wind_mode = d.wind_mode;
   in.wind[0] = d.wind[0]; in.wind[1] = d.wind[1];
@@ -864,6 +867,7 @@ int simp_batch_status(simp_batch *b, int id, int *itimestep, int *steps_left, in
     *status = SIMP_OK;
     if (err & simp::kErrCapacity) *status = SIMP_E_CAPACITY;
     if (err & simp::kErrNan) *status = SIMP_E_NAN;
+    if (err & simp::kErrSync) *status = SIMP_E_HIP;
   }
   return SIMP_OK;
 }
@@ -913,6 +917,7 @@ int simp_batch_list(simp_batch *b, int cap, int *ids, int *itimestep, int *steps
       status[k] = SIMP_OK;
       if (y.err & simp::kErrCapacity) status[k] = SIMP_E_CAPACITY;
       if (y.err & simp::kErrNan) status[k] = SIMP_E_NAN;
+      if (y.err & simp::kErrSync) status[k] = SIMP_E_HIP;
     }
   }
   return n;

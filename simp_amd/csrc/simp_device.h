@@ -60,6 +60,8 @@ constexpr int kParts = 9;
 constexpr int kErrCapacity = 1;
 constexpr int kErrNan = 2;
 constexpr int kErrBndNbr = 4;
+constexpr int kErrSync = 8;     // the rebuild kernel's grid barrier timed out
+constexpr int kBarWords = 8 * 32 + 2;   // its arrival counters and generation
 
 struct InstMeta {
   int p_off, n;          // particle range
@@ -87,6 +89,10 @@ struct InstMeta {
   int fric;              // 1: boundary friction (boundary_friction)
   int bpres;             // 1: boundary normal force (boundary_pressure)
   int wind_on, water_on; // wind_force, water_force (globals.f90:182-183)
+  int novisc;            // 1: viscosity off -- zeta, eta stay zero (internal_
+                         //    stress.f90:20-31); eos_alg 0 then: p stays zero
+  int nostrain;          // 1: viscosity and pressure_force off -- the strain rate
+                         //    is never set and stays zero (single_step.f90:80)
   int pad_;
   double g_x0, g_y0, g_dx, g_dy;
   double wind_u, wind_v, cur_u, cur_v;
@@ -170,6 +176,7 @@ struct DevState {
   // FP32 bits, rounded up
   unsigned long long *cellh;
   int *rebuild;          // 1: the cell lists are rebuilt in this evaluation
+  unsigned *bar;         // the rebuild kernel's grid barrier (kBarWords words)
   float4 *pf;            // FP32 (x, y, hsml) by slot, x and y relative to the
                          // grid origin: the candidate prefilter
   // neighbour list of the evaluation, built by pass A in the reference's

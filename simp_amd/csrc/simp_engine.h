@@ -35,7 +35,7 @@ struct InstanceInput {
   const double *bnormal = nullptr;   // (2, nb) boundary normals (Wang); null: zeros
   // the logical scenario switches of globals.f90:178-187 (without a
   // boundary force no boundary pairs exist: the boundary is dropped)
-  int boundary_friction = 0, boundary_pressure = 1, pressure_force = 1;
+  int boundary_friction = 0, boundary_pressure = 1, pressure_force = 1, viscosity = 1;
   int wind_force = 1, water_force = 1;
   // optional wind grid
   int g_nx = 0, g_ny = 0;

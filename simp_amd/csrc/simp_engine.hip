@@ -24,7 +24,7 @@ static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 static inline bool nondefault(const InstMeta &m) {
   return m.den_alg != 3 || m.pa_sph == 2 || m.eos_alg != 3 || m.rheology != 1 || m.skf != 6 ||
          m.tse == 1 || m.sle != 2 || m.xsph || m.avis || m.bfa != 1 || m.fric || !m.bpres ||
-         !m.wind_on || !m.water_on;
+         !m.wind_on || !m.water_on || m.novisc;
 }
 
 // The switch fields of an instance's record.
@@ -32,8 +32,9 @@ static void meta_switches(InstMeta &m, const InstanceInput &in) {
   m.den_alg = in.den_alg;
   m.pa_sph = in.pa_sph;
   // without the pressure force eos is not called and p stays the ice
-  // strength: the Hibler pressure (internal_stress.f90:21-39)
-  m.eos_alg = in.pressure_force ? in.eos_alg : 1;
+  // strength: the Hibler pressure (internal_stress.f90:21-39), or, with the
+  // viscosity off too, initialize's zero (eos_alg 0)
+  m.eos_alg = in.pressure_force ? in.eos_alg : (in.viscosity ? 1 : 0);
   m.rheology = in.rheology;
   m.skf = in.skf;
   m.sle = in.sle;
@@ -41,6 +42,8 @@ static void meta_switches(InstMeta &m, const InstanceInput &in) {
   m.bpres = in.boundary_pressure ? 1 : 0;
   m.wind_on = in.wind_force ? 1 : 0;
   m.water_on = in.water_force ? 1 : 0;
+  m.novisc = in.viscosity ? 0 : 1;
+  m.nostrain = (!in.viscosity && !in.pressure_force) ? 1 : 0;
   m.xsph = in.xsph ? 1 : 0;
   // regularization runs for XSPH or the artificial viscosity, and then
   // always with the viscosity art_visc_alg names (regularization.f90:29-35)
@@ -211,6 +214,7 @@ static int alloc_state(DevState &d, A &&alloc, int **scan_tmp, size_t N, size_t 
   rc |= dalloc(alloc, &d.cell_intra, C);
   rc |= dalloc(alloc, &d.cellh, C);
   rc |= dalloc(alloc, &d.rebuild, 1);
+  rc |= dalloc(alloc, &d.bar, kBarWords);
   double *sg = nullptr;
   rc |= dalloc(alloc, &sg, 2 * N); d.sig11 = sg;
   d.srec = (float4 *)sg;
@@ -447,6 +451,7 @@ int Engine::repack(const std::vector<int> &keep, const std::vector<InstanceInput
     // the first evaluation of a new layout builds the cell lists
     static const int one = 1;
     RCHK(hipMemcpyAsync(nd.rebuild, &one, sizeof(int), hipMemcpyHostToDevice, st_));
+    RCHK(hipMemsetAsync(nd.bar, 0, kBarWords * sizeof(unsigned), st_));
   }
 
   // kept instances: device to device, from the set holding each one (its
@@ -664,13 +669,14 @@ int Engine::reload(const InstanceInput &in, bool grid_same) {
   const InstMeta &o = meta_[0];
   if (in.n != o.n || in.nb != o.nb || in.tse != o.tse || in.wind_mode != o.wind_mode ||
       in.den_alg != o.den_alg || in.pa_sph != o.pa_sph ||
-      (in.pressure_force ? in.eos_alg : 1) != o.eos_alg ||
+      (in.pressure_force ? in.eos_alg : (in.viscosity ? 1 : 0)) != o.eos_alg ||
       in.rheology != o.rheology || in.skf != o.skf || in.sle != o.sle ||
       (in.xsph ? 1 : 0) != o.xsph || in.bfa != o.bfa ||
       ((in.xsph || in.art_visc) ? in.art_visc_alg : 0) != o.avis ||
       (in.art_visc || in.tse == 1 || in.boundary_friction ? 1 : 0) != o.needc ||
       (in.boundary_friction ? 1 : 0) != o.fric || (in.boundary_pressure ? 1 : 0) != o.bpres ||
-      (in.wind_force ? 1 : 0) != o.wind_on || (in.water_force ? 1 : 0) != o.water_on)
+      (in.wind_force ? 1 : 0) != o.wind_on || (in.water_force ? 1 : 0) != o.water_on ||
+      (in.viscosity ? 0 : 1) != o.novisc)
     return 1;
   // (the boundary normals of the Wang force are laid out with the boundary)
   if (in.bfa == 2) return 1;
@@ -746,6 +752,7 @@ int Engine::reload(const InstanceInput &in, bool grid_same) {
   }
   static const int one = 1;
   HIPCHK(h2d(d_.rebuild, &one, sizeof(int)));
+  HIPCHK(hipMemsetAsync(d_.bar, 0, kBarWords * sizeof(unsigned), st_));
   HIPCHK(h2d(d_.blk, blk_host_.data(), sizeof(BlockEntry) * blk_host_.size()));
   HIPCHK(h2d(d_.meta, &m, sizeof(m)));
   HIPCHK(h2d(d_.dyn, &y, sizeof(y)));
@@ -936,6 +943,7 @@ int Engine::single_in_memory(char *base, size_t bytes, long q_cap, long kz_cap,
   HIPCHK(zero(nd.osig, 0, n * 4));
   static const int one = 1;
   HIPCHK(h2d(nd.rebuild, &one, sizeof(int)));
+  HIPCHK(zero(nd.bar, 0, kBarWords * sizeof(unsigned)));
   // the state: (2, n) arrays through the staging area, the others directly
   double *sx = stage, *sv = stage + 2 * n, *sb = stage + 4 * n;
   HIPCHK(h2d(sx, in.x, 2 * n * 8));
@@ -1022,9 +1030,7 @@ void Engine::L(const char *name, F &&f) {
 void Engine::enqueue_prefix(int buf) {
   const DevState &d = d_;
   L("count", [&] { launch_count(d, buf, st_); });
-  L("scan", [&] { launch_rebuild_scan(d, scan_tmp_, st_); });
-  L("scatter", [&] { launch_scatter(d, st_); });
-  L("permute", [&] { launch_permute(d, buf, st_); });
+  L("rebuild", [&] { launch_rebuild(d, buf, scan_tmp_, st_); });
 }
 
 void Engine::enqueue_eval(int buf, int phase, bool want_diag) {
@@ -1169,12 +1175,14 @@ int Engine::sync_and_check() {
   if (d_.ninst == 1) {
     InstDyn y;
     HIPCHK(hipMemcpy(&y, d_.dyn, sizeof(y), hipMemcpyDeviceToHost));
+    if (y.err & kErrSync) return -2;
     return (y.err & kErrNan) ? -4 : (y.err ? -3 : 0);
   }
   std::vector<InstDyn> hd(d_.ninst);
   HIPCHK(hipMemcpy(hd.data(), d_.dyn, sizeof(InstDyn) * d_.ninst, hipMemcpyDeviceToHost));
   int worst = 0;
   for (auto &y : hd) {
+    if (y.err & kErrSync) return -2;
     if (y.err & kErrNan) return -4;
     if (y.err) worst = -3;
   }

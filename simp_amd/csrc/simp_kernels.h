@@ -16,10 +16,10 @@ void launch_count(const DevState &d, int buf, hipStream_t st);
 // gate: device flag; the scan does nothing while it reads 0 (nullptr: always runs)
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st);
-// rebuilds: slots and the starts of the cell lists (counts are current)
-void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st);
-void launch_scatter(const DevState &d, hipStream_t st);
-void launch_permute(const DevState &d, int buf, hipStream_t st);
+// the rebuild of the cell lists of the instances with a changed key (counts
+// are current): slots, cell starts, new order, state move -- one launch, a
+// flag read between rebuilds.  tmp: the scan's block sums
+void launch_rebuild(const DevState &d, int buf, int *tmp, hipStream_t st);
 // ba / bb: neighbours whose loads go out together in pass A / pass B (tuning
 // parameters; 0 or an untuned value: the default)
 void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag, int ba,

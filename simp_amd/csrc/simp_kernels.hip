@@ -297,7 +297,26 @@ __device__ __forceinline__ int lower_bound_key(const unsigned long long *keys, i
   return a;
 }
 
-__device__ void boundary_lists(const DevState &d, int k) {
+// boundary positions and keys a finalize thread loads before the grid is
+// known (the first kBPre * kTPB boundary particles of the instance)
+constexpr int kBPre = 4;
+struct BPre {
+  double x[kBPre], y[kBPre];
+  int key[kBPre];
+};
+
+__device__ __forceinline__ void boundary_prefetch(const DevState &d, const InstMeta &m, BPre &q) {
+#pragma unroll
+  for (int u = 0; u < kBPre; ++u) {
+    const int b = threadIdx.x + u * kTPB;
+    const int bg = m.b_off + (b < m.nb ? b : 0);
+    q.x[u] = d.xb[bg];
+    q.y[u] = d.yb[bg];
+    q.key[u] = d.bkey[bg];
+  }
+}
+
+__device__ void boundary_lists(const DevState &d, int k, const BPre &q) {
   __shared__ unsigned long long keys[kMaxBoundaryPerInst];
   __shared__ int alloc;
   const int kBprepThreads = (int)blockDim.x;
@@ -306,13 +325,22 @@ __device__ void boundary_lists(const DevState &d, int k) {
   const double bin = y.bin;
   const int W = y.W, xlo = y.xlo, ylo = y.ylo, stamp = y.stamp;
   int changed = 0;
-  for (int b = threadIdx.x; b < m.nb; b += kBprepThreads) {
-    const int bg = m.b_off + b;
-    const int cxb = nint(d.xb[bg] / bin) - xlo + 1, cyb = nint(d.yb[bg] / bin) - ylo + 1;
+  auto rekey = [&](int b, double xb, double yb, int old) {
+    const int cxb = nint(xb / bin) - xlo + 1, cyb = nint(yb / bin) - ylo + 1;
     const int key = cyb * W + cxb;
-    changed |= d.bkey[bg] != key;
-    d.bkey[bg] = key;
+    changed |= old != key;
+    d.bkey[m.b_off + b] = key;
     keys[b] = ((unsigned long long)(unsigned)key << 32) | (unsigned)b;
+  };
+  // (k_finalize runs kTPB threads: the prefetched particles first)
+#pragma unroll
+  for (int u = 0; u < kBPre; ++u) {
+    const int b = threadIdx.x + u * kTPB;
+    if (b < m.nb) rekey(b, q.x[u], q.y[u], q.key[u]);
+  }
+  for (int b = threadIdx.x + kBPre * kTPB; b < m.nb; b += kTPB) {
+    const int bg = m.b_off + b;
+    rekey(b, d.xb[bg], d.yb[bg], d.bkey[bg]);
   }
   if (!__syncthreads_or(changed)) return;
   int P = 1;
@@ -396,8 +424,9 @@ __global__ void __launch_bounds__(kTPB) k_partial(DevState d, int buf) {
   block_partials(d, k, ok, xv, yv, hv, 0.0, 0.0);
 }
 
-__device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
-                             double ymn, double ymx, double hmn, double mforce, double mcfl);
+__device__ int finalize_grid(const InstMeta &m, InstDyn &y, int end_of_step, double xmn,
+                             double xmx, double ymn, double ymx, double hmn, double mforce,
+                             double mcfl);
 
 __device__ __forceinline__ void stats_reset(InstDyn &y) {
   y.ipstep = y.itimestep;
@@ -485,28 +514,49 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
   if (blockIdx.x >= (unsigned)d.ninst) return;
   const int k = blockIdx.x;
   const InstMeta &m = d.meta[k];
-  if (threadIdx.x == 0) {
-    fold_set(d.dyn[k]);
-    if (!inst_off(d.dyn[k])) lists_built(d.dyn[k]);
-  }
+  // thread 0 works on a copy of the instance record, loaded with the
+  // partials (one round of loads, not one dependent round trip per field
+  // it reads) and stored back whole once the grid is set
+  InstDyn y;
+  if (threadIdx.x == 0) y = d.dyn[k];
 
+  // the boundary particles' positions and keys go out first: the boundary
+  // lists below need them once the grid is known
+  BPre bq;
+  if (m.nb > 0) boundary_prefetch(d, m, bq);
   const double big = __builtin_huge_val();
   double v0 = big, v1 = -big, v2 = big, v3 = -big, v4 = big, v5 = 0.0, v6 = 0.0;
   double v7 = big, v8 = big;
   const size_t nb = (size_t)d.nblk;
   // the Monaghan time step's minima exist at the end of such a step only
   const bool mdt = end_of_step && m.tse == 1;
-  for (int b = m.blk_off + threadIdx.x; b < m.blk_off + m.nblk; b += kTPB) {
-    v0 = fmin(v0, d.part[b]);
-    v1 = fmax(v1, d.part[nb + b]);
-    v2 = fmin(v2, d.part[2 * nb + b]);
-    v3 = fmax(v3, d.part[3 * nb + b]);
-    v4 = fmin(v4, d.part[4 * nb + b]);
-    v5 = fmax(v5, d.part[5 * nb + b]);
-    v6 = fmax(v6, d.part[6 * nb + b]);
+  // kFin block partials per thread and round: their loads go out together
+  constexpr int kFin = 4;
+  const int bend = m.blk_off + m.nblk;
+  for (int b0 = m.blk_off + threadIdx.x; b0 < bend; b0 += kFin * kTPB) {
+    double a[kFin][7];
+#pragma unroll
+    for (int u = 0; u < kFin; ++u) {
+      const int b = b0 + u * kTPB < bend ? b0 + u * kTPB : b0;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) a[u][j] = d.part[j * nb + b];
+    }
+#pragma unroll
+    for (int u = 0; u < kFin; ++u) {
+      // (a repeated partial leaves a min / max unchanged)
+      v0 = fmin(v0, a[u][0]);
+      v1 = fmax(v1, a[u][1]);
+      v2 = fmin(v2, a[u][2]);
+      v3 = fmax(v3, a[u][3]);
+      v4 = fmin(v4, a[u][4]);
+      v5 = fmax(v5, a[u][5]);
+      v6 = fmax(v6, a[u][6]);
+    }
     if (mdt) {
-      v7 = fmin(v7, d.part[7 * nb + b]);
-      v8 = fmin(v8, d.part[8 * nb + b]);
+      for (int b = b0; b < bend && b < b0 + kFin * kTPB; b += kTPB) {
+        v7 = fmin(v7, d.part[7 * nb + b]);
+        v8 = fmin(v8, d.part[8 * nb + b]);
+      }
     }
   }
   v0 = wave_min(v0); v1 = wave_max(v1); v2 = wave_min(v2); v3 = wave_max(v3);
@@ -531,21 +581,23 @@ __global__ void __launch_bounds__(kTPB) k_finalize(DevState d, int end_of_step) 
       v7 = fmin(sh[7][0], sh[7][i]); sh[7][0] = v7;
       v8 = fmin(sh[8][0], sh[8][i]); sh[8][0] = v8;
     }
-    move_bound(d.dyn[k], sh[5][0], sh[6][0]);
-    go = finalize_grid(d, k, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0],
+    fold_set(y);
+    if (!inst_off(y)) lists_built(y);
+    move_bound(y, sh[5][0], sh[6][0]);
+    go = finalize_grid(m, y, end_of_step, sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0],
                        sh[7][0], sh[8][0]);
+    d.dyn[k] = y;
   }
   __syncthreads();
   // the boundary particles' cell lists for the next evaluation
-  if (go && m.nb > 0) boundary_lists(d, k);
+  if (go && m.nb > 0) boundary_lists(d, k, bq);
 }
 
 // Thread 0 of k_finalize: close the step, then the grid of the next
 // evaluation.  Returns whether the instance evaluates next.
-__device__ int finalize_grid(const DevState &d, int k, int end_of_step, double xmn, double xmx,
-                             double ymn, double ymx, double hmn, double mforce, double mcfl) {
-  const InstMeta &m = d.meta[k];
-  InstDyn &y = d.dyn[k];
+__device__ int finalize_grid(const InstMeta &m, InstDyn &y, int end_of_step, double xmn,
+                             double xmx, double ymn, double ymx, double hmn, double mforce,
+                             double mcfl) {
   if (inst_off(y)) return 0;
   if (end_of_step) {
     // VP_time_step: MINVAL(rho_ice hsml^2 delta_min / (P_star (1 + k_t))) is
@@ -683,24 +735,6 @@ __global__ void __launch_bounds__(kTPB) k_count(DevState d, int buf) {
   prefilter_inputs(d, valid && lists_stale(d.dyn[k]), k, p, key, xq, yq, hq);
 }
 
-// Slot of each particle of a rebuilt instance in its new cell: consecutive
-// slots mostly share a cell, one atomic per run.  fill is zero again after
-// the cell sort.
-__device__ void hist_block(const DevState &d) {
-  if (blockIdx.x >= (unsigned)d.nblk) return;
-  if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
-  int p = 0, k = 0;
-  const int lane = (int)threadIdx.x & 63;
-  int key = -1 - lane, coff = 0;
-  if (particle_of(d, p, k)) {
-    const InstDyn &y = d.dyn[k];
-    coff = d.meta[k].c_off;
-    key = d.cellid[y.sel][p];
-  }
-  const int sl = run_add(d.fill + coff, key, 1);
-  if (key >= 0) d.slot[p] = sl;
-}
-
 // ------------------------------------------------------------------- scan
 
 __global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum, int nblocks,
@@ -716,23 +750,6 @@ __global__ void __launch_bounds__(kTPB) k_scan1(const int *in, int L, int *bsum,
   }
   s = block_sum(s);
   if (threadIdx.x == 0) bsum[blockIdx.x] = s;
-}
-
-// First launch of a rebuild: the slots (k_hist's work) and the block sums of
-// the cell counts (k_scan1's), by the same blocks.
-__global__ void __launch_bounds__(kTPB) k_hist_sum(DevState d, int *bsum, int nsb) {
-  if (*d.rebuild == 0) return;
-  if (blockIdx.x < (unsigned)nsb) {
-    constexpr int IPT = kScanItems / kTPB;
-    const int base = blockIdx.x * kScanItems + threadIdx.x * IPT;
-    int sum = 0;
-#pragma unroll
-    for (int i = 0; i < IPT; ++i)
-      if (base + i < d.ctot) sum += d.count[base + i];
-    sum = block_sum(sum);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = sum;
-  }
-  hist_block(d);
 }
 
 // out[i] = sum(in[0..i-1]); out[L] = total.  Optionally clears in[].
@@ -766,32 +783,106 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
   if (blockIdx.x == (unsigned)nblocks - 1 && threadIdx.x == kTPB - 1) out[L] = ex;
 }
 
-// ------------------------------------------------------- scatter/cellsort
+// --------------------------------------------------------------- rebuild
+//
+// The rebuild of the cell lists is one kernel of kRebuildBlocks blocks
+// (four per CU: every block of it is resident at once) whose blocks stride
+// over the block table.  Between rebuilds -- most
+// evaluations -- it reads one flag and returns: one launch instead of the
+// four a rebuild needs.  A rebuild runs four phases separated by grid
+// barriers: (1) slots in the new cells and the block sums of the cell
+// counts, (2) the cell starts, (3) the new order, (4) the state move.
+//
+// The data crossing a barrier (block sums, slots, cell starts, the new
+// order) is written and read with agent-scope atomic stores and loads,
+// which reach memory past the XCDs' L2 caches; a barrier waits until each
+// block's stores have completed before the block arrives.  No L2 writeback
+// or invalidate is needed (an agent-scope fence per barrier costs ~30 us
+// on MI355X, this barrier a few).
 
-// cl[new slot] = old slot
-__global__ void __launch_bounds__(kTPB) k_scatter(DevState d) {
-  if (*d.rebuild == 0) return;
-  int p = 0, k = 0;
-  if (!particle_of(d, p, k)) return;
-  const InstDyn &y = d.dyn[k];
-  if (y.rb == 0) return;
-  d.cl[d.cell_start[d.meta[k].c_off + d.cellid[y.sel][p]] + d.slot[p]] = p;
+constexpr int kRebuildBlocks = 1024;
+constexpr int kBarSpin = 1 << 18;   // bound of a barrier's wait (polls)
+// barrier words: kBarGroups arrival counters (one 128-byte line each), then
+// the counter of the groups arrived and the barrier generation
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 32;
+static_assert(kBarWords == kBarGroups * kBarStride + 2, "barrier words");
+
+__device__ __forceinline__ int ld_ag(const int *p) {
+  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The member of cell c (scattered into the cell's list range [s0, s0 + n)
-// in arbitrary order by k_scatter) whose particle id ranks r among the
-// cell's members: the cell's members in ascending particle id are the order
-// in which the reference fills its bins (nnps.f90:246-254).  Cells of up to
-// kRankMax members (the common case) are ranked in registers.
+// Grid barrier in two levels: the blocks of group blockIdx % kBarGroups
+// arrive on their group's counter, the last of each group on the top
+// counter, and the last group advances the generation (every counter is
+// back to zero by then).  Returns false when the wait ran past its bound
+// (the kernel then flags every instance).
+__device__ bool grid_sync(unsigned *bar) {
+  __shared__ int ok;
+  __builtin_amdgcn_s_waitcnt(0);     // this thread's stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned grp = blockIdx.x % kBarGroups;
+    const unsigned gsize = (gridDim.x + kBarGroups - 1 - grp) / kBarGroups;
+    unsigned *count = bar + grp * kBarStride;
+    unsigned *top = bar + kBarGroups * kBarStride, *gen = top + 1;
+    const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);   // gen read before arriving
+    bool last = false;
+    if (__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+        gsize - 1) {
+      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      const unsigned ng = min((unsigned)kBarGroups, gridDim.x);
+      last = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             ng - 1;
+    }
+    int good = 1;
+    if (last) {
+      __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int it = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++it > kBarSpin) {
+          good = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__device__ __forceinline__ bool particle_at(const DevState &d, int vb, int &p, int &k) {
+  const BlockEntry e = d.blk[vb];
+  k = e.inst;
+  if ((int)threadIdx.x >= e.nloc) return false;
+  p = e.pos + (int)threadIdx.x;
+  return true;
+}
+
+// The member of cell c (listed in cl's range [s0, s0 + n) in arbitrary
+// order) whose particle id ranks r among the cell's members: the cell's
+// members in ascending particle id are the order in which the reference
+// fills its bins (nnps.f90:246-254).  Cells of up to kRankMax members (the
+// common case) are ranked in registers.
 constexpr int kRankMax = 16;
 
 __device__ __forceinline__ int cell_member_at(const int *cl, const int *orig, int s0, int n,
                                               int r) {
-  if (n == 1) return cl[s0];
+  if (n == 1) return ld_ag(cl + s0);
   if (n <= kRankMax) {
     int v[kRankMax], key[kRankMax];
 #pragma unroll
-    for (int i = 0; i < kRankMax; ++i) v[i] = i < n ? cl[s0 + i] : 0;
+    for (int i = 0; i < kRankMax; ++i) v[i] = i < n ? ld_ag(cl + s0 + i) : 0;
 #pragma unroll
     for (int i = 0; i < kRankMax; ++i) key[i] = i < n ? orig[v[i]] : 0x7fffffff;
     int q = v[0];
@@ -805,56 +896,134 @@ __device__ __forceinline__ int cell_member_at(const int *cl, const int *orig, in
     return q;
   }
   for (int i = 0; i < n; ++i) {
-    const int vi = cl[s0 + i], ki = orig[vi];
+    const int vi = ld_ag(cl + s0 + i), ki = orig[vi];
     int rk = 0;
-    for (int j = 0; j < n; ++j) rk += orig[cl[s0 + j]] < ki;
+    for (int j = 0; j < n; ++j) rk += orig[ld_ag(cl + s0 + j)] < ki;
     if (rk == r) return vi;
   }
-  return cl[s0];
+  return ld_ag(cl + s0);
 }
 
-// The state of a rebuilt instance moves to the new slot order, from its set
-// sel to set sel ^ 1: each slot takes the member of its cell that ranks there
-// by particle id (its old slot), and writes the prefilter positions of the
-// evaluation (buffer buf).  The half-step buffer moves only when it is live
-// (the corrector's evaluation).  The first slot of each cell clears the
-// cell's fill counter for the next rebuild.
-__global__ void __launch_bounds__(kTPB) k_permute(DevState d, int buf) {
-  if (*d.rebuild == 0 || blockIdx.x >= (unsigned)d.nblk) return;
-  if (d.dyn[d.blk[blockIdx.x].inst].rb == 0) return;   // one instance per block
-  if (threadIdx.x == 0) d.blk[blockIdx.x].cs = d.dyn[d.blk[blockIdx.x].inst].sel ^ 1;
-  int s = 0, k = 0;
-  const bool act = particle_of(d, s, k);
-  int key = 0;
-  double xq = 0., yq = 0., hq = 0.;
-  if (act) {
-    const InstDyn &y = d.dyn[k];
-    const int f = y.sel, t = y.sel ^ 1;
-    const int coff = d.meta[k].c_off;
-    // the cell of slot s: that of any particle scattered into its range
-    const int c = d.cellid[f][d.cl[s]];
-    const int s0 = d.cell_start[coff + c], s1 = d.cell_start[coff + c + 1];
-    if (s == s0) d.fill[coff + c] = 0;
-    const int q = cell_member_at(d.cl, d.orig[f], s0, s1 - s0, s - s0);
-    for (int b = 0; b <= buf; ++b) {
-      const int i = 2 * f + b, o = 2 * t + b;
-      d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
-      d.vx[o][s] = d.vx[i][q]; d.vy[o][s] = d.vy[i][q];
-      d.h[o][s] = d.h[i][q]; d.A[o][s] = d.A[i][q];
-      d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
+// bsum: nsb block sums of the cell counts (kScanItems cells each)
+__global__ void __launch_bounds__(kTPB) k_rebuild(DevState d, int *bsum, int nsb, int buf) {
+  if (*d.rebuild == 0) return;       // the same for every block: no rebuild
+  const int G = (int)gridDim.x;
+  constexpr int IPT = kScanItems / kTPB;
+  const int L = d.ctot;
+  bool ok = true;
+
+  // (1) each particle's slot in its new cell (consecutive slots mostly share
+  //     a cell: one atomic per run; fill is zero again after the state
+  //     move), and the block sums of the cell counts
+  for (int vb = blockIdx.x; vb < max(nsb, d.nblk); vb += G) {
+    if (vb < nsb) {
+      const int base = vb * kScanItems + threadIdx.x * IPT;
+      int sum = 0;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+        if (base + i < L) sum += d.count[base + i];
+      sum = block_sum(sum);
+      if (threadIdx.x == 0) st_ag(bsum + vb, sum);
     }
-    d.mass[t][s] = d.mass[f][q];
-    if (buf == 1 && d.masst[0]) d.masst[t][s] = d.masst[f][q];
-    d.orig[t][s] = d.orig[f][q];
-    key = c;
-    d.cellid[t][s] = key;
-    const int i = 2 * f + buf;
-    xq = d.x[i][q];
-    yq = d.y[i][q];
-    hq = d.hsml[i][q];
+    if (vb < d.nblk && d.dyn[d.blk[vb].inst].rb != 0) {   // one instance per block
+      int p = 0, k = 0;
+      const int lane = (int)threadIdx.x & 63;
+      int key = -1 - lane, coff = 0;
+      if (particle_at(d, vb, p, k)) {
+        coff = d.meta[k].c_off;
+        key = d.cellid[d.dyn[k].sel][p];
+      }
+      const int sl = run_add(d.fill + coff, key, 1);
+      if (key >= 0) st_ag(d.slot + p, sl);
+    }
   }
-  // pass A prefilters after a rebuild: its inputs in the new order
-  prefilter_inputs(d, act, k, s, key, xq, yq, hq);
+  ok = grid_sync(d.bar) && ok;
+
+  // (2) cell starts: cell_start[i] = counts before cell i, cell_start[L]
+  //     the total
+  for (int vb = blockIdx.x; vb < nsb; vb += G) {
+    int acc = 0;
+    for (int j = threadIdx.x; j < vb; j += kTPB) acc += ld_ag(bsum + j);
+    const int pre = block_sum(acc);
+    const int base = vb * kScanItems + threadIdx.x * IPT;
+    int v[IPT];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      v[i] = base + i < L ? d.count[base + i] : 0;
+      s += v[i];
+    }
+    int tot = 0;
+    int ex = block_exclusive_scan(s, &tot) + pre;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      if (base + i < L) st_ag(d.cell_start + base + i, ex);
+      ex += v[i];
+    }
+    if (vb == nsb - 1 && threadIdx.x == kTPB - 1) st_ag(d.cell_start + L, ex);
+  }
+  ok = grid_sync(d.bar) && ok;
+
+  // (3) the new order: cl[new slot] = old slot
+  for (int vb = blockIdx.x; vb < d.nblk; vb += G) {
+    int p = 0, k = 0;
+    if (!particle_at(d, vb, p, k)) continue;
+    const InstDyn &y = d.dyn[k];
+    if (y.rb == 0) continue;
+    const int c = d.meta[k].c_off + d.cellid[y.sel][p];
+    st_ag(d.cl + ld_ag(d.cell_start + c) + ld_ag(d.slot + p), p);
+  }
+  ok = grid_sync(d.bar) && ok;
+  if (!ok) {
+    // a barrier's wait ran out: the rebuilt lists cannot be trusted
+    for (int k = blockIdx.x * kTPB + threadIdx.x; k < d.ninst; k += G * kTPB)
+      atomicOr(&d.dyn[k].err, kErrSync);
+    return;
+  }
+
+  // (4) the state of a rebuilt instance moves to the new slot order, from
+  //     set sel to set sel ^ 1: each slot takes the member of its cell that
+  //     ranks there by particle id (its old slot), and writes the prefilter
+  //     positions of the evaluation (buffer buf).  The half-step buffer
+  //     moves only when it is live (the corrector's evaluation).  The first
+  //     slot of each cell clears the cell's fill counter for the next
+  //     rebuild.
+  for (int vb = blockIdx.x; vb < d.nblk; vb += G) {
+    if (d.dyn[d.blk[vb].inst].rb == 0) continue;   // one instance per block
+    if (threadIdx.x == 0) d.blk[vb].cs = d.dyn[d.blk[vb].inst].sel ^ 1;
+    int s = 0, k = 0;
+    const bool act = particle_at(d, vb, s, k);
+    int key = 0;
+    double xq = 0., yq = 0., hq = 0.;
+    if (act) {
+      const InstDyn &y = d.dyn[k];
+      const int f = y.sel, t = y.sel ^ 1;
+      const int coff = d.meta[k].c_off;
+      // the cell of slot s: that of any particle placed in its range
+      const int c = d.cellid[f][ld_ag(d.cl + s)];
+      const int s0 = ld_ag(d.cell_start + coff + c), s1 = ld_ag(d.cell_start + coff + c + 1);
+      if (s == s0) d.fill[coff + c] = 0;
+      const int q = cell_member_at(d.cl, d.orig[f], s0, s1 - s0, s - s0);
+      for (int b = 0; b <= buf; ++b) {
+        const int i = 2 * f + b, o = 2 * t + b;
+        d.x[o][s] = d.x[i][q]; d.y[o][s] = d.y[i][q];
+        d.vx[o][s] = d.vx[i][q]; d.vy[o][s] = d.vy[i][q];
+        d.h[o][s] = d.h[i][q]; d.A[o][s] = d.A[i][q];
+        d.rho[o][s] = d.rho[i][q]; d.hsml[o][s] = d.hsml[i][q];
+      }
+      d.mass[t][s] = d.mass[f][q];
+      if (buf == 1 && d.masst[0]) d.masst[t][s] = d.masst[f][q];
+      d.orig[t][s] = d.orig[f][q];
+      key = c;
+      d.cellid[t][s] = key;
+      const int i = 2 * f + buf;
+      xq = d.x[i][q];
+      yq = d.y[i][q];
+      hq = d.hsml[i][q];
+    }
+    // pass A prefilters after a rebuild: its inputs in the new order
+    prefilter_inputs(d, act, k, s, key, xq, yq, hq);
+  }
 }
 
 // -------------------------------------------------------------- boundary
@@ -1668,8 +1837,11 @@ k_passA(DevState d, int buf, int want_diag) {
     atomicOr(&d.dyn[k].rchg, 1);
   d.own_cnt[self] = oc_new;
 
-  // strain rate (single_step.f90:80-84)
-  const double e11 = g1, e22 = g4, e12 = 0.5 * (g2 + g3);
+  // strain rate (single_step.f90:80-84); with viscosity and the pressure
+  // force off it is never set and keeps initialize's zero
+  const bool strain = !(XSW && m.nostrain);
+  const double e11 = strain ? g1 : 0.0, e22 = strain ? g4 : 0.0;
+  const double e12 = strain ? 0.5 * (g2 + g3) : 0.0;
   // Hibler P (internal_stress.f90:24), elliptical zeta / eta
   // (bulk_viscosity.f90:65-76, shear_viscosity.f90:57), Kreyscher
   // replacement pressure (eos.f90:141-147), stress tensor
@@ -1686,9 +1858,17 @@ k_passA(DevState d, int buf, int want_diag) {
   double z = 0.5 * P * (1 + kKT) / dcl;
   double et = z / (kE * kE);
   if (XSW && m.rheology == 2) mc_viscosity(e11, e12, e22, P, z, et);
+  if (XSW && m.novisc) {
+    // the viscosity block (internal_stress.f90:20-31) is skipped: zeta and
+    // eta keep initialize's zero
+    z = 0.0;
+    et = 0.0;
+  }
   double pr;
   if (eos == 1) {
     pr = P;                                    // hibler_eos (eos.f90:59)
+  } else if (eos == 0) {
+    pr = 0.0;                                  // neither eos nor the viscosity block
   } else if (eos == 2) {
     // gutfraind_eos (eos.f90:94-102), rho_0 = rho_ice, negative
     // pressures set to zero
@@ -3270,10 +3450,9 @@ void launch_fold(const DevState &d, hipStream_t st) {
 void launch_count(const DevState &d, int buf, hipStream_t st) {
   k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
-void launch_rebuild_scan(const DevState &d, int *tmp, hipStream_t st) {
+void launch_rebuild(const DevState &d, int buf, int *tmp, hipStream_t st) {
   const int nb = std::max(cdiv(d.ctot, kScanItems), 1);
-  k_hist_sum<<<g8(std::max(nb, d.nblk)), kTPB, 0, st>>>(d, tmp, nb);
-  k_scan2<<<g8(nb), kTPB, 0, st>>>(d.count, d.ctot, tmp, d.cell_start, 0, nb, d.rebuild);
+  k_rebuild<<<kRebuildBlocks, kTPB, 0, st>>>(d, tmp, nb, buf);
 }
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st) {
@@ -3281,12 +3460,6 @@ void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gat
   if (nb < 1) nb = 1;
   k_scan1<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, nb, gate);
   k_scan2<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, out, zero_in, nb, gate);
-}
-void launch_scatter(const DevState &d, hipStream_t st) {
-  k_scatter<<<g8(d.nblk), kTPB, 0, st>>>(d);
-}
-void launch_permute(const DevState &d, int buf, hipStream_t st) {
-  k_permute<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
 }
 // The tuned variants exist for the reference's default switch values; the
 // variants for other values run the default batch sizes.

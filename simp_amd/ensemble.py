@@ -47,7 +47,8 @@ def _desc(sc, nsteps: int, keep: list, itimestep: int = 0) -> SimpInstanceDesc:
     off = 0
     for flag, name in ((_lib.SIMP_OFF_BOUNDARY_PRESSURE, "boundary_pressure"),
                        (_lib.SIMP_OFF_PRESSURE_FORCE, "pressure_force"),
-                       (_lib.SIMP_OFF_WIND, "wind_force"), (_lib.SIMP_OFF_WATER, "water_force")):
+                       (_lib.SIMP_OFF_WIND, "wind_force"), (_lib.SIMP_OFF_WATER, "water_force"),
+                       (_lib.SIMP_OFF_VISCOSITY, "viscosity")):
         if not getattr(sc, name, 1):
             off |= flag
     d.forces_off = off

@@ -22,7 +22,7 @@ module simp_binding
         integer(c_int32_t) :: artificial_visc, art_visc_alg,               &
                               xsph_regularization, boundary_friction,      &
                               boundary_pressure, pressure_force,           &
-                              wind_force, water_force
+                              wind_force, water_force, viscosity
     end type simp_config
 
     interface

@@ -30,12 +30,6 @@ subroutine single_step
     integer(c_int) :: st, nipc, bnipc
 
     if (.not. c_associated(handle)) then
-        if (.not. viscosity) then
-            ! internal_stress would leave p, zeta and eta as earlier calls
-            ! left them (internal_stress.f90:21-31): not built
-            write(*,*) 'single_step (GPU): viscosity = .FALSE. is not supported'
-            stop
-        endif
         call simp_config_default(cfg)
         cfg%max_hsml = max_hsml
         cfg%den_alg = den_alg
@@ -55,6 +49,9 @@ subroutine single_step
         cfg%pressure_force = merge(1, 0, pressure_force)
         cfg%wind_force = merge(1, 0, wind_force)
         cfg%water_force = merge(1, 0, water_force)
+        ! without viscosity zeta and eta stay at initialize's zero, as the
+        ! device path returns them (internal_stress.f90:20-31)
+        cfg%viscosity = merge(1, 0, viscosity)
         st = simp_create(cfg, handle)
         call simp_check(st, 'simp_create')
     endif

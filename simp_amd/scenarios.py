@@ -75,6 +75,7 @@ class Scenario:
     boundary_friction: int = 0
     boundary_pressure: int = 1
     pressure_force: int = 1
+    viscosity: int = 1       # 0 (with pressure_force 0): no strain rate
     wind_force: int = 1
     water_force: int = 1
     #: inward unit normal of each boundary particle (nb, 2), as

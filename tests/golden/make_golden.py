@@ -353,10 +353,23 @@ def friction():
     _pc10_edge("noforce", H["noforce"], wind_force=0, water_force=0, pressure_force=0)
 
 
+def novisc():
+    """13. viscosity off: zeta and eta stay zero, p is what eos sets -- with
+    the replacement pressure off too p stays zero and the strain rate is
+    never set (single_step.f90:80); with the Hibler eos p is the ice
+    strength -- a single step of the closed box and steps of the ice edge
+    from the harness built with those PARAMETERs each."""
+    H = refrun.HARNESS_VARIANT
+    _single_box("novisc", H["novisc"], viscosity=0, pressure_force=0)
+    _pc10_edge("novisc", H["novisc"], viscosity=0, pressure_force=0)
+    _single_box("novisc1", H["novisc1"], viscosity=0, eos_alg=1)
+    _pc10_edge("novisc1", H["novisc1"], viscosity=0, eos_alg=1)
+
+
 if __name__ == "__main__":
     # no argument: every fixture; otherwise the named groups ("main", "den1",
-    # "pa2", "eos", "switches")
-    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches", "regularization",
+    # "pa2", "eos", "switches", "regularization", "novisc", "friction")
+    todo = sys.argv[1:] or ["main", "den1", "pa2", "eos", "switches", "regularization", "novisc",
                             "friction"]
     if "main" in todo:
         main()
@@ -373,3 +386,5 @@ if __name__ == "__main__":
         regularization()
     if "friction" in todo:
         friction()
+    if "novisc" in todo:
+        novisc()

@@ -34,7 +34,7 @@ def scenario(g, prefix="in_") -> Scenario:
     sc.art_visc_alg = int(g.get("art_visc_alg", 1))
     sc.bfa = int(g.get("bfa", 1))
     for k, v in (("boundary_friction", 0), ("boundary_pressure", 1), ("pressure_force", 1),
-                 ("wind_force", 1), ("water_force", 1)):
+                 ("wind_force", 1), ("water_force", 1), ("viscosity", 1)):
         setattr(sc, k, int(g.get(k, v)))
     if prefix + "bnormal" in g:
         sc.bnormal = g[prefix + "bnormal"].reshape(-1, 2).copy()

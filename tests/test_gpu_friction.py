@@ -1,7 +1,7 @@
 """Boundary friction (bf_Monaghan's tangential term with the gradients of
 six boundary kernels, Wang's friction cap, Marquis' braking with the
-boundary pressure off), the boundary pressure off, and wind, water drag and
-the replacement pressure off, on the device through the C ABI: bit for bit
+boundary pressure off), the boundary pressure off, wind, water drag and
+the replacement pressure off, and the viscosity off, on the device through the C ABI: bit for bit
 against the reference built with those PARAMETERs for one evaluation,
 within 1e-12 over ten steps, within the mixed path's tolerance in mixed
 precision, and per instance in a packed ensemble (alone equals mixed)."""
@@ -35,7 +35,7 @@ def nrel(a, b):
 
 
 @pytest.mark.parametrize("tag", ["fric", "fric1", "fricg", "fricq", "fric4", "fric5", "bfa2f",
-                                 "bfa3f", "nobp", "noforce"])
+                                 "bfa3f", "nobp", "noforce", "novisc", "novisc1"])
 def test_friction_single_step_fp64_vs_reference_golden(tag):
     say(f"single_step fp64 {tag} golden box2500")
     g = load(f"box2500_single_step_{tag}.npz")
@@ -55,7 +55,8 @@ def test_friction_single_step_fp64_vs_reference_golden(tag):
 
 @pytest.mark.parametrize("name", ["edge3600_pc10_fric.npz", "edge3600_pc2_bfa2f.npz",
                                   "edge3600_pc10_bfa3f.npz", "edge3600_pc10_nobp.npz",
-                                  "edge3600_pc10_noforce.npz"])
+                                  "edge3600_pc10_noforce.npz", "edge3600_pc10_novisc.npz",
+                                  "edge3600_pc10_novisc1.npz"])
 def test_friction_time_integration_fp64_vs_reference_golden(name):
     say(f"time_integration fp64 {name}")
     g = load(name)
@@ -72,7 +73,8 @@ def test_friction_time_integration_fp64_vs_reference_golden(name):
 
 @pytest.mark.parametrize("sw", [{"boundary_friction": 1},
                                 {"boundary_friction": 1, "bfa": 3, "boundary_pressure": 0},
-                                {"wind_force": 0, "water_force": 0, "pressure_force": 0}])
+                                {"wind_force": 0, "water_force": 0, "pressure_force": 0},
+                                {"viscosity": 0, "eos_alg": 1}])
 def test_friction_mixed_precision_within_tolerance(sw):
     say(f"time_integration mixed {sw} vortex 40k, 20 steps")
     sc = S.vortex()
@@ -97,7 +99,8 @@ def test_friction_per_instance_in_an_ensemble(precision):
     sws = ({}, {"boundary_friction": 1}, {"boundary_friction": 1, "skf": 3},
            {"boundary_friction": 1, "bfa": 3, "boundary_pressure": 0},
            {"boundary_pressure": 0}, {"wind_force": 0}, {"water_force": 0},
-           {"pressure_force": 0})
+           {"pressure_force": 0}, {"viscosity": 0, "eos_alg": 1},
+           {"viscosity": 0, "pressure_force": 0})
     scs = S.ensemble(count=len(sws), nmin=1500, nmax=4000, seed=67)
     for sc, sw in zip(scs, sws):
         sc.time = 1e-6

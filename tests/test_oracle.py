@@ -729,3 +729,72 @@ def test_boundary_friction_opposes_the_tangential_motion_and_pressure_off_is_no_
     assert a["bnip"] == 0
     for k in ("dxdt", "dvxdt", "dhdt", "dAdt"):
         assert _same(a[k], b[k]), k
+
+
+# ------------------------------------------------------------ viscosity off
+
+@pytest.mark.parametrize("tag", ["novisc", "novisc1"])
+def test_viscosity_off_single_step_bit_exact_vs_reference(tag):
+    """viscosity = .FALSE.: the viscosity block of internal_stress
+    (internal_stress.f90:20-31) is skipped, so zeta and eta keep
+    initialize's zero and p is what eos sets (novisc1: the Hibler pressure)
+    or, with the pressure force off too (novisc), initialize's zero, and
+    the strain rate is then never set (single_step.f90:80) -- against the
+    reference built with those PARAMETERs."""
+    g = load(f"box2500_single_step_{tag}.npz")
+    sc = scenario(g)
+    out = capi.Oracle().single_step(sc)
+    assert out["nip"] == int(g["out_nip"]) and out["bnip"] == int(g["out_bnip"])
+    for k in ("dxdt", "dvxdt", "dhdt", "dAdt", "eps", "zeta", "eta", "p", "sigma"):
+        assert _same(out[k], g["out_" + k]), k
+
+
+@pytest.mark.parametrize("tag", ["novisc", "novisc1"])
+def test_viscosity_off_time_integration_bit_exact_vs_reference(tag):
+    g = load(f"edge3600_pc10_{tag}.npz")
+    sc = scenario(g)
+    o = capi.Oracle()
+    steps, out = run_pc_like_reference(o, sc, float(g["integration_time"]))
+    assert steps == int(g["out_itimestep"])
+    for k in ("x", "vx", "rho", "hsml", "A", "h"):
+        assert _same(getattr(sc, k), g["out_" + k]), k
+
+
+def test_viscosity_off_leaves_the_pressure_term_only():
+    """Closed forms: without viscosity zeta = eta = 0, so the stress tensor
+    (stress_tensor.f90:31-39) reduces to -p (1 - k_t) / 2 on the diagonal
+    and 0 off it; with the Hibler eos p is the ice strength of the run with
+    viscosity (same eos), and the strain rate is unchanged; with the
+    pressure force off as well p, the strain rate and the stress vanish and
+    only the external forces remain: the tendencies equal those of a run
+    whose ice strength is zero (thickness-independent forcing aside)."""
+    sc = S.box(nside=50, spacing=2000.0, seed=4)
+    sc.time = 2.0
+    sc.vx = np.random.default_rng(5).normal(0.0, 0.1, size=(sc.n, 2))
+    sc.eos_alg = 1
+    on = capi.Oracle().single_step(sc)
+    off = sc.copy()
+    off.viscosity = 0
+    r = capi.Oracle().single_step(off)
+    kt = 0.0   # k_t = T_star / P_star (data_variables.f90:178), T_star = 0
+    assert np.all(r["zeta"] == 0.0) and np.all(r["eta"] == 0.0)
+    assert _same(r["p"], on["p"]) and _same(r["eps"], on["eps"])
+    assert _same(r["sigma"][:, 0], -r["p"] * (1 - kt) / 2)
+    assert _same(r["sigma"][:, 2], r["sigma"][:, 0])
+    assert np.all(r["sigma"][:, 1] == 0.0)
+    assert not _same(r["dvxdt"], on["dvxdt"])
+    both = off.copy()
+    both.pressure_force = 0
+    both.boundary_pressure = 0
+    r2 = capi.Oracle().single_step(both)
+    for k in ("p", "zeta", "eta"):
+        assert np.all(r2[k] == 0.0), k
+    assert np.all(r2["eps"] == 0.0) and np.all(r2["sigma"] == 0.0)
+    # no internal force is left: the velocity tendency is the per-particle
+    # wind and water drag (wind_forcing.f90, water_drag.f90), the same as in
+    # a run whose supports are too small for any particle pair
+    lone = both.copy()
+    lone.hsml = both.hsml * 1e-3
+    r3 = capi.Oracle().single_step(lone)
+    assert r2["nip"] > 0 and r3["nip"] == 0
+    assert np.allclose(r2["dvxdt"], r3["dvxdt"], rtol=1e-12, atol=0.0)
