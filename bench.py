@@ -172,7 +172,9 @@ def _copy_calibration(mib):
 def hbm_traffic(kernel_substr, timeout=240):
     """HBM bytes per launch of the kernel whose name contains kernel_substr,
     from two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a
-    short mixed-precision run of this script.  Each counter is scaled by
+    short mixed-precision run of this script (eager launches: the counters
+    of a kernel do not depend on how it was launched, and the profiler's
+    instrumentation of graph launches is the fragile part).  Each counter is scaled by
     what it reads for plain device-to-device copies of known size (3 x 256
     MiB, run first in the profiled process).  Returns (bytes, factors) or
     (None, reason)."""
@@ -189,9 +191,9 @@ def hbm_traffic(kernel_substr, timeout=240):
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             d = os.path.join(tmp, counter)
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--",
-                   sys.executable, os.path.abspath(__file__), "--precision", "mixed",
-                   "--steps", "10", "--warmup", "3", "--latency-steps", "1", "--no-cpu",
-                   "--copy-calib", "256"]
+                   sys.executable, os.path.abspath(__file__), "--eager", "--precision",
+                   "mixed", "--steps", "10", "--warmup", "3", "--latency-steps", "1",
+                   "--no-cpu", "--copy-calib", "256"]
             try:
                 subprocess.run(cmd, check=True, timeout=timeout, cwd=tmp,
                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)

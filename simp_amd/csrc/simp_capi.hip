@@ -663,14 +663,19 @@ int simp_profile_steps(simp_handle *h, int nsteps, int maxk, char *buf, int bufl
   if (hipStreamSynchronize(h->eng->stream()) != hipSuccess) return -SIMP_E_HIP;
   std::vector<std::string> names;
   std::map<std::string, std::pair<double, int>> agg;
+  // per launch (one or more kernels): the sum of its kernels' durations
   for (auto &r : prof.recs) {
-    float t = 0.f;
-    hipEventElapsedTime(&t, r.a, r.b);
+    double t = 0.0;
+    for (auto &e : r.ev) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e.first, e.second);
+      t += ms;
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
     if (!agg.count(r.name)) names.push_back(r.name);
     agg[r.name].first += t;
     agg[r.name].second += 1;
-    hipEventDestroy(r.a);
-    hipEventDestroy(r.b);
   }
   std::string out;
   int k = 0;

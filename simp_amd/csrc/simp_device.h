@@ -181,8 +181,9 @@ struct DevState {
                          // grid origin: the candidate prefilter
   // neighbour list of the evaluation, built by pass A in the reference's
   // pair order for each particle: nbr[j * ntot + s] = list position of the
-  // j-th neighbour of position s; nnb[s] = count (> kNbrMax: not stored,
-  // the gathers rescan the 3x3 block)
+  // j-th neighbour of position s (kNbrMax + 1 rows, the last one spare for
+  // pass A's stores past a full list); nnb[s] = count (> kNbrMax: not
+  // stored, the gathers rescan the 3x3 block)
   int *nbr, *nnb;
   // candidate lists: per slot, the list positions of the 3x3 block within
   // mhsml + skin at the evaluation that built them (j-major like nbr, at

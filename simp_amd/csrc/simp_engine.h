@@ -46,7 +46,9 @@ struct InstanceInput {
 struct Profiler {
   struct Rec {
     std::string name;
-    hipEvent_t a, b;
+    // each kernel of the launch: its start and end (the dispatch's own
+    // timestamps, hipExtLaunchKernelGGL)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   };
   std::vector<Rec> recs;
 };

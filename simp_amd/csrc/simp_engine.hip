@@ -136,7 +136,7 @@ constexpr size_t kBytesPerParticle =
     2 * (8 + 4 + 4) +       // mass, orig, cellid per set
     2 * 4 + 2 * 4 + 8 + 4 + // slot, cl; own_cnt, moff; gpre, osig
     16 +                    // pf
-    kNbrMax * 4 + 4 +       // nbr, nnb
+    (kNbrMax + 1) * 4 + 4 + // nbr (a spare row), nnb
     kNbrMax * 4 + 4 +       // vnb, vnc
     5 * 8 + 3 * 8 +         // sig11 (x2: srec)/12/22, div; kz, dnum, dsum
     6 * 8 + 3 * 8 + 6 * 8;  // diag, sig_out, tend
@@ -198,7 +198,7 @@ static int alloc_state(DevState &d, A &&alloc, int **scan_tmp, size_t N, size_t 
   rc |= dalloc(alloc, &d.gpre, N);
   rc |= dalloc(alloc, &d.osig, N);
   rc |= dalloc(alloc, &d.pf, N);
-  rc |= dalloc(alloc, &d.nbr, (size_t)kNbrMax * N);
+  rc |= dalloc(alloc, &d.nbr, (size_t)(kNbrMax + 1) * N);   // row kNbrMax: spare
   rc |= dalloc(alloc, &d.nnb, N);
   rc |= dalloc(alloc, &d.vnb, (size_t)kNbrMax * N);
   rc |= dalloc(alloc, &d.vnc, N);
@@ -1017,14 +1017,14 @@ void Engine::L(const char *name, F &&f) {
     f();
     return;
   }
+  KernelEvents ke;
+  g_kernel_events = &ke;
+  f();
+  g_kernel_events = nullptr;
   Profiler::Rec r;
   r.name = name;
-  hipEventCreate(&r.a);
-  hipEventCreate(&r.b);
-  hipEventRecord(r.a, st_);
-  f();
-  hipEventRecord(r.b, st_);
-  prof->recs.push_back(r);
+  r.ev = std::move(ke.ev);
+  prof->recs.push_back(std::move(r));
 }
 
 void Engine::enqueue_prefix(int buf) {

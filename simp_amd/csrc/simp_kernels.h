@@ -2,9 +2,20 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
+#include <vector>
+
 #include "simp_device.h"
 
 namespace simp {
+
+// Start / end events of each kernel launched while a pointer to this is set
+// in g_kernel_events (the calling thread's launches only); the owner
+// destroys them.
+struct KernelEvents {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+extern thread_local KernelEvents *g_kernel_events;
 
 void launch_partial(const DevState &d, int buf, hipStream_t st);
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st);

@@ -44,6 +44,7 @@
 // contraction off, each FP64 sum is formed in the same order as the
 // reference's, so the FP64 path reproduces the reference's results.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <type_traits>
@@ -1740,20 +1741,30 @@ k_passA(DevState d, int buf, int want_diag) {
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) nx[u] = d.vnb[min(j0 + kBatch + u, nc - 1) * N + self];
       }
+      bool hit[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        if (j0 + u >= nc) continue;
         const double dx = xi - xq[u], dy = yi - yq[u];
         const double mh = (hi + hq[u]) / 2.;
         double r = 0.;
-        if (!support(dx * dx + dy * dy, mh, r)) continue;
+        hit[u] = j0 + u < nc && support(dx * dx + dy * dy, mh, r);
+        if (!hit[u]) continue;
         const int kind = pair_kind(R, sj[u], self);
         nI += kind == 1;
         nX += kind == 2;
         if (kind != 0) osg = osg * 0x9e3779b1u + (unsigned)sj[u];
-        d.nbr[nn * N + self] = sj[u];
-        ++nn;
         pair(dx, dy, r, mh, mq[u], rq[u], uq[u], wq[u]);
+      }
+      // The batch's list entries: one store per candidate, hit or not, at
+      // the next free position -- a miss's entry is overwritten by the next
+      // hit or lies past the list's end (row kNbrMax is spare).  With a
+      // fixed number of stores per batch the next batch's wait for its
+      // list entries, issued before these, does not wait for the stores to
+      // complete (a data-dependent store count made it wait for all).
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        d.nbr[nn * N + self] = sj[u];
+        nn += hit[u] ? 1 : 0;
       }
     }
   } else {
@@ -3433,76 +3444,97 @@ __global__ void __launch_bounds__(kTPB) k_single_pack(DevState d, double *out, i
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // grid rounded up to a multiple of the 8 XCDs (see particle_of)
+// Every launch goes through launch_k: while g_kernel_events is set (the
+// engine's profiler), the launch records its kernel's own start and end
+// (hipExtLaunchKernelGGL: timestamps of the dispatch itself, the figures a
+// kernel trace reports), one event pair per kernel.
+thread_local KernelEvents *g_kernel_events = nullptr;
+
+template <class K, class... A>
+static void launch_k(K kern, long grid, int block, hipStream_t st, A... args) {
+  KernelEvents *ke = g_kernel_events;
+  if (ke) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+      hipExtLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, st, a, b, 0, args...);
+      ke->ev.emplace_back(a, b);
+      return;
+    }
+    if (a) (void)hipEventDestroy(a);
+  }
+  kern<<<(unsigned)grid, block, 0, st>>>(args...);
+}
+
 static inline int g8(long blocks) { return (int)((std::max<long>(blocks, 1) + 7) / 8 * 8); }
 
 void launch_partial(const DevState &d, int buf, hipStream_t st) {
-  k_partial<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  launch_k(k_partial, g8(d.nblk), kTPB, st, d, buf);
 }
 void launch_finalize(const DevState &d, int end_of_step, hipStream_t st) {
-  k_finalize<<<g8(d.ninst), kTPB, 0, st>>>(d, end_of_step);
+  launch_k(k_finalize, g8(d.ninst), kTPB, st, d, end_of_step);
 }
 void launch_stats_reset(const DevState &d, hipStream_t st) {
-  k_stats_reset<<<cdiv(std::max(d.ninst, 1), kTPB), kTPB, 0, st>>>(d);
+  launch_k(k_stats_reset, cdiv(std::max(d.ninst, 1), kTPB), kTPB, st, d);
 }
 void launch_fold(const DevState &d, hipStream_t st) {
-  k_fold<<<cdiv(std::max(d.ninst, 1), kTPB), kTPB, 0, st>>>(d);
+  launch_k(k_fold, cdiv(std::max(d.ninst, 1), kTPB), kTPB, st, d);
 }
 void launch_count(const DevState &d, int buf, hipStream_t st) {
-  k_count<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  launch_k(k_count, g8(d.nblk), kTPB, st, d, buf);
 }
 void launch_rebuild(const DevState &d, int buf, int *tmp, hipStream_t st) {
   const int nb = std::max(cdiv(d.ctot, kScanItems), 1);
-  k_rebuild<<<kRebuildBlocks, kTPB, 0, st>>>(d, tmp, nb, buf);
+  launch_k(k_rebuild, kRebuildBlocks, kTPB, st, d, tmp, nb, buf);
 }
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st) {
   int nb = cdiv(L, kScanItems);
   if (nb < 1) nb = 1;
-  k_scan1<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, nb, gate);
-  k_scan2<<<g8(nb), kTPB, 0, st>>>(in, L, tmp, out, zero_in, nb, gate);
+  launch_k(k_scan1, g8(nb), kTPB, st, in, L, tmp, nb, gate);
+  launch_k(k_scan2, g8(nb), kTPB, st, in, L, tmp, out, zero_in, nb, gate);
 }
 // The tuned variants exist for the reference's default switch values; the
 // variants for other values run the default batch sizes.
 template <bool MIXED>
 static void launch_passA_t(const DevState &d, int buf, int want_diag, int ba, hipStream_t st) {
   if (ba == 1)
-    k_passA<MIXED, false, 1><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<MIXED, false, 1>, g8(d.nblk), kTPB, st, d, buf, want_diag);
   else if (ba == 3)
-    k_passA<MIXED, false, 3><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<MIXED, false, 3>, g8(d.nblk), kTPB, st, d, buf, want_diag);
   else if (ba == 4)
-    k_passA<MIXED, false, 4><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<MIXED, false, 4>, g8(d.nblk), kTPB, st, d, buf, want_diag);
   else
-    k_passA<MIXED, false, 2><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<MIXED, false, 2>, g8(d.nblk), kTPB, st, d, buf, want_diag);
 }
 void launch_passA(const DevState &d, int buf, int mixed, int xsw, int want_diag, int ba,
                   hipStream_t st) {
   if (mixed && xsw)
-    k_passA<true, true, kBatchA><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<true, true, kBatchA>, g8(d.nblk), kTPB, st, d, buf, want_diag);
   else if (mixed)
     launch_passA_t<true>(d, buf, want_diag, ba, st);
   else if (xsw)
-    k_passA<false, true, kBatchA><<<g8(d.nblk), kTPB, 0, st>>>(d, buf, want_diag);
+    launch_k(k_passA<false, true, kBatchA>, g8(d.nblk), kTPB, st, d, buf, want_diag);
   else
     launch_passA_t<false>(d, buf, want_diag, ba, st);
 }
 void launch_pairtot(const DevState &d, hipStream_t st) {
-  k_pairtot<<<g8(cdiv(d.ctot, kTPB)), kTPB, 0, st>>>(d);
+  launch_k(k_pairtot, g8(cdiv(d.ctot, kTPB)), kTPB, st, d);
 }
 void launch_ranks(const DevState &d, int buf, hipStream_t st) {
-  k_ranks<<<g8(d.ninst), kRankThreads, 0, st>>>(d, buf);
+  launch_k(k_ranks, g8(d.ninst), kRankThreads, st, d, buf);
 }
 template <bool MIXED, bool XSW, int phase>
 static void launch_passB_b(const DevState &d, int bb, hipStream_t st) {
   constexpr int def = MIXED ? kBatchBMixed : kBatchBFP64;
   // tuned variants: the predictor and corrector of the default switches
   if (!XSW && phase <= 1 && bb == 2)
-    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 2 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 2 : def>, g8(d.nblk), kTPB, st, d);
   else if (!XSW && phase <= 1 && bb == 3)
-    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 3 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 3 : def>, g8(d.nblk), kTPB, st, d);
   else if (!XSW && phase <= 1 && bb == 4)
-    k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 4 : def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_passB<MIXED, XSW, phase, (!XSW && phase <= 1) ? 4 : def>, g8(d.nblk), kTPB, st, d);
   else
-    k_passB<MIXED, XSW, phase, def><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_passB<MIXED, XSW, phase, def>, g8(d.nblk), kTPB, st, d);
 }
 template <bool MIXED, bool XSW>
 static void launch_passB_t(const DevState &d, int phase, int bb, hipStream_t st) {
@@ -3518,27 +3550,27 @@ static void launch_passB_t(const DevState &d, int phase, int bb, hipStream_t st)
     launch_passB_b<MIXED, XSW, 4>(d, bb, st);
 }
 void launch_euler(const DevState &d, hipStream_t st) {
-  k_euler<<<g8(d.nblk), kTPB, 0, st>>>(d);
+  launch_k(k_euler, g8(d.nblk), kTPB, st, d);
 }
 void launch_condens(const DevState &d, int mode, hipStream_t st) {
   if (mode == 0)
-    k_condens<0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_condens<0>, g8(d.nblk), kTPB, st, d);
   else if (mode == 1)
-    k_condens<1><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_condens<1>, g8(d.nblk), kTPB, st, d);
   else
-    k_condens<2><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_condens<2>, g8(d.nblk), kTPB, st, d);
 }
 void launch_av(const DevState &d, int buf, hipStream_t st) {
-  k_avpair<<<g8(d.nblk), kTPB, 0, st>>>(d, buf);
+  launch_k(k_avpair, g8(d.nblk), kTPB, st, d, buf);
   const int nbp = g8(cdiv(std::max(d.pcap, 1), kPairBlk));
-  k_avlast<<<nbp, kPairBlk, 0, st>>>(d);
-  k_avres<<<nbp, kPairBlk, 0, st>>>(d);
+  launch_k(k_avlast, nbp, kPairBlk, st, d);
+  launch_k(k_avres, nbp, kPairBlk, st, d);
 }
 void launch_endpass(const DevState &d, int mode, hipStream_t st) {
   if (mode == 0)
-    k_endpass<0><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_endpass<0>, g8(d.nblk), kTPB, st, d);
   else
-    k_endpass<1><<<g8(d.nblk), kTPB, 0, st>>>(d);
+    launch_k(k_endpass<1>, g8(d.nblk), kTPB, st, d);
 }
 void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipStream_t st) {
   if (mixed && xsw)
@@ -3552,23 +3584,23 @@ void launch_passB(const DevState &d, int mixed, int xsw, int phase, int bb, hipS
 }
 void launch_export_pairs(const DevState &d, int buf, int cap, int *pi, int *pj, double *rij,
                          double *hij, double *w, double *dwdx, hipStream_t st) {
-  k_export_pairs<<<g8(d.nblk), kTPB, 0, st>>>(d, buf, cap, pi, pj, rij, hij, w, dwdx);
+  launch_k(k_export_pairs, g8(d.nblk), kTPB, st, d, buf, cap, pi, pj, rij, hij, w, dwdx);
 }
 void launch_export_bpairs(const DevState &d, int buf, int nb, int bcap, int *bpi, int *bpj,
                           double *brij, hipStream_t st) {
-  if (nb > 0) k_export_bpairs<<<g8(cdiv(nb, kTPB)), kTPB, 0, st>>>(d, buf, bcap, bpi, bpj, brij);
+  if (nb > 0) launch_k(k_export_bpairs, g8(cdiv(nb, kTPB)), kTPB, st, d, buf, bcap, bpi, bpj, brij);
 }
 void launch_single_blocks(const DevState &d, int n, hipStream_t st) {
-  k_single_blocks<<<cdiv(std::max(d.nblk, 1), kTPB), kTPB, 0, st>>>(d.blk, n, d.nblk);
+  launch_k(k_single_blocks, cdiv(std::max(d.nblk, 1), kTPB), kTPB, st, d.blk, n, d.nblk);
 }
 void launch_single_unpack(const DevState &d, const double *x2, const double *v2,
                           const double *xb2, int n, int nb, hipStream_t st) {
-  k_single_unpack<<<cdiv(std::max(std::max(n, nb), 1), kTPB), kTPB, 0, st>>>(
+  launch_k(k_single_unpack, cdiv(std::max(std::max(n, nb), 1), kTPB), kTPB, st, 
       d, (const double2 *)x2, (const double2 *)v2, (const double2 *)xb2, n, nb);
 }
 void launch_single_pack(const DevState &d, double *out, int n, int want_t, int want_d,
                         hipStream_t st) {
-  k_single_pack<<<cdiv(std::max(n, 1), kTPB), kTPB, 0, st>>>(d, out, n, want_t, want_d);
+  launch_k(k_single_pack, cdiv(std::max(n, 1), kTPB), kTPB, st, d, out, n, want_t, want_d);
 }
 
 }  // namespace simp
