@@ -8,7 +8,7 @@ full predictor-corrector step (two right-hand-side evaluations + update).
 
 Prints one JSON line: mixed-precision particle-steps/s (``value``), the FP64
 rate, p50/p99 step latency, the autotuner's choice, the search statistics,
-the dominant kernel's roofline (HIP events on the library's stream) and HBM
+the dominant kernel's roofline (the kernel's own start / stop events on the library's stream) and HBM
 traffic, the mixed-precision error per field, the per-call drop-in,
 snapshots, the smaller box and vortex workloads, the packed ensemble and the
 served request stream, and the host-CPU baseline (the compiled reference on
