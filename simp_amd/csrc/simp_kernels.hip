@@ -786,9 +786,9 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 
 // --------------------------------------------------------------- rebuild
 //
-// The rebuild of the cell lists is one kernel of kRebuildBlocks blocks
-// (four per CU: every block of it is resident at once) whose blocks stride
-// over the block table.  Between rebuilds -- most
+// The rebuild of the cell lists is one kernel of at most kRebuildBlocks
+// blocks (at most four per CU: every block of it is resident at once) whose
+// blocks stride over the block table.  Between rebuilds -- most
 // evaluations -- it reads one flag and returns: one launch instead of the
 // four a rebuild needs.  A rebuild runs four phases separated by grid
 // barriers: (1) slots in the new cells and the block sums of the cell
@@ -801,7 +801,7 @@ __global__ void __launch_bounds__(kTPB) k_scan2(int *in, int L, const int *bsum,
 // or invalidate is needed (an agent-scope fence per barrier costs ~30 us
 // on MI355X, this barrier a few).
 
-constexpr int kRebuildBlocks = 1024;
+constexpr int kRebuildBlocks = 1024;   // at most; four per CU on MI355X
 constexpr int kBarSpin = 1 << 18;   // bound of a barrier's wait (polls)
 // barrier words: kBarGroups arrival counters (one 128-byte line each), then
 // the counter of the groups arrived and the barrier generation
@@ -3484,7 +3484,16 @@ void launch_count(const DevState &d, int buf, hipStream_t st) {
 }
 void launch_rebuild(const DevState &d, int buf, int *tmp, hipStream_t st) {
   const int nb = std::max(cdiv(d.ctot, kScanItems), 1);
-  launch_k(k_rebuild, kRebuildBlocks, kTPB, st, d, tmp, nb, buf);
+  // one block per block-table entry (or scan block) up to four per CU, so
+  // that every block of the grid is resident at once
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = kRebuildBlocks / 4;
+  const long grid = std::min<long>({(long)g8(std::max(d.nblk, nb)), (long)kRebuildBlocks,
+                                    4L * cus});
+  launch_k(k_rebuild, grid, kTPB, st, d, tmp, nb, buf);
 }
 void launch_scan(int *in, int L, int *tmp, int *out, int zero_in, const int *gate,
                  hipStream_t st) {
