@@ -147,7 +147,7 @@ struct BlockEntry {
   int nloc;          // slots in the block (min(n - first, kTPB))
   int pos;           // packed index of the first slot (p_off + first)
   int cs;            // the instance's current state set (sel ^ rb): 0 after a
-                     // repack, moved by k_permute when the instance rebuilds,
+                     // repack, moved by k_rebuild when the instance rebuilds,
                      // unchanged by the fold that ends the evaluation
   int tmass;         // 1: the instance's masses evolve (sle 0 or 1) and the
                      // half-step state's are in masst

@@ -13,11 +13,11 @@
 //                boundary particle's cell changes
 //   count        cell key NINT(x/bin) (nnps.f90:246-254); flags a rebuild
 //                of the cell lists when any key changed
-//   hist, scan x2, scatter, cellsort, permute
-//                the cell lists, members in ascending particle id (the
+//   rebuild      the cell lists, members in ascending particle id (the
 //                reference's bin order), and the state moved to the new
-//                list order; all return at once when no key changed (the
-//                corrector evaluation normally reuses the predictor's lists)
+//                list order, in one persistent launch with grid barriers;
+//                it returns at once when no key changed (the corrector
+//                evaluation normally reuses the predictor's lists)
 //   passA        neighbour gather: Wendland C6 gradient, velocity divergence
 //                and gradient, strain rate and the VP rheology epilogue
 //                (kernel.f90:471-520, vector_divergence.f90:27-28,
